@@ -1,0 +1,253 @@
+/*
+ * numrs_amd.h -- C ABI of the MI355X (gfx950) kernels layer behind NumRS2's Array<T>.
+ *
+ * Every entry point replaces one function of the reference's crate-private `src/kernels`
+ * dispatch layer (or a public entry point that bypasses it: argmax/argmin, axis folds,
+ * the i64 sum). The citation above each declaration names the reference function
+ * (file:line under the reference's `src/`) whose argument meaning and results it keeps.
+ *
+ * Conventions (all entry points):
+ *   - Return an nrk_status; 0 (NRK_OK) on success. nrk_last_error() describes the last failure
+ *     of the calling thread.
+ *   - Matrices are dense, row-major, in logical order (the reference's `borrow::operand` layout).
+ *   - GEMM output is caller-allocated and overwritten (beta = 0).
+ *   - Entry points named nrk_<fn> take HOST pointers, exactly like the Rust slices they replace:
+ *     they stage the operands through the device and block until the result is on the host.
+ *   - Entry points named nrk_dev_<fn> take DEVICE pointers (from nrk_device_malloc) and an
+ *     nrk_stream; they enqueue work and return without waiting. Scalar results are written to
+ *     device memory.
+ *   - Reductions are deterministic: the result bits do not depend on the grid size, the number
+ *     of CUs or the order in which workgroups finish (fixed segment tree, fixed-order finish).
+ *   - Empty input: sum/mean/min/max/var return 0 and prod returns 1, as the reference's
+ *     kernels do (kernels/reduce.rs:207-421). argmax/argmin of an empty input is an error
+ *     (NRK_ERR_INVALID_ARGUMENT), as math/statistics/mod.rs:83-87 errors.
+ *
+ * Concurrency: calls on different streams may run concurrently; each stream has its own
+ * scratch workspace. Calls on one stream are serialised by that stream.
+ */
+#ifndef NUMRS_AMD_H
+#define NUMRS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t nrk_status;
+enum {
+    NRK_OK = 0,
+    NRK_ERR_INVALID_ARGUMENT = 1, /* shape/length/argument contract violated            */
+    NRK_ERR_NO_DEVICE = 2,        /* no gfx950 device, or the device index is out of range */
+    NRK_ERR_OUT_OF_MEMORY = 3,    /* hipMalloc of an operand or of scratch failed          */
+    NRK_ERR_HIP = 4,              /* any other HIP runtime error (see nrk_last_error)       */
+    NRK_ERR_UNSUPPORTED = 5       /* a combination the library does not implement          */
+};
+
+/* A hipStream_t. NULL means the null stream of the current device. */
+typedef void *nrk_stream;
+
+/* ------------------------------------------------------------------------------------------ */
+/* Runtime                                                                                     */
+/* ------------------------------------------------------------------------------------------ */
+
+/* Select the device used by subsequent calls from this thread (default 0) and check that it
+ * is a gfx950 part. Idempotent. */
+nrk_status nrk_init(int device);
+const char *nrk_status_string(nrk_status status);
+/* Text of the last error raised on the calling thread ("" if none). */
+const char *nrk_last_error(void);
+/* Library version as "MAJOR.MINOR.PATCH". */
+const char *nrk_version(void);
+
+nrk_status nrk_device_malloc(void **ptr, size_t bytes);
+nrk_status nrk_device_free(void *ptr);
+nrk_status nrk_memcpy_htod(void *dst, const void *src, size_t bytes, nrk_stream stream);
+nrk_status nrk_memcpy_dtoh(void *dst, const void *src, size_t bytes, nrk_stream stream);
+nrk_status nrk_memcpy_dtod(void *dst, const void *src, size_t bytes, nrk_stream stream);
+nrk_status nrk_memset_zero(void *dst, size_t bytes, nrk_stream stream);
+nrk_status nrk_stream_create(nrk_stream *stream);
+nrk_status nrk_stream_destroy(nrk_stream stream);
+nrk_status nrk_stream_synchronize(nrk_stream stream);
+nrk_status nrk_device_synchronize(void);
+
+/* HIP events for timing work on a stream. */
+nrk_status nrk_event_create(void **event);
+nrk_status nrk_event_destroy(void *event);
+nrk_status nrk_event_record(void *event, nrk_stream stream);
+nrk_status nrk_event_elapsed_ms(float *ms, void *start, void *stop);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Elementwise: the reference's closures become op codes                                       */
+/* ------------------------------------------------------------------------------------------ */
+
+/* One step of a scalar chain; every step is one IEEE rounding (never contracted to an FMA),
+ * the contract of array/operations.rs:448-474 and expr/fused_eval.rs:15-21. */
+enum {
+    NRK_SOP_ADD = 0,    /* y = x + s  (Array::add_scalar, operations.rs:448)      */
+    NRK_SOP_SUB = 1,    /* y = x - s  (Array::subtract_scalar, operations.rs:455) */
+    NRK_SOP_MUL = 2,    /* y = x * s  (Array::multiply_scalar, operations.rs:462) */
+    NRK_SOP_DIV = 3,    /* y = x / s  (Array::divide_scalar, operations.rs:469)   */
+    NRK_SOP_RSUB = 4,   /* y = s - x  (scalar - &Array, arithmetic.rs:342-431)    */
+    NRK_SOP_RDIV = 5,   /* y = s / x                                              */
+    NRK_SOP_NEG = 6,    /* y = -x     (Neg for &Array, arithmetic.rs:448-468)     */
+    NRK_SOP_ABS = 7,    /* y = |x|                                                */
+    NRK_SOP_SQUARE = 8, /* y = x * x                                              */
+    NRK_SOP_SQRT = 9,   /* y = sqrt(x), correctly rounded                         */
+    NRK_SOP_COUNT = 10
+};
+typedef struct nrk_scalar_op {
+    int32_t op;     /* NRK_SOP_* */
+    int32_t reserved;
+    double scalar;  /* rounded to the element type before use */
+} nrk_scalar_op;
+#define NRK_MAX_CHAIN 8
+
+/* Binary op codes for zip: out[i] = a[i] (op) b[i], one rounding. */
+enum {
+    NRK_BOP_ADD = 0, /* arithmetic.rs:35-100  Add            */
+    NRK_BOP_SUB = 1, /* arithmetic.rs:100-160 Sub            */
+    NRK_BOP_MUL = 2, /* arithmetic.rs:160-220 Mul            */
+    NRK_BOP_DIV = 3, /* arithmetic.rs:220-270 Div            */
+    NRK_BOP_REM = 4, /* arithmetic.rs:273-320 Rem (truncated, fmod) */
+    NRK_BOP_MAXIMUM = 5, /* ufuncs.rs:169  if a > b {a} else {b} */
+    NRK_BOP_MINIMUM = 6, /* ufuncs.rs:173  if a < b {a} else {b} */
+    NRK_BOP_COUNT = 7
+};
+
+/* kernels/elementwise.rs:237-258 unary_serial / unary_dispatch with an op-coded closure:
+ * y[i] = chain(x[i]). In-place (y == x) is allowed. */
+nrk_status nrk_map_f64(const nrk_scalar_op *chain, int32_t chain_len, const double *x, double *y, size_t n);
+nrk_status nrk_map_f32(const nrk_scalar_op *chain, int32_t chain_len, const float *x, float *y, size_t n);
+nrk_status nrk_dev_map_f64(const nrk_scalar_op *chain, int32_t chain_len, const double *x, double *y, size_t n, nrk_stream stream);
+nrk_status nrk_dev_map_f32(const nrk_scalar_op *chain, int32_t chain_len, const float *x, float *y, size_t n, nrk_stream stream);
+
+/* kernels/elementwise.rs:70-109 binary_f64/binary_f32 (and :217 binary_dispatch) with an
+ * op-coded closure: y[i] = a[i] (op) b[i]. */
+nrk_status nrk_zip_f64(int32_t op, const double *a, const double *b, double *y, size_t n);
+nrk_status nrk_zip_f32(int32_t op, const float *a, const float *b, float *y, size_t n);
+nrk_status nrk_dev_zip_f64(int32_t op, const double *a, const double *b, double *y, size_t n, nrk_stream stream);
+nrk_status nrk_dev_zip_f32(int32_t op, const float *a, const float *b, float *y, size_t n, nrk_stream stream);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Reductions over a flat slice                                                                */
+/* ------------------------------------------------------------------------------------------ */
+
+/* kernels/reduce.rs:207  sum_{f64,f32}(a) -> T                       (0 for empty)          */
+nrk_status nrk_sum_f64(const double *a, size_t n, double *out);
+nrk_status nrk_sum_f32(const float *a, size_t n, float *out);
+/* kernels/reduce.rs:227  mean_{f64,f32}(a) -> T  = sum / n            (0 for empty)          */
+nrk_status nrk_mean_f64(const double *a, size_t n, double *out);
+nrk_status nrk_mean_f32(const float *a, size_t n, float *out);
+/* kernels/reduce.rs:239  sum_sq_dev_{f64,f32}(a, mean) -> sum((a-mean)^2)                    */
+nrk_status nrk_sum_sq_dev_f64(const double *a, size_t n, double mean, double *out);
+nrk_status nrk_sum_sq_dev_f32(const float *a, size_t n, float mean, float *out);
+/* kernels/reduce.rs:284  var_{f64,f32}(a, ddof) -> T, divisor (n - ddof) taken in floating
+ * point, so ddof >= n gives inf/NaN as in the reference                 (0 for empty)          */
+nrk_status nrk_var_f64(const double *a, size_t n, size_t ddof, double *out);
+nrk_status nrk_var_f32(const float *a, size_t n, size_t ddof, float *out);
+/* kernels/reduce.rs:323,376  min/max_{f64,f32}(a): any NaN -> NaN      (0 for empty)          */
+nrk_status nrk_min_f64(const double *a, size_t n, double *out);
+nrk_status nrk_min_f32(const float *a, size_t n, float *out);
+nrk_status nrk_max_f64(const double *a, size_t n, double *out);
+nrk_status nrk_max_f32(const float *a, size_t n, float *out);
+/* kernels/reduce.rs:419  prod_{f64,f32}(a)                            (1 for empty)          */
+nrk_status nrk_prod_f64(const double *a, size_t n, double *out);
+nrk_status nrk_prod_f32(const float *a, size_t n, float *out);
+/* array/operations.rs:303-318  Array::<i64>::sum: wrapping integer fold, bit-exact           */
+nrk_status nrk_sum_i64(const int64_t *a, size_t n, int64_t *out);
+/* math/statistics/mod.rs:79-103 / :218-242  argmax/argmin(None): index of the first strict
+ * maximum (minimum); NaN elements are skipped, except that a NaN at index 0 yields 0.         */
+nrk_status nrk_argmax_f64(const double *a, size_t n, uint64_t *out);
+nrk_status nrk_argmax_f32(const float *a, size_t n, uint64_t *out);
+nrk_status nrk_argmin_f64(const double *a, size_t n, uint64_t *out);
+nrk_status nrk_argmin_f32(const float *a, size_t n, uint64_t *out);
+
+/* Device-pointer forms of the above (out is a device pointer to one element). */
+nrk_status nrk_dev_sum_f64(const double *a, size_t n, double *out, nrk_stream stream);
+nrk_status nrk_dev_sum_f32(const float *a, size_t n, float *out, nrk_stream stream);
+nrk_status nrk_dev_mean_f64(const double *a, size_t n, double *out, nrk_stream stream);
+nrk_status nrk_dev_mean_f32(const float *a, size_t n, float *out, nrk_stream stream);
+nrk_status nrk_dev_var_f64(const double *a, size_t n, size_t ddof, double *out, nrk_stream stream);
+nrk_status nrk_dev_var_f32(const float *a, size_t n, size_t ddof, float *out, nrk_stream stream);
+nrk_status nrk_dev_min_f64(const double *a, size_t n, double *out, nrk_stream stream);
+nrk_status nrk_dev_min_f32(const float *a, size_t n, float *out, nrk_stream stream);
+nrk_status nrk_dev_max_f64(const double *a, size_t n, double *out, nrk_stream stream);
+nrk_status nrk_dev_max_f32(const float *a, size_t n, float *out, nrk_stream stream);
+nrk_status nrk_dev_prod_f64(const double *a, size_t n, double *out, nrk_stream stream);
+nrk_status nrk_dev_prod_f32(const float *a, size_t n, float *out, nrk_stream stream);
+nrk_status nrk_dev_sum_i64(const int64_t *a, size_t n, int64_t *out, nrk_stream stream);
+nrk_status nrk_dev_argmax_f64(const double *a, size_t n, uint64_t *out, nrk_stream stream);
+nrk_status nrk_dev_argmax_f32(const float *a, size_t n, uint64_t *out, nrk_stream stream);
+nrk_status nrk_dev_argmin_f64(const double *a, size_t n, uint64_t *out, nrk_stream stream);
+nrk_status nrk_dev_argmin_f32(const float *a, size_t n, uint64_t *out, nrk_stream stream);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Axis reductions over an (outer, len, inner) view of a row-major array                       */
+/* ------------------------------------------------------------------------------------------ */
+/* The reduced axis has `len` elements with stride `inner`; there are outer*inner outputs, in
+ * row-major order of the remaining axes. A 2-D (R, C) array reduced along axis 1 is
+ * (outer=R, len=C, inner=1); along axis 0 it is (outer=1, len=R, inner=C).
+ * The reference folds each output left to right (math/aggregation.rs:548-561 for sum,
+ * :244-261 for max/min with NaN short-circuit; math/statistics/mod.rs:1064-1071 mean,
+ * :1403-1494 var = mean(axis) then sum((x-mean)^2)/(len-ddof); :105-183 argmax). len == 0
+ * is an error for max/min/argmax/argmin/var (the reference errors on empty input) and gives
+ * zeros for sum/mean. For var, len <= ddof is an error (statistics/mod.rs:1423-1428). */
+enum {
+    NRK_AXIS_SUM = 0, NRK_AXIS_MEAN = 1, NRK_AXIS_VAR = 2, NRK_AXIS_MIN = 3,
+    NRK_AXIS_MAX = 4, NRK_AXIS_ARGMIN = 5, NRK_AXIS_ARGMAX = 6
+};
+/* out is T* for SUM/MEAN/VAR/MIN/MAX and uint64_t* for ARGMIN/ARGMAX. */
+nrk_status nrk_reduce_axis_f64(int32_t op, const double *x, size_t outer, size_t len, size_t inner, size_t ddof, void *out);
+nrk_status nrk_reduce_axis_f32(int32_t op, const float *x, size_t outer, size_t len, size_t inner, size_t ddof, void *out);
+nrk_status nrk_dev_reduce_axis_f64(int32_t op, const double *x, size_t outer, size_t len, size_t inner, size_t ddof, void *out, nrk_stream stream);
+nrk_status nrk_dev_reduce_axis_f32(int32_t op, const float *x, size_t outer, size_t len, size_t inner, size_t ddof, void *out, nrk_stream stream);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Fused elementwise -> reduction (one HBM pass)                                               */
+/* ------------------------------------------------------------------------------------------ */
+/* No reference counterpart as one call: it fuses `(x * a) + b` (two roundings,
+ * array/operations.rs:448-474) with the reductions above, which the reference runs as separate
+ * passes over a materialised intermediate (SURVEY call stack E). The input is read once.
+ *
+ * x is (rows, cols) row-major; y = chain(x) elementwise. Outputs are device pointers; any may
+ * be NULL to skip it. row_* outputs have `rows` elements (reduction along axis 1); full_*
+ * outputs have one element (reduction over all rows*cols elements, argmax as a flat index).
+ * var uses divisor (count - ddof). */
+typedef struct nrk_reduce_outputs {
+    void *sum;      /* T        */
+    void *mean;     /* T        */
+    void *var;      /* T        */
+    void *min;      /* T        */
+    void *max;      /* T        */
+    uint64_t *argmin;
+    uint64_t *argmax;
+} nrk_reduce_outputs;
+nrk_status nrk_dev_map_reduce_f64(const nrk_scalar_op *chain, int32_t chain_len, const double *x, size_t rows, size_t cols,
+                                  size_t ddof, const nrk_reduce_outputs *row_out, const nrk_reduce_outputs *full_out,
+                                  nrk_stream stream);
+nrk_status nrk_dev_map_reduce_f32(const nrk_scalar_op *chain, int32_t chain_len, const float *x, size_t rows, size_t cols,
+                                  size_t ddof, const nrk_reduce_outputs *row_out, const nrk_reduce_outputs *full_out,
+                                  nrk_stream stream);
+
+/* ------------------------------------------------------------------------------------------ */
+/* GEMM: C := A * B, row-major, overwrite                                                      */
+/* ------------------------------------------------------------------------------------------ */
+/* kernels/gemm.rs:246-268 gemm_2d<T>(m, k, n, a, b, c) for T = f64 / f32. The slice lengths
+ * are passed so that the reference's length-mismatch rule holds: if a_len != m*k, b_len != k*n
+ * or c_len != m*n, C is zero-filled where it exists and nothing is read (gemm_generic's
+ * defensive branch, gemm.rs:408-423). m == 0 or n == 0 returns without writing; k == 0 writes
+ * zeros (gemm.rs:295-303). */
+nrk_status nrk_gemm_2d_f64(size_t m, size_t k, size_t n, const double *a, size_t a_len, const double *b, size_t b_len,
+                           double *c, size_t c_len);
+nrk_status nrk_gemm_2d_f32(size_t m, size_t k, size_t n, const float *a, size_t a_len, const float *b, size_t b_len,
+                           float *c, size_t c_len);
+/* Device forms: plain FP64 / FP32 on MFMA (v_mfma_f64_16x16x4 / v_mfma_f32_16x16x4). */
+nrk_status nrk_dev_gemm_f64(size_t m, size_t k, size_t n, const double *a, const double *b, double *c, nrk_stream stream);
+nrk_status nrk_dev_gemm_f32(size_t m, size_t k, size_t n, const float *a, const float *b, float *c, nrk_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NUMRS_AMD_H */

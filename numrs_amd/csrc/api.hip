@@ -1,0 +1,252 @@
+// extern "C" entry points for reductions and elementwise ops: argument checking, the
+// host-pointer staging forms, and dispatch onto the launchers in ops.hpp.
+#include "ops.hpp"
+
+using namespace nrk;
+
+namespace {
+
+inline hipStream_t S(nrk_stream s) { return (hipStream_t)s; }
+
+// Runs `dev(d_in, d_out, stream)` on a device copy of a host slice and copies `out_count`
+// results of type OutT back. Blocks until the result is on the host.
+template <typename T, typename OutT, typename F>
+nrk_status host_call(const T *a, size_t n, OutT *out, size_t out_count, F dev) {
+    NRK_TRY(ensure_device());
+    if (!out) return fail(NRK_ERR_INVALID_ARGUMENT, "out is NULL");
+    if (n > 0 && !a) return fail(NRK_ERR_INVALID_ARGUMENT, "input is NULL");
+    void *da = nullptr, *dout = nullptr;
+    NRK_TRY(staging(0, n * sizeof(T) + 16, &da));
+    NRK_TRY(staging(1, out_count * sizeof(OutT) + 16, &dout));
+    hipStream_t s = nullptr;
+    if (n) NRK_HIP(hipMemcpyAsync(da, a, n * sizeof(T), hipMemcpyHostToDevice, s));
+    NRK_TRY(dev((const T *)da, (OutT *)dout, s));
+    if (out_count) NRK_HIP(hipMemcpyAsync(out, dout, out_count * sizeof(OutT), hipMemcpyDeviceToHost, s));
+    NRK_HIP(hipStreamSynchronize(s));
+    return NRK_OK;
+}
+
+template <typename T>
+nrk_status full_reduce(const T *a, size_t n, double ddof, nrk_reduce_outputs o, hipStream_t s) {
+    Chain c;
+    NRK_TRY(make_chain(nullptr, 0, &c));
+    return seg_reduce<T>(c, a, 1, n, ddof, nullptr, &o, s);
+}
+
+template <typename T>
+nrk_status dev_sum(const T *a, size_t n, T *out, hipStream_t s) {
+    nrk_reduce_outputs o{};
+    o.sum = out;
+    return full_reduce<T>(a, n, 0.0, o, s);
+}
+template <typename T>
+nrk_status dev_mean(const T *a, size_t n, T *out, hipStream_t s) {
+    nrk_reduce_outputs o{};
+    o.mean = out;
+    return full_reduce<T>(a, n, 0.0, o, s);
+}
+template <typename T>
+nrk_status dev_var(const T *a, size_t n, size_t ddof, T *out, hipStream_t s) {
+    nrk_reduce_outputs o{};
+    o.var = out;
+    return full_reduce<T>(a, n, (double)ddof, o, s);
+}
+template <typename T>
+nrk_status dev_min(const T *a, size_t n, T *out, hipStream_t s) {
+    nrk_reduce_outputs o{};
+    o.min = out;
+    return full_reduce<T>(a, n, 0.0, o, s);
+}
+template <typename T>
+nrk_status dev_max(const T *a, size_t n, T *out, hipStream_t s) {
+    nrk_reduce_outputs o{};
+    o.max = out;
+    return full_reduce<T>(a, n, 0.0, o, s);
+}
+template <typename T>
+nrk_status dev_argmax(const T *a, size_t n, uint64_t *out, hipStream_t s) {
+    if (n == 0) return fail(NRK_ERR_INVALID_ARGUMENT, "Cannot find argmax of empty array");
+    nrk_reduce_outputs o{};
+    o.argmax = out;
+    return full_reduce<T>(a, n, 0.0, o, s);
+}
+template <typename T>
+nrk_status dev_argmin(const T *a, size_t n, uint64_t *out, hipStream_t s) {
+    if (n == 0) return fail(NRK_ERR_INVALID_ARGUMENT, "Cannot find argmin of empty array");
+    nrk_reduce_outputs o{};
+    o.argmin = out;
+    return full_reduce<T>(a, n, 0.0, o, s);
+}
+template <typename T>
+nrk_status dev_sum_sq_dev(const T *a, size_t n, T mean, T *out, hipStream_t s) {
+    nrk_scalar_op ops[2] = {{NRK_SOP_SUB, 0, (double)mean}, {NRK_SOP_SQUARE, 0, 0.0}};
+    Chain c;
+    NRK_TRY(make_chain(ops, 2, &c));
+    nrk_reduce_outputs o{};
+    o.sum = out;
+    return seg_reduce<T>(c, a, 1, n, 0.0, nullptr, &o, s);
+}
+
+template <typename T>
+nrk_status dev_axis(int32_t op, const T *x, size_t outer, size_t len, size_t inner, size_t ddof, void *out,
+                    hipStream_t s) {
+    NRK_TRY(ensure_device());
+    if (op < NRK_AXIS_SUM || op > NRK_AXIS_ARGMAX) return fail(NRK_ERR_INVALID_ARGUMENT, "op %d is not an NRK_AXIS_* code", op);
+    if (!out && outer * inner > 0) return fail(NRK_ERR_INVALID_ARGUMENT, "out is NULL");
+    if (len == 0 && op != NRK_AXIS_SUM && op != NRK_AXIS_MEAN)
+        return fail(NRK_ERR_INVALID_ARGUMENT, "reduction along an empty axis");
+    if (op == NRK_AXIS_VAR && len <= ddof)
+        return fail(NRK_ERR_INVALID_ARGUMENT, "Degrees of freedom %zu >= axis size %zu", ddof, len);
+    if (outer * inner == 0) return NRK_OK;
+    if (inner > 1) return strided_reduce<T>(op, x, outer, len, inner, (double)ddof, out, s);
+    nrk_reduce_outputs o{};
+    switch (op) {
+        case NRK_AXIS_SUM: o.sum = out; break;
+        case NRK_AXIS_MEAN: o.mean = out; break;
+        case NRK_AXIS_VAR: o.var = out; break;
+        case NRK_AXIS_MIN: o.min = out; break;
+        case NRK_AXIS_MAX: o.max = out; break;
+        case NRK_AXIS_ARGMIN: o.argmin = (uint64_t *)out; break;
+        default: o.argmax = (uint64_t *)out; break;
+    }
+    Chain c;
+    NRK_TRY(make_chain(nullptr, 0, &c));
+    return seg_reduce<T>(c, x, outer, len, (double)ddof, &o, nullptr, s);
+}
+
+template <typename T>
+nrk_status host_axis(int32_t op, const T *x, size_t outer, size_t len, size_t inner, size_t ddof, void *out) {
+    const size_t nout = outer * inner;
+    const bool idx = (op == NRK_AXIS_ARGMAX || op == NRK_AXIS_ARGMIN);
+    if (idx) {
+        return host_call<T, uint64_t>(x, outer * len * inner, (uint64_t *)out, nout, [&](const T *d, uint64_t *dout, hipStream_t s) {
+            return dev_axis<T>(op, d, outer, len, inner, ddof, dout, s);
+        });
+    }
+    return host_call<T, T>(x, outer * len * inner, (T *)out, nout, [&](const T *d, T *dout, hipStream_t s) {
+        return dev_axis<T>(op, d, outer, len, inner, ddof, dout, s);
+    });
+}
+
+template <typename T>
+nrk_status host_map(const nrk_scalar_op *chain, int32_t chain_len, const T *x, T *y, size_t n) {
+    Chain c;
+    NRK_TRY(make_chain(chain, chain_len, &c));
+    if (n && !y) return fail(NRK_ERR_INVALID_ARGUMENT, "y is NULL");
+    return host_call<T, T>(x, n, y, n, [&](const T *d, T *dout, hipStream_t s) { return map<T>(c, d, dout, n, s); });
+}
+
+template <typename T>
+nrk_status host_zip(int32_t op, const T *a, const T *b, T *y, size_t n) {
+    NRK_TRY(ensure_device());
+    if (n && (!a || !b || !y)) return fail(NRK_ERR_INVALID_ARGUMENT, "a, b or y is NULL");
+    void *db = nullptr;
+    NRK_TRY(staging(2, n * sizeof(T) + 16, &db));
+    if (n) NRK_HIP(hipMemcpyAsync(db, b, n * sizeof(T), hipMemcpyHostToDevice, nullptr));
+    return host_call<T, T>(a, n, y, n, [&](const T *d, T *dout, hipStream_t s) { return zip<T>(op, d, (const T *)db, dout, n, s); });
+}
+
+template <typename T>
+nrk_status dev_map_reduce(const nrk_scalar_op *chain, int32_t chain_len, const T *x, size_t rows, size_t cols, size_t ddof,
+                          const nrk_reduce_outputs *row_out, const nrk_reduce_outputs *full_out, hipStream_t s) {
+    Chain c;
+    NRK_TRY(make_chain(chain, chain_len, &c));
+    if (row_out && row_out->var && cols <= ddof) return fail(NRK_ERR_INVALID_ARGUMENT, "Degrees of freedom %zu >= axis size %zu", ddof, cols);
+    return seg_reduce<T>(c, x, rows, cols, (double)ddof, row_out, full_out, s);
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---- elementwise ----
+nrk_status nrk_map_f64(const nrk_scalar_op *chain, int32_t chain_len, const double *x, double *y, size_t n) { return host_map<double>(chain, chain_len, x, y, n); }
+nrk_status nrk_map_f32(const nrk_scalar_op *chain, int32_t chain_len, const float *x, float *y, size_t n) { return host_map<float>(chain, chain_len, x, y, n); }
+nrk_status nrk_dev_map_f64(const nrk_scalar_op *chain, int32_t chain_len, const double *x, double *y, size_t n, nrk_stream stream) {
+    Chain c;
+    NRK_TRY(make_chain(chain, chain_len, &c));
+    return map<double>(c, x, y, n, S(stream));
+}
+nrk_status nrk_dev_map_f32(const nrk_scalar_op *chain, int32_t chain_len, const float *x, float *y, size_t n, nrk_stream stream) {
+    Chain c;
+    NRK_TRY(make_chain(chain, chain_len, &c));
+    return map<float>(c, x, y, n, S(stream));
+}
+nrk_status nrk_zip_f64(int32_t op, const double *a, const double *b, double *y, size_t n) { return host_zip<double>(op, a, b, y, n); }
+nrk_status nrk_zip_f32(int32_t op, const float *a, const float *b, float *y, size_t n) { return host_zip<float>(op, a, b, y, n); }
+nrk_status nrk_dev_zip_f64(int32_t op, const double *a, const double *b, double *y, size_t n, nrk_stream stream) { return zip<double>(op, a, b, y, n, S(stream)); }
+nrk_status nrk_dev_zip_f32(int32_t op, const float *a, const float *b, float *y, size_t n, nrk_stream stream) { return zip<float>(op, a, b, y, n, S(stream)); }
+
+// ---- flat reductions, host slices ----
+#define NRK_HOST_REDUCE(NAME, T, OUT_T, CALL)                                                  \
+    nrk_status NAME(const T *a, size_t n, OUT_T *out) {                                        \
+        return host_call<T, OUT_T>(a, n, out, 1, [&](const T *d, OUT_T *dout, hipStream_t s) { \
+            return CALL(d, n, dout, s);                                                        \
+        });                                                                                    \
+    }
+NRK_HOST_REDUCE(nrk_sum_f64, double, double, dev_sum<double>)
+NRK_HOST_REDUCE(nrk_sum_f32, float, float, dev_sum<float>)
+NRK_HOST_REDUCE(nrk_mean_f64, double, double, dev_mean<double>)
+NRK_HOST_REDUCE(nrk_mean_f32, float, float, dev_mean<float>)
+NRK_HOST_REDUCE(nrk_min_f64, double, double, dev_min<double>)
+NRK_HOST_REDUCE(nrk_min_f32, float, float, dev_min<float>)
+NRK_HOST_REDUCE(nrk_max_f64, double, double, dev_max<double>)
+NRK_HOST_REDUCE(nrk_max_f32, float, float, dev_max<float>)
+NRK_HOST_REDUCE(nrk_prod_f64, double, double, prod<double>)
+NRK_HOST_REDUCE(nrk_prod_f32, float, float, prod<float>)
+NRK_HOST_REDUCE(nrk_sum_i64, int64_t, int64_t, sum_i64)
+NRK_HOST_REDUCE(nrk_argmax_f64, double, uint64_t, dev_argmax<double>)
+NRK_HOST_REDUCE(nrk_argmax_f32, float, uint64_t, dev_argmax<float>)
+NRK_HOST_REDUCE(nrk_argmin_f64, double, uint64_t, dev_argmin<double>)
+NRK_HOST_REDUCE(nrk_argmin_f32, float, uint64_t, dev_argmin<float>)
+#undef NRK_HOST_REDUCE
+
+nrk_status nrk_var_f64(const double *a, size_t n, size_t ddof, double *out) {
+    return host_call<double, double>(a, n, out, 1, [&](const double *d, double *o, hipStream_t s) { return dev_var<double>(d, n, ddof, o, s); });
+}
+nrk_status nrk_var_f32(const float *a, size_t n, size_t ddof, float *out) {
+    return host_call<float, float>(a, n, out, 1, [&](const float *d, float *o, hipStream_t s) { return dev_var<float>(d, n, ddof, o, s); });
+}
+nrk_status nrk_sum_sq_dev_f64(const double *a, size_t n, double mean, double *out) {
+    return host_call<double, double>(a, n, out, 1, [&](const double *d, double *o, hipStream_t s) { return dev_sum_sq_dev<double>(d, n, mean, o, s); });
+}
+nrk_status nrk_sum_sq_dev_f32(const float *a, size_t n, float mean, float *out) {
+    return host_call<float, float>(a, n, out, 1, [&](const float *d, float *o, hipStream_t s) { return dev_sum_sq_dev<float>(d, n, mean, o, s); });
+}
+
+// ---- flat reductions, device pointers ----
+nrk_status nrk_dev_sum_f64(const double *a, size_t n, double *out, nrk_stream stream) { return dev_sum<double>(a, n, out, S(stream)); }
+nrk_status nrk_dev_sum_f32(const float *a, size_t n, float *out, nrk_stream stream) { return dev_sum<float>(a, n, out, S(stream)); }
+nrk_status nrk_dev_mean_f64(const double *a, size_t n, double *out, nrk_stream stream) { return dev_mean<double>(a, n, out, S(stream)); }
+nrk_status nrk_dev_mean_f32(const float *a, size_t n, float *out, nrk_stream stream) { return dev_mean<float>(a, n, out, S(stream)); }
+nrk_status nrk_dev_var_f64(const double *a, size_t n, size_t ddof, double *out, nrk_stream stream) { return dev_var<double>(a, n, ddof, out, S(stream)); }
+nrk_status nrk_dev_var_f32(const float *a, size_t n, size_t ddof, float *out, nrk_stream stream) { return dev_var<float>(a, n, ddof, out, S(stream)); }
+nrk_status nrk_dev_min_f64(const double *a, size_t n, double *out, nrk_stream stream) { return dev_min<double>(a, n, out, S(stream)); }
+nrk_status nrk_dev_min_f32(const float *a, size_t n, float *out, nrk_stream stream) { return dev_min<float>(a, n, out, S(stream)); }
+nrk_status nrk_dev_max_f64(const double *a, size_t n, double *out, nrk_stream stream) { return dev_max<double>(a, n, out, S(stream)); }
+nrk_status nrk_dev_max_f32(const float *a, size_t n, float *out, nrk_stream stream) { return dev_max<float>(a, n, out, S(stream)); }
+nrk_status nrk_dev_prod_f64(const double *a, size_t n, double *out, nrk_stream stream) { return prod<double>(a, n, out, S(stream)); }
+nrk_status nrk_dev_prod_f32(const float *a, size_t n, float *out, nrk_stream stream) { return prod<float>(a, n, out, S(stream)); }
+nrk_status nrk_dev_sum_i64(const int64_t *a, size_t n, int64_t *out, nrk_stream stream) { return sum_i64(a, n, out, S(stream)); }
+nrk_status nrk_dev_argmax_f64(const double *a, size_t n, uint64_t *out, nrk_stream stream) { return dev_argmax<double>(a, n, out, S(stream)); }
+nrk_status nrk_dev_argmax_f32(const float *a, size_t n, uint64_t *out, nrk_stream stream) { return dev_argmax<float>(a, n, out, S(stream)); }
+nrk_status nrk_dev_argmin_f64(const double *a, size_t n, uint64_t *out, nrk_stream stream) { return dev_argmin<double>(a, n, out, S(stream)); }
+nrk_status nrk_dev_argmin_f32(const float *a, size_t n, uint64_t *out, nrk_stream stream) { return dev_argmin<float>(a, n, out, S(stream)); }
+
+// ---- axis reductions ----
+nrk_status nrk_reduce_axis_f64(int32_t op, const double *x, size_t outer, size_t len, size_t inner, size_t ddof, void *out) { return host_axis<double>(op, x, outer, len, inner, ddof, out); }
+nrk_status nrk_reduce_axis_f32(int32_t op, const float *x, size_t outer, size_t len, size_t inner, size_t ddof, void *out) { return host_axis<float>(op, x, outer, len, inner, ddof, out); }
+nrk_status nrk_dev_reduce_axis_f64(int32_t op, const double *x, size_t outer, size_t len, size_t inner, size_t ddof, void *out, nrk_stream stream) { return dev_axis<double>(op, x, outer, len, inner, ddof, out, S(stream)); }
+nrk_status nrk_dev_reduce_axis_f32(int32_t op, const float *x, size_t outer, size_t len, size_t inner, size_t ddof, void *out, nrk_stream stream) { return dev_axis<float>(op, x, outer, len, inner, ddof, out, S(stream)); }
+
+// ---- fused map -> reduce ----
+nrk_status nrk_dev_map_reduce_f64(const nrk_scalar_op *chain, int32_t chain_len, const double *x, size_t rows, size_t cols, size_t ddof,
+                                  const nrk_reduce_outputs *row_out, const nrk_reduce_outputs *full_out, nrk_stream stream) {
+    return dev_map_reduce<double>(chain, chain_len, x, rows, cols, ddof, row_out, full_out, S(stream));
+}
+nrk_status nrk_dev_map_reduce_f32(const nrk_scalar_op *chain, int32_t chain_len, const float *x, size_t rows, size_t cols, size_t ddof,
+                                  const nrk_reduce_outputs *row_out, const nrk_reduce_outputs *full_out, nrk_stream stream) {
+    return dev_map_reduce<float>(chain, chain_len, x, rows, cols, ddof, row_out, full_out, S(stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,131 @@
+// Internal helpers shared by every translation unit of libnumrs_amd.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "numrs_amd.h"
+
+namespace nrk {
+
+// ---- error reporting (thread-local last error text) ----
+void set_error(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+nrk_status fail(nrk_status code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+nrk_status hip_fail(hipError_t e, const char *what, const char *file, int line);
+
+#define NRK_HIP(call)                                                                   \
+    do {                                                                                \
+        hipError_t nrk_e_ = (call);                                                     \
+        if (nrk_e_ != hipSuccess) return ::nrk::hip_fail(nrk_e_, #call, __FILE__, __LINE__); \
+    } while (0)
+
+#define NRK_TRY(expr)                         \
+    do {                                      \
+        nrk_status nrk_s_ = (expr);           \
+        if (nrk_s_ != NRK_OK) return nrk_s_;  \
+    } while (0)
+
+// Make sure the calling thread has a current gfx950 device (nrk_init(0) on first use).
+nrk_status ensure_device();
+// Number of compute units of the current device (cached).
+int cu_count();
+
+// ---- per-stream scratch ----
+// Every stream gets its own scratch: a zero-invariant counter area (kernels that use a counter
+// reset it to zero before they exit, so it is zero at every launch) and a partials area whose
+// contents are undefined at launch.
+struct Scratch {
+    unsigned int *counters;  // zero at every launch boundary
+    size_t n_counters;
+    void *partials;
+    size_t partial_bytes;
+};
+nrk_status get_scratch(hipStream_t stream, size_t n_counters, size_t partial_bytes, Scratch *out);
+
+// Host-pointer staging: device copies of host operands, cached per thread and grown on demand.
+struct Staging {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+nrk_status staging(int slot, size_t bytes, void **ptr);
+
+inline size_t ceil_div(size_t a, size_t b) { return (a + b - 1) / b; }
+inline size_t align_up(size_t a, size_t b) { return ceil_div(a, b) * b; }
+
+// ---- device-side helpers ----
+template <typename T>
+struct Vec16;  // 16-byte vector of T
+template <>
+struct Vec16<float> {
+    typedef float type __attribute__((ext_vector_type(4)));
+    static constexpr int N = 4;
+};
+template <>
+struct Vec16<double> {
+    typedef double type __attribute__((ext_vector_type(2)));
+    static constexpr int N = 2;
+};
+
+template <typename T>
+__device__ __forceinline__ T dev_nan();
+template <>
+__device__ __forceinline__ float dev_nan<float>() { return __builtin_nanf(""); }
+template <>
+__device__ __forceinline__ double dev_nan<double>() { return __builtin_nan(""); }
+
+template <typename T>
+__device__ __forceinline__ T dev_inf();
+template <>
+__device__ __forceinline__ float dev_inf<float>() { return __builtin_inff(); }
+template <>
+__device__ __forceinline__ double dev_inf<double>() { return __builtin_inf(); }
+
+// One rounding per operation, never contracted.
+__device__ __forceinline__ float rn_add(float a, float b) { return __fadd_rn(a, b); }
+__device__ __forceinline__ double rn_add(double a, double b) { return __dadd_rn(a, b); }
+__device__ __forceinline__ float rn_sub(float a, float b) { return __fsub_rn(a, b); }
+__device__ __forceinline__ double rn_sub(double a, double b) { return __dsub_rn(a, b); }
+__device__ __forceinline__ float rn_mul(float a, float b) { return __fmul_rn(a, b); }
+__device__ __forceinline__ double rn_mul(double a, double b) { return __dmul_rn(a, b); }
+__device__ __forceinline__ float rn_div(float a, float b) { return __fdiv_rn(a, b); }
+__device__ __forceinline__ double rn_div(double a, double b) { return __ddiv_rn(a, b); }
+__device__ __forceinline__ float rn_sqrt(float a) { return __fsqrt_rn(a); }
+__device__ __forceinline__ double rn_sqrt(double a) { return __dsqrt_rn(a); }
+
+__device__ __forceinline__ float dev_fabs(float a) { return __builtin_fabsf(a); }
+__device__ __forceinline__ double dev_fabs(double a) { return __builtin_fabs(a); }
+// fmax/fmin return the non-NaN operand when exactly one operand is NaN.
+__device__ __forceinline__ float dev_fmax(float a, float b) { return __builtin_fmaxf(a, b); }
+__device__ __forceinline__ double dev_fmax(double a, double b) { return __builtin_fmax(a, b); }
+__device__ __forceinline__ float dev_fmin(float a, float b) { return __builtin_fminf(a, b); }
+__device__ __forceinline__ double dev_fmin(double a, double b) { return __builtin_fmin(a, b); }
+
+// Apply one scalar-chain step (NRK_SOP_*) with one IEEE rounding.
+template <typename T>
+__device__ __forceinline__ T apply_sop(int op, T s, T x) {
+    switch (op) {
+        case NRK_SOP_ADD: return rn_add(x, s);
+        case NRK_SOP_SUB: return rn_sub(x, s);
+        case NRK_SOP_MUL: return rn_mul(x, s);
+        case NRK_SOP_DIV: return rn_div(x, s);
+        case NRK_SOP_RSUB: return rn_sub(s, x);
+        case NRK_SOP_RDIV: return rn_div(s, x);
+        case NRK_SOP_NEG: return -x;
+        case NRK_SOP_ABS: return dev_fabs(x);
+        case NRK_SOP_SQUARE: return rn_mul(x, x);
+        case NRK_SOP_SQRT: return rn_sqrt(x);
+        default: return x;
+    }
+}
+
+// Scalar chain passed by value to kernels.
+struct Chain {
+    int len;
+    int op[NRK_MAX_CHAIN];
+    double s[NRK_MAX_CHAIN];
+};
+nrk_status make_chain(const nrk_scalar_op *chain, int32_t chain_len, Chain *out);
+
+}  // namespace nrk

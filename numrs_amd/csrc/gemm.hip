@@ -1,0 +1,243 @@
+// Plain GEMM on gfx950 matrix cores: C(m,n) := A(m,k) * B(k,n), row-major, overwrite.
+//
+// Reference: kernels/gemm.rs:246-312 (gemm_2d -> gemm_float_tier). Contract kept:
+//   m == 0 or n == 0 -> return without writing; k == 0 -> C zero-filled (NaN garbage
+//   overwritten, gemm.rs:295-303); beta = 0 always. K is never split across workgroups, so each
+//   C element is produced by exactly one wavefront in a fixed k order: results are bit-identical
+//   across launches, like the reference's M-only row split (gemm.rs:364-381).
+//
+// Kernel: workgroup tile BM x BN, K-tile BK, (BM/WM)*(BN/WN) wavefronts each owning a WM x WN
+// sub-tile of 16x16 MFMA accumulators. Operands are staged global -> registers -> LDS with two
+// LDS buffers, so the next K-tile's global loads are in flight while the current one feeds the
+// MFMAs. A is stored K-major in LDS (As[k][m]) so that the 16 lanes of an MFMA operand column
+// read consecutive words. Workgroup ids are remapped so that the tiles one XCD executes are
+// neighbours in the tile grid (each XCD has its own L2), then grouped GROUP_M tile-rows at a
+// time for A/B reuse.
+//
+// MFMA operand layout (16x16x4, f64 and f32): lane l supplies A[l%16][l/16] and B[l/16][l%16];
+// accumulator element r of lane l is D[4*(l/16) + r][l%16].
+#include "ops.hpp"
+
+namespace nrk {
+
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+    typedef double acc_t __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct Mfma<float> {
+    typedef float acc_t __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+constexpr int XCDS = 8;
+
+// Map the linear workgroup id to a (tile_m, tile_n) so that consecutive ids on one XCD cover
+// a compact block of the tile grid.
+__device__ __forceinline__ void tile_of(int nbm, int nbn, int group_m, int &tm, int &tn) {
+    const int total = nbm * nbn;
+    int g = blockIdx.x;
+    if (total % XCDS == 0) {
+        const int per = total / XCDS;
+        g = (g % XCDS) * per + g / XCDS;  // hardware places workgroup g on XCD g % 8
+    }
+    const int in_group = group_m * nbn;
+    const int grp = g / in_group;
+    const int first_m = grp * group_m;
+    const int gm = min(nbm - first_m, group_m);
+    const int r = g % in_group;
+    tm = first_m + r % gm;
+    tn = r / gm;
+}
+
+template <typename T, int BM, int BN, int BK, int WM, int WN>
+__global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_kernel(int M, int N, int K, const T *__restrict__ A,
+                                                                         const T *__restrict__ B, T *__restrict__ C,
+                                                                         int group_m) {
+    constexpr int WGN = BN / WN;
+    constexpr int NT = (BM / WM) * WGN * 64;
+    constexpr int TM = WM / 16, TN = WN / 16;
+    constexpr int PAD = 4;
+    constexpr int LA = (BM * BK) / NT;  // A elements per thread per K-tile
+    constexpr int LB = (BK * BN) / NT;
+    static_assert((BM * BK) % NT == 0 && (BK * BN) % NT == 0, "tile not divisible by threads");
+    static_assert(BK % 4 == 0, "BK multiple of the MFMA K");
+    typedef typename Mfma<T>::acc_t acc_t;
+
+    __shared__ T As[2][BK][BM + PAD];
+    __shared__ T Bs[2][BK][BN + PAD];
+
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int wm = wave / WGN, wn = wave % WGN;
+    int tm, tn;
+    tile_of((M + BM - 1) / BM, (N + BN - 1) / BN, group_m, tm, tn);
+    const int bm0 = tm * BM, bn0 = tn * BN;
+
+    acc_t acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+
+    T ra[LA], rb[LB];
+    auto load_tile = [&](int k0) {
+#pragma unroll
+        for (int s = 0; s < LA; ++s) {
+            const int q = tid + s * NT;
+            const int row = q / BK, col = q % BK;
+            const int gr = bm0 + row, gc = k0 + col;
+            ra[s] = (gr < M && gc < K) ? A[(size_t)gr * K + gc] : (T)0;
+        }
+#pragma unroll
+        for (int s = 0; s < LB; ++s) {
+            const int q = tid + s * NT;
+            const int row = q / BN, col = q % BN;
+            const int gr = k0 + row, gc = bn0 + col;
+            rb[s] = (gr < K && gc < N) ? B[(size_t)gr * N + gc] : (T)0;
+        }
+    };
+    auto store_tile = [&](int buf) {
+#pragma unroll
+        for (int s = 0; s < LA; ++s) {
+            const int q = tid + s * NT;
+            As[buf][q % BK][q / BK] = ra[s];
+        }
+#pragma unroll
+        for (int s = 0; s < LB; ++s) {
+            const int q = tid + s * NT;
+            Bs[buf][q / BN][q % BN] = rb[s];
+        }
+    };
+
+    const int nk = (K + BK - 1) / BK;
+    load_tile(0);
+    store_tile(0);
+    __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+        const int cur = kt & 1;
+        if (kt + 1 < nk) load_tile((kt + 1) * BK);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            T a[TM], b[TN];
+#pragma unroll
+            for (int i = 0; i < TM; ++i) a[i] = As[cur][kk + (lane >> 4)][wm * WM + i * 16 + (lane & 15)];
+#pragma unroll
+            for (int j = 0; j < TN; ++j) b[j] = Bs[cur][kk + (lane >> 4)][wn * WN + j * 16 + (lane & 15)];
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+#pragma unroll
+                for (int j = 0; j < TN; ++j) acc[i][j] = Mfma<T>::mma(a[i], b[j], acc[i][j]);
+        }
+        if (kt + 1 < nk) store_tile(cur ^ 1);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            const int col = bn0 + wn * WN + j * 16 + (lane & 15);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = bm0 + wm * WM + i * 16 + 4 * (lane >> 4) + r;
+                if (row < M && col < N) C[(size_t)row * N + col] = acc[i][j][r];
+            }
+        }
+    }
+}
+
+template <typename T>
+__global__ void zero_fill_kernel(T *c, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) c[i] = (T)0;
+}
+
+template <typename T, int BM, int BN, int BK, int WM, int WN>
+static nrk_status launch_gemm(int m, int n, int k, const T *a, const T *b, T *c, hipStream_t stream) {
+    const int nbm = (m + BM - 1) / BM, nbn = (n + BN - 1) / BN;
+    const int threads = (BM / WM) * (BN / WN) * 64;
+    const int group_m = 8;
+    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WM, WN>), dim3(nbm * nbn), dim3(threads), 0, stream, m, n, k, a, b, c,
+                       group_m);
+    NRK_HIP(hipGetLastError());
+    return NRK_OK;
+}
+
+template <typename T>
+nrk_status gemm(size_t m, size_t k, size_t n, const T *a, const T *b, T *c, hipStream_t stream) {
+    NRK_TRY(ensure_device());
+    if (m == 0 || n == 0) return NRK_OK;
+    if (!c) return fail(NRK_ERR_INVALID_ARGUMENT, "C is NULL");
+    if (k == 0) {
+        const size_t total = m * n;
+        int grid = (int)std::min<size_t>(ceil_div(total, 256), (size_t)cu_count() * 8);
+        hipLaunchKernelGGL(zero_fill_kernel<T>, dim3(grid), dim3(256), 0, stream, c, total);
+        NRK_HIP(hipGetLastError());
+        return NRK_OK;
+    }
+    if (!a || !b) return fail(NRK_ERR_INVALID_ARGUMENT, "A or B is NULL");
+    if (m > (size_t)INT32_MAX / 2 || n > (size_t)INT32_MAX / 2 || k > (size_t)INT32_MAX / 2)
+        return fail(NRK_ERR_INVALID_ARGUMENT, "dimension exceeds 2^30");
+    const size_t tiles128 = ceil_div(m, 128) * ceil_div(n, 128);
+    if (tiles128 >= (size_t)cu_count()) {
+        return launch_gemm<T, 128, 128, 16, 64, 64>((int)m, (int)n, (int)k, a, b, c, stream);
+    }
+    return launch_gemm<T, 64, 64, 16, 32, 32>((int)m, (int)n, (int)k, a, b, c, stream);
+}
+
+template nrk_status gemm<double>(size_t, size_t, size_t, const double *, const double *, double *, hipStream_t);
+template nrk_status gemm<float>(size_t, size_t, size_t, const float *, const float *, float *, hipStream_t);
+
+}  // namespace nrk
+
+using namespace nrk;
+
+namespace {
+template <typename T>
+nrk_status host_gemm(size_t m, size_t k, size_t n, const T *a, size_t a_len, const T *b, size_t b_len, T *c, size_t c_len) {
+    NRK_TRY(ensure_device());
+    if (c_len > 0 && !c) return fail(NRK_ERR_INVALID_ARGUMENT, "c is NULL");
+    const bool lens_ok = a_len == m * k && b_len == k * n && c_len == m * n;
+    if (!lens_ok) {
+        // gemm.rs:250-253 -> gemm_generic's defensive branch (gemm.rs:408-423): zero C, read nothing
+        for (size_t i = 0; i < c_len; ++i) c[i] = (T)0;
+        return NRK_OK;
+    }
+    if (m == 0 || n == 0) return NRK_OK;
+    void *da = nullptr, *db = nullptr, *dc = nullptr;
+    NRK_TRY(staging(0, a_len * sizeof(T) + 16, &da));
+    NRK_TRY(staging(2, b_len * sizeof(T) + 16, &db));
+    NRK_TRY(staging(1, c_len * sizeof(T) + 16, &dc));
+    hipStream_t s = nullptr;
+    if (a_len) NRK_HIP(hipMemcpyAsync(da, a, a_len * sizeof(T), hipMemcpyHostToDevice, s));
+    if (b_len) NRK_HIP(hipMemcpyAsync(db, b, b_len * sizeof(T), hipMemcpyHostToDevice, s));
+    NRK_TRY(gemm<T>(m, k, n, (const T *)da, (const T *)db, (T *)dc, s));
+    NRK_HIP(hipMemcpyAsync(c, dc, c_len * sizeof(T), hipMemcpyDeviceToHost, s));
+    NRK_HIP(hipStreamSynchronize(s));
+    return NRK_OK;
+}
+}  // namespace
+
+extern "C" {
+
+nrk_status nrk_gemm_2d_f64(size_t m, size_t k, size_t n, const double *a, size_t a_len, const double *b, size_t b_len,
+                           double *c, size_t c_len) {
+    return host_gemm<double>(m, k, n, a, a_len, b, b_len, c, c_len);
+}
+nrk_status nrk_gemm_2d_f32(size_t m, size_t k, size_t n, const float *a, size_t a_len, const float *b, size_t b_len,
+                           float *c, size_t c_len) {
+    return host_gemm<float>(m, k, n, a, a_len, b, b_len, c, c_len);
+}
+nrk_status nrk_dev_gemm_f64(size_t m, size_t k, size_t n, const double *a, const double *b, double *c, nrk_stream stream) {
+    return gemm<double>(m, k, n, a, b, c, (hipStream_t)stream);
+}
+nrk_status nrk_dev_gemm_f32(size_t m, size_t k, size_t n, const float *a, const float *b, float *c, nrk_stream stream) {
+    return gemm<float>(m, k, n, a, b, c, (hipStream_t)stream);
+}
+
+}  // extern "C"

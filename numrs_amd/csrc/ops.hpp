@@ -1,0 +1,33 @@
+// Internal launchers (device pointers, one stream) behind the extern "C" entry points.
+#pragma once
+
+#include "common.hpp"
+
+namespace nrk {
+
+// Fused chain -> {sum, mean, var, min, max, argmin, argmax} along axis 1 of a (rows, cols)
+// matrix and/or over all of it, in one pass (reduce.hip).
+template <typename T>
+nrk_status seg_reduce(const Chain &chain, const T *x, size_t rows, size_t cols, double ddof,
+                      const nrk_reduce_outputs *row_out, const nrk_reduce_outputs *full_out, hipStream_t stream);
+
+// Reductions along a strided axis (inner > 1): one thread per output, left-to-right fold in the
+// reference's order (reduce_axis.hip).
+template <typename T>
+nrk_status strided_reduce(int op, const T *x, size_t outer, size_t len, size_t inner, double ddof, void *out,
+                          hipStream_t stream);
+
+nrk_status sum_i64(const int64_t *x, size_t n, int64_t *out, hipStream_t stream);
+template <typename T>
+nrk_status prod(const T *x, size_t n, T *out, hipStream_t stream);
+
+template <typename T>
+nrk_status map(const Chain &chain, const T *x, T *y, size_t n, hipStream_t stream);
+template <typename T>
+nrk_status zip(int op, const T *a, const T *b, T *y, size_t n, hipStream_t stream);
+
+// Plain GEMM on MFMA: C(m,n) = A(m,k) * B(k,n), row-major, overwrite (gemm.hip).
+template <typename T>
+nrk_status gemm(size_t m, size_t k, size_t n, const T *a, const T *b, T *c, hipStream_t stream);
+
+}  // namespace nrk

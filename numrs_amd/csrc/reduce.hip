@@ -1,0 +1,722 @@
+// Streaming reductions for gfx950: sum / mean / var / min / max / argmin / argmax over rows of
+// a row-major matrix and over the whole matrix, optionally after an elementwise scalar chain
+// (the fused map -> reduce path), in ONE pass over HBM.
+//
+// Reference semantics restated here (paths under the reference's src/):
+//   kernels/reduce.rs:207-421   sum/mean/var(ddof)/min/max; empty -> 0; any NaN -> min/max NaN
+//   math/statistics/mod.rs:79-183   argmax: first strict maximum, interior NaN skipped, NaN at
+//                                   index 0 pins the result to 0 (argmin symmetric, :218-322)
+//   math/aggregation.rs:244-261     axis max/min: NaN short-circuits to NaN
+//   math/statistics/mod.rs:1403-1494 axis var = sum((x - mean)^2) / (len - ddof)
+//
+// Work decomposition. Row r of `cols` elements is cut into fixed segments of SEG elements
+// (SEG = 256 threads * VPT 16-byte vectors). One workgroup-iteration reduces one segment in
+// registers: the values stay in VGPRs between the sum pass and the second-moment pass, so
+// variance costs no second read. Each segment yields a partial record; rows that fit in one
+// segment are finalised in place, longer rows and the whole-matrix result are finished by the
+// workgroup that retires the last segment (agent-scope acq_rel counter, self-resetting).
+//
+// Determinism. Every sum is a fixed tree: a pairwise tree over each thread's elements, an xor
+// butterfly across the wave, a fixed-order combine of the 4 waves, and a fixed strided
+// assignment of partials to threads in the finish. None of it depends on the grid size or on
+// which workgroup finishes last, so results are bit-identical across launches and devices.
+//
+// Precision ("compensated" mode, the only mode): per-thread pairwise partials in T,
+// double-word (TwoSum) combination across the workgroup, FP64 double-word cross-segment
+// finish, and the second moment combined in FP64 with Chan's shift formula. max/min/argmax/
+// argmin are exact (order-independent): the winner is the largest value, ties going to the
+// lowest index, and the value written is the element itself, so even the sign of a zero
+// maximum matches the reference's left-to-right fold.
+#include <cstring>
+
+#include "common.hpp"
+
+namespace nrk {
+
+constexpr int RB = 256;  // threads per reduction workgroup
+constexpr int WAVES = RB / 64;
+constexpr uint64_t NO_INDEX = ~0ull;
+
+enum : int { F_M2 = 1, F_MAX = 2, F_MIN = 4 };
+
+template <typename T>
+struct OutPtrs {
+    T *sum, *mean, *var, *min, *max;
+    uint64_t *argmin, *argmax;
+};
+
+template <typename T>
+struct SegParams {
+    const T *x;
+    size_t rows, cols, spr, nseg;
+    Chain chain;
+    double ddof;
+    int vec_ok;
+    int want_row, want_full, need_partials;
+    unsigned int *row_counters;  // rows entries (only when spr > 1 && want_row)
+    unsigned int *full_counter;
+    // partial records, structure of arrays, nseg entries each
+    double *p_hi, *p_lo, *p_q, *p_center, *p_max, *p_min;
+    uint64_t *p_imax, *p_imin;
+    unsigned int *p_flags;  // bit0 saw NaN, bit1 first element of the row is NaN
+    OutPtrs<T> row, full;
+};
+
+// ---------------- small device utilities ----------------
+template <typename T>
+__device__ __forceinline__ void two_sum(T a, T b, T &s, T &e) {
+    s = rn_add(a, b);
+    T v = rn_sub(s, a);
+    e = rn_add(rn_sub(a, rn_sub(s, v)), rn_sub(b, v));
+}
+
+// double-word addition (a_hi + a_lo) + (b_hi + b_lo), symmetric in its arguments
+template <typename T>
+__device__ __forceinline__ void dw_add(T &hi, T &lo, T bhi, T blo) {
+    T s, e;
+    two_sum(hi, bhi, s, e);
+    e = rn_add(e, rn_add(lo, blo));
+    hi = rn_add(s, e);
+    lo = rn_sub(e, rn_sub(hi, s));
+}
+
+__device__ __forceinline__ float shfl_xor(float v, int m) { return __shfl_xor(v, m, 64); }
+__device__ __forceinline__ double shfl_xor(double v, int m) { return __shfl_xor(v, m, 64); }
+__device__ __forceinline__ uint64_t shfl_xor(uint64_t v, int m) {
+    unsigned lo = __shfl_xor((unsigned)(v & 0xffffffffu), m, 64);
+    unsigned hi = __shfl_xor((unsigned)(v >> 32), m, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// (value, index) winner for argmax: larger value wins, equal values go to the lower index.
+// Values are never NaN here (NaN elements are excluded before this point).
+template <typename T>
+__device__ __forceinline__ void take_max(T &v, uint64_t &i, T bv, uint64_t bi) {
+    if (bv > v || (bv == v && bi < i)) { v = bv; i = bi; }
+}
+template <typename T>
+__device__ __forceinline__ void take_min(T &v, uint64_t &i, T bv, uint64_t bi) {
+    if (bv < v || (bv == v && bi < i)) { v = bv; i = bi; }
+}
+
+// Per-wave results exchanged through LDS; double-buffered by segment parity so that the next
+// segment's writes never race with a slower wave still reading this segment's values.
+template <typename T>
+struct WaveSlots {
+    T hi[WAVES], lo[WAVES];
+    T vmax[WAVES], vmin[WAVES];
+    uint64_t imax[WAVES], imin[WAVES];
+    int nan[WAVES];
+    double q[WAVES];
+};
+struct FinishSlots {
+    double hi[WAVES], lo[WAVES], q[WAVES], vmax[WAVES], vmin[WAVES];
+    uint64_t imax[WAVES], imin[WAVES];
+    int nan[WAVES];
+};
+template <typename T>
+struct BlockShared {
+    WaveSlots<T> w[2];
+    FinishSlots f;
+    int last;
+};
+
+// ---- wave-level reductions (xor butterfly: every lane ends with the same value) ----
+template <typename T>
+__device__ __forceinline__ void wave_dw_sum(T &hi, T &lo) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        T bh = shfl_xor(hi, m), bl = shfl_xor(lo, m);
+        dw_add(hi, lo, bh, bl);
+    }
+}
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += shfl_xor(v, m);
+    return v;
+}
+template <typename T, bool IS_MAX>
+__device__ __forceinline__ void wave_arg(T &v, uint64_t &i) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        T bv = shfl_xor(v, m);
+        uint64_t bi = shfl_xor(i, m);
+        if (IS_MAX) take_max(v, i, bv, bi); else take_min(v, i, bv, bi);
+    }
+}
+
+// Pairwise (balanced binary tree) sum of f(OFF) .. f(OFF+N-1), written depth-first.
+template <int N, int OFF, typename T, typename F>
+__device__ __forceinline__ T pairwise(const F &f) {
+    if constexpr (N == 1) {
+        return f(OFF);
+    } else {
+        return rn_add(pairwise<N / 2, OFF, T>(f), pairwise<N / 2, OFF + N / 2, T>(f));
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void write_outputs(const OutPtrs<T> &o, size_t at, double s_hi, double s_lo, double q_true,
+                                              double count, double ddof, int nan, int first_nan, T vmax, uint64_t imax,
+                                              T vmin, uint64_t imin) {
+    if (o.sum) o.sum[at] = (T)(s_hi + s_lo);
+    if (o.mean) o.mean[at] = (T)((s_hi + s_lo) / count);
+    if (o.var) o.var[at] = (T)(q_true / (count - ddof));
+    if (o.max) o.max[at] = nan ? dev_nan<T>() : vmax;
+    if (o.min) o.min[at] = nan ? dev_nan<T>() : vmin;
+    if (o.argmax) o.argmax[at] = first_nan ? 0 : imax;
+    if (o.argmin) o.argmin[at] = first_nan ? 0 : imin;
+}
+
+// Combine the partial records of segments [lo, hi) (a whole row, or every segment) with the
+// whole workgroup and write one set of outputs. Partial s goes to thread (s - lo) % RB, each
+// thread folds its partials in increasing s, then a fixed butterfly + fixed wave order:
+// deterministic whichever workgroup runs it.
+template <typename T, int FLAGS, int SEG>
+__device__ __noinline__ void finish_range(const SegParams<T> &p, size_t lo, size_t hi, bool full, const OutPtrs<T> &o,
+                                          size_t at, FinishSlots *fs) {
+    const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+    const size_t count_total = full ? p.rows * p.cols : p.cols;
+    double shi = 0.0, slo = 0.0;
+    int nan = 0;
+    double vmax = -__builtin_inf(), vmin = __builtin_inf();
+    uint64_t imax = NO_INDEX, imin = NO_INDEX;
+    for (size_t s = lo + t; s < hi; s += RB) {
+        dw_add(shi, slo, p.p_hi[s], p.p_lo[s]);
+        nan |= p.p_flags[s] & 1;
+        const size_t base = full ? (s / p.spr) * p.cols : 0;
+        if (FLAGS & F_MAX) {
+            const uint64_t ix = p.p_imax[s];
+            take_max(vmax, imax, p.p_max[s], ix == NO_INDEX ? NO_INDEX : base + ix);
+        }
+        if (FLAGS & F_MIN) {
+            const uint64_t ix = p.p_imin[s];
+            take_min(vmin, imin, p.p_min[s], ix == NO_INDEX ? NO_INDEX : base + ix);
+        }
+    }
+    wave_dw_sum(shi, slo);
+    nan = __any(nan) ? 1 : 0;
+    if (FLAGS & F_MAX) wave_arg<double, true>(vmax, imax);
+    if (FLAGS & F_MIN) wave_arg<double, false>(vmin, imin);
+    __syncthreads();
+    if (lane == 0) {
+        fs->hi[w] = shi; fs->lo[w] = slo; fs->nan[w] = nan;
+        fs->vmax[w] = vmax; fs->imax[w] = imax; fs->vmin[w] = vmin; fs->imin[w] = imin;
+    }
+    __syncthreads();
+    shi = fs->hi[0]; slo = fs->lo[0]; nan = fs->nan[0];
+    vmax = fs->vmax[0]; imax = fs->imax[0]; vmin = fs->vmin[0]; imin = fs->imin[0];
+#pragma unroll
+    for (int k = 1; k < WAVES; ++k) {
+        dw_add(shi, slo, fs->hi[k], fs->lo[k]);
+        nan |= fs->nan[k];
+        take_max(vmax, imax, fs->vmax[k], fs->imax[k]);
+        take_min(vmin, imin, fs->vmin[k], fs->imin[k]);
+    }
+    const double mu = (shi + slo) / (double)count_total;
+    double q = 0.0;
+    if (FLAGS & F_M2) {
+        for (size_t s = lo + t; s < hi; s += RB) {
+            const size_t c = s % p.spr;
+            const double n_s = (double)min((size_t)SEG, p.cols - c * SEG);
+            const double mu_s = (p.p_hi[s] + p.p_lo[s]) / n_s;
+            const double m_s = p.p_center[s];
+            // Q_s is taken about m_s; shift it to the global mean mu (Chan et al.)
+            q += p.p_q[s] + n_s * (m_s - mu) * (2.0 * mu_s - mu - m_s);
+        }
+        q = wave_sum_f64(q);
+        if (lane == 0) fs->q[w] = q;
+        __syncthreads();
+        q = fs->q[0];
+#pragma unroll
+        for (int k = 1; k < WAVES; ++k) q += fs->q[k];
+        if (q < 0.0) q = 0.0;
+    }
+    if (t == 0) {
+        const int first_nan = (p.p_flags[lo] >> 1) & 1;
+        write_outputs<T>(o, at, shi, slo, q, (double)count_total, p.ddof, nan, first_nan, (T)vmax, imax, (T)vmin, imin);
+    }
+    __syncthreads();
+}
+
+template <typename T, int VPT, int FLAGS, bool FULLSEG>
+__device__ __forceinline__ void reduce_segment(const SegParams<T> &p, size_t seg, int parity, BlockShared<T> *sh) {
+    typedef typename Vec16<T>::type V;
+    constexpr int VN = Vec16<T>::N;
+    constexpr int M = VPT * VN;  // elements per thread (<= 32: one validity bit each)
+    constexpr int SEG = RB * M;
+    static_assert(M <= 32, "validity mask holds 32 elements");
+    const unsigned t = threadIdx.x, w = t >> 6, lane = t & 63;
+    const size_t r = seg / p.spr, c = seg - r * p.spr;
+    const size_t col0 = c * SEG;
+    const unsigned len = FULLSEG ? (unsigned)SEG : (unsigned)min((size_t)SEG, p.cols - col0);
+    const T *base = p.x + r * p.cols + col0;  // uniform across the workgroup
+
+    // ---- load: element i = j*VN + e of this thread sits at segment position (j*RB + t)*VN + e
+    T y[M];
+    unsigned vmask = ~0u;
+    if (FULLSEG && p.vec_ok) {
+        const V *bv = reinterpret_cast<const V *>(base);
+#pragma unroll
+        for (int j = 0; j < VPT; ++j) {
+            V v = __builtin_nontemporal_load(bv + (j * RB + t));
+#pragma unroll
+            for (int e = 0; e < VN; ++e) y[j * VN + e] = v[e];
+        }
+    } else {
+        if (!FULLSEG) vmask = 0u;
+#pragma unroll
+        for (int j = 0; j < VPT; ++j) {
+#pragma unroll
+            for (int e = 0; e < VN; ++e) {
+                const unsigned pos = (j * RB + t) * VN + e;
+                const bool ok = FULLSEG || pos < len;
+                y[j * VN + e] = ok ? base[pos] : (T)0;
+                if (!FULLSEG) vmask |= (ok ? 1u : 0u) << (j * VN + e);
+            }
+        }
+    }
+    auto valid = [&](int i) -> bool { return FULLSEG || ((vmask >> i) & 1u); };
+    auto pos_of = [&](int i) -> unsigned { return ((i / VN) * RB + t) * VN + (i % VN); };
+
+    // ---- elementwise chain, one rounding per step (uniform branch per step)
+    for (int k = 0; k < p.chain.len; ++k) {
+        const int op = p.chain.op[k];
+        const T s = (T)p.chain.s[k];
+#pragma unroll
+        for (int i = 0; i < M; ++i) y[i] = apply_sop<T>(op, s, y[i]);
+    }
+
+    // ---- phase 1: sum, max/argmax, min/argmin, NaN flag
+    T hi = pairwise<M, 0, T>([&](int i) { return valid(i) ? y[i] : (T)0; });
+    T lo = (T)0;
+    int nan = 0;
+    if (__builtin_isnan(hi)) {  // rare: tell a NaN element apart from inf - inf
+#pragma unroll
+        for (int i = 0; i < M; ++i) nan |= (valid(i) && __builtin_isnan(y[i])) ? 1 : 0;
+    }
+    T vmax = -dev_inf<T>(), vmin = dev_inf<T>();
+    uint64_t imax = NO_INDEX, imin = NO_INDEX;
+    if (FLAGS & F_MAX) {
+        T m = -dev_inf<T>();
+#pragma unroll
+        for (int i = 0; i < M; ++i) m = valid(i) ? dev_fmax(m, y[i]) : m;  // fmax skips NaN
+        int first = -1;
+#pragma unroll
+        for (int i = M - 1; i >= 0; --i) first = (valid(i) && y[i] == m) ? i : first;
+        if (first >= 0) {
+            vmax = m;
+#pragma unroll
+            for (int i = 0; i < M; ++i) vmax = (i == first) ? y[i] : vmax;  // keeps the sign of a zero
+            imax = col0 + pos_of(first);
+        }
+    }
+    if (FLAGS & F_MIN) {
+        T m = dev_inf<T>();
+#pragma unroll
+        for (int i = 0; i < M; ++i) m = valid(i) ? dev_fmin(m, y[i]) : m;
+        int first = -1;
+#pragma unroll
+        for (int i = M - 1; i >= 0; --i) first = (valid(i) && y[i] == m) ? i : first;
+        if (first >= 0) {
+            vmin = m;
+#pragma unroll
+            for (int i = 0; i < M; ++i) vmin = (i == first) ? y[i] : vmin;
+            imin = col0 + pos_of(first);
+        }
+    }
+    wave_dw_sum(hi, lo);
+    nan = __any(nan) ? 1 : 0;
+    if (FLAGS & F_MAX) wave_arg<T, true>(vmax, imax);
+    if (FLAGS & F_MIN) wave_arg<T, false>(vmin, imin);
+    WaveSlots<T> &ws = sh->w[parity];
+    if (lane == 0) {
+        ws.hi[w] = hi; ws.lo[w] = lo; ws.nan[w] = nan;
+        if (FLAGS & F_MAX) { ws.vmax[w] = vmax; ws.imax[w] = imax; }
+        if (FLAGS & F_MIN) { ws.vmin[w] = vmin; ws.imin[w] = imin; }
+    }
+    __syncthreads();
+    hi = ws.hi[0]; lo = ws.lo[0]; nan = ws.nan[0];
+    if (FLAGS & F_MAX) { vmax = ws.vmax[0]; imax = ws.imax[0]; }
+    if (FLAGS & F_MIN) { vmin = ws.vmin[0]; imin = ws.imin[0]; }
+#pragma unroll
+    for (int k = 1; k < WAVES; ++k) {
+        dw_add(hi, lo, ws.hi[k], ws.lo[k]);
+        nan |= ws.nan[k];
+        if (FLAGS & F_MAX) take_max(vmax, imax, ws.vmax[k], ws.imax[k]);
+        if (FLAGS & F_MIN) take_min(vmin, imin, ws.vmin[k], ws.imin[k]);
+    }
+    const T center = rn_div(hi, (T)len);
+
+    // ---- phase 2: second moment about `center`, from registers (no second read)
+    double q = 0.0;
+    if (FLAGS & F_M2) {
+        T qt = pairwise<M, 0, T>([&](int i) {
+            const T d = valid(i) ? rn_sub(y[i], center) : (T)0;
+            return rn_mul(d, d);
+        });
+        q = wave_sum_f64((double)qt);
+        if (lane == 0) ws.q[w] = q;
+        __syncthreads();
+        q = ws.q[0];
+#pragma unroll
+        for (int k = 1; k < WAVES; ++k) q += ws.q[k];
+    }
+    const int first_nan = (c == 0 && t == 0 && __builtin_isnan(y[0])) ? 1 : 0;
+
+    if (p.spr == 1 && p.want_row && t == 0) {
+        // the whole row is this segment: finalise in place
+        const double s_hi = (double)hi, s_lo = (double)lo;
+        const double mu = (s_hi + s_lo) / (double)len;
+        const double dc = (double)center - mu;
+        const double q_true = fmax(q - (double)len * dc * dc, 0.0);
+        write_outputs<T>(p.row, r, s_hi, s_lo, q_true, (double)len, p.ddof, nan, first_nan, vmax, imax, vmin, imin);
+    }
+    if (!p.need_partials) return;
+    if (t == 0) {
+        p.p_hi[seg] = (double)hi;
+        p.p_lo[seg] = (double)lo;
+        p.p_q[seg] = q;
+        p.p_center[seg] = (double)center;
+        p.p_max[seg] = (double)vmax;
+        p.p_min[seg] = (double)vmin;
+        p.p_imax[seg] = imax;
+        p.p_imin[seg] = imin;
+        p.p_flags[seg] = (unsigned)nan | ((unsigned)first_nan << 1);
+    }
+    // ---- row finish (rows spanning several segments): the last segment of the row does it
+    if (p.spr > 1 && p.want_row) {
+        __syncthreads();
+        if (t == 0) {
+            __threadfence();
+            unsigned old = __hip_atomic_fetch_add(&p.row_counters[r], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            int last = (old == (unsigned)(p.spr - 1));
+            if (last) __hip_atomic_store(&p.row_counters[r], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            sh->last = last;
+        }
+        __syncthreads();
+        if (sh->last) {
+            __threadfence();
+            finish_range<T, FLAGS, SEG>(p, r * p.spr, (r + 1) * p.spr, false, p.row, r, &sh->f);
+        }
+    }
+    // ---- whole-matrix finish: the workgroup that retires the last segment does it
+    if (p.want_full) {
+        __syncthreads();
+        if (t == 0) {
+            __threadfence();
+            unsigned old = __hip_atomic_fetch_add(p.full_counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            int last = (old == (unsigned)(p.nseg - 1));
+            if (last) __hip_atomic_store(p.full_counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            sh->last = last;
+        }
+        __syncthreads();
+        if (sh->last) {
+            __threadfence();
+            finish_range<T, FLAGS, SEG>(p, 0, p.nseg, true, p.full, 0, &sh->f);
+        }
+    }
+}
+
+template <typename T, int VPT, int FLAGS>
+__global__ __launch_bounds__(RB) void seg_reduce_kernel(SegParams<T> p) {
+    constexpr int SEG = RB * VPT * Vec16<T>::N;
+    __shared__ BlockShared<T> sh;
+    int parity = 0;
+    for (size_t seg = blockIdx.x; seg < p.nseg; seg += gridDim.x, parity ^= 1) {
+        const size_t c = seg % p.spr;
+        if (c * SEG + SEG <= p.cols) {
+            reduce_segment<T, VPT, FLAGS, true>(p, seg, parity, &sh);
+        } else {
+            reduce_segment<T, VPT, FLAGS, false>(p, seg, parity, &sh);
+        }
+    }
+}
+
+// Writes T zero (or NaN) into up to 7 scalar outputs; used for empty inputs so that no
+// hipMemsetAsync is ever recorded into a caller's graph.
+template <typename T>
+__global__ void fill_outputs_kernel(OutPtrs<T> o, size_t n, T value) {
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (o.sum) o.sum[i] = value;
+    if (o.mean) o.mean[i] = value;
+    if (o.var) o.var[i] = value;
+    if (o.min) o.min[i] = value;
+    if (o.max) o.max[i] = value;
+    if (o.argmin) o.argmin[i] = 0;
+    if (o.argmax) o.argmax[i] = 0;
+}
+
+template <typename T>
+static OutPtrs<T> to_ptrs(const nrk_reduce_outputs *o) {
+    OutPtrs<T> p{};
+    if (!o) return p;
+    p.sum = (T *)o->sum;
+    p.mean = (T *)o->mean;
+    p.var = (T *)o->var;
+    p.min = (T *)o->min;
+    p.max = (T *)o->max;
+    p.argmin = o->argmin;
+    p.argmax = o->argmax;
+    return p;
+}
+
+template <typename T>
+static bool any_out(const OutPtrs<T> &o) {
+    return o.sum || o.mean || o.var || o.min || o.max || o.argmin || o.argmax;
+}
+
+template <typename T, int VPT, int FLAGS>
+static nrk_status launch_flags(SegParams<T> &p, hipStream_t stream) {
+    int grid = (int)std::min<size_t>(p.nseg, (size_t)cu_count() * 8);
+    hipLaunchKernelGGL((seg_reduce_kernel<T, VPT, FLAGS>), dim3(grid), dim3(RB), 0, stream, p);
+    NRK_HIP(hipGetLastError());
+    return NRK_OK;
+}
+
+// Host launcher for the fused map -> reduce over a (rows, cols) matrix.
+template <typename T>
+nrk_status seg_reduce(const Chain &chain, const T *x, size_t rows, size_t cols, double ddof,
+                      const nrk_reduce_outputs *row_out, const nrk_reduce_outputs *full_out, hipStream_t stream) {
+    constexpr int VPT = 8;
+    constexpr int SEG = RB * VPT * Vec16<T>::N;
+    NRK_TRY(ensure_device());
+    SegParams<T> p{};
+    p.row = to_ptrs<T>(row_out);
+    p.full = to_ptrs<T>(full_out);
+    p.want_row = any_out(p.row);
+    p.want_full = any_out(p.full);
+    if (!p.want_row && !p.want_full) return NRK_OK;
+    if (p.want_row && (p.row.argmax || p.row.argmin || p.row.max || p.row.min) && cols == 0)
+        return fail(NRK_ERR_INVALID_ARGUMENT, "max/min/argmax/argmin along an empty axis");
+    if (p.want_full && (p.full.argmax || p.full.argmin) && rows * cols == 0)
+        return fail(NRK_ERR_INVALID_ARGUMENT, "argmax/argmin of an empty array");
+    if (!x && rows * cols > 0) return fail(NRK_ERR_INVALID_ARGUMENT, "x is NULL");
+    if (rows * cols == 0) {
+        // reference: empty -> 0 for sum/mean/min/max/var (kernels/reduce.rs:207-421)
+        if (p.want_full) {
+            hipLaunchKernelGGL(fill_outputs_kernel<T>, dim3(1), dim3(64), 0, stream, p.full, (size_t)1, (T)0);
+            NRK_HIP(hipGetLastError());
+        }
+        if (p.want_row && rows > 0) {
+            hipLaunchKernelGGL(fill_outputs_kernel<T>, dim3((unsigned)ceil_div(rows, 256)), dim3(256), 0, stream, p.row, rows,
+                               (T)0);
+            NRK_HIP(hipGetLastError());
+        }
+        return NRK_OK;
+    }
+    p.x = x;
+    p.rows = rows;
+    p.cols = cols;
+    p.chain = chain;
+    p.ddof = ddof;
+    p.spr = ceil_div(cols, SEG);
+    p.nseg = rows * p.spr;
+    if (p.nseg >= (size_t)UINT32_MAX) return fail(NRK_ERR_INVALID_ARGUMENT, "too many segments (%zu)", p.nseg);
+    p.vec_ok = ((reinterpret_cast<uintptr_t>(x) % 16) == 0) && (cols % Vec16<T>::N == 0);
+    p.need_partials = p.want_full || (p.spr > 1 && p.want_row);
+    const size_t n_counters = 1 + ((p.spr > 1 && p.want_row) ? rows : 0);
+    const size_t rec = 8 * 8 + 4;
+    const size_t pbytes = p.need_partials ? align_up(p.nseg * rec + 64 * 9, 256) : 0;
+    Scratch s;
+    NRK_TRY(get_scratch(stream, n_counters, pbytes, &s));
+    p.full_counter = s.counters;
+    p.row_counters = s.counters + 1;
+    if (p.need_partials) {
+        char *b = (char *)s.partials;
+        size_t off = 0;
+        auto take = [&](size_t bytes) {
+            void *r = b + off;
+            off = align_up(off + bytes, 256);
+            return r;
+        };
+        p.p_hi = (double *)take(p.nseg * 8);
+        p.p_lo = (double *)take(p.nseg * 8);
+        p.p_q = (double *)take(p.nseg * 8);
+        p.p_center = (double *)take(p.nseg * 8);
+        p.p_max = (double *)take(p.nseg * 8);
+        p.p_min = (double *)take(p.nseg * 8);
+        p.p_imax = (uint64_t *)take(p.nseg * 8);
+        p.p_imin = (uint64_t *)take(p.nseg * 8);
+        p.p_flags = (unsigned *)take(p.nseg * 4);
+    }
+    const bool m2 = (p.row.var || p.full.var);
+    const bool mx = (p.row.max || p.row.argmax || p.full.max || p.full.argmax);
+    const bool mn = (p.row.min || p.row.argmin || p.full.min || p.full.argmin);
+    const int flags = (m2 ? F_M2 : 0) | (mx ? F_MAX : 0) | (mn ? F_MIN : 0);
+    switch (flags) {
+        case 0: return launch_flags<T, VPT, 0>(p, stream);
+        case 1: return launch_flags<T, VPT, 1>(p, stream);
+        case 2: return launch_flags<T, VPT, 2>(p, stream);
+        case 3: return launch_flags<T, VPT, 3>(p, stream);
+        case 4: return launch_flags<T, VPT, 4>(p, stream);
+        case 5: return launch_flags<T, VPT, 5>(p, stream);
+        case 6: return launch_flags<T, VPT, 6>(p, stream);
+        default: return launch_flags<T, VPT, 7>(p, stream);
+    }
+}
+
+template nrk_status seg_reduce<float>(const Chain &, const float *, size_t, size_t, double, const nrk_reduce_outputs *,
+                                      const nrk_reduce_outputs *, hipStream_t);
+template nrk_status seg_reduce<double>(const Chain &, const double *, size_t, size_t, double, const nrk_reduce_outputs *,
+                                       const nrk_reduce_outputs *, hipStream_t);
+
+// ---------------- i64 sum: wrapping integer adds, exact and order-independent ----------------
+constexpr int I64_VPT = 8;                     // 16-byte vectors per thread per segment
+constexpr int I64_SEG = RB * I64_VPT * 2;      // 4096 elements = 32 KiB
+
+__global__ __launch_bounds__(RB) void sum_i64_kernel(const int64_t *x, size_t n, size_t nseg, int vec_ok,
+                                                     unsigned long long *partials, unsigned int *counter,
+                                                     int64_t *out) {
+    typedef long long V2 __attribute__((ext_vector_type(2)));
+    __shared__ unsigned long long sred[WAVES];
+    __shared__ int slast;
+    const int t = threadIdx.x;
+    for (size_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
+        const size_t off = seg * I64_SEG;
+        unsigned long long acc = 0;  // unsigned: wrapping is defined, bits equal two's complement i64
+        if (vec_ok && off + I64_SEG <= n) {
+            const V2 *bv = reinterpret_cast<const V2 *>(x + off);
+#pragma unroll
+            for (int j = 0; j < I64_VPT; ++j) {
+                V2 v = __builtin_nontemporal_load(bv + j * RB + t);
+                acc += (unsigned long long)v[0] + (unsigned long long)v[1];
+            }
+        } else {
+            for (int j = 0; j < I64_VPT * 2; ++j) {
+                size_t pos = off + (size_t)j * RB + t;
+                if (pos < n) acc += (unsigned long long)x[pos];
+            }
+        }
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) acc += shfl_xor((uint64_t)acc, m);
+        if ((t & 63) == 0) sred[t >> 6] = acc;
+        __syncthreads();
+        if (t == 0) {
+            unsigned long long s = 0;
+            for (int k = 0; k < WAVES; ++k) s += sred[k];
+            partials[seg] = s;
+            __threadfence();
+            unsigned old = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            int last = old == (unsigned)(nseg - 1);
+            if (last) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            slast = last;
+        }
+        __syncthreads();
+        const int last = slast;
+        __syncthreads();
+        if (last) {
+            __threadfence();
+            unsigned long long s = 0;
+            for (size_t k = t; k < nseg; k += RB) s += partials[k];
+#pragma unroll
+            for (int m = 32; m >= 1; m >>= 1) s += shfl_xor((uint64_t)s, m);
+            if ((t & 63) == 0) sred[t >> 6] = s;
+            __syncthreads();
+            if (t == 0) {
+                unsigned long long tot = 0;
+                for (int k = 0; k < WAVES; ++k) tot += sred[k];
+                *out = (int64_t)tot;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+__global__ void store_i64_kernel(int64_t *out, int64_t v) { *out = v; }
+
+nrk_status sum_i64(const int64_t *x, size_t n, int64_t *out, hipStream_t stream) {
+    NRK_TRY(ensure_device());
+    if (n == 0) {
+        hipLaunchKernelGGL(store_i64_kernel, dim3(1), dim3(1), 0, stream, out, (int64_t)0);
+        NRK_HIP(hipGetLastError());
+        return NRK_OK;
+    }
+    if (!x) return fail(NRK_ERR_INVALID_ARGUMENT, "x is NULL");
+    const size_t nseg = ceil_div(n, I64_SEG);
+    if (nseg >= (size_t)UINT32_MAX) return fail(NRK_ERR_INVALID_ARGUMENT, "input too large");
+    Scratch s;
+    NRK_TRY(get_scratch(stream, 1, nseg * 8, &s));
+    const int vec_ok = (reinterpret_cast<uintptr_t>(x) % 16) == 0;
+    int grid = (int)std::min<size_t>(nseg, (size_t)cu_count() * 8);
+    hipLaunchKernelGGL(sum_i64_kernel, dim3(grid), dim3(RB), 0, stream, x, n, nseg, vec_ok,
+                       (unsigned long long *)s.partials, s.counters, out);
+    NRK_HIP(hipGetLastError());
+    return NRK_OK;
+}
+
+// ---------------- product: fixed-tree product in T ----------------
+template <typename T>
+__global__ __launch_bounds__(RB) void prod_kernel(const T *x, size_t n, size_t nseg, T *partials, unsigned int *counter,
+                                                   T *out) {
+    constexpr int PER = 16;
+    __shared__ T sred[WAVES];
+    __shared__ int slast;
+    const int t = threadIdx.x;
+    for (size_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
+        const size_t off = seg * (size_t)(RB * PER);
+        T acc = (T)1;
+        for (int j = 0; j < PER; ++j) {
+            size_t pos = off + (size_t)j * RB + t;
+            if (pos < n) acc = rn_mul(acc, x[pos]);
+        }
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) acc = rn_mul(acc, shfl_xor(acc, m));
+        if ((t & 63) == 0) sred[t >> 6] = acc;
+        __syncthreads();
+        if (t == 0) {
+            T s = sred[0];
+            for (int k = 1; k < WAVES; ++k) s = rn_mul(s, sred[k]);
+            partials[seg] = s;
+            __threadfence();
+            unsigned old = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            int last = old == (unsigned)(nseg - 1);
+            if (last) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            slast = last;
+        }
+        __syncthreads();
+        const int last = slast;
+        __syncthreads();
+        if (last) {
+            __threadfence();
+            T s = (T)1;
+            for (size_t k = t; k < nseg; k += RB) s = rn_mul(s, partials[k]);
+#pragma unroll
+            for (int m = 32; m >= 1; m >>= 1) s = rn_mul(s, shfl_xor(s, m));
+            if ((t & 63) == 0) sred[t >> 6] = s;
+            __syncthreads();
+            if (t == 0) {
+                T tot = sred[0];
+                for (int k = 1; k < WAVES; ++k) tot = rn_mul(tot, sred[k]);
+                *out = tot;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+template <typename T>
+__global__ void store_kernel(T *out, T v) { *out = v; }
+
+template <typename T>
+nrk_status prod(const T *x, size_t n, T *out, hipStream_t stream) {
+    NRK_TRY(ensure_device());
+    if (n == 0) {
+        hipLaunchKernelGGL(store_kernel<T>, dim3(1), dim3(1), 0, stream, out, (T)1);
+        NRK_HIP(hipGetLastError());
+        return NRK_OK;
+    }
+    if (!x) return fail(NRK_ERR_INVALID_ARGUMENT, "x is NULL");
+    const size_t nseg = ceil_div(n, (size_t)RB * 16);
+    Scratch s;
+    NRK_TRY(get_scratch(stream, 1, nseg * sizeof(T), &s));
+    int grid = (int)std::min<size_t>(nseg, (size_t)cu_count() * 8);
+    hipLaunchKernelGGL(prod_kernel<T>, dim3(grid), dim3(RB), 0, stream, x, n, nseg, (T *)s.partials, s.counters, out);
+    NRK_HIP(hipGetLastError());
+    return NRK_OK;
+}
+template nrk_status prod<float>(const float *, size_t, float *, hipStream_t);
+template nrk_status prod<double>(const double *, size_t, double *, hipStream_t);
+
+}  // namespace nrk

@@ -1,0 +1,80 @@
+// Reductions along a strided axis: the (outer, len, inner) view with inner > 1, e.g. axis 0 of
+// a 2-D array. One thread owns one output and walks the axis left to right, so the fold order is
+// exactly the reference's (math/aggregation.rs:548-561 sum; :244-261 max/min with the NaN
+// short-circuit; math/statistics/mod.rs:1064-1071 mean = left-to-right sum / len;
+// :1403-1494 var = mean(axis), then left-to-right sum of (x - mean)^2, / (len - ddof);
+// :164-175 argmax, first strict maximum). Adjacent threads own adjacent outputs, so every step of
+// the walk is one coalesced row read across the wavefront.
+#include "ops.hpp"
+
+namespace nrk {
+
+template <typename T>
+__global__ __launch_bounds__(256) void strided_reduce_kernel(int op, const T *x, size_t outer, size_t len, size_t inner,
+                                                             T divisor, void *out) {
+    const size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (id >= outer * inner) return;
+    const size_t o = id / inner, i = id % inner;
+    const T *p = x + o * len * inner + i;
+    switch (op) {
+        case NRK_AXIS_SUM:
+        case NRK_AXIS_MEAN:
+        case NRK_AXIS_VAR: {
+            T s = (T)0;
+            for (size_t j = 0; j < len; ++j) s = rn_add(s, p[j * inner]);
+            if (op == NRK_AXIS_SUM) { ((T *)out)[id] = s; break; }
+            const T mean = rn_div(s, (T)len);
+            if (op == NRK_AXIS_MEAN) { ((T *)out)[id] = mean; break; }
+            T q = (T)0;
+            for (size_t j = 0; j < len; ++j) {
+                const T d = rn_sub(p[j * inner], mean);
+                q = rn_add(q, rn_mul(d, d));
+            }
+            ((T *)out)[id] = rn_div(q, divisor);
+            break;
+        }
+        case NRK_AXIS_MAX:
+        case NRK_AXIS_MIN: {
+            T acc = p[0];
+            bool nan = __builtin_isnan(acc);
+            for (size_t j = 1; j < len && !nan; ++j) {
+                const T v = p[j * inner];
+                if (__builtin_isnan(v)) { acc = v; nan = true; break; }
+                if (op == NRK_AXIS_MAX ? (v > acc) : (v < acc)) acc = v;
+            }
+            ((T *)out)[id] = acc;
+            break;
+        }
+        case NRK_AXIS_ARGMAX:
+        case NRK_AXIS_ARGMIN: {
+            T best = p[0];
+            uint64_t bi = 0;
+            for (size_t j = 1; j < len; ++j) {
+                const T v = p[j * inner];
+                if (op == NRK_AXIS_ARGMAX ? (v > best) : (v < best)) { best = v; bi = j; }
+            }
+            ((uint64_t *)out)[id] = bi;
+            break;
+        }
+        default: break;
+    }
+}
+
+template <typename T>
+nrk_status strided_reduce(int op, const T *x, size_t outer, size_t len, size_t inner, double ddof, void *out,
+                          hipStream_t stream) {
+    NRK_TRY(ensure_device());
+    const size_t nout = outer * inner;
+    if (nout == 0) return NRK_OK;
+    if (!x || !out) return fail(NRK_ERR_INVALID_ARGUMENT, "x or out is NULL");
+    const T divisor = (T)((double)len - ddof);
+    hipLaunchKernelGGL(strided_reduce_kernel<T>, dim3((unsigned)ceil_div(nout, 256)), dim3(256), 0, stream, op, x, outer,
+                       len, inner, divisor, out);
+    NRK_HIP(hipGetLastError());
+    return NRK_OK;
+}
+
+template nrk_status strided_reduce<float>(int, const float *, size_t, size_t, size_t, double, void *, hipStream_t);
+template nrk_status strided_reduce<double>(int, const double *, size_t, size_t, size_t, double, void *, hipStream_t);
+
+}  // namespace nrk

@@ -1,0 +1,295 @@
+// Runtime part of the C ABI: device selection, errors, memory, streams, events, and the
+// per-stream scratch that the reduction kernels use for their cross-workgroup finish.
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+
+#include "common.hpp"
+
+namespace nrk {
+
+static thread_local char g_last_error[512] = "";
+static thread_local int g_thread_device = -1;
+
+void set_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
+    va_end(ap);
+}
+
+nrk_status fail(nrk_status code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+nrk_status hip_fail(hipError_t e, const char *what, const char *file, int line) {
+    (void)hipGetLastError();  // clear the sticky non-fatal error state
+    set_error("%s failed: %s (%d) at %s:%d", what, hipGetErrorString(e), (int)e, file, line);
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return NRK_ERR_OUT_OF_MEMORY;
+    if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return NRK_ERR_NO_DEVICE;
+    return NRK_ERR_HIP;
+}
+
+struct DeviceInfo {
+    bool checked = false;
+    bool ok = false;
+    int cus = 0;
+};
+static std::mutex g_dev_mutex;
+static std::map<int, DeviceInfo> g_devices;
+
+static nrk_status check_device(int device) {
+    std::lock_guard<std::mutex> lock(g_dev_mutex);
+    DeviceInfo &info = g_devices[device];
+    if (info.checked) {
+        return info.ok ? NRK_OK : fail(NRK_ERR_NO_DEVICE, "device %d is not a gfx950 device", device);
+    }
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count <= 0) return fail(NRK_ERR_NO_DEVICE, "no HIP device visible");
+    if (device < 0 || device >= count) return fail(NRK_ERR_NO_DEVICE, "device %d out of range [0,%d)", device, count);
+    hipDeviceProp_t prop;
+    NRK_HIP(hipGetDeviceProperties(&prop, device));
+    info.checked = true;
+    info.ok = std::strncmp(prop.gcnArchName, "gfx950", 6) == 0;
+    info.cus = prop.multiProcessorCount;
+    if (!info.ok) return fail(NRK_ERR_NO_DEVICE, "device %d is %s, this library is built for gfx950", device, prop.gcnArchName);
+    return NRK_OK;
+}
+
+nrk_status ensure_device() {
+    if (g_thread_device >= 0) return NRK_OK;
+    return nrk_init(0);
+}
+
+int cu_count() {
+    int dev = g_thread_device < 0 ? 0 : g_thread_device;
+    std::lock_guard<std::mutex> lock(g_dev_mutex);
+    auto it = g_devices.find(dev);
+    return (it != g_devices.end() && it->second.cus > 0) ? it->second.cus : 256;
+}
+
+// ---- per-(device, stream) scratch ----
+struct StreamScratch {
+    unsigned int *counters = nullptr;
+    size_t n_counters = 0;
+    void *partials = nullptr;
+    size_t partial_bytes = 0;
+};
+static std::mutex g_scratch_mutex;
+static std::map<std::pair<int, hipStream_t>, StreamScratch> g_scratch;
+
+nrk_status get_scratch(hipStream_t stream, size_t n_counters, size_t partial_bytes, Scratch *out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    StreamScratch &s = g_scratch[{g_thread_device, stream}];
+    if (n_counters > s.n_counters) {
+        size_t want = align_up(n_counters < 4096 ? 4096 : n_counters, 4096);
+        if (s.counters) {
+            NRK_HIP(hipStreamSynchronize(stream));
+            NRK_HIP(hipFree(s.counters));
+            s.counters = nullptr;
+            s.n_counters = 0;
+        }
+        NRK_HIP(hipMalloc(&s.counters, want * sizeof(unsigned int)));
+        // Establishes the zero invariant once; kernels keep it.
+        NRK_HIP(hipMemset(s.counters, 0, want * sizeof(unsigned int)));
+        s.n_counters = want;
+    }
+    if (partial_bytes > s.partial_bytes) {
+        size_t want = align_up(partial_bytes < (1u << 20) ? (1u << 20) : partial_bytes, 1u << 20);
+        if (s.partials) {
+            NRK_HIP(hipStreamSynchronize(stream));
+            NRK_HIP(hipFree(s.partials));
+            s.partials = nullptr;
+            s.partial_bytes = 0;
+        }
+        NRK_HIP(hipMalloc(&s.partials, want));
+        s.partial_bytes = want;
+    }
+    out->counters = s.counters;
+    out->n_counters = s.n_counters;
+    out->partials = s.partials;
+    out->partial_bytes = s.partial_bytes;
+    return NRK_OK;
+}
+
+// ---- host staging buffers (per thread, per slot) ----
+static thread_local Staging g_staging[4];
+
+nrk_status staging(int slot, size_t bytes, void **ptr) {
+    Staging &s = g_staging[slot];
+    if (bytes > s.bytes) {
+        if (s.ptr) {
+            NRK_HIP(hipDeviceSynchronize());
+            NRK_HIP(hipFree(s.ptr));
+            s.ptr = nullptr;
+            s.bytes = 0;
+        }
+        size_t want = align_up(bytes < 4096 ? 4096 : bytes, 4096);
+        NRK_HIP(hipMalloc(&s.ptr, want));
+        s.bytes = want;
+    }
+    *ptr = s.ptr;
+    return NRK_OK;
+}
+
+nrk_status make_chain(const nrk_scalar_op *chain, int32_t chain_len, Chain *out) {
+    if (chain_len < 0 || chain_len > NRK_MAX_CHAIN) return fail(NRK_ERR_INVALID_ARGUMENT, "chain_len %d outside [0,%d]", chain_len, NRK_MAX_CHAIN);
+    if (chain_len > 0 && chain == nullptr) return fail(NRK_ERR_INVALID_ARGUMENT, "chain is NULL");
+    out->len = chain_len;
+    for (int i = 0; i < NRK_MAX_CHAIN; ++i) {
+        out->op[i] = -1;
+        out->s[i] = 0.0;
+    }
+    for (int i = 0; i < chain_len; ++i) {
+        if (chain[i].op < 0 || chain[i].op >= NRK_SOP_COUNT) return fail(NRK_ERR_INVALID_ARGUMENT, "chain[%d].op = %d is not an NRK_SOP_* code", i, chain[i].op);
+        out->op[i] = chain[i].op;
+        out->s[i] = chain[i].scalar;
+    }
+    return NRK_OK;
+}
+
+}  // namespace nrk
+
+using namespace nrk;
+
+extern "C" {
+
+nrk_status nrk_init(int device) {
+    NRK_TRY(check_device(device));
+    NRK_HIP(hipSetDevice(device));
+    g_thread_device = device;
+    return NRK_OK;
+}
+
+const char *nrk_status_string(nrk_status status) {
+    switch (status) {
+        case NRK_OK: return "ok";
+        case NRK_ERR_INVALID_ARGUMENT: return "invalid argument";
+        case NRK_ERR_NO_DEVICE: return "no usable gfx950 device";
+        case NRK_ERR_OUT_OF_MEMORY: return "out of device memory";
+        case NRK_ERR_HIP: return "HIP runtime error";
+        case NRK_ERR_UNSUPPORTED: return "unsupported";
+        default: return "unknown status";
+    }
+}
+
+const char *nrk_last_error(void) { return g_last_error; }
+const char *nrk_version(void) { return "0.1.0"; }
+
+nrk_status nrk_device_malloc(void **ptr, size_t bytes) {
+    if (!ptr) return fail(NRK_ERR_INVALID_ARGUMENT, "ptr is NULL");
+    NRK_TRY(ensure_device());
+    *ptr = nullptr;
+    if (bytes == 0) bytes = 16;
+    NRK_HIP(hipMalloc(ptr, bytes));
+    return NRK_OK;
+}
+
+nrk_status nrk_device_free(void *ptr) {
+    if (!ptr) return NRK_OK;
+    NRK_HIP(hipFree(ptr));
+    return NRK_OK;
+}
+
+nrk_status nrk_memcpy_htod(void *dst, const void *src, size_t bytes, nrk_stream stream) {
+    if (bytes == 0) return NRK_OK;
+    NRK_TRY(ensure_device());
+    NRK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+    NRK_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return NRK_OK;
+}
+
+nrk_status nrk_memcpy_dtoh(void *dst, const void *src, size_t bytes, nrk_stream stream) {
+    if (bytes == 0) return NRK_OK;
+    NRK_TRY(ensure_device());
+    NRK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
+    NRK_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return NRK_OK;
+}
+
+nrk_status nrk_memcpy_dtod(void *dst, const void *src, size_t bytes, nrk_stream stream) {
+    if (bytes == 0) return NRK_OK;
+    NRK_TRY(ensure_device());
+    NRK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    return NRK_OK;
+}
+
+nrk_status nrk_memset_zero(void *dst, size_t bytes, nrk_stream stream) {
+    if (bytes == 0) return NRK_OK;
+    NRK_TRY(ensure_device());
+    NRK_HIP(hipMemsetAsync(dst, 0, bytes, (hipStream_t)stream));
+    return NRK_OK;
+}
+
+nrk_status nrk_stream_create(nrk_stream *stream) {
+    if (!stream) return fail(NRK_ERR_INVALID_ARGUMENT, "stream is NULL");
+    NRK_TRY(ensure_device());
+    hipStream_t s;
+    NRK_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = (nrk_stream)s;
+    return NRK_OK;
+}
+
+nrk_status nrk_stream_destroy(nrk_stream stream) {
+    if (!stream) return NRK_OK;
+    NRK_HIP(hipStreamSynchronize((hipStream_t)stream));
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mutex);
+        auto it = g_scratch.find({g_thread_device, (hipStream_t)stream});
+        if (it != g_scratch.end()) {
+            if (it->second.counters) (void)hipFree(it->second.counters);
+            if (it->second.partials) (void)hipFree(it->second.partials);
+            g_scratch.erase(it);
+        }
+    }
+    NRK_HIP(hipStreamDestroy((hipStream_t)stream));
+    return NRK_OK;
+}
+
+nrk_status nrk_stream_synchronize(nrk_stream stream) {
+    NRK_TRY(ensure_device());
+    NRK_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return NRK_OK;
+}
+
+nrk_status nrk_device_synchronize(void) {
+    NRK_TRY(ensure_device());
+    NRK_HIP(hipDeviceSynchronize());
+    return NRK_OK;
+}
+
+nrk_status nrk_event_create(void **event) {
+    if (!event) return fail(NRK_ERR_INVALID_ARGUMENT, "event is NULL");
+    NRK_TRY(ensure_device());
+    hipEvent_t e;
+    NRK_HIP(hipEventCreate(&e));
+    *event = (void *)e;
+    return NRK_OK;
+}
+
+nrk_status nrk_event_destroy(void *event) {
+    if (!event) return NRK_OK;
+    NRK_HIP(hipEventDestroy((hipEvent_t)event));
+    return NRK_OK;
+}
+
+nrk_status nrk_event_record(void *event, nrk_stream stream) {
+    NRK_HIP(hipEventRecord((hipEvent_t)event, (hipStream_t)stream));
+    return NRK_OK;
+}
+
+nrk_status nrk_event_elapsed_ms(float *ms, void *start, void *stop) {
+    if (!ms) return fail(NRK_ERR_INVALID_ARGUMENT, "ms is NULL");
+    NRK_HIP(hipEventSynchronize((hipEvent_t)stop));
+    NRK_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+    return NRK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,156 @@
+"""Host-side mirror of the reference's `src/kernels` interface over the C ABI.
+
+Names, argument meaning and error behaviour follow the reference's crate-private kernels module
+(paths under the reference's src/), so that parity tests read like the reference's own tests:
+
+    reduce.rs:207-421     sum_f64 / mean_f64 / sum_sq_dev_f64 / var_f64 / min_f64 / max_f64 /
+                          prod_f64 and their _f32 twins -> a Python float
+    gemm.rs:246-268       gemm_2d(m, k, n, a, b, c): overwrites c in place
+    elementwise.rs:51-258 binary_f64 / binary_f32 / unary_dispatch with op codes instead of closures
+    operations.rs:303-318 sum_i64 (Array::<i64>::sum)
+
+Arrays are numpy arrays standing in for Rust slices (host memory, logical order). Every call
+goes to the GPU through libnumrs_amd.so; there is no CPU path here.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import ScalarOp, call
+
+# NRK_SOP_* / NRK_BOP_* / NRK_AXIS_* codes of include/numrs_amd.h
+SOP = {"add": 0, "sub": 1, "mul": 2, "div": 3, "rsub": 4, "rdiv": 5, "neg": 6, "abs": 7, "square": 8, "sqrt": 9}
+BOP = {"add": 0, "sub": 1, "mul": 2, "div": 3, "rem": 4, "maximum": 5, "minimum": 6}
+AXIS = {"sum": 0, "mean": 1, "var": 2, "min": 3, "max": 4, "argmin": 5, "argmax": 6}
+
+PARALLEL_MIN_LEN = 10_000  # kernels/mod.rs:74 (kept for callers; the GPU has no CPU tiers)
+PARALLEL_CHUNK = 8_192  # kernels/mod.rs:92
+
+
+def chain(*steps) -> tuple[C.Array, int]:
+    """Build an nrk_scalar_op chain from ("mul", 1.5), ("add", -0.25), ("neg",) ... steps."""
+    arr = (ScalarOp * max(1, len(steps)))()
+    for i, st in enumerate(steps):
+        name = st[0]
+        arr[i].op = SOP[name]
+        arr[i].reserved = 0
+        arr[i].scalar = float(st[1]) if len(st) > 1 else 0.0
+    return arr, len(steps)
+
+
+def _vec(a, dtype) -> np.ndarray:
+    a = np.ascontiguousarray(a)
+    if a.dtype != np.dtype(dtype):
+        raise TypeError(f"expected {np.dtype(dtype)}, got {a.dtype}")
+    return a.reshape(-1)
+
+
+def _ptr(a: np.ndarray) -> C.c_void_p:
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _scalar_reduce(fn: str, a, dtype, out_ctype, *extra):
+    a = _vec(a, dtype)
+    out = out_ctype()
+    call(fn, _ptr(a), C.c_size_t(a.size), *extra, C.byref(out))
+    return out.value
+
+
+# ---- kernels/reduce.rs ----
+def sum_f64(a) -> float: return _scalar_reduce("nrk_sum_f64", a, np.float64, C.c_double)
+def sum_f32(a) -> float: return _scalar_reduce("nrk_sum_f32", a, np.float32, C.c_float)
+def mean_f64(a) -> float: return _scalar_reduce("nrk_mean_f64", a, np.float64, C.c_double)
+def mean_f32(a) -> float: return _scalar_reduce("nrk_mean_f32", a, np.float32, C.c_float)
+def min_f64(a) -> float: return _scalar_reduce("nrk_min_f64", a, np.float64, C.c_double)
+def min_f32(a) -> float: return _scalar_reduce("nrk_min_f32", a, np.float32, C.c_float)
+def max_f64(a) -> float: return _scalar_reduce("nrk_max_f64", a, np.float64, C.c_double)
+def max_f32(a) -> float: return _scalar_reduce("nrk_max_f32", a, np.float32, C.c_float)
+def prod_f64(a) -> float: return _scalar_reduce("nrk_prod_f64", a, np.float64, C.c_double)
+def prod_f32(a) -> float: return _scalar_reduce("nrk_prod_f32", a, np.float32, C.c_float)
+
+
+def sum_sq_dev_f64(a, mean: float) -> float:
+    return _scalar_reduce("nrk_sum_sq_dev_f64", a, np.float64, C.c_double, C.c_double(mean))
+
+
+def sum_sq_dev_f32(a, mean: float) -> float:
+    return _scalar_reduce("nrk_sum_sq_dev_f32", a, np.float32, C.c_float, C.c_float(mean))
+
+
+def var_f64(a, ddof: int) -> float:
+    return _scalar_reduce("nrk_var_f64", a, np.float64, C.c_double, C.c_size_t(ddof))
+
+
+def var_f32(a, ddof: int) -> float:
+    return _scalar_reduce("nrk_var_f32", a, np.float32, C.c_float, C.c_size_t(ddof))
+
+
+def sum_i64(a) -> int:
+    """Array::<i64>::sum (operations.rs:303-318): wrapping integer fold."""
+    return _scalar_reduce("nrk_sum_i64", a, np.int64, C.c_int64)
+
+
+def argmax_f64(a) -> int: return _scalar_reduce("nrk_argmax_f64", a, np.float64, C.c_uint64)
+def argmax_f32(a) -> int: return _scalar_reduce("nrk_argmax_f32", a, np.float32, C.c_uint64)
+def argmin_f64(a) -> int: return _scalar_reduce("nrk_argmin_f64", a, np.float64, C.c_uint64)
+def argmin_f32(a) -> int: return _scalar_reduce("nrk_argmin_f32", a, np.float32, C.c_uint64)
+
+
+def reduce_axis(op: str, x: np.ndarray, outer: int, length: int, inner: int, ddof: int = 0) -> np.ndarray:
+    """Reduce the (outer, length, inner) view of x along `length`; returns outer*inner values."""
+    x = np.ascontiguousarray(x)
+    if x.dtype not in (np.float64, np.float32):
+        raise TypeError("reduce_axis takes f64 or f32")
+    out_dtype = np.uint64 if op in ("argmax", "argmin") else x.dtype
+    out = np.empty(outer * inner, out_dtype)
+    fn = "nrk_reduce_axis_f64" if x.dtype == np.float64 else "nrk_reduce_axis_f32"
+    call(fn, C.c_int32(AXIS[op]), _ptr(x), C.c_size_t(outer), C.c_size_t(length), C.c_size_t(inner),
+         C.c_size_t(ddof), _ptr(out))
+    return out
+
+
+# ---- kernels/elementwise.rs ----
+def unary_dispatch(a, steps) -> np.ndarray:
+    """unary_dispatch (elementwise.rs:246-258) with an op-coded closure: chain(("mul", 1.5), ...)."""
+    a = np.ascontiguousarray(a)
+    ops, n_ops = chain(*steps)
+    out = np.empty_like(a)
+    fn = {np.dtype(np.float64): "nrk_map_f64", np.dtype(np.float32): "nrk_map_f32"}[a.dtype]
+    call(fn, C.cast(ops, C.c_void_p), C.c_int32(n_ops), _ptr(a), _ptr(out), C.c_size_t(a.size))
+    return out
+
+
+def binary_f64(a, b, op: str) -> np.ndarray:
+    """binary_f64 (elementwise.rs:70-89) with an op-coded closure."""
+    return _binary("nrk_zip_f64", a, b, op, np.float64)
+
+
+def binary_f32(a, b, op: str) -> np.ndarray:
+    """binary_f32 (elementwise.rs:90-109) with an op-coded closure."""
+    return _binary("nrk_zip_f32", a, b, op, np.float32)
+
+
+def _binary(fn, a, b, op, dtype):
+    a = _vec(a, dtype)
+    b = _vec(b, dtype)
+    if a.size != b.size:
+        raise ValueError("binary kernels require equal-length slices")
+    out = np.empty_like(a)
+    call(fn, C.c_int32(BOP[op]), _ptr(a), _ptr(b), _ptr(out), C.c_size_t(a.size))
+    return out
+
+
+# ---- kernels/gemm.rs ----
+def gemm_2d(m: int, k: int, n: int, a, b, c: np.ndarray) -> None:
+    """gemm_2d (gemm.rs:246-268): C := A*B into the caller's c (overwrite). Slice lengths that
+    do not match m*k, k*n, m*n zero c and read nothing, like gemm_generic's defensive branch."""
+    if not isinstance(c, np.ndarray) or not c.flags.c_contiguous:
+        raise TypeError("c must be a C-contiguous numpy array")
+    dtype = c.dtype
+    a = np.ascontiguousarray(a, dtype=dtype).reshape(-1)
+    b = np.ascontiguousarray(b, dtype=dtype).reshape(-1)
+    fn = {np.dtype(np.float64): "nrk_gemm_2d_f64", np.dtype(np.float32): "nrk_gemm_2d_f32"}[dtype]
+    call(fn, C.c_size_t(m), C.c_size_t(k), C.c_size_t(n), _ptr(a), C.c_size_t(a.size), _ptr(b), C.c_size_t(b.size),
+         _ptr(c), C.c_size_t(c.size))

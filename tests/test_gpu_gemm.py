@@ -1,0 +1,88 @@
+"""Plain GEMM (f64 / f32 on MFMA) vs the oracle, through the C ABI.
+
+Tolerance (tests/test_matmul_dispatch.rs:91-107 style, magnitude-relative):
+|C - C_exact| <= tol * max(|C_exact|, 1), tol = 1e-12 (f64) and 1e-4 (f32), with C_exact the
+double-double product. Also: the 3x3 known answer, degenerate dims, k = 0 overwrite of NaN
+garbage, the length-mismatch rule and bit-identity across launches."""
+import numpy as np
+import pytest
+
+from numrs_amd import kernels as K
+from numrs_amd import rng
+
+pytestmark = pytest.mark.gpu
+
+TOL = {np.float64: 1e-12, np.float32: 1e-4}
+SHAPES = [(1, 1, 1), (3, 3, 3), (17, 31, 13), (64, 64, 64), (63, 65, 127), (128, 16, 128), (129, 257, 130),
+          (200, 1, 300), (1, 500, 1), (512, 512, 512)]
+
+
+def gemm(a, b):
+    m, k = a.shape
+    n = b.shape[1]
+    c = np.full((m, n), np.nan, a.dtype)
+    K.gemm_2d(m, k, n, a, b, c)
+    return c
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_gemm_vs_double_double(gpu, oracle, dtype):
+    worst = 0.0
+    for i, (m, k, n) in enumerate(SHAPES):
+        a = rng.uniform(rng.BASE_SEED + 1500 + i, m * k).reshape(m, k).astype(dtype)
+        b = rng.uniform(rng.BASE_SEED + 1600 + i, k * n).reshape(k, n).astype(dtype)
+        c = gemm(a, b).astype(np.float64)
+        ex = oracle.dd_gemm(a.astype(np.float64), b.astype(np.float64))
+        err = np.abs(c - ex) / np.maximum(np.abs(ex), 1.0)
+        worst = max(worst, float(err.max()))
+        assert err.max() <= TOL[dtype], (m, k, n, err.max())
+    print(f"max scaled err {worst:.3e}")
+
+
+def test_known_3x3(gpu, known):
+    k = known["matmul_3x3"]
+    a = np.array(k["a"], np.float64).reshape(3, 3)
+    np.testing.assert_array_equal(gemm(a, a).reshape(-1), np.array(k["c"], np.float64))
+    np.testing.assert_array_equal(gemm(a, np.array(k["v"], np.float64).reshape(3, 1)).reshape(-1), np.array(k["av"], np.float64))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_degenerate_dims(gpu, dtype):
+    c = np.full((4, 5), np.nan, dtype)
+    K.gemm_2d(4, 0, 5, np.zeros(0, dtype), np.zeros(0, dtype), c)  # k == 0 -> zeros (gemm.rs:295-303)
+    assert np.all(c == 0)
+    c = np.full((0, 5), np.nan, dtype)
+    K.gemm_2d(0, 3, 5, np.zeros(0, dtype), np.ones(15, dtype), c)
+    c = np.full((3, 5), 7.0, dtype)
+    K.gemm_2d(3, 2, 5, np.ones(5, dtype), np.ones(10, dtype), c)  # a_len != m*k -> zero C, read nothing
+    assert np.all(c == 0)
+
+
+def test_integer_valued_exact_and_identity(gpu):
+    r = np.random.default_rng(3)
+    a = r.integers(-100, 100, (130, 70)).astype(np.float64)
+    b = r.integers(-100, 100, (70, 90)).astype(np.float64)
+    np.testing.assert_array_equal(gemm(a, b), a @ b)
+    e = np.eye(70)
+    np.testing.assert_array_equal(gemm(a, e), a)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_bit_identical_across_launches(gpu, dtype):
+    a = rng.uniform(7, 300 * 400).reshape(300, 400).astype(dtype)
+    b = rng.uniform(8, 400 * 500).reshape(400, 500).astype(dtype)
+    c1, c2 = gemm(a, b), gemm(a, b)
+    np.testing.assert_array_equal(c1, c2)
+
+
+def test_large_f64_sampled(gpu, oracle):
+    """2048^3 f64 (big-tile path) checked on 512 sampled entries against double-double."""
+    n = 2048
+    a = rng.uniform(rng.BASE_SEED + 1700, n * n).reshape(n, n)
+    b = rng.uniform(rng.BASE_SEED + 1701, n * n).reshape(n, n)
+    c = gemm(a, b)
+    r = np.random.default_rng(1)
+    rows, cols = r.integers(0, n, 512), r.integers(0, n, 512)
+    ex = oracle.dd_gemm_entries(a, b, rows, cols)
+    err = np.abs(c[rows, cols] - ex) / np.maximum(np.abs(ex), 1.0)
+    assert err.max() <= 1e-12
