@@ -247,6 +247,23 @@ nrk_status nrk_gemm_2d_f32(size_t m, size_t k, size_t n, const float *a, size_t 
 nrk_status nrk_dev_gemm_f64(size_t m, size_t k, size_t n, const double *a, const double *b, double *c, nrk_stream stream);
 nrk_status nrk_dev_gemm_f32(size_t m, size_t k, size_t n, const float *a, const float *b, float *c, nrk_stream stream);
 
+/* Split-precision GEMMs. No reference counterpart: alternative engines for gemm_2d's f32 / f64
+ * tier (kernels/gemm.rs:290-312) with the same C := A*B overwrite contract.
+ *
+ * FP32 as BF16 split products on v_mfma_f32_16x16x32_bf16 with FP32 accumulation:
+ * products = 3 (BF16x3: a1b0 + a0b1 + a0b0), 6 (BF16x6, adds a2b0 + a1b1 + a0b2) or 1 (inputs
+ * rounded to bf16). Inputs must be finite and below 2^127 in magnitude. */
+nrk_status nrk_gemm_f32_bf16x(size_t m, size_t k, size_t n, const float *a, const float *b, float *c, int32_t products);
+nrk_status nrk_dev_gemm_f32_bf16x(size_t m, size_t k, size_t n, const float *a, const float *b, float *c, int32_t products,
+                                  nrk_stream stream);
+/* FP64 by the Ozaki scheme II on v_mfma_i32_16x16x64_i8: `moduli` int8 residue products
+ * (2..20; 0 selects 16) with exact INT32 accumulation and an exact CRT recombination. The only
+ * rounding is of A's rows and B's columns to P-bit integers (P <= 53, the largest value with
+ * k * 2^(2P) < prod(moduli) / 2). k <= 130000. A row/column holding Inf/NaN gives NaN outputs. */
+nrk_status nrk_gemm_f64_ozaki(size_t m, size_t k, size_t n, const double *a, const double *b, double *c, int32_t moduli);
+nrk_status nrk_dev_gemm_f64_ozaki(size_t m, size_t k, size_t n, const double *a, const double *b, double *c, int32_t moduli,
+                                  nrk_stream stream);
+
 #ifdef __cplusplus
 }
 #endif

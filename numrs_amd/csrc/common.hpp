@@ -44,6 +44,10 @@ struct Scratch {
 };
 nrk_status get_scratch(hipStream_t stream, size_t n_counters, size_t partial_bytes, Scratch *out);
 
+// Per-stream workspace for multi-pass algorithms (split operands, residues). Contents undefined
+// at entry; grows on demand (growing synchronises the stream first).
+nrk_status get_workspace(hipStream_t stream, size_t bytes, void **out);
+
 // Host-pointer staging: device copies of host operands, cached per thread and grown on demand.
 struct Staging {
     void *ptr = nullptr;

@@ -1,0 +1,288 @@
+// FP32 GEMM as BF16 split products on gfx950 matrix cores (v_mfma_f32_16x16x32_bf16), and the
+// plain bf16 -> fp32 GEMM tile that the ragged batch reuses.
+//
+// Split (Henry, Tang & Heinecke 2019, "Leveraging the bfloat16 Artificial Intelligence Datatype
+// for Higher-Precision Computations"): x = x0 + x1 + x2 with x0 = bf16(x), x1 = bf16(x - x0),
+// x2 = bf16(x - x0 - x1); each difference is exact in f32. The product keeps the pairs
+//   BF16x3: A1*B0 + A0*B1 + A0*B0              (3 MFMA products, drops terms of order 2^-16)
+//   BF16x6: A2*B0 + A1*B1 + A0*B2 + A1*B0 + A0*B1 + A0*B0   (drops terms of order 2^-24)
+// accumulated smallest-first into one FP32 accumulator per output.
+//
+// Data path: a split pass writes the terms as bf16 matrices zero-padded to the tile grid, A as
+// (Mp, Kp) and B TRANSPOSED as (Np, Kp) (K contiguous for both, so every MFMA operand is one
+// 16-byte LDS read), then one GEMM kernel streams all terms: global -> registers -> LDS (two
+// buffers), XCD-aware grouped tile order. Padding removes every bounds check from the main loop;
+// only the C store is predicated.
+#include "gemm_lowp.hpp"
+#include "ops.hpp"
+
+namespace nrk {
+
+constexpr int SPLIT_TILE = 32;
+
+// out[t][r][c] (or out[t][c][r] when TRANSPOSE) = term t of X[r][c]; zero in the padding.
+template <int T, bool TRANSPOSE>
+__global__ __launch_bounds__(256) void split_bf16_kernel(const float *__restrict__ X, int R, int Cc,
+                                                        unsigned short *__restrict__ out, int out_rows, int out_cols,
+                                                        size_t term_stride) {
+    __shared__ unsigned short tile[T][SPLIT_TILE][SPLIT_TILE + 2];
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    const int r0 = blockIdx.y * SPLIT_TILE, c0 = blockIdx.x * SPLIT_TILE;
+    for (int dy = ty; dy < SPLIT_TILE; dy += 8) {
+        const int r = r0 + dy, c = c0 + tx;
+        const float x = (r < R && c < Cc) ? X[(size_t)r * Cc + c] : 0.0f;
+        unsigned short h[T];
+        float rem = x;
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            h[t] = f32_to_bf16_rn(rem);
+            rem = rem - bf16_to_f32(h[t]);  // exact
+        }
+        if (TRANSPOSE) {
+#pragma unroll
+            for (int t = 0; t < T; ++t) tile[t][tx][dy] = h[t];
+        } else {
+#pragma unroll
+            for (int t = 0; t < T; ++t) tile[t][dy][tx] = h[t];
+        }
+    }
+    __syncthreads();
+    // TRANSPOSE: output row = input column
+    const int o_r0 = TRANSPOSE ? c0 : r0, o_c0 = TRANSPOSE ? r0 : c0;
+    for (int dy = ty; dy < SPLIT_TILE; dy += 8) {
+        const int orow = o_r0 + dy, ocol = o_c0 + tx;
+        if (orow < out_rows && ocol < out_cols) {
+#pragma unroll
+            for (int t = 0; t < T; ++t) out[t * term_stride + (size_t)orow * out_cols + ocol] = tile[t][dy][tx];
+        }
+    }
+}
+
+// Pair p of the product list (smallest terms first): A term pair_a(p) times B term pair_b(p).
+template <int NPAIR>
+__device__ __forceinline__ constexpr int pair_a(int p) {
+    return NPAIR == 1 ? 0 : NPAIR == 3 ? (p == 0 ? 1 : 0) : (p == 0 ? 2 : (p == 1 || p == 3) ? 1 : 0);
+}
+template <int NPAIR>
+__device__ __forceinline__ constexpr int pair_b(int p) {
+    return NPAIR == 1 ? 0 : NPAIR == 3 ? (p == 1 ? 1 : 0) : (p == 2 ? 2 : (p == 1 || p == 4) ? 1 : 0);
+}
+
+// C(M,N) f32 = sum over pairs p of A[a_p] * Bt[b_p]^T. A terms: (Mp, Kp) each, Bt terms: (Np, Kp).
+template <int BM, int BN, int BK, int WM, int WN, int NTA, int NTB, int NPAIR>
+__global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_bf16_nt_kernel(
+    int M, int N, int Kp, const unsigned short *__restrict__ A, size_t a_term_stride,
+    const unsigned short *__restrict__ Bt, size_t b_term_stride, float *__restrict__ C, int ldc, int group_m) {
+    constexpr int NT = (BM / WM) * (BN / WN) * 64;
+    constexpr int WGN = BN / WN;
+    constexpr int TM = WM / 16, TN = WN / 16;
+    constexpr int LDK = BK + 8;           // padded row (bf16 elements) against bank conflicts
+    constexpr int CH = BK / 8;            // 16-byte chunks per row
+    constexpr int LA = BM * CH / NT;      // chunks per thread per A term
+    constexpr int LB = BN * CH / NT;
+    static_assert((BM * CH) % NT == 0 && (BN * CH) % NT == 0, "tile/threads mismatch");
+    static_assert(BK % 32 == 0, "BK multiple of the MFMA K (32)");
+
+    extern __shared__ __attribute__((aligned(16))) unsigned short smem[];
+    unsigned short *As = smem;                                  // [2][NTA][BM][LDK]
+    unsigned short *Bs = smem + 2 * NTA * BM * LDK;             // [2][NTB][BN][LDK]
+
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int wm = wave / WGN, wn = wave % WGN;
+    int tm, tn;
+    lowp_tile_of(blockIdx.x, (M + BM - 1) / BM, (N + BN - 1) / BN, group_m, tm, tn);
+    const int bm0 = tm * BM, bn0 = tn * BN;
+
+    f32x4_t acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[i][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+
+    u32x4_t ra[NTA][LA], rb[NTB][LB];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int t = 0; t < NTA; ++t)
+#pragma unroll
+            for (int s = 0; s < LA; ++s) {
+                const int q = tid + s * NT, row = q / CH, ch = q % CH;
+                ra[t][s] = *reinterpret_cast<const u32x4_t *>(A + t * a_term_stride + (size_t)(bm0 + row) * Kp + k0 + ch * 8);
+            }
+#pragma unroll
+        for (int t = 0; t < NTB; ++t)
+#pragma unroll
+            for (int s = 0; s < LB; ++s) {
+                const int q = tid + s * NT, row = q / CH, ch = q % CH;
+                rb[t][s] = *reinterpret_cast<const u32x4_t *>(Bt + t * b_term_stride + (size_t)(bn0 + row) * Kp + k0 + ch * 8);
+            }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int t = 0; t < NTA; ++t)
+#pragma unroll
+            for (int s = 0; s < LA; ++s) {
+                const int q = tid + s * NT, row = q / CH, ch = q % CH;
+                *reinterpret_cast<u32x4_t *>(As + ((buf * NTA + t) * BM + row) * LDK + ch * 8) = ra[t][s];
+            }
+#pragma unroll
+        for (int t = 0; t < NTB; ++t)
+#pragma unroll
+            for (int s = 0; s < LB; ++s) {
+                const int q = tid + s * NT, row = q / CH, ch = q % CH;
+                *reinterpret_cast<u32x4_t *>(Bs + ((buf * NTB + t) * BN + row) * LDK + ch * 8) = rb[t][s];
+            }
+    };
+
+    const int nk = Kp / BK;
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+        const int cur = kt & 1;
+        if (kt + 1 < nk) gload((kt + 1) * BK);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 32) {
+            bf16x8_t fa[NTA][TM], fb[NTB][TN];
+            const int koff = kk + 8 * (lane >> 4);
+#pragma unroll
+            for (int t = 0; t < NTA; ++t)
+#pragma unroll
+                for (int i = 0; i < TM; ++i)
+                    fa[t][i] = *reinterpret_cast<const bf16x8_t *>(
+                        As + ((cur * NTA + t) * BM + wm * WM + i * 16 + (lane & 15)) * LDK + koff);
+#pragma unroll
+            for (int t = 0; t < NTB; ++t)
+#pragma unroll
+                for (int j = 0; j < TN; ++j)
+                    fb[t][j] = *reinterpret_cast<const bf16x8_t *>(
+                        Bs + ((cur * NTB + t) * BN + wn * WN + j * 16 + (lane & 15)) * LDK + koff);
+#pragma unroll
+            for (int p = 0; p < NPAIR; ++p)
+#pragma unroll
+                for (int i = 0; i < TM; ++i)
+#pragma unroll
+                    for (int j = 0; j < TN; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[pair_a<NPAIR>(p)][i], fb[pair_b<NPAIR>(p)][j],
+                                                                            acc[i][j], 0, 0, 0);
+        }
+        if (kt + 1 < nk) sstore(cur ^ 1);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            const int col = bn0 + wn * WN + j * 16 + (lane & 15);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = bm0 + wm * WM + i * 16 + 4 * (lane >> 4) + r;
+                if (row < M && col < N) C[(size_t)row * ldc + col] = acc[i][j][r];
+            }
+        }
+}
+
+template <int BM, int BN, int BK, int WM, int WN, int NTA, int NTB, int NPAIR>
+static nrk_status launch_bf16_nt(int M, int N, int Kp, const unsigned short *A, size_t a_stride, const unsigned short *Bt,
+                                 size_t b_stride, float *C, int ldc, hipStream_t stream) {
+    constexpr int threads = (BM / WM) * (BN / WN) * 64;
+    constexpr size_t lds = (size_t)2 * (NTA * BM + NTB * BN) * (BK + 8) * sizeof(unsigned short);
+    auto kern = gemm_bf16_nt_kernel<BM, BN, BK, WM, WN, NTA, NTB, NPAIR>;
+    static bool attr_set = false;
+    if (!attr_set) {
+        NRK_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        attr_set = true;
+    }
+    const int grid = ((M + BM - 1) / BM) * ((N + BN - 1) / BN);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), lds, stream, M, N, Kp, A, a_stride, Bt, b_stride, C, ldc, 8);
+    NRK_HIP(hipGetLastError());
+    return NRK_OK;
+}
+
+// Padding granularity of the split matrices: every tile config below divides it.
+constexpr int PAD_MN = 256;
+constexpr int PAD_K = 64;
+
+template <int T>
+static nrk_status split(const float *X, int R, int Cc, bool transpose, unsigned short *out, int out_rows, int out_cols,
+                        hipStream_t stream) {
+    const size_t stride = (size_t)out_rows * out_cols;
+    // grid covers the OUTPUT (padded) extent so that the padding is written with zeros
+    const int in_rows_cover = transpose ? out_cols : out_rows;
+    const int in_cols_cover = transpose ? out_rows : out_cols;
+    dim3 grid((in_cols_cover + SPLIT_TILE - 1) / SPLIT_TILE, (in_rows_cover + SPLIT_TILE - 1) / SPLIT_TILE);
+    if (transpose)
+        hipLaunchKernelGGL((split_bf16_kernel<T, true>), grid, dim3(256), 0, stream, X, R, Cc, out, out_rows, out_cols, stride);
+    else
+        hipLaunchKernelGGL((split_bf16_kernel<T, false>), grid, dim3(256), 0, stream, X, R, Cc, out, out_rows, out_cols, stride);
+    NRK_HIP(hipGetLastError());
+    return NRK_OK;
+}
+
+nrk_status gemm_f32_bf16x(size_t m, size_t k, size_t n, const float *a, const float *b, float *c, int products,
+                          hipStream_t stream) {
+    NRK_TRY(ensure_device());
+    if (products != 3 && products != 6 && products != 1)
+        return fail(NRK_ERR_INVALID_ARGUMENT, "products must be 1, 3 or 6 (got %d)", products);
+    if (m == 0 || n == 0) return NRK_OK;
+    if (k == 0) return gemm<float>(m, 0, n, a, b, c, stream);  // zero fill
+    if (!a || !b || !c) return fail(NRK_ERR_INVALID_ARGUMENT, "A, B or C is NULL");
+    if (m > (1u << 30) || n > (1u << 30) || k > (1u << 30)) return fail(NRK_ERR_INVALID_ARGUMENT, "dimension too large");
+    const int T = products == 6 ? 3 : (products == 3 ? 2 : 1);
+    const size_t Mp = align_up(m, PAD_MN), Np = align_up(n, PAD_MN), Kp = align_up(k, PAD_K);
+    const size_t bytes = (size_t)T * (Mp + Np) * Kp * sizeof(unsigned short);
+    void *ws = nullptr;
+    NRK_TRY(get_workspace(stream, bytes, &ws));
+    unsigned short *As = (unsigned short *)ws;
+    unsigned short *Bts = As + (size_t)T * Mp * Kp;
+    if (T == 3) {
+        NRK_TRY(split<3>(a, (int)m, (int)k, false, As, (int)Mp, (int)Kp, stream));
+        NRK_TRY(split<3>(b, (int)k, (int)n, true, Bts, (int)Np, (int)Kp, stream));
+    } else if (T == 2) {
+        NRK_TRY(split<2>(a, (int)m, (int)k, false, As, (int)Mp, (int)Kp, stream));
+        NRK_TRY(split<2>(b, (int)k, (int)n, true, Bts, (int)Np, (int)Kp, stream));
+    } else {
+        NRK_TRY(split<1>(a, (int)m, (int)k, false, As, (int)Mp, (int)Kp, stream));
+        NRK_TRY(split<1>(b, (int)k, (int)n, true, Bts, (int)Np, (int)Kp, stream));
+    }
+    const size_t sa = Mp * Kp, sb = Np * Kp;
+    const bool big = ceil_div(m, 128) * ceil_div(n, 128) >= (size_t)cu_count();
+    if (products == 6) {
+        if (big) return launch_bf16_nt<128, 128, 32, 64, 64, 3, 3, 6>((int)m, (int)n, (int)Kp, As, sa, Bts, sb, c, (int)n, stream);
+        return launch_bf16_nt<64, 64, 32, 32, 32, 3, 3, 6>((int)m, (int)n, (int)Kp, As, sa, Bts, sb, c, (int)n, stream);
+    }
+    if (products == 3) {
+        if (big) return launch_bf16_nt<128, 128, 64, 64, 64, 2, 2, 3>((int)m, (int)n, (int)Kp, As, sa, Bts, sb, c, (int)n, stream);
+        return launch_bf16_nt<64, 64, 64, 32, 32, 2, 2, 3>((int)m, (int)n, (int)Kp, As, sa, Bts, sb, c, (int)n, stream);
+    }
+    if (big) return launch_bf16_nt<128, 128, 64, 64, 64, 1, 1, 1>((int)m, (int)n, (int)Kp, As, sa, Bts, sb, c, (int)n, stream);
+    return launch_bf16_nt<64, 64, 64, 32, 32, 1, 1, 1>((int)m, (int)n, (int)Kp, As, sa, Bts, sb, c, (int)n, stream);
+}
+
+}  // namespace nrk
+
+using namespace nrk;
+
+extern "C" {
+
+nrk_status nrk_dev_gemm_f32_bf16x(size_t m, size_t k, size_t n, const float *a, const float *b, float *c, int32_t products,
+                                  nrk_stream stream) {
+    return gemm_f32_bf16x(m, k, n, a, b, c, products, (hipStream_t)stream);
+}
+
+nrk_status nrk_gemm_f32_bf16x(size_t m, size_t k, size_t n, const float *a, const float *b, float *c, int32_t products) {
+    NRK_TRY(ensure_device());
+    if (m == 0 || n == 0) return NRK_OK;
+    void *da = nullptr, *db = nullptr, *dc = nullptr;
+    NRK_TRY(staging(0, m * k * 4 + 16, &da));
+    NRK_TRY(staging(2, k * n * 4 + 16, &db));
+    NRK_TRY(staging(1, m * n * 4 + 16, &dc));
+    hipStream_t s = nullptr;
+    if (m * k) NRK_HIP(hipMemcpyAsync(da, a, m * k * 4, hipMemcpyHostToDevice, s));
+    if (k * n) NRK_HIP(hipMemcpyAsync(db, b, k * n * 4, hipMemcpyHostToDevice, s));
+    NRK_TRY(gemm_f32_bf16x(m, k, n, (const float *)da, (const float *)db, (float *)dc, products, s));
+    NRK_HIP(hipMemcpyAsync(c, dc, m * n * 4, hipMemcpyDeviceToHost, s));
+    NRK_HIP(hipStreamSynchronize(s));
+    return NRK_OK;
+}
+
+}  // extern "C"

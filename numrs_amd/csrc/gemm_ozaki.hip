@@ -1,0 +1,454 @@
+// FP64 GEMM emulated on gfx950 INT8 matrix cores: the Ozaki scheme II (Ozaki, Uchino & Imamura,
+// "Ozaki Scheme II: A GEMM-oriented emulation of floating-point matrix multiplication using an
+// integer modular technique", 2024/25), built on the Ozaki-Ogita-Oishi-Rump error-free splitting
+// idea and the INT8-unit DGEMM of Ootomo, Ozaki & Yokota (2024).
+//
+//   1. Scale rows of A and columns of B by powers of two, 2^s_i and 2^t_j, so that every scaled
+//      entry is below 2^P in magnitude, and round to integers: A' = round(2^s A), B' = round(B 2^t).
+//      (P <= 53, so A' and B' are exact doubles; P is the largest value with k * 2^(2P) < M / 2.)
+//   2. For each of N pairwise-coprime odd moduli m_j <= 255, take symmetric residues
+//      a_j = A' mod m_j, b_j = B' mod m_j in [-127, 127] (int8, exact).
+//   3. C_j = a_j * b_j^T on v_mfma_i32_16x16x64_i8: int32 accumulation is exact
+//      (k * 127^2 < 2^31 for k <= 133 000), then c_j = C_j mod m_j (int8).
+//   4. CRT: C' = sum_j c_j w_j mod M (w_j = (M/m_j) * ((M/m_j)^-1 mod m_j), M = prod m_j) is the
+//      EXACT integer product A'B'. It is formed with 24-bit limbs whose partial sums are exact
+//      in FP64, reduced mod M, normalised, and evaluated in double-double; C = C' 2^(-s_i - t_j).
+// The only error is the rounding of A and B to P-bit integers in step 1 (relative to each row /
+// column maximum); all integer work is exact.
+//
+// Inputs must be finite: a row of A or column of B holding Inf/NaN yields NaN in the
+// corresponding outputs.
+#include <cmath>
+#include <vector>
+
+#include "gemm_lowp.hpp"
+#include "ops.hpp"
+
+namespace nrk {
+
+constexpr int OZ_MAX_MODULI = 20;
+constexpr int OZ_LIMBS = 7;       // 7 x 24-bit limbs = 168 bits > log2(M) for 20 moduli
+constexpr int OZ_DEFAULT_MODULI = 16;
+constexpr int OZ_BAD_SCALE = 100000;  // marks a row/column holding Inf/NaN
+
+// Pairwise-coprime odd moduli <= 255, largest first (greedy).
+static const int kModuli[OZ_MAX_MODULI] = {255, 253, 251, 247, 241, 239, 233, 229, 227, 223,
+                                           217, 211, 199, 197, 193, 191, 181, 179, 173, 167};
+
+struct OzConsts {
+    int n;                                  // number of moduli
+    int m[OZ_MAX_MODULI];
+    double inv_m[OZ_MAX_MODULI];
+    double w[OZ_MAX_MODULI][OZ_LIMBS];      // CRT weights, 24-bit limbs, least significant first
+    double Mlimb[OZ_LIMBS];                 // M in 24-bit limbs
+    double M_approx;                        // M as a double (rounded)
+};
+
+// ---------------- host-side multiprecision for the CRT constants ----------------
+struct Big {  // little-endian base 2^24 limbs, non-negative
+    std::vector<uint64_t> d;
+    explicit Big(uint64_t v = 0) {
+        while (v) { d.push_back(v & 0xffffff); v >>= 24; }
+    }
+    void mul_small(uint64_t k) {
+        uint64_t carry = 0;
+        for (auto &x : d) {
+            uint64_t t = x * k + carry;
+            x = t & 0xffffff;
+            carry = t >> 24;
+        }
+        while (carry) { d.push_back(carry & 0xffffff); carry >>= 24; }
+    }
+    double to_double() const {
+        double r = 0.0;
+        for (size_t i = d.size(); i-- > 0;) r = r * 16777216.0 + (double)d[i];
+        return r;
+    }
+};
+
+static int modinv(int a, int m) {  // a^-1 mod m, gcd(a, m) = 1
+    int t = 0, nt = 1, r = m, nr = ((a % m) + m) % m;
+    while (nr) {
+        int q = r / nr, tmp = t - q * nt;
+        t = nt; nt = tmp;
+        tmp = r - q * nr; r = nr; nr = tmp;
+    }
+    return ((t % m) + m) % m;
+}
+
+static nrk_status make_consts(int n, OzConsts *c) {
+    if (n < 2 || n > OZ_MAX_MODULI) return fail(NRK_ERR_INVALID_ARGUMENT, "moduli count %d outside [2,%d]", n, OZ_MAX_MODULI);
+    *c = OzConsts{};
+    c->n = n;
+    Big M(1);
+    for (int j = 0; j < n; ++j) M.mul_small((uint64_t)kModuli[j]);
+    for (int t = 0; t < OZ_LIMBS; ++t) c->Mlimb[t] = t < (int)M.d.size() ? (double)M.d[t] : 0.0;
+    c->M_approx = M.to_double();
+    for (int j = 0; j < n; ++j) {
+        const int mj = kModuli[j];
+        c->m[j] = mj;
+        c->inv_m[j] = 1.0 / (double)mj;
+        Big Mj(1);
+        int Mj_mod = 1;
+        for (int i = 0; i < n; ++i) {
+            if (i == j) continue;
+            Mj.mul_small((uint64_t)kModuli[i]);
+            Mj_mod = (Mj_mod * (kModuli[i] % mj)) % mj;
+        }
+        const int y = modinv(Mj_mod, mj);
+        Mj.mul_small((uint64_t)y);  // w_j = M_j * y_j  (< M)
+        for (int t = 0; t < OZ_LIMBS; ++t) c->w[j][t] = t < (int)Mj.d.size() ? (double)Mj.d[t] : 0.0;
+    }
+    return NRK_OK;
+}
+
+// ---------------- 1. per-row / per-column power-of-two scales ----------------
+// scale[i] = P - 1 - floor(log2(max_k |X[i][k]|)) so that every scaled entry is < 2^P.
+__global__ __launch_bounds__(256) void row_scale_kernel(const double *__restrict__ X, int R, int Cc, int P,
+                                                        int *__restrict__ scale) {
+    const int r = blockIdx.x;
+    double mx = 0.0;
+    int bad = 0;
+    for (int c = threadIdx.x; c < Cc; c += 256) {
+        const double v = X[(size_t)r * Cc + c];
+        bad |= !__builtin_isfinite(v);
+        mx = fmax(mx, fabs(v));
+    }
+    __shared__ double smx[256];
+    __shared__ int sbad[256];
+    smx[threadIdx.x] = mx;
+    sbad[threadIdx.x] = bad;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            smx[threadIdx.x] = fmax(smx[threadIdx.x], smx[threadIdx.x + s]);
+            sbad[threadIdx.x] |= sbad[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        int e = 0;
+        if (smx[0] > 0.0) (void)frexp(smx[0], &e);  // smx = f * 2^e, f in [0.5, 1)
+        scale[r] = sbad[0] ? OZ_BAD_SCALE : (smx[0] > 0.0 ? P - e : 0);
+    }
+}
+
+// column maxima of X (R x Cc, row-major): one thread per column, coalesced across the row
+__global__ __launch_bounds__(256) void col_scale_kernel(const double *__restrict__ X, int R, int Cc, int P,
+                                                        int *__restrict__ scale) {
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= Cc) return;
+    double mx = 0.0;
+    int bad = 0;
+    for (int r = 0; r < R; ++r) {
+        const double v = X[(size_t)r * Cc + c];
+        bad |= !__builtin_isfinite(v);
+        mx = fmax(mx, fabs(v));
+    }
+    int e = 0;
+    if (mx > 0.0) (void)frexp(mx, &e);
+    scale[c] = bad ? OZ_BAD_SCALE : (mx > 0.0 ? P - e : 0);
+}
+
+// ---------------- 2. residues ----------------
+__device__ __forceinline__ signed char sym_mod(double v, double m, double inv_m) {
+    const double q = rint(v * inv_m);
+    double r = fma(-q, m, v);  // exact: the true value is a small integer
+    const double h = 0.5 * (m - 1.0);
+    r = r > h ? r - m : r;
+    r = r < -h ? r + m : r;
+    return (signed char)(int)r;
+}
+
+// out[j][orow][ocol] = residue of round(2^scale * X) mod m_j; X is (R, Cc); with TRANSPOSE the
+// output is (Cc, R) (row = input column). Zero in the padding. Scale applies along `scale_by_row`.
+template <bool TRANSPOSE>
+__global__ __launch_bounds__(256) void residue_kernel(const double *__restrict__ X, int R, int Cc,
+                                                     const int *__restrict__ scale, OzConsts k,
+                                                     signed char *__restrict__ out, int out_rows, int out_cols) {
+    __shared__ signed char tile[OZ_MAX_MODULI][32][33];
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const int r0 = blockIdx.y * 32, c0 = blockIdx.x * 32;
+    for (int dy = ty; dy < 32; dy += 8) {
+        const int r = r0 + dy, c = c0 + tx;
+        double v = 0.0;
+        if (r < R && c < Cc) {
+            const int s = TRANSPOSE ? scale[c] : scale[r];
+            v = (s == OZ_BAD_SCALE) ? 0.0 : rint(ldexp(X[(size_t)r * Cc + c], s));
+        }
+        for (int j = 0; j < k.n; ++j) {
+            const signed char q = sym_mod(v, (double)k.m[j], k.inv_m[j]);
+            if (TRANSPOSE) tile[j][tx][dy] = q; else tile[j][dy][tx] = q;
+        }
+    }
+    __syncthreads();
+    const int o_r0 = TRANSPOSE ? c0 : r0, o_c0 = TRANSPOSE ? r0 : c0;
+    const size_t stride = (size_t)out_rows * out_cols;
+    for (int dy = ty; dy < 32; dy += 8) {
+        const int orow = o_r0 + dy, ocol = o_c0 + tx;
+        if (orow < out_rows && ocol < out_cols)
+            for (int j = 0; j < k.n; ++j) out[j * stride + (size_t)orow * out_cols + ocol] = tile[j][dy][tx];
+    }
+}
+
+// ---------------- 3. batched int8 GEMM with a mod-m epilogue ----------------
+typedef signed char i8x16_t __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ signed char mod_i32(int v, int m) {
+    int r = v % m;  // sign of v
+    const int h = (m - 1) >> 1;
+    r = r > h ? r - m : r;
+    r = r < -h ? r + m : r;
+    return (signed char)r;
+}
+
+// Cres[j] (M x N, int8) = (Ares[j] (Mp x Kp) * Btres[j] (Np x Kp)^T) mod m_j, j = blockIdx.y.
+template <int BM, int BN, int BK, int WM, int WN>
+__global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_i8_nt_mod_kernel(
+    int M, int N, int Kp, const signed char *__restrict__ A, size_t a_stride, const signed char *__restrict__ Bt,
+    size_t b_stride, signed char *__restrict__ Cres, size_t c_stride, OzConsts k, int group_m) {
+    constexpr int NT = (BM / WM) * (BN / WN) * 64;
+    constexpr int WGN = BN / WN;
+    constexpr int TM = WM / 16, TN = WN / 16;
+    constexpr int LDK = BK + 16;   // bytes per padded LDS row
+    constexpr int CH = BK / 16;    // 16-byte chunks per row
+    constexpr int LA = BM * CH / NT, LB = BN * CH / NT;
+    static_assert((BM * CH) % NT == 0 && (BN * CH) % NT == 0, "tile/threads mismatch");
+    static_assert(BK % 64 == 0, "BK multiple of the MFMA K (64)");
+
+    extern __shared__ __attribute__((aligned(16))) signed char smem8[];
+    signed char *As = smem8;                   // [2][BM][LDK]
+    signed char *Bs = smem8 + 2 * BM * LDK;    // [2][BN][LDK]
+
+    const int j = blockIdx.y;
+    A += j * a_stride;
+    Bt += j * b_stride;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int wm = wave / WGN, wn = wave % WGN;
+    int tm, tn;
+    lowp_tile_of(blockIdx.x, (M + BM - 1) / BM, (N + BN - 1) / BN, group_m, tm, tn);
+    const int bm0 = tm * BM, bn0 = tn * BN;
+
+    i32x4_t acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int jj = 0; jj < TN; ++jj) acc[i][jj] = i32x4_t{0, 0, 0, 0};
+
+    u32x4_t ra[LA], rb[LB];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int s = 0; s < LA; ++s) {
+            const int q = tid + s * NT, row = q / CH, ch = q % CH;
+            ra[s] = *reinterpret_cast<const u32x4_t *>(A + (size_t)(bm0 + row) * Kp + k0 + ch * 16);
+        }
+#pragma unroll
+        for (int s = 0; s < LB; ++s) {
+            const int q = tid + s * NT, row = q / CH, ch = q % CH;
+            rb[s] = *reinterpret_cast<const u32x4_t *>(Bt + (size_t)(bn0 + row) * Kp + k0 + ch * 16);
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int s = 0; s < LA; ++s) {
+            const int q = tid + s * NT, row = q / CH, ch = q % CH;
+            *reinterpret_cast<u32x4_t *>(As + (buf * BM + row) * LDK + ch * 16) = ra[s];
+        }
+#pragma unroll
+        for (int s = 0; s < LB; ++s) {
+            const int q = tid + s * NT, row = q / CH, ch = q % CH;
+            *reinterpret_cast<u32x4_t *>(Bs + (buf * BN + row) * LDK + ch * 16) = rb[s];
+        }
+    };
+    const int nk = Kp / BK;
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+        const int cur = kt & 1;
+        if (kt + 1 < nk) gload((kt + 1) * BK);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 64) {
+            i8x16_t fa[TM], fb[TN];
+            const int koff = kk + 16 * (lane >> 4);
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+                fa[i] = *reinterpret_cast<const i8x16_t *>(As + (cur * BM + wm * WM + i * 16 + (lane & 15)) * LDK + koff);
+#pragma unroll
+            for (int jj = 0; jj < TN; ++jj)
+                fb[jj] = *reinterpret_cast<const i8x16_t *>(Bs + (cur * BN + wn * WN + jj * 16 + (lane & 15)) * LDK + koff);
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+#pragma unroll
+                for (int jj = 0; jj < TN; ++jj)
+                    acc[i][jj] = __builtin_amdgcn_mfma_i32_16x16x64_i8(__builtin_bit_cast(i32x4_t, fa[i]),
+                                                                       __builtin_bit_cast(i32x4_t, fb[jj]), acc[i][jj], 0, 0, 0);
+        }
+        if (kt + 1 < nk) sstore(cur ^ 1);
+        __syncthreads();
+    }
+    const int mj = k.m[j];
+    signed char *Cj = Cres + j * c_stride;
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int jj = 0; jj < TN; ++jj) {
+            const int col = bn0 + wn * WN + jj * 16 + (lane & 15);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = bm0 + wm * WM + i * 16 + 4 * (lane >> 4) + r;
+                if (row < M && col < N) Cj[(size_t)row * N + col] = mod_i32(acc[i][jj][r], mj);
+            }
+        }
+}
+
+// ---------------- 4. CRT reconstruction ----------------
+struct dd_t {
+    double hi, lo;
+};
+__device__ __forceinline__ dd_t dd_two_sum(double a, double b) {
+    const double s = a + b, v = s - a;
+    return dd_t{s, (a - (s - v)) + (b - v)};
+}
+
+__global__ __launch_bounds__(256) void crt_kernel(const signed char *__restrict__ Cres, size_t c_stride, int M, int N,
+                                                  const int *__restrict__ srow, const int *__restrict__ scol, OzConsts k,
+                                                  double *__restrict__ C) {
+    const size_t idx = blockIdx.x * (size_t)256 + threadIdx.x;
+    if (idx >= (size_t)M * N) return;
+    const int row = (int)(idx / N), col = (int)(idx % N);
+    const int sr = srow[row], sc = scol[col];
+    if (sr == OZ_BAD_SCALE || sc == OZ_BAD_SCALE) {
+        C[idx] = __builtin_nan("");
+        return;
+    }
+    double L[OZ_LIMBS];
+#pragma unroll
+    for (int t = 0; t < OZ_LIMBS; ++t) L[t] = 0.0;
+    for (int j = 0; j < k.n; ++j) {
+        const double c = (double)Cres[j * c_stride + idx];
+#pragma unroll
+        for (int t = 0; t < OZ_LIMBS; ++t) L[t] = fma(c, k.w[j][t], L[t]);  // exact: |sum| < 2^36
+    }
+    // X_raw = sum_t L[t] 2^(24 t); subtract q M with q = round(X_raw / M)
+    double approx = 0.0;
+#pragma unroll
+    for (int t = OZ_LIMBS - 1; t >= 0; --t) approx = approx * 16777216.0 + L[t];
+    const double q = rint(approx / k.M_approx);
+#pragma unroll
+    for (int t = 0; t < OZ_LIMBS; ++t) L[t] = fma(-q, k.Mlimb[t], L[t]);  // exact
+    // balanced carry normalisation: every limb in [-2^23, 2^23]
+#pragma unroll
+    for (int t = 0; t < OZ_LIMBS - 1; ++t) {
+        const double carry = rint(L[t] * (1.0 / 16777216.0));
+        L[t] = fma(-carry, 16777216.0, L[t]);
+        L[t + 1] += carry;
+    }
+    // double-double Horner from the top limb
+    dd_t x{0.0, 0.0};
+#pragma unroll
+    for (int t = OZ_LIMBS - 1; t >= 0; --t) {
+        x.hi *= 16777216.0;
+        x.lo *= 16777216.0;
+        dd_t s = dd_two_sum(x.hi, L[t]);
+        s.lo += x.lo;
+        x = dd_two_sum(s.hi, s.lo);
+    }
+    C[idx] = ldexp(x.hi + x.lo, -(sr + sc));
+}
+
+// ---------------- host driver ----------------
+template <int BM, int BN, int BK, int WM, int WN>
+static nrk_status launch_i8(int M, int N, int Kp, const signed char *A, size_t sa, const signed char *Bt, size_t sb,
+                            signed char *Cres, size_t sc, const OzConsts &k, hipStream_t stream) {
+    constexpr int threads = (BM / WM) * (BN / WN) * 64;
+    constexpr size_t lds = (size_t)2 * (BM + BN) * (BK + 16);
+    auto kern = gemm_i8_nt_mod_kernel<BM, BN, BK, WM, WN>;
+    static bool attr_set = false;
+    if (!attr_set) {
+        NRK_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        attr_set = true;
+    }
+    dim3 grid(((M + BM - 1) / BM) * ((N + BN - 1) / BN), k.n);
+    hipLaunchKernelGGL(kern, grid, dim3(threads), lds, stream, M, N, Kp, A, sa, Bt, sb, Cres, sc, k, 8);
+    NRK_HIP(hipGetLastError());
+    return NRK_OK;
+}
+
+nrk_status gemm_f64_ozaki(size_t m, size_t k, size_t n, const double *a, const double *b, double *c, int moduli,
+                          hipStream_t stream) {
+    NRK_TRY(ensure_device());
+    if (moduli == 0) moduli = OZ_DEFAULT_MODULI;
+    OzConsts kc;
+    NRK_TRY(make_consts(moduli, &kc));
+    if (m == 0 || n == 0) return NRK_OK;
+    if (k == 0) return gemm<double>(m, 0, n, a, b, c, stream);
+    if (!a || !b || !c) return fail(NRK_ERR_INVALID_ARGUMENT, "A, B or C is NULL");
+    if (k > 130000) return fail(NRK_ERR_INVALID_ARGUMENT, "k = %zu exceeds the exact int32 accumulation bound", k);
+    if (m > (1u << 28) || n > (1u << 28)) return fail(NRK_ERR_INVALID_ARGUMENT, "dimension too large");
+    // largest P with k * 2^(2P) < M / 2, capped at 53 (scaled entries stay exact doubles)
+    const double log2M = std::log2(kc.M_approx);
+    int P = (int)std::floor((log2M - 2.0 - std::log2((double)k)) / 2.0);
+    if (P > 53) P = 53;
+    if (P < 8) return fail(NRK_ERR_INVALID_ARGUMENT, "too few moduli (%d) for k = %zu", moduli, k);
+    const size_t Mp = align_up(m, 256), Np = align_up(n, 256), Kp = align_up(k, 128);
+    const size_t sa = Mp * Kp, sb = Np * Kp, sc = m * n;
+    const size_t bytes = align_up((size_t)moduli * (sa + sb + sc), 256) + (m + n) * sizeof(int);
+    void *ws = nullptr;
+    NRK_TRY(get_workspace(stream, bytes, &ws));
+    signed char *Ares = (signed char *)ws;
+    signed char *Bres = Ares + (size_t)moduli * sa;
+    signed char *Cres = Bres + (size_t)moduli * sb;
+    int *srow = (int *)((char *)ws + align_up((size_t)moduli * (sa + sb + sc), 256));
+    int *scol = srow + m;
+
+    hipLaunchKernelGGL(row_scale_kernel, dim3((unsigned)m), dim3(256), 0, stream, a, (int)m, (int)k, P, srow);
+    NRK_HIP(hipGetLastError());
+    hipLaunchKernelGGL(col_scale_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, stream, b, (int)k, (int)n, P, scol);
+    NRK_HIP(hipGetLastError());
+    hipLaunchKernelGGL((residue_kernel<false>), dim3((unsigned)(Kp / 32), (unsigned)(Mp / 32)), dim3(256), 0, stream, a,
+                       (int)m, (int)k, srow, kc, Ares, (int)Mp, (int)Kp);
+    NRK_HIP(hipGetLastError());
+    hipLaunchKernelGGL((residue_kernel<true>), dim3((unsigned)(Np / 32), (unsigned)(Kp / 32)), dim3(256), 0, stream, b,
+                       (int)k, (int)n, scol, kc, Bres, (int)Np, (int)Kp);
+    NRK_HIP(hipGetLastError());
+    const bool big = ceil_div(m, 128) * ceil_div(n, 128) * moduli >= (size_t)cu_count() * 2;
+    if (big) {
+        NRK_TRY((launch_i8<128, 128, 128, 64, 64>((int)m, (int)n, (int)Kp, Ares, sa, Bres, sb, Cres, sc, kc, stream)));
+    } else {
+        NRK_TRY((launch_i8<64, 64, 128, 32, 32>((int)m, (int)n, (int)Kp, Ares, sa, Bres, sb, Cres, sc, kc, stream)));
+    }
+    hipLaunchKernelGGL(crt_kernel, dim3((unsigned)ceil_div(m * n, 256)), dim3(256), 0, stream, Cres, sc, (int)m, (int)n,
+                       srow, scol, kc, c);
+    NRK_HIP(hipGetLastError());
+    return NRK_OK;
+}
+
+}  // namespace nrk
+
+using namespace nrk;
+
+extern "C" {
+
+nrk_status nrk_dev_gemm_f64_ozaki(size_t m, size_t k, size_t n, const double *a, const double *b, double *c, int32_t moduli,
+                                  nrk_stream stream) {
+    return gemm_f64_ozaki(m, k, n, a, b, c, moduli, (hipStream_t)stream);
+}
+
+nrk_status nrk_gemm_f64_ozaki(size_t m, size_t k, size_t n, const double *a, const double *b, double *c, int32_t moduli) {
+    NRK_TRY(ensure_device());
+    if (m == 0 || n == 0) return NRK_OK;
+    void *da = nullptr, *db = nullptr, *dc = nullptr;
+    NRK_TRY(staging(0, m * k * 8 + 16, &da));
+    NRK_TRY(staging(2, k * n * 8 + 16, &db));
+    NRK_TRY(staging(1, m * n * 8 + 16, &dc));
+    hipStream_t s = nullptr;
+    if (m * k) NRK_HIP(hipMemcpyAsync(da, a, m * k * 8, hipMemcpyHostToDevice, s));
+    if (k * n) NRK_HIP(hipMemcpyAsync(db, b, k * n * 8, hipMemcpyHostToDevice, s));
+    NRK_TRY(gemm_f64_ozaki(m, k, n, (const double *)da, (const double *)db, (double *)dc, moduli, s));
+    NRK_HIP(hipMemcpyAsync(c, dc, m * n * 8, hipMemcpyDeviceToHost, s));
+    NRK_HIP(hipStreamSynchronize(s));
+    return NRK_OK;
+}
+
+}  // extern "C"

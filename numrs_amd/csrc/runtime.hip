@@ -119,6 +119,31 @@ nrk_status get_scratch(hipStream_t stream, size_t n_counters, size_t partial_byt
     return NRK_OK;
 }
 
+// ---- per-(device, stream) workspace ----
+struct StreamWorkspace {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+static std::map<std::pair<int, hipStream_t>, StreamWorkspace> g_workspace;
+
+nrk_status get_workspace(hipStream_t stream, size_t bytes, void **out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    StreamWorkspace &w = g_workspace[{g_thread_device, stream}];
+    if (bytes > w.bytes) {
+        if (w.ptr) {
+            NRK_HIP(hipStreamSynchronize(stream));
+            NRK_HIP(hipFree(w.ptr));
+            w.ptr = nullptr;
+            w.bytes = 0;
+        }
+        const size_t want = align_up(bytes, 1u << 21);
+        NRK_HIP(hipMalloc(&w.ptr, want));
+        w.bytes = want;
+    }
+    *out = w.ptr;
+    return NRK_OK;
+}
+
 // ---- host staging buffers (per thread, per slot) ----
 static thread_local Staging g_staging[4];
 
@@ -247,6 +272,11 @@ nrk_status nrk_stream_destroy(nrk_stream stream) {
             if (it->second.counters) (void)hipFree(it->second.counters);
             if (it->second.partials) (void)hipFree(it->second.partials);
             g_scratch.erase(it);
+        }
+        auto wt = g_workspace.find({g_thread_device, (hipStream_t)stream});
+        if (wt != g_workspace.end()) {
+            if (wt->second.ptr) (void)hipFree(wt->second.ptr);
+            g_workspace.erase(wt);
         }
     }
     NRK_HIP(hipStreamDestroy((hipStream_t)stream));

@@ -154,3 +154,25 @@ def gemm_2d(m: int, k: int, n: int, a, b, c: np.ndarray) -> None:
     fn = {np.dtype(np.float64): "nrk_gemm_2d_f64", np.dtype(np.float32): "nrk_gemm_2d_f32"}[dtype]
     call(fn, C.c_size_t(m), C.c_size_t(k), C.c_size_t(n), _ptr(a), C.c_size_t(a.size), _ptr(b), C.c_size_t(b.size),
          _ptr(c), C.c_size_t(c.size))
+
+
+def gemm_f32_bf16x(a: np.ndarray, b: np.ndarray, products: int = 3) -> np.ndarray:
+    """FP32 GEMM as BF16 split products (3 or 6 MFMA products; 1 = inputs rounded to bf16)."""
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    m, k = a.shape
+    n = b.shape[1]
+    c = np.empty((m, n), np.float32)
+    call("nrk_gemm_f32_bf16x", C.c_size_t(m), C.c_size_t(k), C.c_size_t(n), _ptr(a), _ptr(b), _ptr(c), C.c_int32(products))
+    return c
+
+
+def gemm_f64_ozaki(a: np.ndarray, b: np.ndarray, moduli: int = 0) -> np.ndarray:
+    """FP64 GEMM by the Ozaki scheme II on INT8 matrix cores (moduli 0 = default 16)."""
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    m, k = a.shape
+    n = b.shape[1]
+    c = np.empty((m, n), np.float64)
+    call("nrk_gemm_f64_ozaki", C.c_size_t(m), C.c_size_t(k), C.c_size_t(n), _ptr(a), _ptr(b), _ptr(c), C.c_int32(moduli))
+    return c

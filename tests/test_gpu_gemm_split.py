@@ -1,0 +1,115 @@
+"""Split-precision GEMMs vs the double-double oracle (no reference counterpart: FP64 oracle).
+
+Error measure: e_ij = |C_ij - C*_ij| / sum_k |A_ik||B_kj|, the scale GEMM error bounds use
+(C* = double-double product). Bars, written here:
+  BF16x6 (f32):  max e <= 4 * max e(native FP32 MFMA GEMM) + 2^-22   ("same error" as native)
+  BF16x3 (f32):  max e <= 2^-13                                      (drops ~2^-16-order terms)
+  bf16 inputs:   exact products of the rounded inputs, FP32 accumulation only
+  Ozaki-II (f64, 16 moduli): max e <= 2 * max e(native FP64 MFMA GEMM) + 2^-52
+Inputs follow the split-precision workload: s * 2^e * u, e uniform in [-10, 10], u in [1, 2)."""
+import numpy as np
+import pytest
+
+from numrs_amd import kernels as K
+from numrs_amd import rng
+
+pytestmark = pytest.mark.gpu
+SHAPES = [(17, 31, 13), (64, 64, 64), (200, 300, 100), (512, 512, 512), (257, 1000, 129)]
+
+
+def rel_err(c, exact, scale):
+    return np.abs(c.astype(np.float64) - exact) / np.maximum(scale, 1e-300)
+
+
+def native(a, b):
+    m, k = a.shape
+    n = b.shape[1]
+    c = np.empty((m, n), a.dtype)
+    K.gemm_2d(m, k, n, a, b, c)
+    return c
+
+
+def full_refs(oracle, a, b):
+    a64, b64 = a.astype(np.float64), b.astype(np.float64)
+    return oracle.dd_gemm(a64, b64), np.abs(a64) @ np.abs(b64)
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_bf16x_vs_native_fp32(gpu, oracle, shape):
+    m, k, n = shape
+    a = rng.scaled_pow2(rng.BASE_SEED + 2000 + m, m * k, dtype=np.float32).reshape(m, k)
+    b = rng.scaled_pow2(rng.BASE_SEED + 2100 + n, k * n, dtype=np.float32).reshape(k, n)
+    exact, scale = full_refs(oracle, a, b)
+    e_nat = rel_err(native(a, b), exact, scale).max()
+    e6 = rel_err(K.gemm_f32_bf16x(a, b, 6), exact, scale).max()
+    e3 = rel_err(K.gemm_f32_bf16x(a, b, 3), exact, scale).max()
+    print(f"{shape}: native {e_nat:.2e}  bf16x6 {e6:.2e}  bf16x3 {e3:.2e}")
+    assert e6 <= 4 * e_nat + 2.0 ** -22
+    assert e3 <= 2.0 ** -13
+
+
+def test_bf16_plain_products_exact(gpu, oracle):
+    m, k, n = 96, 160, 80
+    a = rng.normal(1, m * k, np.float32).reshape(m, k)
+    b = rng.normal(2, k * n, np.float32).reshape(k, n)
+    # round inputs to bf16 (RNE) on the host; then products are exact, only FP32 accumulation rounds
+    def to_bf16(x):
+        u = x.view(np.uint32).astype(np.uint64)
+        u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
+        return u.astype(np.uint32).view(np.float32)
+    ab, bb = to_bf16(a), to_bf16(b)
+    exact, scale = full_refs(oracle, ab, bb)
+    e = rel_err(K.gemm_f32_bf16x(a, b, 1), exact, scale).max()
+    assert e <= k * 2.0 ** -24
+
+
+def test_bf16x_degenerate(gpu):
+    c = K.gemm_f32_bf16x(np.zeros((3, 0), np.float32), np.zeros((0, 4), np.float32), 3)
+    assert c.shape == (3, 4) and np.all(c == 0)
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_ozaki_vs_native_fp64(gpu, oracle, shape):
+    m, k, n = shape
+    a = rng.scaled_pow2(rng.BASE_SEED + 2200 + m, m * k).reshape(m, k)
+    b = rng.scaled_pow2(rng.BASE_SEED + 2300 + n, k * n).reshape(k, n)
+    exact, scale = full_refs(oracle, a, b)
+    e_nat = rel_err(native(a, b), exact, scale).max()
+    e_oz = rel_err(K.gemm_f64_ozaki(a, b), exact, scale).max()
+    print(f"{shape}: native fp64 {e_nat:.2e}  ozaki16 {e_oz:.2e}")
+    assert e_oz <= 2 * e_nat + 2.0 ** -52
+
+
+def test_ozaki_integer_inputs_exact(gpu):
+    r = np.random.default_rng(9)
+    a = r.integers(-1000, 1000, (100, 333)).astype(np.float64)
+    b = r.integers(-1000, 1000, (333, 77)).astype(np.float64)
+    np.testing.assert_array_equal(K.gemm_f64_ozaki(a, b), a @ b)
+
+
+def test_ozaki_fewer_moduli_is_coarser(gpu, oracle):
+    """Fewer moduli -> smaller P -> larger (but still bounded) error: e <= 2^-(P-1) with the P the
+    library derives (14 moduli, k = 256: P = floor((log2 M - 2 - 8) / 2) = 49)."""
+    m = k = n = 256
+    a = rng.scaled_pow2(5, m * k).reshape(m, k)
+    b = rng.scaled_pow2(6, k * n).reshape(k, n)
+    exact, scale = full_refs(oracle, a, b)
+    e14 = rel_err(K.gemm_f64_ozaki(a, b, 14), exact, scale).max()
+    e16 = rel_err(K.gemm_f64_ozaki(a, b, 16), exact, scale).max()
+    assert e16 <= e14 <= 2.0 ** -48
+
+
+def test_ozaki_nonfinite_rows_are_nan(gpu):
+    a = np.ones((4, 8))
+    b = np.ones((8, 5))
+    a[2, 3] = np.inf
+    c = K.gemm_f64_ozaki(a, b)
+    assert np.all(np.isnan(c[2])) and np.all(c[[0, 1, 3]] == 8.0)
+
+
+def test_ozaki_zero_k_and_zero_rows(gpu):
+    c = K.gemm_f64_ozaki(np.zeros((3, 0)), np.zeros((0, 4)))
+    assert np.all(c == 0)
+    a = np.zeros((5, 7))
+    b = rng.normal(3, 7 * 6).reshape(7, 6)
+    assert np.all(K.gemm_f64_ozaki(a, b) == 0)
