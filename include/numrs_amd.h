@@ -264,6 +264,43 @@ nrk_status nrk_gemm_f64_ozaki(size_t m, size_t k, size_t n, const double *a, con
 nrk_status nrk_dev_gemm_f64_ozaki(size_t m, size_t k, size_t n, const double *a, const double *b, double *c, int32_t moduli,
                                   nrk_stream stream);
 
+/* ------------------------------------------------------------------------------------------ */
+/* Ragged batches: bf16 GEMMs (fp32 accumulate/output) + f32 row sums in one persistent launch  */
+/* ------------------------------------------------------------------------------------------ */
+/* Generalises array/linalg.rs:56-136 (batched_gemm / par_batch_f64, uniform panels) to ragged
+ * shapes. One problem: C (m x n, f32, row-major) := A (m x k, bf16 bits, row-major) *
+ * B (k x n, bf16 bits, row-major). All pointers are device pointers. */
+typedef struct nrk_gemm_desc {
+    int32_t m, n, k, reserved;
+    const void *a;  /* uint16 bf16 bits */
+    const void *b;  /* uint16 bf16 bits */
+    float *c;
+} nrk_gemm_desc;
+typedef struct nrk_grouped_plan nrk_grouped_plan;
+
+/* Builds the work queue for `gemm_count` GEMMs (host array of device-pointer descriptors) and
+ * `row_count` ragged row sums: rows_out[r] = sum(rows_x[row_offsets[r] .. row_offsets[r+1]))
+ * (row_offsets: HOST array of row_count+1 offsets; rows_x, rows_out: device pointers; sums are
+ * compensated and deterministic). item_cost_ns (host, gemm_count entries, may be NULL) is the
+ * per-tile cost estimate used to order the queue largest-first; NULL uses a flop/byte model.
+ * The plan keeps device copies of the descriptors and offsets. */
+nrk_status nrk_grouped_plan_create(int32_t gemm_count, const nrk_gemm_desc *descs, int32_t row_count, const float *rows_x,
+                                   const int64_t *row_offsets, float *rows_out, const float *item_cost_ns,
+                                   nrk_grouped_plan **plan);
+/* One persistent launch over the whole queue. Replayable (also from a captured graph): the
+ * kernel resets its own queue counters. Runs of one plan must not overlap. */
+nrk_status nrk_grouped_plan_run(const nrk_grouped_plan *plan, nrk_stream stream);
+nrk_status nrk_grouped_plan_info(const nrk_grouped_plan *plan, int32_t *n_items, int32_t *n_gemm_tiles, int32_t *grid);
+nrk_status nrk_grouped_plan_destroy(nrk_grouped_plan *plan);
+/* One bf16 GEMM per launch (the unbatched per-call path the grouped launch is compared to). */
+nrk_status nrk_dev_gemm_bf16(const nrk_gemm_desc *desc, nrk_stream stream);
+
+/* hipGraph capture of work enqueued on a created (non-NULL) stream. */
+nrk_status nrk_graph_begin_capture(nrk_stream stream);
+nrk_status nrk_graph_end_capture(nrk_stream stream, void **graph_exec);
+nrk_status nrk_graph_launch(void *graph_exec, nrk_stream stream);
+nrk_status nrk_graph_destroy(void *graph_exec);
+
 #ifdef __cplusplus
 }
 #endif

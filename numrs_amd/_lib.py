@@ -31,6 +31,12 @@ class ReduceOutputs(C.Structure):
     _fields_ = [("sum", vp), ("mean", vp), ("var", vp), ("min", vp), ("max", vp), ("argmin", vp), ("argmax", vp)]
 
 
+class GemmDesc(C.Structure):
+    """nrk_gemm_desc: C (m x n f32) := A (m x k bf16) * B (k x n bf16), device pointers"""
+    _fields_ = [("m", C.c_int32), ("n", C.c_int32), ("k", C.c_int32), ("reserved", C.c_int32),
+                ("a", vp), ("b", vp), ("c", vp)]
+
+
 # name -> argtypes (all return nrk_status = int32)
 _SIGS: dict[str, list] = {
     "nrk_init": [C.c_int],
@@ -95,6 +101,16 @@ _SIGS: dict[str, list] = {
     "nrk_dev_gemm_f32_bf16x": [c_size, c_size, c_size, vp, vp, vp, c_i32, vp],
     "nrk_gemm_f64_ozaki": [c_size, c_size, c_size, vp, vp, vp, c_i32],
     "nrk_dev_gemm_f64_ozaki": [c_size, c_size, c_size, vp, vp, vp, c_i32, vp],
+    # ragged batches
+    "nrk_grouped_plan_create": [c_i32, vp, c_i32, vp, vp, vp, vp, C.POINTER(vp)],
+    "nrk_grouped_plan_run": [vp, vp],
+    "nrk_grouped_plan_info": [vp, C.POINTER(c_i32), C.POINTER(c_i32), C.POINTER(c_i32)],
+    "nrk_grouped_plan_destroy": [vp],
+    "nrk_dev_gemm_bf16": [vp, vp],
+    "nrk_graph_begin_capture": [vp],
+    "nrk_graph_end_capture": [vp, C.POINTER(vp)],
+    "nrk_graph_launch": [vp, vp],
+    "nrk_graph_destroy": [vp],
 }
 
 _lock = threading.Lock()

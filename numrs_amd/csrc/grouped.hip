@@ -1,0 +1,485 @@
+// Ragged batches in ONE persistent launch: many independent bf16 GEMMs (fp32 accumulate and
+// output) plus many ragged f32 row sums, scheduled as a queue of work items.
+//
+// Reference: array/linalg.rs:56-136 (batched_gemm / par_batch_f64: uniform panels, one task per
+// panel). This generalises it to ragged shapes and adds row reductions to the same launch.
+//
+// Plan (host, once per batch shape): every GEMM is cut into BM x BN output tiles; every row sum
+// becomes either a member of a "short-row group" (rows of <= SHORT_MAX elements packed until the
+// group holds ~GROUP_ELEMS elements; each wavefront of the workgroup sums whole rows) or, for long
+// rows, fixed LONG_CHUNK-element chunks whose partials are combined by the chunk that finishes
+// last (per-row acq_rel counter, fixed combine order => deterministic). Items are sorted by
+// estimated cost, largest first (LPT), using the cost model the caller provides (or the default
+// flop/byte estimate), and uploaded with the GEMM descriptors.
+//
+// Run: a persistent grid (CUs x blocks/CU) pulls items from a device queue counter; the last
+// workgroup to leave resets the counters, so a plan can be replayed -- e.g. from a captured
+// hipGraph -- with no host work and no memset nodes.
+//
+// GEMM tile: 64 x 64 outputs, K-step 32, 4 wavefronts of 32 x 32 (2 x 2 v_mfma_f32_16x16x32_bf16).
+// A (m x k) row-major is staged K-contiguous; B (k x n) row-major is transposed into LDS as
+// Bt[n][k] so that each lane's MFMA operand is 8 contiguous bf16 along K.
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <vector>
+
+#include "gemm_lowp.hpp"
+#include "ops.hpp"
+
+namespace nrk {
+
+constexpr int GB = 256;          // threads per workgroup
+constexpr int GT_M = 64, GT_N = 64, GT_K = 32;
+constexpr int G_LDK = GT_K + 8;  // padded LDS row (bf16)
+constexpr int SHORT_MAX = 2048;  // rows up to this length are summed whole by one wavefront
+constexpr int GROUP_ELEMS = 8192;
+constexpr int LONG_CHUNK = 8192; // chunk of a long row reduced by one workgroup
+
+enum : int { ITEM_GEMM = 0, ITEM_ROWS = 1, ITEM_CHUNK = 2 };
+
+struct Item {
+    int kind;
+    int a;  // GEMM: problem index   ROWS: first row   CHUNK: row
+    int b;  // GEMM: tile index      ROWS: end row     CHUNK: chunk index
+    int c;  // CHUNK: chunks in the row
+};
+
+struct GroupedArgs {
+    const nrk_gemm_desc *descs;
+    const Item *items;
+    int n_items;
+    const float *rows_x;
+    const int64_t *rows_off;
+    float *rows_out;
+    double *chunk_hi, *chunk_lo;        // one per chunk item
+    const int64_t *chunk_base;          // index of the row's first chunk partial, per row (or -1)
+    unsigned int *row_counters;         // per row (long rows only), zero between launches
+    unsigned int *queue;                // [0] = next item, [1] = workgroups exited
+};
+
+// ---------------- GEMM tile ----------------
+__device__ __forceinline__ unsigned short ld_bf16(const unsigned short *p) { return *p; }
+
+__device__ void gemm_tile(const nrk_gemm_desc &d, int tile, unsigned short *As, unsigned short *Bs) {
+    const int M = d.m, N = d.n, K = d.k;
+    const unsigned short *A = (const unsigned short *)d.a;
+    const unsigned short *B = (const unsigned short *)d.b;
+    const int nbn = (N + GT_N - 1) / GT_N;
+    const int bm0 = (tile / nbn) * GT_M, bn0 = (tile % nbn) * GT_N;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int wm = wave >> 1, wn = wave & 1;  // 2 x 2 waves of 32 x 32
+
+    f32x4_t acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+
+    // A tile 64 x 32: thread loads 8 consecutive k of one row  (256 threads = 64 rows x 4 chunks)
+    const int a_row = tid >> 2, a_ch = tid & 3;
+    // B tile 32 x 64: thread loads 8 consecutive n of one k row (256 threads = 32 rows x 8 chunks)
+    const int b_row = tid >> 3, b_ch = tid & 7;
+    const bool a_vec = (K % 8 == 0) && ((reinterpret_cast<uintptr_t>(A) & 15) == 0);
+    const bool b_vec = (N % 8 == 0) && ((reinterpret_cast<uintptr_t>(B) & 15) == 0);
+
+    for (int k0 = 0; k0 < K; k0 += GT_K) {
+        // ---- global -> registers (zero outside the problem)
+        unsigned short av[8], bv[8];
+        {
+            const int r = bm0 + a_row, kk = k0 + a_ch * 8;
+            if (a_vec && r < M && kk + 8 <= K) {
+                u32x4_t v = *reinterpret_cast<const u32x4_t *>(A + (size_t)r * K + kk);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    av[2 * e] = (unsigned short)(v[e] & 0xffff);
+                    av[2 * e + 1] = (unsigned short)(v[e] >> 16);
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) av[e] = (r < M && kk + e < K) ? ld_bf16(A + (size_t)r * K + kk + e) : 0;
+            }
+        }
+        {
+            const int kr = k0 + b_row, c = bn0 + b_ch * 8;
+            if (b_vec && kr < K && c + 8 <= N) {
+                u32x4_t v = *reinterpret_cast<const u32x4_t *>(B + (size_t)kr * N + c);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    bv[2 * e] = (unsigned short)(v[e] & 0xffff);
+                    bv[2 * e + 1] = (unsigned short)(v[e] >> 16);
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) bv[e] = (kr < K && c + e < N) ? ld_bf16(B + (size_t)kr * N + c + e) : 0;
+            }
+        }
+        __syncthreads();  // previous step's fragments are consumed
+#pragma unroll
+        for (int e = 0; e < 8; ++e) As[a_row * G_LDK + a_ch * 8 + e] = av[e];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) Bs[(b_ch * 8 + e) * G_LDK + b_row] = bv[e];  // transpose to Bt[n][k]
+        __syncthreads();
+        const int koff = 8 * (lane >> 4);
+        bf16x8_t fa[2], fb[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+            fa[i] = *reinterpret_cast<const bf16x8_t *>(As + (wm * 32 + i * 16 + (lane & 15)) * G_LDK + koff);
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+            fb[j] = *reinterpret_cast<const bf16x8_t *>(Bs + (wn * 32 + j * 16 + (lane & 15)) * G_LDK + koff);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+    }
+    float *C = d.c;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int col = bn0 + wn * 32 + j * 16 + (lane & 15);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = bm0 + wm * 32 + i * 16 + 4 * (lane >> 4) + r;
+                if (row < M && col < N) C[(size_t)row * N + col] = acc[i][j][r];
+            }
+        }
+}
+
+// ---------------- row sums ----------------
+__device__ __forceinline__ void two_sum_f(float a, float b, float &s, float &e) {
+    s = __fadd_rn(a, b);
+    const float v = __fsub_rn(s, a);
+    e = __fadd_rn(__fsub_rn(a, __fsub_rn(s, v)), __fsub_rn(b, v));
+}
+
+// Double-word wave sum; every lane returns (hi, lo).
+__device__ __forceinline__ void wave_dw(float &hi, float &lo) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        const float bh = __shfl_xor(hi, m, 64), bl = __shfl_xor(lo, m, 64);
+        float s, e;
+        two_sum_f(hi, bh, s, e);
+        e = __fadd_rn(e, __fadd_rn(lo, bl));
+        hi = __fadd_rn(s, e);
+        lo = __fsub_rn(e, __fsub_rn(hi, s));
+    }
+}
+
+// Sum of x[0..len) by one wavefront: lane l adds elements l, l+64, ... pairwise in blocks of 8.
+__device__ __forceinline__ void wave_row_sum(const float *x, int64_t len, float &hi, float &lo) {
+    const int lane = threadIdx.x & 63;
+    float acc = 0.f, comp = 0.f;  // per-lane Kahan-Babuska
+    for (int64_t base = 0; base < len; base += 64 * 8) {
+        float v[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int64_t i = base + e * 64 + lane;
+            v[e] = i < len ? x[i] : 0.f;
+        }
+        const float p = __fadd_rn(__fadd_rn(__fadd_rn(v[0], v[1]), __fadd_rn(v[2], v[3])),
+                                  __fadd_rn(__fadd_rn(v[4], v[5]), __fadd_rn(v[6], v[7])));
+        float s, e;
+        two_sum_f(acc, p, s, e);
+        acc = s;
+        comp = __fadd_rn(comp, e);
+    }
+    hi = acc;
+    lo = comp;
+    wave_dw(hi, lo);
+}
+
+__device__ void rows_item(const GroupedArgs &g, int r0, int r1) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int r = r0 + wave; r < r1; r += GB / 64) {
+        const int64_t off = g.rows_off[r], len = g.rows_off[r + 1] - off;
+        float hi, lo;
+        wave_row_sum(g.rows_x + off, len, hi, lo);
+        if (lane == 0) g.rows_out[r] = (float)((double)hi + (double)lo);
+    }
+}
+
+__device__ void chunk_item(const GroupedArgs &g, int item, int row, int chunk, int nchunks, double *sh) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t off = g.rows_off[row], len = g.rows_off[row + 1] - off;
+    const int64_t c0 = (int64_t)chunk * LONG_CHUNK;
+    const int64_t clen = min((int64_t)LONG_CHUNK, len - c0);
+    // 4 waves each take a quarter of the chunk
+    const int64_t q = (clen + 3) / 4;
+    const int64_t s0 = min(clen, wave * q), s1 = min(clen, s0 + q);
+    float hi, lo;
+    wave_row_sum(g.rows_x + off + c0 + s0, s1 - s0, hi, lo);
+    __syncthreads();
+    if (lane == 0) {
+        sh[2 * wave] = (double)hi;
+        sh[2 * wave + 1] = (double)lo;
+    }
+    __syncthreads();
+    const int64_t pidx = g.chunk_base[row] + chunk;
+    __shared__ int s_last;
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < GB / 64; ++w) t += sh[2 * w] + sh[2 * w + 1];
+        g.chunk_hi[pidx] = t;
+        __threadfence();
+        const unsigned old = __hip_atomic_fetch_add(&g.row_counters[row], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = old == (unsigned)(nchunks - 1);
+        if (s_last) {
+            __hip_atomic_store(&g.row_counters[row], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __threadfence();
+            double tot = 0.0;
+            for (int c = 0; c < nchunks; ++c) tot += __hip_atomic_load(&g.chunk_hi[g.chunk_base[row] + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            g.rows_out[row] = (float)tot;
+        }
+    }
+    __syncthreads();
+    (void)item;
+}
+
+__global__ __launch_bounds__(GB) void grouped_kernel(GroupedArgs g) {
+    __shared__ __attribute__((aligned(16))) unsigned short As[GT_M * G_LDK];
+    __shared__ __attribute__((aligned(16))) unsigned short Bs[GT_N * G_LDK];
+    __shared__ double sh[2 * GB / 64];
+    __shared__ int s_item;
+    for (;;) {
+        if (threadIdx.x == 0)
+            s_item = (int)__hip_atomic_fetch_add(&g.queue[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const int it = s_item;
+        __syncthreads();
+        if (it >= g.n_items) break;
+        const Item w = g.items[it];
+        if (w.kind == ITEM_GEMM) {
+            gemm_tile(g.descs[w.a], w.b, As, Bs);
+        } else if (w.kind == ITEM_ROWS) {
+            rows_item(g, w.a, w.b);
+        } else {
+            chunk_item(g, it, w.a, w.b, w.c, sh);
+        }
+    }
+    if (threadIdx.x == 0) {
+        const unsigned done = __hip_atomic_fetch_add(&g.queue[1], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (done == gridDim.x - 1) {  // every workgroup has made its final fetch: reset for replay
+            __hip_atomic_store(&g.queue[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&g.queue[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+// One plain bf16 GEMM (the unbatched per-call path): one workgroup per tile.
+__global__ __launch_bounds__(GB) void single_gemm_kernel(nrk_gemm_desc d) {
+    __shared__ __attribute__((aligned(16))) unsigned short As[GT_M * G_LDK];
+    __shared__ __attribute__((aligned(16))) unsigned short Bs[GT_N * G_LDK];
+    gemm_tile(d, blockIdx.x, As, Bs);
+}
+
+}  // namespace nrk
+
+// ---------------- plan object ----------------
+struct nrk_grouped_plan {
+    int device;
+    std::vector<nrk_gemm_desc> h_descs;
+    nrk_gemm_desc *d_descs = nullptr;
+    nrk::Item *d_items = nullptr;
+    int n_items = 0;
+    int n_gemm_items = 0, n_row_items = 0;
+    const float *rows_x = nullptr;
+    int64_t *d_rows_off = nullptr;
+    float *rows_out = nullptr;
+    int n_rows = 0;
+    double *d_chunk = nullptr;
+    int64_t *d_chunk_base = nullptr;
+    unsigned int *d_counters = nullptr;  // rows + 2 (queue)
+    int grid = 0;
+};
+
+using namespace nrk;
+
+namespace {
+void free_plan(nrk_grouped_plan *p) {
+    if (!p) return;
+    (void)hipFree(p->d_descs);
+    (void)hipFree(p->d_items);
+    (void)hipFree(p->d_rows_off);
+    (void)hipFree(p->d_chunk);
+    (void)hipFree(p->d_chunk_base);
+    (void)hipFree(p->d_counters);
+    delete p;
+}
+}  // namespace
+
+extern "C" {
+
+nrk_status nrk_grouped_plan_create(int32_t gemm_count, const nrk_gemm_desc *descs, int32_t row_count, const float *rows_x,
+                                   const int64_t *row_offsets, float *rows_out, const float *item_cost_ns,
+                                   nrk_grouped_plan **out_plan) {
+    NRK_TRY(ensure_device());
+    if (!out_plan) return fail(NRK_ERR_INVALID_ARGUMENT, "out_plan is NULL");
+    *out_plan = nullptr;
+    if (gemm_count < 0 || row_count < 0) return fail(NRK_ERR_INVALID_ARGUMENT, "negative count");
+    if (gemm_count > 0 && !descs) return fail(NRK_ERR_INVALID_ARGUMENT, "descs is NULL");
+    if (row_count > 0 && (!rows_x || !row_offsets || !rows_out)) return fail(NRK_ERR_INVALID_ARGUMENT, "row arrays are NULL");
+    auto *p = new nrk_grouped_plan();
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    p->device = dev;
+    std::vector<Item> items;
+    std::vector<double> cost;
+    // GEMM tiles; default cost: MFMA work of the tile plus its operand bytes
+    for (int g = 0; g < gemm_count; ++g) {
+        const nrk_gemm_desc &d = descs[g];
+        if (d.m < 0 || d.n < 0 || d.k < 0 || (d.m * (int64_t)d.n > 0 && !d.c) || (d.k > 0 && d.m * (int64_t)d.n > 0 && (!d.a || !d.b))) {
+            free_plan(p);
+            return fail(NRK_ERR_INVALID_ARGUMENT, "descriptor %d is invalid", g);
+        }
+        const int tiles = ((d.m + GT_M - 1) / GT_M) * ((d.n + GT_N - 1) / GT_N);
+        for (int t = 0; t < tiles; ++t) {
+            items.push_back(Item{ITEM_GEMM, g, t, 0});
+            cost.push_back(item_cost_ns ? (double)item_cost_ns[g] : 200.0 + (double)std::max(d.k, 1) * 2.0);
+        }
+    }
+    p->n_gemm_items = (int)items.size();
+    // rows: short rows grouped, long rows chunked
+    std::vector<int64_t> chunk_base(row_count, -1);
+    int64_t n_chunks = 0;
+    int r = 0;
+    while (r < row_count) {
+        const int64_t len = row_offsets[r + 1] - row_offsets[r];
+        if (len < 0) {
+            free_plan(p);
+            return fail(NRK_ERR_INVALID_ARGUMENT, "row_offsets decreases at row %d", r);
+        }
+        if (len > SHORT_MAX) {
+            const int nc = (int)((len + LONG_CHUNK - 1) / LONG_CHUNK);
+            chunk_base[r] = n_chunks;
+            for (int c = 0; c < nc; ++c) {
+                items.push_back(Item{ITEM_CHUNK, r, c, nc});
+                cost.push_back(300.0 + std::min<int64_t>(LONG_CHUNK, len - (int64_t)c * LONG_CHUNK) * 0.6);
+            }
+            n_chunks += nc;
+            ++r;
+        } else {
+            int r1 = r;
+            int64_t acc = 0;
+            while (r1 < row_count) {
+                const int64_t l = row_offsets[r1 + 1] - row_offsets[r1];
+                if (l > SHORT_MAX || (acc + l > GROUP_ELEMS && r1 > r)) break;
+                acc += l;
+                ++r1;
+            }
+            items.push_back(Item{ITEM_ROWS, r, r1, 0});
+            cost.push_back(300.0 + acc * 0.6);
+            r = r1;
+        }
+    }
+    p->n_row_items = (int)items.size() - p->n_gemm_items;
+    // LPT: most expensive first
+    std::vector<int> order(items.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+    std::vector<Item> sorted(items.size());
+    for (size_t i = 0; i < order.size(); ++i) sorted[i] = items[order[i]];
+    p->n_items = (int)sorted.size();
+    p->h_descs.assign(descs, descs + gemm_count);
+    p->rows_x = rows_x;
+    p->rows_out = rows_out;
+    p->n_rows = row_count;
+    auto up = [&](void **dst, const void *src, size_t bytes) -> nrk_status {
+        NRK_HIP(hipMalloc(dst, bytes ? bytes : 16));
+        if (bytes) NRK_HIP(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+        return NRK_OK;
+    };
+    nrk_status st = NRK_OK;
+    if ((st = up((void **)&p->d_descs, descs, sizeof(nrk_gemm_desc) * gemm_count)) != NRK_OK ||
+        (st = up((void **)&p->d_items, sorted.data(), sizeof(Item) * sorted.size())) != NRK_OK ||
+        (st = up((void **)&p->d_rows_off, row_offsets, row_count ? sizeof(int64_t) * (row_count + 1) : 0)) != NRK_OK ||
+        (st = up((void **)&p->d_chunk_base, chunk_base.data(), sizeof(int64_t) * chunk_base.size())) != NRK_OK) {
+        free_plan(p);
+        return st;
+    }
+    if (hipMalloc(&p->d_chunk, sizeof(double) * (n_chunks ? n_chunks : 1)) != hipSuccess ||
+        hipMalloc(&p->d_counters, sizeof(unsigned) * (row_count + 2)) != hipSuccess ||
+        hipMemset(p->d_counters, 0, sizeof(unsigned) * (row_count + 2)) != hipSuccess) {
+        free_plan(p);
+        return fail(NRK_ERR_OUT_OF_MEMORY, "plan allocation failed");
+    }
+    p->grid = std::max(1, std::min(p->n_items, cu_count() * 4));
+    *out_plan = p;
+    return NRK_OK;
+}
+
+nrk_status nrk_grouped_plan_run(const nrk_grouped_plan *p, nrk_stream stream) {
+    if (!p) return fail(NRK_ERR_INVALID_ARGUMENT, "plan is NULL");
+    if (p->n_items == 0) return NRK_OK;
+    GroupedArgs g{};
+    g.descs = p->d_descs;
+    g.items = p->d_items;
+    g.n_items = p->n_items;
+    g.rows_x = p->rows_x;
+    g.rows_off = p->d_rows_off;
+    g.rows_out = p->rows_out;
+    g.chunk_hi = p->d_chunk;
+    g.chunk_base = p->d_chunk_base;
+    g.row_counters = p->d_counters;
+    g.queue = p->d_counters + p->n_rows;
+    hipLaunchKernelGGL(grouped_kernel, dim3(p->grid), dim3(GB), 0, (hipStream_t)stream, g);
+    NRK_HIP(hipGetLastError());
+    return NRK_OK;
+}
+
+nrk_status nrk_grouped_plan_info(const nrk_grouped_plan *p, int32_t *n_items, int32_t *n_gemm_tiles, int32_t *grid) {
+    if (!p) return fail(NRK_ERR_INVALID_ARGUMENT, "plan is NULL");
+    if (n_items) *n_items = p->n_items;
+    if (n_gemm_tiles) *n_gemm_tiles = p->n_gemm_items;
+    if (grid) *grid = p->grid;
+    return NRK_OK;
+}
+
+nrk_status nrk_grouped_plan_destroy(nrk_grouped_plan *p) {
+    free_plan(p);
+    return NRK_OK;
+}
+
+nrk_status nrk_dev_gemm_bf16(const nrk_gemm_desc *desc, nrk_stream stream) {
+    NRK_TRY(ensure_device());
+    if (!desc) return fail(NRK_ERR_INVALID_ARGUMENT, "desc is NULL");
+    const nrk_gemm_desc d = *desc;
+    if (d.m <= 0 || d.n <= 0) return NRK_OK;
+    const int tiles = ((d.m + GT_M - 1) / GT_M) * ((d.n + GT_N - 1) / GT_N);
+    hipLaunchKernelGGL(single_gemm_kernel, dim3(tiles), dim3(GB), 0, (hipStream_t)stream, d);
+    NRK_HIP(hipGetLastError());
+    return NRK_OK;
+}
+
+// ---- hipGraph capture helpers ----
+nrk_status nrk_graph_begin_capture(nrk_stream stream) {
+    NRK_TRY(ensure_device());
+    if (!stream) return fail(NRK_ERR_INVALID_ARGUMENT, "capture needs a created stream (not NULL)");
+    NRK_HIP(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal));
+    return NRK_OK;
+}
+
+nrk_status nrk_graph_end_capture(nrk_stream stream, void **graph_exec) {
+    if (!graph_exec) return fail(NRK_ERR_INVALID_ARGUMENT, "graph_exec is NULL");
+    hipGraph_t g = nullptr;
+    NRK_HIP(hipStreamEndCapture((hipStream_t)stream, &g));
+    hipGraphExec_t e = nullptr;
+    hipError_t err = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (err != hipSuccess) return hip_fail(err, "hipGraphInstantiate", __FILE__, __LINE__);
+    *graph_exec = (void *)e;
+    return NRK_OK;
+}
+
+nrk_status nrk_graph_launch(void *graph_exec, nrk_stream stream) {
+    NRK_HIP(hipGraphLaunch((hipGraphExec_t)graph_exec, (hipStream_t)stream));
+    return NRK_OK;
+}
+
+nrk_status nrk_graph_destroy(void *graph_exec) {
+    if (graph_exec) NRK_HIP(hipGraphExecDestroy((hipGraphExec_t)graph_exec));
+    return NRK_OK;
+}
+
+}  // extern "C"

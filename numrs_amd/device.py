@@ -109,3 +109,8 @@ def synchronize() -> None:
 def init(device: int = 0) -> None:
     load()
     call("nrk_init", C.c_int(device))
+
+
+def call_memset(buf: DeviceBuffer, stream=None) -> None:
+    """Zero a device buffer (eager; never recorded into a graph by this library)."""
+    call("nrk_memset_zero", C.c_void_p(buf.ptr), C.c_size_t(buf.nbytes), C.c_void_p(stream))
