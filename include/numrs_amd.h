@@ -295,6 +295,37 @@ nrk_status nrk_grouped_plan_destroy(nrk_grouped_plan *plan);
 /* One bf16 GEMM per launch (the unbatched per-call path the grouped launch is compared to). */
 nrk_status nrk_dev_gemm_bf16(const nrk_gemm_desc *desc, nrk_stream stream);
 
+/* ------------------------------------------------------------------------------------------ */
+/* Autotuner and learned selector                                                             */
+/* ------------------------------------------------------------------------------------------ */
+/* No reference counterpart: the reference hard-codes its tiers (kernels/mod.rs:63-106,
+ * kernels/gemm.rs:290-312). Operation families and their launch-configuration tables:
+ *   NRK_DT_F64 / NRK_DT_F32 GEMM (d0=m, d1=n, d2=k): configs 0..4 = MFMA tiles 32x32, 64x64,
+ *   128x64, 64x128, 128x128;  NRK_DT_BF16 NN GEMM (bf16 bits in, f32 out), same tiles;
+ *   NRK_DT_ROWSUM_F32 row sums of a (d0 rows, d1 cols) matrix: configs 0..3 = 1, 2, 4, 8
+ *   workgroups per CU (results identical across configs). */
+enum { NRK_DT_F64 = 0, NRK_DT_F32 = 1, NRK_DT_BF16 = 2, NRK_DT_ROWSUM_F32 = 3 };
+nrk_status nrk_config_count(int32_t family, int32_t *count);
+/* Run one configuration (device pointers; c receives the result). */
+nrk_status nrk_dev_run_config(int32_t family, int32_t config, int32_t d0, int32_t d1, int32_t d2, const void *a,
+                              const void *b, void *c, nrk_stream stream);
+/* Exhaustive search: times every configuration on every shape (one warm-up, then the mean of
+ * `reps` launches, HIP events) on library-owned operands; times_ns is count x n_configs. */
+nrk_status nrk_tune(int32_t family, int32_t count, const int32_t *d0, const int32_t *d1, const int32_t *d2, int32_t reps,
+                    float *times_ns);
+/* MLP cost model 6 -> 64 -> 64 -> K (ReLU, fp32), K <= 8. Parameters (device, contiguous):
+ * W1[64][6] b1[64] W2[64][64] b2[64] W3[K][64] b3[K]; nrk_mlp_param_count(K) floats. */
+size_t nrk_mlp_param_count(int32_t K);
+/* y (N x K) = MLP(x (N x 6)); device pointers. */
+nrk_status nrk_mlp_forward(const float *params, int32_t K, const float *x, int32_t N, float *y, nrk_stream stream);
+/* One full-batch step on loss = sum(mask * (y - target)^2) / sum(mask): forward, backward,
+ * weight gradients (written to grad_out if not NULL), then Adam (lr = 0 skips the update;
+ * step >= 1 is the Adam step count). loss_out (device, may be NULL) receives
+ * sum(mask * (y - target)^2). All pointers are device pointers except none. */
+nrk_status nrk_mlp_train_step(float *params, float *adam_m, float *adam_v, int32_t step, int32_t K, const float *x,
+                              const float *target, const float *mask, int32_t N, float lr, float *loss_out,
+                              float *grad_out, nrk_stream stream);
+
 /* hipGraph capture of work enqueued on a created (non-NULL) stream. */
 nrk_status nrk_graph_begin_capture(nrk_stream stream);
 nrk_status nrk_graph_end_capture(nrk_stream stream, void **graph_exec);

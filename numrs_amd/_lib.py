@@ -111,7 +111,14 @@ _SIGS: dict[str, list] = {
     "nrk_graph_end_capture": [vp, C.POINTER(vp)],
     "nrk_graph_launch": [vp, vp],
     "nrk_graph_destroy": [vp],
+    # tuner / selector
+    "nrk_config_count": [c_i32, C.POINTER(c_i32)],
+    "nrk_dev_run_config": [c_i32, c_i32, c_i32, c_i32, c_i32, vp, vp, vp, vp],
+    "nrk_tune": [c_i32, c_i32, vp, vp, vp, c_i32, vp],
+    "nrk_mlp_forward": [vp, c_i32, vp, c_i32, vp, vp],
+    "nrk_mlp_train_step": [vp, vp, vp, c_i32, c_i32, vp, vp, vp, c_i32, C.c_float, vp, vp, vp],
 }
+_SIZE_FNS = {"nrk_mlp_param_count": [c_i32]}
 
 _lock = threading.Lock()
 _lib: C.CDLL | None = None
@@ -138,6 +145,10 @@ def load() -> C.CDLL:
             fn = getattr(lib, name)
             fn.argtypes = args
             fn.restype = C.c_int32
+        for name, args in _SIZE_FNS.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = c_size
         for name in ("nrk_status_string", "nrk_last_error", "nrk_version"):
             getattr(lib, name).restype = C.c_char_p
         lib.nrk_status_string.argtypes = [C.c_int32]
@@ -162,4 +173,4 @@ def declared_symbols() -> list[str]:
 
     header = Path(__file__).resolve().parent.parent / "include" / "numrs_amd.h"
     text = header.read_text()
-    return sorted(set(re.findall(r"^\s*(?:nrk_status|const char \*)\s*(nrk_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:nrk_status|const char \*|size_t)\s*(nrk_\w+)\s*\(", text, re.M)))

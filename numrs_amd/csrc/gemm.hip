@@ -193,6 +193,25 @@ nrk_status gemm(size_t m, size_t k, size_t n, const T *a, const T *b, T *c, hipS
 template nrk_status gemm<double>(size_t, size_t, size_t, const double *, const double *, double *, hipStream_t);
 template nrk_status gemm<float>(size_t, size_t, size_t, const float *, const float *, float *, hipStream_t);
 
+// Tile configurations the tuner times and the selector chooses from (all 4 wavefronts).
+template <typename T>
+nrk_status gemm_config(int config, size_t m, size_t k, size_t n, const T *a, const T *b, T *c, hipStream_t stream) {
+    NRK_TRY(ensure_device());
+    if (m == 0 || n == 0 || k == 0) return gemm<T>(m, k, n, a, b, c, stream);
+    if (!a || !b || !c) return fail(NRK_ERR_INVALID_ARGUMENT, "A, B or C is NULL");
+    const int M = (int)m, N = (int)n, K = (int)k;
+    switch (config) {
+        case 0: return launch_gemm<T, 32, 32, 16, 16, 16>(M, N, K, a, b, c, stream);
+        case 1: return launch_gemm<T, 64, 64, 16, 32, 32>(M, N, K, a, b, c, stream);
+        case 2: return launch_gemm<T, 128, 64, 16, 64, 32>(M, N, K, a, b, c, stream);
+        case 3: return launch_gemm<T, 64, 128, 16, 32, 64>(M, N, K, a, b, c, stream);
+        case 4: return launch_gemm<T, 128, 128, 16, 64, 64>(M, N, K, a, b, c, stream);
+        default: return fail(NRK_ERR_INVALID_ARGUMENT, "GEMM config %d outside [0,4]", config);
+    }
+}
+template nrk_status gemm_config<double>(int, size_t, size_t, size_t, const double *, const double *, double *, hipStream_t);
+template nrk_status gemm_config<float>(int, size_t, size_t, size_t, const float *, const float *, float *, hipStream_t);
+
 }  // namespace nrk
 
 using namespace nrk;

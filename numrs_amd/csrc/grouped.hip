@@ -61,90 +61,113 @@ struct GroupedArgs {
 // ---------------- GEMM tile ----------------
 __device__ __forceinline__ unsigned short ld_bf16(const unsigned short *p) { return *p; }
 
-__device__ void gemm_tile(const nrk_gemm_desc &d, int tile, unsigned short *As, unsigned short *Bs) {
+// One BM x BN output tile of an NN bf16 problem, 4 wavefronts in a 2 x 2 arrangement, K-step 32.
+template <int BM, int BN>
+__device__ void gemm_tile_t(const nrk_gemm_desc &d, int tile, unsigned short *As, unsigned short *Bs) {
+    constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
+    constexpr int ACH = BM * (GT_K / 8);   // 16-byte chunks of the A tile
+    constexpr int BCH = GT_K * (BN / 8);   // 16-byte chunks of the B tile
+    constexpr int LA = (ACH + GB - 1) / GB, LB = (BCH + GB - 1) / GB;
     const int M = d.m, N = d.n, K = d.k;
     const unsigned short *A = (const unsigned short *)d.a;
     const unsigned short *B = (const unsigned short *)d.b;
-    const int nbn = (N + GT_N - 1) / GT_N;
-    const int bm0 = (tile / nbn) * GT_M, bn0 = (tile % nbn) * GT_N;
+    const int nbn = (N + BN - 1) / BN;
+    const int bm0 = (tile / nbn) * BM, bn0 = (tile % nbn) * BN;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int wm = wave >> 1, wn = wave & 1;  // 2 x 2 waves of 32 x 32
+    const int wm = wave >> 1, wn = wave & 1;
 
-    f32x4_t acc[2][2];
+    f32x4_t acc[TM][TN];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < TM; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
-
-    // A tile 64 x 32: thread loads 8 consecutive k of one row  (256 threads = 64 rows x 4 chunks)
-    const int a_row = tid >> 2, a_ch = tid & 3;
-    // B tile 32 x 64: thread loads 8 consecutive n of one k row (256 threads = 32 rows x 8 chunks)
-    const int b_row = tid >> 3, b_ch = tid & 7;
+        for (int j = 0; j < TN; ++j) acc[i][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
     const bool a_vec = (K % 8 == 0) && ((reinterpret_cast<uintptr_t>(A) & 15) == 0);
     const bool b_vec = (N % 8 == 0) && ((reinterpret_cast<uintptr_t>(B) & 15) == 0);
 
     for (int k0 = 0; k0 < K; k0 += GT_K) {
         // ---- global -> registers (zero outside the problem)
-        unsigned short av[8], bv[8];
-        {
-            const int r = bm0 + a_row, kk = k0 + a_ch * 8;
-            if (a_vec && r < M && kk + 8 <= K) {
-                u32x4_t v = *reinterpret_cast<const u32x4_t *>(A + (size_t)r * K + kk);
+        unsigned short av[LA][8], bv[LB][8];
+#pragma unroll
+        for (int s = 0; s < LA; ++s) {
+            const int q = tid + s * GB;
+            const int r = bm0 + q / (GT_K / 8), kk = k0 + (q % (GT_K / 8)) * 8;
+            if (q < ACH && a_vec && r < M && kk + 8 <= K) {
+                const u32x4_t v = *reinterpret_cast<const u32x4_t *>(A + (size_t)r * K + kk);
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    av[2 * e] = (unsigned short)(v[e] & 0xffff);
-                    av[2 * e + 1] = (unsigned short)(v[e] >> 16);
+                    av[s][2 * e] = (unsigned short)(v[e] & 0xffff);
+                    av[s][2 * e + 1] = (unsigned short)(v[e] >> 16);
                 }
             } else {
 #pragma unroll
-                for (int e = 0; e < 8; ++e) av[e] = (r < M && kk + e < K) ? ld_bf16(A + (size_t)r * K + kk + e) : 0;
+                for (int e = 0; e < 8; ++e) av[s][e] = (q < ACH && r < M && kk + e < K) ? ld_bf16(A + (size_t)r * K + kk + e) : 0;
             }
         }
-        {
-            const int kr = k0 + b_row, c = bn0 + b_ch * 8;
-            if (b_vec && kr < K && c + 8 <= N) {
-                u32x4_t v = *reinterpret_cast<const u32x4_t *>(B + (size_t)kr * N + c);
+#pragma unroll
+        for (int s = 0; s < LB; ++s) {
+            const int q = tid + s * GB;
+            const int kr = k0 + q / (BN / 8), c = bn0 + (q % (BN / 8)) * 8;
+            if (q < BCH && b_vec && kr < K && c + 8 <= N) {
+                const u32x4_t v = *reinterpret_cast<const u32x4_t *>(B + (size_t)kr * N + c);
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    bv[2 * e] = (unsigned short)(v[e] & 0xffff);
-                    bv[2 * e + 1] = (unsigned short)(v[e] >> 16);
+                    bv[s][2 * e] = (unsigned short)(v[e] & 0xffff);
+                    bv[s][2 * e + 1] = (unsigned short)(v[e] >> 16);
                 }
             } else {
 #pragma unroll
-                for (int e = 0; e < 8; ++e) bv[e] = (kr < K && c + e < N) ? ld_bf16(B + (size_t)kr * N + c + e) : 0;
+                for (int e = 0; e < 8; ++e) bv[s][e] = (q < BCH && kr < K && c + e < N) ? ld_bf16(B + (size_t)kr * N + c + e) : 0;
             }
         }
         __syncthreads();  // previous step's fragments are consumed
 #pragma unroll
-        for (int e = 0; e < 8; ++e) As[a_row * G_LDK + a_ch * 8 + e] = av[e];
+        for (int s = 0; s < LA; ++s) {
+            const int q = tid + s * GB;
+            if (q < ACH) {
+                const int row = q / (GT_K / 8), ch = q % (GT_K / 8);
 #pragma unroll
-        for (int e = 0; e < 8; ++e) Bs[(b_ch * 8 + e) * G_LDK + b_row] = bv[e];  // transpose to Bt[n][k]
+                for (int e = 0; e < 8; ++e) As[row * G_LDK + ch * 8 + e] = av[s][e];
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < LB; ++s) {
+            const int q = tid + s * GB;
+            if (q < BCH) {
+                const int krow = q / (BN / 8), ch = q % (BN / 8);
+#pragma unroll
+                for (int e = 0; e < 8; ++e) Bs[(ch * 8 + e) * G_LDK + krow] = bv[s][e];  // transpose to Bt[n][k]
+            }
+        }
         __syncthreads();
         const int koff = 8 * (lane >> 4);
-        bf16x8_t fa[2], fb[2];
+        bf16x8_t fa[TM], fb[TN];
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
-            fa[i] = *reinterpret_cast<const bf16x8_t *>(As + (wm * 32 + i * 16 + (lane & 15)) * G_LDK + koff);
+        for (int i = 0; i < TM; ++i)
+            fa[i] = *reinterpret_cast<const bf16x8_t *>(As + (wm * WM + i * 16 + (lane & 15)) * G_LDK + koff);
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
-            fb[j] = *reinterpret_cast<const bf16x8_t *>(Bs + (wn * 32 + j * 16 + (lane & 15)) * G_LDK + koff);
+        for (int j = 0; j < TN; ++j)
+            fb[j] = *reinterpret_cast<const bf16x8_t *>(Bs + (wn * WN + j * 16 + (lane & 15)) * G_LDK + koff);
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < TM; ++i)
 #pragma unroll
-            for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+            for (int j = 0; j < TN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
     }
     float *C = d.c;
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < TM; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const int col = bn0 + wn * 32 + j * 16 + (lane & 15);
+        for (int j = 0; j < TN; ++j) {
+            const int col = bn0 + wn * WN + j * 16 + (lane & 15);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int row = bm0 + wm * 32 + i * 16 + 4 * (lane >> 4) + r;
+                const int row = bm0 + wm * WM + i * 16 + 4 * (lane >> 4) + r;
                 if (row < M && col < N) C[(size_t)row * N + col] = acc[i][j][r];
             }
         }
+}
+
+__device__ __forceinline__ void gemm_tile(const nrk_gemm_desc &d, int tile, unsigned short *As, unsigned short *Bs) {
+    gemm_tile_t<GT_M, GT_N>(d, tile, As, Bs);
 }
 
 // ---------------- row sums ----------------
@@ -267,11 +290,34 @@ __global__ __launch_bounds__(GB) void grouped_kernel(GroupedArgs g) {
     }
 }
 
-// One plain bf16 GEMM (the unbatched per-call path): one workgroup per tile.
+// One plain bf16 GEMM per launch (the unbatched per-call path and the tuner's candidates):
+// one workgroup per BM x BN tile.
+template <int BM, int BN>
 __global__ __launch_bounds__(GB) void single_gemm_kernel(nrk_gemm_desc d) {
-    __shared__ __attribute__((aligned(16))) unsigned short As[GT_M * G_LDK];
-    __shared__ __attribute__((aligned(16))) unsigned short Bs[GT_N * G_LDK];
-    gemm_tile(d, blockIdx.x, As, Bs);
+    __shared__ __attribute__((aligned(16))) unsigned short As[BM * G_LDK];
+    __shared__ __attribute__((aligned(16))) unsigned short Bs[BN * G_LDK];
+    gemm_tile_t<BM, BN>(d, blockIdx.x, As, Bs);
+}
+
+template <int BM, int BN>
+nrk_status launch_single_bf16(const nrk_gemm_desc &d, hipStream_t stream) {
+    const int tiles = ((d.m + BM - 1) / BM) * ((d.n + BN - 1) / BN);
+    hipLaunchKernelGGL((single_gemm_kernel<BM, BN>), dim3(tiles), dim3(GB), 0, stream, d);
+    NRK_HIP(hipGetLastError());
+    return NRK_OK;
+}
+
+// Tile configurations of the bf16 NN GEMM, indexed by the tuner / selector.
+nrk_status gemm_bf16_config(int config, const nrk_gemm_desc &d, hipStream_t stream) {
+    if (d.m <= 0 || d.n <= 0) return NRK_OK;
+    switch (config) {
+        case 0: return launch_single_bf16<32, 32>(d, stream);
+        case 1: return launch_single_bf16<64, 64>(d, stream);
+        case 2: return launch_single_bf16<128, 64>(d, stream);
+        case 3: return launch_single_bf16<64, 128>(d, stream);
+        case 4: return launch_single_bf16<128, 128>(d, stream);
+        default: return fail(NRK_ERR_INVALID_ARGUMENT, "bf16 GEMM config %d outside [0,4]", config);
+    }
 }
 
 }  // namespace nrk
@@ -444,12 +490,7 @@ nrk_status nrk_grouped_plan_destroy(nrk_grouped_plan *p) {
 nrk_status nrk_dev_gemm_bf16(const nrk_gemm_desc *desc, nrk_stream stream) {
     NRK_TRY(ensure_device());
     if (!desc) return fail(NRK_ERR_INVALID_ARGUMENT, "desc is NULL");
-    const nrk_gemm_desc d = *desc;
-    if (d.m <= 0 || d.n <= 0) return NRK_OK;
-    const int tiles = ((d.m + GT_M - 1) / GT_M) * ((d.n + GT_N - 1) / GT_N);
-    hipLaunchKernelGGL(single_gemm_kernel, dim3(tiles), dim3(GB), 0, (hipStream_t)stream, d);
-    NRK_HIP(hipGetLastError());
-    return NRK_OK;
+    return gemm_bf16_config(1, *desc, (hipStream_t)stream);  // the 64 x 64 tile the grouped launch uses
 }
 
 // ---- hipGraph capture helpers ----

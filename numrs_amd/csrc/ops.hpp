@@ -11,6 +11,15 @@ template <typename T>
 nrk_status seg_reduce(const Chain &chain, const T *x, size_t rows, size_t cols, double ddof,
                       const nrk_reduce_outputs *row_out, const nrk_reduce_outputs *full_out, hipStream_t stream);
 
+// Workgroups per CU of the segment reducer's persistent grid (result bits are independent of it).
+int reduce_blocks_per_cu();
+void set_reduce_blocks_per_cu(int b);
+
+// Tile-configuration dispatch for the tuner and the learned selector (configs.hip).
+template <typename T>
+nrk_status gemm_config(int config, size_t m, size_t k, size_t n, const T *a, const T *b, T *c, hipStream_t stream);
+nrk_status gemm_bf16_config(int config, const nrk_gemm_desc &d, hipStream_t stream);
+
 // Reductions along a strided axis (inner > 1): one thread per output, left-to-right fold in the
 // reference's order (reduce_axis.hip).
 template <typename T>

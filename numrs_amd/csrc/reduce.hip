@@ -39,6 +39,12 @@ constexpr uint64_t NO_INDEX = ~0ull;
 
 enum : int { F_M2 = 1, F_MAX = 2, F_MIN = 4 };
 
+// Persistent-grid size of the segment reducer (workgroups per CU). The result bits do not depend
+// on it (fixed segment tree); only speed does, so the tuner may choose it per shape.
+static thread_local int g_blocks_per_cu = 8;
+int reduce_blocks_per_cu() { return g_blocks_per_cu; }
+void set_reduce_blocks_per_cu(int b) { g_blocks_per_cu = b < 1 ? 1 : (b > 16 ? 16 : b); }
+
 template <typename T>
 struct OutPtrs {
     T *sum, *mean, *var, *min, *max;
@@ -469,7 +475,7 @@ static bool any_out(const OutPtrs<T> &o) {
 
 template <typename T, int VPT, int FLAGS>
 static nrk_status launch_flags(SegParams<T> &p, hipStream_t stream) {
-    int grid = (int)std::min<size_t>(p.nseg, (size_t)cu_count() * 8);
+    int grid = (int)std::min<size_t>(p.nseg, (size_t)cu_count() * reduce_blocks_per_cu());
     hipLaunchKernelGGL((seg_reduce_kernel<T, VPT, FLAGS>), dim3(grid), dim3(RB), 0, stream, p);
     NRK_HIP(hipGetLastError());
     return NRK_OK;
