@@ -93,9 +93,12 @@ __device__ __forceinline__ float rn_sub(float a, float b) { return __fsub_rn(a, 
 __device__ __forceinline__ double rn_sub(double a, double b) { return __dsub_rn(a, b); }
 __device__ __forceinline__ float rn_mul(float a, float b) { return __fmul_rn(a, b); }
 __device__ __forceinline__ double rn_mul(double a, double b) { return __dmul_rn(a, b); }
-__device__ __forceinline__ float rn_div(float a, float b) { return __fdiv_rn(a, b); }
+// binary32 division and square root are evaluated in binary64 and rounded once: since
+// 53 >= 2*24 + 2 the double rounding is innocuous (Figueroa 1995), so the result is the
+// correctly rounded binary32 value that the reference's CPU code produces.
+__device__ __forceinline__ float rn_div(float a, float b) { return (float)__ddiv_rn((double)a, (double)b); }
 __device__ __forceinline__ double rn_div(double a, double b) { return __ddiv_rn(a, b); }
-__device__ __forceinline__ float rn_sqrt(float a) { return __fsqrt_rn(a); }
+__device__ __forceinline__ float rn_sqrt(float a) { return (float)__dsqrt_rn((double)a); }
 __device__ __forceinline__ double rn_sqrt(double a) { return __dsqrt_rn(a); }
 
 __device__ __forceinline__ float dev_fabs(float a) { return __builtin_fabsf(a); }

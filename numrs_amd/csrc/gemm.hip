@@ -15,7 +15,8 @@
 // time for A/B reuse.
 //
 // MFMA operand layout (16x16x4, f64 and f32): lane l supplies A[l%16][l/16] and B[l/16][l%16];
-// accumulator element r of lane l is D[4*(l/16) + r][l%16].
+// accumulator element r of lane l is D[4*(l/16) + r][l%16] for the f32 instruction and
+// D[l/16 + 4*r][l%16] for the f64 one (Mfma<T>::row below).
 #include "ops.hpp"
 
 namespace nrk {
@@ -28,6 +29,7 @@ struct Mfma<double> {
     static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4 * r; }
 };
 template <>
 struct Mfma<float> {
@@ -35,6 +37,7 @@ struct Mfma<float> {
     static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
 constexpr int XCDS = 8;
@@ -145,7 +148,7 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_kernel(int M,
             const int col = bn0 + wn * WN + j * 16 + (lane & 15);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int row = bm0 + wm * WM + i * 16 + 4 * (lane >> 4) + r;
+                const int row = bm0 + wm * WM + i * 16 + Mfma<T>::row(lane, r);
                 if (row < M && col < N) C[(size_t)row * N + col] = acc[i][j][r];
             }
         }

@@ -20,6 +20,17 @@ def test_scalar_chain_bit_exact(gpu, oracle, dtype, n):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("step", [("add", 0.1), ("sub", 0.3), ("mul", 1.1), ("div", 3.0), ("rsub", 2.0), ("rdiv", 1.0),
+                                  ("neg",), ("abs",), ("square",), ("sqrt",)])
+def test_each_scalar_op_correctly_rounded(gpu, oracle, dtype, step):
+    """Every op code alone, bit-exact against the CPU's IEEE result (division and square root
+    included: the CPU rounds them correctly, so the GPU must too)."""
+    x = np.abs(rng.normal(rng.BASE_SEED + 950, 200_003, dtype)) + dtype(0.01)
+    y = K.unary_dispatch(x, [step])
+    np.testing.assert_array_equal(y, oracle.ref_map([K.SOP[step[0]]], [step[1] if len(step) > 1 else 0], x))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("op", list(K.BOP))
 def test_zip_bit_exact(gpu, oracle, dtype, op):
     n = 300_007

@@ -87,16 +87,29 @@ def test_ozaki_integer_inputs_exact(gpu):
     np.testing.assert_array_equal(K.gemm_f64_ozaki(a, b), a @ b)
 
 
-def test_ozaki_fewer_moduli_is_coarser(gpu, oracle):
-    """Fewer moduli -> smaller P -> larger (but still bounded) error: e <= 2^-(P-1) with the P the
-    library derives (14 moduli, k = 256: P = floor((log2 M - 2 - 8) / 2) = 49)."""
+def test_ozaki_error_within_truncation_bound(gpu, oracle):
+    """The only error is rounding A's rows / B's columns to P-bit integers: |dA_ik| <= 2^-P
+    rowmax_i and |dB_kj| <= 2^-P colmax_j, so |C - C*|_ij <= 2^-P (rowmax_i sum_k |B_kj| +
+    colmax_j sum_k |A_ik|) + O(2^-2P). P from the library's rule: the largest value with
+    k 2^(2P) < M/2 for M the product of the first N moduli (14 moduli, k = 256: P = 49)."""
+    import math
+    moduli = [255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197, 193, 191]
     m = k = n = 256
     a = rng.scaled_pow2(5, m * k).reshape(m, k)
     b = rng.scaled_pow2(6, k * n).reshape(k, n)
-    exact, scale = full_refs(oracle, a, b)
-    e14 = rel_err(K.gemm_f64_ozaki(a, b, 14), exact, scale).max()
-    e16 = rel_err(K.gemm_f64_ozaki(a, b, 16), exact, scale).max()
-    assert e16 <= e14 <= 2.0 ** -48
+    exact, _ = full_refs(oracle, a, b)
+    rowmax = np.abs(a).max(axis=1)[:, None]
+    colmax = np.abs(b).max(axis=0)[None, :]
+    bound_unit = rowmax * np.abs(b).sum(axis=0)[None, :] + colmax * np.abs(a).sum(axis=1)[:, None]
+    errs = {}
+    for nm in (14, 16):
+        M = math.prod(moduli[:nm])
+        P = min(53, int(math.floor((math.log2(M) - 2 - math.log2(k)) / 2)))
+        err = np.abs(K.gemm_f64_ozaki(a, b, nm) - exact)
+        # + 1 ulp of |exact| for the final rounding of the exact integer product
+        assert np.all(err <= 2.0 ** -P * bound_unit * 1.001 + np.abs(exact) * 2.0 ** -52), nm
+        errs[nm] = err.max()
+    assert errs[16] <= errs[14]
 
 
 def test_ozaki_nonfinite_rows_are_nan(gpu):
