@@ -7,8 +7,8 @@
 // Plan (host, once per batch shape): every GEMM is cut into BM x BN output tiles; every row sum
 // becomes either a member of a "short-row group" (rows of <= SHORT_MAX elements packed until the
 // group holds ~GROUP_ELEMS elements; each wavefront of the workgroup sums whole rows) or, for long
-// rows, fixed LONG_CHUNK-element chunks whose partials are combined by the chunk that finishes
-// last (per-row acq_rel counter, fixed combine order => deterministic). Items are sorted by
+// rows, one workgroup per row (its 4 wavefronts take fixed quarters, combined in wave order, so
+// nothing crosses workgroups and results are deterministic). Items are sorted by
 // estimated cost, largest first (LPT), using the cost model the caller provides (or the default
 // flop/byte estimate), and uploaded with the GEMM descriptors.
 //
@@ -223,41 +223,27 @@ __device__ void rows_item(const GroupedArgs &g, int r0, int r1) {
     }
 }
 
-__device__ void chunk_item(const GroupedArgs &g, int item, int row, int chunk, int nchunks, double *sh) {
+// A long row is summed whole by one workgroup: the 4 wavefronts take fixed contiguous quarters
+// and their double-word results are combined in wave order. No partial ever leaves the
+// workgroup, so no cross-workgroup synchronisation (and no agent-scope fence) is needed.
+__device__ void long_row_item(const GroupedArgs &g, int row, double *sh) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t off = g.rows_off[row], len = g.rows_off[row + 1] - off;
-    const int64_t c0 = (int64_t)chunk * LONG_CHUNK;
-    const int64_t clen = min((int64_t)LONG_CHUNK, len - c0);
-    // 4 waves each take a quarter of the chunk
-    const int64_t q = (clen + 3) / 4;
-    const int64_t s0 = min(clen, wave * q), s1 = min(clen, s0 + q);
+    const int64_t q = ((len + 3) / 4 + 63) / 64 * 64;  // quarter, rounded to whole wave strides
+    const int64_t s0 = min(len, wave * q), s1 = min(len, s0 + q);
     float hi, lo;
-    wave_row_sum(g.rows_x + off + c0 + s0, s1 - s0, hi, lo);
+    wave_row_sum(g.rows_x + off + s0, s1 - s0, hi, lo);
     __syncthreads();
     if (lane == 0) {
         sh[2 * wave] = (double)hi;
         sh[2 * wave + 1] = (double)lo;
     }
     __syncthreads();
-    const int64_t pidx = g.chunk_base[row] + chunk;
-    __shared__ int s_last;
     if (threadIdx.x == 0) {
         double t = 0.0;
         for (int w = 0; w < GB / 64; ++w) t += sh[2 * w] + sh[2 * w + 1];
-        g.chunk_hi[pidx] = t;
-        __threadfence();
-        const unsigned old = __hip_atomic_fetch_add(&g.row_counters[row], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        s_last = old == (unsigned)(nchunks - 1);
-        if (s_last) {
-            __hip_atomic_store(&g.row_counters[row], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __threadfence();
-            double tot = 0.0;
-            for (int c = 0; c < nchunks; ++c) tot += __hip_atomic_load(&g.chunk_hi[g.chunk_base[row] + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            g.rows_out[row] = (float)tot;
-        }
+        g.rows_out[row] = (float)t;
     }
-    __syncthreads();
-    (void)item;
 }
 
 __global__ __launch_bounds__(GB) void grouped_kernel(GroupedArgs g) {
@@ -278,11 +264,13 @@ __global__ __launch_bounds__(GB) void grouped_kernel(GroupedArgs g) {
         } else if (w.kind == ITEM_ROWS) {
             rows_item(g, w.a, w.b);
         } else {
-            chunk_item(g, it, w.a, w.b, w.c, sh);
+            long_row_item(g, w.a, sh);
         }
     }
     if (threadIdx.x == 0) {
-        const unsigned done = __hip_atomic_fetch_add(&g.queue[1], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        // Relaxed is enough: this workgroup's last queue fetch has returned (its value decided the
+        // loop exit) before this increment is issued, so every fetch precedes the reset below.
+        const unsigned done = __hip_atomic_fetch_add(&g.queue[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (done == gridDim.x - 1) {  // every workgroup has made its final fetch: reset for replay
             __hip_atomic_store(&g.queue[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_store(&g.queue[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -386,7 +374,7 @@ nrk_status nrk_grouped_plan_create(int32_t gemm_count, const nrk_gemm_desc *desc
         }
     }
     p->n_gemm_items = (int)items.size();
-    // rows: short rows grouped, long rows chunked
+    // rows: short rows grouped, long rows one workgroup each
     std::vector<int64_t> chunk_base(row_count, -1);
     int64_t n_chunks = 0;
     int r = 0;
@@ -397,13 +385,8 @@ nrk_status nrk_grouped_plan_create(int32_t gemm_count, const nrk_gemm_desc *desc
             return fail(NRK_ERR_INVALID_ARGUMENT, "row_offsets decreases at row %d", r);
         }
         if (len > SHORT_MAX) {
-            const int nc = (int)((len + LONG_CHUNK - 1) / LONG_CHUNK);
-            chunk_base[r] = n_chunks;
-            for (int c = 0; c < nc; ++c) {
-                items.push_back(Item{ITEM_CHUNK, r, c, nc});
-                cost.push_back(300.0 + std::min<int64_t>(LONG_CHUNK, len - (int64_t)c * LONG_CHUNK) * 0.6);
-            }
-            n_chunks += nc;
+            items.push_back(Item{ITEM_CHUNK, r, 0, 1});
+            cost.push_back(300.0 + (double)len * 0.15);
             ++r;
         } else {
             int r1 = r;

@@ -59,8 +59,6 @@ struct SegParams {
     double ddof;
     int vec_ok;
     int want_row, want_full, need_partials;
-    unsigned int *row_counters;  // rows entries (only when spr > 1 && want_row)
-    unsigned int *full_counter;
     // partial records, structure of arrays, nseg entries each
     double *p_hi, *p_lo, *p_q, *p_center, *p_max, *p_min;
     uint64_t *p_imax, *p_imin;
@@ -390,38 +388,6 @@ __device__ __forceinline__ void reduce_segment(const SegParams<T> &p, size_t seg
         p.p_imin[seg] = imin;
         p.p_flags[seg] = (unsigned)nan | ((unsigned)first_nan << 1);
     }
-    // ---- row finish (rows spanning several segments): the last segment of the row does it
-    if (p.spr > 1 && p.want_row) {
-        __syncthreads();
-        if (t == 0) {
-            __threadfence();
-            unsigned old = __hip_atomic_fetch_add(&p.row_counters[r], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-            int last = (old == (unsigned)(p.spr - 1));
-            if (last) __hip_atomic_store(&p.row_counters[r], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            sh->last = last;
-        }
-        __syncthreads();
-        if (sh->last) {
-            __threadfence();
-            finish_range<T, FLAGS, SEG>(p, r * p.spr, (r + 1) * p.spr, false, p.row, r, &sh->f);
-        }
-    }
-    // ---- whole-matrix finish: the workgroup that retires the last segment does it
-    if (p.want_full) {
-        __syncthreads();
-        if (t == 0) {
-            __threadfence();
-            unsigned old = __hip_atomic_fetch_add(p.full_counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-            int last = (old == (unsigned)(p.nseg - 1));
-            if (last) __hip_atomic_store(p.full_counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            sh->last = last;
-        }
-        __syncthreads();
-        if (sh->last) {
-            __threadfence();
-            finish_range<T, FLAGS, SEG>(p, 0, p.nseg, true, p.full, 0, &sh->f);
-        }
-    }
 }
 
 template <typename T, int VPT, int FLAGS>
@@ -436,6 +402,23 @@ __global__ __launch_bounds__(RB) void seg_reduce_kernel(SegParams<T> p) {
         } else {
             reduce_segment<T, VPT, FLAGS, false>(p, seg, parity, &sh);
         }
+    }
+}
+
+// Cross-segment finish, launched on the same stream right after the streaming kernel (the
+// kernel boundary makes every partial visible; no fence or atomic is needed inside the stream).
+// Workgroup b < row_blocks finishes row b (rows of more than one segment); the last workgroup
+// finishes the whole-matrix result. On gfx950 a per-workgroup agent-scope release/acquire
+// (L2 write-back + invalidate on each XCD) costs far more than this extra launch.
+template <typename T, int VPT, int FLAGS>
+__global__ __launch_bounds__(RB) void seg_finish_kernel(SegParams<T> p, int row_blocks) {
+    constexpr int SEG = RB * VPT * Vec16<T>::N;
+    __shared__ FinishSlots fs;
+    const size_t b = blockIdx.x;
+    if ((int)b < row_blocks) {
+        finish_range<T, FLAGS, SEG>(p, b * p.spr, (b + 1) * p.spr, false, p.row, b, &fs);
+    } else {
+        finish_range<T, FLAGS, SEG>(p, 0, p.nseg, true, p.full, 0, &fs);
     }
 }
 
@@ -478,6 +461,12 @@ static nrk_status launch_flags(SegParams<T> &p, hipStream_t stream) {
     int grid = (int)std::min<size_t>(p.nseg, (size_t)cu_count() * reduce_blocks_per_cu());
     hipLaunchKernelGGL((seg_reduce_kernel<T, VPT, FLAGS>), dim3(grid), dim3(RB), 0, stream, p);
     NRK_HIP(hipGetLastError());
+    const int row_blocks = (p.spr > 1 && p.want_row) ? (int)p.rows : 0;
+    const int fin = row_blocks + (p.want_full ? 1 : 0);
+    if (fin > 0) {
+        hipLaunchKernelGGL((seg_finish_kernel<T, VPT, FLAGS>), dim3(fin), dim3(RB), 0, stream, p, row_blocks);
+        NRK_HIP(hipGetLastError());
+    }
     return NRK_OK;
 }
 
@@ -522,13 +511,10 @@ nrk_status seg_reduce(const Chain &chain, const T *x, size_t rows, size_t cols, 
     if (p.nseg >= (size_t)UINT32_MAX) return fail(NRK_ERR_INVALID_ARGUMENT, "too many segments (%zu)", p.nseg);
     p.vec_ok = ((reinterpret_cast<uintptr_t>(x) % 16) == 0) && (cols % Vec16<T>::N == 0);
     p.need_partials = p.want_full || (p.spr > 1 && p.want_row);
-    const size_t n_counters = 1 + ((p.spr > 1 && p.want_row) ? rows : 0);
     const size_t rec = 8 * 8 + 4;
     const size_t pbytes = p.need_partials ? align_up(p.nseg * rec + 64 * 9, 256) : 0;
     Scratch s;
-    NRK_TRY(get_scratch(stream, n_counters, pbytes, &s));
-    p.full_counter = s.counters;
-    p.row_counters = s.counters + 1;
+    NRK_TRY(get_scratch(stream, 0, pbytes, &s));
     if (p.need_partials) {
         char *b = (char *)s.partials;
         size_t off = 0;
@@ -573,12 +559,11 @@ constexpr int I64_VPT = 8;                     // 16-byte vectors per thread per
 constexpr int I64_SEG = RB * I64_VPT * 2;      // 4096 elements = 32 KiB
 
 __global__ __launch_bounds__(RB) void sum_i64_kernel(const int64_t *x, size_t n, size_t nseg, int vec_ok,
-                                                     unsigned long long *partials, unsigned int *counter,
-                                                     int64_t *out) {
+                                                     unsigned long long *partials) {
     typedef long long V2 __attribute__((ext_vector_type(2)));
-    __shared__ unsigned long long sred[WAVES];
-    __shared__ int slast;
+    __shared__ unsigned long long sred[2][WAVES];
     const int t = threadIdx.x;
+    int parity = 0;
     for (size_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
         const size_t off = seg * I64_SEG;
         unsigned long long acc = 0;  // unsigned: wrapping is defined, bits equal two's complement i64
@@ -597,36 +582,32 @@ __global__ __launch_bounds__(RB) void sum_i64_kernel(const int64_t *x, size_t n,
         }
 #pragma unroll
         for (int m = 32; m >= 1; m >>= 1) acc += shfl_xor((uint64_t)acc, m);
-        if ((t & 63) == 0) sred[t >> 6] = acc;
+        if ((t & 63) == 0) sred[parity][t >> 6] = acc;
         __syncthreads();
         if (t == 0) {
             unsigned long long s = 0;
-            for (int k = 0; k < WAVES; ++k) s += sred[k];
+            for (int k = 0; k < WAVES; ++k) s += sred[parity][k];
             partials[seg] = s;
-            __threadfence();
-            unsigned old = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-            int last = old == (unsigned)(nseg - 1);
-            if (last) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            slast = last;
         }
-        __syncthreads();
-        const int last = slast;
-        __syncthreads();
-        if (last) {
-            __threadfence();
-            unsigned long long s = 0;
-            for (size_t k = t; k < nseg; k += RB) s += partials[k];
+        parity ^= 1;  // the next segment writes the other slot set: no second barrier needed
+    }
+}
+
+// One workgroup adds the per-segment partials (integer adds: exact in any order).
+__global__ __launch_bounds__(RB) void sum_u64_finish_kernel(const unsigned long long *partials, size_t nseg,
+                                                            int64_t *out) {
+    __shared__ unsigned long long sred[WAVES];
+    const int t = threadIdx.x;
+    unsigned long long s = 0;
+    for (size_t k = t; k < nseg; k += RB) s += partials[k];
 #pragma unroll
-            for (int m = 32; m >= 1; m >>= 1) s += shfl_xor((uint64_t)s, m);
-            if ((t & 63) == 0) sred[t >> 6] = s;
-            __syncthreads();
-            if (t == 0) {
-                unsigned long long tot = 0;
-                for (int k = 0; k < WAVES; ++k) tot += sred[k];
-                *out = (int64_t)tot;
-            }
-            __syncthreads();
-        }
+    for (int m = 32; m >= 1; m >>= 1) s += shfl_xor((uint64_t)s, m);
+    if ((t & 63) == 0) sred[t >> 6] = s;
+    __syncthreads();
+    if (t == 0) {
+        unsigned long long tot = 0;
+        for (int k = 0; k < WAVES; ++k) tot += sred[k];
+        *out = (int64_t)tot;
     }
 }
 
@@ -643,24 +624,24 @@ nrk_status sum_i64(const int64_t *x, size_t n, int64_t *out, hipStream_t stream)
     const size_t nseg = ceil_div(n, I64_SEG);
     if (nseg >= (size_t)UINT32_MAX) return fail(NRK_ERR_INVALID_ARGUMENT, "input too large");
     Scratch s;
-    NRK_TRY(get_scratch(stream, 1, nseg * 8, &s));
+    NRK_TRY(get_scratch(stream, 0, nseg * 8, &s));
     const int vec_ok = (reinterpret_cast<uintptr_t>(x) % 16) == 0;
-    int grid = (int)std::min<size_t>(nseg, (size_t)cu_count() * 8);
-    hipLaunchKernelGGL(sum_i64_kernel, dim3(grid), dim3(RB), 0, stream, x, n, nseg, vec_ok,
-                       (unsigned long long *)s.partials, s.counters, out);
+    int grid = (int)std::min<size_t>(nseg, (size_t)cu_count() * reduce_blocks_per_cu());
+    hipLaunchKernelGGL(sum_i64_kernel, dim3(grid), dim3(RB), 0, stream, x, n, nseg, vec_ok, (unsigned long long *)s.partials);
+    NRK_HIP(hipGetLastError());
+    hipLaunchKernelGGL(sum_u64_finish_kernel, dim3(1), dim3(RB), 0, stream, (const unsigned long long *)s.partials, nseg, out);
     NRK_HIP(hipGetLastError());
     return NRK_OK;
 }
 
 // ---------------- product: fixed-tree product in T ----------------
 template <typename T>
-__global__ __launch_bounds__(RB) void prod_kernel(const T *x, size_t n, size_t nseg, T *partials, unsigned int *counter,
-                                                   T *out) {
+__global__ __launch_bounds__(RB) void prod_kernel(const T *x, size_t n, size_t nseg, T *partials) {
     constexpr int PER = 16;
-    __shared__ T sred[WAVES];
-    __shared__ int slast;
+    __shared__ T sred[2][WAVES];
     const int t = threadIdx.x;
-    for (size_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
+    int parity = 0;
+    for (size_t seg = blockIdx.x; seg < nseg; seg += gridDim.x, parity ^= 1) {
         const size_t off = seg * (size_t)(RB * PER);
         T acc = (T)1;
         for (int j = 0; j < PER; ++j) {
@@ -669,36 +650,31 @@ __global__ __launch_bounds__(RB) void prod_kernel(const T *x, size_t n, size_t n
         }
 #pragma unroll
         for (int m = 32; m >= 1; m >>= 1) acc = rn_mul(acc, shfl_xor(acc, m));
-        if ((t & 63) == 0) sred[t >> 6] = acc;
+        if ((t & 63) == 0) sred[parity][t >> 6] = acc;
         __syncthreads();
         if (t == 0) {
-            T s = sred[0];
-            for (int k = 1; k < WAVES; ++k) s = rn_mul(s, sred[k]);
+            T s = sred[parity][0];
+            for (int k = 1; k < WAVES; ++k) s = rn_mul(s, sred[parity][k]);
             partials[seg] = s;
-            __threadfence();
-            unsigned old = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-            int last = old == (unsigned)(nseg - 1);
-            if (last) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            slast = last;
         }
-        __syncthreads();
-        const int last = slast;
-        __syncthreads();
-        if (last) {
-            __threadfence();
-            T s = (T)1;
-            for (size_t k = t; k < nseg; k += RB) s = rn_mul(s, partials[k]);
+    }
+}
+
+// One workgroup multiplies the per-segment partials in a fixed tree.
+template <typename T>
+__global__ __launch_bounds__(RB) void prod_finish_kernel(const T *partials, size_t nseg, T *out) {
+    __shared__ T sred[WAVES];
+    const int t = threadIdx.x;
+    T s = (T)1;
+    for (size_t k = t; k < nseg; k += RB) s = rn_mul(s, partials[k]);
 #pragma unroll
-            for (int m = 32; m >= 1; m >>= 1) s = rn_mul(s, shfl_xor(s, m));
-            if ((t & 63) == 0) sred[t >> 6] = s;
-            __syncthreads();
-            if (t == 0) {
-                T tot = sred[0];
-                for (int k = 1; k < WAVES; ++k) tot = rn_mul(tot, sred[k]);
-                *out = tot;
-            }
-            __syncthreads();
-        }
+    for (int m = 32; m >= 1; m >>= 1) s = rn_mul(s, shfl_xor(s, m));
+    if ((t & 63) == 0) sred[t >> 6] = s;
+    __syncthreads();
+    if (t == 0) {
+        T tot = sred[0];
+        for (int k = 1; k < WAVES; ++k) tot = rn_mul(tot, sred[k]);
+        *out = tot;
     }
 }
 
@@ -716,9 +692,11 @@ nrk_status prod(const T *x, size_t n, T *out, hipStream_t stream) {
     if (!x) return fail(NRK_ERR_INVALID_ARGUMENT, "x is NULL");
     const size_t nseg = ceil_div(n, (size_t)RB * 16);
     Scratch s;
-    NRK_TRY(get_scratch(stream, 1, nseg * sizeof(T), &s));
+    NRK_TRY(get_scratch(stream, 0, nseg * sizeof(T), &s));
     int grid = (int)std::min<size_t>(nseg, (size_t)cu_count() * 8);
-    hipLaunchKernelGGL(prod_kernel<T>, dim3(grid), dim3(RB), 0, stream, x, n, nseg, (T *)s.partials, s.counters, out);
+    hipLaunchKernelGGL(prod_kernel<T>, dim3(grid), dim3(RB), 0, stream, x, n, nseg, (T *)s.partials);
+    NRK_HIP(hipGetLastError());
+    hipLaunchKernelGGL(prod_finish_kernel<T>, dim3(1), dim3(RB), 0, stream, (const T *)s.partials, nseg, out);
     NRK_HIP(hipGetLastError());
     return NRK_OK;
 }
