@@ -107,11 +107,31 @@ __device__ __forceinline__ void take_min(T &v, uint64_t &i, T bv, uint64_t bi) {
 // segment's writes never race with a slower wave still reading this segment's values.
 template <typename T>
 struct WaveSlots {
-    T hi[WAVES], lo[WAVES];
+    double hi[WAVES], lo[WAVES], m2[WAVES];
+    int cnt[WAVES];
     T vmax[WAVES], vmin[WAVES];
     uint64_t imax[WAVES], imin[WAVES];
     int nan[WAVES];
-    double q[WAVES];
+};
+
+// Running statistics of a set of elements: count, double-word sum, and M2 = sum of squared
+// deviations from the set's own mean. merge() is Chan, Golub & LeVeque's pairwise update
+// (1979): M2 = M2a + M2b + (na nb / n) (mean_a - mean_b)^2, evaluated in FP64.
+struct Moments {
+    int n;
+    double hi, lo, m2;
+    __device__ __forceinline__ void merge(int nb, double bhi, double blo, double bm2) {
+        if (nb == 0) return;
+        if (n == 0) {
+            n = nb; hi = bhi; lo = blo; m2 = bm2;
+            return;
+        }
+        const double na = (double)n, nbd = (double)nb, nt = na + nbd;
+        const double d = (hi + lo) / na - (bhi + blo) / nbd;
+        m2 = m2 + bm2 + d * d * (na * nbd / nt);
+        dw_add(hi, lo, bhi, blo);
+        n += nb;
+    }
 };
 struct FinishSlots {
     double hi[WAVES], lo[WAVES], q[WAVES], vmax[WAVES], vmin[WAVES];
@@ -243,7 +263,7 @@ __device__ __noinline__ void finish_range(const SegParams<T> &p, size_t lo, size
     __syncthreads();
 }
 
-template <typename T, int VPT, int FLAGS, bool FULLSEG>
+template <typename T, int VPT, int FLAGS, bool FULLSEG, bool VEC>
 __device__ __forceinline__ void reduce_segment(const SegParams<T> &p, size_t seg, int parity, BlockShared<T> *sh) {
     typedef typename Vec16<T>::type V;
     constexpr int VN = Vec16<T>::N;
@@ -259,13 +279,20 @@ __device__ __forceinline__ void reduce_segment(const SegParams<T> &p, size_t seg
     // ---- load: element i = j*VN + e of this thread sits at segment position (j*RB + t)*VN + e
     T y[M];
     unsigned vmask = ~0u;
-    if (FULLSEG && p.vec_ok) {
+    if (VEC) {
+        // aligned rows: 16-byte loads; a tail segment predicates whole vectors (len % VN == 0)
         const V *bv = reinterpret_cast<const V *>(base);
+        const unsigned nvec = len / VN;
+        if (!FULLSEG) vmask = 0u;
 #pragma unroll
         for (int j = 0; j < VPT; ++j) {
-            V v = __builtin_nontemporal_load(bv + (j * RB + t));
+            const unsigned jv = j * RB + t;
+            const bool ok = FULLSEG || jv < nvec;
+            V v = V{};
+            if (ok) v = __builtin_nontemporal_load(bv + jv);
 #pragma unroll
             for (int e = 0; e < VN; ++e) y[j * VN + e] = v[e];
+            if (!FULLSEG) vmask |= (ok ? ((1u << VN) - 1u) : 0u) << (j * VN);
         }
     } else {
         if (!FULLSEG) vmask = 0u;
@@ -291,13 +318,26 @@ __device__ __forceinline__ void reduce_segment(const SegParams<T> &p, size_t seg
         for (int i = 0; i < M; ++i) y[i] = apply_sop<T>(op, s, y[i]);
     }
 
-    // ---- phase 1: sum, max/argmax, min/argmin, NaN flag
-    T hi = pairwise<M, 0, T>([&](int i) { return valid(i) ? y[i] : (T)0; });
-    T lo = (T)0;
+    // ---- per-thread statistics straight from registers; nothing here waits for other threads:
+    // count, pairwise sum, M2 about the thread's own mean, extremum winners, NaN flag.
+    const int nt = FULLSEG ? M : __builtin_popcount(vmask);
+    const T st = pairwise<M, 0, T>([&](int i) { return valid(i) ? y[i] : (T)0; });
     int nan = 0;
-    if (__builtin_isnan(hi)) {  // rare: tell a NaN element apart from inf - inf
+    if (__builtin_isnan(st)) {  // rare: tell a NaN element apart from inf - inf
 #pragma unroll
         for (int i = 0; i < M; ++i) nan |= (valid(i) && __builtin_isnan(y[i])) ? 1 : 0;
+    }
+    double m2t = 0.0;
+    if ((FLAGS & F_M2) && nt > 0) {
+        // mean of M elements: an exact power-of-two scaling for full segments
+        const T mt = FULLSEG ? rn_mul(st, (T)(1.0 / M)) : rn_div(st, (T)nt);
+        const T qt = pairwise<M, 0, T>([&](int i) {
+            const T d = valid(i) ? rn_sub(y[i], mt) : (T)0;
+            return rn_mul(d, d);
+        });
+        // qt is taken about the rounded mean mt; shift it to the mean st/nt: Q = M2 + n (mt - mean)^2
+        const double dm = (double)mt - (double)st / (double)nt;
+        m2t = fmax((double)qt - (double)nt * dm * dm, 0.0);
     }
     T vmax = -dev_inf<T>(), vmin = dev_inf<T>();
     uint64_t imax = NO_INDEX, imin = NO_INDEX;
@@ -329,68 +369,67 @@ __device__ __forceinline__ void reduce_segment(const SegParams<T> &p, size_t seg
             imin = col0 + pos_of(first);
         }
     }
-    wave_dw_sum(hi, lo);
+    const int first_nan = (c == 0 && t == 0 && __builtin_isnan(y[0])) ? 1 : 0;
+
+    // ---- wavefront combine (fixed xor butterfly; lane 0's result is the one used)
+    Moments mo{nt, (double)st, 0.0, m2t};
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        const int bn = __shfl_xor(mo.n, m, 64);
+        const double bhi = shfl_xor(mo.hi, m), blo = shfl_xor(mo.lo, m);
+        const double bm2 = (FLAGS & F_M2) ? shfl_xor(mo.m2, m) : 0.0;
+        if (FLAGS & F_M2) {
+            mo.merge(bn, bhi, blo, bm2);
+        } else {
+            dw_add(mo.hi, mo.lo, bhi, blo);
+            mo.n += bn;
+        }
+    }
     nan = __any(nan) ? 1 : 0;
     if (FLAGS & F_MAX) wave_arg<T, true>(vmax, imax);
     if (FLAGS & F_MIN) wave_arg<T, false>(vmin, imin);
     WaveSlots<T> &ws = sh->w[parity];
     if (lane == 0) {
-        ws.hi[w] = hi; ws.lo[w] = lo; ws.nan[w] = nan;
+        ws.hi[w] = mo.hi; ws.lo[w] = mo.lo; ws.m2[w] = mo.m2; ws.cnt[w] = mo.n; ws.nan[w] = nan;
         if (FLAGS & F_MAX) { ws.vmax[w] = vmax; ws.imax[w] = imax; }
         if (FLAGS & F_MIN) { ws.vmin[w] = vmin; ws.imin[w] = imin; }
     }
-    __syncthreads();
-    hi = ws.hi[0]; lo = ws.lo[0]; nan = ws.nan[0];
+    __syncthreads();  // the only barrier of the segment
+    if (t != 0) return;
+    // ---- workgroup combine in wave order, by thread 0
+    Moments tot{ws.cnt[0], ws.hi[0], ws.lo[0], ws.m2[0]};
+    nan = ws.nan[0];
     if (FLAGS & F_MAX) { vmax = ws.vmax[0]; imax = ws.imax[0]; }
     if (FLAGS & F_MIN) { vmin = ws.vmin[0]; imin = ws.imin[0]; }
 #pragma unroll
     for (int k = 1; k < WAVES; ++k) {
-        dw_add(hi, lo, ws.hi[k], ws.lo[k]);
+        if (FLAGS & F_M2) {
+            tot.merge(ws.cnt[k], ws.hi[k], ws.lo[k], ws.m2[k]);
+        } else {
+            dw_add(tot.hi, tot.lo, ws.hi[k], ws.lo[k]);
+            tot.n += ws.cnt[k];
+        }
         nan |= ws.nan[k];
         if (FLAGS & F_MAX) take_max(vmax, imax, ws.vmax[k], ws.imax[k]);
         if (FLAGS & F_MIN) take_min(vmin, imin, ws.vmin[k], ws.imin[k]);
     }
-    const T center = rn_div(hi, (T)len);
-
-    // ---- phase 2: second moment about `center`, from registers (no second read)
-    double q = 0.0;
-    if (FLAGS & F_M2) {
-        T qt = pairwise<M, 0, T>([&](int i) {
-            const T d = valid(i) ? rn_sub(y[i], center) : (T)0;
-            return rn_mul(d, d);
-        });
-        q = wave_sum_f64((double)qt);
-        if (lane == 0) ws.q[w] = q;
-        __syncthreads();
-        q = ws.q[0];
-#pragma unroll
-        for (int k = 1; k < WAVES; ++k) q += ws.q[k];
-    }
-    const int first_nan = (c == 0 && t == 0 && __builtin_isnan(y[0])) ? 1 : 0;
-
-    if (p.spr == 1 && p.want_row && t == 0) {
+    if (p.spr == 1 && p.want_row) {
         // the whole row is this segment: finalise in place
-        const double s_hi = (double)hi, s_lo = (double)lo;
-        const double mu = (s_hi + s_lo) / (double)len;
-        const double dc = (double)center - mu;
-        const double q_true = fmax(q - (double)len * dc * dc, 0.0);
-        write_outputs<T>(p.row, r, s_hi, s_lo, q_true, (double)len, p.ddof, nan, first_nan, vmax, imax, vmin, imin);
+        write_outputs<T>(p.row, r, tot.hi, tot.lo, tot.m2, (double)len, p.ddof, nan, first_nan, vmax, imax, vmin, imin);
     }
     if (!p.need_partials) return;
-    if (t == 0) {
-        p.p_hi[seg] = (double)hi;
-        p.p_lo[seg] = (double)lo;
-        p.p_q[seg] = q;
-        p.p_center[seg] = (double)center;
-        p.p_max[seg] = (double)vmax;
-        p.p_min[seg] = (double)vmin;
-        p.p_imax[seg] = imax;
-        p.p_imin[seg] = imin;
-        p.p_flags[seg] = (unsigned)nan | ((unsigned)first_nan << 1);
-    }
+    p.p_hi[seg] = tot.hi;
+    p.p_lo[seg] = tot.lo;
+    p.p_q[seg] = tot.m2;                              // about the segment's own mean ...
+    p.p_center[seg] = (tot.hi + tot.lo) / (double)len;  // ... which is this
+    p.p_max[seg] = (double)vmax;
+    p.p_min[seg] = (double)vmin;
+    p.p_imax[seg] = imax;
+    p.p_imin[seg] = imin;
+    p.p_flags[seg] = (unsigned)nan | ((unsigned)first_nan << 1);
 }
 
-template <typename T, int VPT, int FLAGS>
+template <typename T, int VPT, int FLAGS, bool VEC>
 __global__ __launch_bounds__(RB) void seg_reduce_kernel(SegParams<T> p) {
     constexpr int SEG = RB * VPT * Vec16<T>::N;
     __shared__ BlockShared<T> sh;
@@ -398,9 +437,9 @@ __global__ __launch_bounds__(RB) void seg_reduce_kernel(SegParams<T> p) {
     for (size_t seg = blockIdx.x; seg < p.nseg; seg += gridDim.x, parity ^= 1) {
         const size_t c = seg % p.spr;
         if (c * SEG + SEG <= p.cols) {
-            reduce_segment<T, VPT, FLAGS, true>(p, seg, parity, &sh);
+            reduce_segment<T, VPT, FLAGS, true, VEC>(p, seg, parity, &sh);
         } else {
-            reduce_segment<T, VPT, FLAGS, false>(p, seg, parity, &sh);
+            reduce_segment<T, VPT, FLAGS, false, VEC>(p, seg, parity, &sh);
         }
     }
 }
@@ -459,7 +498,10 @@ static bool any_out(const OutPtrs<T> &o) {
 template <typename T, int VPT, int FLAGS>
 static nrk_status launch_flags(SegParams<T> &p, hipStream_t stream) {
     int grid = (int)std::min<size_t>(p.nseg, (size_t)cu_count() * reduce_blocks_per_cu());
-    hipLaunchKernelGGL((seg_reduce_kernel<T, VPT, FLAGS>), dim3(grid), dim3(RB), 0, stream, p);
+    if (p.vec_ok)
+        hipLaunchKernelGGL((seg_reduce_kernel<T, VPT, FLAGS, true>), dim3(grid), dim3(RB), 0, stream, p);
+    else
+        hipLaunchKernelGGL((seg_reduce_kernel<T, VPT, FLAGS, false>), dim3(grid), dim3(RB), 0, stream, p);
     NRK_HIP(hipGetLastError());
     const int row_blocks = (p.spr > 1 && p.want_row) ? (int)p.rows : 0;
     const int fin = row_blocks + (p.want_full ? 1 : 0);
