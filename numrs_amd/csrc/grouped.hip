@@ -12,9 +12,9 @@
 // estimated cost, largest first (LPT), using the cost model the caller provides (or the default
 // flop/byte estimate), and uploaded with the GEMM descriptors.
 //
-// Run: a persistent grid (CUs x blocks/CU) pulls items from a device queue counter; the last
-// workgroup to leave resets the counters, so a plan can be replayed -- e.g. from a captured
-// hipGraph -- with no host work and no memset nodes.
+// Run: a persistent grid (CUs x blocks/CU) deals the sorted items in a fixed snake order
+// (no shared queue state), so a plan can be replayed -- e.g. from a captured hipGraph -- with no
+// host work and no memset nodes.
 //
 // GEMM tile: 64 x 64 outputs, K-step 32, 4 wavefronts of 32 x 32 (2 x 2 v_mfma_f32_16x16x32_bf16).
 // A (m x k) row-major is staged K-contiguous; B (k x n) row-major is transposed into LDS as
@@ -34,15 +34,14 @@ constexpr int GT_M = 64, GT_N = 64, GT_K = 32;
 constexpr int G_LDK = GT_K + 8;  // padded LDS row (bf16)
 constexpr int SHORT_MAX = 2048;  // rows up to this length are summed whole by one wavefront
 constexpr int GROUP_ELEMS = 8192;
-constexpr int LONG_CHUNK = 8192; // chunk of a long row reduced by one workgroup
 
-enum : int { ITEM_GEMM = 0, ITEM_ROWS = 1, ITEM_CHUNK = 2 };
+enum : int { ITEM_GEMM = 0, ITEM_ROWS = 1, ITEM_LONGROW = 2 };
 
 struct Item {
     int kind;
-    int a;  // GEMM: problem index   ROWS: first row   CHUNK: row
-    int b;  // GEMM: tile index      ROWS: end row     CHUNK: chunk index
-    int c;  // CHUNK: chunks in the row
+    int a;  // GEMM: problem index   ROWS: first row   LONGROW: row
+    int b;  // GEMM: tile index      ROWS: end row
+    int c;
 };
 
 struct GroupedArgs {
@@ -52,10 +51,6 @@ struct GroupedArgs {
     const float *rows_x;
     const int64_t *rows_off;
     float *rows_out;
-    double *chunk_hi, *chunk_lo;        // one per chunk item
-    const int64_t *chunk_base;          // index of the row's first chunk partial, per row (or -1)
-    unsigned int *row_counters;         // per row (long rows only), zero between launches
-    unsigned int *queue;                // [0] = next item, [1] = workgroups exited
 };
 
 // ---------------- GEMM tile ----------------
@@ -250,13 +245,13 @@ __global__ __launch_bounds__(GB) void grouped_kernel(GroupedArgs g) {
     __shared__ __attribute__((aligned(16))) unsigned short As[GT_M * G_LDK];
     __shared__ __attribute__((aligned(16))) unsigned short Bs[GT_N * G_LDK];
     __shared__ double sh[2 * GB / 64];
-    __shared__ int s_item;
-    for (;;) {
-        if (threadIdx.x == 0)
-            s_item = (int)__hip_atomic_fetch_add(&g.queue[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __syncthreads();
-        const int it = s_item;
-        __syncthreads();
+    // Static "snake" deal of the cost-sorted queue: in round r workgroup b takes item
+    // r*G + b (r even) or r*G + G-1-b (r odd). Every quantity below derives from blockIdx/gridDim
+    // and kernel arguments, so all control flow around the barriers inside the item bodies is
+    // uniform, and there is no shared queue state to reset between replays.
+    const int G = (int)gridDim.x, b = (int)blockIdx.x;
+    for (int round = 0;; ++round) {
+        const int it = round * G + ((round & 1) ? (G - 1 - b) : b);
         if (it >= g.n_items) break;
         const Item w = g.items[it];
         if (w.kind == ITEM_GEMM) {
@@ -266,15 +261,7 @@ __global__ __launch_bounds__(GB) void grouped_kernel(GroupedArgs g) {
         } else {
             long_row_item(g, w.a, sh);
         }
-    }
-    if (threadIdx.x == 0) {
-        // Relaxed is enough: this workgroup's last queue fetch has returned (its value decided the
-        // loop exit) before this increment is issued, so every fetch precedes the reset below.
-        const unsigned done = __hip_atomic_fetch_add(&g.queue[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (done == gridDim.x - 1) {  // every workgroup has made its final fetch: reset for replay
-            __hip_atomic_store(&g.queue[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&g.queue[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
+        __syncthreads();  // the next item may reuse the LDS this one used
     }
 }
 
@@ -322,9 +309,6 @@ struct nrk_grouped_plan {
     int64_t *d_rows_off = nullptr;
     float *rows_out = nullptr;
     int n_rows = 0;
-    double *d_chunk = nullptr;
-    int64_t *d_chunk_base = nullptr;
-    unsigned int *d_counters = nullptr;  // rows + 2 (queue)
     int grid = 0;
 };
 
@@ -336,9 +320,6 @@ void free_plan(nrk_grouped_plan *p) {
     (void)hipFree(p->d_descs);
     (void)hipFree(p->d_items);
     (void)hipFree(p->d_rows_off);
-    (void)hipFree(p->d_chunk);
-    (void)hipFree(p->d_chunk_base);
-    (void)hipFree(p->d_counters);
     delete p;
 }
 }  // namespace
@@ -375,8 +356,6 @@ nrk_status nrk_grouped_plan_create(int32_t gemm_count, const nrk_gemm_desc *desc
     }
     p->n_gemm_items = (int)items.size();
     // rows: short rows grouped, long rows one workgroup each
-    std::vector<int64_t> chunk_base(row_count, -1);
-    int64_t n_chunks = 0;
     int r = 0;
     while (r < row_count) {
         const int64_t len = row_offsets[r + 1] - row_offsets[r];
@@ -385,7 +364,7 @@ nrk_status nrk_grouped_plan_create(int32_t gemm_count, const nrk_gemm_desc *desc
             return fail(NRK_ERR_INVALID_ARGUMENT, "row_offsets decreases at row %d", r);
         }
         if (len > SHORT_MAX) {
-            items.push_back(Item{ITEM_CHUNK, r, 0, 1});
+            items.push_back(Item{ITEM_LONGROW, r, 0, 0});
             cost.push_back(300.0 + (double)len * 0.15);
             ++r;
         } else {
@@ -422,16 +401,9 @@ nrk_status nrk_grouped_plan_create(int32_t gemm_count, const nrk_gemm_desc *desc
     nrk_status st = NRK_OK;
     if ((st = up((void **)&p->d_descs, descs, sizeof(nrk_gemm_desc) * gemm_count)) != NRK_OK ||
         (st = up((void **)&p->d_items, sorted.data(), sizeof(Item) * sorted.size())) != NRK_OK ||
-        (st = up((void **)&p->d_rows_off, row_offsets, row_count ? sizeof(int64_t) * (row_count + 1) : 0)) != NRK_OK ||
-        (st = up((void **)&p->d_chunk_base, chunk_base.data(), sizeof(int64_t) * chunk_base.size())) != NRK_OK) {
+        (st = up((void **)&p->d_rows_off, row_offsets, row_count ? sizeof(int64_t) * (row_count + 1) : 0)) != NRK_OK) {
         free_plan(p);
         return st;
-    }
-    if (hipMalloc(&p->d_chunk, sizeof(double) * (n_chunks ? n_chunks : 1)) != hipSuccess ||
-        hipMalloc(&p->d_counters, sizeof(unsigned) * (row_count + 2)) != hipSuccess ||
-        hipMemset(p->d_counters, 0, sizeof(unsigned) * (row_count + 2)) != hipSuccess) {
-        free_plan(p);
-        return fail(NRK_ERR_OUT_OF_MEMORY, "plan allocation failed");
     }
     p->grid = std::max(1, std::min(p->n_items, cu_count() * 4));
     *out_plan = p;
@@ -448,10 +420,6 @@ nrk_status nrk_grouped_plan_run(const nrk_grouped_plan *p, nrk_stream stream) {
     g.rows_x = p->rows_x;
     g.rows_off = p->d_rows_off;
     g.rows_out = p->rows_out;
-    g.chunk_hi = p->d_chunk;
-    g.chunk_base = p->d_chunk_base;
-    g.row_counters = p->d_counters;
-    g.queue = p->d_counters + p->n_rows;
     hipLaunchKernelGGL(grouped_kernel, dim3(p->grid), dim3(GB), 0, (hipStream_t)stream, g);
     NRK_HIP(hipGetLastError());
     return NRK_OK;
