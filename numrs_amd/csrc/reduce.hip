@@ -1,6 +1,6 @@
 // Streaming reductions for gfx950: sum / mean / var / min / max / argmin / argmax over rows of
 // a row-major matrix and over the whole matrix, optionally after an elementwise scalar chain
-// (the fused map -> reduce path), in ONE pass over HBM.
+// (the fused map -> reduce path), reading the input from HBM exactly once.
 //
 // Reference semantics restated here (paths under the reference's src/):
 //   kernels/reduce.rs:207-421   sum/mean/var(ddof)/min/max; empty -> 0; any NaN -> min/max NaN
@@ -10,37 +10,41 @@
 //   math/statistics/mod.rs:1403-1494 axis var = sum((x - mean)^2) / (len - ddof)
 //
 // Work decomposition. Row r of `cols` elements is cut into fixed segments of SEG elements
-// (SEG = 256 threads * VPT 16-byte vectors). One workgroup-iteration reduces one segment in
-// registers: the values stay in VGPRs between the sum pass and the second-moment pass, so
-// variance costs no second read. Each segment yields a partial record; rows that fit in one
-// segment are finalised in place, longer rows and the whole-matrix result are finished by the
-// workgroup that retires the last segment (agent-scope acq_rel counter, self-resetting).
+// (SEG = 256 threads x VPT 16-byte vectors). One workgroup-iteration of the persistent streaming
+// kernel reduces one segment: every thread first reduces its own elements straight from
+// registers (count, pairwise sum, M2 about its own mean, extremum winners), so no thread waits
+// for another until the single end-of-segment barrier; the wavefront and workgroup combine use
+// Chan, Golub & LeVeque's pairwise update for the second moment, in FP64. Rows that fit in one
+// segment are finalised in place. Everything else is finished by two small dependent kernels on
+// the same stream: one wavefront per row (or per group of 64 segments) combines segment
+// partials, then one 1024-thread workgroup combines those for the whole-matrix result. The
+// kernel boundaries provide the cross-workgroup visibility; there is no fence or atomic in the
+// stream (an agent-scope release/acquire writes back / invalidates each XCD's L2 on gfx950,
+// which cost 20x the stream itself when it was done per segment).
 //
-// Determinism. Every sum is a fixed tree: a pairwise tree over each thread's elements, an xor
-// butterfly across the wave, a fixed-order combine of the 4 waves, and a fixed strided
-// assignment of partials to threads in the finish. None of it depends on the grid size or on
-// which workgroup finishes last, so results are bit-identical across launches and devices.
+// Determinism. Every combine is a fixed tree: pairwise within a thread, an xor butterfly within
+// a wavefront (lane 0's value is used), wave order within a workgroup, fixed per-lane strided
+// sequences in the finish. None of it depends on the grid size or on scheduling, so results are
+// bit-identical across launches and devices, and across the tuner's grid-size configurations.
 //
-// Precision ("compensated" mode, the only mode): per-thread pairwise partials in T,
-// double-word (TwoSum) combination across the workgroup, FP64 double-word cross-segment
-// finish, and the second moment combined in FP64 with Chan's shift formula. max/min/argmax/
-// argmin are exact (order-independent): the winner is the largest value, ties going to the
-// lowest index, and the value written is the element itself, so even the sign of a zero
-// maximum matches the reference's left-to-right fold.
-#include <cstring>
-
+// Precision. Per-thread partials in T (pairwise), everything after in FP64 (double-word for
+// sums); the second moment is combined with Chan's formula. max/min/argmax/argmin are exact:
+// the winner is the largest value, ties going to the lowest index, and the value written is the
+// element itself, so even the sign of a zero maximum matches the reference's left-to-right fold.
 #include "common.hpp"
 
 namespace nrk {
 
-constexpr int RB = 256;  // threads per reduction workgroup
+constexpr int RB = 256;  // threads per streaming workgroup
 constexpr int WAVES = RB / 64;
 constexpr uint64_t NO_INDEX = ~0ull;
+constexpr int GROUP = 64;    // segments per finish wavefront when rows need no combining
+constexpr int FB = 1024;     // threads of the final workgroup
 
 enum : int { F_M2 = 1, F_MAX = 2, F_MIN = 4 };
 
-// Persistent-grid size of the segment reducer (workgroups per CU). The result bits do not depend
-// on it (fixed segment tree); only speed does, so the tuner may choose it per shape.
+// Persistent-grid size of the streaming kernels (workgroups per CU). Result bits do not depend on
+// it, only speed, so the tuner may choose it per shape.
 static thread_local int g_blocks_per_cu = 8;
 int reduce_blocks_per_cu() { return g_blocks_per_cu; }
 void set_reduce_blocks_per_cu(int b) { g_blocks_per_cu = b < 1 ? 1 : (b > 16 ? 16 : b); }
@@ -51,18 +55,22 @@ struct OutPtrs {
     uint64_t *argmin, *argmax;
 };
 
+// A set of partial records, structure of arrays. Indices are flat (row * cols + col).
+struct Parts {
+    double *n, *hi, *lo, *m2, *vmax, *vmin;
+    uint64_t *imax, *imin;
+    unsigned int *flags;  // bit0: a NaN was seen; bit1: the set's first element is NaN
+};
+
 template <typename T>
 struct SegParams {
     const T *x;
     size_t rows, cols, spr, nseg;
     Chain chain;
     double ddof;
-    int vec_ok;
     int want_row, want_full, need_partials;
-    // partial records, structure of arrays, nseg entries each
-    double *p_hi, *p_lo, *p_q, *p_center, *p_max, *p_min;
-    uint64_t *p_imax, *p_imin;
-    unsigned int *p_flags;  // bit0 saw NaN, bit1 first element of the row is NaN
+    size_t ngroups;  // first-level finish groups (rows when spr > 1 and rows are wanted)
+    Parts l1, l2;    // segment partials, group partials
     OutPtrs<T> row, full;
 };
 
@@ -74,7 +82,7 @@ __device__ __forceinline__ void two_sum(T a, T b, T &s, T &e) {
     e = rn_add(rn_sub(a, rn_sub(s, v)), rn_sub(b, v));
 }
 
-// double-word addition (a_hi + a_lo) + (b_hi + b_lo), symmetric in its arguments
+// double-word addition (hi + lo) += (bhi + blo)
 template <typename T>
 __device__ __forceinline__ void dw_add(T &hi, T &lo, T bhi, T blo) {
     T s, e;
@@ -92,7 +100,7 @@ __device__ __forceinline__ uint64_t shfl_xor(uint64_t v, int m) {
     return ((uint64_t)hi << 32) | lo;
 }
 
-// (value, index) winner for argmax: larger value wins, equal values go to the lower index.
+// (value, index) winners: larger (smaller) value wins, equal values go to the lower index.
 // Values are never NaN here (NaN elements are excluded before this point).
 template <typename T>
 __device__ __forceinline__ void take_max(T &v, uint64_t &i, T bv, uint64_t bi) {
@@ -103,62 +111,6 @@ __device__ __forceinline__ void take_min(T &v, uint64_t &i, T bv, uint64_t bi) {
     if (bv < v || (bv == v && bi < i)) { v = bv; i = bi; }
 }
 
-// Per-wave results exchanged through LDS; double-buffered by segment parity so that the next
-// segment's writes never race with a slower wave still reading this segment's values.
-template <typename T>
-struct WaveSlots {
-    double hi[WAVES], lo[WAVES], m2[WAVES];
-    int cnt[WAVES];
-    T vmax[WAVES], vmin[WAVES];
-    uint64_t imax[WAVES], imin[WAVES];
-    int nan[WAVES];
-};
-
-// Running statistics of a set of elements: count, double-word sum, and M2 = sum of squared
-// deviations from the set's own mean. merge() is Chan, Golub & LeVeque's pairwise update
-// (1979): M2 = M2a + M2b + (na nb / n) (mean_a - mean_b)^2, evaluated in FP64.
-struct Moments {
-    int n;
-    double hi, lo, m2;
-    __device__ __forceinline__ void merge(int nb, double bhi, double blo, double bm2) {
-        if (nb == 0) return;
-        if (n == 0) {
-            n = nb; hi = bhi; lo = blo; m2 = bm2;
-            return;
-        }
-        const double na = (double)n, nbd = (double)nb, nt = na + nbd;
-        const double d = (hi + lo) / na - (bhi + blo) / nbd;
-        m2 = m2 + bm2 + d * d * (na * nbd / nt);
-        dw_add(hi, lo, bhi, blo);
-        n += nb;
-    }
-};
-struct FinishSlots {
-    double hi[WAVES], lo[WAVES], q[WAVES], vmax[WAVES], vmin[WAVES];
-    uint64_t imax[WAVES], imin[WAVES];
-    int nan[WAVES];
-};
-template <typename T>
-struct BlockShared {
-    WaveSlots<T> w[2];
-    FinishSlots f;
-    int last;
-};
-
-// ---- wave-level reductions (xor butterfly: every lane ends with the same value) ----
-template <typename T>
-__device__ __forceinline__ void wave_dw_sum(T &hi, T &lo) {
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) {
-        T bh = shfl_xor(hi, m), bl = shfl_xor(lo, m);
-        dw_add(hi, lo, bh, bl);
-    }
-}
-__device__ __forceinline__ double wave_sum_f64(double v) {
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v += shfl_xor(v, m);
-    return v;
-}
 template <typename T, bool IS_MAX>
 __device__ __forceinline__ void wave_arg(T &v, uint64_t &i) {
 #pragma unroll
@@ -179,92 +131,107 @@ __device__ __forceinline__ T pairwise(const F &f) {
     }
 }
 
-template <typename T>
-__device__ __forceinline__ void write_outputs(const OutPtrs<T> &o, size_t at, double s_hi, double s_lo, double q_true,
-                                              double count, double ddof, int nan, int first_nan, T vmax, uint64_t imax,
-                                              T vmin, uint64_t imin) {
-    if (o.sum) o.sum[at] = (T)(s_hi + s_lo);
-    if (o.mean) o.mean[at] = (T)((s_hi + s_lo) / count);
-    if (o.var) o.var[at] = (T)(q_true / (count - ddof));
-    if (o.max) o.max[at] = nan ? dev_nan<T>() : vmax;
-    if (o.min) o.min[at] = nan ? dev_nan<T>() : vmin;
-    if (o.argmax) o.argmax[at] = first_nan ? 0 : imax;
-    if (o.argmin) o.argmin[at] = first_nan ? 0 : imin;
+// A partial record in registers; merge() = Chan, Golub & LeVeque (1979) for the second moment:
+// M2 = M2a + M2b + (na nb / n) (mean_a - mean_b)^2, in FP64; double-word sums.
+struct Part {
+    double n, hi, lo, m2, vmax, vmin;
+    uint64_t imax, imin;
+    unsigned nan;
+};
+
+__device__ __forceinline__ Part part_empty() {
+    Part a;
+    a.n = 0.0; a.hi = 0.0; a.lo = 0.0; a.m2 = 0.0;
+    a.vmax = -__builtin_inf(); a.vmin = __builtin_inf();
+    a.imax = NO_INDEX; a.imin = NO_INDEX; a.nan = 0;
+    return a;
 }
 
-// Combine the partial records of segments [lo, hi) (a whole row, or every segment) with the
-// whole workgroup and write one set of outputs. Partial s goes to thread (s - lo) % RB, each
-// thread folds its partials in increasing s, then a fixed butterfly + fixed wave order:
-// deterministic whichever workgroup runs it.
-template <typename T, int FLAGS, int SEG>
-__device__ __noinline__ void finish_range(const SegParams<T> &p, size_t lo, size_t hi, bool full, const OutPtrs<T> &o,
-                                          size_t at, FinishSlots *fs) {
-    const int t = threadIdx.x, w = t >> 6, lane = t & 63;
-    const size_t count_total = full ? p.rows * p.cols : p.cols;
-    double shi = 0.0, slo = 0.0;
-    int nan = 0;
-    double vmax = -__builtin_inf(), vmin = __builtin_inf();
-    uint64_t imax = NO_INDEX, imin = NO_INDEX;
-    for (size_t s = lo + t; s < hi; s += RB) {
-        dw_add(shi, slo, p.p_hi[s], p.p_lo[s]);
-        nan |= p.p_flags[s] & 1;
-        const size_t base = full ? (s / p.spr) * p.cols : 0;
-        if (FLAGS & F_MAX) {
-            const uint64_t ix = p.p_imax[s];
-            take_max(vmax, imax, p.p_max[s], ix == NO_INDEX ? NO_INDEX : base + ix);
-        }
-        if (FLAGS & F_MIN) {
-            const uint64_t ix = p.p_imin[s];
-            take_min(vmin, imin, p.p_min[s], ix == NO_INDEX ? NO_INDEX : base + ix);
-        }
+template <int FLAGS>
+__device__ __forceinline__ void part_merge(Part &a, const Part &b) {
+    a.nan |= b.nan;
+    if (FLAGS & F_MAX) take_max(a.vmax, a.imax, b.vmax, b.imax);
+    if (FLAGS & F_MIN) take_min(a.vmin, a.imin, b.vmin, b.imin);
+    if (b.n == 0.0) return;
+    if (a.n == 0.0) {
+        a.n = b.n; a.hi = b.hi; a.lo = b.lo; a.m2 = b.m2;
+        return;
     }
-    wave_dw_sum(shi, slo);
-    nan = __any(nan) ? 1 : 0;
-    if (FLAGS & F_MAX) wave_arg<double, true>(vmax, imax);
-    if (FLAGS & F_MIN) wave_arg<double, false>(vmin, imin);
-    __syncthreads();
-    if (lane == 0) {
-        fs->hi[w] = shi; fs->lo[w] = slo; fs->nan[w] = nan;
-        fs->vmax[w] = vmax; fs->imax[w] = imax; fs->vmin[w] = vmin; fs->imin[w] = imin;
-    }
-    __syncthreads();
-    shi = fs->hi[0]; slo = fs->lo[0]; nan = fs->nan[0];
-    vmax = fs->vmax[0]; imax = fs->imax[0]; vmin = fs->vmin[0]; imin = fs->imin[0];
-#pragma unroll
-    for (int k = 1; k < WAVES; ++k) {
-        dw_add(shi, slo, fs->hi[k], fs->lo[k]);
-        nan |= fs->nan[k];
-        take_max(vmax, imax, fs->vmax[k], fs->imax[k]);
-        take_min(vmin, imin, fs->vmin[k], fs->imin[k]);
-    }
-    const double mu = (shi + slo) / (double)count_total;
-    double q = 0.0;
     if (FLAGS & F_M2) {
-        for (size_t s = lo + t; s < hi; s += RB) {
-            const size_t c = s % p.spr;
-            const double n_s = (double)min((size_t)SEG, p.cols - c * SEG);
-            const double mu_s = (p.p_hi[s] + p.p_lo[s]) / n_s;
-            const double m_s = p.p_center[s];
-            // Q_s is taken about m_s; shift it to the global mean mu (Chan et al.)
-            q += p.p_q[s] + n_s * (m_s - mu) * (2.0 * mu_s - mu - m_s);
-        }
-        q = wave_sum_f64(q);
-        if (lane == 0) fs->q[w] = q;
-        __syncthreads();
-        q = fs->q[0];
-#pragma unroll
-        for (int k = 1; k < WAVES; ++k) q += fs->q[k];
-        if (q < 0.0) q = 0.0;
+        const double nt = a.n + b.n;
+        const double d = (a.hi + a.lo) / a.n - (b.hi + b.lo) / b.n;
+        a.m2 = a.m2 + b.m2 + d * d * (a.n * b.n / nt);
     }
-    if (t == 0) {
-        const int first_nan = (p.p_flags[lo] >> 1) & 1;
-        write_outputs<T>(o, at, shi, slo, q, (double)count_total, p.ddof, nan, first_nan, (T)vmax, imax, (T)vmin, imin);
-    }
-    __syncthreads();
+    dw_add(a.hi, a.lo, b.hi, b.lo);
+    a.n += b.n;
 }
 
+template <int FLAGS>
+__device__ __forceinline__ Part part_shfl(const Part &a, int m) {
+    Part b;
+    b.n = shfl_xor(a.n, m);
+    b.hi = shfl_xor(a.hi, m);
+    b.lo = shfl_xor(a.lo, m);
+    b.m2 = (FLAGS & F_M2) ? shfl_xor(a.m2, m) : 0.0;
+    b.vmax = (FLAGS & F_MAX) ? shfl_xor(a.vmax, m) : -__builtin_inf();
+    b.imax = (FLAGS & F_MAX) ? shfl_xor(a.imax, m) : NO_INDEX;
+    b.vmin = (FLAGS & F_MIN) ? shfl_xor(a.vmin, m) : __builtin_inf();
+    b.imin = (FLAGS & F_MIN) ? shfl_xor(a.imin, m) : NO_INDEX;
+    b.nan = (unsigned)__shfl_xor((int)a.nan, m, 64);
+    return b;
+}
+
+template <int FLAGS>
+__device__ __forceinline__ void wave_merge(Part &a) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        const Part b = part_shfl<FLAGS>(a, m);
+        part_merge<FLAGS>(a, b);
+    }
+}
+
+__device__ __forceinline__ Part part_load(const Parts &P, size_t s) {
+    Part a;
+    a.n = P.n[s]; a.hi = P.hi[s]; a.lo = P.lo[s]; a.m2 = P.m2[s];
+    a.vmax = P.vmax[s]; a.vmin = P.vmin[s]; a.imax = P.imax[s]; a.imin = P.imin[s];
+    a.nan = P.flags[s] & 1u;
+    return a;
+}
+
+__device__ __forceinline__ void part_store(const Parts &P, size_t s, const Part &a, unsigned first_nan) {
+    P.n[s] = a.n; P.hi[s] = a.hi; P.lo[s] = a.lo; P.m2[s] = a.m2;
+    P.vmax[s] = a.vmax; P.vmin[s] = a.vmin; P.imax[s] = a.imax; P.imin[s] = a.imin;
+    P.flags[s] = a.nan | (first_nan << 1);
+}
+
+// Write one set of results. `index_base` is subtracted from the flat winner indices (a row's
+// first flat index for row results, 0 for whole-matrix results).
+template <typename T>
+__device__ __forceinline__ void write_outputs(const OutPtrs<T> &o, size_t at, const Part &a, double ddof, unsigned first_nan,
+                                              uint64_t index_base) {
+    const double s = a.hi + a.lo;
+    if (o.sum) o.sum[at] = (T)s;
+    if (o.mean) o.mean[at] = (T)(s / a.n);
+    if (o.var) o.var[at] = (T)(a.m2 / (a.n - ddof));
+    if (o.max) o.max[at] = a.nan ? dev_nan<T>() : (T)a.vmax;
+    if (o.min) o.min[at] = a.nan ? dev_nan<T>() : (T)a.vmin;
+    if (o.argmax) o.argmax[at] = first_nan ? 0 : (a.imax == NO_INDEX ? 0 : a.imax - index_base);
+    if (o.argmin) o.argmin[at] = first_nan ? 0 : (a.imin == NO_INDEX ? 0 : a.imin - index_base);
+}
+
+// Per-wave results of a segment, double-buffered by segment parity so that the next segment's
+// writes never race with thread 0 still reading this segment's values.
+template <typename T>
+struct WaveSlots {
+    double n[WAVES], hi[WAVES], lo[WAVES], m2[WAVES];
+    T vmax[WAVES], vmin[WAVES];
+    uint64_t imax[WAVES], imin[WAVES];
+    unsigned nan[WAVES];
+};
+
+// ---------------- the streaming kernel ----------------
 template <typename T, int VPT, int FLAGS, bool FULLSEG, bool VEC>
-__device__ __forceinline__ void reduce_segment(const SegParams<T> &p, size_t seg, int parity, BlockShared<T> *sh) {
+__device__ __forceinline__ void reduce_segment(const SegParams<T> &p, size_t seg, WaveSlots<T> &ws) {
     typedef typename Vec16<T>::type V;
     constexpr int VN = Vec16<T>::N;
     constexpr int M = VPT * VN;  // elements per thread (<= 32: one validity bit each)
@@ -274,7 +241,8 @@ __device__ __forceinline__ void reduce_segment(const SegParams<T> &p, size_t seg
     const size_t r = seg / p.spr, c = seg - r * p.spr;
     const size_t col0 = c * SEG;
     const unsigned len = FULLSEG ? (unsigned)SEG : (unsigned)min((size_t)SEG, p.cols - col0);
-    const T *base = p.x + r * p.cols + col0;  // uniform across the workgroup
+    const size_t flat0 = r * p.cols + col0;
+    const T *base = p.x + flat0;  // uniform across the workgroup
 
     // ---- load: element i = j*VN + e of this thread sits at segment position (j*RB + t)*VN + e
     T y[M];
@@ -318,16 +286,18 @@ __device__ __forceinline__ void reduce_segment(const SegParams<T> &p, size_t seg
         for (int i = 0; i < M; ++i) y[i] = apply_sop<T>(op, s, y[i]);
     }
 
-    // ---- per-thread statistics straight from registers; nothing here waits for other threads:
-    // count, pairwise sum, M2 about the thread's own mean, extremum winners, NaN flag.
+    // ---- per-thread statistics straight from registers; nothing here waits for other threads
     const int nt = FULLSEG ? M : __builtin_popcount(vmask);
     const T st = pairwise<M, 0, T>([&](int i) { return valid(i) ? y[i] : (T)0; });
-    int nan = 0;
+    unsigned nan = 0;
     if (__builtin_isnan(st)) {  // rare: tell a NaN element apart from inf - inf
 #pragma unroll
-        for (int i = 0; i < M; ++i) nan |= (valid(i) && __builtin_isnan(y[i])) ? 1 : 0;
+        for (int i = 0; i < M; ++i) nan |= (valid(i) && __builtin_isnan(y[i])) ? 1u : 0u;
     }
-    double m2t = 0.0;
+    Part a = part_empty();
+    a.n = (double)nt;
+    a.hi = (double)st;
+    a.nan = nan;
     if ((FLAGS & F_M2) && nt > 0) {
         // mean of M elements: an exact power-of-two scaling for full segments
         const T mt = FULLSEG ? rn_mul(st, (T)(1.0 / M)) : rn_div(st, (T)nt);
@@ -337,7 +307,7 @@ __device__ __forceinline__ void reduce_segment(const SegParams<T> &p, size_t seg
         });
         // qt is taken about the rounded mean mt; shift it to the mean st/nt: Q = M2 + n (mt - mean)^2
         const double dm = (double)mt - (double)st / (double)nt;
-        m2t = fmax((double)qt - (double)nt * dm * dm, 0.0);
+        a.m2 = fmax((double)qt - (double)nt * dm * dm, 0.0);
     }
     T vmax = -dev_inf<T>(), vmin = dev_inf<T>();
     uint64_t imax = NO_INDEX, imin = NO_INDEX;
@@ -352,7 +322,7 @@ __device__ __forceinline__ void reduce_segment(const SegParams<T> &p, size_t seg
             vmax = m;
 #pragma unroll
             for (int i = 0; i < M; ++i) vmax = (i == first) ? y[i] : vmax;  // keeps the sign of a zero
-            imax = col0 + pos_of(first);
+            imax = flat0 + pos_of(first);
         }
     }
     if (FLAGS & F_MIN) {
@@ -366,103 +336,107 @@ __device__ __forceinline__ void reduce_segment(const SegParams<T> &p, size_t seg
             vmin = m;
 #pragma unroll
             for (int i = 0; i < M; ++i) vmin = (i == first) ? y[i] : vmin;
-            imin = col0 + pos_of(first);
+            imin = flat0 + pos_of(first);
         }
     }
-    const int first_nan = (c == 0 && t == 0 && __builtin_isnan(y[0])) ? 1 : 0;
+    const unsigned first_nan = (c == 0 && t == 0 && __builtin_isnan(y[0])) ? 1u : 0u;
 
     // ---- wavefront combine (fixed xor butterfly; lane 0's result is the one used)
-    Moments mo{nt, (double)st, 0.0, m2t};
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) {
-        const int bn = __shfl_xor(mo.n, m, 64);
-        const double bhi = shfl_xor(mo.hi, m), blo = shfl_xor(mo.lo, m);
-        const double bm2 = (FLAGS & F_M2) ? shfl_xor(mo.m2, m) : 0.0;
-        if (FLAGS & F_M2) {
-            mo.merge(bn, bhi, blo, bm2);
-        } else {
-            dw_add(mo.hi, mo.lo, bhi, blo);
-            mo.n += bn;
-        }
+        Part b = part_empty();
+        b.n = shfl_xor(a.n, m);
+        b.hi = shfl_xor(a.hi, m);
+        b.lo = shfl_xor(a.lo, m);
+        if (FLAGS & F_M2) b.m2 = shfl_xor(a.m2, m);
+        part_merge<FLAGS & F_M2>(a, b);
     }
-    nan = __any(nan) ? 1 : 0;
+    a.nan = __any((int)a.nan) ? 1u : 0u;
     if (FLAGS & F_MAX) wave_arg<T, true>(vmax, imax);
     if (FLAGS & F_MIN) wave_arg<T, false>(vmin, imin);
-    WaveSlots<T> &ws = sh->w[parity];
     if (lane == 0) {
-        ws.hi[w] = mo.hi; ws.lo[w] = mo.lo; ws.m2[w] = mo.m2; ws.cnt[w] = mo.n; ws.nan[w] = nan;
+        ws.n[w] = a.n; ws.hi[w] = a.hi; ws.lo[w] = a.lo; ws.m2[w] = a.m2; ws.nan[w] = a.nan;
         if (FLAGS & F_MAX) { ws.vmax[w] = vmax; ws.imax[w] = imax; }
         if (FLAGS & F_MIN) { ws.vmin[w] = vmin; ws.imin[w] = imin; }
     }
     __syncthreads();  // the only barrier of the segment
     if (t != 0) return;
     // ---- workgroup combine in wave order, by thread 0
-    Moments tot{ws.cnt[0], ws.hi[0], ws.lo[0], ws.m2[0]};
-    nan = ws.nan[0];
-    if (FLAGS & F_MAX) { vmax = ws.vmax[0]; imax = ws.imax[0]; }
-    if (FLAGS & F_MIN) { vmin = ws.vmin[0]; imin = ws.imin[0]; }
+    Part tot = part_empty();
 #pragma unroll
-    for (int k = 1; k < WAVES; ++k) {
-        if (FLAGS & F_M2) {
-            tot.merge(ws.cnt[k], ws.hi[k], ws.lo[k], ws.m2[k]);
-        } else {
-            dw_add(tot.hi, tot.lo, ws.hi[k], ws.lo[k]);
-            tot.n += ws.cnt[k];
-        }
-        nan |= ws.nan[k];
-        if (FLAGS & F_MAX) take_max(vmax, imax, ws.vmax[k], ws.imax[k]);
-        if (FLAGS & F_MIN) take_min(vmin, imin, ws.vmin[k], ws.imin[k]);
+    for (int k = 0; k < WAVES; ++k) {
+        Part b = part_empty();
+        b.n = ws.n[k]; b.hi = ws.hi[k]; b.lo = ws.lo[k]; b.m2 = ws.m2[k]; b.nan = ws.nan[k];
+        if (FLAGS & F_MAX) { b.vmax = (double)ws.vmax[k]; b.imax = ws.imax[k]; }
+        if (FLAGS & F_MIN) { b.vmin = (double)ws.vmin[k]; b.imin = ws.imin[k]; }
+        part_merge<FLAGS>(tot, b);
     }
-    if (p.spr == 1 && p.want_row) {
-        // the whole row is this segment: finalise in place
-        write_outputs<T>(p.row, r, tot.hi, tot.lo, tot.m2, (double)len, p.ddof, nan, first_nan, vmax, imax, vmin, imin);
-    }
-    if (!p.need_partials) return;
-    p.p_hi[seg] = tot.hi;
-    p.p_lo[seg] = tot.lo;
-    p.p_q[seg] = tot.m2;                              // about the segment's own mean ...
-    p.p_center[seg] = (tot.hi + tot.lo) / (double)len;  // ... which is this
-    p.p_max[seg] = (double)vmax;
-    p.p_min[seg] = (double)vmin;
-    p.p_imax[seg] = imax;
-    p.p_imin[seg] = imin;
-    p.p_flags[seg] = (unsigned)nan | ((unsigned)first_nan << 1);
+    if (p.spr == 1 && p.want_row) write_outputs<T>(p.row, r, tot, p.ddof, first_nan, r * p.cols);
+    if (p.need_partials) part_store(p.l1, seg, tot, first_nan);
 }
 
 template <typename T, int VPT, int FLAGS, bool VEC>
 __global__ __launch_bounds__(RB) void seg_reduce_kernel(SegParams<T> p) {
     constexpr int SEG = RB * VPT * Vec16<T>::N;
-    __shared__ BlockShared<T> sh;
+    __shared__ WaveSlots<T> ws[2];
     int parity = 0;
     for (size_t seg = blockIdx.x; seg < p.nseg; seg += gridDim.x, parity ^= 1) {
         const size_t c = seg % p.spr;
         if (c * SEG + SEG <= p.cols) {
-            reduce_segment<T, VPT, FLAGS, true, VEC>(p, seg, parity, &sh);
+            reduce_segment<T, VPT, FLAGS, true, VEC>(p, seg, ws[parity]);
         } else {
-            reduce_segment<T, VPT, FLAGS, false, VEC>(p, seg, parity, &sh);
+            reduce_segment<T, VPT, FLAGS, false, VEC>(p, seg, ws[parity]);
         }
     }
 }
 
-// Cross-segment finish, launched on the same stream right after the streaming kernel (the
-// kernel boundary makes every partial visible; no fence or atomic is needed inside the stream).
-// Workgroup b < row_blocks finishes row b (rows of more than one segment); the last workgroup
-// finishes the whole-matrix result. On gfx950 a per-workgroup agent-scope release/acquire
-// (L2 write-back + invalidate on each XCD) costs far more than this extra launch.
-template <typename T, int VPT, int FLAGS>
-__global__ __launch_bounds__(RB) void seg_finish_kernel(SegParams<T> p, int row_blocks) {
-    constexpr int SEG = RB * VPT * Vec16<T>::N;
-    __shared__ FinishSlots fs;
-    const size_t b = blockIdx.x;
-    if ((int)b < row_blocks) {
-        finish_range<T, FLAGS, SEG>(p, b * p.spr, (b + 1) * p.spr, false, p.row, b, &fs);
-    } else {
-        finish_range<T, FLAGS, SEG>(p, 0, p.nseg, true, p.full, 0, &fs);
+// ---------------- finish, level 1: one wavefront per group of segment partials ----------------
+// Group g is row g's segments when rows need combining (spr > 1), else GROUP consecutive
+// segments. Lane l folds partials lo+l, lo+l+64, ... in order, then the xor butterfly.
+template <typename T, int FLAGS>
+__global__ __launch_bounds__(256) void finish_groups_kernel(SegParams<T> p) {
+    const size_t g = (blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (g >= p.ngroups) return;  // whole wavefronts exit together
+    const bool rowmode = p.want_row && p.spr > 1;
+    const size_t per = rowmode ? p.spr : (size_t)GROUP;
+    const size_t lo = g * per, hi = min(p.nseg, lo + per);
+    Part a = part_empty();
+    for (size_t s = lo + lane; s < hi; s += 64) {
+        const Part b = part_load(p.l1, s);
+        part_merge<FLAGS>(a, b);
+    }
+    wave_merge<FLAGS>(a);
+    if (lane == 0) {
+        const unsigned first_nan = (p.l1.flags[lo] >> 1) & 1u;
+        if (rowmode) write_outputs<T>(p.row, g, a, p.ddof, first_nan, g * p.cols);
+        if (p.want_full) part_store(p.l2, g, a, first_nan);
     }
 }
 
-// Writes T zero (or NaN) into up to 7 scalar outputs; used for empty inputs so that no
-// hipMemsetAsync is ever recorded into a caller's graph.
+// ---------------- finish, level 2: one workgroup over the group partials ----------------
+template <typename T, int FLAGS>
+__global__ __launch_bounds__(FB) void finish_full_kernel(SegParams<T> p) {
+    __shared__ Part wp[FB / 64];
+    const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+    Part a = part_empty();
+    for (size_t s = t; s < p.ngroups; s += FB) {
+        const Part b = part_load(p.l2, s);
+        part_merge<FLAGS>(a, b);
+    }
+    wave_merge<FLAGS>(a);
+    if (lane == 0) wp[w] = a;
+    __syncthreads();
+    if (t == 0) {
+        Part tot = wp[0];
+        for (int k = 1; k < FB / 64; ++k) part_merge<FLAGS>(tot, wp[k]);
+        const unsigned first_nan = (p.l1.flags[0] >> 1) & 1u;
+        write_outputs<T>(p.full, 0, tot, p.ddof, first_nan, 0);
+    }
+}
+
+// Writes T zero into up to 7 outputs; used for empty inputs so that no hipMemsetAsync is ever
+// recorded into a caller's graph.
 template <typename T>
 __global__ void fill_outputs_kernel(OutPtrs<T> o, size_t n, T value) {
     size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
@@ -497,19 +471,39 @@ static bool any_out(const OutPtrs<T> &o) {
 
 template <typename T, int VPT, int FLAGS>
 static nrk_status launch_flags(SegParams<T> &p, hipStream_t stream) {
-    int grid = (int)std::min<size_t>(p.nseg, (size_t)cu_count() * reduce_blocks_per_cu());
-    if (p.vec_ok)
+    const int grid = (int)std::min<size_t>(p.nseg, (size_t)cu_count() * reduce_blocks_per_cu());
+    if (((reinterpret_cast<uintptr_t>(p.x) % 16) == 0) && (p.cols % Vec16<T>::N == 0))
         hipLaunchKernelGGL((seg_reduce_kernel<T, VPT, FLAGS, true>), dim3(grid), dim3(RB), 0, stream, p);
     else
         hipLaunchKernelGGL((seg_reduce_kernel<T, VPT, FLAGS, false>), dim3(grid), dim3(RB), 0, stream, p);
     NRK_HIP(hipGetLastError());
-    const int row_blocks = (p.spr > 1 && p.want_row) ? (int)p.rows : 0;
-    const int fin = row_blocks + (p.want_full ? 1 : 0);
-    if (fin > 0) {
-        hipLaunchKernelGGL((seg_finish_kernel<T, VPT, FLAGS>), dim3(fin), dim3(RB), 0, stream, p, row_blocks);
+    if (!p.need_partials) return NRK_OK;
+    hipLaunchKernelGGL((finish_groups_kernel<T, FLAGS>), dim3((unsigned)ceil_div(p.ngroups, 4)), dim3(256), 0, stream, p);
+    NRK_HIP(hipGetLastError());
+    if (p.want_full) {
+        hipLaunchKernelGGL((finish_full_kernel<T, FLAGS>), dim3(1), dim3(FB), 0, stream, p);
         NRK_HIP(hipGetLastError());
     }
     return NRK_OK;
+}
+
+static Parts carve_parts(char *&b, size_t count) {
+    auto take = [&](size_t bytes) {
+        void *r = b;
+        b += align_up(bytes, 256);
+        return r;
+    };
+    Parts P;
+    P.n = (double *)take(count * 8);
+    P.hi = (double *)take(count * 8);
+    P.lo = (double *)take(count * 8);
+    P.m2 = (double *)take(count * 8);
+    P.vmax = (double *)take(count * 8);
+    P.vmin = (double *)take(count * 8);
+    P.imax = (uint64_t *)take(count * 8);
+    P.imin = (uint64_t *)take(count * 8);
+    P.flags = (unsigned *)take(count * 4);
+    return P;
 }
 
 // Host launcher for the fused map -> reduce over a (rows, cols) matrix.
@@ -551,29 +545,17 @@ nrk_status seg_reduce(const Chain &chain, const T *x, size_t rows, size_t cols, 
     p.spr = ceil_div(cols, SEG);
     p.nseg = rows * p.spr;
     if (p.nseg >= (size_t)UINT32_MAX) return fail(NRK_ERR_INVALID_ARGUMENT, "too many segments (%zu)", p.nseg);
-    p.vec_ok = ((reinterpret_cast<uintptr_t>(x) % 16) == 0) && (cols % Vec16<T>::N == 0);
-    p.need_partials = p.want_full || (p.spr > 1 && p.want_row);
-    const size_t rec = 8 * 8 + 4;
-    const size_t pbytes = p.need_partials ? align_up(p.nseg * rec + 64 * 9, 256) : 0;
+    const bool rowmode = p.want_row && p.spr > 1;
+    p.need_partials = p.want_full || rowmode;
+    p.ngroups = rowmode ? rows : ceil_div(p.nseg, (size_t)GROUP);
+    const size_t rec = 8 * 8 + 4 + 9 * 256;
+    const size_t pbytes = p.need_partials ? (p.nseg + p.ngroups) * rec + 4096 : 0;
     Scratch s;
     NRK_TRY(get_scratch(stream, 0, pbytes, &s));
     if (p.need_partials) {
         char *b = (char *)s.partials;
-        size_t off = 0;
-        auto take = [&](size_t bytes) {
-            void *r = b + off;
-            off = align_up(off + bytes, 256);
-            return r;
-        };
-        p.p_hi = (double *)take(p.nseg * 8);
-        p.p_lo = (double *)take(p.nseg * 8);
-        p.p_q = (double *)take(p.nseg * 8);
-        p.p_center = (double *)take(p.nseg * 8);
-        p.p_max = (double *)take(p.nseg * 8);
-        p.p_min = (double *)take(p.nseg * 8);
-        p.p_imax = (uint64_t *)take(p.nseg * 8);
-        p.p_imin = (uint64_t *)take(p.nseg * 8);
-        p.p_flags = (unsigned *)take(p.nseg * 4);
+        p.l1 = carve_parts(b, p.nseg);
+        p.l2 = carve_parts(b, p.ngroups);
     }
     const bool m2 = (p.row.var || p.full.var);
     const bool mx = (p.row.max || p.row.argmax || p.full.max || p.full.argmax);
@@ -597,18 +579,19 @@ template nrk_status seg_reduce<double>(const Chain &, const double *, size_t, si
                                        const nrk_reduce_outputs *, hipStream_t);
 
 // ---------------- i64 sum: wrapping integer adds, exact and order-independent ----------------
-constexpr int I64_VPT = 8;                     // 16-byte vectors per thread per segment
-constexpr int I64_SEG = RB * I64_VPT * 2;      // 4096 elements = 32 KiB
+constexpr int I64_VPT = 8;                 // 16-byte vectors per thread per segment
+constexpr int I64_SEG = RB * I64_VPT * 2;  // 4096 elements = 32 KiB
 
+// Each workgroup adds all of its segments (integer adds are exact, so the grouping cannot change
+// the result) and writes one partial.
 __global__ __launch_bounds__(RB) void sum_i64_kernel(const int64_t *x, size_t n, size_t nseg, int vec_ok,
                                                      unsigned long long *partials) {
     typedef long long V2 __attribute__((ext_vector_type(2)));
-    __shared__ unsigned long long sred[2][WAVES];
+    __shared__ unsigned long long sred[WAVES];
     const int t = threadIdx.x;
-    int parity = 0;
+    unsigned long long acc = 0;  // unsigned: wrapping is defined, bits equal two's complement i64
     for (size_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
         const size_t off = seg * I64_SEG;
-        unsigned long long acc = 0;  // unsigned: wrapping is defined, bits equal two's complement i64
         if (vec_ok && off + I64_SEG <= n) {
             const V2 *bv = reinterpret_cast<const V2 *>(x + off);
 #pragma unroll
@@ -622,33 +605,30 @@ __global__ __launch_bounds__(RB) void sum_i64_kernel(const int64_t *x, size_t n,
                 if (pos < n) acc += (unsigned long long)x[pos];
             }
         }
+    }
 #pragma unroll
-        for (int m = 32; m >= 1; m >>= 1) acc += shfl_xor((uint64_t)acc, m);
-        if ((t & 63) == 0) sred[parity][t >> 6] = acc;
-        __syncthreads();
-        if (t == 0) {
-            unsigned long long s = 0;
-            for (int k = 0; k < WAVES; ++k) s += sred[parity][k];
-            partials[seg] = s;
-        }
-        parity ^= 1;  // the next segment writes the other slot set: no second barrier needed
+    for (int m = 32; m >= 1; m >>= 1) acc += shfl_xor((uint64_t)acc, m);
+    if ((t & 63) == 0) sred[t >> 6] = acc;
+    __syncthreads();
+    if (t == 0) {
+        unsigned long long s = 0;
+        for (int k = 0; k < WAVES; ++k) s += sred[k];
+        partials[blockIdx.x] = s;
     }
 }
 
-// One workgroup adds the per-segment partials (integer adds: exact in any order).
-__global__ __launch_bounds__(RB) void sum_u64_finish_kernel(const unsigned long long *partials, size_t nseg,
-                                                            int64_t *out) {
-    __shared__ unsigned long long sred[WAVES];
+__global__ __launch_bounds__(FB) void sum_u64_finish_kernel(const unsigned long long *partials, size_t count, int64_t *out) {
+    __shared__ unsigned long long sred[FB / 64];
     const int t = threadIdx.x;
     unsigned long long s = 0;
-    for (size_t k = t; k < nseg; k += RB) s += partials[k];
+    for (size_t k = t; k < count; k += FB) s += partials[k];
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) s += shfl_xor((uint64_t)s, m);
     if ((t & 63) == 0) sred[t >> 6] = s;
     __syncthreads();
     if (t == 0) {
         unsigned long long tot = 0;
-        for (int k = 0; k < WAVES; ++k) tot += sred[k];
+        for (int k = 0; k < FB / 64; ++k) tot += sred[k];
         *out = (int64_t)tot;
     }
 }
@@ -664,14 +644,14 @@ nrk_status sum_i64(const int64_t *x, size_t n, int64_t *out, hipStream_t stream)
     }
     if (!x) return fail(NRK_ERR_INVALID_ARGUMENT, "x is NULL");
     const size_t nseg = ceil_div(n, I64_SEG);
-    if (nseg >= (size_t)UINT32_MAX) return fail(NRK_ERR_INVALID_ARGUMENT, "input too large");
+    const int grid = (int)std::min<size_t>(nseg, (size_t)cu_count() * reduce_blocks_per_cu());
     Scratch s;
-    NRK_TRY(get_scratch(stream, 0, nseg * 8, &s));
+    NRK_TRY(get_scratch(stream, 0, (size_t)grid * 8, &s));
     const int vec_ok = (reinterpret_cast<uintptr_t>(x) % 16) == 0;
-    int grid = (int)std::min<size_t>(nseg, (size_t)cu_count() * reduce_blocks_per_cu());
     hipLaunchKernelGGL(sum_i64_kernel, dim3(grid), dim3(RB), 0, stream, x, n, nseg, vec_ok, (unsigned long long *)s.partials);
     NRK_HIP(hipGetLastError());
-    hipLaunchKernelGGL(sum_u64_finish_kernel, dim3(1), dim3(RB), 0, stream, (const unsigned long long *)s.partials, nseg, out);
+    hipLaunchKernelGGL(sum_u64_finish_kernel, dim3(1), dim3(FB), 0, stream, (const unsigned long long *)s.partials, (size_t)grid,
+                       out);
     NRK_HIP(hipGetLastError());
     return NRK_OK;
 }
@@ -704,18 +684,18 @@ __global__ __launch_bounds__(RB) void prod_kernel(const T *x, size_t n, size_t n
 
 // One workgroup multiplies the per-segment partials in a fixed tree.
 template <typename T>
-__global__ __launch_bounds__(RB) void prod_finish_kernel(const T *partials, size_t nseg, T *out) {
-    __shared__ T sred[WAVES];
+__global__ __launch_bounds__(FB) void prod_finish_kernel(const T *partials, size_t nseg, T *out) {
+    __shared__ T sred[FB / 64];
     const int t = threadIdx.x;
     T s = (T)1;
-    for (size_t k = t; k < nseg; k += RB) s = rn_mul(s, partials[k]);
+    for (size_t k = t; k < nseg; k += FB) s = rn_mul(s, partials[k]);
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) s = rn_mul(s, shfl_xor(s, m));
     if ((t & 63) == 0) sred[t >> 6] = s;
     __syncthreads();
     if (t == 0) {
         T tot = sred[0];
-        for (int k = 1; k < WAVES; ++k) tot = rn_mul(tot, sred[k]);
+        for (int k = 1; k < FB / 64; ++k) tot = rn_mul(tot, sred[k]);
         *out = tot;
     }
 }
@@ -738,7 +718,7 @@ nrk_status prod(const T *x, size_t n, T *out, hipStream_t stream) {
     int grid = (int)std::min<size_t>(nseg, (size_t)cu_count() * 8);
     hipLaunchKernelGGL(prod_kernel<T>, dim3(grid), dim3(RB), 0, stream, x, n, nseg, (T *)s.partials);
     NRK_HIP(hipGetLastError());
-    hipLaunchKernelGGL(prod_finish_kernel<T>, dim3(1), dim3(RB), 0, stream, (const T *)s.partials, nseg, out);
+    hipLaunchKernelGGL(prod_finish_kernel<T>, dim3(1), dim3(FB), 0, stream, (const T *)s.partials, nseg, out);
     NRK_HIP(hipGetLastError());
     return NRK_OK;
 }
