@@ -9,25 +9,25 @@
 //   math/aggregation.rs:244-261     axis max/min: NaN short-circuits to NaN
 //   math/statistics/mod.rs:1403-1494 axis var = sum((x - mean)^2) / (len - ddof)
 //
-// Work decomposition. Row r of `cols` elements is cut into fixed segments of SEG elements
-// (SEG = 256 threads x VPT 16-byte vectors). One workgroup-iteration of the persistent streaming
-// kernel reduces one segment: every thread first reduces its own elements straight from
-// registers (count, pairwise sum, M2 about its own mean, extremum winners), so no thread waits
-// for another until the single end-of-segment barrier; the wavefront and workgroup combine use
-// Chan, Golub & LeVeque's pairwise update for the second moment, in FP64. Rows that fit in one
-// segment are finalised in place. Everything else is finished by two small dependent kernels on
-// the same stream: one wavefront per row (or per group of 64 segments) combines segment
-// partials, then one 1024-thread workgroup combines those for the whole-matrix result. The
-// kernel boundaries provide the cross-workgroup visibility; there is no fence or atomic in the
-// stream (an agent-scope release/acquire writes back / invalidates each XCD's L2 on gfx950,
-// which cost 20x the stream itself when it was done per segment).
+// Work decomposition. Row r of `cols` elements is cut into fixed segments of WSEG elements
+// (64 lanes x VPT 16-byte vectors), each owned by ONE wavefront of the persistent streaming
+// kernel. The wavefront reduces its segment from registers -- pairwise sums per lane, the
+// segment mean, then squared deviations about it, combined across lanes with xor butterflies of
+// FP64 additions -- and writes one partial record. Wavefronts never wait for each other (no
+// barrier, no LDS). Two small dependent kernels on the same stream finish: one wavefront per
+// row (or per group of 64 segments) combines segment partials and writes row results, then one
+// 1024-thread workgroup combines those for the whole-matrix result, using Chan, Golub & LeVeque's
+// pairwise update for the second moment. The kernel boundaries provide the cross-workgroup
+// visibility; there is no fence or atomic in the stream (an agent-scope release/acquire writes
+// back / invalidates each XCD's L2 on gfx950, which cost 20x the stream itself when it was done
+// per segment).
 //
-// Determinism. Every combine is a fixed tree: pairwise within a thread, an xor butterfly within
-// a wavefront (lane 0's value is used), wave order within a workgroup, fixed per-lane strided
-// sequences in the finish. None of it depends on the grid size or on scheduling, so results are
-// bit-identical across launches and devices, and across the tuner's grid-size configurations.
+// Determinism. Every combine is a fixed tree: pairwise within a lane, an xor butterfly within
+// a wavefront (lane 0's value is used), fixed per-lane strided sequences and butterflies in the
+// finish. None of it depends on the grid size or on scheduling, so results are bit-identical
+// across launches and devices, and across the tuner's grid-size configurations.
 //
-// Precision. Per-thread partials in T (pairwise), everything after in FP64 (double-word for
+// Precision. Per-lane partials in T (pairwise), everything after in FP64 (double-word for
 // sums); the second moment is combined with Chan's formula. max/min/argmax/argmin are exact:
 // the winner is the largest value, ties going to the lowest index, and the value written is the
 // element itself, so even the sign of a zero maximum matches the reference's left-to-right fold.
@@ -219,32 +219,28 @@ __device__ __forceinline__ void write_outputs(const OutPtrs<T> &o, size_t at, co
     if (o.argmin) o.argmin[at] = first_nan ? 0 : (a.imin == NO_INDEX ? 0 : a.imin - index_base);
 }
 
-// Per-wave results of a segment, double-buffered by segment parity so that the next segment's
-// writes never race with thread 0 still reading this segment's values.
-template <typename T>
-struct WaveSlots {
-    double n[WAVES], hi[WAVES], lo[WAVES], m2[WAVES];
-    T vmax[WAVES], vmin[WAVES];
-    uint64_t imax[WAVES], imin[WAVES];
-    unsigned nan[WAVES];
-};
-
 // ---------------- the streaming kernel ----------------
+// A segment is WSEG = 64 lanes x VPT 16-byte vectors of one row, owned by ONE wavefront: lane l
+// holds element i = j*VN + e at segment position (j*64 + l)*VN + e. The wavefront computes its
+// count, double-word sum, M2 (two passes over registers: the wavefront mean first, then the
+// squared deviations about it, both combined with xor butterflies of plain FP64 additions) and
+// extremum winners, and lane 0 writes one partial record. Wavefronts never wait for each other:
+// the kernel has no barrier and no LDS.
 template <typename T, int VPT, int FLAGS, bool FULLSEG, bool VEC>
-__device__ __forceinline__ void reduce_segment(const SegParams<T> &p, size_t seg, WaveSlots<T> &ws) {
+__device__ __forceinline__ void reduce_segment(const SegParams<T> &p, size_t seg) {
     typedef typename Vec16<T>::type V;
     constexpr int VN = Vec16<T>::N;
-    constexpr int M = VPT * VN;  // elements per thread (<= 32: one validity bit each)
-    constexpr int SEG = RB * M;
+    constexpr int M = VPT * VN;  // elements per lane (<= 32: one validity bit each)
+    constexpr int WSEG = 64 * M;
     static_assert(M <= 32, "validity mask holds 32 elements");
-    const unsigned t = threadIdx.x, w = t >> 6, lane = t & 63;
+    const unsigned lane = threadIdx.x & 63;
     const size_t r = seg / p.spr, c = seg - r * p.spr;
-    const size_t col0 = c * SEG;
-    const unsigned len = FULLSEG ? (unsigned)SEG : (unsigned)min((size_t)SEG, p.cols - col0);
+    const size_t col0 = c * WSEG;
+    const unsigned len = FULLSEG ? (unsigned)WSEG : (unsigned)min((size_t)WSEG, p.cols - col0);
     const size_t flat0 = r * p.cols + col0;
-    const T *base = p.x + flat0;  // uniform across the workgroup
+    const T *base = p.x + flat0;  // uniform across the wavefront
 
-    // ---- load: element i = j*VN + e of this thread sits at segment position (j*RB + t)*VN + e
+    // ---- load
     T y[M];
     unsigned vmask = ~0u;
     if (VEC) {
@@ -254,7 +250,7 @@ __device__ __forceinline__ void reduce_segment(const SegParams<T> &p, size_t seg
         if (!FULLSEG) vmask = 0u;
 #pragma unroll
         for (int j = 0; j < VPT; ++j) {
-            const unsigned jv = j * RB + t;
+            const unsigned jv = j * 64 + lane;
             const bool ok = FULLSEG || jv < nvec;
             V v = V{};
             if (ok) v = __builtin_nontemporal_load(bv + jv);
@@ -268,7 +264,7 @@ __device__ __forceinline__ void reduce_segment(const SegParams<T> &p, size_t seg
         for (int j = 0; j < VPT; ++j) {
 #pragma unroll
             for (int e = 0; e < VN; ++e) {
-                const unsigned pos = (j * RB + t) * VN + e;
+                const unsigned pos = (j * 64 + lane) * VN + e;
                 const bool ok = FULLSEG || pos < len;
                 y[j * VN + e] = ok ? base[pos] : (T)0;
                 if (!FULLSEG) vmask |= (ok ? 1u : 0u) << (j * VN + e);
@@ -276,7 +272,7 @@ __device__ __forceinline__ void reduce_segment(const SegParams<T> &p, size_t seg
         }
     }
     auto valid = [&](int i) -> bool { return FULLSEG || ((vmask >> i) & 1u); };
-    auto pos_of = [&](int i) -> unsigned { return ((i / VN) * RB + t) * VN + (i % VN); };
+    auto pos_of = [&](int i) -> unsigned { return ((i / VN) * 64 + lane) * VN + (i % VN); };
 
     // ---- elementwise chain, one rounding per step (uniform branch per step)
     for (int k = 0; k < p.chain.len; ++k) {
@@ -286,29 +282,37 @@ __device__ __forceinline__ void reduce_segment(const SegParams<T> &p, size_t seg
         for (int i = 0; i < M; ++i) y[i] = apply_sop<T>(op, s, y[i]);
     }
 
-    // ---- per-thread statistics straight from registers; nothing here waits for other threads
-    const int nt = FULLSEG ? M : __builtin_popcount(vmask);
+    // ---- sum: pairwise per lane in T, then a double-word FP64 butterfly
     const T st = pairwise<M, 0, T>([&](int i) { return valid(i) ? y[i] : (T)0; });
     unsigned nan = 0;
     if (__builtin_isnan(st)) {  // rare: tell a NaN element apart from inf - inf
 #pragma unroll
         for (int i = 0; i < M; ++i) nan |= (valid(i) && __builtin_isnan(y[i])) ? 1u : 0u;
     }
-    Part a = part_empty();
-    a.n = (double)nt;
-    a.hi = (double)st;
-    a.nan = nan;
-    if ((FLAGS & F_M2) && nt > 0) {
-        // mean of M elements: an exact power-of-two scaling for full segments
-        const T mt = FULLSEG ? rn_mul(st, (T)(1.0 / M)) : rn_div(st, (T)nt);
+    double hi = (double)st, lo = 0.0;
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        const double bh = shfl_xor(hi, m), bl = shfl_xor(lo, m);
+        dw_add(hi, lo, bh, bl);
+    }
+    const double n = (double)len;
+    // ---- second moment about the wavefront mean (values still in registers)
+    double m2 = 0.0;
+    if (FLAGS & F_M2) {
+        const double mean = FULLSEG ? (hi + lo) * (1.0 / WSEG) : (hi + lo) / n;
+        const T mt = (T)mean;
         const T qt = pairwise<M, 0, T>([&](int i) {
             const T d = valid(i) ? rn_sub(y[i], mt) : (T)0;
             return rn_mul(d, d);
         });
-        // qt is taken about the rounded mean mt; shift it to the mean st/nt: Q = M2 + n (mt - mean)^2
-        const double dm = (double)mt - (double)st / (double)nt;
-        a.m2 = fmax((double)qt - (double)nt * dm * dm, 0.0);
+        double q = (double)qt;
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) q += shfl_xor(q, m);
+        // q is taken about mt, the mean rounded to T: Q = M2 + n (mt - mean)^2
+        const double dm = (double)mt - mean;
+        m2 = fmax(q - n * dm * dm, 0.0);
     }
+    // ---- extremum winners
     T vmax = -dev_inf<T>(), vmin = dev_inf<T>();
     uint64_t imax = NO_INDEX, imin = NO_INDEX;
     if (FLAGS & F_MAX) {
@@ -324,6 +328,7 @@ __device__ __forceinline__ void reduce_segment(const SegParams<T> &p, size_t seg
             for (int i = 0; i < M; ++i) vmax = (i == first) ? y[i] : vmax;  // keeps the sign of a zero
             imax = flat0 + pos_of(first);
         }
+        wave_arg<T, true>(vmax, imax);
     }
     if (FLAGS & F_MIN) {
         T m = dev_inf<T>();
@@ -338,67 +343,47 @@ __device__ __forceinline__ void reduce_segment(const SegParams<T> &p, size_t seg
             for (int i = 0; i < M; ++i) vmin = (i == first) ? y[i] : vmin;
             imin = flat0 + pos_of(first);
         }
+        wave_arg<T, false>(vmin, imin);
     }
-    const unsigned first_nan = (c == 0 && t == 0 && __builtin_isnan(y[0])) ? 1u : 0u;
-
-    // ---- wavefront combine (fixed xor butterfly; lane 0's result is the one used)
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) {
-        Part b = part_empty();
-        b.n = shfl_xor(a.n, m);
-        b.hi = shfl_xor(a.hi, m);
-        b.lo = shfl_xor(a.lo, m);
-        if (FLAGS & F_M2) b.m2 = shfl_xor(a.m2, m);
-        part_merge<FLAGS & F_M2>(a, b);
-    }
-    a.nan = __any((int)a.nan) ? 1u : 0u;
-    if (FLAGS & F_MAX) wave_arg<T, true>(vmax, imax);
-    if (FLAGS & F_MIN) wave_arg<T, false>(vmin, imin);
+    nan = __any((int)nan) ? 1u : 0u;
+    const unsigned first_nan = (c == 0 && __builtin_isnan(y[0])) ? 1u : 0u;  // lane 0 holds element 0
     if (lane == 0) {
-        ws.n[w] = a.n; ws.hi[w] = a.hi; ws.lo[w] = a.lo; ws.m2[w] = a.m2; ws.nan[w] = a.nan;
-        if (FLAGS & F_MAX) { ws.vmax[w] = vmax; ws.imax[w] = imax; }
-        if (FLAGS & F_MIN) { ws.vmin[w] = vmin; ws.imin[w] = imin; }
+        const Parts &P = p.l1;
+        P.n[seg] = n;
+        P.hi[seg] = hi;
+        P.lo[seg] = lo;
+        P.m2[seg] = m2;
+        P.vmax[seg] = (double)vmax;
+        P.vmin[seg] = (double)vmin;
+        P.imax[seg] = imax;
+        P.imin[seg] = imin;
+        P.flags[seg] = nan | (first_nan << 1);
     }
-    __syncthreads();  // the only barrier of the segment
-    if (t != 0) return;
-    // ---- workgroup combine in wave order, by thread 0
-    Part tot = part_empty();
-#pragma unroll
-    for (int k = 0; k < WAVES; ++k) {
-        Part b = part_empty();
-        b.n = ws.n[k]; b.hi = ws.hi[k]; b.lo = ws.lo[k]; b.m2 = ws.m2[k]; b.nan = ws.nan[k];
-        if (FLAGS & F_MAX) { b.vmax = (double)ws.vmax[k]; b.imax = ws.imax[k]; }
-        if (FLAGS & F_MIN) { b.vmin = (double)ws.vmin[k]; b.imin = ws.imin[k]; }
-        part_merge<FLAGS>(tot, b);
-    }
-    if (p.spr == 1 && p.want_row) write_outputs<T>(p.row, r, tot, p.ddof, first_nan, r * p.cols);
-    if (p.need_partials) part_store(p.l1, seg, tot, first_nan);
 }
 
 template <typename T, int VPT, int FLAGS, bool VEC>
 __global__ __launch_bounds__(RB) void seg_reduce_kernel(SegParams<T> p) {
-    constexpr int SEG = RB * VPT * Vec16<T>::N;
-    __shared__ WaveSlots<T> ws[2];
-    int parity = 0;
-    for (size_t seg = blockIdx.x; seg < p.nseg; seg += gridDim.x, parity ^= 1) {
+    constexpr int WSEG = 64 * VPT * Vec16<T>::N;
+    const size_t nwaves = (size_t)gridDim.x * WAVES;
+    for (size_t seg = blockIdx.x * (size_t)WAVES + (threadIdx.x >> 6); seg < p.nseg; seg += nwaves) {
         const size_t c = seg % p.spr;
-        if (c * SEG + SEG <= p.cols) {
-            reduce_segment<T, VPT, FLAGS, true, VEC>(p, seg, ws[parity]);
+        if (c * WSEG + WSEG <= p.cols) {
+            reduce_segment<T, VPT, FLAGS, true, VEC>(p, seg);
         } else {
-            reduce_segment<T, VPT, FLAGS, false, VEC>(p, seg, ws[parity]);
+            reduce_segment<T, VPT, FLAGS, false, VEC>(p, seg);
         }
     }
 }
 
 // ---------------- finish, level 1: one wavefront per group of segment partials ----------------
-// Group g is row g's segments when rows need combining (spr > 1), else GROUP consecutive
-// segments. Lane l folds partials lo+l, lo+l+64, ... in order, then the xor butterfly.
+// Group g is row g's segments when row results are wanted, else GROUP consecutive segments.
+// Lane l folds partials lo+l, lo+l+64, ... in order, then the xor butterfly.
 template <typename T, int FLAGS>
 __global__ __launch_bounds__(256) void finish_groups_kernel(SegParams<T> p) {
     const size_t g = (blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     if (g >= p.ngroups) return;  // whole wavefronts exit together
-    const bool rowmode = p.want_row && p.spr > 1;
+    const bool rowmode = p.want_row;
     const size_t per = rowmode ? p.spr : (size_t)GROUP;
     const size_t lo = g * per, hi = min(p.nseg, lo + per);
     Part a = part_empty();
@@ -471,7 +456,7 @@ static bool any_out(const OutPtrs<T> &o) {
 
 template <typename T, int VPT, int FLAGS>
 static nrk_status launch_flags(SegParams<T> &p, hipStream_t stream) {
-    const int grid = (int)std::min<size_t>(p.nseg, (size_t)cu_count() * reduce_blocks_per_cu());
+    const int grid = (int)std::min<size_t>(ceil_div(p.nseg, (size_t)WAVES), (size_t)cu_count() * reduce_blocks_per_cu());
     if (((reinterpret_cast<uintptr_t>(p.x) % 16) == 0) && (p.cols % Vec16<T>::N == 0))
         hipLaunchKernelGGL((seg_reduce_kernel<T, VPT, FLAGS, true>), dim3(grid), dim3(RB), 0, stream, p);
     else
@@ -511,7 +496,7 @@ template <typename T>
 nrk_status seg_reduce(const Chain &chain, const T *x, size_t rows, size_t cols, double ddof,
                       const nrk_reduce_outputs *row_out, const nrk_reduce_outputs *full_out, hipStream_t stream) {
     constexpr int VPT = 8;
-    constexpr int SEG = RB * VPT * Vec16<T>::N;
+    constexpr int WSEG = 64 * VPT * Vec16<T>::N;  // elements per wavefront segment
     NRK_TRY(ensure_device());
     SegParams<T> p{};
     p.row = to_ptrs<T>(row_out);
@@ -542,21 +527,18 @@ nrk_status seg_reduce(const Chain &chain, const T *x, size_t rows, size_t cols, 
     p.cols = cols;
     p.chain = chain;
     p.ddof = ddof;
-    p.spr = ceil_div(cols, SEG);
+    p.spr = ceil_div(cols, WSEG);
     p.nseg = rows * p.spr;
     if (p.nseg >= (size_t)UINT32_MAX) return fail(NRK_ERR_INVALID_ARGUMENT, "too many segments (%zu)", p.nseg);
-    const bool rowmode = p.want_row && p.spr > 1;
-    p.need_partials = p.want_full || rowmode;
-    p.ngroups = rowmode ? rows : ceil_div(p.nseg, (size_t)GROUP);
-    const size_t rec = 8 * 8 + 4 + 9 * 256;
-    const size_t pbytes = p.need_partials ? (p.nseg + p.ngroups) * rec + 4096 : 0;
+    p.need_partials = 1;  // every result goes through the finish kernels
+    p.ngroups = p.want_row ? rows : ceil_div(p.nseg, (size_t)GROUP);
+    const size_t rec = 8 * 8 + 4;  // bytes of one partial record
+    const size_t pbytes = (p.nseg + p.ngroups) * rec + 2 * 9 * 256 + 4096;
     Scratch s;
     NRK_TRY(get_scratch(stream, 0, pbytes, &s));
-    if (p.need_partials) {
-        char *b = (char *)s.partials;
-        p.l1 = carve_parts(b, p.nseg);
-        p.l2 = carve_parts(b, p.ngroups);
-    }
+    char *b = (char *)s.partials;
+    p.l1 = carve_parts(b, p.nseg);
+    p.l2 = carve_parts(b, p.ngroups);
     const bool m2 = (p.row.var || p.full.var);
     const bool mx = (p.row.max || p.row.argmax || p.full.max || p.full.argmax);
     const bool mn = (p.row.min || p.row.argmin || p.full.min || p.full.argmin);
