@@ -133,6 +133,36 @@ struct Chain {
     int op[NRK_MAX_CHAIN];
     double s[NRK_MAX_CHAIN];
 };
+
+// Apply the whole chain to N register values: one (wavefront-uniform) branch per step, then N
+// straight-line operations, so the op dispatch is not repeated per element.
+template <typename T, int N>
+__device__ __forceinline__ void apply_chain_n(const Chain &c, T (&v)[N]) {
+    for (int k = 0; k < c.len; ++k) {
+        const T s = (T)c.s[k];
+        switch (c.op[k]) {
+#define NRK_CHAIN_CASE(OP, EXPR)                      \
+    case OP:                                          \
+        _Pragma("unroll") for (int i = 0; i < N; ++i) { \
+            const T x = v[i];                         \
+            v[i] = (EXPR);                            \
+        }                                             \
+        break;
+            NRK_CHAIN_CASE(NRK_SOP_ADD, rn_add(x, s))
+            NRK_CHAIN_CASE(NRK_SOP_SUB, rn_sub(x, s))
+            NRK_CHAIN_CASE(NRK_SOP_MUL, rn_mul(x, s))
+            NRK_CHAIN_CASE(NRK_SOP_DIV, rn_div(x, s))
+            NRK_CHAIN_CASE(NRK_SOP_RSUB, rn_sub(s, x))
+            NRK_CHAIN_CASE(NRK_SOP_RDIV, rn_div(s, x))
+            NRK_CHAIN_CASE(NRK_SOP_NEG, -x)
+            NRK_CHAIN_CASE(NRK_SOP_ABS, dev_fabs(x))
+            NRK_CHAIN_CASE(NRK_SOP_SQUARE, rn_mul(x, x))
+            NRK_CHAIN_CASE(NRK_SOP_SQRT, rn_sqrt(x))
+#undef NRK_CHAIN_CASE
+            default: break;
+        }
+    }
+}
 nrk_status make_chain(const nrk_scalar_op *chain, int32_t chain_len, Chain *out);
 
 }  // namespace nrk

@@ -6,7 +6,9 @@
 // expr/fused_eval.rs:15-21).
 //
 // Both kernels stream 16-byte vectors with non-temporal loads, grid-stride over the array, and
-// fall back to scalar accesses for a misaligned base or the tail.
+// fall back to scalar accesses for a misaligned base or the tail. The op dispatch is hoisted out
+// of the element loops: the binary op is a template parameter, and a scalar chain is applied
+// one step at a time to a whole batch of registers (one uniform branch per step).
 #include "common.hpp"
 
 namespace nrk {
@@ -14,24 +16,15 @@ namespace nrk {
 constexpr int EB = 256;  // threads per workgroup
 constexpr int EUNROLL = 4;
 
-template <typename T>
-__device__ __forceinline__ T apply_bop(int op, T a, T b) {
-    switch (op) {
-        case NRK_BOP_ADD: return rn_add(a, b);
-        case NRK_BOP_SUB: return rn_sub(a, b);
-        case NRK_BOP_MUL: return rn_mul(a, b);
-        case NRK_BOP_DIV: return rn_div(a, b);
-        case NRK_BOP_REM: return __builtin_fmod(a, b);  // exact, truncated like Rust's %
-        case NRK_BOP_MAXIMUM: return a > b ? a : b;      // ufuncs.rs:169
-        case NRK_BOP_MINIMUM: return a < b ? a : b;      // ufuncs.rs:173
-        default: return a;
-    }
-}
-
-template <typename T>
-__device__ __forceinline__ T apply_chain(const Chain &c, T v) {
-    for (int k = 0; k < c.len; ++k) v = apply_sop<T>(c.op[k], (T)c.s[k], v);
-    return v;
+template <typename T, int OP>
+__device__ __forceinline__ T apply_bop(T a, T b) {
+    if constexpr (OP == NRK_BOP_ADD) return rn_add(a, b);
+    else if constexpr (OP == NRK_BOP_SUB) return rn_sub(a, b);
+    else if constexpr (OP == NRK_BOP_MUL) return rn_mul(a, b);
+    else if constexpr (OP == NRK_BOP_DIV) return rn_div(a, b);
+    else if constexpr (OP == NRK_BOP_REM) return __builtin_fmod(a, b);  // exact, truncated like Rust's %
+    else if constexpr (OP == NRK_BOP_MAXIMUM) return a > b ? a : b;     // ufuncs.rs:169
+    else return a < b ? a : b;                                           // ufuncs.rs:173 (MINIMUM)
 }
 
 template <typename T>
@@ -47,29 +40,44 @@ __global__ __launch_bounds__(EB) void map_kernel(Chain chain, const T *x, T *y, 
         V *yv = reinterpret_cast<V *>(y);
         size_t i = tid;
         for (; i + (EUNROLL - 1) * nthreads < nv; i += EUNROLL * nthreads) {
-            V v[EUNROLL];
-#pragma unroll
-            for (int u = 0; u < EUNROLL; ++u) v[u] = __builtin_nontemporal_load(xv + i + u * nthreads);
+            T r[EUNROLL * VN];
 #pragma unroll
             for (int u = 0; u < EUNROLL; ++u) {
+                const V v = __builtin_nontemporal_load(xv + i + u * nthreads);
 #pragma unroll
-                for (int e = 0; e < VN; ++e) v[u][e] = apply_chain(chain, v[u][e]);
-                __builtin_nontemporal_store(v[u], yv + i + u * nthreads);
+                for (int e = 0; e < VN; ++e) r[u * VN + e] = v[e];
+            }
+            apply_chain_n(chain, r);
+#pragma unroll
+            for (int u = 0; u < EUNROLL; ++u) {
+                V v;
+#pragma unroll
+                for (int e = 0; e < VN; ++e) v[e] = r[u * VN + e];
+                __builtin_nontemporal_store(v, yv + i + u * nthreads);
             }
         }
         for (; i < nv; i += nthreads) {
-            V v = __builtin_nontemporal_load(xv + i);
+            const V v = __builtin_nontemporal_load(xv + i);
+            T r[VN];
 #pragma unroll
-            for (int e = 0; e < VN; ++e) v[e] = apply_chain(chain, v[e]);
-            __builtin_nontemporal_store(v, yv + i);
+            for (int e = 0; e < VN; ++e) r[e] = v[e];
+            apply_chain_n(chain, r);
+            V w;
+#pragma unroll
+            for (int e = 0; e < VN; ++e) w[e] = r[e];
+            __builtin_nontemporal_store(w, yv + i);
         }
         done = nv * VN;
     }
-    for (size_t i = done + tid; i < n; i += nthreads) y[i] = apply_chain(chain, x[i]);
+    for (size_t i = done + tid; i < n; i += nthreads) {
+        T r[1] = {x[i]};
+        apply_chain_n(chain, r);
+        y[i] = r[0];
+    }
 }
 
-template <typename T>
-__global__ __launch_bounds__(EB) void zip_kernel(int op, const T *a, const T *b, T *y, size_t n, int vec_ok) {
+template <typename T, int OP>
+__global__ __launch_bounds__(EB) void zip_kernel(const T *a, const T *b, T *y, size_t n, int vec_ok) {
     typedef typename Vec16<T>::type V;
     constexpr int VN = Vec16<T>::N;
     const size_t tid = blockIdx.x * (size_t)EB + threadIdx.x;
@@ -91,7 +99,7 @@ __global__ __launch_bounds__(EB) void zip_kernel(int op, const T *a, const T *b,
 #pragma unroll
             for (int u = 0; u < EUNROLL; ++u) {
 #pragma unroll
-                for (int e = 0; e < VN; ++e) va[u][e] = apply_bop(op, va[u][e], vb[u][e]);
+                for (int e = 0; e < VN; ++e) va[u][e] = apply_bop<T, OP>(va[u][e], vb[u][e]);
                 __builtin_nontemporal_store(va[u], yv + i + u * nthreads);
             }
         }
@@ -99,12 +107,12 @@ __global__ __launch_bounds__(EB) void zip_kernel(int op, const T *a, const T *b,
             V va = __builtin_nontemporal_load(av + i);
             V vb = __builtin_nontemporal_load(bv + i);
 #pragma unroll
-            for (int e = 0; e < VN; ++e) va[e] = apply_bop(op, va[e], vb[e]);
+            for (int e = 0; e < VN; ++e) va[e] = apply_bop<T, OP>(va[e], vb[e]);
             __builtin_nontemporal_store(va, yv + i);
         }
         done = nv * VN;
     }
-    for (size_t i = done + tid; i < n; i += nthreads) y[i] = apply_bop(op, a[i], b[i]);
+    for (size_t i = done + tid; i < n; i += nthreads) y[i] = apply_bop<T, OP>(a[i], b[i]);
 }
 
 static int stream_grid(size_t n_vec) {
@@ -133,7 +141,19 @@ nrk_status zip(int op, const T *a, const T *b, T *y, size_t n, hipStream_t strea
     if (!a || !b || !y) return fail(NRK_ERR_INVALID_ARGUMENT, "a, b or y is NULL");
     const int vec_ok = (reinterpret_cast<uintptr_t>(a) % 16 == 0) && (reinterpret_cast<uintptr_t>(b) % 16 == 0) &&
                        (reinterpret_cast<uintptr_t>(y) % 16 == 0);
-    hipLaunchKernelGGL(zip_kernel<T>, dim3(stream_grid(n / Vec16<T>::N + 1)), dim3(EB), 0, stream, op, a, b, y, n, vec_ok);
+    const dim3 grid(stream_grid(n / Vec16<T>::N + 1));
+    switch (op) {
+#define NRK_ZIP_CASE(OP) \
+    case OP: hipLaunchKernelGGL((zip_kernel<T, OP>), grid, dim3(EB), 0, stream, a, b, y, n, vec_ok); break;
+        NRK_ZIP_CASE(NRK_BOP_ADD)
+        NRK_ZIP_CASE(NRK_BOP_SUB)
+        NRK_ZIP_CASE(NRK_BOP_MUL)
+        NRK_ZIP_CASE(NRK_BOP_DIV)
+        NRK_ZIP_CASE(NRK_BOP_REM)
+        NRK_ZIP_CASE(NRK_BOP_MAXIMUM)
+        NRK_ZIP_CASE(NRK_BOP_MINIMUM)
+#undef NRK_ZIP_CASE
+    }
     NRK_HIP(hipGetLastError());
     return NRK_OK;
 }

@@ -9,39 +9,53 @@
 //   math/aggregation.rs:244-261     axis max/min: NaN short-circuits to NaN
 //   math/statistics/mod.rs:1403-1494 axis var = sum((x - mean)^2) / (len - ddof)
 //
-// Work decomposition. Row r of `cols` elements is cut into fixed segments of WSEG elements
-// (64 lanes x VPT 16-byte vectors), each owned by ONE wavefront of the persistent streaming
-// kernel. The wavefront reduces its segment from registers -- pairwise sums per lane, the
-// segment mean, then squared deviations about it, combined across lanes with xor butterflies of
-// FP64 additions -- and writes one partial record. Wavefronts never wait for each other (no
-// barrier, no LDS). Two small dependent kernels on the same stream finish: one wavefront per
-// row (or per group of 64 segments) combines segment partials and writes row results, then one
-// 1024-thread workgroup combines those for the whole-matrix result, using Chan, Golub & LeVeque's
-// pairwise update for the second moment. The kernel boundaries provide the cross-workgroup
-// visibility; there is no fence or atomic in the stream (an agent-scope release/acquire writes
-// back / invalidates each XCD's L2 on gfx950, which cost 20x the stream itself when it was done
-// per segment).
+// Work decomposition. Row r of `cols` elements is cut into segments of WSEG elements (64 lanes x
+// VPT 16-byte vectors) and consecutive segments of a row are grouped into chunks of `cl`
+// segments (a whole row when there are enough rows to fill the GPU). One wavefront owns a chunk:
+// each lane keeps running accumulators over its elements of every segment of the chunk (count,
+// double-word FP64 sum, second moment, max/min with the winner's position), so the per-element
+// work is a handful of VALU operations and nothing crosses lanes until the chunk ends. Then one
+// xor butterfly merges the 64 lanes, and lane `k` of the wavefront keeps the record of its k-th
+// chunk in registers; the records (and the row results when a chunk is a row) are stored 64 at a
+// time, so the streaming loop issues loads only. Wavefronts never wait for each other: no
+// barrier, no LDS, no atomic, no fence. Chunk records are combined by a fixed tree of small
+// dependent kernels on the same stream (up to 256 records per wavefront per level): rows split
+// into several chunks first, then the whole-matrix result. The kernel boundaries provide the
+// cross-workgroup visibility (an agent-scope release/acquire writes back / invalidates each XCD's
+// L2 on gfx950, which cost 20x the stream itself when it was done per segment).
 //
-// Determinism. Every combine is a fixed tree: pairwise within a lane, an xor butterfly within
-// a wavefront (lane 0's value is used), fixed per-lane strided sequences and butterflies in the
-// finish. None of it depends on the grid size or on scheduling, so results are bit-identical
-// across launches and devices, and across the tuner's grid-size configurations.
+// Determinism. The chunking depends on the shape only, each chunk is reduced in a fixed order
+// by one wavefront (per-lane sequences, then a butterfly whose lane-0 value is used) and the
+// record tree is fixed, so results are bit-identical across launches, grid sizes and devices.
 //
-// Precision. Per-lane partials in T (pairwise), everything after in FP64 (double-word for
-// sums); the second moment is combined with Chan's formula. max/min/argmax/argmin are exact:
-// the winner is the largest value, ties going to the lowest index, and the value written is the
-// element itself, so even the sign of a zero maximum matches the reference's left-to-right fold.
+// Precision. Per lane and segment: pairwise sums in T; across segments and lanes: FP64
+// double-word sums, and the second moment is combined with Chan, Golub & LeVeque's pairwise
+// update (each segment's squared deviations are taken about its own lane mean). max/min/argmax/
+// argmin are exact: a strict comparison scan per lane in index order (NaN never wins, ties keep
+// the earlier element), then the larger value with ties to the lower index, and the value
+// written is the element itself, so even the sign of a zero maximum matches the reference's
+// left-to-right fold.
 #include "common.hpp"
+
+#include <algorithm>
 
 namespace nrk {
 
 constexpr int RB = 256;  // threads per streaming workgroup
 constexpr int WAVES = RB / 64;
 constexpr uint64_t NO_INDEX = ~0ull;
-constexpr int GROUP = 64;    // segments per finish wavefront when rows need no combining
-constexpr int FB = 1024;     // threads of the final workgroup
+constexpr unsigned NO_CODE = 0xffffffffu;
+constexpr int FB = 1024;                // threads of the single-workgroup finish kernels
+constexpr size_t CHUNK_TARGET = 8192;   // chunks per matrix when rows are few (about one per wavefront)
 
 enum : int { F_M2 = 1, F_MAX = 2, F_MIN = 4 };
+
+// Elements per lane in one wavefront segment of the f32/f64 streaming kernel (16: 4 16-byte
+// vectors of f32 or 8 of f64 per lane; a segment is 4 KiB of f32 or 8 KiB of f64). It fixes the
+// segment boundaries and hence the summation tree, so results are reproducible for a given build.
+#ifndef NRK_REDUCE_M
+#define NRK_REDUCE_M 16
+#endif
 
 // Persistent-grid size of the streaming kernels (workgroups per CU). Result bits do not depend on
 // it, only speed, so the tuner may choose it per shape.
@@ -65,13 +79,33 @@ struct Parts {
 template <typename T>
 struct SegParams {
     const T *x;
-    size_t rows, cols, spr, nseg;
+    size_t rows, cols;
+    size_t spr;     // segments per row
+    size_t cl;      // segments per chunk (the last chunk of a row may hold fewer)
+    size_t cpr;     // chunks per row
+    size_t nchunk;  // rows * cpr
     Chain chain;
     double ddof;
-    int want_row, want_full, need_partials;
-    size_t ngroups;  // first-level finish groups (rows when spr > 1 and rows are wanted)
-    Parts l1, l2;    // segment partials, group partials
-    OutPtrs<T> row, full;
+    int row_direct;  // cpr == 1 and row results wanted: the stream writes them
+    int need_l1;     // chunk records are read by a tree level
+    Parts l1;        // chunk records, chunk c = row * cpr + q
+    OutPtrs<T> row;
+};
+
+// One level of the record tree: `nsets` sets of `per` consecutive input records; a wavefront
+// merges up to TREE_SPAN consecutive records of one set into one output record.
+constexpr int TREE_LOADS = 4;               // records per lane per level, loaded together
+constexpr int TREE_SPAN = 64 * TREE_LOADS;  // records per wavefront per level
+template <typename T>
+struct TreeParams {
+    Parts in, out;
+    size_t nsets, per, per_out;  // per_out = ceil(per / TREE_SPAN)
+    int results;                 // per_out == 1: write the set's results to `o`
+    int keep;                    // write output records (a later level reads them)
+    int rowmode;                 // set r is row r: flat index base r * cols
+    size_t cols;
+    double ddof;
+    OutPtrs<T> o;
 };
 
 // ---------------- small device utilities ----------------
@@ -100,8 +134,17 @@ __device__ __forceinline__ uint64_t shfl_xor(uint64_t v, int m) {
     return ((uint64_t)hi << 32) | lo;
 }
 
+// Lane 0's value, for every lane (the wavefront is fully active where these are used).
+__device__ __forceinline__ unsigned lane0(unsigned v) { return (unsigned)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ __forceinline__ uint64_t lane0(uint64_t v) {
+    return ((uint64_t)lane0((unsigned)(v >> 32)) << 32) | lane0((unsigned)(v & 0xffffffffu));
+}
+__device__ __forceinline__ double lane0(double v) {
+    return __builtin_bit_cast(double, lane0(__builtin_bit_cast(uint64_t, v)));
+}
+
 // (value, index) winners: larger (smaller) value wins, equal values go to the lower index.
-// Values are never NaN here (NaN elements are excluded before this point).
+// Values are never NaN here (NaN elements never become winners).
 template <typename T>
 __device__ __forceinline__ void take_max(T &v, uint64_t &i, T bv, uint64_t bi) {
     if (bv > v || (bv == v && bi < i)) { v = bv; i = bi; }
@@ -109,16 +152,6 @@ __device__ __forceinline__ void take_max(T &v, uint64_t &i, T bv, uint64_t bi) {
 template <typename T>
 __device__ __forceinline__ void take_min(T &v, uint64_t &i, T bv, uint64_t bi) {
     if (bv < v || (bv == v && bi < i)) { v = bv; i = bi; }
-}
-
-template <typename T, bool IS_MAX>
-__device__ __forceinline__ void wave_arg(T &v, uint64_t &i) {
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) {
-        T bv = shfl_xor(v, m);
-        uint64_t bi = shfl_xor(i, m);
-        if (IS_MAX) take_max(v, i, bv, bi); else take_min(v, i, bv, bi);
-    }
 }
 
 // Pairwise (balanced binary tree) sum of f(OFF) .. f(OFF+N-1), written depth-first.
@@ -158,9 +191,9 @@ __device__ __forceinline__ void part_merge(Part &a, const Part &b) {
         return;
     }
     if (FLAGS & F_M2) {
-        const double nt = a.n + b.n;
-        const double d = (a.hi + a.lo) / a.n - (b.hi + b.lo) / b.n;
-        a.m2 = a.m2 + b.m2 + d * d * (a.n * b.n / nt);
+        // (na nb / n)(mean_a - mean_b)^2 = (nb Sa - na Sb)^2 / (na nb n): one division
+        const double d = b.n * (a.hi + a.lo) - a.n * (b.hi + b.lo);
+        a.m2 = a.m2 + b.m2 + d * d / (a.n * b.n * (a.n + b.n));
     }
     dw_add(a.hi, a.lo, b.hi, b.lo);
     a.n += b.n;
@@ -190,22 +223,30 @@ __device__ __forceinline__ void wave_merge(Part &a) {
     }
 }
 
+// Record I/O touches only the fields the op set uses.
+template <int FLAGS>
 __device__ __forceinline__ Part part_load(const Parts &P, size_t s) {
-    Part a;
-    a.n = P.n[s]; a.hi = P.hi[s]; a.lo = P.lo[s]; a.m2 = P.m2[s];
-    a.vmax = P.vmax[s]; a.vmin = P.vmin[s]; a.imax = P.imax[s]; a.imin = P.imin[s];
+    Part a = part_empty();
+    a.n = P.n[s]; a.hi = P.hi[s]; a.lo = P.lo[s];
+    if (FLAGS & F_M2) a.m2 = P.m2[s];
+    if (FLAGS & F_MAX) { a.vmax = P.vmax[s]; a.imax = P.imax[s]; }
+    if (FLAGS & F_MIN) { a.vmin = P.vmin[s]; a.imin = P.imin[s]; }
     a.nan = P.flags[s] & 1u;
     return a;
 }
 
+template <int FLAGS>
 __device__ __forceinline__ void part_store(const Parts &P, size_t s, const Part &a, unsigned first_nan) {
-    P.n[s] = a.n; P.hi[s] = a.hi; P.lo[s] = a.lo; P.m2[s] = a.m2;
-    P.vmax[s] = a.vmax; P.vmin[s] = a.vmin; P.imax[s] = a.imax; P.imin[s] = a.imin;
+    P.n[s] = a.n; P.hi[s] = a.hi; P.lo[s] = a.lo;
+    if (FLAGS & F_M2) P.m2[s] = a.m2;
+    if (FLAGS & F_MAX) { P.vmax[s] = a.vmax; P.imax[s] = a.imax; }
+    if (FLAGS & F_MIN) { P.vmin[s] = a.vmin; P.imin[s] = a.imin; }
     P.flags[s] = a.nan | (first_nan << 1);
 }
 
 // Write one set of results. `index_base` is subtracted from the flat winner indices (a row's
-// first flat index for row results, 0 for whole-matrix results).
+// first flat index for row results, 0 for whole-matrix results). No winner means every element
+// is -inf (+inf) or NaN; the first one is then the answer, and index 0 is its position.
 template <typename T>
 __device__ __forceinline__ void write_outputs(const OutPtrs<T> &o, size_t at, const Part &a, double ddof, unsigned first_nan,
                                               uint64_t index_base) {
@@ -220,203 +261,313 @@ __device__ __forceinline__ void write_outputs(const OutPtrs<T> &o, size_t at, co
 }
 
 // ---------------- the streaming kernel ----------------
-// A segment is WSEG = 64 lanes x VPT 16-byte vectors of one row, owned by ONE wavefront: lane l
-// holds element i = j*VN + e at segment position (j*64 + l)*VN + e. The wavefront computes its
-// count, double-word sum, M2 (two passes over registers: the wavefront mean first, then the
-// squared deviations about it, both combined with xor butterflies of plain FP64 additions) and
-// extremum winners, and lane 0 writes one partial record. Wavefronts never wait for each other:
-// the kernel has no barrier and no LDS.
-template <typename T, int VPT, int FLAGS, bool FULLSEG, bool VEC>
-__device__ __forceinline__ void reduce_segment(const SegParams<T> &p, size_t seg) {
+// A segment is WSEG = 64 lanes x VPT 16-byte vectors of one row: lane l holds element
+// i = j*VN + e at segment position (j*64 + l)*VN + e, so every load instruction of the
+// wavefront reads 1 KiB of consecutive memory.
+
+// Per-lane running state over the segments of one chunk.
+template <typename T>
+struct LaneAcc {
+    double n, hi, lo, m2;
+    T vmax, vmin;
+    unsigned cmax, cmin;  // winner codes: segment-in-chunk * M + element slot; NO_CODE: none yet
+    unsigned nan;
+};
+
+// Issue the loads of the segment at `base` (len elements of the row remain there) into y, zero
+// outside the row, and record which elements exist. VEC: len % VN == 0, so whole vectors.
+template <typename T, int VPT, bool VEC>
+__device__ __forceinline__ void seg_load(const T *base, size_t len, T (&y)[VPT * Vec16<T>::N], unsigned &vmask) {
     typedef typename Vec16<T>::type V;
     constexpr int VN = Vec16<T>::N;
     constexpr int M = VPT * VN;  // elements per lane (<= 32: one validity bit each)
-    constexpr int WSEG = 64 * M;
     static_assert(M <= 32, "validity mask holds 32 elements");
     const unsigned lane = threadIdx.x & 63;
-    const size_t r = seg / p.spr, c = seg - r * p.spr;
-    const size_t col0 = c * WSEG;
-    const unsigned len = FULLSEG ? (unsigned)WSEG : (unsigned)min((size_t)WSEG, p.cols - col0);
-    const size_t flat0 = r * p.cols + col0;
-    const T *base = p.x + flat0;  // uniform across the wavefront
-
-    // ---- load
-    T y[M];
-    unsigned vmask = ~0u;
+    vmask = 0u;
     if (VEC) {
-        // aligned rows: 16-byte loads; a tail segment predicates whole vectors (len % VN == 0)
         const V *bv = reinterpret_cast<const V *>(base);
-        const unsigned nvec = len / VN;
-        if (!FULLSEG) vmask = 0u;
+        const size_t nvec = len / VN;
 #pragma unroll
         for (int j = 0; j < VPT; ++j) {
             const unsigned jv = j * 64 + lane;
-            const bool ok = FULLSEG || jv < nvec;
+            const bool ok = jv < nvec;
             V v = V{};
             if (ok) v = __builtin_nontemporal_load(bv + jv);
 #pragma unroll
             for (int e = 0; e < VN; ++e) y[j * VN + e] = v[e];
-            if (!FULLSEG) vmask |= (ok ? ((1u << VN) - 1u) : 0u) << (j * VN);
+            vmask |= (ok ? ((1u << VN) - 1u) : 0u) << (j * VN);
         }
     } else {
-        if (!FULLSEG) vmask = 0u;
 #pragma unroll
         for (int j = 0; j < VPT; ++j) {
 #pragma unroll
             for (int e = 0; e < VN; ++e) {
                 const unsigned pos = (j * 64 + lane) * VN + e;
-                const bool ok = FULLSEG || pos < len;
+                const bool ok = pos < len;
                 y[j * VN + e] = ok ? base[pos] : (T)0;
-                if (!FULLSEG) vmask |= (ok ? 1u : 0u) << (j * VN + e);
+                vmask |= (ok ? 1u : 0u) << (j * VN + e);
             }
         }
     }
+}
+
+// Fold one loaded segment (segment `sl` of its chunk) into the lane accumulators. FULLSEG: every
+// element exists (vmask is all ones).
+template <typename T, int VPT, int FLAGS, bool FULLSEG>
+__device__ __forceinline__ void seg_body(const Chain &chain, T (&y)[VPT * Vec16<T>::N], unsigned vmask, unsigned sl,
+                                         LaneAcc<T> &a, unsigned &fnan, bool row_start) {
+    constexpr int M = VPT * Vec16<T>::N;
     auto valid = [&](int i) -> bool { return FULLSEG || ((vmask >> i) & 1u); };
-    auto pos_of = [&](int i) -> unsigned { return ((i / VN) * 64 + lane) * VN + (i % VN); };
 
-    // ---- elementwise chain, one rounding per step (uniform branch per step)
-    for (int k = 0; k < p.chain.len; ++k) {
-        const int op = p.chain.op[k];
-        const T s = (T)p.chain.s[k];
-#pragma unroll
-        for (int i = 0; i < M; ++i) y[i] = apply_sop<T>(op, s, y[i]);
-    }
+    // ---- elementwise chain, one rounding per step
+    apply_chain_n(chain, y);
+    if (row_start) fnan = __builtin_isnan(y[0]) ? 1u : 0u;  // lane 0's y[0] is the row's element 0
 
-    // ---- sum: pairwise per lane in T, then a double-word FP64 butterfly
+    // ---- sum: pairwise in T over the lane's elements, then into the lane's FP64 double word
     const T st = pairwise<M, 0, T>([&](int i) { return valid(i) ? y[i] : (T)0; });
-    unsigned nan = 0;
     if (__builtin_isnan(st)) {  // rare: tell a NaN element apart from inf - inf
 #pragma unroll
-        for (int i = 0; i < M; ++i) nan |= (valid(i) && __builtin_isnan(y[i])) ? 1u : 0u;
+        for (int i = 0; i < M; ++i) a.nan |= (valid(i) && __builtin_isnan(y[i])) ? 1u : 0u;
     }
-    double hi = (double)st, lo = 0.0;
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) {
-        const double bh = shfl_xor(hi, m), bl = shfl_xor(lo, m);
-        dw_add(hi, lo, bh, bl);
-    }
-    const double n = (double)len;
-    // ---- second moment about the wavefront mean (values still in registers)
-    double m2 = 0.0;
-    if (FLAGS & F_M2) {
-        const double mean = FULLSEG ? (hi + lo) * (1.0 / WSEG) : (hi + lo) / n;
-        const T mt = (T)mean;
+    const double cnt = FULLSEG ? (double)M : (double)__builtin_popcount(vmask);
+    const double sb = (double)st;
+    // ---- second moment of this segment's lane elements about their own mean, merged (Chan)
+    if ((FLAGS & F_M2) && (FULLSEG || cnt > 0.0)) {
+        const double mb = FULLSEG ? sb * (1.0 / M) : sb / cnt;
+        const T mt = (T)mb;
         const T qt = pairwise<M, 0, T>([&](int i) {
             const T d = valid(i) ? rn_sub(y[i], mt) : (T)0;
             return rn_mul(d, d);
         });
-        double q = (double)qt;
-#pragma unroll
-        for (int m = 32; m >= 1; m >>= 1) q += shfl_xor(q, m);
-        // q is taken about mt, the mean rounded to T: Q = M2 + n (mt - mean)^2
-        const double dm = (double)mt - mean;
-        m2 = fmax(q - n * dm * dm, 0.0);
-    }
-    // ---- extremum winners
-    T vmax = -dev_inf<T>(), vmin = dev_inf<T>();
-    uint64_t imax = NO_INDEX, imin = NO_INDEX;
-    if (FLAGS & F_MAX) {
-        T m = -dev_inf<T>();
-#pragma unroll
-        for (int i = 0; i < M; ++i) m = valid(i) ? dev_fmax(m, y[i]) : m;  // fmax skips NaN
-        int first = -1;
-#pragma unroll
-        for (int i = M - 1; i >= 0; --i) first = (valid(i) && y[i] == m) ? i : first;
-        if (first >= 0) {
-            vmax = m;
-#pragma unroll
-            for (int i = 0; i < M; ++i) vmax = (i == first) ? y[i] : vmax;  // keeps the sign of a zero
-            imax = flat0 + pos_of(first);
+        // qt is taken about mt, the mean rounded to T: Q = M2 + cnt (mt - mean)^2
+        const double dm = (double)mt - mb;
+        double m2b = (double)qt - cnt * dm * dm;
+        m2b = m2b < 0.0 ? 0.0 : m2b;  // rounding can leave a tiny negative; NaN passes through
+        if (a.n == 0.0) {
+            a.m2 = m2b;
+        } else {
+            const double sa = a.hi + a.lo;
+            const double d = cnt * sa - a.n * sb;  // = na nb (mean_a - mean_b)
+            a.m2 = a.m2 + m2b + d * d / (a.n * cnt * (a.n + cnt));
         }
-        wave_arg<T, true>(vmax, imax);
+    }
+    {
+        double s, e;
+        two_sum(a.hi, sb, s, e);
+        e = e + a.lo;
+        a.hi = s + e;
+        a.lo = e - (a.hi - s);
+    }
+    a.n += cnt;
+    // ---- extremum winners: strict comparison in index order (NaN never compares greater)
+    if (FLAGS & F_MAX) {
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            const bool gt = valid(i) && y[i] > a.vmax;
+            a.vmax = gt ? y[i] : a.vmax;
+            a.cmax = gt ? sl * M + i : a.cmax;
+        }
     }
     if (FLAGS & F_MIN) {
-        T m = dev_inf<T>();
 #pragma unroll
-        for (int i = 0; i < M; ++i) m = valid(i) ? dev_fmin(m, y[i]) : m;
-        int first = -1;
-#pragma unroll
-        for (int i = M - 1; i >= 0; --i) first = (valid(i) && y[i] == m) ? i : first;
-        if (first >= 0) {
-            vmin = m;
-#pragma unroll
-            for (int i = 0; i < M; ++i) vmin = (i == first) ? y[i] : vmin;
-            imin = flat0 + pos_of(first);
+        for (int i = 0; i < M; ++i) {
+            const bool lt = valid(i) && y[i] < a.vmin;
+            a.vmin = lt ? y[i] : a.vmin;
+            a.cmin = lt ? sl * M + i : a.cmin;
         }
-        wave_arg<T, false>(vmin, imin);
-    }
-    nan = __any((int)nan) ? 1u : 0u;
-    const unsigned first_nan = (c == 0 && __builtin_isnan(y[0])) ? 1u : 0u;  // lane 0 holds element 0
-    if (lane == 0) {
-        const Parts &P = p.l1;
-        P.n[seg] = n;
-        P.hi[seg] = hi;
-        P.lo[seg] = lo;
-        P.m2[seg] = m2;
-        P.vmax[seg] = (double)vmax;
-        P.vmin[seg] = (double)vmin;
-        P.imax[seg] = imax;
-        P.imin[seg] = imin;
-        P.flags[seg] = nan | (first_nan << 1);
     }
 }
 
-template <typename T, int VPT, int FLAGS, bool VEC>
-__global__ __launch_bounds__(RB) void seg_reduce_kernel(SegParams<T> p) {
+// Position of a wavefront in its work: chunk c = row r, chunk-of-row q, segments [s0, s1).
+struct ChunkPos {
+    size_t c, r, q, s0, s1;
+};
+template <typename T>
+__device__ __forceinline__ ChunkPos chunk_pos(const SegParams<T> &p, size_t c) {
+    ChunkPos k;
+    k.c = c;
+    k.r = c / p.cpr;
+    k.q = c - k.r * p.cpr;
+    k.s0 = k.q * p.cl;
+    k.s1 = min(p.spr, k.s0 + p.cl);
+    return k;
+}
+
+template <typename T>
+__device__ __forceinline__ void acc_reset(LaneAcc<T> &a) {
+    a.n = 0.0; a.hi = 0.0; a.lo = 0.0; a.m2 = 0.0;
+    a.vmax = -dev_inf<T>(); a.vmin = dev_inf<T>();
+    a.cmax = NO_CODE; a.cmin = NO_CODE; a.nan = 0;
+}
+
+// Per-wavefront batch of up to 64 chunk records, kept in LDS (slot k: the k-th chunk of the
+// batch) until they are stored together. Only the owning wavefront touches its slots.
+struct LdsRecs {
+    double n[64], hi[64], lo[64], m2[64], vmax[64], vmin[64];
+    uint64_t imax[64], imin[64];
+    unsigned flags[64];
+};
+struct Batch {
+    LdsRecs *lds;
+    unsigned nb;   // chunks in the batch
+    size_t cbase;  // first chunk of the batch
+};
+
+// Lanes 0..nb-1 store the records of chunks cbase, cbase + W, ... (and the row results when chunks
+// are rows).
+template <typename T, int FLAGS>
+__device__ __forceinline__ void flush_batch(const SegParams<T> &p, Batch &bt, size_t W) {
+    const unsigned lane = threadIdx.x & 63;
+    __builtin_amdgcn_wave_barrier();  // lane 0's slot writes are ordered before the other lanes' reads
+    if (lane < bt.nb) {
+        const LdsRecs &L = *bt.lds;
+        Part r;
+        r.n = L.n[lane]; r.hi = L.hi[lane]; r.lo = L.lo[lane];
+        r.m2 = (FLAGS & F_M2) ? L.m2[lane] : 0.0;
+        r.vmax = (FLAGS & F_MAX) ? L.vmax[lane] : -__builtin_inf();
+        r.imax = (FLAGS & F_MAX) ? L.imax[lane] : NO_INDEX;
+        r.vmin = (FLAGS & F_MIN) ? L.vmin[lane] : __builtin_inf();
+        r.imin = (FLAGS & F_MIN) ? L.imin[lane] : NO_INDEX;
+        r.nan = L.flags[lane] & 1u;
+        const unsigned first = L.flags[lane] >> 1;
+        const size_t c = bt.cbase + lane * W;
+        if (p.row_direct) write_outputs<T>(p.row, c, r, p.ddof, first, (uint64_t)c * p.cols);
+        if (p.need_l1) part_store<FLAGS>(p.l1, c, r, first);
+    }
+    __builtin_amdgcn_wave_barrier();  // ... and the reads before the next batch's writes
+}
+
+// The last segment of chunk `k` has been folded: merge the lanes and put the record in slot nb.
+template <typename T, int VPT, int FLAGS>
+__device__ __forceinline__ void chunk_end(const SegParams<T> &p, const ChunkPos &k, LaneAcc<T> &a, unsigned &fnan, Batch &bt,
+                                          size_t W) {
+    constexpr int VN = Vec16<T>::N;
+    constexpr int M = VPT * VN;
+    constexpr int WSEG = 64 * M;
+    const unsigned lane = threadIdx.x & 63;
+    const uint64_t fbase = (uint64_t)k.r * p.cols + (uint64_t)k.s0 * WSEG;
+    auto flat = [&](unsigned code) -> uint64_t {
+        const unsigned sl = code / M, i = code % M;
+        return fbase + (uint64_t)sl * WSEG + (uint64_t)(((i / VN) * 64 + lane) * VN + (i % VN));
+    };
+    Part pa;
+    pa.n = a.n; pa.hi = a.hi; pa.lo = a.lo; pa.m2 = a.m2; pa.nan = a.nan;
+    pa.vmax = a.cmax == NO_CODE ? -__builtin_inf() : (double)a.vmax;
+    pa.imax = a.cmax == NO_CODE ? NO_INDEX : flat(a.cmax);
+    pa.vmin = a.cmin == NO_CODE ? __builtin_inf() : (double)a.vmin;
+    pa.imin = a.cmin == NO_CODE ? NO_INDEX : flat(a.cmin);
+    wave_merge<FLAGS>(pa);
+    if (lane == 0) {  // lane 0's value is the chunk's record; fnan of lane 0 is the row's element 0
+        LdsRecs &L = *bt.lds;
+        const unsigned s = bt.nb;
+        L.n[s] = pa.n; L.hi[s] = pa.hi; L.lo[s] = pa.lo;
+        if (FLAGS & F_M2) L.m2[s] = pa.m2;
+        if (FLAGS & F_MAX) { L.vmax[s] = pa.vmax; L.imax[s] = pa.imax; }
+        if (FLAGS & F_MIN) { L.vmin[s] = pa.vmin; L.imin[s] = pa.imin; }
+        L.flags[s] = pa.nan | ((k.q == 0 ? fnan : 0u) << 1);
+    }
+    if (++bt.nb == 64) {
+        flush_batch<T, FLAGS>(p, bt, W);
+        bt.nb = 0;
+        bt.cbase = k.c + W;
+    }
+    acc_reset(a);
+    fnan = 0;
+}
+
+// Fold segment s of chunk k (already loaded into y) and close the chunk after its last segment.
+template <typename T, int VPT, int FLAGS>
+__device__ __forceinline__ void seg_step(const SegParams<T> &p, const ChunkPos &k, size_t s, T (&y)[VPT * Vec16<T>::N],
+                                         unsigned vmask, LaneAcc<T> &a, unsigned &fnan, Batch &bt, size_t W) {
     constexpr int WSEG = 64 * VPT * Vec16<T>::N;
-    const size_t nwaves = (size_t)gridDim.x * WAVES;
-    for (size_t seg = blockIdx.x * (size_t)WAVES + (threadIdx.x >> 6); seg < p.nseg; seg += nwaves) {
-        const size_t c = seg % p.spr;
-        if (c * WSEG + WSEG <= p.cols) {
-            reduce_segment<T, VPT, FLAGS, true, VEC>(p, seg);
-        } else {
-            reduce_segment<T, VPT, FLAGS, false, VEC>(p, seg);
-        }
+    const unsigned sl = (unsigned)(s - k.s0);
+    if (s * WSEG + WSEG <= p.cols) {
+        seg_body<T, VPT, FLAGS, true>(p.chain, y, vmask, sl, a, fnan, s == 0);
+    } else {
+        seg_body<T, VPT, FLAGS, false>(p.chain, y, vmask, sl, a, fnan, s == 0);
     }
+    if (s + 1 == k.s1) chunk_end<T, VPT, FLAGS>(p, k, a, fnan, bt, W);
 }
 
-// ---------------- finish, level 1: one wavefront per group of segment partials ----------------
-// Group g is row g's segments when row results are wanted, else GROUP consecutive segments.
-// Lane l folds partials lo+l, lo+l+64, ... in order, then the xor butterfly.
-template <typename T, int FLAGS>
-__global__ __launch_bounds__(256) void finish_groups_kernel(SegParams<T> p) {
-    const size_t g = (blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6;
-    const int lane = threadIdx.x & 63;
-    if (g >= p.ngroups) return;  // whole wavefronts exit together
-    const bool rowmode = p.want_row;
-    const size_t per = rowmode ? p.spr : (size_t)GROUP;
-    const size_t lo = g * per, hi = min(p.nseg, lo + per);
-    Part a = part_empty();
-    for (size_t s = lo + lane; s < hi; s += 64) {
-        const Part b = part_load(p.l1, s);
-        part_merge<FLAGS>(a, b);
+// Wavefront w of the grid (W wavefronts) reduces chunks w, w + W, w + 2W, ..., segment by
+// segment, with two register buffers: the loads of the next segment (possibly of its next
+// chunk) are in flight while the current one is folded.
+template <typename T, int VPT, int FLAGS, bool VEC>
+__global__ __launch_bounds__(RB) void stream_kernel(SegParams<T> p) {
+    constexpr int M = VPT * Vec16<T>::N;
+    constexpr int WSEG = 64 * M;
+    __shared__ LdsRecs recs[WAVES];
+    const size_t W = (size_t)gridDim.x * WAVES;
+    const unsigned wave = lane0((unsigned)(threadIdx.x >> 6));
+    const size_t c0 = blockIdx.x * (size_t)WAVES + wave;
+    if (c0 >= p.nchunk) return;  // whole wavefronts exit together
+    Batch bt;
+    bt.lds = &recs[wave];
+    bt.nb = 0;
+    bt.cbase = c0;
+    LaneAcc<T> a;
+    acc_reset(a);
+    unsigned fnan = 0;
+    auto base_of = [&](const ChunkPos &k, size_t s) { return p.x + k.r * p.cols + s * WSEG; };
+    auto len_of = [&](size_t s) { return p.cols - s * WSEG; };
+    // advance (k, s) to the wavefront's next segment
+    auto next = [&](ChunkPos &k, size_t &s) {
+        if (s + 1 < k.s1) {
+            ++s;
+        } else {
+            k = chunk_pos(p, k.c + W);
+            s = k.s0;
+        }
+    };
+    ChunkPos ka = chunk_pos(p, c0);
+    size_t sa = ka.s0;
+    T ya[M], yb[M];
+    unsigned ma, mb;
+    seg_load<T, VPT, VEC>(base_of(ka, sa), len_of(sa), ya, ma);
+    for (;;) {
+        ChunkPos kb = ka;
+        size_t sb = sa;
+        next(kb, sb);
+        const bool hb = kb.c < p.nchunk;
+        if (hb) seg_load<T, VPT, VEC>(base_of(kb, sb), len_of(sb), yb, mb);
+        seg_step<T, VPT, FLAGS>(p, ka, sa, ya, ma, a, fnan, bt, W);
+        if (!hb) break;
+        ka = kb;
+        sa = sb;
+        next(ka, sa);
+        const bool ha = ka.c < p.nchunk;
+        if (ha) seg_load<T, VPT, VEC>(base_of(ka, sa), len_of(sa), ya, ma);
+        seg_step<T, VPT, FLAGS>(p, kb, sb, yb, mb, a, fnan, bt, W);
+        if (!ha) break;
     }
+    flush_batch<T, FLAGS>(p, bt, W);
+}
+
+// ---------------- the record tree ----------------
+// Wavefront g merges input records lo .. hi-1 of its set: lane l folds lo+l, lo+l+64, ... (all
+// loads issued first) in order, then the xor butterfly; lane 0 writes.
+template <typename T, int FLAGS>
+__global__ __launch_bounds__(256) void tree_kernel(TreeParams<T> t) {
+    const size_t g = (blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6;
+    const unsigned lane = threadIdx.x & 63;
+    if (g >= t.nsets * t.per_out) return;  // whole wavefronts exit together
+    const size_t set = g / t.per_out, b = g - set * t.per_out;
+    const size_t first = set * t.per;
+    const size_t lo = first + b * TREE_SPAN, hi = min(first + t.per, lo + TREE_SPAN);
+    Part v[TREE_LOADS];
+#pragma unroll
+    for (int k = 0; k < TREE_LOADS; ++k) {
+        const size_t s = lo + lane + 64 * k;
+        v[k] = s < hi ? part_load<FLAGS>(t.in, s) : part_empty();
+    }
+    Part a = v[0];
+#pragma unroll
+    for (int k = 1; k < TREE_LOADS; ++k) part_merge<FLAGS>(a, v[k]);
     wave_merge<FLAGS>(a);
     if (lane == 0) {
-        const unsigned first_nan = (p.l1.flags[lo] >> 1) & 1u;
-        if (rowmode) write_outputs<T>(p.row, g, a, p.ddof, first_nan, g * p.cols);
-        if (p.want_full) part_store(p.l2, g, a, first_nan);
-    }
-}
-
-// ---------------- finish, level 2: one workgroup over the group partials ----------------
-template <typename T, int FLAGS>
-__global__ __launch_bounds__(FB) void finish_full_kernel(SegParams<T> p) {
-    __shared__ Part wp[FB / 64];
-    const int t = threadIdx.x, w = t >> 6, lane = t & 63;
-    Part a = part_empty();
-    for (size_t s = t; s < p.ngroups; s += FB) {
-        const Part b = part_load(p.l2, s);
-        part_merge<FLAGS>(a, b);
-    }
-    wave_merge<FLAGS>(a);
-    if (lane == 0) wp[w] = a;
-    __syncthreads();
-    if (t == 0) {
-        Part tot = wp[0];
-        for (int k = 1; k < FB / 64; ++k) part_merge<FLAGS>(tot, wp[k]);
-        const unsigned first_nan = (p.l1.flags[0] >> 1) & 1u;
-        write_outputs<T>(p.full, 0, tot, p.ddof, first_nan, 0);
+        const unsigned fn = (t.in.flags[lo] >> 1) & 1u;
+        if (t.results) write_outputs<T>(t.o, set, a, t.ddof, fn, t.rowmode ? (uint64_t)set * t.cols : 0);
+        if (t.keep) part_store<FLAGS>(t.out, g, a, fn);
     }
 }
 
@@ -454,24 +605,6 @@ static bool any_out(const OutPtrs<T> &o) {
     return o.sum || o.mean || o.var || o.min || o.max || o.argmin || o.argmax;
 }
 
-template <typename T, int VPT, int FLAGS>
-static nrk_status launch_flags(SegParams<T> &p, hipStream_t stream) {
-    const int grid = (int)std::min<size_t>(ceil_div(p.nseg, (size_t)WAVES), (size_t)cu_count() * reduce_blocks_per_cu());
-    if (((reinterpret_cast<uintptr_t>(p.x) % 16) == 0) && (p.cols % Vec16<T>::N == 0))
-        hipLaunchKernelGGL((seg_reduce_kernel<T, VPT, FLAGS, true>), dim3(grid), dim3(RB), 0, stream, p);
-    else
-        hipLaunchKernelGGL((seg_reduce_kernel<T, VPT, FLAGS, false>), dim3(grid), dim3(RB), 0, stream, p);
-    NRK_HIP(hipGetLastError());
-    if (!p.need_partials) return NRK_OK;
-    hipLaunchKernelGGL((finish_groups_kernel<T, FLAGS>), dim3((unsigned)ceil_div(p.ngroups, 4)), dim3(256), 0, stream, p);
-    NRK_HIP(hipGetLastError());
-    if (p.want_full) {
-        hipLaunchKernelGGL((finish_full_kernel<T, FLAGS>), dim3(1), dim3(FB), 0, stream, p);
-        NRK_HIP(hipGetLastError());
-    }
-    return NRK_OK;
-}
-
 static Parts carve_parts(char *&b, size_t count) {
     auto take = [&](size_t bytes) {
         void *r = b;
@@ -490,69 +623,159 @@ static Parts carve_parts(char *&b, size_t count) {
     P.flags = (unsigned *)take(count * 4);
     return P;
 }
+static size_t parts_bytes(size_t count) { return 8 * align_up(count * 8, 256) + align_up(count * 4, 256); }
+
+// Workgroups of `kernel` that fit on one CU at once (register / LDS limited), cached per kernel.
+template <typename K>
+static int resident_blocks(K kernel, int *cache) {
+    if (*cache == 0) {
+        int b = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kernel, RB, 0) != hipSuccess || b < 1) b = 1;
+        *cache = b;
+    }
+    return *cache;
+}
+
+// The grid is persistent: at most the workgroups that are resident together (and at most the
+// tuner's workgroups per CU), each wavefront then looping over its chunks.
+template <typename T, int VPT, int FLAGS>
+static nrk_status launch_stream(const SegParams<T> &p, hipStream_t stream) {
+    const bool vec = ((reinterpret_cast<uintptr_t>(p.x) % 16) == 0) && (p.cols % Vec16<T>::N == 0);
+    static int occ[2] = {0, 0};
+    const int res = vec ? resident_blocks(stream_kernel<T, VPT, FLAGS, true>, &occ[1])
+                        : resident_blocks(stream_kernel<T, VPT, FLAGS, false>, &occ[0]);
+    const size_t per_cu = (size_t)std::min(res, reduce_blocks_per_cu());
+    const int grid = (int)std::min<size_t>(ceil_div(p.nchunk, (size_t)WAVES), (size_t)cu_count() * per_cu);
+    if (vec)
+        hipLaunchKernelGGL((stream_kernel<T, VPT, FLAGS, true>), dim3(grid), dim3(RB), 0, stream, p);
+    else
+        hipLaunchKernelGGL((stream_kernel<T, VPT, FLAGS, false>), dim3(grid), dim3(RB), 0, stream, p);
+    NRK_HIP(hipGetLastError());
+    return NRK_OK;
+}
+
+// Reduce `nsets` sets of `per` records (`in`) to one result per set, level by level. With `keep`
+// the last level also stores the per-set records into `final_out` (index = set).
+template <typename T, int FLAGS>
+static nrk_status run_tree(Parts in, size_t nsets, size_t per, Parts tmp0, Parts tmp1, bool rowmode, size_t cols, double ddof,
+                           const OutPtrs<T> &o, bool keep, Parts final_out, hipStream_t stream) {
+    int flip = 0;
+    for (;;) {
+        TreeParams<T> t{};
+        t.in = in;
+        t.nsets = nsets;
+        t.per = per;
+        t.per_out = ceil_div(per, (size_t)TREE_SPAN);
+        t.results = t.per_out == 1;
+        t.keep = !t.results || keep;
+        t.out = t.results ? final_out : (flip ? tmp1 : tmp0);
+        t.rowmode = rowmode;
+        t.cols = cols;
+        t.ddof = ddof;
+        t.o = t.results ? o : OutPtrs<T>{};
+        const size_t waves = nsets * t.per_out;
+        hipLaunchKernelGGL((tree_kernel<T, FLAGS>), dim3((unsigned)ceil_div(waves, (size_t)4)), dim3(256), 0, stream, t);
+        NRK_HIP(hipGetLastError());
+        if (t.results) return NRK_OK;
+        in = t.out;
+        per = t.per_out;
+        flip ^= 1;
+    }
+}
+
+template <typename T, int VPT, int FLAGS>
+static nrk_status launch_all(SegParams<T> &p, bool want_row, bool want_full, const OutPtrs<T> &row, const OutPtrs<T> &full,
+                             const Parts &rowrec, const Parts &tmp0, const Parts &tmp1, hipStream_t stream) {
+    NRK_TRY((launch_stream<T, VPT, FLAGS>(p, stream)));
+    const bool split_rows = want_row && p.cpr > 1;
+    if (split_rows)
+        NRK_TRY((run_tree<T, FLAGS>(p.l1, p.rows, p.cpr, tmp0, tmp1, true, p.cols, p.ddof, row, want_full, rowrec, stream)));
+    if (want_full) {
+        const Parts in = split_rows ? rowrec : p.l1;
+        const size_t per = split_rows ? p.rows : p.nchunk;
+        NRK_TRY((run_tree<T, FLAGS>(in, 1, per, tmp0, tmp1, false, p.cols, p.ddof, full, false, Parts{}, stream)));
+    }
+    return NRK_OK;
+}
 
 // Host launcher for the fused map -> reduce over a (rows, cols) matrix.
 template <typename T>
 nrk_status seg_reduce(const Chain &chain, const T *x, size_t rows, size_t cols, double ddof,
                       const nrk_reduce_outputs *row_out, const nrk_reduce_outputs *full_out, hipStream_t stream) {
-    constexpr int VPT = 8;
-    constexpr int WSEG = 64 * VPT * Vec16<T>::N;  // elements per wavefront segment
+    constexpr int M = NRK_REDUCE_M;
+    constexpr int VPT = M / Vec16<T>::N;
+    static_assert(VPT * Vec16<T>::N == M, "elements per lane must be whole 16-byte vectors");
+    constexpr int WSEG = 64 * M;  // elements per wavefront segment
     NRK_TRY(ensure_device());
-    SegParams<T> p{};
-    p.row = to_ptrs<T>(row_out);
-    p.full = to_ptrs<T>(full_out);
-    p.want_row = any_out(p.row);
-    p.want_full = any_out(p.full);
-    if (!p.want_row && !p.want_full) return NRK_OK;
-    if (p.want_row && (p.row.argmax || p.row.argmin || p.row.max || p.row.min) && cols == 0)
+    const OutPtrs<T> row = to_ptrs<T>(row_out);
+    const OutPtrs<T> full = to_ptrs<T>(full_out);
+    const bool want_row = any_out(row);
+    const bool want_full = any_out(full);
+    if (!want_row && !want_full) return NRK_OK;
+    if (want_row && (row.argmax || row.argmin || row.max || row.min) && cols == 0)
         return fail(NRK_ERR_INVALID_ARGUMENT, "max/min/argmax/argmin along an empty axis");
-    if (p.want_full && (p.full.argmax || p.full.argmin) && rows * cols == 0)
+    if (want_full && (full.argmax || full.argmin) && rows * cols == 0)
         return fail(NRK_ERR_INVALID_ARGUMENT, "argmax/argmin of an empty array");
     if (!x && rows * cols > 0) return fail(NRK_ERR_INVALID_ARGUMENT, "x is NULL");
     if (rows * cols == 0) {
         // reference: empty -> 0 for sum/mean/min/max/var (kernels/reduce.rs:207-421)
-        if (p.want_full) {
-            hipLaunchKernelGGL(fill_outputs_kernel<T>, dim3(1), dim3(64), 0, stream, p.full, (size_t)1, (T)0);
+        if (want_full) {
+            hipLaunchKernelGGL(fill_outputs_kernel<T>, dim3(1), dim3(64), 0, stream, full, (size_t)1, (T)0);
             NRK_HIP(hipGetLastError());
         }
-        if (p.want_row && rows > 0) {
-            hipLaunchKernelGGL(fill_outputs_kernel<T>, dim3((unsigned)ceil_div(rows, 256)), dim3(256), 0, stream, p.row, rows,
-                               (T)0);
+        if (want_row && rows > 0) {
+            hipLaunchKernelGGL(fill_outputs_kernel<T>, dim3((unsigned)ceil_div(rows, 256)), dim3(256), 0, stream, row, rows, (T)0);
             NRK_HIP(hipGetLastError());
         }
         return NRK_OK;
     }
+    if (!want_row) {  // a whole-array reduction: the rows are contiguous, so take them as one
+        cols *= rows;
+        rows = 1;
+    }
+    SegParams<T> p{};
     p.x = x;
     p.rows = rows;
     p.cols = cols;
     p.chain = chain;
     p.ddof = ddof;
-    p.spr = ceil_div(cols, WSEG);
-    p.nseg = rows * p.spr;
-    if (p.nseg >= (size_t)UINT32_MAX) return fail(NRK_ERR_INVALID_ARGUMENT, "too many segments (%zu)", p.nseg);
-    p.need_partials = 1;  // every result goes through the finish kernels
-    p.ngroups = p.want_row ? rows : ceil_div(p.nseg, (size_t)GROUP);
-    const size_t rec = 8 * 8 + 4;  // bytes of one partial record
-    const size_t pbytes = (p.nseg + p.ngroups) * rec + 2 * 9 * 256 + 4096;
+    p.spr = ceil_div(cols, (size_t)WSEG);
+    // chunks: whole rows when there are enough of them, else rows split into ~CHUNK_TARGET
+    // chunks in total; depends on the shape only.
+    size_t cpr = std::min(p.spr, std::max<size_t>(1, ceil_div(CHUNK_TARGET, rows)));
+    p.cl = ceil_div(p.spr, cpr);
+    p.cpr = ceil_div(p.spr, p.cl);
+    p.nchunk = rows * p.cpr;
+    if (p.cl >= ((size_t)1 << 26)) return fail(NRK_ERR_INVALID_ARGUMENT, "row of %zu elements is too long", cols);
+    p.row_direct = want_row && p.cpr == 1;
+    p.need_l1 = want_full || (want_row && p.cpr > 1);
+    p.row = p.row_direct ? row : OutPtrs<T>{};
+    // scratch: chunk records, row records, two ping-pong buffers for the tree levels
+    const size_t tmp_count = std::max(rows * ceil_div(p.cpr, (size_t)TREE_SPAN), ceil_div(std::max(rows, p.nchunk), (size_t)TREE_SPAN)) + 1;
+    const size_t pbytes = parts_bytes(p.nchunk) + parts_bytes(rows) + 2 * parts_bytes(tmp_count) + 256;
     Scratch s;
     NRK_TRY(get_scratch(stream, 0, pbytes, &s));
     char *b = (char *)s.partials;
-    p.l1 = carve_parts(b, p.nseg);
-    p.l2 = carve_parts(b, p.ngroups);
-    const bool m2 = (p.row.var || p.full.var);
-    const bool mx = (p.row.max || p.row.argmax || p.full.max || p.full.argmax);
-    const bool mn = (p.row.min || p.row.argmin || p.full.min || p.full.argmin);
+    p.l1 = carve_parts(b, p.nchunk);
+    const Parts rowrec = carve_parts(b, rows);
+    const Parts tmp0 = carve_parts(b, tmp_count);
+    const Parts tmp1 = carve_parts(b, tmp_count);
+    const bool m2 = (row.var || full.var);
+    const bool mx = (row.max || row.argmax || full.max || full.argmax);
+    const bool mn = (row.min || row.argmin || full.min || full.argmin);
     const int flags = (m2 ? F_M2 : 0) | (mx ? F_MAX : 0) | (mn ? F_MIN : 0);
+#define NRK_LAUNCH_FLAGS(F) return launch_all<T, VPT, F>(p, want_row, want_full, row, full, rowrec, tmp0, tmp1, stream)
     switch (flags) {
-        case 0: return launch_flags<T, VPT, 0>(p, stream);
-        case 1: return launch_flags<T, VPT, 1>(p, stream);
-        case 2: return launch_flags<T, VPT, 2>(p, stream);
-        case 3: return launch_flags<T, VPT, 3>(p, stream);
-        case 4: return launch_flags<T, VPT, 4>(p, stream);
-        case 5: return launch_flags<T, VPT, 5>(p, stream);
-        case 6: return launch_flags<T, VPT, 6>(p, stream);
-        default: return launch_flags<T, VPT, 7>(p, stream);
+        case 0: NRK_LAUNCH_FLAGS(0);
+        case 1: NRK_LAUNCH_FLAGS(1);
+        case 2: NRK_LAUNCH_FLAGS(2);
+        case 3: NRK_LAUNCH_FLAGS(3);
+        case 4: NRK_LAUNCH_FLAGS(4);
+        case 5: NRK_LAUNCH_FLAGS(5);
+        case 6: NRK_LAUNCH_FLAGS(6);
+        default: NRK_LAUNCH_FLAGS(7);
     }
+#undef NRK_LAUNCH_FLAGS
 }
 
 template nrk_status seg_reduce<float>(const Chain &, const float *, size_t, size_t, double, const nrk_reduce_outputs *,
