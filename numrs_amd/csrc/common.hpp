@@ -109,24 +109,6 @@ __device__ __forceinline__ double dev_fmax(double a, double b) { return __builti
 __device__ __forceinline__ float dev_fmin(float a, float b) { return __builtin_fminf(a, b); }
 __device__ __forceinline__ double dev_fmin(double a, double b) { return __builtin_fmin(a, b); }
 
-// Apply one scalar-chain step (NRK_SOP_*) with one IEEE rounding.
-template <typename T>
-__device__ __forceinline__ T apply_sop(int op, T s, T x) {
-    switch (op) {
-        case NRK_SOP_ADD: return rn_add(x, s);
-        case NRK_SOP_SUB: return rn_sub(x, s);
-        case NRK_SOP_MUL: return rn_mul(x, s);
-        case NRK_SOP_DIV: return rn_div(x, s);
-        case NRK_SOP_RSUB: return rn_sub(s, x);
-        case NRK_SOP_RDIV: return rn_div(s, x);
-        case NRK_SOP_NEG: return -x;
-        case NRK_SOP_ABS: return dev_fabs(x);
-        case NRK_SOP_SQUARE: return rn_mul(x, x);
-        case NRK_SOP_SQRT: return rn_sqrt(x);
-        default: return x;
-    }
-}
-
 // Scalar chain passed by value to kernels.
 struct Chain {
     int len;
@@ -134,8 +116,9 @@ struct Chain {
     double s[NRK_MAX_CHAIN];
 };
 
-// Apply the whole chain to N register values: one (wavefront-uniform) branch per step, then N
-// straight-line operations, so the op dispatch is not repeated per element.
+// Apply the whole chain to N register values, each step (NRK_SOP_*) with one IEEE rounding: one
+// (wavefront-uniform) branch per step, then N straight-line operations, so the op dispatch is
+// not repeated per element.
 template <typename T, int N>
 __device__ __forceinline__ void apply_chain_n(const Chain &c, T (&v)[N]) {
     for (int k = 0; k < c.len; ++k) {

@@ -186,6 +186,7 @@ static nrk_status launch_bf16_nt(int M, int N, int Kp, const unsigned short *A, 
                                  size_t b_stride, float *C, int ldc, hipStream_t stream) {
     constexpr int threads = (BM / WM) * (BN / WN) * 64;
     constexpr size_t lds = (size_t)2 * (NTA * BM + NTB * BN) * (BK + 8) * sizeof(unsigned short);
+    static_assert(lds <= 160 * 1024, "tile does not fit in LDS");
     auto kern = gemm_bf16_nt_kernel<BM, BN, BK, WM, WN, NTA, NTB, NPAIR>;
     static bool attr_set = false;
     if (!attr_set) {
@@ -218,9 +219,28 @@ static nrk_status split(const float *X, int R, int Cc, bool transpose, unsigned 
     return NRK_OK;
 }
 
-nrk_status gemm_f32_bf16x(size_t m, size_t k, size_t n, const float *a, const float *b, float *c, int products,
+// Tile configurations of the split-product GEMM, BM x BN x BK (wave tile); every term of both
+// operands sits in LDS (double-buffered), which bounds the tile: 0: 64x64xBK (32x32),
+// 1: 128x128xBK (64x64), 2: 128x128x32 (64x64), 3: 128x64xBK (64x32), 4: 64x128xBK (32x64),
+// with BK = 64 (32 for the six-product form).
+constexpr int SPLIT_TILE_CONFIGS = 5;
+template <int NTA, int NTB, int NPAIR, int BK>
+static nrk_status launch_split_tile(int tile, size_t m, size_t n, size_t Kp, const unsigned short *As, size_t sa,
+                                    const unsigned short *Bts, size_t sb, float *c, hipStream_t stream) {
+    switch (tile) {
+        case 0: return launch_bf16_nt<64, 64, BK, 32, 32, NTA, NTB, NPAIR>((int)m, (int)n, (int)Kp, As, sa, Bts, sb, c, (int)n, stream);
+        case 1: return launch_bf16_nt<128, 128, BK, 64, 64, NTA, NTB, NPAIR>((int)m, (int)n, (int)Kp, As, sa, Bts, sb, c, (int)n, stream);
+        case 2: return launch_bf16_nt<128, 128, 32, 64, 64, NTA, NTB, NPAIR>((int)m, (int)n, (int)Kp, As, sa, Bts, sb, c, (int)n, stream);
+        case 3: return launch_bf16_nt<128, 64, BK, 64, 32, NTA, NTB, NPAIR>((int)m, (int)n, (int)Kp, As, sa, Bts, sb, c, (int)n, stream);
+        default: return launch_bf16_nt<64, 128, BK, 32, 64, NTA, NTB, NPAIR>((int)m, (int)n, (int)Kp, As, sa, Bts, sb, c, (int)n, stream);
+    }
+}
+
+nrk_status gemm_f32_bf16x(size_t m, size_t k, size_t n, const float *a, const float *b, float *c, int products, int tile,
                           hipStream_t stream) {
     NRK_TRY(ensure_device());
+    if (tile < -1 || tile >= SPLIT_TILE_CONFIGS)
+        return fail(NRK_ERR_INVALID_ARGUMENT, "split GEMM tile config %d outside [0,%d)", tile, SPLIT_TILE_CONFIGS);
     if (products != 3 && products != 6 && products != 1)
         return fail(NRK_ERR_INVALID_ARGUMENT, "products must be 1, 3 or 6 (got %d)", products);
     if (m == 0 || n == 0) return NRK_OK;
@@ -245,17 +265,13 @@ nrk_status gemm_f32_bf16x(size_t m, size_t k, size_t n, const float *a, const fl
         NRK_TRY(split<1>(b, (int)k, (int)n, true, Bts, (int)Np, (int)Kp, stream));
     }
     const size_t sa = Mp * Kp, sb = Np * Kp;
-    const bool big = ceil_div(m, 128) * ceil_div(n, 128) >= (size_t)cu_count();
-    if (products == 6) {
-        if (big) return launch_bf16_nt<128, 128, 32, 64, 64, 3, 3, 6>((int)m, (int)n, (int)Kp, As, sa, Bts, sb, c, (int)n, stream);
-        return launch_bf16_nt<64, 64, 32, 32, 32, 3, 3, 6>((int)m, (int)n, (int)Kp, As, sa, Bts, sb, c, (int)n, stream);
+    if (tile < 0) {
+        const bool big = ceil_div(m, 128) * ceil_div(n, 128) >= (size_t)cu_count();
+        tile = big ? 1 : 0;
     }
-    if (products == 3) {
-        if (big) return launch_bf16_nt<128, 128, 64, 64, 64, 2, 2, 3>((int)m, (int)n, (int)Kp, As, sa, Bts, sb, c, (int)n, stream);
-        return launch_bf16_nt<64, 64, 64, 32, 32, 2, 2, 3>((int)m, (int)n, (int)Kp, As, sa, Bts, sb, c, (int)n, stream);
-    }
-    if (big) return launch_bf16_nt<128, 128, 64, 64, 64, 1, 1, 1>((int)m, (int)n, (int)Kp, As, sa, Bts, sb, c, (int)n, stream);
-    return launch_bf16_nt<64, 64, 64, 32, 32, 1, 1, 1>((int)m, (int)n, (int)Kp, As, sa, Bts, sb, c, (int)n, stream);
+    if (products == 6) return launch_split_tile<3, 3, 6, 32>(tile, m, n, Kp, As, sa, Bts, sb, c, stream);
+    if (products == 3) return launch_split_tile<2, 2, 3, 64>(tile, m, n, Kp, As, sa, Bts, sb, c, stream);
+    return launch_split_tile<1, 1, 1, 64>(tile, m, n, Kp, As, sa, Bts, sb, c, stream);
 }
 
 }  // namespace nrk
@@ -266,7 +282,7 @@ extern "C" {
 
 nrk_status nrk_dev_gemm_f32_bf16x(size_t m, size_t k, size_t n, const float *a, const float *b, float *c, int32_t products,
                                   nrk_stream stream) {
-    return gemm_f32_bf16x(m, k, n, a, b, c, products, (hipStream_t)stream);
+    return gemm_f32_bf16x(m, k, n, a, b, c, products, -1, (hipStream_t)stream);
 }
 
 nrk_status nrk_gemm_f32_bf16x(size_t m, size_t k, size_t n, const float *a, const float *b, float *c, int32_t products) {
@@ -279,7 +295,7 @@ nrk_status nrk_gemm_f32_bf16x(size_t m, size_t k, size_t n, const float *a, cons
     hipStream_t s = nullptr;
     if (m * k) NRK_HIP(hipMemcpyAsync(da, a, m * k * 4, hipMemcpyHostToDevice, s));
     if (k * n) NRK_HIP(hipMemcpyAsync(db, b, k * n * 4, hipMemcpyHostToDevice, s));
-    NRK_TRY(gemm_f32_bf16x(m, k, n, (const float *)da, (const float *)db, (float *)dc, products, s));
+    NRK_TRY(gemm_f32_bf16x(m, k, n, (const float *)da, (const float *)db, (float *)dc, products, -1, s));
     NRK_HIP(hipMemcpyAsync(c, dc, m * n * 4, hipMemcpyDeviceToHost, s));
     NRK_HIP(hipStreamSynchronize(s));
     return NRK_OK;

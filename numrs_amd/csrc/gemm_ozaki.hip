@@ -29,6 +29,9 @@ namespace nrk {
 constexpr int OZ_MAX_MODULI = 20;
 constexpr int OZ_LIMBS = 7;       // 7 x 24-bit limbs = 168 bits > log2(M) for 20 moduli
 constexpr int OZ_DEFAULT_MODULI = 16;
+constexpr int OZ_TILE_CONFIGS = 5;  // INT8 GEMM tiles (BMxBNxBK, wave tile): 64x64x128 (32x32),
+                                     // 128x128x128 (64x64), 256x256x128 (128x64), 256x256x64 (128x64),
+                                     // 256x256x64 (64x128)
 constexpr int OZ_BAD_SCALE = 100000;  // marks a row/column holding Inf/NaN
 
 // Pairwise-coprime odd moduli <= 255, largest first (greedy).
@@ -133,21 +136,34 @@ __global__ __launch_bounds__(256) void row_scale_kernel(const double *__restrict
     }
 }
 
-// column maxima of X (R x Cc, row-major): one thread per column, coalesced across the row
-__global__ __launch_bounds__(256) void col_scale_kernel(const double *__restrict__ X, int R, int Cc, int P,
-                                                        int *__restrict__ scale) {
+// Column maxima of X (R x Cc, row-major) in two passes: chunks of COL_ROWS rows per thread
+// (threads of a workgroup read consecutive columns, so every row read is coalesced), then one
+// thread per column over the chunk maxima. A non-finite entry is recorded as +inf.
+constexpr int COL_ROWS = 128;
+__global__ __launch_bounds__(256) void col_max_partial_kernel(const double *__restrict__ X, int R, int Cc,
+                                                              double *__restrict__ pmax) {
     const int c = blockIdx.x * 256 + threadIdx.x;
     if (c >= Cc) return;
+    const int r0 = blockIdx.y * COL_ROWS, r1 = min(R, r0 + COL_ROWS);
     double mx = 0.0;
     int bad = 0;
-    for (int r = 0; r < R; ++r) {
+    for (int r = r0; r < r1; ++r) {
         const double v = X[(size_t)r * Cc + c];
         bad |= !__builtin_isfinite(v);
         mx = fmax(mx, fabs(v));
     }
+    pmax[(size_t)blockIdx.y * Cc + c] = bad ? __builtin_inf() : mx;
+}
+
+__global__ __launch_bounds__(256) void col_scale_kernel(const double *__restrict__ pmax, int nchunk, int Cc, int P,
+                                                        int *__restrict__ scale) {
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= Cc) return;
+    double mx = 0.0;
+    for (int h = 0; h < nchunk; ++h) mx = fmax(mx, pmax[(size_t)h * Cc + c]);
     int e = 0;
-    if (mx > 0.0) (void)frexp(mx, &e);
-    scale[c] = bad ? OZ_BAD_SCALE : (mx > 0.0 ? P - e : 0);
+    if (mx > 0.0 && __builtin_isfinite(mx)) (void)frexp(mx, &e);
+    scale[c] = __builtin_isinf(mx) ? OZ_BAD_SCALE : (mx > 0.0 ? P - e : 0);
 }
 
 // ---------------- 2. residues ----------------
@@ -160,34 +176,56 @@ __device__ __forceinline__ signed char sym_mod(double v, double m, double inv_m)
     return (signed char)(int)r;
 }
 
-// out[j][orow][ocol] = residue of round(2^scale * X) mod m_j; X is (R, Cc); with TRANSPOSE the
-// output is (Cc, R) (row = input column). Zero in the padding. Scale applies along `scale_by_row`.
+// Each thread produces 16 consecutive K positions of one output row for every modulus and
+// stores them as one 16-byte vector per modulus: out[j] is (out_rows, out_cols) with out_cols
+// (the padded K) a multiple of 16. Without TRANSPOSE X is (R, Cc) = (rows, K) and consecutive
+// threads take consecutive chunks of a row. With TRANSPOSE X is (R, Cc) = (K, rows), output row o
+// is input column o, and a workgroup covers 16 output rows x 16 chunks (lanes 0-15 read 16
+// consecutive doubles of one row of X; 16 consecutive chunks of one output row are stored by
+// 16 consecutive threads' groups). Zero in the padding.
 template <bool TRANSPOSE>
 __global__ __launch_bounds__(256) void residue_kernel(const double *__restrict__ X, int R, int Cc,
                                                      const int *__restrict__ scale, OzConsts k,
                                                      signed char *__restrict__ out, int out_rows, int out_cols) {
-    __shared__ signed char tile[OZ_MAX_MODULI][32][33];
-    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-    const int r0 = blockIdx.y * 32, c0 = blockIdx.x * 32;
-    for (int dy = ty; dy < 32; dy += 8) {
-        const int r = r0 + dy, c = c0 + tx;
-        double v = 0.0;
-        if (r < R && c < Cc) {
-            const int s = TRANSPOSE ? scale[c] : scale[r];
-            v = (s == OZ_BAD_SCALE) ? 0.0 : rint(ldexp(X[(size_t)r * Cc + c], s));
-        }
-        for (int j = 0; j < k.n; ++j) {
-            const signed char q = sym_mod(v, (double)k.m[j], k.inv_m[j]);
-            if (TRANSPOSE) tile[j][tx][dy] = q; else tile[j][dy][tx] = q;
-        }
+    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+    const int nkc = out_cols / 16;
+    int orow, kc;
+    if (TRANSPOSE) {
+        const int nob = (out_rows + 15) / 16;
+        const int ob = blockIdx.x % nob, kb = blockIdx.x / nob;
+        orow = ob * 16 + (threadIdx.x & 15);
+        kc = kb * 16 + (threadIdx.x >> 4);
+    } else {
+        const size_t t = blockIdx.x * (size_t)256 + threadIdx.x;
+        orow = (int)(t / nkc);
+        kc = (int)(t % nkc);
     }
-    __syncthreads();
-    const int o_r0 = TRANSPOSE ? c0 : r0, o_c0 = TRANSPOSE ? r0 : c0;
+    if (orow >= out_rows || kc >= nkc) return;
+    const int k0 = kc * 16;
+    const int nrows = TRANSPOSE ? Cc : R;  // logical output rows that exist
+    const int klen = TRANSPOSE ? R : Cc;   // logical K
+    const int s = orow < nrows ? scale[orow] : OZ_BAD_SCALE;
+    double v[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        const int kk = k0 + e;
+        double x = 0.0;
+        if (s != OZ_BAD_SCALE && kk < klen) x = TRANSPOSE ? X[(size_t)kk * Cc + orow] : X[(size_t)orow * Cc + kk];
+        v[e] = rint(ldexp(x, s == OZ_BAD_SCALE ? 0 : s));
+    }
     const size_t stride = (size_t)out_rows * out_cols;
-    for (int dy = ty; dy < 32; dy += 8) {
-        const int orow = o_r0 + dy, ocol = o_c0 + tx;
-        if (orow < out_rows && ocol < out_cols)
-            for (int j = 0; j < k.n; ++j) out[j * stride + (size_t)orow * out_cols + ocol] = tile[j][dy][tx];
+    signed char *o = out + (size_t)orow * out_cols + k0;
+    for (int j = 0; j < k.n; ++j) {
+        const double m = (double)k.m[j], im = k.inv_m[j];
+        unsigned w[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            unsigned x = 0;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) x |= ((unsigned)(unsigned char)sym_mod(v[4 * q + b], m, im)) << (8 * b);
+            w[q] = x;
+        }
+        *reinterpret_cast<u32x4 *>(o + j * stride) = u32x4{w[0], w[1], w[2], w[3]};
     }
 }
 
@@ -202,6 +240,11 @@ __device__ __forceinline__ signed char mod_i32(int v, int m) {
     return (signed char)r;
 }
 
+// Bytes of padding per LDS row. 0, 16 and 32 measure the same at 8192^3 (the bank-conflict
+// counter changes 3x between them, the time does not); paddings that break 16-byte alignment
+// are 3x slower.
+constexpr int OZ_LDS_PAD = 16;
+
 // Cres[j] (M x N, int8) = (Ares[j] (Mp x Kp) * Btres[j] (Np x Kp)^T) mod m_j, j = blockIdx.y.
 template <int BM, int BN, int BK, int WM, int WN>
 __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_i8_nt_mod_kernel(
@@ -210,7 +253,7 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_i8_nt_mod_ker
     constexpr int NT = (BM / WM) * (BN / WN) * 64;
     constexpr int WGN = BN / WN;
     constexpr int TM = WM / 16, TN = WN / 16;
-    constexpr int LDK = BK + 16;   // bytes per padded LDS row
+    constexpr int LDK = BK + OZ_LDS_PAD;  // bytes per padded LDS row
     constexpr int CH = BK / 16;    // 16-byte chunks per row
     constexpr int LA = BM * CH / NT, LB = BN * CH / NT;
     static_assert((BM * CH) % NT == 0 && (BN * CH) % NT == 0, "tile/threads mismatch");
@@ -235,38 +278,35 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_i8_nt_mod_ker
 #pragma unroll
         for (int jj = 0; jj < TN; ++jj) acc[i][jj] = i32x4_t{0, 0, 0, 0};
 
-    u32x4_t ra[LA], rb[LB];
-    auto gload = [&](int k0) {
+    // Two register staging sets: the global loads of K-tile t are issued two tiles ahead of the
+    // tile's use, so they have two tiles of MFMA work to arrive in (tile t: global -> stage t%2
+    // during tile t-2, stage -> LDS buffer t%2 at the end of tile t-1, MFMAs during tile t).
+    u32x4_t ra[2][LA], rb[2][LB];
+    auto gload = [&](int st, int k0) {
 #pragma unroll
         for (int s = 0; s < LA; ++s) {
             const int q = tid + s * NT, row = q / CH, ch = q % CH;
-            ra[s] = *reinterpret_cast<const u32x4_t *>(A + (size_t)(bm0 + row) * Kp + k0 + ch * 16);
+            ra[st][s] = *reinterpret_cast<const u32x4_t *>(A + (size_t)(bm0 + row) * Kp + k0 + ch * 16);
         }
 #pragma unroll
         for (int s = 0; s < LB; ++s) {
             const int q = tid + s * NT, row = q / CH, ch = q % CH;
-            rb[s] = *reinterpret_cast<const u32x4_t *>(Bt + (size_t)(bn0 + row) * Kp + k0 + ch * 16);
+            rb[st][s] = *reinterpret_cast<const u32x4_t *>(Bt + (size_t)(bn0 + row) * Kp + k0 + ch * 16);
         }
     };
-    auto sstore = [&](int buf) {
+    auto sstore = [&](int st, int buf) {
 #pragma unroll
         for (int s = 0; s < LA; ++s) {
             const int q = tid + s * NT, row = q / CH, ch = q % CH;
-            *reinterpret_cast<u32x4_t *>(As + (buf * BM + row) * LDK + ch * 16) = ra[s];
+            *reinterpret_cast<u32x4_t *>(As + (buf * BM + row) * LDK + ch * 16) = ra[st][s];
         }
 #pragma unroll
         for (int s = 0; s < LB; ++s) {
             const int q = tid + s * NT, row = q / CH, ch = q % CH;
-            *reinterpret_cast<u32x4_t *>(Bs + (buf * BN + row) * LDK + ch * 16) = rb[s];
+            *reinterpret_cast<u32x4_t *>(Bs + (buf * BN + row) * LDK + ch * 16) = rb[st][s];
         }
     };
-    const int nk = Kp / BK;
-    gload(0);
-    sstore(0);
-    __syncthreads();
-    for (int kt = 0; kt < nk; ++kt) {
-        const int cur = kt & 1;
-        if (kt + 1 < nk) gload((kt + 1) * BK);
+    auto compute = [&](int cur) {
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 64) {
             i8x16_t fa[TM], fb[TN];
@@ -284,8 +324,25 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_i8_nt_mod_ker
                     acc[i][jj] = __builtin_amdgcn_mfma_i32_16x16x64_i8(__builtin_bit_cast(i32x4_t, fa[i]),
                                                                        __builtin_bit_cast(i32x4_t, fb[jj]), acc[i][jj], 0, 0, 0);
         }
-        if (kt + 1 < nk) sstore(cur ^ 1);
+    };
+    const int nk = Kp / BK;
+    // tile t (even: stage/buffer 0, odd: 1): move tile t+1 from its stage into the other LDS
+    // buffer (nobody reads that buffer any more: the previous barrier ended tile t-1) and refill
+    // the stage with tile t+3, so the LDS writes overlap this tile's MFMAs; then compute tile t.
+    auto step = [&](int t, int par) {
+        if (t + 1 < nk) sstore(par ^ 1, par ^ 1);
+        if (t + 3 < nk) gload(par ^ 1, (t + 3) * BK);
+        compute(par);
         __syncthreads();
+    };
+    gload(0, 0);
+    if (nk > 1) gload(1, BK);
+    sstore(0, 0);
+    if (nk > 2) gload(0, 2 * BK);
+    __syncthreads();
+    for (int t = 0; t < nk; t += 2) {
+        step(t, 0);
+        if (t + 1 < nk) step(t + 1, 1);
     }
     const int mj = k.m[j];
     signed char *Cj = Cres + j * c_stride;
@@ -311,50 +368,60 @@ __device__ __forceinline__ dd_t dd_two_sum(double a, double b) {
     return dd_t{s, (a - (s - v)) + (b - v)};
 }
 
+// Thread t reconstructs outputs 4t .. 4t+3 (flat, row-major), reading the four residues of each
+// modulus as one 32-bit word (c_stride and the base are multiples of 16).
 __global__ __launch_bounds__(256) void crt_kernel(const signed char *__restrict__ Cres, size_t c_stride, int M, int N,
                                                   const int *__restrict__ srow, const int *__restrict__ scol, OzConsts k,
                                                   double *__restrict__ C) {
-    const size_t idx = blockIdx.x * (size_t)256 + threadIdx.x;
-    if (idx >= (size_t)M * N) return;
-    const int row = (int)(idx / N), col = (int)(idx % N);
-    const int sr = srow[row], sc = scol[col];
-    if (sr == OZ_BAD_SCALE || sc == OZ_BAD_SCALE) {
-        C[idx] = __builtin_nan("");
-        return;
+    const size_t base = (blockIdx.x * (size_t)256 + threadIdx.x) * 4;
+    const size_t total = (size_t)M * N;
+    if (base >= total) return;
+    unsigned words[OZ_MAX_MODULI];
+    for (int j = 0; j < k.n; ++j) words[j] = *reinterpret_cast<const unsigned *>(Cres + j * c_stride + base);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const size_t idx = base + u;
+        if (idx >= total) break;
+        const int row = (int)(idx / N), col = (int)(idx % N);
+        const int sr = srow[row], sc = scol[col];
+        if (sr == OZ_BAD_SCALE || sc == OZ_BAD_SCALE) {
+            C[idx] = __builtin_nan("");
+            continue;
+        }
+        double L[OZ_LIMBS];
+#pragma unroll
+        for (int t = 0; t < OZ_LIMBS; ++t) L[t] = 0.0;
+        for (int j = 0; j < k.n; ++j) {
+            const double c = (double)(signed char)(words[j] >> (8 * u));
+#pragma unroll
+            for (int t = 0; t < OZ_LIMBS; ++t) L[t] = fma(c, k.w[j][t], L[t]);  // exact: |sum| < 2^36
+        }
+        // X_raw = sum_t L[t] 2^(24 t); subtract q M with q = round(X_raw / M)
+        double approx = 0.0;
+#pragma unroll
+        for (int t = OZ_LIMBS - 1; t >= 0; --t) approx = approx * 16777216.0 + L[t];
+        const double q = rint(approx / k.M_approx);
+#pragma unroll
+        for (int t = 0; t < OZ_LIMBS; ++t) L[t] = fma(-q, k.Mlimb[t], L[t]);  // exact
+        // balanced carry normalisation: every limb in [-2^23, 2^23]
+#pragma unroll
+        for (int t = 0; t < OZ_LIMBS - 1; ++t) {
+            const double carry = rint(L[t] * (1.0 / 16777216.0));
+            L[t] = fma(-carry, 16777216.0, L[t]);
+            L[t + 1] += carry;
+        }
+        // double-double Horner from the top limb
+        dd_t x{0.0, 0.0};
+#pragma unroll
+        for (int t = OZ_LIMBS - 1; t >= 0; --t) {
+            x.hi *= 16777216.0;
+            x.lo *= 16777216.0;
+            dd_t s = dd_two_sum(x.hi, L[t]);
+            s.lo += x.lo;
+            x = dd_two_sum(s.hi, s.lo);
+        }
+        C[idx] = ldexp(x.hi + x.lo, -(sr + sc));
     }
-    double L[OZ_LIMBS];
-#pragma unroll
-    for (int t = 0; t < OZ_LIMBS; ++t) L[t] = 0.0;
-    for (int j = 0; j < k.n; ++j) {
-        const double c = (double)Cres[j * c_stride + idx];
-#pragma unroll
-        for (int t = 0; t < OZ_LIMBS; ++t) L[t] = fma(c, k.w[j][t], L[t]);  // exact: |sum| < 2^36
-    }
-    // X_raw = sum_t L[t] 2^(24 t); subtract q M with q = round(X_raw / M)
-    double approx = 0.0;
-#pragma unroll
-    for (int t = OZ_LIMBS - 1; t >= 0; --t) approx = approx * 16777216.0 + L[t];
-    const double q = rint(approx / k.M_approx);
-#pragma unroll
-    for (int t = 0; t < OZ_LIMBS; ++t) L[t] = fma(-q, k.Mlimb[t], L[t]);  // exact
-    // balanced carry normalisation: every limb in [-2^23, 2^23]
-#pragma unroll
-    for (int t = 0; t < OZ_LIMBS - 1; ++t) {
-        const double carry = rint(L[t] * (1.0 / 16777216.0));
-        L[t] = fma(-carry, 16777216.0, L[t]);
-        L[t + 1] += carry;
-    }
-    // double-double Horner from the top limb
-    dd_t x{0.0, 0.0};
-#pragma unroll
-    for (int t = OZ_LIMBS - 1; t >= 0; --t) {
-        x.hi *= 16777216.0;
-        x.lo *= 16777216.0;
-        dd_t s = dd_two_sum(x.hi, L[t]);
-        s.lo += x.lo;
-        x = dd_two_sum(s.hi, s.lo);
-    }
-    C[idx] = ldexp(x.hi + x.lo, -(sr + sc));
 }
 
 // ---------------- host driver ----------------
@@ -362,7 +429,8 @@ template <int BM, int BN, int BK, int WM, int WN>
 static nrk_status launch_i8(int M, int N, int Kp, const signed char *A, size_t sa, const signed char *Bt, size_t sb,
                             signed char *Cres, size_t sc, const OzConsts &k, hipStream_t stream) {
     constexpr int threads = (BM / WM) * (BN / WN) * 64;
-    constexpr size_t lds = (size_t)2 * (BM + BN) * (BK + 16);
+    constexpr size_t lds = (size_t)2 * (BM + BN) * (BK + OZ_LDS_PAD);
+    static_assert(lds <= 160 * 1024, "tile does not fit in LDS");
     auto kern = gemm_i8_nt_mod_kernel<BM, BN, BK, WM, WN>;
     static bool attr_set = false;
     if (!attr_set) {
@@ -375,9 +443,10 @@ static nrk_status launch_i8(int M, int N, int Kp, const signed char *A, size_t s
     return NRK_OK;
 }
 
-nrk_status gemm_f64_ozaki(size_t m, size_t k, size_t n, const double *a, const double *b, double *c, int moduli,
+nrk_status gemm_f64_ozaki(size_t m, size_t k, size_t n, const double *a, const double *b, double *c, int moduli, int tile,
                           hipStream_t stream) {
     NRK_TRY(ensure_device());
+    if (tile < -1 || tile >= OZ_TILE_CONFIGS) return fail(NRK_ERR_INVALID_ARGUMENT, "INT8 tile config %d outside [0,%d)", tile, OZ_TILE_CONFIGS);
     if (moduli == 0) moduli = OZ_DEFAULT_MODULI;
     OzConsts kc;
     NRK_TRY(make_consts(moduli, &kc));
@@ -392,33 +461,47 @@ nrk_status gemm_f64_ozaki(size_t m, size_t k, size_t n, const double *a, const d
     if (P > 53) P = 53;
     if (P < 8) return fail(NRK_ERR_INVALID_ARGUMENT, "too few moduli (%d) for k = %zu", moduli, k);
     const size_t Mp = align_up(m, 256), Np = align_up(n, 256), Kp = align_up(k, 128);
-    const size_t sa = Mp * Kp, sb = Np * Kp, sc = m * n;
-    const size_t bytes = align_up((size_t)moduli * (sa + sb + sc), 256) + (m + n) * sizeof(int);
+    const size_t sa = Mp * Kp, sb = Np * Kp, sc = align_up(m * n, 16);
+    const size_t nchunk = ceil_div(k, (size_t)COL_ROWS);
+    const size_t res_bytes = align_up((size_t)moduli * (sa + sb + sc), 256);
+    const size_t bytes = res_bytes + align_up(nchunk * n * sizeof(double), 256) + (m + n) * sizeof(int);
     void *ws = nullptr;
     NRK_TRY(get_workspace(stream, bytes, &ws));
     signed char *Ares = (signed char *)ws;
     signed char *Bres = Ares + (size_t)moduli * sa;
     signed char *Cres = Bres + (size_t)moduli * sb;
-    int *srow = (int *)((char *)ws + align_up((size_t)moduli * (sa + sb + sc), 256));
+    double *pmax = (double *)((char *)ws + res_bytes);
+    int *srow = (int *)((char *)pmax + align_up(nchunk * n * sizeof(double), 256));
     int *scol = srow + m;
 
     hipLaunchKernelGGL(row_scale_kernel, dim3((unsigned)m), dim3(256), 0, stream, a, (int)m, (int)k, P, srow);
     NRK_HIP(hipGetLastError());
-    hipLaunchKernelGGL(col_scale_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, stream, b, (int)k, (int)n, P, scol);
+    hipLaunchKernelGGL(col_max_partial_kernel, dim3((unsigned)ceil_div(n, 256), (unsigned)nchunk), dim3(256), 0, stream, b,
+                       (int)k, (int)n, pmax);
     NRK_HIP(hipGetLastError());
-    hipLaunchKernelGGL((residue_kernel<false>), dim3((unsigned)(Kp / 32), (unsigned)(Mp / 32)), dim3(256), 0, stream, a,
-                       (int)m, (int)k, srow, kc, Ares, (int)Mp, (int)Kp);
+    hipLaunchKernelGGL(col_scale_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, stream, pmax, (int)nchunk, (int)n, P,
+                       scol);
     NRK_HIP(hipGetLastError());
-    hipLaunchKernelGGL((residue_kernel<true>), dim3((unsigned)(Np / 32), (unsigned)(Kp / 32)), dim3(256), 0, stream, b,
-                       (int)k, (int)n, scol, kc, Bres, (int)Np, (int)Kp);
+    const size_t nkc = Kp / 16;
+    hipLaunchKernelGGL((residue_kernel<false>), dim3((unsigned)ceil_div(Mp * nkc, 256)), dim3(256), 0, stream, a, (int)m,
+                       (int)k, srow, kc, Ares, (int)Mp, (int)Kp);
     NRK_HIP(hipGetLastError());
-    const bool big = ceil_div(m, 128) * ceil_div(n, 128) * moduli >= (size_t)cu_count() * 2;
-    if (big) {
-        NRK_TRY((launch_i8<128, 128, 128, 64, 64>((int)m, (int)n, (int)Kp, Ares, sa, Bres, sb, Cres, sc, kc, stream)));
-    } else {
-        NRK_TRY((launch_i8<64, 64, 128, 32, 32>((int)m, (int)n, (int)Kp, Ares, sa, Bres, sb, Cres, sc, kc, stream)));
+    hipLaunchKernelGGL((residue_kernel<true>), dim3((unsigned)(ceil_div(Np, 16) * ceil_div(nkc, 16))), dim3(256), 0, stream,
+                       b, (int)k, (int)n, scol, kc, Bres, (int)Np, (int)Kp);
+    NRK_HIP(hipGetLastError());
+    if (tile < 0) {  // default: the largest tiles that still give every CU a workgroup
+        if (ceil_div(m, 256) * ceil_div(n, 256) * moduli >= (size_t)cu_count()) tile = 3;
+        else if (ceil_div(m, 128) * ceil_div(n, 128) * moduli >= (size_t)cu_count() * 2) tile = 1;
+        else tile = 0;
     }
-    hipLaunchKernelGGL(crt_kernel, dim3((unsigned)ceil_div(m * n, 256)), dim3(256), 0, stream, Cres, sc, (int)m, (int)n,
+    switch (tile) {
+        case 0: NRK_TRY((launch_i8<64, 64, 128, 32, 32>((int)m, (int)n, (int)Kp, Ares, sa, Bres, sb, Cres, sc, kc, stream))); break;
+        case 1: NRK_TRY((launch_i8<128, 128, 128, 64, 64>((int)m, (int)n, (int)Kp, Ares, sa, Bres, sb, Cres, sc, kc, stream))); break;
+        case 2: NRK_TRY((launch_i8<256, 256, 128, 128, 64>((int)m, (int)n, (int)Kp, Ares, sa, Bres, sb, Cres, sc, kc, stream))); break;
+        case 3: NRK_TRY((launch_i8<256, 256, 64, 128, 64>((int)m, (int)n, (int)Kp, Ares, sa, Bres, sb, Cres, sc, kc, stream))); break;
+        default: NRK_TRY((launch_i8<256, 256, 64, 64, 128>((int)m, (int)n, (int)Kp, Ares, sa, Bres, sb, Cres, sc, kc, stream)));
+    }
+    hipLaunchKernelGGL(crt_kernel, dim3((unsigned)ceil_div(ceil_div(m * n, 4), 256)), dim3(256), 0, stream, Cres, sc, (int)m, (int)n,
                        srow, scol, kc, c);
     NRK_HIP(hipGetLastError());
     return NRK_OK;
@@ -432,7 +515,7 @@ extern "C" {
 
 nrk_status nrk_dev_gemm_f64_ozaki(size_t m, size_t k, size_t n, const double *a, const double *b, double *c, int32_t moduli,
                                   nrk_stream stream) {
-    return gemm_f64_ozaki(m, k, n, a, b, c, moduli, (hipStream_t)stream);
+    return gemm_f64_ozaki(m, k, n, a, b, c, moduli, -1, (hipStream_t)stream);
 }
 
 nrk_status nrk_gemm_f64_ozaki(size_t m, size_t k, size_t n, const double *a, const double *b, double *c, int32_t moduli) {
@@ -445,7 +528,7 @@ nrk_status nrk_gemm_f64_ozaki(size_t m, size_t k, size_t n, const double *a, con
     hipStream_t s = nullptr;
     if (m * k) NRK_HIP(hipMemcpyAsync(da, a, m * k * 8, hipMemcpyHostToDevice, s));
     if (k * n) NRK_HIP(hipMemcpyAsync(db, b, k * n * 8, hipMemcpyHostToDevice, s));
-    NRK_TRY(gemm_f64_ozaki(m, k, n, (const double *)da, (const double *)db, (double *)dc, moduli, s));
+    NRK_TRY(gemm_f64_ozaki(m, k, n, (const double *)da, (const double *)db, (double *)dc, moduli, -1, s));
     NRK_HIP(hipMemcpyAsync(c, dc, m * n * 8, hipMemcpyDeviceToHost, s));
     NRK_HIP(hipStreamSynchronize(s));
     return NRK_OK;

@@ -39,4 +39,11 @@ nrk_status zip(int op, const T *a, const T *b, T *y, size_t n, hipStream_t strea
 template <typename T>
 nrk_status gemm(size_t m, size_t k, size_t n, const T *a, const T *b, T *c, hipStream_t stream);
 
+// Split-precision GEMMs with an explicit tile configuration (-1: built-in choice):
+// FP32 as BF16 split products (gemm_bf16.hip), FP64 by the Ozaki scheme II (gemm_ozaki.hip).
+nrk_status gemm_f32_bf16x(size_t m, size_t k, size_t n, const float *a, const float *b, float *c, int products, int tile,
+                          hipStream_t stream);
+nrk_status gemm_f64_ozaki(size_t m, size_t k, size_t n, const double *a, const double *b, double *c, int moduli, int tile,
+                          hipStream_t stream);
+
 }  // namespace nrk

@@ -134,14 +134,8 @@ __device__ __forceinline__ uint64_t shfl_xor(uint64_t v, int m) {
     return ((uint64_t)hi << 32) | lo;
 }
 
-// Lane 0's value, for every lane (the wavefront is fully active where these are used).
+// Lane 0's value, for every lane (the wavefront is fully active where this is used).
 __device__ __forceinline__ unsigned lane0(unsigned v) { return (unsigned)__builtin_amdgcn_readfirstlane((int)v); }
-__device__ __forceinline__ uint64_t lane0(uint64_t v) {
-    return ((uint64_t)lane0((unsigned)(v >> 32)) << 32) | lane0((unsigned)(v & 0xffffffffu));
-}
-__device__ __forceinline__ double lane0(double v) {
-    return __builtin_bit_cast(double, lane0(__builtin_bit_cast(uint64_t, v)));
-}
 
 // (value, index) winners: larger (smaller) value wins, equal values go to the lower index.
 // Values are never NaN here (NaN elements never become winners).

@@ -23,6 +23,8 @@ static int config_count(int dtype) {
         case NRK_DT_F32:
         case NRK_DT_BF16: return 5;
         case NRK_DT_ROWSUM_F32: return 4;
+        case NRK_DT_F32_BF16X3:
+        case NRK_DT_F64_OZAKI: return 5;
         default: return 0;
     }
 }
@@ -48,6 +50,12 @@ static nrk_status run_config(int dtype, int config, int d0, int d1, int d2, cons
             set_reduce_blocks_per_cu(saved);
             return st;
         }
+        case NRK_DT_F32_BF16X3:
+            if (config < 0 || config > 4) return fail(NRK_ERR_INVALID_ARGUMENT, "config %d outside [0,4]", config);
+            return gemm_f32_bf16x(d0, d2, d1, (const float *)a, (const float *)b, (float *)c, 3, config, s);
+        case NRK_DT_F64_OZAKI:
+            if (config < 0 || config > 4) return fail(NRK_ERR_INVALID_ARGUMENT, "config %d outside [0,4]", config);
+            return gemm_f64_ozaki(d0, d2, d1, (const double *)a, (const double *)b, (double *)c, 16, config, s);
         default: return fail(NRK_ERR_INVALID_ARGUMENT, "unknown operation family %d", dtype);
     }
 }
@@ -255,8 +263,9 @@ nrk_status nrk_tune(int32_t family, int32_t count, const int32_t *d0, const int3
             cmax = std::max(cmax, (size_t)d0[i] * d1[i]);
         }
     }
-    const size_t es = family == NRK_DT_F64 ? 8 : (family == NRK_DT_BF16 ? 2 : 4);
-    const size_t cs = family == NRK_DT_F64 ? 8 : 4;
+    const bool f64 = family == NRK_DT_F64 || family == NRK_DT_F64_OZAKI;
+    const size_t es = f64 ? 8 : (family == NRK_DT_BF16 ? 2 : 4);
+    const size_t cs = f64 ? 8 : 4;
     void *a = nullptr, *b = nullptr, *c = nullptr;
     hipStream_t s = nullptr;
     NRK_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
