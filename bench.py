@@ -1,23 +1,30 @@
 #!/usr/bin/env python3
-"""Benchmark of the fused elementwise -> reduction workload on one MI355X.
+"""Benchmark of the kernels dispatch path on one MI355X.
 
-Workload (one "step"): for x of shape (8192, 8192) with x ~ N(0,1), in f32 and in f64,
+Headline workload (one "step"): for x of shape (8192, 8192) with x ~ N(0,1), in f32 and in f64,
 y = 1.5 * x - 0.25 (two roundings) followed by sum, max, argmax and var along axis 1 AND over the
 whole array -- all eight results from ONE fused pass over x per dtype -- plus the exact sum of an
 i64 array of 2^28 values uniform in [-2^20, 2^20]. Inputs are resident in HBM before timing.
 
 value = algorithmic bytes of one step (each input read once: 2^26*4 + 2^26*8 + 2^28*8 B) divided
-by the measured step time, in GB/s. The roofline object reports the dominant kernel's achieved
-HBM bandwidth against MI355X's 8 TB/s; cpu_baseline times the oracle's restatement of the
-reference's CPU path (unfused: two map passes, then each reduction separately) on the host.
+by the measured step time, in GB/s; the timed region holds the K steps and nothing else. A second
+pass times each call on its own (HIP events on the launching stream) for the per-kernel figures
+and the roofline of the dominant kernel against MI355X's 8 TB/s.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+Also reported (same JSON line): the split-precision GEMMs against the native MFMA GEMMs (f32
+8192^3 BF16x3/x6, f64 4096^3 and 8192^3 Ozaki-II with 14/15/16 moduli; error of sampled entries
+against correctly rounded dot products) and the ragged batch (4096 bf16 GEMMs + 65536 f32 row
+sums, one persistent launch replayed from a hipGraph vs the per-call loop). cpu_baseline times the
+oracle's restatement of the reference's CPU path (unfused) on this host.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-gemm] [--no-ragged]
 """
 from __future__ import annotations
 
 import argparse
 import ctypes as C
 import json
+import math
 import os
 import sys
 import time
@@ -39,6 +46,8 @@ def parse():
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-gemm", action="store_true")
+    p.add_argument("--no-ragged", action="store_true")
     return p.parse_args()
 
 
@@ -50,7 +59,7 @@ def main() -> int:
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
-                          "(multi-GPU sharding is described in DESIGN.md, not launched)"}))
+                          "(how the path would shard is a design note, not a launched configuration)"}))
         return 2
 
     from numrs_amd import device, kernels, rng
@@ -112,35 +121,42 @@ def main() -> int:
     if got != want:
         raise SystemExit(f"i64 sum mismatch: {got} != {want}")
 
-    # per-kernel events inside the timed region (same stream as the launches)
-    evs = [[device.Event() for _ in range(len(kern) + 1)] for _ in range(args.steps)]
+    # ---- the timed region: K steps, nothing else on the stream
     start, stop = device.Event(), device.Event()
     device.synchronize()
     start.record(s)
-    for st in range(args.steps):
-        evs[st][0].record(s)
-        for j, (_, f, _) in enumerate(kern):
+    for _ in range(args.steps):
+        for _, f, _ in kern:
             f()
-            evs[st][j + 1].record(s)
     stop.record(s)
     stream.synchronize()
     device.synchronize()
-    total_ms = start.elapsed_ms(stop)
-    ms_per_step = total_ms / args.steps
-    per = {name: float(np.mean([evs[st][j].elapsed_ms(evs[st][j + 1]) for st in range(args.steps)]))
-           for j, (name, _, _) in enumerate(kern)}
+    ms_per_step = start.elapsed_ms(stop) / args.steps
+    value = step_bytes / (ms_per_step * 1e-3) / 1e9
+    log(f"step {ms_per_step * 1e3:.1f} us, {value:.1f} GB/s")
+
+    # ---- per-call durations: each call launched K times back to back between two events
+    per = {}
+    for name, f, _ in kern:
+        a, b = device.Event(), device.Event()
+        a.record(s)
+        for _ in range(args.steps):
+            f()
+        b.record(s)
+        stream.synchronize()
+        per[name] = a.elapsed_ms(b) / args.steps
     kernels_out = {}
-    for name, _, b in kern:
-        gbs = b / (per[name] * 1e-3) / 1e9
-        kernels_out[name] = {"ms": round(per[name], 5), "bytes": b, "GB/s": round(gbs, 1),
+    for name, _, nbytes in kern:
+        gbs = nbytes / (per[name] * 1e-3) / 1e9
+        kernels_out[name] = {"ms": round(per[name], 5), "bytes": nbytes, "GB/s": round(gbs, 1),
                              "frac_hbm": round(gbs / HBM_PEAK_GBS, 4)}
     dom = max(kern, key=lambda k: per[k[0]])
     dom_gbs = dom[2] / (per[dom[0]] * 1e-3) / 1e9
-    value = step_bytes / (ms_per_step * 1e-3) / 1e9
+    del x64
 
-    cpu = None
-    if not args.no_cpu_baseline:
-        cpu = cpu_baseline(x32, x64, xi, step_bytes)
+    gemm = None if args.no_gemm else bench_gemm(device, rng, call, s, stream)
+    ragged = None if args.no_ragged else bench_ragged(device, rng, call, s, stream)
+    cpu = None if args.no_cpu_baseline else cpu_baseline(x32, d64, xi, step_bytes)
 
     line = {
         "metric": "GEMM TFLOP/s (split-precision vs native, same err) + reduction GB/s % of HBM",
@@ -162,35 +178,217 @@ def main() -> int:
         "kernels": kernels_out,
         "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": round(dom_gbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(dom_gbs / HBM_PEAK_GBS, 4), "traffic": None},
+        "gemm": gemm,
+        "ragged": ragged,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
     return 0
 
 
-def cpu_baseline(x32, x64, xi, step_bytes):
-    """The oracle's restatement of the reference's unfused CPU sequence, timed on this host."""
+# ---------------------------------------------------------------------------------------------
+# split-precision GEMMs
+
+def _split(x):
+    c = x * 134217729.0  # Veltkamp split for FP64 (2^27 + 1)
+    hi = c - (c - x)
+    return hi, x - hi
+
+
+def _exact_dot(a: np.ndarray, b: np.ndarray) -> float:
+    """Correctly rounded sum_k a_k b_k of FP64 vectors: Dekker's exact products + math.fsum."""
+    ah, al = _split(a)
+    bh, bl = _split(b)
+    p = a * b
+    e = al * bl - (((p - ah * bh) - al * bh) - ah * bl)
+    return math.fsum(np.concatenate([p, e]).tolist())
+
+
+def _sampled_error(A, B, Cm, idx) -> float:
+    """max over sampled (i, j) of |C_ij - C*_ij| / sum_k |A_ik||B_kj| (the GEMM error-bound scale)."""
+    worst = 0.0
+    for i, j in idx:
+        a = A[i].astype(np.float64)
+        b = B[:, j].astype(np.float64)
+        scale = float(np.abs(a) @ np.abs(b))
+        worst = max(worst, abs(float(Cm[i, j]) - _exact_dot(a, b)) / scale)
+    return worst
+
+
+def bench_gemm(device, rng, call, s, stream):
+    def timeit(f, reps):
+        f()
+        stream.synchronize()
+        a, b = device.Event(), device.Event()
+        a.record(s)
+        for _ in range(reps):
+            f()
+        b.record(s)
+        stream.synchronize()
+        return a.elapsed_ms(b) / reps
+
+    out = {}
+    pick = np.random.default_rng(11)
+    cases = [("f32_8192", np.float32, 8192), ("f64_4096", np.float64, 4096), ("f64_8192", np.float64, 8192)]
+    for tag, dt, n in cases:
+        t0 = time.time()
+        A = rng.scaled_pow2(rng.BASE_SEED + 40 + n, n * n, dtype=dt).reshape(n, n)
+        B = rng.scaled_pow2(rng.BASE_SEED + 41 + n, n * n, dtype=dt).reshape(n, n)
+        dA, dB = device.DeviceArray.upload(A), device.DeviceArray.upload(B)
+        dC = device.DeviceArray((n, n), dt)
+        idx = [(int(pick.integers(n)), int(pick.integers(n))) for _ in range(24)]
+        flop = 2.0 * n ** 3
+        pa, pb, pc = C.c_void_p(dA.ptr), C.c_void_p(dB.ptr), C.c_void_p(dC.ptr)
+        if dt == np.float32:
+            runs = [("native_fp32", lambda: call("nrk_dev_gemm_f32", n, n, n, pa, pb, pc, C.c_void_p(s)), 1)]
+            for p in (3, 6):
+                runs.append((f"bf16x{p}", (lambda p=p: call("nrk_dev_gemm_f32_bf16x", n, n, n, pa, pb, pc, p, C.c_void_p(s))), p))
+        else:
+            runs = [("native_fp64", lambda: call("nrk_dev_gemm_f64", n, n, n, pa, pb, pc, C.c_void_p(s)), 1)]
+            for mo in (14, 15, 16):
+                runs.append((f"ozaki{mo}", (lambda mo=mo: call("nrk_dev_gemm_f64_ozaki", n, n, n, pa, pb, pc, mo, C.c_void_p(s))), mo))
+        res = {}
+        for name, f, products in runs:
+            ms = timeit(f, 3)
+            err = _sampled_error(A, B, dC.download(), idx)
+            res[name] = {"ms": round(ms, 3), "TFLOP/s": round(flop / ms / 1e9, 2), "max_rel_err": float(f"{err:.3e}"),
+                         "mfma_products": products}
+        native = res[runs[0][0]]
+        same_err = [k for k in res if k != runs[0][0] and res[k]["max_rel_err"] <= native["max_rel_err"]]
+        best = min(same_err, key=lambda k: res[k]["ms"]) if same_err else None
+        res["best_split_at_native_err"] = best
+        res["speedup_vs_native"] = round(native["ms"] / res[best]["ms"], 3) if best else None
+        out[tag] = res
+        log(f"gemm {tag}: " + ", ".join(f"{k} {v['ms']}ms err {v['max_rel_err']}" for k, v in res.items() if isinstance(v, dict))
+            + f"  ({time.time() - t0:.1f}s)")
+        del dA, dB, dC, A, B
+    out["error_measure"] = ("max over 24 sampled entries of |C - C*| / sum_k |A_ik||B_kj|, C* = correctly rounded "
+                            "dot product (Dekker products + fsum); inputs sign*2^e*u, e in [-10,10], u in [1,2)")
+    return out
+
+
+# ---------------------------------------------------------------------------------------------
+# ragged batch: 4096 bf16 GEMMs + 65536 f32 row sums in one persistent launch
+
+def bench_ragged(device, rng, call, s, stream):
+    from numrs_amd.grouped import Graph, GroupedPlan, to_bf16_bits
+
+    t0 = time.time()
+    seed = rng.BASE_SEED + 5000
+
+    def log_uniform_int(sd, n, lo, hi):
+        u = rng.uniform(sd, n, 0.0, 1.0)
+        return np.floor(np.exp(np.log(lo) + u * (np.log(hi + 1) - np.log(lo)))).astype(np.int64).clip(lo, hi)
+
+    n_gemm, n_rows = 4096, 65536
+    m, n, k = (log_uniform_int(seed + i, n_gemm, 16, 512) for i in range(3))
+    a_off = np.concatenate([[0], np.cumsum(m * k)])
+    b_off = np.concatenate([[0], np.cumsum(k * n)])
+    c_off = np.concatenate([[0], np.cumsum(m * n)])
+    lens = log_uniform_int(seed + 7, n_rows, 32, 65536)
+    r_off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    flops = float(np.sum(2.0 * m * n * k))
+    row_bytes = int(r_off[-1]) * 4
+
+    # operands: a 16 Mi-element block of N(0,1) values repeated over each arena (timing only)
+    block = rng.normal(seed + 9, 1 << 24, np.float32)
+    bblock = to_bf16_bits(block)
+
+    def arena(total, src):
+        buf = device.DeviceArray((int(total),), src.dtype)
+        blk = device.DeviceArray.upload(src)
+        done = 0
+        while done < total:
+            cnt = min(len(src), int(total) - done)
+            call("nrk_memcpy_dtod", C.c_void_p(buf.ptr + done * src.itemsize), C.c_void_p(blk.ptr), cnt * src.itemsize, C.c_void_p(s))
+            done += cnt
+        stream.synchronize()
+        return buf
+
+    dA, dB = arena(a_off[-1], bblock), arena(b_off[-1], bblock)
+    dC = device.DeviceArray((int(c_off[-1]),), np.float32)
+    dX = arena(r_off[-1], block)
+    dR = device.DeviceArray((n_rows,), np.float32)
+    descs = [(int(m[g]), int(n[g]), int(k[g]), dA.ptr + 2 * int(a_off[g]), dB.ptr + 2 * int(b_off[g]), dC.ptr + 4 * int(c_off[g]))
+             for g in range(n_gemm)]
+    plan = GroupedPlan(descs, dX.ptr, r_off, dR.ptr)
+    info = plan.info()
+    graph = Graph(s, lambda: plan.run(s))
+
+    def timed(f, reps):
+        f()
+        stream.synchronize()
+        a, b = device.Event(), device.Event()
+        a.record(s)
+        for _ in range(reps):
+            f()
+        b.record(s)
+        stream.synchronize()
+        return a.elapsed_ms(b) / reps
+
+    ms_graph = timed(graph.launch, 5)
+    ms_plan = timed(lambda: plan.run(s), 5)
+
+    from numrs_amd._lib import GemmDesc
+
+    dl = [GemmDesc(*d[:3], 0, *d[3:]) for d in descs]
+    xs = [(dX.ptr + 4 * int(r_off[r]), int(lens[r]), dR.ptr + 4 * r) for r in range(n_rows)]
+
+    def loop():
+        for d in dl:
+            call("nrk_dev_gemm_bf16", C.byref(d), C.c_void_p(s))
+        for (px, ln, po) in xs:
+            call("nrk_dev_sum_f32", C.c_void_p(px), C.c_size_t(ln), C.c_void_p(po), C.c_void_p(s))
+
+    ms_loop = timed(loop, 1)
+    graph.close()
+    plan.close()
+    log(f"ragged: graph {ms_graph:.3f} ms, plan {ms_plan:.3f} ms, per-call loop {ms_loop:.3f} ms ({time.time() - t0:.1f}s)")
+    return {"gemms": n_gemm, "rows": n_rows, "gemm_flop": flops, "row_bytes": row_bytes, "plan": info,
+            "graph_ms": round(ms_graph, 4), "launch_ms": round(ms_plan, 4), "per_call_loop_ms": round(ms_loop, 3),
+            "speedup_vs_loop": round(ms_loop / ms_graph, 2),
+            "graph_gemm_TFLOP/s_if_alone": round(flops / ms_graph / 1e9, 2),
+            "graph_row_GB/s_if_alone": round(row_bytes / ms_graph / 1e6, 1),
+            "data": "m,n,k log-uniform-int [16,512]; row lengths log-uniform-int [32,65536]; operand values: one seeded "
+                    "N(0,1) block of 2^24 elements repeated (timing only)"}
+
+
+# ---------------------------------------------------------------------------------------------
+
+def cpu_baseline(x32, d64, xi, step_bytes):
+    """The oracle's restatement of the reference's unfused CPU sequence, timed on this host:
+    minimum over 3 samples of one full step."""
     import oracle
 
+    x64 = d64.download()
     threads = max(1, min(16, os.cpu_count() or 1))
     oracle.set_threads(threads)
-    t = time.perf_counter()
-    for x in (x32, x64):
-        y = oracle.ref_map([2], [1.5], x)        # &x * 1.5   (operations.rs:462, one map pass)
-        y = oracle.ref_map([0], [-0.25], y)      # ... + -0.25 (operations.rs:448, second pass)
-        for op in (0, 4, 6):                     # math::sum / max / argmax along axis 1
-            oracle.ref_axis(op, y, ROWS, COLS, 1)
-        oracle.ref_axis(2, y, ROWS, COLS, 1, 0)  # math::var along axis 1
-        oracle.ref("sum", y)                     # kernels::reduce::sum (chunked, threads)
-        oracle.ref("max", y)
-        oracle.ref("argmax", y)
-        oracle.ref("var", y, 0)
-    oracle.sum_i64(xi)                           # Array::<i64>::sum (serial fold)
-    dt = time.perf_counter() - t
-    return {"value": round(step_bytes / dt / 1e9, 3), "unit": "GB/s", "cores": threads, "kind": "port",
-            "seconds": round(dt, 3),
-            "sample": "one full step of the same workload (both dtypes + the i64 sum) through the C restatement; "
-                      "axis folds single-threaded as in the reference, chunked full reductions on the pool"}
+    best = None
+    for _ in range(3):
+        t = time.perf_counter()
+        for x in (x32, x64):
+            y = oracle.ref_map([2], [1.5], x)        # &x * 1.5   (operations.rs:462, one map pass)
+            y = oracle.ref_map([0], [-0.25], y)      # ... + -0.25 (operations.rs:448, second pass)
+            for op in (0, 4, 6):                     # math::sum / max / argmax along axis 1
+                oracle.ref_axis(op, y, ROWS, COLS, 1)
+            oracle.ref_axis(2, y, ROWS, COLS, 1, 0)  # math::var along axis 1
+            oracle.ref("sum", y)                     # kernels::reduce::sum (chunked, threads)
+            oracle.ref("max", y)
+            oracle.ref("argmax", y)
+            oracle.ref("var", y, 0)
+        oracle.sum_i64(xi)                           # Array::<i64>::sum (serial fold)
+        dt = time.perf_counter() - t
+        best = dt if best is None else min(best, dt)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
+    return {"value": round(step_bytes / best / 1e9, 3), "unit": "GB/s", "cores": threads, "kind": "port",
+            "seconds": round(best, 3), "nproc": os.cpu_count(), "cpu": model,
+            "sample": "one full step of the same workload (both dtypes + the i64 sum) through the C restatement, "
+                      "minimum of 3; axis folds single-threaded as in the reference, chunked full reductions on the pool"}
 
 
 if __name__ == "__main__":
