@@ -38,6 +38,13 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 ROWS = COLS = 8192
 N_I64 = 1 << 28
+# HBM bytes per launch measured with `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE` over
+# `python3 bench.py --pmc-pass`, each counter scaled by its reading for a plain 2 GiB copy
+# (FETCH_SIZE reads half the bytes moved: x2.000; WRITE_SIZE x1.000). The run and its figures are
+# kept in profiles/hbm_traffic_reduce_step.md; the i64 sum reads 1.00006x its algorithmic bytes.
+MEASURED_TRAFFIC = {"sum_i64": (1048609.5 * 2 + 64.28125) * 1024,
+                    "fused_map_reduce_f32": (131255.25 * 2 + 704.0) * 1024,
+                    "fused_map_reduce_f64": (262348.4375 * 2 + 704.0) * 1024}
 
 
 def parse():
@@ -48,7 +55,43 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-gemm", action="store_true")
     p.add_argument("--no-ragged", action="store_true")
+    p.add_argument("--pmc-pass", action="store_true",
+                   help="run each hot call once after a plain copy of known size (2 GiB, the counter "
+                        "calibration), for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE; prints no JSON line")
     return p.parse_args()
+
+
+def pmc_pass() -> int:
+    """Dispatches, in order: a plain copy of 2^28 f64 (2 GiB read + 2 GiB written; the elementwise
+    map with an empty chain), the fused f32 and f64 calls and the i64 sum of the headline step."""
+    from numrs_amd import device, kernels, rng
+    from numrs_amd._lib import ReduceOutputs, call
+
+    device.init(0)
+    stream = device.Stream()
+    s = stream.handle
+    src = device.DeviceArray.upload(rng.normal(rng.BASE_SEED + 6, N_I64, np.float64))
+    dst = device.DeviceArray((N_I64,), np.float64)
+    call("nrk_dev_map_f64", None, 0, C.c_void_p(src.ptr), C.c_void_p(dst.ptr), C.c_size_t(N_I64), C.c_void_p(s))
+    stream.synchronize()
+    del src, dst
+    ops, n_ops = kernels.chain(("mul", 1.5), ("add", -0.25))
+    for dt, fn, seed in ((np.float32, "nrk_dev_map_reduce_f32", 2), (np.float64, "nrk_dev_map_reduce_f64", 3)):
+        x = device.DeviceArray.upload(rng.normal(rng.BASE_SEED + seed, ROWS * COLS, dt))
+        ro, fo, keep = ReduceOutputs(), ReduceOutputs(), []
+        for f in ("sum", "max", "argmax", "var"):
+            r, g = device.DeviceArray((ROWS,), np.float64), device.DeviceArray((1,), np.float64)
+            keep += [r, g]
+            setattr(ro, f, r.ptr)
+            setattr(fo, f, g.ptr)
+        call(fn, C.cast(ops, C.c_void_p), n_ops, C.c_void_p(x.ptr), ROWS, COLS, 0, C.byref(ro), C.byref(fo), C.c_void_p(s))
+        stream.synchronize()
+    xi = device.DeviceArray.upload(rng.int_uniform(rng.BASE_SEED + 4, N_I64, -(1 << 20), 1 << 20))
+    out = device.DeviceArray((1,), np.int64)
+    call("nrk_dev_sum_i64", C.c_void_p(xi.ptr), C.c_size_t(N_I64), C.c_void_p(out.ptr), C.c_void_p(s))
+    stream.synchronize()
+    log("pmc pass done")
+    return 0
 
 
 def log(msg: str) -> None:
@@ -61,6 +104,8 @@ def main() -> int:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
                           "(how the path would shard is a design note, not a launched configuration)"}))
         return 2
+    if args.pmc_pass:
+        return pmc_pass()
 
     from numrs_amd import device, kernels, rng
     from numrs_amd._lib import ReduceOutputs, call
@@ -114,6 +159,7 @@ def main() -> int:
     for _ in range(args.warmup):
         for _, f, _ in kern:
             f()
+    k_i64()
     stream.synchronize()
     # correctness guard on the measured data: the i64 sum must be exact
     got = int(isum.download()[0])
@@ -177,7 +223,8 @@ def main() -> int:
                    "frac_hbm_step": round(value / HBM_PEAK_GBS, 4)},
         "kernels": kernels_out,
         "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": round(dom_gbs, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(dom_gbs / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(dom_gbs / HBM_PEAK_GBS, 4),
+                     "traffic": MEASURED_TRAFFIC.get(dom[0]), "algorithmic_bytes": dom[2]},
         "gemm": gemm,
         "ragged": ragged,
         "cpu_baseline": cpu,
