@@ -200,6 +200,17 @@ def main() -> int:
     dom_gbs = dom[2] / (per[dom[0]] * 1e-3) / 1e9
     del x64
 
+    # host-pointer boundary (the reference's &[T] slices): sum of x (f32, 256 MiB) through
+    # nrk_sum_f32, which stages the slice to the device; PCIe-inclusive, not the headline value
+    host_ms = []
+    for _ in range(3):
+        t = time.perf_counter()
+        kernels.sum_f32(x32.reshape(-1))
+        host_ms.append((time.perf_counter() - t) * 1e3)
+    host_path = {"call": "nrk_sum_f32 (host slice, staged to HBM)", "bytes": x32.nbytes, "ms": round(min(host_ms), 3),
+                 "GB/s": round(x32.nbytes / min(host_ms) / 1e6, 1)}
+    log(f"host-pointer sum_f32: {host_path['ms']} ms, {host_path['GB/s']} GB/s")
+
     gemm = None if args.no_gemm else bench_gemm(device, rng, call, s, stream)
     ragged = None if args.no_ragged else bench_ragged(device, rng, call, s, stream)
     cpu = None if args.no_cpu_baseline else cpu_baseline(x32, d64, xi, step_bytes)
@@ -225,6 +236,7 @@ def main() -> int:
         "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": round(dom_gbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(dom_gbs / HBM_PEAK_GBS, 4),
                      "traffic": MEASURED_TRAFFIC.get(dom[0]), "algorithmic_bytes": dom[2]},
+        "host_pointer_path": host_path,
         "gemm": gemm,
         "ragged": ragged,
         "cpu_baseline": cpu,
