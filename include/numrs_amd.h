@@ -60,6 +60,9 @@ const char *nrk_status_string(nrk_status status);
 const char *nrk_last_error(void);
 /* Library version as "MAJOR.MINOR.PATCH". */
 const char *nrk_version(void);
+/* Compute units of the calling thread's device (the built-in tile heuristics depend on it). No
+ * reference counterpart: the reference sizes its CPU tiers by rayon's thread count instead. */
+nrk_status nrk_device_cu_count(int32_t *count);
 
 nrk_status nrk_device_malloc(void **ptr, size_t bytes);
 nrk_status nrk_device_free(void *ptr);

@@ -40,6 +40,7 @@ class GemmDesc(C.Structure):
 # name -> argtypes (all return nrk_status = int32)
 _SIGS: dict[str, list] = {
     "nrk_init": [C.c_int],
+    "nrk_device_cu_count": [C.POINTER(c_i32)],
     "nrk_device_malloc": [C.POINTER(vp), c_size],
     "nrk_device_free": [vp],
     "nrk_memcpy_htod": [vp, vp, c_size, vp],

@@ -208,6 +208,13 @@ const char *nrk_status_string(nrk_status status) {
 const char *nrk_last_error(void) { return g_last_error; }
 const char *nrk_version(void) { return "0.1.0"; }
 
+nrk_status nrk_device_cu_count(int32_t *count) {
+    if (!count) return fail(NRK_ERR_INVALID_ARGUMENT, "count is NULL");
+    NRK_TRY(ensure_device());
+    *count = cu_count();
+    return NRK_OK;
+}
+
 nrk_status nrk_device_malloc(void **ptr, size_t bytes) {
     if (!ptr) return fail(NRK_ERR_INVALID_ARGUMENT, "ptr is NULL");
     NRK_TRY(ensure_device());

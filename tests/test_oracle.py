@@ -265,3 +265,49 @@ def test_planted_mistake_is_caught(oracle, known):
     pins = known["determinism_pins"]
     assert F64_BITS(oracle.ref("sum", x[:-1])) != pins["sum_f64_bits"]
     assert F64_BITS(oracle.ref("var", x, 1)) != pins["var_ddof0_f64_bits"]
+
+
+def _mlp_case(K=3, N=9, seed=31):
+    from oracle import mlp
+
+    r = np.random.default_rng(seed)
+    p = r.normal(0, 0.4, mlp.param_count(K))
+    x = r.normal(0, 1, (N, 6))
+    t = r.normal(0, 1, (N, K))
+    mask = (r.uniform(size=(N, K)) > 0.25).astype(np.float64)
+    return mlp, p, x, t, mask
+
+
+def test_mlp_oracle_gradient_matches_finite_differences():
+    """Central differences (h = 1e-6, FP64) pin the analytic backward of the MLP oracle."""
+    mlp, p, x, t, mask = _mlp_case()
+    _, _, g = mlp.loss_and_grad(p, 3, x, t, mask)
+    h = 1e-6
+    fd = np.empty_like(p)
+    for i in range(len(p)):
+        e = np.zeros_like(p)
+        e[i] = h
+        fd[i] = (mlp.loss_and_grad(p + e, 3, x, t, mask)[1] - mlp.loss_and_grad(p - e, 3, x, t, mask)[1]) / (2 * h)
+    np.testing.assert_allclose(g, fd, rtol=1e-5, atol=1e-8)
+
+
+def test_mlp_oracle_planted_mistake_is_caught():
+    """A backward that forgets the ReLU mask of the second layer no longer matches differences."""
+    mlp, p, x, t, mask = _mlp_case(seed=32)
+    K = 3
+    W1, b1, W2, b2, W3, b3 = mlp.unpack(p, K)
+    y, (z1, h1, z2, h2) = mlp.forward(p, K, x)
+    dy = 2.0 * (y - t) * mask / mask.sum()
+    wrong_gb2 = (dy @ W3).sum(axis=0)  # missing * (z2 > 0)
+    _, _, g = mlp.loss_and_grad(p, K, x, t, mask)
+    o = 6 * 64 + 64 + 64 * 64
+    assert (z2 <= 0).any() and not np.allclose(wrong_gb2, g[o:o + 64])
+
+
+def test_mlp_oracle_adam_closed_form():
+    """First Adam step from m = v = 0 moves every parameter by lr * sign(g) (up to eps)."""
+    from oracle import mlp
+
+    g = np.array([0.5, -2.0, 1e-3, -7.0])
+    p1, m, v = mlp.adam(np.zeros(4), g, np.zeros(4), np.zeros(4), 1, 0.01)
+    np.testing.assert_allclose(p1, -0.01 * np.sign(g), rtol=1e-4)
