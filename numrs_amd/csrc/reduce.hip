@@ -106,6 +106,12 @@ struct TreeParams {
     size_t cols;
     double ddof;
     OutPtrs<T> o;
+    // fused last level (one set, per_out <= TREE_SPAN): the workgroup that finishes last merges
+    // the level's output records and writes the results; `counter` is zero at launch and is
+    // reset to zero by that workgroup
+    int fuse_last;
+    OutPtrs<T> last_o;
+    unsigned int *counter;
 };
 
 // ---------------- small device utilities ----------------
@@ -544,24 +550,52 @@ template <typename T, int FLAGS>
 __global__ __launch_bounds__(256) void tree_kernel(TreeParams<T> t) {
     const size_t g = (blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6;
     const unsigned lane = threadIdx.x & 63;
-    if (g >= t.nsets * t.per_out) return;  // whole wavefronts exit together
-    const size_t set = g / t.per_out, b = g - set * t.per_out;
-    const size_t first = set * t.per;
-    const size_t lo = first + b * TREE_SPAN, hi = min(first + t.per, lo + TREE_SPAN);
-    Part v[TREE_LOADS];
+    if (g < t.nsets * t.per_out) {  // whole wavefronts take the same branch
+        const size_t set = g / t.per_out, b = g - set * t.per_out;
+        const size_t first = set * t.per;
+        const size_t lo = first + b * TREE_SPAN, hi = min(first + t.per, lo + TREE_SPAN);
+        Part v[TREE_LOADS];
 #pragma unroll
-    for (int k = 0; k < TREE_LOADS; ++k) {
-        const size_t s = lo + lane + 64 * k;
-        v[k] = s < hi ? part_load<FLAGS>(t.in, s) : part_empty();
+        for (int k = 0; k < TREE_LOADS; ++k) {
+            const size_t s = lo + lane + 64 * k;
+            v[k] = s < hi ? part_load<FLAGS>(t.in, s) : part_empty();
+        }
+        Part a = v[0];
+#pragma unroll
+        for (int k = 1; k < TREE_LOADS; ++k) part_merge<FLAGS>(a, v[k]);
+        wave_merge<FLAGS>(a);
+        if (lane == 0) {
+            const unsigned fn = (t.in.flags[lo] >> 1) & 1u;
+            if (t.results) write_outputs<T>(t.o, set, a, t.ddof, fn, t.rowmode ? (uint64_t)set * t.cols : 0);
+            if (t.keep) part_store<FLAGS>(t.out, g, a, fn);
+        }
     }
-    Part a = v[0];
+    if (!t.fuse_last) return;  // uniform over the grid
+    // The last workgroup to finish merges the per_out records of this level (one set) in a fixed
+    // order -- lane l folds records l, l+64, ..., then the butterfly -- whichever workgroup it is.
+    __shared__ unsigned is_last;
+    __threadfence();  // release this workgroup's records (device scope)
+    __syncthreads();
+    if (threadIdx.x == 0) is_last = atomicAdd(t.counter, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!is_last) return;  // uniform over the workgroup
+    __threadfence();       // acquire the other workgroups' records
+    if (threadIdx.x < 64) {
+        Part v[TREE_LOADS];
 #pragma unroll
-    for (int k = 1; k < TREE_LOADS; ++k) part_merge<FLAGS>(a, v[k]);
-    wave_merge<FLAGS>(a);
-    if (lane == 0) {
-        const unsigned fn = (t.in.flags[lo] >> 1) & 1u;
-        if (t.results) write_outputs<T>(t.o, set, a, t.ddof, fn, t.rowmode ? (uint64_t)set * t.cols : 0);
-        if (t.keep) part_store<FLAGS>(t.out, g, a, fn);
+        for (int k = 0; k < TREE_LOADS; ++k) {
+            const size_t s = lane + 64 * k;
+            v[k] = s < t.per_out ? part_load<FLAGS>(t.out, s) : part_empty();
+        }
+        Part a = v[0];
+#pragma unroll
+        for (int k = 1; k < TREE_LOADS; ++k) part_merge<FLAGS>(a, v[k]);
+        wave_merge<FLAGS>(a);
+        if (lane == 0) {
+            const unsigned fn = (t.out.flags[0] >> 1) & 1u;
+            write_outputs<T>(t.last_o, 0, a, t.ddof, fn, 0);
+            *t.counter = 0u;  // the zero-at-launch invariant for the next kernel on this stream
+        }
     }
 }
 
@@ -652,7 +686,7 @@ static nrk_status launch_stream(const SegParams<T> &p, hipStream_t stream) {
 // the last level also stores the per-set records into `final_out` (index = set).
 template <typename T, int FLAGS>
 static nrk_status run_tree(Parts in, size_t nsets, size_t per, Parts tmp0, Parts tmp1, bool rowmode, size_t cols, double ddof,
-                           const OutPtrs<T> &o, bool keep, Parts final_out, hipStream_t stream) {
+                           const OutPtrs<T> &o, bool keep, Parts final_out, unsigned int *counter, hipStream_t stream) {
     int flip = 0;
     for (;;) {
         TreeParams<T> t{};
@@ -667,10 +701,14 @@ static nrk_status run_tree(Parts in, size_t nsets, size_t per, Parts tmp0, Parts
         t.cols = cols;
         t.ddof = ddof;
         t.o = t.results ? o : OutPtrs<T>{};
+        // one set whose next level would be the last: let this launch finish it (saves a launch)
+        t.fuse_last = counter && nsets == 1 && !keep && !t.results && t.per_out <= (size_t)TREE_SPAN;
+        t.last_o = t.fuse_last ? o : OutPtrs<T>{};
+        t.counter = counter;
         const size_t waves = nsets * t.per_out;
         hipLaunchKernelGGL((tree_kernel<T, FLAGS>), dim3((unsigned)ceil_div(waves, (size_t)4)), dim3(256), 0, stream, t);
         NRK_HIP(hipGetLastError());
-        if (t.results) return NRK_OK;
+        if (t.results || t.fuse_last) return NRK_OK;
         in = t.out;
         per = t.per_out;
         flip ^= 1;
@@ -679,15 +717,17 @@ static nrk_status run_tree(Parts in, size_t nsets, size_t per, Parts tmp0, Parts
 
 template <typename T, int VPT, int FLAGS>
 static nrk_status launch_all(SegParams<T> &p, bool want_row, bool want_full, const OutPtrs<T> &row, const OutPtrs<T> &full,
-                             const Parts &rowrec, const Parts &tmp0, const Parts &tmp1, hipStream_t stream) {
+                             const Parts &rowrec, const Parts &tmp0, const Parts &tmp1, unsigned int *counter,
+                             hipStream_t stream) {
     NRK_TRY((launch_stream<T, VPT, FLAGS>(p, stream)));
     const bool split_rows = want_row && p.cpr > 1;
     if (split_rows)
-        NRK_TRY((run_tree<T, FLAGS>(p.l1, p.rows, p.cpr, tmp0, tmp1, true, p.cols, p.ddof, row, want_full, rowrec, stream)));
+        NRK_TRY((run_tree<T, FLAGS>(p.l1, p.rows, p.cpr, tmp0, tmp1, true, p.cols, p.ddof, row, want_full, rowrec, nullptr,
+                                    stream)));
     if (want_full) {
         const Parts in = split_rows ? rowrec : p.l1;
         const size_t per = split_rows ? p.rows : p.nchunk;
-        NRK_TRY((run_tree<T, FLAGS>(in, 1, per, tmp0, tmp1, false, p.cols, p.ddof, full, false, Parts{}, stream)));
+        NRK_TRY((run_tree<T, FLAGS>(in, 1, per, tmp0, tmp1, false, p.cols, p.ddof, full, false, Parts{}, counter, stream)));
     }
     return NRK_OK;
 }
@@ -748,7 +788,7 @@ nrk_status seg_reduce(const Chain &chain, const T *x, size_t rows, size_t cols, 
     const size_t tmp_count = std::max(rows * ceil_div(p.cpr, (size_t)TREE_SPAN), ceil_div(std::max(rows, p.nchunk), (size_t)TREE_SPAN)) + 1;
     const size_t pbytes = parts_bytes(p.nchunk) + parts_bytes(rows) + 2 * parts_bytes(tmp_count) + 256;
     Scratch s;
-    NRK_TRY(get_scratch(stream, 0, pbytes, &s));
+    NRK_TRY(get_scratch(stream, 1, pbytes, &s));
     char *b = (char *)s.partials;
     p.l1 = carve_parts(b, p.nchunk);
     const Parts rowrec = carve_parts(b, rows);
@@ -758,7 +798,8 @@ nrk_status seg_reduce(const Chain &chain, const T *x, size_t rows, size_t cols, 
     const bool mx = (row.max || row.argmax || full.max || full.argmax);
     const bool mn = (row.min || row.argmin || full.min || full.argmin);
     const int flags = (m2 ? F_M2 : 0) | (mx ? F_MAX : 0) | (mn ? F_MIN : 0);
-#define NRK_LAUNCH_FLAGS(F) return launch_all<T, VPT, F>(p, want_row, want_full, row, full, rowrec, tmp0, tmp1, stream)
+#define NRK_LAUNCH_FLAGS(F) \
+    return launch_all<T, VPT, F>(p, want_row, want_full, row, full, rowrec, tmp0, tmp1, s.counters, stream)
     switch (flags) {
         case 0: NRK_LAUNCH_FLAGS(0);
         case 1: NRK_LAUNCH_FLAGS(1);
