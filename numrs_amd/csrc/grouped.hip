@@ -185,24 +185,41 @@ __device__ __forceinline__ void wave_dw(float &hi, float &lo) {
     }
 }
 
-// Sum of x[0..len) by one wavefront: lane l adds elements l, l+64, ... pairwise in blocks of 8.
+// Sum of x[0..len) by one wavefront. The (at most 3) elements before the first 16-byte boundary
+// and the (at most 3) after the last whole vector are added by single lanes; the body streams
+// 16-byte non-temporal vectors, lane l taking vectors l, l+64, ..., 8 per lane per step (8 KiB
+// in flight per wavefront), summed pairwise per lane and accumulated with TwoSum.
 __device__ __forceinline__ void wave_row_sum(const float *x, int64_t len, float &hi, float &lo) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    constexpr int NV = 8;
     const int lane = threadIdx.x & 63;
-    float acc = 0.f, comp = 0.f;  // per-lane Kahan-Babuska
-    for (int64_t base = 0; base < len; base += 64 * 8) {
-        float v[8];
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int64_t i = base + e * 64 + lane;
-            v[e] = i < len ? x[i] : 0.f;
-        }
-        const float p = __fadd_rn(__fadd_rn(__fadd_rn(v[0], v[1]), __fadd_rn(v[2], v[3])),
-                                  __fadd_rn(__fadd_rn(v[4], v[5]), __fadd_rn(v[6], v[7])));
+    float acc = 0.f, comp = 0.f;  // per-lane TwoSum accumulation
+    auto add = [&](float p) {
         float s, e;
         two_sum_f(acc, p, s, e);
         acc = s;
         comp = __fadd_rn(comp, e);
+    };
+    const int64_t head = min(len, (int64_t)(((16u - ((unsigned)(uintptr_t)x & 15u)) & 15u) >> 2));
+    if (lane < head) add(x[lane]);
+    const f4 *xv = reinterpret_cast<const f4 *>(x + head);
+    const int64_t nv = (len - head) >> 2;
+    for (int64_t base = 0; base < nv; base += 64 * NV) {
+        f4 v[NV];
+#pragma unroll
+        for (int e = 0; e < NV; ++e) {
+            const int64_t i = base + e * 64 + lane;
+            v[e] = f4{0.f, 0.f, 0.f, 0.f};
+            if (i < nv) v[e] = __builtin_nontemporal_load(xv + i);
+        }
+        float q[NV];
+#pragma unroll
+        for (int e = 0; e < NV; ++e) q[e] = __fadd_rn(__fadd_rn(v[e][0], v[e][1]), __fadd_rn(v[e][2], v[e][3]));
+        add(__fadd_rn(__fadd_rn(__fadd_rn(q[0], q[1]), __fadd_rn(q[2], q[3])),
+                      __fadd_rn(__fadd_rn(q[4], q[5]), __fadd_rn(q[6], q[7]))));
     }
+    const int64_t t0 = head + (nv << 2);
+    if (t0 + lane < len) add(x[t0 + lane]);
     hi = acc;
     lo = comp;
     wave_dw(hi, lo);
@@ -405,7 +422,12 @@ nrk_status nrk_grouped_plan_create(int32_t gemm_count, const nrk_gemm_desc *desc
         free_plan(p);
         return st;
     }
-    p->grid = std::max(1, std::min(p->n_items, cu_count() * 4));
+    // as many workgroups as are resident together: small GEMM tiles are latency bound, so every
+    // resident workgroup is another tile in flight
+    int resident = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, grouped_kernel, GB, 0) != hipSuccess || resident < 1)
+        resident = 4;
+    p->grid = std::max(1, std::min(p->n_items, cu_count() * resident));
     *out_plan = p;
     return NRK_OK;
 }
