@@ -40,11 +40,13 @@ ROWS = COLS = 8192
 N_I64 = 1 << 28
 # HBM bytes per launch measured with `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE` over
 # `python3 bench.py --pmc-pass`, each counter scaled by its reading for a plain 2 GiB copy
-# (FETCH_SIZE reads half the bytes moved: x2.000; WRITE_SIZE x1.000). The run and its figures are
-# kept in profiles/hbm_traffic_reduce_step.md; the i64 sum reads 1.00006x its algorithmic bytes.
-MEASURED_TRAFFIC = {"sum_i64": (1048609.5 * 2 + 64.28125) * 1024,
-                    "fused_map_reduce_f32": (131255.25 * 2 + 704.0) * 1024,
-                    "fused_map_reduce_f64": (262348.4375 * 2 + 704.0) * 1024}
+# (FETCH factor 2097152 / 1048662.5 KiB, WRITE factor 2097152 / 2097217.375 KiB). The run and its
+# figures are kept in profiles/hbm_traffic_reduce_step.md; the i64 sum fetches 0.99995x its
+# algorithmic bytes (no re-reads).
+_FETCH, _WRITE = 2097152 / 1048662.5, 2097152 / 2097217.375
+MEASURED_TRAFFIC = {"sum_i64": (1048608.875 * _FETCH + 16.0 * _WRITE) * 1024,
+                    "fused_map_reduce_f32": (131269.1875 * _FETCH + 704.0 * _WRITE) * 1024,
+                    "fused_map_reduce_f64": (262350.9375 * _FETCH + 704.125 * _WRITE) * 1024}
 
 
 def parse():
