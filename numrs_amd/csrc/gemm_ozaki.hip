@@ -29,9 +29,9 @@ namespace nrk {
 constexpr int OZ_MAX_MODULI = 20;
 constexpr int OZ_LIMBS = 7;       // 7 x 24-bit limbs = 168 bits > log2(M) for 20 moduli
 constexpr int OZ_DEFAULT_MODULI = 16;
-constexpr int OZ_TILE_CONFIGS = 5;  // INT8 GEMM tiles (BMxBNxBK, wave tile): 64x64x128 (32x32),
-                                     // 128x128x128 (64x64), 256x256x128 (128x64), 256x256x64 (128x64),
-                                     // 256x256x64 (64x128)
+constexpr int OZ_TILE_CONFIGS = 5;  // INT8 GEMM tiles (workgroup tile, wave tile): 64x64 (32x32),
+                                     // 128x128 (64x64), 128x256 (64x64), 256x256 (128x64),
+                                     // 256x256 (128x128)
 constexpr int OZ_BAD_SCALE = 100000;  // marks a row/column holding Inf/NaN
 
 // Pairwise-coprime odd moduli <= 255, largest first (greedy).
@@ -232,45 +232,76 @@ __global__ __launch_bounds__(256) void residue_kernel(const double *__restrict__
 // ---------------- 3. batched int8 GEMM with a mod-m epilogue ----------------
 typedef signed char i8x16_t __attribute__((ext_vector_type(16)));
 
-__device__ __forceinline__ signed char mod_i32(int v, int m) {
-    int r = v % m;  // sign of v
-    const int h = (m - 1) >> 1;
-    r = r > h ? r - m : r;
-    r = r < -h ? r + m : r;
-    return (signed char)r;
+// Cres[j] = (Ares[j] (Mp x Kp) * Btres[j] (Np x Kp)^T) mod m_j for modulus j = blockIdx.y, stored
+// in 16 x 16 blocks in the MFMA accumulator order (see crt_kernel).
+//
+// Operands move global -> LDS by the load unit itself (global_load_lds, 16 bytes per lane): no
+// staging registers and no LDS write instructions, so a wave's registers go to accumulators and
+// fragments. Three LDS buffers of one 64-deep K tile each: tile t+2 streams in while tile t is
+// multiplied, one barrier per tile.
+//
+// LDS layout of a buffer: A rows then Bt rows, 64 bytes (four 16-byte chunks) per row, chunk c of
+// row r stored at chunk position c ^ oz_swz(r). A fragment read puts lane l on row l % 16, chunk
+// l / 16. Measured on gfx950 (SQ_LDS_BANK_CONFLICT, 8192^3 kernel with the loads switched off):
+// xor with (r >> 1) & 3 makes these reads conflict-free and halves the LDS busy cycles; no
+// swizzle, and xors with r & 3, (r >> 2) & 3 or ((r >> 1) & 3) ^ ((r >> 3) & 1), all leave them
+// 2-way conflicted.
+constexpr int OZ_DMA_BK = 64;
+__device__ __forceinline__ int oz_swz(int r) { return (r >> 1) & 3; }
+
+__device__ __forceinline__ void oz_lds_dma16(const signed char *g, signed char *l) {
+    __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1))) *)g,
+                                     (void __attribute__((address_space(3))) *)l, 16, 0, 0);
 }
+// s_waitcnt vmcnt(n) lgkmcnt(0), expcnt unconstrained (gfx9 encoding: vmcnt in bits 3:0 and
+// 15:14, expcnt 6:4, lgkmcnt 11:8).
+#define OZ_WAIT_VM_LGKM0(n) __builtin_amdgcn_s_waitcnt(((n) & 15) | ((((n) >> 4) & 3) << 14) | (0x7 << 4))
 
-// Bytes of padding per LDS row. 0, 16 and 32 measure the same at 8192^3 (the bank-conflict
-// counter changes 3x between them, the time does not); paddings that break 16-byte alignment
-// are 3x slower.
-constexpr int OZ_LDS_PAD = 16;
-
-// Cres[j] (M x N, int8) = (Ares[j] (Mp x Kp) * Btres[j] (Np x Kp)^T) mod m_j, j = blockIdx.y.
-template <int BM, int BN, int BK, int WM, int WN>
-__global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_i8_nt_mod_kernel(
+template <int BM, int BN, int WM, int WN>
+__global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_i8_dma_kernel(
     int M, int N, int Kp, const signed char *__restrict__ A, size_t a_stride, const signed char *__restrict__ Bt,
     size_t b_stride, signed char *__restrict__ Cres, size_t c_stride, OzConsts k, int group_m) {
     constexpr int NT = (BM / WM) * (BN / WN) * 64;
     constexpr int WGN = BN / WN;
     constexpr int TM = WM / 16, TN = WN / 16;
-    constexpr int LDK = BK + OZ_LDS_PAD;  // bytes per padded LDS row
-    constexpr int CH = BK / 16;    // 16-byte chunks per row
-    constexpr int LA = BM * CH / NT, LB = BN * CH / NT;
-    static_assert((BM * CH) % NT == 0 && (BN * CH) % NT == 0, "tile/threads mismatch");
-    static_assert(BK % 64 == 0, "BK multiple of the MFMA K (64)");
+    constexpr int BK = OZ_DMA_BK;
+    constexpr int RA = BM * 4 / NT, RB = BN * 4 / NT;  // 16-byte loads per thread per tile
+    constexpr int BUF = (BM + BN) * BK;
+    static_assert((BM * 4) % NT == 0 && (BN * 4) % NT == 0, "tile/threads mismatch");
+    static_assert(RA + RB <= 63, "vmcnt range");
 
     extern __shared__ __attribute__((aligned(16))) signed char smem8[];
-    signed char *As = smem8;                   // [2][BM][LDK]
-    signed char *Bs = smem8 + 2 * BM * LDK;    // [2][BN][LDK]
-
     const int j = blockIdx.y;
     A += j * a_stride;
     Bt += j * b_stride;
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = wave / WGN, wn = wave % WGN;
     int tm, tn;
     lowp_tile_of(blockIdx.x, (M + BM - 1) / BM, (N + BN - 1) / BN, group_m, tm, tn);
     const int bm0 = tm * BM, bn0 = tn * BN;
+
+    // Load s of this thread covers LDS chunk position p = s*NT + tid of its operand: row p/4,
+    // stored chunk p%4, which holds logical chunk (p%4) ^ oz_swz(row).
+    const signed char *srcA[RA], *srcB[RB];
+#pragma unroll
+    for (int s = 0; s < RA; ++s) {
+        const int p = s * NT + tid, row = p >> 2, c = (p & 3) ^ oz_swz(row);
+        srcA[s] = A + (size_t)(bm0 + row) * Kp + c * 16;
+    }
+#pragma unroll
+    for (int s = 0; s < RB; ++s) {
+        const int p = s * NT + tid, row = p >> 2, c = (p & 3) ^ oz_swz(row);
+        srcB[s] = Bt + (size_t)(bn0 + row) * Kp + c * 16;
+    }
+    auto issue = [&](int t) {
+        signed char *base = smem8 + (t % 3) * BUF + wave * 64 * 16;  // this wave's 1 KiB slot
+        const size_t k0 = (size_t)t * BK;
+#pragma unroll
+        for (int s = 0; s < RA; ++s) oz_lds_dma16(srcA[s] + k0, base + s * NT * 16);
+#pragma unroll
+        for (int s = 0; s < RB; ++s) oz_lds_dma16(srcB[s] + k0, base + BM * BK + s * NT * 16);
+    };
 
     i32x4_t acc[TM][TN];
 #pragma unroll
@@ -278,83 +309,78 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_i8_nt_mod_ker
 #pragma unroll
         for (int jj = 0; jj < TN; ++jj) acc[i][jj] = i32x4_t{0, 0, 0, 0};
 
-    // Two register staging sets: the global loads of K-tile t are issued two tiles ahead of the
-    // tile's use, so they have two tiles of MFMA work to arrive in (tile t: global -> stage t%2
-    // during tile t-2, stage -> LDS buffer t%2 at the end of tile t-1, MFMAs during tile t).
-    u32x4_t ra[2][LA], rb[2][LB];
-    auto gload = [&](int st, int k0) {
-#pragma unroll
-        for (int s = 0; s < LA; ++s) {
-            const int q = tid + s * NT, row = q / CH, ch = q % CH;
-            ra[st][s] = *reinterpret_cast<const u32x4_t *>(A + (size_t)(bm0 + row) * Kp + k0 + ch * 16);
-        }
-#pragma unroll
-        for (int s = 0; s < LB; ++s) {
-            const int q = tid + s * NT, row = q / CH, ch = q % CH;
-            rb[st][s] = *reinterpret_cast<const u32x4_t *>(Bt + (size_t)(bn0 + row) * Kp + k0 + ch * 16);
-        }
+    const int r16 = lane & 15;
+    const int pc = ((lane >> 4) ^ oz_swz(r16)) * 16;  // stored position of this lane's chunk (row = r16 mod 16)
+    const int nk = Kp / BK;
+    struct Frags {
+        i8x16_t a[TM], b[TN];
     };
-    auto sstore = [&](int st, int buf) {
+    auto fetch = [&](Frags &f, int t) {  // this wave's operand fragments of tile t, LDS -> registers
+        const signed char *As = smem8 + (t % 3) * BUF, *Bs = As + BM * BK;
 #pragma unroll
-        for (int s = 0; s < LA; ++s) {
-            const int q = tid + s * NT, row = q / CH, ch = q % CH;
-            *reinterpret_cast<u32x4_t *>(As + (buf * BM + row) * LDK + ch * 16) = ra[st][s];
-        }
+        for (int i = 0; i < TM; ++i) f.a[i] = *reinterpret_cast<const i8x16_t *>(As + (wm * WM + i * 16 + r16) * BK + pc);
 #pragma unroll
-        for (int s = 0; s < LB; ++s) {
-            const int q = tid + s * NT, row = q / CH, ch = q % CH;
-            *reinterpret_cast<u32x4_t *>(Bs + (buf * BN + row) * LDK + ch * 16) = rb[st][s];
-        }
+        for (int jj = 0; jj < TN; ++jj) f.b[jj] = *reinterpret_cast<const i8x16_t *>(Bs + (wn * WN + jj * 16 + r16) * BK + pc);
     };
-    auto compute = [&](int cur) {
+    auto mma = [&](const Frags &f) {
 #pragma unroll
-        for (int kk = 0; kk < BK; kk += 64) {
-            i8x16_t fa[TM], fb[TN];
-            const int koff = kk + 16 * (lane >> 4);
-#pragma unroll
-            for (int i = 0; i < TM; ++i)
-                fa[i] = *reinterpret_cast<const i8x16_t *>(As + (cur * BM + wm * WM + i * 16 + (lane & 15)) * LDK + koff);
+        for (int i = 0; i < TM; ++i)
 #pragma unroll
             for (int jj = 0; jj < TN; ++jj)
-                fb[jj] = *reinterpret_cast<const i8x16_t *>(Bs + (cur * BN + wn * WN + jj * 16 + (lane & 15)) * LDK + koff);
-#pragma unroll
-            for (int i = 0; i < TM; ++i)
-#pragma unroll
-                for (int jj = 0; jj < TN; ++jj)
-                    acc[i][jj] = __builtin_amdgcn_mfma_i32_16x16x64_i8(__builtin_bit_cast(i32x4_t, fa[i]),
-                                                                       __builtin_bit_cast(i32x4_t, fb[jj]), acc[i][jj], 0, 0, 0);
-        }
+                acc[i][jj] = __builtin_amdgcn_mfma_i32_16x16x64_i8(__builtin_bit_cast(i32x4_t, f.a[i]),
+                                                                   __builtin_bit_cast(i32x4_t, f.b[jj]), acc[i][jj], 0, 0, 0);
     };
-    const int nk = Kp / BK;
-    // tile t (even: stage/buffer 0, odd: 1): move tile t+1 from its stage into the other LDS
-    // buffer (nobody reads that buffer any more: the previous barrier ended tile t-1) and refill
-    // the stage with tile t+3, so the LDS writes overlap this tile's MFMAs; then compute tile t.
-    auto step = [&](int t, int par) {
-        if (t + 1 < nk) sstore(par ^ 1, par ^ 1);
-        if (t + 3 < nk) gload(par ^ 1, (t + 3) * BK);
-        compute(par);
-        __syncthreads();
+    // Before the barrier of tile t: this wave's loads of tile t have landed (those of tile t+1 may
+    // still be in flight) and its LDS reads of tile t-1 are complete. After it, tile t is visible to
+    // every wave and nobody reads tile t-1's buffer any more, so tile t+2 can stream into it. A bare
+    // s_barrier: the workgroup fence of __syncthreads would also wait for tile t+1's loads; the
+    // empty asm statements keep the compiler from moving LDS accesses across it.
+    auto sync_tile = [&](int t) {
+        if (t + 1 < nk) OZ_WAIT_VM_LGKM0(RA + RB);
+        else OZ_WAIT_VM_LGKM0(0);
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        if (t + 2 < nk) issue(t + 2);
     };
-    gload(0, 0);
-    if (nk > 1) gload(1, BK);
-    sstore(0, 0);
-    if (nk > 2) gload(0, 2 * BK);
-    __syncthreads();
+    // Fragments are double-buffered in registers: the MFMAs of tile t-1 run while tile t's LDS
+    // reads are in flight, so the matrix cores never wait on LDS latency.
+    // Kp is a multiple of 128, so the tile count nk is even and tiles pair up (t, t+1). The loop
+    // body has no branch around fragment reads or MFMAs (a branch would make the compiler copy
+    // fragments and accumulators between register assignments): the last pair also reads the
+    // fragments of "tile nk" from a buffer holding stale data, and never uses them.
+    Frags f0, f1;
+    issue(0);
+    issue(1);
+    sync_tile(0);
+    fetch(f0, 0);
     for (int t = 0; t < nk; t += 2) {
-        step(t, 0);
-        if (t + 1 < nk) step(t + 1, 1);
+        sync_tile(t + 1);
+        fetch(f1, t + 1);
+        mma(f0);
+        sync_tile(t + 2);
+        fetch(f0, t + 2);
+        mma(f1);
     }
+    // Epilogue: symmetric residues mod m_j, written in 16 x 16 blocks (block (bi, bj) at byte
+    // (bi * Nb + bj) * 256, lane l's word at + 4 l holding rows 4 (l/16) + 0..3 of column l % 16,
+    // its accumulator order), so each block is one 256-byte store of the wave. Blocks outside
+    // ceil(M/16) x ceil(N/16) are not stored.
     const int mj = k.m[j];
-    signed char *Cj = Cres + j * c_stride;
+    const double inv_mj = k.inv_m[j];
+    const int Mb = (M + 15) >> 4, Nb = (N + 15) >> 4;
+    // v - m round(v / m): v / m is never a half-integer (m odd), and |v| < 2^31 keeps v / m far
+    // more accurate in FP64 than its distance to one, so this is the residue in [-(m-1)/2, (m-1)/2].
+    auto smod = [&](int v) -> unsigned { return (unsigned)(v - (int)rint((double)v * inv_mj) * mj) & 0xffu; };
+    unsigned *Cj = reinterpret_cast<unsigned *>(Cres + j * c_stride);
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll
         for (int jj = 0; jj < TN; ++jj) {
-            const int col = bn0 + wn * WN + jj * 16 + (lane & 15);
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = bm0 + wm * WM + i * 16 + 4 * (lane >> 4) + r;
-                if (row < M && col < N) Cj[(size_t)row * N + col] = mod_i32(acc[i][jj][r], mj);
+            const int bi = (bm0 + wm * WM + i * 16) >> 4, bj = (bn0 + wn * WN + jj * 16) >> 4;
+            if (bi < Mb && bj < Nb) {
+                const i32x4_t v = acc[i][jj];
+                Cj[((size_t)bi * Nb + bj) * 64 + lane] = smod(v[0]) | smod(v[1]) << 8 | smod(v[2]) << 16 | smod(v[3]) << 24;
             }
         }
 }
@@ -368,21 +394,26 @@ __device__ __forceinline__ dd_t dd_two_sum(double a, double b) {
     return dd_t{s, (a - (s - v)) + (b - v)};
 }
 
-// Thread t reconstructs outputs 4t .. 4t+3 (flat, row-major), reading the four residues of each
-// modulus as one 32-bit word (c_stride and the base are multiples of 16).
+// Thread t reads word t of every modulus' residue array: the residues of block t / 64 (16 x 16
+// blocks, Nb = ceil(N/16) per block row) at lane l = t % 64, i.e. rows 4 (l/16) + 0..3 of column
+// l % 16, and reconstructs those four outputs. A wavefront reads 256 contiguous bytes per modulus.
 __global__ __launch_bounds__(256) void crt_kernel(const signed char *__restrict__ Cres, size_t c_stride, int M, int N,
                                                   const int *__restrict__ srow, const int *__restrict__ scol, OzConsts k,
                                                   double *__restrict__ C) {
-    const size_t base = (blockIdx.x * (size_t)256 + threadIdx.x) * 4;
-    const size_t total = (size_t)M * N;
-    if (base >= total) return;
+    const size_t w = blockIdx.x * (size_t)256 + threadIdx.x;
+    const int Nb = (N + 15) >> 4;
+    const size_t blk = w >> 6;
+    const int l = (int)(w & 63);
+    const int col = (int)(blk % Nb) * 16 + (l & 15);
+    const int row0 = (int)(blk / Nb) * 16 + 4 * (l >> 4);
+    if (row0 >= M || col >= N) return;
     unsigned words[OZ_MAX_MODULI];
-    for (int j = 0; j < k.n; ++j) words[j] = *reinterpret_cast<const unsigned *>(Cres + j * c_stride + base);
+    for (int j = 0; j < k.n; ++j) words[j] = reinterpret_cast<const unsigned *>(Cres + j * c_stride)[w];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-        const size_t idx = base + u;
-        if (idx >= total) break;
-        const int row = (int)(idx / N), col = (int)(idx % N);
+        const int row = row0 + u;
+        if (row >= M) break;
+        const size_t idx = (size_t)row * N + col;
         const int sr = srow[row], sc = scol[col];
         if (sr == OZ_BAD_SCALE || sc == OZ_BAD_SCALE) {
             C[idx] = __builtin_nan("");
@@ -425,13 +456,13 @@ __global__ __launch_bounds__(256) void crt_kernel(const signed char *__restrict_
 }
 
 // ---------------- host driver ----------------
-template <int BM, int BN, int BK, int WM, int WN>
+template <int BM, int BN, int WM, int WN>
 static nrk_status launch_i8(int M, int N, int Kp, const signed char *A, size_t sa, const signed char *Bt, size_t sb,
                             signed char *Cres, size_t sc, const OzConsts &k, hipStream_t stream) {
     constexpr int threads = (BM / WM) * (BN / WN) * 64;
-    constexpr size_t lds = (size_t)2 * (BM + BN) * (BK + OZ_LDS_PAD);
+    constexpr size_t lds = (size_t)3 * (BM + BN) * OZ_DMA_BK;
     static_assert(lds <= 160 * 1024, "tile does not fit in LDS");
-    auto kern = gemm_i8_nt_mod_kernel<BM, BN, BK, WM, WN>;
+    auto kern = gemm_i8_dma_kernel<BM, BN, WM, WN>;
     static bool attr_set = false;
     if (!attr_set) {
         NRK_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -461,7 +492,7 @@ nrk_status gemm_f64_ozaki(size_t m, size_t k, size_t n, const double *a, const d
     if (P > 53) P = 53;
     if (P < 8) return fail(NRK_ERR_INVALID_ARGUMENT, "too few moduli (%d) for k = %zu", moduli, k);
     const size_t Mp = align_up(m, 256), Np = align_up(n, 256), Kp = align_up(k, 128);
-    const size_t sa = Mp * Kp, sb = Np * Kp, sc = align_up(m * n, 16);
+    const size_t sa = Mp * Kp, sb = Np * Kp, sc = ceil_div(m, 16) * ceil_div(n, 16) * 256;  // residues in 16 x 16 blocks
     const size_t nchunk = ceil_div(k, (size_t)COL_ROWS);
     const size_t res_bytes = align_up((size_t)moduli * (sa + sb + sc), 256);
     const size_t bytes = res_bytes + align_up(nchunk * n * sizeof(double), 256) + (m + n) * sizeof(int);
@@ -495,13 +526,13 @@ nrk_status gemm_f64_ozaki(size_t m, size_t k, size_t n, const double *a, const d
         else tile = 0;
     }
     switch (tile) {
-        case 0: NRK_TRY((launch_i8<64, 64, 128, 32, 32>((int)m, (int)n, (int)Kp, Ares, sa, Bres, sb, Cres, sc, kc, stream))); break;
-        case 1: NRK_TRY((launch_i8<128, 128, 128, 64, 64>((int)m, (int)n, (int)Kp, Ares, sa, Bres, sb, Cres, sc, kc, stream))); break;
-        case 2: NRK_TRY((launch_i8<256, 256, 128, 128, 64>((int)m, (int)n, (int)Kp, Ares, sa, Bres, sb, Cres, sc, kc, stream))); break;
-        case 3: NRK_TRY((launch_i8<256, 256, 64, 128, 64>((int)m, (int)n, (int)Kp, Ares, sa, Bres, sb, Cres, sc, kc, stream))); break;
-        default: NRK_TRY((launch_i8<256, 256, 64, 64, 128>((int)m, (int)n, (int)Kp, Ares, sa, Bres, sb, Cres, sc, kc, stream)));
+        case 0: NRK_TRY((launch_i8<64, 64, 32, 32>((int)m, (int)n, (int)Kp, Ares, sa, Bres, sb, Cres, sc, kc, stream))); break;
+        case 1: NRK_TRY((launch_i8<128, 128, 64, 64>((int)m, (int)n, (int)Kp, Ares, sa, Bres, sb, Cres, sc, kc, stream))); break;
+        case 2: NRK_TRY((launch_i8<128, 256, 64, 64>((int)m, (int)n, (int)Kp, Ares, sa, Bres, sb, Cres, sc, kc, stream))); break;
+        case 3: NRK_TRY((launch_i8<256, 256, 128, 64>((int)m, (int)n, (int)Kp, Ares, sa, Bres, sb, Cres, sc, kc, stream))); break;
+        default: NRK_TRY((launch_i8<256, 256, 128, 128>((int)m, (int)n, (int)Kp, Ares, sa, Bres, sb, Cres, sc, kc, stream)));
     }
-    hipLaunchKernelGGL(crt_kernel, dim3((unsigned)ceil_div(ceil_div(m * n, 4), 256)), dim3(256), 0, stream, Cres, sc, (int)m, (int)n,
+    hipLaunchKernelGGL(crt_kernel, dim3((unsigned)ceil_div(sc / 4, 256)), dim3(256), 0, stream, Cres, sc, (int)m, (int)n,
                        srow, scol, kc, c);
     NRK_HIP(hipGetLastError());
     return NRK_OK;

@@ -129,3 +129,27 @@ def test_ozaki_zero_k_and_zero_rows(gpu):
     a = np.zeros((5, 7))
     b = rng.normal(3, 7 * 6).reshape(7, 6)
     assert np.all(K.gemm_f64_ozaki(a, b) == 0)
+
+
+@pytest.mark.parametrize("shape", [(300, 200, 520), (513, 257, 129), (1024, 640, 768)])
+def test_ozaki_every_int8_tile_config_bit_identical(gpu, shape):
+    """All INT8 GEMM tile configurations compute the same exact integer residues, so the whole
+    Ozaki product (16 moduli) is bit-identical across them, and equal to the default path's."""
+    import ctypes as C
+    from numrs_amd import device
+    from numrs_amd import selector as S
+    from numrs_amd._lib import call
+    m, k, n = shape
+    a = rng.scaled_pow2(70, m * k).reshape(m, k)
+    b = rng.scaled_pow2(71, k * n).reshape(k, n)
+    want = K.gemm_f64_ozaki(a, b, 16)
+    st = device.Stream()
+    dA, dB = device.DeviceArray.upload(a), device.DeviceArray.upload(b)
+    dC = device.DeviceArray((m, n), np.float64)
+    for cfg in range(S.config_count(S.F64_OZAKI)):
+        print(f"ozaki tile config {cfg}, shape {shape}", flush=True)
+        call("nrk_dev_run_config", S.F64_OZAKI, cfg, m, n, k, C.c_void_p(dA.ptr), C.c_void_p(dB.ptr),
+             C.c_void_p(dC.ptr), C.c_void_p(st.handle))
+        st.synchronize()
+        got = dC.download()
+        np.testing.assert_array_equal(got.view(np.uint64), want.view(np.uint64), err_msg=f"config {cfg}")
