@@ -308,10 +308,11 @@ nrk_status nrk_dev_gemm_bf16(const nrk_gemm_desc *desc, nrk_stream stream);
  *   NRK_DT_ROWSUM_F32 row sums of a (d0 rows, d1 cols) matrix: configs 0..3 = 1, 2, 4, 8
  *   workgroups per CU (results identical across configs);
  *   NRK_DT_F32_BF16X3 FP32 GEMM as 3 BF16 products (d0=m, d1=n, d2=k): configs 0..4 = tiles
- *   64x64, 128x128, 256x128, 128x256, 256x256 (results identical across configs);
+ *   (workgroup / wave) 64x64 / 32x32, 128x128 / 64x64, 128x64 / 64x32, 256x128 / 64x64,
+ *   128x256 / 64x64 (results identical across configs);
  *   NRK_DT_F64_OZAKI FP64 GEMM by the Ozaki scheme II, 16 moduli: configs 0..4 = INT8 tiles
- *   (BM x BN x BK / wave tile) 64x64x128 / 32x32, 128x128x128 / 64x64, 256x256x128 / 128x64,
- *   256x256x64 / 128x64, 256x256x64 / 64x128 (results identical across configs). */
+ *   (workgroup / wave) 64x64 / 32x32, 128x128 / 64x64, 128x256 / 64x64, 256x256 / 128x64,
+ *   256x256 / 128x128 (results identical across configs). */
 enum { NRK_DT_F64 = 0, NRK_DT_F32 = 1, NRK_DT_BF16 = 2, NRK_DT_ROWSUM_F32 = 3, NRK_DT_F32_BF16X3 = 4, NRK_DT_F64_OZAKI = 5 };
 nrk_status nrk_config_count(int32_t family, int32_t *count);
 /* Run one configuration (device pointers; c receives the result). */

@@ -10,9 +10,9 @@
 //
 // Data path: a split pass writes the terms as bf16 matrices zero-padded to the tile grid, A as
 // (Mp, Kp) and B TRANSPOSED as (Np, Kp) (K contiguous for both, so every MFMA operand is one
-// 16-byte LDS read), then one GEMM kernel streams all terms: global -> registers -> LDS (two
-// buffers), XCD-aware grouped tile order. Padding removes every bounds check from the main loop;
-// only the C store is predicated.
+// 16-byte LDS read), then one GEMM kernel streams all terms: global -> LDS by the load unit
+// (three buffers), XCD-aware grouped tile order. Padding removes every bounds check from the main
+// loop; only the C store is predicated.
 #include "gemm_lowp.hpp"
 #include "ops.hpp"
 
@@ -69,29 +69,52 @@ __device__ __forceinline__ constexpr int pair_b(int p) {
 }
 
 // C(M,N) f32 = sum over pairs p of A[a_p] * Bt[b_p]^T. A terms: (Mp, Kp) each, Bt terms: (Np, Kp).
-template <int BM, int BN, int BK, int WM, int WN, int NTA, int NTB, int NPAIR>
+//
+// K tiles of 32 bf16 (64 bytes per row) move global -> LDS by the load unit itself
+// (global_load_lds), all terms of both operands in one buffer (NTA*BM A rows, then NTB*BN Bt
+// rows, swizzled by lds_swz), NBUF buffers deep: tiles t+1 .. t+NBUF-2 are in flight while tile
+// t is multiplied, one barrier per tile. MFMA fragments are double-buffered in registers, so the
+// products of tile t-1 run while tile t's fragments are read. Per K step the accumulation order
+// is: pairs smallest first, then the 16 x 16 blocks.
+constexpr int BF_BK = 32;  // bf16 values per K tile (one MFMA K step)
+
+template <int BM, int BN, int WM, int WN, int NTA, int NTB, int NPAIR, int NBUF>
 __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_bf16_nt_kernel(
     int M, int N, int Kp, const unsigned short *__restrict__ A, size_t a_term_stride,
     const unsigned short *__restrict__ Bt, size_t b_term_stride, float *__restrict__ C, int ldc, int group_m) {
     constexpr int NT = (BM / WM) * (BN / WN) * 64;
     constexpr int WGN = BN / WN;
     constexpr int TM = WM / 16, TN = WN / 16;
-    constexpr int LDK = BK + 8;           // padded row (bf16 elements) against bank conflicts
-    constexpr int CH = BK / 8;            // 16-byte chunks per row
-    constexpr int LA = BM * CH / NT;      // chunks per thread per A term
-    constexpr int LB = BN * CH / NT;
-    static_assert((BM * CH) % NT == 0 && (BN * CH) % NT == 0, "tile/threads mismatch");
-    static_assert(BK % 32 == 0, "BK multiple of the MFMA K (32)");
+    constexpr int ROWS = NTA * BM + NTB * BN;  // 64-byte LDS rows per buffer
+    constexpr int BUF = ROWS * 64;             // bytes per buffer
+    constexpr int R = ROWS * 4 / NT;           // 16-byte loads per thread per tile
+    static_assert((ROWS * 4) % NT == 0, "tile/threads mismatch");
 
-    extern __shared__ __attribute__((aligned(16))) unsigned short smem[];
-    unsigned short *As = smem;                                  // [2][NTA][BM][LDK]
-    unsigned short *Bs = smem + 2 * NTA * BM * LDK;             // [2][NTB][BN][LDK]
-
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_bf[];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = wave / WGN, wn = wave % WGN;
     int tm, tn;
     lowp_tile_of(blockIdx.x, (M + BM - 1) / BM, (N + BN - 1) / BN, group_m, tm, tn);
     const int bm0 = tm * BM, bn0 = tn * BN;
+
+    // Load s of this thread covers chunk position p = s*NT + tid of the buffer: LDS row p/4 (A
+    // term rows first, then Bt term rows), stored chunk p%4 holding logical chunk (p%4) ^ lds_swz.
+    const unsigned short *src[R];
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+        const int p = s * NT + tid, row = p >> 2, c = (p & 3) ^ lds_swz(row);
+        if (row < NTA * BM)
+            src[s] = A + (row / BM) * a_term_stride + (size_t)(bm0 + row % BM) * Kp + c * 8;
+        else
+            src[s] = Bt + ((row - NTA * BM) / BN) * b_term_stride + (size_t)(bn0 + (row - NTA * BM) % BN) * Kp + c * 8;
+    }
+    auto issue = [&](int t) {
+        unsigned char *base = smem_bf + (t % NBUF) * BUF + wave * 64 * 16;  // this wave's 1 KiB slot
+        const size_t k0 = (size_t)t * BF_BK;
+#pragma unroll
+        for (int s = 0; s < R; ++s) lds_dma16(src[s] + k0, base + s * NT * 16);
+    };
 
     f32x4_t acc[TM][TN];
 #pragma unroll
@@ -99,80 +122,61 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_bf16_nt_kerne
 #pragma unroll
         for (int j = 0; j < TN; ++j) acc[i][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
 
-    u32x4_t ra[NTA][LA], rb[NTB][LB];
-    auto gload = [&](int k0) {
-#pragma unroll
-        for (int t = 0; t < NTA; ++t)
-#pragma unroll
-            for (int s = 0; s < LA; ++s) {
-                const int q = tid + s * NT, row = q / CH, ch = q % CH;
-                ra[t][s] = *reinterpret_cast<const u32x4_t *>(A + t * a_term_stride + (size_t)(bm0 + row) * Kp + k0 + ch * 8);
-            }
-#pragma unroll
-        for (int t = 0; t < NTB; ++t)
-#pragma unroll
-            for (int s = 0; s < LB; ++s) {
-                const int q = tid + s * NT, row = q / CH, ch = q % CH;
-                rb[t][s] = *reinterpret_cast<const u32x4_t *>(Bt + t * b_term_stride + (size_t)(bn0 + row) * Kp + k0 + ch * 8);
-            }
+    const int r16 = lane & 15;
+    const int pc = ((lane >> 4) ^ lds_swz(r16)) * 16;  // stored position of this lane's chunk (row = r16 mod 16)
+    const int nk = Kp / BF_BK;
+    struct Frags {
+        bf16x8_t a[NTA][TM], b[NTB][TN];
     };
-    auto sstore = [&](int buf) {
+    auto fetch = [&](Frags &f, int t) {
+        const unsigned char *As = smem_bf + (t % NBUF) * BUF, *Bs = As + NTA * BM * 64;
 #pragma unroll
-        for (int t = 0; t < NTA; ++t)
+        for (int u = 0; u < NTA; ++u)
 #pragma unroll
-            for (int s = 0; s < LA; ++s) {
-                const int q = tid + s * NT, row = q / CH, ch = q % CH;
-                *reinterpret_cast<u32x4_t *>(As + ((buf * NTA + t) * BM + row) * LDK + ch * 8) = ra[t][s];
-            }
+            for (int i = 0; i < TM; ++i)
+                f.a[u][i] = *reinterpret_cast<const bf16x8_t *>(As + (u * BM + wm * WM + i * 16 + r16) * 64 + pc);
 #pragma unroll
-        for (int t = 0; t < NTB; ++t)
+        for (int u = 0; u < NTB; ++u)
 #pragma unroll
-            for (int s = 0; s < LB; ++s) {
-                const int q = tid + s * NT, row = q / CH, ch = q % CH;
-                *reinterpret_cast<u32x4_t *>(Bs + ((buf * NTB + t) * BN + row) * LDK + ch * 8) = rb[t][s];
-            }
+            for (int j = 0; j < TN; ++j)
+                f.b[u][j] = *reinterpret_cast<const bf16x8_t *>(Bs + (u * BN + wn * WN + j * 16 + r16) * 64 + pc);
     };
-
-    const int nk = Kp / BK;
-    gload(0);
-    sstore(0);
-    __syncthreads();
-    for (int kt = 0; kt < nk; ++kt) {
-        const int cur = kt & 1;
-        if (kt + 1 < nk) gload((kt + 1) * BK);
+    auto mma = [&](const Frags &f) {
 #pragma unroll
-        for (int kk = 0; kk < BK; kk += 32) {
-            bf16x8_t fa[NTA][TM], fb[NTB][TN];
-            const int koff = kk + 8 * (lane >> 4);
+        for (int p = 0; p < NPAIR; ++p)
 #pragma unroll
-            for (int t = 0; t < NTA; ++t)
-#pragma unroll
-                for (int i = 0; i < TM; ++i)
-                    fa[t][i] = *reinterpret_cast<const bf16x8_t *>(
-                        As + ((cur * NTA + t) * BM + wm * WM + i * 16 + (lane & 15)) * LDK + koff);
-#pragma unroll
-            for (int t = 0; t < NTB; ++t)
+            for (int i = 0; i < TM; ++i)
 #pragma unroll
                 for (int j = 0; j < TN; ++j)
-                    fb[t][j] = *reinterpret_cast<const bf16x8_t *>(
-                        Bs + ((cur * NTB + t) * BN + wn * WN + j * 16 + (lane & 15)) * LDK + koff);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(f.a[pair_a<NPAIR>(p)][i], f.b[pair_b<NPAIR>(p)][j],
+                                                                        acc[i][j], 0, 0, 0);
+    };
+    auto sync_tile = [&](int t) {
+        lowp_tile_barrier<NBUF, R>(t, nk);
+        if (t + NBUF - 1 < nk) issue(t + NBUF - 1);  // into the buffer tile t-1 used
+    };
+    // Kp is a multiple of 64, so nk is even and tiles pair up (t, t+1); the last pair also reads
+    // the fragments of "tile nk" from a stale buffer and never uses them (no branch around the
+    // MFMAs, which would give the accumulators a second register assignment).
+    Frags f0, f1;
 #pragma unroll
-            for (int p = 0; p < NPAIR; ++p)
-#pragma unroll
-                for (int i = 0; i < TM; ++i)
-#pragma unroll
-                    for (int j = 0; j < TN; ++j)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[pair_a<NPAIR>(p)][i], fb[pair_b<NPAIR>(p)][j],
-                                                                            acc[i][j], 0, 0, 0);
-        }
-        if (kt + 1 < nk) sstore(cur ^ 1);
-        __syncthreads();
+    for (int t = 0; t < NBUF - 1; ++t)
+        if (t < nk) issue(t);
+    sync_tile(0);
+    fetch(f0, 0);
+    for (int t = 0; t < nk; t += 2) {
+        sync_tile(t + 1);
+        fetch(f1, t + 1);
+        mma(f0);
+        sync_tile(t + 2);
+        fetch(f0, t + 2);
+        mma(f1);
     }
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll
         for (int j = 0; j < TN; ++j) {
-            const int col = bn0 + wn * WN + j * 16 + (lane & 15);
+            const int col = bn0 + wn * WN + j * 16 + r16;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int row = bm0 + wm * WM + i * 16 + 4 * (lane >> 4) + r;
@@ -181,13 +185,14 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_bf16_nt_kerne
         }
 }
 
-template <int BM, int BN, int BK, int WM, int WN, int NTA, int NTB, int NPAIR>
+template <int BM, int BN, int WM, int WN, int NTA, int NTB, int NPAIR>
 static nrk_status launch_bf16_nt(int M, int N, int Kp, const unsigned short *A, size_t a_stride, const unsigned short *Bt,
                                  size_t b_stride, float *C, int ldc, hipStream_t stream) {
+    constexpr int NBUF = 3;
     constexpr int threads = (BM / WM) * (BN / WN) * 64;
-    constexpr size_t lds = (size_t)2 * (NTA * BM + NTB * BN) * (BK + 8) * sizeof(unsigned short);
+    constexpr size_t lds = (size_t)NBUF * (NTA * BM + NTB * BN) * 64;
     static_assert(lds <= 160 * 1024, "tile does not fit in LDS");
-    auto kern = gemm_bf16_nt_kernel<BM, BN, BK, WM, WN, NTA, NTB, NPAIR>;
+    auto kern = gemm_bf16_nt_kernel<BM, BN, WM, WN, NTA, NTB, NPAIR, NBUF>;
     static bool attr_set = false;
     if (!attr_set) {
         NRK_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -219,21 +224,29 @@ static nrk_status split(const float *X, int R, int Cc, bool transpose, unsigned 
     return NRK_OK;
 }
 
-// Tile configurations of the split-product GEMM, BM x BN x BK (wave tile); every term of both
-// operands sits in LDS (double-buffered), which bounds the tile: 0: 64x64xBK (32x32),
-// 1: 128x128xBK (64x64), 2: 128x128x32 (64x64), 3: 128x64xBK (64x32), 4: 64x128xBK (32x64),
-// with BK = 64 (32 for the six-product form).
+// Tile configurations of the split-product GEMM, BM x BN (wave tile): 0: 64x64 (32x32),
+// 1: 128x128 (64x64), 2: 128x64 (64x32), 3: 256x128 (64x64, 8 waves), 4: 128x256 (64x64, 8 waves).
+// Every term of both operands sits in LDS, three K tiles deep; for the six-product form
+// configurations 3 and 4 exceed the LDS and run as configuration 1.
 constexpr int SPLIT_TILE_CONFIGS = 5;
-template <int NTA, int NTB, int NPAIR, int BK>
+template <int NTA, int NTB, int NPAIR>
 static nrk_status launch_split_tile(int tile, size_t m, size_t n, size_t Kp, const unsigned short *As, size_t sa,
                                     const unsigned short *Bts, size_t sb, float *c, hipStream_t stream) {
+    constexpr bool wide_fits = (size_t)3 * (NTA * 256 + NTB * 128) * 64 <= 160 * 1024;
     switch (tile) {
-        case 0: return launch_bf16_nt<64, 64, BK, 32, 32, NTA, NTB, NPAIR>((int)m, (int)n, (int)Kp, As, sa, Bts, sb, c, (int)n, stream);
-        case 1: return launch_bf16_nt<128, 128, BK, 64, 64, NTA, NTB, NPAIR>((int)m, (int)n, (int)Kp, As, sa, Bts, sb, c, (int)n, stream);
-        case 2: return launch_bf16_nt<128, 128, 32, 64, 64, NTA, NTB, NPAIR>((int)m, (int)n, (int)Kp, As, sa, Bts, sb, c, (int)n, stream);
-        case 3: return launch_bf16_nt<128, 64, BK, 64, 32, NTA, NTB, NPAIR>((int)m, (int)n, (int)Kp, As, sa, Bts, sb, c, (int)n, stream);
-        default: return launch_bf16_nt<64, 128, BK, 32, 64, NTA, NTB, NPAIR>((int)m, (int)n, (int)Kp, As, sa, Bts, sb, c, (int)n, stream);
+        case 0: return launch_bf16_nt<64, 64, 32, 32, NTA, NTB, NPAIR>((int)m, (int)n, (int)Kp, As, sa, Bts, sb, c, (int)n, stream);
+        case 2: return launch_bf16_nt<128, 64, 64, 32, NTA, NTB, NPAIR>((int)m, (int)n, (int)Kp, As, sa, Bts, sb, c, (int)n, stream);
+        case 3:
+            if constexpr (wide_fits)
+                return launch_bf16_nt<256, 128, 64, 64, NTA, NTB, NPAIR>((int)m, (int)n, (int)Kp, As, sa, Bts, sb, c, (int)n, stream);
+            break;
+        case 4:
+            if constexpr (wide_fits)
+                return launch_bf16_nt<128, 256, 64, 64, NTA, NTB, NPAIR>((int)m, (int)n, (int)Kp, As, sa, Bts, sb, c, (int)n, stream);
+            break;
+        default: break;
     }
+    return launch_bf16_nt<128, 128, 64, 64, NTA, NTB, NPAIR>((int)m, (int)n, (int)Kp, As, sa, Bts, sb, c, (int)n, stream);
 }
 
 nrk_status gemm_f32_bf16x(size_t m, size_t k, size_t n, const float *a, const float *b, float *c, int products, int tile,
@@ -265,13 +278,14 @@ nrk_status gemm_f32_bf16x(size_t m, size_t k, size_t n, const float *a, const fl
         NRK_TRY(split<1>(b, (int)k, (int)n, true, Bts, (int)Np, (int)Kp, stream));
     }
     const size_t sa = Mp * Kp, sb = Np * Kp;
-    if (tile < 0) {
-        const bool big = ceil_div(m, 128) * ceil_div(n, 128) >= (size_t)cu_count();
-        tile = big ? 1 : 0;
+    if (tile < 0) {  // the largest tiles that still give every CU a workgroup
+        if (ceil_div(m, 128) * ceil_div(n, 256) >= (size_t)cu_count()) tile = 4;
+        else if (ceil_div(m, 128) * ceil_div(n, 128) >= (size_t)cu_count()) tile = 1;
+        else tile = 0;
     }
-    if (products == 6) return launch_split_tile<3, 3, 6, 32>(tile, m, n, Kp, As, sa, Bts, sb, c, stream);
-    if (products == 3) return launch_split_tile<2, 2, 3, 64>(tile, m, n, Kp, As, sa, Bts, sb, c, stream);
-    return launch_split_tile<1, 1, 1, 64>(tile, m, n, Kp, As, sa, Bts, sb, c, stream);
+    if (products == 6) return launch_split_tile<3, 3, 6>(tile, m, n, Kp, As, sa, Bts, sb, c, stream);
+    if (products == 3) return launch_split_tile<2, 2, 3>(tile, m, n, Kp, As, sa, Bts, sb, c, stream);
+    return launch_split_tile<1, 1, 1>(tile, m, n, Kp, As, sa, Bts, sb, c, stream);
 }
 
 }  // namespace nrk

@@ -40,4 +40,43 @@ __device__ __forceinline__ void lowp_tile_of(int g, int nbm, int nbn, int group_
     tn = r / gm;
 }
 
+// ---- LDS tiles filled by the load unit (global_load_lds), shared by the INT8 and BF16 kernels ----
+// A K tile is 64 bytes per row (four 16-byte chunks: 64 int8 or 32 bf16 values), rows of all
+// operand terms stacked. Chunk c of row r is stored at chunk position c ^ lds_swz(r). A fragment
+// read puts lane l on row l % 16, chunk l / 16. Measured on gfx950 (SQ_LDS_BANK_CONFLICT, INT8
+// 8192^3 kernel with the loads switched off): xor with (r >> 1) & 3 makes these reads
+// conflict-free and halves the LDS busy cycles; no swizzle, and xors with r & 3, (r >> 2) & 3 or
+// ((r >> 1) & 3) ^ ((r >> 3) & 1), all leave them 2-way conflicted.
+__device__ __forceinline__ int lds_swz(int r) { return (r >> 1) & 3; }
+
+// One 16-byte global -> LDS load per lane: lane i's bytes land at l + 16 i (l must be the same
+// for the whole wavefront; it goes to M0).
+__device__ __forceinline__ void lds_dma16(const void *g, void *l) {
+    __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1))) *)g,
+                                     (void __attribute__((address_space(3))) *)l, 16, 0, 0);
+}
+
+// s_waitcnt vmcnt(n) lgkmcnt(0), expcnt unconstrained (gfx9 encoding: vmcnt in bits 3:0 and
+// 15:14, expcnt 6:4, lgkmcnt 11:8).
+#define LOWP_WAIT_VM_LGKM0(n) __builtin_amdgcn_s_waitcnt(((n) & 15) | ((((n) >> 4) & 3) << 14) | (0x7 << 4))
+
+// Start of K tile t's pass in an NBUF-deep pipeline of nk tiles, R loads per thread per tile.
+// Before the barrier: this wave's loads of tile t have landed (those of the up to NBUF-2 later
+// tiles already issued may still be in flight) and its LDS reads of tile t-1 are complete. After
+// it, tile t is visible to every wave and nobody reads tile t-1's buffer any more, so the caller
+// may stream tile t+NBUF-1 into it. A bare s_barrier: the workgroup fence of __syncthreads would
+// also wait for the later tiles' loads; the empty asm statements keep the compiler from moving
+// LDS accesses across it.
+template <int NBUF, int R>
+__device__ __forceinline__ void lowp_tile_barrier(int t, int nk) {
+    static_assert(NBUF >= 3 && NBUF <= 4 && (NBUF - 2) * R <= 63, "vmcnt range");
+    const int ahead = min(nk - 1 - t, NBUF - 2);  // tiles after t already issued
+    if (ahead >= 2) LOWP_WAIT_VM_LGKM0(2 * R);
+    else if (ahead == 1) LOWP_WAIT_VM_LGKM0(R);
+    else LOWP_WAIT_VM_LGKM0(0);
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
 }  // namespace nrk

@@ -237,27 +237,13 @@ typedef signed char i8x16_t __attribute__((ext_vector_type(16)));
 //
 // Operands move global -> LDS by the load unit itself (global_load_lds, 16 bytes per lane): no
 // staging registers and no LDS write instructions, so a wave's registers go to accumulators and
-// fragments. Three LDS buffers of one 64-deep K tile each: tile t+2 streams in while tile t is
-// multiplied, one barrier per tile.
+// fragments. NBUF LDS buffers of one 64-deep K tile each: tiles t+1 .. t+NBUF-2 are in flight
+// while tile t is multiplied, one barrier per tile.
 //
-// LDS layout of a buffer: A rows then Bt rows, 64 bytes (four 16-byte chunks) per row, chunk c of
-// row r stored at chunk position c ^ oz_swz(r). A fragment read puts lane l on row l % 16, chunk
-// l / 16. Measured on gfx950 (SQ_LDS_BANK_CONFLICT, 8192^3 kernel with the loads switched off):
-// xor with (r >> 1) & 3 makes these reads conflict-free and halves the LDS busy cycles; no
-// swizzle, and xors with r & 3, (r >> 2) & 3 or ((r >> 1) & 3) ^ ((r >> 3) & 1), all leave them
-// 2-way conflicted.
+// LDS layout of a buffer: A rows then Bt rows, 64 bytes per row, swizzled by lds_swz.
 constexpr int OZ_DMA_BK = 64;
-__device__ __forceinline__ int oz_swz(int r) { return (r >> 1) & 3; }
 
-__device__ __forceinline__ void oz_lds_dma16(const signed char *g, signed char *l) {
-    __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1))) *)g,
-                                     (void __attribute__((address_space(3))) *)l, 16, 0, 0);
-}
-// s_waitcnt vmcnt(n) lgkmcnt(0), expcnt unconstrained (gfx9 encoding: vmcnt in bits 3:0 and
-// 15:14, expcnt 6:4, lgkmcnt 11:8).
-#define OZ_WAIT_VM_LGKM0(n) __builtin_amdgcn_s_waitcnt(((n) & 15) | ((((n) >> 4) & 3) << 14) | (0x7 << 4))
-
-template <int BM, int BN, int WM, int WN>
+template <int BM, int BN, int WM, int WN, int NBUF>
 __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_i8_dma_kernel(
     int M, int N, int Kp, const signed char *__restrict__ A, size_t a_stride, const signed char *__restrict__ Bt,
     size_t b_stride, signed char *__restrict__ Cres, size_t c_stride, OzConsts k, int group_m) {
@@ -268,7 +254,6 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_i8_dma_kernel
     constexpr int RA = BM * 4 / NT, RB = BN * 4 / NT;  // 16-byte loads per thread per tile
     constexpr int BUF = (BM + BN) * BK;
     static_assert((BM * 4) % NT == 0 && (BN * 4) % NT == 0, "tile/threads mismatch");
-    static_assert(RA + RB <= 63, "vmcnt range");
 
     extern __shared__ __attribute__((aligned(16))) signed char smem8[];
     const int j = blockIdx.y;
@@ -282,25 +267,25 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_i8_dma_kernel
     const int bm0 = tm * BM, bn0 = tn * BN;
 
     // Load s of this thread covers LDS chunk position p = s*NT + tid of its operand: row p/4,
-    // stored chunk p%4, which holds logical chunk (p%4) ^ oz_swz(row).
+    // stored chunk p%4, which holds logical chunk (p%4) ^ lds_swz(row).
     const signed char *srcA[RA], *srcB[RB];
 #pragma unroll
     for (int s = 0; s < RA; ++s) {
-        const int p = s * NT + tid, row = p >> 2, c = (p & 3) ^ oz_swz(row);
+        const int p = s * NT + tid, row = p >> 2, c = (p & 3) ^ lds_swz(row);
         srcA[s] = A + (size_t)(bm0 + row) * Kp + c * 16;
     }
 #pragma unroll
     for (int s = 0; s < RB; ++s) {
-        const int p = s * NT + tid, row = p >> 2, c = (p & 3) ^ oz_swz(row);
+        const int p = s * NT + tid, row = p >> 2, c = (p & 3) ^ lds_swz(row);
         srcB[s] = Bt + (size_t)(bn0 + row) * Kp + c * 16;
     }
     auto issue = [&](int t) {
-        signed char *base = smem8 + (t % 3) * BUF + wave * 64 * 16;  // this wave's 1 KiB slot
+        signed char *base = smem8 + (t % NBUF) * BUF + wave * 64 * 16;  // this wave's 1 KiB slot
         const size_t k0 = (size_t)t * BK;
 #pragma unroll
-        for (int s = 0; s < RA; ++s) oz_lds_dma16(srcA[s] + k0, base + s * NT * 16);
+        for (int s = 0; s < RA; ++s) lds_dma16(srcA[s] + k0, base + s * NT * 16);
 #pragma unroll
-        for (int s = 0; s < RB; ++s) oz_lds_dma16(srcB[s] + k0, base + BM * BK + s * NT * 16);
+        for (int s = 0; s < RB; ++s) lds_dma16(srcB[s] + k0, base + BM * BK + s * NT * 16);
     };
 
     i32x4_t acc[TM][TN];
@@ -310,13 +295,13 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_i8_dma_kernel
         for (int jj = 0; jj < TN; ++jj) acc[i][jj] = i32x4_t{0, 0, 0, 0};
 
     const int r16 = lane & 15;
-    const int pc = ((lane >> 4) ^ oz_swz(r16)) * 16;  // stored position of this lane's chunk (row = r16 mod 16)
+    const int pc = ((lane >> 4) ^ lds_swz(r16)) * 16;  // stored position of this lane's chunk (row = r16 mod 16)
     const int nk = Kp / BK;
     struct Frags {
         i8x16_t a[TM], b[TN];
     };
     auto fetch = [&](Frags &f, int t) {  // this wave's operand fragments of tile t, LDS -> registers
-        const signed char *As = smem8 + (t % 3) * BUF, *Bs = As + BM * BK;
+        const signed char *As = smem8 + (t % NBUF) * BUF, *Bs = As + BM * BK;
 #pragma unroll
         for (int i = 0; i < TM; ++i) f.a[i] = *reinterpret_cast<const i8x16_t *>(As + (wm * WM + i * 16 + r16) * BK + pc);
 #pragma unroll
@@ -330,18 +315,9 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_i8_dma_kernel
                 acc[i][jj] = __builtin_amdgcn_mfma_i32_16x16x64_i8(__builtin_bit_cast(i32x4_t, f.a[i]),
                                                                    __builtin_bit_cast(i32x4_t, f.b[jj]), acc[i][jj], 0, 0, 0);
     };
-    // Before the barrier of tile t: this wave's loads of tile t have landed (those of tile t+1 may
-    // still be in flight) and its LDS reads of tile t-1 are complete. After it, tile t is visible to
-    // every wave and nobody reads tile t-1's buffer any more, so tile t+2 can stream into it. A bare
-    // s_barrier: the workgroup fence of __syncthreads would also wait for tile t+1's loads; the
-    // empty asm statements keep the compiler from moving LDS accesses across it.
     auto sync_tile = [&](int t) {
-        if (t + 1 < nk) OZ_WAIT_VM_LGKM0(RA + RB);
-        else OZ_WAIT_VM_LGKM0(0);
-        asm volatile("" ::: "memory");
-        __builtin_amdgcn_s_barrier();
-        asm volatile("" ::: "memory");
-        if (t + 2 < nk) issue(t + 2);
+        lowp_tile_barrier<NBUF, RA + RB>(t, nk);
+        if (t + NBUF - 1 < nk) issue(t + NBUF - 1);  // into the buffer tile t-1 used
     };
     // Fragments are double-buffered in registers: the MFMAs of tile t-1 run while tile t's LDS
     // reads are in flight, so the matrix cores never wait on LDS latency.
@@ -350,8 +326,9 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_i8_dma_kernel
     // fragments and accumulators between register assignments): the last pair also reads the
     // fragments of "tile nk" from a buffer holding stale data, and never uses them.
     Frags f0, f1;
-    issue(0);
-    issue(1);
+#pragma unroll
+    for (int t = 0; t < NBUF - 1; ++t)
+        if (t < nk) issue(t);
     sync_tile(0);
     fetch(f0, 0);
     for (int t = 0; t < nk; t += 2) {
@@ -456,13 +433,13 @@ __global__ __launch_bounds__(256) void crt_kernel(const signed char *__restrict_
 }
 
 // ---------------- host driver ----------------
-template <int BM, int BN, int WM, int WN>
+template <int BM, int BN, int WM, int WN, int NBUF = 3>
 static nrk_status launch_i8(int M, int N, int Kp, const signed char *A, size_t sa, const signed char *Bt, size_t sb,
                             signed char *Cres, size_t sc, const OzConsts &k, hipStream_t stream) {
     constexpr int threads = (BM / WM) * (BN / WN) * 64;
-    constexpr size_t lds = (size_t)3 * (BM + BN) * OZ_DMA_BK;
+    constexpr size_t lds = (size_t)NBUF * (BM + BN) * OZ_DMA_BK;
     static_assert(lds <= 160 * 1024, "tile does not fit in LDS");
-    auto kern = gemm_i8_dma_kernel<BM, BN, WM, WN>;
+    auto kern = gemm_i8_dma_kernel<BM, BN, WM, WN, NBUF>;
     static bool attr_set = false;
     if (!attr_set) {
         NRK_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
