@@ -44,6 +44,7 @@ struct OzConsts {
     double inv_m[OZ_MAX_MODULI];
     double w[OZ_MAX_MODULI][OZ_LIMBS];      // CRT weights, 24-bit limbs, least significant first
     double Mlimb[OZ_LIMBS];                 // M in 24-bit limbs
+    int limbs;                              // 24-bit limbs of M (and of every weight)
     double M_approx;                        // M as a double (rounded)
 };
 
@@ -86,6 +87,7 @@ static nrk_status make_consts(int n, OzConsts *c) {
     Big M(1);
     for (int j = 0; j < n; ++j) M.mul_small((uint64_t)kModuli[j]);
     for (int t = 0; t < OZ_LIMBS; ++t) c->Mlimb[t] = t < (int)M.d.size() ? (double)M.d[t] : 0.0;
+    c->limbs = (int)M.d.size();
     c->M_approx = M.to_double();
     for (int j = 0; j < n; ++j) {
         const int mj = kModuli[j];
@@ -374,6 +376,8 @@ __device__ __forceinline__ dd_t dd_two_sum(double a, double b) {
 // Thread t reads word t of every modulus' residue array: the residues of block t / 64 (16 x 16
 // blocks, Nb = ceil(N/16) per block row) at lane l = t % 64, i.e. rows 4 (l/16) + 0..3 of column
 // l % 16, and reconstructs those four outputs. A wavefront reads 256 contiguous bytes per modulus.
+// NL >= k.limbs limbs: the weights, M and every intermediate fit (limbs above k.limbs are zero).
+template <int NL>
 __global__ __launch_bounds__(256) void crt_kernel(const signed char *__restrict__ Cres, size_t c_stride, int M, int N,
                                                   const int *__restrict__ srow, const int *__restrict__ scol, OzConsts k,
                                                   double *__restrict__ C) {
@@ -396,24 +400,24 @@ __global__ __launch_bounds__(256) void crt_kernel(const signed char *__restrict_
             C[idx] = __builtin_nan("");
             continue;
         }
-        double L[OZ_LIMBS];
+        double L[NL];
 #pragma unroll
-        for (int t = 0; t < OZ_LIMBS; ++t) L[t] = 0.0;
+        for (int t = 0; t < NL; ++t) L[t] = 0.0;
         for (int j = 0; j < k.n; ++j) {
             const double c = (double)(signed char)(words[j] >> (8 * u));
 #pragma unroll
-            for (int t = 0; t < OZ_LIMBS; ++t) L[t] = fma(c, k.w[j][t], L[t]);  // exact: |sum| < 2^36
+            for (int t = 0; t < NL; ++t) L[t] = fma(c, k.w[j][t], L[t]);  // exact: |sum| < 2^36
         }
         // X_raw = sum_t L[t] 2^(24 t); subtract q M with q = round(X_raw / M)
         double approx = 0.0;
 #pragma unroll
-        for (int t = OZ_LIMBS - 1; t >= 0; --t) approx = approx * 16777216.0 + L[t];
+        for (int t = NL - 1; t >= 0; --t) approx = approx * 16777216.0 + L[t];
         const double q = rint(approx / k.M_approx);
 #pragma unroll
-        for (int t = 0; t < OZ_LIMBS; ++t) L[t] = fma(-q, k.Mlimb[t], L[t]);  // exact
+        for (int t = 0; t < NL; ++t) L[t] = fma(-q, k.Mlimb[t], L[t]);  // exact
         // balanced carry normalisation: every limb in [-2^23, 2^23]
 #pragma unroll
-        for (int t = 0; t < OZ_LIMBS - 1; ++t) {
+        for (int t = 0; t < NL - 1; ++t) {
             const double carry = rint(L[t] * (1.0 / 16777216.0));
             L[t] = fma(-carry, 16777216.0, L[t]);
             L[t + 1] += carry;
@@ -421,7 +425,7 @@ __global__ __launch_bounds__(256) void crt_kernel(const signed char *__restrict_
         // double-double Horner from the top limb
         dd_t x{0.0, 0.0};
 #pragma unroll
-        for (int t = OZ_LIMBS - 1; t >= 0; --t) {
+        for (int t = NL - 1; t >= 0; --t) {
             x.hi *= 16777216.0;
             x.lo *= 16777216.0;
             dd_t s = dd_two_sum(x.hi, L[t]);
@@ -509,8 +513,13 @@ nrk_status gemm_f64_ozaki(size_t m, size_t k, size_t n, const double *a, const d
         case 3: NRK_TRY((launch_i8<256, 256, 128, 64>((int)m, (int)n, (int)Kp, Ares, sa, Bres, sb, Cres, sc, kc, stream))); break;
         default: NRK_TRY((launch_i8<256, 256, 128, 128>((int)m, (int)n, (int)Kp, Ares, sa, Bres, sb, Cres, sc, kc, stream)));
     }
-    hipLaunchKernelGGL(crt_kernel, dim3((unsigned)ceil_div(sc / 4, 256)), dim3(256), 0, stream, Cres, sc, (int)m, (int)n,
-                       srow, scol, kc, c);
+    {  // the fewest limbs that hold M: 5 for up to 15 moduli, 6 for 16
+        const dim3 g((unsigned)ceil_div(sc / 4, 256));
+        if (kc.limbs <= 4) hipLaunchKernelGGL(crt_kernel<4>, g, dim3(256), 0, stream, Cres, sc, (int)m, (int)n, srow, scol, kc, c);
+        else if (kc.limbs == 5) hipLaunchKernelGGL(crt_kernel<5>, g, dim3(256), 0, stream, Cres, sc, (int)m, (int)n, srow, scol, kc, c);
+        else if (kc.limbs == 6) hipLaunchKernelGGL(crt_kernel<6>, g, dim3(256), 0, stream, Cres, sc, (int)m, (int)n, srow, scol, kc, c);
+        else hipLaunchKernelGGL(crt_kernel<OZ_LIMBS>, g, dim3(256), 0, stream, Cres, sc, (int)m, (int)n, srow, scol, kc, c);
+    }
     NRK_HIP(hipGetLastError());
     return NRK_OK;
 }
