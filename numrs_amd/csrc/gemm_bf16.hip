@@ -73,9 +73,8 @@ __device__ __forceinline__ constexpr int pair_b(int p) {
 // K tiles of 32 bf16 (64 bytes per row) move global -> LDS by the load unit itself
 // (global_load_lds), all terms of both operands in one buffer (NTA*BM A rows, then NTB*BN Bt
 // rows, swizzled by lds_swz), NBUF buffers deep: tiles t+1 .. t+NBUF-2 are in flight while tile
-// t is multiplied, one barrier per tile. MFMA fragments are double-buffered in registers, so the
-// products of tile t-1 run while tile t's fragments are read. Per K step the accumulation order
-// is: pairs smallest first, then the 16 x 16 blocks.
+// t is multiplied, one barrier per tile. Per output and K step the products are accumulated
+// pairs smallest first.
 constexpr int BF_BK = 32;  // bf16 values per K tile (one MFMA K step)
 
 template <int BM, int BN, int WM, int WN, int NTA, int NTB, int NPAIR, int NBUF>
@@ -109,8 +108,8 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_bf16_nt_kerne
         else
             src[s] = Bt + ((row - NTA * BM) / BN) * b_term_stride + (size_t)(bn0 + (row - NTA * BM) % BN) * Kp + c * 8;
     }
-    auto issue = [&](int t) {
-        unsigned char *base = smem_bf + (t % NBUF) * BUF + wave * 64 * 16;  // this wave's 1 KiB slot
+    auto issue = [&](int t, int slot) {  // K tile t into LDS buffer slot
+        unsigned char *base = smem_bf + slot * BUF + wave * 64 * 16;  // this wave's 1 KiB slot
         const size_t k0 = (size_t)t * BF_BK;
 #pragma unroll
         for (int s = 0; s < R; ++s) lds_dma16(src[s] + k0, base + s * NT * 16);
@@ -125,53 +124,68 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_bf16_nt_kerne
     const int r16 = lane & 15;
     const int pc = ((lane >> 4) ^ lds_swz(r16)) * 16;  // stored position of this lane's chunk (row = r16 mod 16)
     const int nk = Kp / BF_BK;
-    struct Frags {
-        bf16x8_t a[NTA][TM], b[NTB][TN];
-    };
-    auto fetch = [&](Frags &f, int t) {
-        const unsigned char *As = smem_bf + (t % NBUF) * BUF, *Bs = As + NTA * BM * 64;
+    // Operand fragments in registers: the A row blocks (all terms) of the tile being multiplied,
+    // each reloaded with the next tile's as soon as its products are issued, and the B column
+    // blocks of two tiles (the next tile's are read while this one's products run), so the matrix
+    // cores never wait on LDS latency. The scheduling fences keep the compiler from sinking the
+    // reads to the next barrier, where their latency would be exposed.
+    bf16x8_t fa[NTA][TM], fb0[NTB][TN], fb1[NTB][TN];
+    auto load_a = [&](int i, int t) {
+        const unsigned char *As = smem_bf + (t % NBUF) * BUF;
 #pragma unroll
         for (int u = 0; u < NTA; ++u)
-#pragma unroll
-            for (int i = 0; i < TM; ++i)
-                f.a[u][i] = *reinterpret_cast<const bf16x8_t *>(As + (u * BM + wm * WM + i * 16 + r16) * 64 + pc);
+            fa[u][i] = *reinterpret_cast<const bf16x8_t *>(As + (u * BM + wm * WM + i * 16 + r16) * 64 + pc);
+    };
+    auto load_b = [&](bf16x8_t(&fb)[NTB][TN], int t) {
+        const unsigned char *Bs = smem_bf + (t % NBUF) * BUF + NTA * BM * 64;
 #pragma unroll
         for (int u = 0; u < NTB; ++u)
 #pragma unroll
             for (int j = 0; j < TN; ++j)
-                f.b[u][j] = *reinterpret_cast<const bf16x8_t *>(Bs + (u * BN + wn * WN + j * 16 + r16) * 64 + pc);
+                fb[u][j] = *reinterpret_cast<const bf16x8_t *>(Bs + (u * BN + wn * WN + j * 16 + r16) * 64 + pc);
     };
-    auto mma = [&](const Frags &f) {
+    // products of the tile held in fa / fb (per output: the pairs in order), A row block i then
+    // reloaded from tile tn
+    auto mma_reload = [&](const bf16x8_t(&fb)[NTB][TN], int tn) {
 #pragma unroll
-        for (int p = 0; p < NPAIR; ++p)
+        for (int i = 0; i < TM; ++i) {
 #pragma unroll
-            for (int i = 0; i < TM; ++i)
+            for (int p = 0; p < NPAIR; ++p)
 #pragma unroll
                 for (int j = 0; j < TN; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(f.a[pair_a<NPAIR>(p)][i], f.b[pair_b<NPAIR>(p)][j],
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[pair_a<NPAIR>(p)][i], fb[pair_b<NPAIR>(p)][j],
                                                                         acc[i][j], 0, 0, 0);
+            load_a(i, tn);
+            __builtin_amdgcn_sched_barrier(0);
+        }
     };
+    // Every pass issues one group of loads (past the last tile, a harmless reload of the last
+    // tile into the free buffer), so exactly NBUF-2 groups are in flight behind tile t and the
+    // wait count is a constant: no branch around the barrier.
     auto sync_tile = [&](int t) {
-        lowp_tile_barrier<NBUF, R>(t, nk);
-        if (t + NBUF - 1 < nk) issue(t + NBUF - 1);  // into the buffer tile t-1 used
+        lowp_tile_barrier<(NBUF - 2) * R>();
+        issue(min(t + NBUF - 1, nk - 1), (t + NBUF - 1) % NBUF);  // into the buffer tile t-1 used
     };
     // Kp is a multiple of 64, so nk is even and tiles pair up (t, t+1); the last pair also reads
     // the fragments of "tile nk" from a stale buffer and never uses them (no branch around the
     // MFMAs, which would give the accumulators a second register assignment).
-    Frags f0, f1;
 #pragma unroll
-    for (int t = 0; t < NBUF - 1; ++t)
-        if (t < nk) issue(t);
+    for (int t = 0; t < NBUF - 1; ++t) issue(min(t, nk - 1), t);
     sync_tile(0);
-    fetch(f0, 0);
+#pragma unroll
+    for (int i = 0; i < TM; ++i) load_a(i, 0);
+    load_b(fb0, 0);
     for (int t = 0; t < nk; t += 2) {
         sync_tile(t + 1);
-        fetch(f1, t + 1);
-        mma(f0);
+        load_b(fb1, t + 1);
+        __builtin_amdgcn_sched_barrier(0);
+        mma_reload(fb0, t + 1);
         sync_tile(t + 2);
-        fetch(f0, t + 2);
-        mma(f1);
+        load_b(fb0, t + 2);
+        __builtin_amdgcn_sched_barrier(0);
+        mma_reload(fb1, t + 2);
     }
+    LOWP_WAIT_VM_LGKM0(0);  // the trailing reloads have landed before the workgroup ends
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll

@@ -60,20 +60,16 @@ __device__ __forceinline__ void lds_dma16(const void *g, void *l) {
 // 15:14, expcnt 6:4, lgkmcnt 11:8).
 #define LOWP_WAIT_VM_LGKM0(n) __builtin_amdgcn_s_waitcnt(((n) & 15) | ((((n) >> 4) & 3) << 14) | (0x7 << 4))
 
-// Start of K tile t's pass in an NBUF-deep pipeline of nk tiles, R loads per thread per tile.
-// Before the barrier: this wave's loads of tile t have landed (those of the up to NBUF-2 later
-// tiles already issued may still be in flight) and its LDS reads of tile t-1 are complete. After
-// it, tile t is visible to every wave and nobody reads tile t-1's buffer any more, so the caller
-// may stream tile t+NBUF-1 into it. A bare s_barrier: the workgroup fence of __syncthreads would
-// also wait for the later tiles' loads; the empty asm statements keep the compiler from moving
-// LDS accesses across it.
-template <int NBUF, int R>
-__device__ __forceinline__ void lowp_tile_barrier(int t, int nk) {
-    static_assert(NBUF >= 3 && NBUF <= 4 && (NBUF - 2) * R <= 63, "vmcnt range");
-    const int ahead = min(nk - 1 - t, NBUF - 2);  // tiles after t already issued
-    if (ahead >= 2) LOWP_WAIT_VM_LGKM0(2 * R);
-    else if (ahead == 1) LOWP_WAIT_VM_LGKM0(R);
-    else LOWP_WAIT_VM_LGKM0(0);
+// Start of a K tile's pass in a pipeline that always has NPEND of this thread's loads issued
+// after the tile's own. Before the barrier: this wave's loads of the tile have landed and its LDS
+// reads of the previous tile are complete. After it, the tile is visible to every wave and nobody
+// reads the previous tile's buffer any more, so the caller may stream a new tile into it. A bare
+// s_barrier: the workgroup fence of __syncthreads would also wait for the later tiles' loads; the
+// empty asm statements keep the compiler from moving LDS accesses across it.
+template <int NPEND>
+__device__ __forceinline__ void lowp_tile_barrier() {
+    static_assert(NPEND >= 0 && NPEND <= 63, "vmcnt range");
+    LOWP_WAIT_VM_LGKM0(NPEND);
     asm volatile("" ::: "memory");
     __builtin_amdgcn_s_barrier();
     asm volatile("" ::: "memory");

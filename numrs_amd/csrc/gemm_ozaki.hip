@@ -281,8 +281,8 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_i8_dma_kernel
         const int p = s * NT + tid, row = p >> 2, c = (p & 3) ^ lds_swz(row);
         srcB[s] = Bt + (size_t)(bn0 + row) * Kp + c * 16;
     }
-    auto issue = [&](int t) {
-        signed char *base = smem8 + (t % NBUF) * BUF + wave * 64 * 16;  // this wave's 1 KiB slot
+    auto issue = [&](int t, int slot) {  // K tile t into LDS buffer slot
+        signed char *base = smem8 + slot * BUF + wave * 64 * 16;  // this wave's 1 KiB slot
         const size_t k0 = (size_t)t * BK;
 #pragma unroll
         for (int s = 0; s < RA; ++s) lds_dma16(srcA[s] + k0, base + s * NT * 16);
@@ -299,48 +299,60 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_i8_dma_kernel
     const int r16 = lane & 15;
     const int pc = ((lane >> 4) ^ lds_swz(r16)) * 16;  // stored position of this lane's chunk (row = r16 mod 16)
     const int nk = Kp / BK;
-    struct Frags {
-        i8x16_t a[TM], b[TN];
+    // Operand fragments in registers: the A row blocks of the tile being multiplied (each reloaded
+    // with the next tile's as soon as its products are issued) and the B column blocks of two
+    // tiles (the next tile's are read while this one's products run), so the matrix cores never
+    // wait on LDS latency. The scheduling fences keep the compiler from sinking the reads to the
+    // next barrier, where their latency would be exposed.
+    i8x16_t fa[TM], fb0[TN], fb1[TN];
+    auto load_a = [&](int i, int t) {
+        fa[i] = *reinterpret_cast<const i8x16_t *>(smem8 + (t % NBUF) * BUF + (wm * WM + i * 16 + r16) * BK + pc);
     };
-    auto fetch = [&](Frags &f, int t) {  // this wave's operand fragments of tile t, LDS -> registers
-        const signed char *As = smem8 + (t % NBUF) * BUF, *Bs = As + BM * BK;
+    auto load_b = [&](i8x16_t(&fb)[TN], int t) {
+        const signed char *Bs = smem8 + (t % NBUF) * BUF + BM * BK;
 #pragma unroll
-        for (int i = 0; i < TM; ++i) f.a[i] = *reinterpret_cast<const i8x16_t *>(As + (wm * WM + i * 16 + r16) * BK + pc);
-#pragma unroll
-        for (int jj = 0; jj < TN; ++jj) f.b[jj] = *reinterpret_cast<const i8x16_t *>(Bs + (wn * WN + jj * 16 + r16) * BK + pc);
+        for (int jj = 0; jj < TN; ++jj) fb[jj] = *reinterpret_cast<const i8x16_t *>(Bs + (wn * WN + jj * 16 + r16) * BK + pc);
     };
-    auto mma = [&](const Frags &f) {
+    // products of the tile held in fa / fb; A row block i then reloaded from tile tn
+    auto mma_reload = [&](const i8x16_t(&fb)[TN], int tn) {
 #pragma unroll
-        for (int i = 0; i < TM; ++i)
+        for (int i = 0; i < TM; ++i) {
 #pragma unroll
             for (int jj = 0; jj < TN; ++jj)
-                acc[i][jj] = __builtin_amdgcn_mfma_i32_16x16x64_i8(__builtin_bit_cast(i32x4_t, f.a[i]),
-                                                                   __builtin_bit_cast(i32x4_t, f.b[jj]), acc[i][jj], 0, 0, 0);
+                acc[i][jj] = __builtin_amdgcn_mfma_i32_16x16x64_i8(__builtin_bit_cast(i32x4_t, fa[i]),
+                                                                   __builtin_bit_cast(i32x4_t, fb[jj]), acc[i][jj], 0, 0, 0);
+            load_a(i, tn);
+            __builtin_amdgcn_sched_barrier(0);
+        }
     };
+    // Every pass issues one group of loads (past the last tile, a harmless reload of the last
+    // tile into the free buffer), so exactly NBUF-2 groups are in flight behind tile t and the
+    // wait count is a constant: no branch around the barrier.
     auto sync_tile = [&](int t) {
-        lowp_tile_barrier<NBUF, RA + RB>(t, nk);
-        if (t + NBUF - 1 < nk) issue(t + NBUF - 1);  // into the buffer tile t-1 used
+        lowp_tile_barrier<(NBUF - 2) * (RA + RB)>();
+        issue(min(t + NBUF - 1, nk - 1), (t + NBUF - 1) % NBUF);  // into the buffer tile t-1 used
     };
-    // Fragments are double-buffered in registers: the MFMAs of tile t-1 run while tile t's LDS
-    // reads are in flight, so the matrix cores never wait on LDS latency.
     // Kp is a multiple of 128, so the tile count nk is even and tiles pair up (t, t+1). The loop
     // body has no branch around fragment reads or MFMAs (a branch would make the compiler copy
     // fragments and accumulators between register assignments): the last pair also reads the
     // fragments of "tile nk" from a buffer holding stale data, and never uses them.
-    Frags f0, f1;
 #pragma unroll
-    for (int t = 0; t < NBUF - 1; ++t)
-        if (t < nk) issue(t);
+    for (int t = 0; t < NBUF - 1; ++t) issue(min(t, nk - 1), t);
     sync_tile(0);
-    fetch(f0, 0);
+#pragma unroll
+    for (int i = 0; i < TM; ++i) load_a(i, 0);
+    load_b(fb0, 0);
     for (int t = 0; t < nk; t += 2) {
         sync_tile(t + 1);
-        fetch(f1, t + 1);
-        mma(f0);
+        load_b(fb1, t + 1);
+        __builtin_amdgcn_sched_barrier(0);
+        mma_reload(fb0, t + 1);
         sync_tile(t + 2);
-        fetch(f0, t + 2);
-        mma(f1);
+        load_b(fb0, t + 2);
+        __builtin_amdgcn_sched_barrier(0);
+        mma_reload(fb1, t + 2);
     }
+    LOWP_WAIT_VM_LGKM0(0);  // the trailing reloads have landed before the workgroup ends
     // Epilogue: symmetric residues mod m_j, written in 16 x 16 blocks (block (bi, bj) at byte
     // (bi * Nb + bj) * 256, lane l's word at + 4 l holding rows 4 (l/16) + 0..3 of column l % 16,
     // its accumulator order), so each block is one 256-byte store of the wave. Blocks outside
@@ -437,7 +449,7 @@ __global__ __launch_bounds__(256) void crt_kernel(const signed char *__restrict_
 }
 
 // ---------------- host driver ----------------
-template <int BM, int BN, int WM, int WN, int NBUF = 3>
+template <int BM, int BN, int WM, int WN, int NBUF = 4>
 static nrk_status launch_i8(int M, int N, int Kp, const signed char *A, size_t sa, const signed char *Bt, size_t sb,
                             signed char *Cres, size_t sc, const OzConsts &k, hipStream_t stream) {
     constexpr int threads = (BM / WM) * (BN / WN) * 64;
