@@ -274,8 +274,13 @@ struct LaneAcc {
     unsigned nan;
 };
 
-// Issue the loads of the segment at `base` (len elements of the row remain there) into y, zero
-// outside the row, and record which elements exist. VEC: len % VN == 0, so whole vectors.
+// Issue the loads of the segment at `base` (len > 0 elements of the row remain there) into y and
+// record in vmask which elements exist (the others hold copies of the segment's first element;
+// the fold reads them only through vmask). VEC: len % VN == 0, so whole vectors. Every lane issues
+// every load, straight into y, so a segment is always exactly VPT (or VPT * VN) loads with no
+// branch around them and no copy after them: the compiler can then wait for one segment's loads
+// while the next segment's are in flight, where a data-dependent branch around a load (or a
+// select that frees the load registers for reuse) makes it wait for all of them.
 template <typename T, int VPT, bool VEC>
 __device__ __forceinline__ void seg_load(const T *base, size_t len, T (&y)[VPT * Vec16<T>::N], unsigned &vmask) {
     typedef typename Vec16<T>::type V;
@@ -291,8 +296,7 @@ __device__ __forceinline__ void seg_load(const T *base, size_t len, T (&y)[VPT *
         for (int j = 0; j < VPT; ++j) {
             const unsigned jv = j * 64 + lane;
             const bool ok = jv < nvec;
-            V v = V{};
-            if (ok) v = __builtin_nontemporal_load(bv + jv);
+            const V v = __builtin_nontemporal_load(bv + (ok ? jv : 0u));
 #pragma unroll
             for (int e = 0; e < VN; ++e) y[j * VN + e] = v[e];
             vmask |= (ok ? ((1u << VN) - 1u) : 0u) << (j * VN);
@@ -304,7 +308,7 @@ __device__ __forceinline__ void seg_load(const T *base, size_t len, T (&y)[VPT *
             for (int e = 0; e < VN; ++e) {
                 const unsigned pos = (j * 64 + lane) * VN + e;
                 const bool ok = pos < len;
-                y[j * VN + e] = ok ? base[pos] : (T)0;
+                y[j * VN + e] = base[ok ? pos : 0u];
                 vmask |= (ok ? 1u : 0u) << (j * VN + e);
             }
         }
@@ -524,19 +528,21 @@ __global__ __launch_bounds__(RB) void stream_kernel(SegParams<T> p) {
     T ya[M], yb[M];
     unsigned ma, mb;
     seg_load<T, VPT, VEC>(base_of(ka, sa), len_of(sa), ya, ma);
+    // The next segment's loads are issued unconditionally (past the wavefront's last segment they
+    // reread the current one and are never used): no branch around them.
     for (;;) {
         ChunkPos kb = ka;
         size_t sb = sa;
         next(kb, sb);
         const bool hb = kb.c < p.nchunk;
-        if (hb) seg_load<T, VPT, VEC>(base_of(kb, sb), len_of(sb), yb, mb);
+        seg_load<T, VPT, VEC>(hb ? base_of(kb, sb) : base_of(ka, sa), hb ? len_of(sb) : len_of(sa), yb, mb);
         seg_step<T, VPT, FLAGS>(p, ka, sa, ya, ma, a, fnan, bt, W);
         if (!hb) break;
         ka = kb;
         sa = sb;
         next(ka, sa);
         const bool ha = ka.c < p.nchunk;
-        if (ha) seg_load<T, VPT, VEC>(base_of(ka, sa), len_of(sa), ya, ma);
+        seg_load<T, VPT, VEC>(ha ? base_of(ka, sa) : base_of(kb, sb), ha ? len_of(sa) : len_of(sb), ya, ma);
         seg_step<T, VPT, FLAGS>(p, kb, sb, yb, mb, a, fnan, bt, W);
         if (!ha) break;
     }
