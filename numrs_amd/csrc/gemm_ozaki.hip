@@ -42,10 +42,12 @@ struct OzConsts {
     int n;                                  // number of moduli
     int m[OZ_MAX_MODULI];
     double inv_m[OZ_MAX_MODULI];
+    double c26[OZ_MAX_MODULI];              // 2^26 mod m_j
     double w[OZ_MAX_MODULI][OZ_LIMBS];      // CRT weights, 24-bit limbs, least significant first
     double Mlimb[OZ_LIMBS];                 // M in 24-bit limbs
     int limbs;                              // 24-bit limbs of M (and of every weight)
     double M_approx;                        // M as a double (rounded)
+    double inv_M;                           // 1 / M_approx
 };
 
 // ---------------- host-side multiprecision for the CRT constants ----------------
@@ -89,10 +91,12 @@ static nrk_status make_consts(int n, OzConsts *c) {
     for (int t = 0; t < OZ_LIMBS; ++t) c->Mlimb[t] = t < (int)M.d.size() ? (double)M.d[t] : 0.0;
     c->limbs = (int)M.d.size();
     c->M_approx = M.to_double();
+    c->inv_M = 1.0 / c->M_approx;
     for (int j = 0; j < n; ++j) {
         const int mj = kModuli[j];
         c->m[j] = mj;
         c->inv_m[j] = 1.0 / (double)mj;
+        c->c26[j] = (double)((1u << 26) % (unsigned)mj);
         Big Mj(1);
         int Mj_mod = 1;
         for (int i = 0; i < n; ++i) {
@@ -169,13 +173,14 @@ __global__ __launch_bounds__(256) void col_scale_kernel(const double *__restrict
 }
 
 // ---------------- 2. residues ----------------
-__device__ __forceinline__ signed char sym_mod(double v, double m, double inv_m) {
-    const double q = rint(v * inv_m);
-    double r = fma(-q, m, v);  // exact: the true value is a small integer
-    const double h = 0.5 * (m - 1.0);
-    r = r > h ? r - m : r;
-    r = r < -h ? r + m : r;
-    return (signed char)(int)r;
+// Symmetric residue of the integer v = vh 2^26 + vl (|vh| < 2^27, 0 <= vl < 2^26) modulo m:
+// t = vh (2^26 mod m) + vl is congruent to v and below 2^35 in magnitude, so t / m is within far
+// less than 1/(2m) of its FP64 value and rint picks the nearest integer quotient (t / m is never
+// a half-integer, m being odd): r = t - m rint(t / m) lies in [-(m-1)/2, (m-1)/2], exactly.
+__device__ __forceinline__ unsigned sym_mod_byte(double vh, double vl, double m, double inv_m, double c26) {
+    const double t = fma(vh, c26, vl);  // exact
+    const double r = fma(-rint(t * inv_m), m, t);
+    return (unsigned)(int)r & 0xffu;
 }
 
 // Each thread produces 16 consecutive K positions of one output row for every modulus and
@@ -207,24 +212,26 @@ __global__ __launch_bounds__(256) void residue_kernel(const double *__restrict__
     const int nrows = TRANSPOSE ? Cc : R;  // logical output rows that exist
     const int klen = TRANSPOSE ? R : Cc;   // logical K
     const int s = orow < nrows ? scale[orow] : OZ_BAD_SCALE;
-    double v[16];
+    double vh[16], vl[16];  // the scaled integer split as vh 2^26 + vl, 0 <= vl < 2^26 (exact)
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
         const int kk = k0 + e;
         double x = 0.0;
         if (s != OZ_BAD_SCALE && kk < klen) x = TRANSPOSE ? X[(size_t)kk * Cc + orow] : X[(size_t)orow * Cc + kk];
-        v[e] = rint(ldexp(x, s == OZ_BAD_SCALE ? 0 : s));
+        const double v = rint(ldexp(x, s == OZ_BAD_SCALE ? 0 : s));
+        vh[e] = floor(ldexp(v, -26));
+        vl[e] = fma(-vh[e], 67108864.0, v);
     }
     const size_t stride = (size_t)out_rows * out_cols;
     signed char *o = out + (size_t)orow * out_cols + k0;
     for (int j = 0; j < k.n; ++j) {
-        const double m = (double)k.m[j], im = k.inv_m[j];
+        const double m = (double)k.m[j], im = k.inv_m[j], c = k.c26[j];
         unsigned w[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             unsigned x = 0;
 #pragma unroll
-            for (int b = 0; b < 4; ++b) x |= ((unsigned)(unsigned char)sym_mod(v[4 * q + b], m, im)) << (8 * b);
+            for (int b = 0; b < 4; ++b) x |= sym_mod_byte(vh[4 * q + b], vl[4 * q + b], m, im, c) << (8 * b);
             w[q] = x;
         }
         *reinterpret_cast<u32x4 *>(o + j * stride) = u32x4{w[0], w[1], w[2], w[3]};
@@ -424,7 +431,8 @@ __global__ __launch_bounds__(256) void crt_kernel(const signed char *__restrict_
         double approx = 0.0;
 #pragma unroll
         for (int t = NL - 1; t >= 0; --t) approx = approx * 16777216.0 + L[t];
-        const double q = rint(approx / k.M_approx);
+        // |C'| < M/4 by the choice of P, so q only has to be within 1/4 of X_raw / M
+        const double q = rint(approx * k.inv_M);
 #pragma unroll
         for (int t = 0; t < NL; ++t) L[t] = fma(-q, k.Mlimb[t], L[t]);  // exact
         // balanced carry normalisation: every limb in [-2^23, 2^23]
@@ -434,13 +442,18 @@ __global__ __launch_bounds__(256) void crt_kernel(const signed char *__restrict_
             L[t] = fma(-carry, 16777216.0, L[t]);
             L[t + 1] += carry;
         }
-        // double-double Horner from the top limb
-        dd_t x{0.0, 0.0};
+        // double-double Horner from the top, two limbs per digit: with every limb in
+        // [-2^23, 2^23] a pair L[2i+1] 2^24 + L[2i] is an exact double, so X = sum_i P_i 2^(48 i)
+        constexpr int NP = (NL + 1) / 2;
+        double P[NP];
 #pragma unroll
-        for (int t = NL - 1; t >= 0; --t) {
-            x.hi *= 16777216.0;
-            x.lo *= 16777216.0;
-            dd_t s = dd_two_sum(x.hi, L[t]);
+        for (int i = 0; i < NP; ++i) P[i] = 2 * i + 1 < NL ? fma(L[2 * i + 1], 16777216.0, L[2 * i]) : L[2 * i];
+        dd_t x{P[NP - 1], 0.0};
+#pragma unroll
+        for (int i = NP - 2; i >= 0; --i) {
+            x.hi *= 281474976710656.0;  // 2^48, exact
+            x.lo *= 281474976710656.0;
+            dd_t s = dd_two_sum(x.hi, P[i]);
             s.lo += x.lo;
             x = dd_two_sum(s.hi, s.lo);
         }
