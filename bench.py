@@ -12,7 +12,7 @@ pass times each call on its own (HIP events on the launching stream) for the per
 and the roofline of the dominant kernel against MI355X's 8 TB/s.
 
 Also reported (same JSON line): the split-precision GEMMs against the native MFMA GEMMs (f32
-8192^3 BF16x3/x6, f64 4096^3 and 8192^3 Ozaki-II with 14/15/16 moduli; error of sampled entries
+8192^3 BF16x3/x6, f64 4096^3 and 8192^3 Ozaki-II with 13-16 moduli; error of sampled entries
 against correctly rounded dot products) and the ragged batch (4096 bf16 GEMMs + 65536 f32 row
 sums, one persistent launch replayed from a hipGraph vs the per-call loop). cpu_baseline times the
 oracle's restatement of the reference's CPU path (unfused) on this host.
@@ -306,7 +306,7 @@ def bench_gemm(device, rng, call, s, stream):
                 runs.append((f"bf16x{p}", (lambda p=p: call("nrk_dev_gemm_f32_bf16x", n, n, n, pa, pb, pc, p, C.c_void_p(s))), p))
         else:
             runs = [("native_fp64", lambda: call("nrk_dev_gemm_f64", n, n, n, pa, pb, pc, C.c_void_p(s)), 1)]
-            for mo in (14, 15, 16):
+            for mo in (13, 14, 15, 16):
                 runs.append((f"ozaki{mo}", (lambda mo=mo: call("nrk_dev_gemm_f64_ozaki", n, n, n, pa, pb, pc, mo, C.c_void_p(s))), mo))
         res = {}
         for name, f, products in runs:

@@ -4,8 +4,9 @@
 // idea and the INT8-unit DGEMM of Ootomo, Ozaki & Yokota (2024).
 //
 //   1. Scale rows of A and columns of B by powers of two, 2^s_i and 2^t_j, so that every scaled
-//      entry is below 2^P in magnitude, and round to integers: A' = round(2^s A), B' = round(B 2^t).
-//      (P <= 53, so A' and B' are exact doubles; P is the largest value with k * 2^(2P) < M / 2.)
+//      row and column has a 2-norm of at most X, X^2 < M / 2 (Cauchy-Schwarz then bounds every
+//      |C'_ij| by X^2), and round to integers: A' = round(2^s A), B' = round(B 2^t). X <= 2^62,
+//      and an entry of 2^52 or more is already an integer, so A' and B' are exact doubles.
 //   2. For each of N pairwise-coprime odd moduli m_j <= 255, take symmetric residues
 //      a_j = A' mod m_j, b_j = B' mod m_j in [-127, 127] (int8, exact).
 //   3. C_j = a_j * b_j^T on v_mfma_i32_16x16x64_i8: int32 accumulation is exact
@@ -13,11 +14,12 @@
 //   4. CRT: C' = sum_j c_j w_j mod M (w_j = (M/m_j) * ((M/m_j)^-1 mod m_j), M = prod m_j) is the
 //      EXACT integer product A'B'. It is formed with 24-bit limbs whose partial sums are exact
 //      in FP64, reduced mod M, normalised, and evaluated in double-double; C = C' 2^(-s_i - t_j).
-// The only error is the rounding of A and B to P-bit integers in step 1 (relative to each row /
-// column maximum); all integer work is exact.
+// The only error is the rounding of A and B to integers in step 1 (at most 1/2 in units of
+// |row|_2 / X); all integer work is exact.
 //
 // Inputs must be finite: a row of A or column of B holding Inf/NaN yields NaN in the
 // corresponding outputs.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -112,71 +114,119 @@ static nrk_status make_consts(int n, OzConsts *c) {
 }
 
 // ---------------- 1. per-row / per-column power-of-two scales ----------------
-// scale[i] = P - 1 - floor(log2(max_k |X[i][k]|)) so that every scaled entry is < 2^P.
-__global__ __launch_bounds__(256) void row_scale_kernel(const double *__restrict__ X, int R, int Cc, int P,
+// The scales follow the Cauchy-Schwarz rule of Ozaki-II's fast mode: |C'_ij| <= |A'_i|_2 |B'_j|_2,
+// so scaling every row of A and column of B to a 2-norm of at most X, X^2 < M/2, keeps the
+// integer product inside the CRT range whatever k is. For flat rows (all |a_ik| equal) this is
+// the same precision as bounding every entry by X / sqrt(k); for every other row it is more.
+//
+// The 2-norm is accumulated as ssq 2^(2e) with the squares taken of |x| 2^-e, e the exponent of
+// the largest |x| seen so far, so no finite input overflows or underflows it.
+struct SsqAcc {
+    double s;  // sum of (|x| 2^-e)^2
+    int e;
+};
+constexpr int OZ_NO_EXP = -4000;  // below every double exponent: an empty accumulator
+__device__ __forceinline__ void ssq_add(SsqAcc &a, double x) {
+    const double ax = fabs(x);
+    if (!(ax > 0.0) || ax == __builtin_inf()) return;  // zero or non-finite (recorded separately)
+    int ex;
+    (void)frexp(ax, &ex);  // ax = f 2^ex, f in [0.5, 1)
+    if (ex > a.e) {
+        a.s = ldexp(a.s, 2 * (a.e - ex));
+        a.e = ex;
+    }
+    const double y = ldexp(ax, -a.e);
+    a.s = fma(y, y, a.s);
+}
+__device__ __forceinline__ SsqAcc ssq_merge(SsqAcc a, SsqAcc b) {
+    if (b.e > a.e) {
+        const SsqAcc t = a;
+        a = b;
+        b = t;
+    }
+    a.s += ldexp(b.s, 2 * (b.e - a.e));
+    return a;
+}
+// The largest power of two 2^s with 2^s |x|_2 <= xeff (the 2^-30 covers the rounding of the sum
+// of squares, sqrt and the division); 0 for a zero vector.
+__device__ __forceinline__ int cs_scale(SsqAcc t, double xeff) {
+    if (!(t.s > 0.0)) return 0;
+    const double r = xeff / (sqrt(t.s) * (1.0 + 0x1p-30));
+    int e;
+    (void)frexp(r, &e);  // r in [2^(e-1), 2^e)
+    return e - 1 - t.e;
+}
+
+// scale[r] for row r of X (R x Cc, row-major): one workgroup per row.
+__global__ __launch_bounds__(256) void row_scale_kernel(const double *__restrict__ X, int R, int Cc, double xeff,
                                                         int *__restrict__ scale) {
     const int r = blockIdx.x;
-    double mx = 0.0;
+    SsqAcc acc{0.0, OZ_NO_EXP};
     int bad = 0;
     for (int c = threadIdx.x; c < Cc; c += 256) {
         const double v = X[(size_t)r * Cc + c];
         bad |= !__builtin_isfinite(v);
-        mx = fmax(mx, fabs(v));
+        ssq_add(acc, v);
     }
-    __shared__ double smx[256];
-    __shared__ int sbad[256];
-    smx[threadIdx.x] = mx;
+    __shared__ double ss[256];
+    __shared__ int se[256], sbad[256];
+    ss[threadIdx.x] = acc.s;
+    se[threadIdx.x] = acc.e;
     sbad[threadIdx.x] = bad;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
-            smx[threadIdx.x] = fmax(smx[threadIdx.x], smx[threadIdx.x + s]);
-            sbad[threadIdx.x] |= sbad[threadIdx.x + s];
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) {
+            const SsqAcc m = ssq_merge(SsqAcc{ss[threadIdx.x], se[threadIdx.x]}, SsqAcc{ss[threadIdx.x + h], se[threadIdx.x + h]});
+            ss[threadIdx.x] = m.s;
+            se[threadIdx.x] = m.e;
+            sbad[threadIdx.x] |= sbad[threadIdx.x + h];
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        int e = 0;
-        if (smx[0] > 0.0) (void)frexp(smx[0], &e);  // smx = f * 2^e, f in [0.5, 1)
-        scale[r] = sbad[0] ? OZ_BAD_SCALE : (smx[0] > 0.0 ? P - e : 0);
-    }
+    if (threadIdx.x == 0) scale[r] = sbad[0] ? OZ_BAD_SCALE : cs_scale(SsqAcc{ss[0], se[0]}, xeff);
 }
 
-// Column maxima of X (R x Cc, row-major) in two passes: chunks of COL_ROWS rows per thread
+// Column 2-norms of X (R x Cc, row-major) in two passes: chunks of COL_ROWS rows per thread
 // (threads of a workgroup read consecutive columns, so every row read is coalesced), then one
-// thread per column over the chunk maxima. A non-finite entry is recorded as +inf.
+// thread per column over the chunk accumulators. part holds the chunks' sums of squares (+inf
+// for a chunk with a non-finite entry) followed by their exponents.
 constexpr int COL_ROWS = 128;
-__global__ __launch_bounds__(256) void col_max_partial_kernel(const double *__restrict__ X, int R, int Cc,
-                                                              double *__restrict__ pmax) {
+__global__ __launch_bounds__(256) void col_ssq_partial_kernel(const double *__restrict__ X, int R, int Cc,
+                                                              double *__restrict__ part) {
     const int c = blockIdx.x * 256 + threadIdx.x;
     if (c >= Cc) return;
     const int r0 = blockIdx.y * COL_ROWS, r1 = min(R, r0 + COL_ROWS);
-    double mx = 0.0;
+    SsqAcc acc{0.0, OZ_NO_EXP};
     int bad = 0;
     for (int r = r0; r < r1; ++r) {
         const double v = X[(size_t)r * Cc + c];
         bad |= !__builtin_isfinite(v);
-        mx = fmax(mx, fabs(v));
+        ssq_add(acc, v);
     }
-    pmax[(size_t)blockIdx.y * Cc + c] = bad ? __builtin_inf() : mx;
+    part[(size_t)blockIdx.y * Cc + c] = bad ? __builtin_inf() : acc.s;
+    part[(size_t)(gridDim.y + blockIdx.y) * Cc + c] = (double)acc.e;
 }
 
-__global__ __launch_bounds__(256) void col_scale_kernel(const double *__restrict__ pmax, int nchunk, int Cc, int P,
+__global__ __launch_bounds__(256) void col_scale_kernel(const double *__restrict__ part, int nchunk, int Cc, double xeff,
                                                         int *__restrict__ scale) {
     const int c = blockIdx.x * 256 + threadIdx.x;
     if (c >= Cc) return;
-    double mx = 0.0;
-    for (int h = 0; h < nchunk; ++h) mx = fmax(mx, pmax[(size_t)h * Cc + c]);
-    int e = 0;
-    if (mx > 0.0 && __builtin_isfinite(mx)) (void)frexp(mx, &e);
-    scale[c] = __builtin_isinf(mx) ? OZ_BAD_SCALE : (mx > 0.0 ? P - e : 0);
+    SsqAcc acc{0.0, OZ_NO_EXP};
+    bool bad = false;
+    for (int h = 0; h < nchunk; ++h) {
+        const double s = part[(size_t)h * Cc + c];
+        bad |= __builtin_isinf(s);
+        acc = ssq_merge(acc, SsqAcc{bad ? 0.0 : s, (int)part[(size_t)(nchunk + h) * Cc + c]});
+    }
+    scale[c] = bad ? OZ_BAD_SCALE : cs_scale(acc, xeff);
 }
 
 // ---------------- 2. residues ----------------
-// Symmetric residue of the integer v = vh 2^26 + vl (|vh| < 2^27, 0 <= vl < 2^26) modulo m:
-// t = vh (2^26 mod m) + vl is congruent to v and below 2^35 in magnitude, so t / m is within far
-// less than 1/(2m) of its FP64 value and rint picks the nearest integer quotient (t / m is never
-// a half-integer, m being odd): r = t - m rint(t / m) lies in [-(m-1)/2, (m-1)/2], exactly.
+// Symmetric residue of the integer v = vh 2^26 + vl (|v| <= 2^62, so |vh| <= 2^36, and
+// 0 <= vl < 2^26) modulo m: t = vh (2^26 mod m) + vl is congruent to v and below 2^45 in
+// magnitude (an exact double), and the FP64 t / m is within 2^-13 of the true quotient while t / m
+// is at least 1/(2m) >= 2^-9 from every half-integer (m odd), so rint picks the nearest integer
+// quotient: r = t - m rint(t / m) lies in [-(m-1)/2, (m-1)/2], exactly.
 __device__ __forceinline__ unsigned sym_mod_byte(double vh, double vl, double m, double inv_m, double c26) {
     const double t = fma(vh, c26, vl);  // exact
     const double r = fma(-rint(t * inv_m), m, t);
@@ -407,31 +457,52 @@ __global__ __launch_bounds__(256) void crt_kernel(const signed char *__restrict_
     const int col = (int)(blk % Nb) * 16 + (l & 15);
     const int row0 = (int)(blk / Nb) * 16 + 4 * (l >> 4);
     if (row0 >= M || col >= N) return;
+    // Lx[u][t]: limb t of X_raw = sum_j c_j w_j for output row row0 + u. Every modulus' word is
+    // requested up front (fully unrolled over the maximum count, uniform branches past k.n), then
+    // the moduli are accumulated in turn, each one's weights read once for the four outputs.
+    const unsigned *src = reinterpret_cast<const unsigned *>(Cres) + w;
+    const size_t wstride = c_stride / 4;
     unsigned words[OZ_MAX_MODULI];
-    for (int j = 0; j < k.n; ++j) words[j] = reinterpret_cast<const unsigned *>(Cres + j * c_stride)[w];
+#pragma unroll
+    for (int j = 0; j < OZ_MAX_MODULI; ++j) words[j] = j < k.n ? src[j * wstride] : 0u;
+    double Lx[4][NL];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int t = 0; t < NL; ++t) Lx[u][t] = 0.0;
+#pragma unroll
+    for (int j = 0; j < OZ_MAX_MODULI; ++j) {
+        if (j >= k.n) break;
+        double wj[NL];
+#pragma unroll
+        for (int t = 0; t < NL; ++t) wj[t] = k.w[j][t];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const double c = (double)(signed char)(words[j] >> (8 * u));
+#pragma unroll
+            for (int t = 0; t < NL; ++t) Lx[u][t] = fma(c, wj[t], Lx[u][t]);  // exact: |sum| < 2^36
+        }
+    }
+    const int sc = scol[col];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
         const int row = row0 + u;
         if (row >= M) break;
         const size_t idx = (size_t)row * N + col;
-        const int sr = srow[row], sc = scol[col];
+        const int sr = srow[row];
         if (sr == OZ_BAD_SCALE || sc == OZ_BAD_SCALE) {
             C[idx] = __builtin_nan("");
             continue;
         }
         double L[NL];
 #pragma unroll
-        for (int t = 0; t < NL; ++t) L[t] = 0.0;
-        for (int j = 0; j < k.n; ++j) {
-            const double c = (double)(signed char)(words[j] >> (8 * u));
-#pragma unroll
-            for (int t = 0; t < NL; ++t) L[t] = fma(c, k.w[j][t], L[t]);  // exact: |sum| < 2^36
-        }
+        for (int t = 0; t < NL; ++t) L[t] = Lx[u][t];
         // X_raw = sum_t L[t] 2^(24 t); subtract q M with q = round(X_raw / M)
         double approx = 0.0;
 #pragma unroll
         for (int t = NL - 1; t >= 0; --t) approx = approx * 16777216.0 + L[t];
-        // |C'| < M/4 by the choice of P, so q only has to be within 1/4 of X_raw / M
+        // |C'| <= (M/2)(1 - 2^-20) by the choice of the scales, and |X_raw| < 2^12 M, so the
+        // FP64 quotient approx / M is within 2^-36 of X_raw / M and rounds to the right q
         const double q = rint(approx * k.inv_M);
 #pragma unroll
         for (int t = 0; t < NL; ++t) L[t] = fma(-q, k.Mlimb[t], L[t]);  // exact
@@ -492,32 +563,34 @@ nrk_status gemm_f64_ozaki(size_t m, size_t k, size_t n, const double *a, const d
     if (!a || !b || !c) return fail(NRK_ERR_INVALID_ARGUMENT, "A, B or C is NULL");
     if (k > 130000) return fail(NRK_ERR_INVALID_ARGUMENT, "k = %zu exceeds the exact int32 accumulation bound", k);
     if (m > (1u << 28) || n > (1u << 28)) return fail(NRK_ERR_INVALID_ARGUMENT, "dimension too large");
-    // largest P with k * 2^(2P) < M / 2, capped at 53 (scaled entries stay exact doubles)
-    const double log2M = std::log2(kc.M_approx);
-    int P = (int)std::floor((log2M - 2.0 - std::log2((double)k)) / 2.0);
-    if (P > 53) P = 53;
-    if (P < 8) return fail(NRK_ERR_INVALID_ARGUMENT, "too few moduli (%d) for k = %zu", moduli, k);
+    // 2-norm bound of the scaled rows / columns: X^2 <= (M/2)(1 - 2^-20), capped at 2^62 (the
+    // residue split), less the rounding of k entries (|round(v) - v|_2 <= sqrt(k) / 2) and a unit
+    const double X = std::min(std::sqrt(kc.M_approx * 0.5 * (1.0 - 0x1p-20)), 0x1p62);
+    const double xeff = X - 0.5 * std::sqrt((double)k) - 1.0;
+    // worst case (all entries of a row equal in magnitude) each entry keeps log2(xeff / sqrt(k)) bits
+    if (std::log2(xeff) - 0.5 * std::log2((double)k) < 8.0)
+        return fail(NRK_ERR_INVALID_ARGUMENT, "too few moduli (%d) for k = %zu", moduli, k);
     const size_t Mp = align_up(m, 256), Np = align_up(n, 256), Kp = align_up(k, 128);
     const size_t sa = Mp * Kp, sb = Np * Kp, sc = ceil_div(m, 16) * ceil_div(n, 16) * 256;  // residues in 16 x 16 blocks
     const size_t nchunk = ceil_div(k, (size_t)COL_ROWS);
     const size_t res_bytes = align_up((size_t)moduli * (sa + sb + sc), 256);
-    const size_t bytes = res_bytes + align_up(nchunk * n * sizeof(double), 256) + (m + n) * sizeof(int);
+    const size_t bytes = res_bytes + align_up(2 * nchunk * n * sizeof(double), 256) + (m + n) * sizeof(int);
     void *ws = nullptr;
     NRK_TRY(get_workspace(stream, bytes, &ws));
     signed char *Ares = (signed char *)ws;
     signed char *Bres = Ares + (size_t)moduli * sa;
     signed char *Cres = Bres + (size_t)moduli * sb;
-    double *pmax = (double *)((char *)ws + res_bytes);
-    int *srow = (int *)((char *)pmax + align_up(nchunk * n * sizeof(double), 256));
+    double *part = (double *)((char *)ws + res_bytes);
+    int *srow = (int *)((char *)part + align_up(2 * nchunk * n * sizeof(double), 256));
     int *scol = srow + m;
 
-    hipLaunchKernelGGL(row_scale_kernel, dim3((unsigned)m), dim3(256), 0, stream, a, (int)m, (int)k, P, srow);
+    hipLaunchKernelGGL(row_scale_kernel, dim3((unsigned)m), dim3(256), 0, stream, a, (int)m, (int)k, xeff, srow);
     NRK_HIP(hipGetLastError());
-    hipLaunchKernelGGL(col_max_partial_kernel, dim3((unsigned)ceil_div(n, 256), (unsigned)nchunk), dim3(256), 0, stream, b,
-                       (int)k, (int)n, pmax);
+    hipLaunchKernelGGL(col_ssq_partial_kernel, dim3((unsigned)ceil_div(n, 256), (unsigned)nchunk), dim3(256), 0, stream, b,
+                       (int)k, (int)n, part);
     NRK_HIP(hipGetLastError());
-    hipLaunchKernelGGL(col_scale_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, stream, pmax, (int)nchunk, (int)n, P,
-                       scol);
+    hipLaunchKernelGGL(col_scale_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, stream, part, (int)nchunk, (int)n,
+                       xeff, scol);
     NRK_HIP(hipGetLastError());
     const size_t nkc = Kp / 16;
     hipLaunchKernelGGL((residue_kernel<false>), dim3((unsigned)ceil_div(Mp * nkc, 256)), dim3(256), 0, stream, a, (int)m,

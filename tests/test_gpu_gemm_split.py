@@ -7,6 +7,8 @@ Error measure: e_ij = |C_ij - C*_ij| / sum_k |A_ik||B_kj|, the scale GEMM error 
   bf16 inputs:   exact products of the rounded inputs, FP32 accumulation only
   Ozaki-II (f64, 16 moduli): max e <= 2 * max e(native FP64 MFMA GEMM) + 2^-52
 Inputs follow the split-precision workload: s * 2^e * u, e uniform in [-10, 10], u in [1, 2)."""
+import math
+
 import numpy as np
 import pytest
 
@@ -90,29 +92,72 @@ def test_ozaki_integer_inputs_exact(gpu):
     np.testing.assert_array_equal(K.gemm_f64_ozaki(a, b), a @ b)
 
 
+MODULI = [255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197, 193, 191, 181, 179, 173, 167]
+
+
+def ozaki_row_bound(nm, k):
+    """The 2-norm every scaled row of A / column of B is held to with N moduli: X^2 <= (M/2)(1 - 2^-20)
+    (X <= 2^62), less sqrt(k)/2 for the rounding of k entries and a unit. Cauchy-Schwarz then keeps
+    |C'_ij| <= |A'_i| |B'_j| inside the CRT range (-M/2, M/2) for every k."""
+    M = math.prod(MODULI[:nm])
+    return min(math.sqrt(M / 2 * (1 - 2.0 ** -20)), 2.0 ** 62) - 0.5 * math.sqrt(k) - 1
+
+
 def test_ozaki_error_within_truncation_bound(gpu, oracle):
-    """The only error is rounding A's rows / B's columns to P-bit integers: |dA_ik| <= 2^-P
-    rowmax_i and |dB_kj| <= 2^-P colmax_j, so |C - C*|_ij <= 2^-P (rowmax_i sum_k |B_kj| +
-    colmax_j sum_k |A_ik|) + O(2^-2P). P from the library's rule: the largest value with
-    k 2^(2P) < M/2 for M the product of the first N moduli (14 moduli, k = 256: P = 49)."""
-    import math
-    moduli = [255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197, 193, 191]
+    """The only error is rounding A's rows / B's columns to integers after scaling by the largest
+    power of two 2^s with 2^s |A_i|_2 <= X: |dA_ik| <= 2^-s / 2 <= |A_i|_2 / X, likewise for B, so
+    |C - C*|_ij <= (|A_i|_2 sum_k |B_kj| + |B_j|_2 sum_k |A_ik|) / X + k |A_i|_2 |B_j|_2 / X^2 (+ the
+    final rounding). That bound is never above the per-entry rule's (every scaled entry below 2^P,
+    P the largest with k 2^(2P) < M/4), which the test checks too."""
     m = k = n = 256
     a = rng.scaled_pow2(5, m * k).reshape(m, k)
     b = rng.scaled_pow2(6, k * n).reshape(k, n)
     exact, _ = full_refs(oracle, a, b)
+    ra = np.sqrt((a * a).sum(axis=1))[:, None]
+    cb = np.sqrt((b * b).sum(axis=0))[None, :]
+    sa = np.abs(a).sum(axis=1)[:, None]
+    sb = np.abs(b).sum(axis=0)[None, :]
     rowmax = np.abs(a).max(axis=1)[:, None]
     colmax = np.abs(b).max(axis=0)[None, :]
-    bound_unit = rowmax * np.abs(b).sum(axis=0)[None, :] + colmax * np.abs(a).sum(axis=1)[:, None]
     errs = {}
-    for nm in (14, 16):
-        M = math.prod(moduli[:nm])
-        P = min(53, int(math.floor((math.log2(M) - 2 - math.log2(k)) / 2)))
+    for nm in (8, 14, 16):
+        X = ozaki_row_bound(nm, k)
         err = np.abs(K.gemm_f64_ozaki(a, b, nm) - exact)
-        # + 1 ulp of |exact| for the final rounding of the exact integer product
-        assert np.all(err <= 2.0 ** -P * bound_unit * 1.001 + np.abs(exact) * 2.0 ** -52), nm
+        ulp = np.abs(exact) * 2.0 ** -52
+        bound = (ra * sb + cb * sa) / X + k * ra * cb / X ** 2
+        assert np.all(err <= bound * 1.001 + ulp), nm
+        P = min(53, math.floor((math.log2(math.prod(MODULI[:nm])) - 2 - math.log2(k)) / 2))
+        assert np.all(err <= 2.0 ** -P * (rowmax * sb + colmax * sa) * 1.001 + ulp), nm
         errs[nm] = err.max()
-    assert errs[16] <= errs[14]
+    assert errs[16] <= errs[14] <= errs[8]
+
+
+@pytest.mark.parametrize("nm", [8, 16, 20])
+def test_ozaki_products_at_the_edge_of_the_crt_range(gpu, nm):
+    """A row of A parallel to a column of B makes Cauchy-Schwarz tight: C'_ii = |A'_i|_2^2 for
+    B = A^T, anywhere in (X^2 / 4, X^2] by where |A_i|_2 falls between powers of two. With
+    integer entries the scaling is exact, so the product must come back exact. One row is picked
+    to land within 2 % of the top of the CRT range; with 64 rows spread over the octave, a range
+    rule too loose by even sqrt(2) would push some products past M/2 and wrap them. With 16 and
+    20 moduli (20: the 2^62 cap on X) the scaled entries reach ~2^60, past the 2^53 of an FP64
+    significand, which exercises the residue split of large integers."""
+    k = 64
+    X = ozaki_row_bound(nm, k)
+    r = np.random.default_rng(21 + nm)
+    rows = [r.integers(-1000, 1001, k).astype(np.float64) for _ in range(63)]
+    for _ in range(20000):
+        v = r.integers(-1000, 1001, k).astype(np.float64)
+        nrm = math.sqrt(float(v @ v))
+        s = math.floor(math.log2(X / nrm))
+        if 0.99 * X < nrm * 2.0 ** s < (1 - 1e-6) * X:
+            break
+    else:
+        pytest.fail("no integer vector with a scaled norm within 1% of the bound")
+    if nm <= 16:  # below the 2^62 cap the bound is the CRT range itself
+        assert (nrm * 2.0 ** s) ** 2 > 0.98 * math.prod(MODULI[:nm]) / 2
+    a = np.stack(rows + [v])
+    b = a.T.copy()
+    np.testing.assert_array_equal(K.gemm_f64_ozaki(a, b, nm), a @ b)
 
 
 def test_ozaki_nonfinite_rows_are_nan(gpu):
