@@ -57,6 +57,7 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-gemm", action="store_true")
     p.add_argument("--no-ragged", action="store_true")
+    p.add_argument("--no-batched", action="store_true")
     p.add_argument("--pmc-pass", action="store_true",
                    help="run each hot call once after a plain copy of known size (2 GiB, the counter "
                         "calibration), for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE; prints no JSON line")
@@ -215,6 +216,7 @@ def main() -> int:
 
     gemm = None if args.no_gemm else bench_gemm(device, rng, call, s, stream)
     ragged = None if args.no_ragged else bench_ragged(device, rng, call, s, stream)
+    batched = None if args.no_batched else bench_batched(device, rng, call, s, stream)
     cpu = None if args.no_cpu_baseline else cpu_baseline(x32, d64, xi, step_bytes)
 
     line = {
@@ -241,6 +243,7 @@ def main() -> int:
         "host_pointer_path": host_path,
         "gemm": gemm,
         "ragged": ragged,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
@@ -325,6 +328,55 @@ def bench_gemm(device, rng, call, s, stream):
         del dA, dB, dC, A, B
     out["error_measure"] = ("max over 24 sampled entries of |C - C*| / sum_k |A_ik||B_kj|, C* = correctly rounded "
                             "dot product (Dekker products + fsum); inputs sign*2^e*u, e in [-10,10], u in [1,2)")
+    return out
+
+
+# ---------------------------------------------------------------------------------------------
+# uniform batches (batched_gemm / matmul batch axes): 64 panels of the 512^3 plain-GEMM shape
+
+def bench_batched(device, rng, call, s, stream):
+    t0 = time.time()
+    batch, n = 64, 512
+    flop = 2.0 * batch * n ** 3
+    out = {"workload": f"{batch} panels of {n}x{n}x{n}, U[-1,1], contiguous [batch, m, k] x [batch, k, n]"}
+    for dt, peak, tag in ((np.float64, 78.6, "f64"), (np.float32, 157.3, "f32")):
+        A = rng.uniform(rng.BASE_SEED + 60, batch * n * n).astype(dt)
+        B = rng.uniform(rng.BASE_SEED + 61, batch * n * n).astype(dt)
+        dA, dB = device.DeviceArray.upload(A), device.DeviceArray.upload(B)
+        dC = device.DeviceArray((batch * n * n,), dt)
+        isz = np.dtype(dt).itemsize
+        fb, f1 = f"nrk_dev_batched_gemm_{tag}", f"nrk_dev_gemm_{tag}"
+        pa, pb, pc = C.c_void_p(dA.ptr), C.c_void_p(dB.ptr), C.c_void_p(dC.ptr)
+
+        def timeit(f, reps=5):
+            f()
+            stream.synchronize()
+            a, b = device.Event(), device.Event()
+            a.record(s)
+            for _ in range(reps):
+                f()
+            b.record(s)
+            stream.synchronize()
+            return a.elapsed_ms(b) / reps
+
+        one = timeit(lambda: call(fb, batch, n, n, n, pa, n * n, pb, n * n, pc, C.c_void_p(s)))
+        bcast = timeit(lambda: call(fb, batch, n, n, n, pa, n * n, pb, 0, pc, C.c_void_p(s)))
+
+        def loop():
+            for t in range(batch):
+                call(f1, n, n, n, C.c_void_p(dA.ptr + t * n * n * isz), C.c_void_p(dB.ptr + t * n * n * isz),
+                     C.c_void_p(dC.ptr + t * n * n * isz), C.c_void_p(s))
+        per_call = timeit(loop)
+        single = timeit(lambda: call(f1, n, n, n, pa, pb, pc, C.c_void_p(s)), 20)
+        out[tag] = {"batched_ms": round(one, 4), "TFLOP/s": round(flop / one / 1e9, 2),
+                    "frac_of_mfma_peak": round(flop / one / 1e9 / peak, 3), "peak_TFLOP/s": peak,
+                    "broadcast_b_ms": round(bcast, 4), "per_panel_loop_ms": round(per_call, 4),
+                    "speedup_vs_loop": round(per_call / one, 2),
+                    "single_512_ms": round(single, 4), "single_512_TFLOP/s": round(2.0 * n ** 3 / single / 1e9, 2)}
+        log(f"batched {tag}: one launch {one:.3f} ms ({flop / one / 1e9:.1f} TFLOP/s), B broadcast {bcast:.3f} ms, "
+            f"per-panel loop {per_call:.3f} ms, single 512^3 {single * 1e3:.1f} us")
+        del dA, dB, dC
+    out["seconds"] = round(time.time() - t0, 1)
     return out
 
 

@@ -250,6 +250,23 @@ nrk_status nrk_gemm_2d_f32(size_t m, size_t k, size_t n, const float *a, size_t 
 nrk_status nrk_dev_gemm_f64(size_t m, size_t k, size_t n, const double *a, const double *b, double *c, nrk_stream stream);
 nrk_status nrk_dev_gemm_f32(size_t m, size_t k, size_t n, const float *a, const float *b, float *c, nrk_stream stream);
 
+/* array/linalg.rs:56-136 batched_gemm<T>(batch, m, k, n, a, b, c) for T = f64 / f32 (the
+ * par_batch_f64 / par_batch_f32 tier): C[t] := A[t] * B[t] over contiguous panels
+ * ([batch, m, k] x [batch, k, n] -> [batch, m, n]). batch, m or n == 0, or slice lengths other
+ * than batch*m*k, batch*k*n, batch*m*n, return without writing C (linalg.rs:60-73); otherwise
+ * panel t is exactly gemm_2d on that panel (k == 0 zero-fills it), bit for bit. */
+nrk_status nrk_batched_gemm_f64(size_t batch, size_t m, size_t k, size_t n, const double *a, size_t a_len, const double *b,
+                                size_t b_len, double *c, size_t c_len);
+nrk_status nrk_batched_gemm_f32(size_t batch, size_t m, size_t k, size_t n, const float *a, size_t a_len, const float *b,
+                                size_t b_len, float *c, size_t c_len);
+/* Device forms with panel strides in elements: A[t] = a + t*stride_a, B[t] = b + t*stride_b,
+ * C[t] = c + t*m*n. A stride of 0 gives every panel the same operand: Array::matmul's batch
+ * broadcast (array/linalg.rs:212-225) without materialising the broadcast_to copy. */
+nrk_status nrk_dev_batched_gemm_f64(size_t batch, size_t m, size_t k, size_t n, const double *a, size_t stride_a,
+                                    const double *b, size_t stride_b, double *c, nrk_stream stream);
+nrk_status nrk_dev_batched_gemm_f32(size_t batch, size_t m, size_t k, size_t n, const float *a, size_t stride_a,
+                                    const float *b, size_t stride_b, float *c, nrk_stream stream);
+
 /* Split-precision GEMMs. No reference counterpart: alternative engines for gemm_2d's f32 / f64
  * tier (kernels/gemm.rs:290-312) with the same C := A*B overwrite contract.
  *
@@ -261,8 +278,9 @@ nrk_status nrk_dev_gemm_f32_bf16x(size_t m, size_t k, size_t n, const float *a, 
                                   nrk_stream stream);
 /* FP64 by the Ozaki scheme II on v_mfma_i32_16x16x64_i8: `moduli` int8 residue products
  * (2..20; 0 selects 16) with exact INT32 accumulation and an exact CRT recombination. The only
- * rounding is of A's rows and B's columns to P-bit integers (P <= 53, the largest value with
- * k * 2^(2P) < prod(moduli) / 2). k <= 130000. A row/column holding Inf/NaN gives NaN outputs. */
+ * rounding is of A's rows and B's columns to integers after scaling each by a power of two to a
+ * 2-norm of at most X, X^2 < prod(moduli) / 2 (Cauchy-Schwarz keeps every product entry in the
+ * CRT range; X <= 2^62). k <= 130000. A row/column holding Inf/NaN gives NaN outputs. */
 nrk_status nrk_gemm_f64_ozaki(size_t m, size_t k, size_t n, const double *a, const double *b, double *c, int32_t moduli);
 nrk_status nrk_dev_gemm_f64_ozaki(size_t m, size_t k, size_t n, const double *a, const double *b, double *c, int32_t moduli,
                                   nrk_stream stream);

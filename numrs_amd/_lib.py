@@ -98,6 +98,10 @@ _SIGS: dict[str, list] = {
     "nrk_gemm_2d_f32": [c_size, c_size, c_size, vp, c_size, vp, c_size, vp, c_size],
     "nrk_dev_gemm_f64": [c_size, c_size, c_size, vp, vp, vp, vp],
     "nrk_dev_gemm_f32": [c_size, c_size, c_size, vp, vp, vp, vp],
+    "nrk_batched_gemm_f64": [c_size, c_size, c_size, c_size, vp, c_size, vp, c_size, vp, c_size],
+    "nrk_batched_gemm_f32": [c_size, c_size, c_size, c_size, vp, c_size, vp, c_size, vp, c_size],
+    "nrk_dev_batched_gemm_f64": [c_size, c_size, c_size, c_size, vp, c_size, vp, c_size, vp, vp],
+    "nrk_dev_batched_gemm_f32": [c_size, c_size, c_size, c_size, vp, c_size, vp, c_size, vp, vp],
     "nrk_gemm_f32_bf16x": [c_size, c_size, c_size, vp, vp, vp, c_i32],
     "nrk_dev_gemm_f32_bf16x": [c_size, c_size, c_size, vp, vp, vp, c_i32, vp],
     "nrk_gemm_f64_ozaki": [c_size, c_size, c_size, vp, vp, vp, c_i32],

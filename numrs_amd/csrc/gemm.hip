@@ -14,6 +14,12 @@
 // neighbours in the tile grid (each XCD has its own L2), then grouped GROUP_M tile-rows at a
 // time for A/B reuse.
 //
+// Batches (array/linalg.rs:56-136, batched_gemm / par_batch_f64/f32): blockIdx.y is the panel
+// t, whose operands start at A + t sa, B + t sb and C + t sc (element strides). A stride of 0
+// repeats one panel for every t, which is matmul's batch broadcast (linalg.rs:212-225) without
+// the broadcast_to copy. Every panel runs the same code as a single gemm_2d, so panel t of a
+// batch is bit-identical to gemm_2d on that panel.
+//
 // MFMA operand layout (16x16x4, f64 and f32): lane l supplies A[l%16][l/16] and B[l/16][l%16];
 // accumulator element r of lane l is D[4*(l/16) + r][l%16] for the f32 instruction and
 // D[l/16 + 4*r][l%16] for the f64 one (Mfma<T>::row below).
@@ -63,7 +69,7 @@ __device__ __forceinline__ void tile_of(int nbm, int nbn, int group_m, int &tm, 
 template <typename T, int BM, int BN, int BK, int WM, int WN>
 __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_kernel(int M, int N, int K, const T *__restrict__ A,
                                                                          const T *__restrict__ B, T *__restrict__ C,
-                                                                         int group_m) {
+                                                                         int group_m, size_t sa, size_t sb, size_t sc) {
     constexpr int WGN = BN / WN;
     constexpr int NT = (BM / WM) * WGN * 64;
     constexpr int TM = WM / 16, TN = WN / 16;
@@ -76,6 +82,9 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_kernel(int M,
 
     __shared__ T As[2][BK][BM + PAD];
     __shared__ T Bs[2][BK][BN + PAD];
+    A += blockIdx.y * sa;
+    B += blockIdx.y * sb;
+    C += blockIdx.y * sc;
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int wm = wave / WGN, wn = wave % WGN;
@@ -160,14 +169,19 @@ __global__ void zero_fill_kernel(T *c, size_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) c[i] = (T)0;
 }
 
+// `batch` panels (grid y), at most 65535 per launch: larger batches go out in several launches.
 template <typename T, int BM, int BN, int BK, int WM, int WN>
-static nrk_status launch_gemm(int m, int n, int k, const T *a, const T *b, T *c, hipStream_t stream) {
+static nrk_status launch_gemm(int m, int n, int k, const T *a, const T *b, T *c, hipStream_t stream, size_t batch = 1,
+                              size_t sa = 0, size_t sb = 0, size_t sc = 0) {
     const int nbm = (m + BM - 1) / BM, nbn = (n + BN - 1) / BN;
     const int threads = (BM / WM) * (BN / WN) * 64;
     const int group_m = 8;
-    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WM, WN>), dim3(nbm * nbn), dim3(threads), 0, stream, m, n, k, a, b, c,
-                       group_m);
-    NRK_HIP(hipGetLastError());
+    for (size_t t0 = 0; t0 < batch; t0 += 65535) {
+        const unsigned nb = (unsigned)std::min<size_t>(65535, batch - t0);
+        hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WM, WN>), dim3(nbm * nbn, nb), dim3(threads), 0, stream, m, n, k,
+                           a + t0 * sa, b + t0 * sb, c + t0 * sc, group_m, sa, sb, sc);
+        NRK_HIP(hipGetLastError());
+    }
     return NRK_OK;
 }
 
@@ -195,6 +209,25 @@ nrk_status gemm(size_t m, size_t k, size_t n, const T *a, const T *b, T *c, hipS
 
 template nrk_status gemm<double>(size_t, size_t, size_t, const double *, const double *, double *, hipStream_t);
 template nrk_status gemm<float>(size_t, size_t, size_t, const float *, const float *, float *, hipStream_t);
+
+// C[t] := A[t] * B[t] for t < batch; C panels are contiguous (sc = m n), A and B panels sit
+// `sa` / `sb` elements apart (0: one panel shared by every t). The tile is chosen as for one
+// gemm_2d but counting the tiles of the whole batch; it does not change any result bit.
+template <typename T>
+nrk_status batched_gemm(size_t batch, size_t m, size_t k, size_t n, const T *a, size_t sa, const T *b, size_t sb, T *c,
+                        hipStream_t stream) {
+    NRK_TRY(ensure_device());
+    if (batch == 0 || m == 0 || n == 0) return NRK_OK;
+    if (!c) return fail(NRK_ERR_INVALID_ARGUMENT, "C is NULL");
+    if (k == 0) return gemm<T>(batch * m, 0, n, a, b, c, stream);  // every panel zero-filled
+    if (!a || !b) return fail(NRK_ERR_INVALID_ARGUMENT, "A or B is NULL");
+    if (m > (size_t)INT32_MAX / 2 || n > (size_t)INT32_MAX / 2 || k > (size_t)INT32_MAX / 2)
+        return fail(NRK_ERR_INVALID_ARGUMENT, "dimension exceeds 2^30");
+    const size_t sc = m * n;
+    if (batch * ceil_div(m, 128) * ceil_div(n, 128) >= (size_t)cu_count())
+        return launch_gemm<T, 128, 128, 16, 64, 64>((int)m, (int)n, (int)k, a, b, c, stream, batch, sa, sb, sc);
+    return launch_gemm<T, 64, 64, 16, 32, 32>((int)m, (int)n, (int)k, a, b, c, stream, batch, sa, sb, sc);
+}
 
 // Tile configurations the tuner times and the selector chooses from (all 4 wavefronts).
 template <typename T>
@@ -243,6 +276,30 @@ nrk_status host_gemm(size_t m, size_t k, size_t n, const T *a, size_t a_len, con
     NRK_HIP(hipStreamSynchronize(s));
     return NRK_OK;
 }
+
+// array/linalg.rs:56-136 batched_gemm on host slices: batch, m or n = 0, or slice lengths other
+// than batch m k / batch k n / batch m n, return without writing C (linalg.rs:60-73); otherwise
+// every panel is gemm_2d's (so k = 0 zero-fills C).
+template <typename T>
+nrk_status host_batched_gemm(size_t batch, size_t m, size_t k, size_t n, const T *a, size_t a_len, const T *b, size_t b_len,
+                             T *c, size_t c_len) {
+    NRK_TRY(ensure_device());
+    if (batch == 0 || m == 0 || n == 0) return NRK_OK;
+    if (a_len != batch * m * k || b_len != batch * k * n || c_len != batch * m * n) return NRK_OK;
+    if (!c) return fail(NRK_ERR_INVALID_ARGUMENT, "c is NULL");
+    if (k > 0 && (!a || !b)) return fail(NRK_ERR_INVALID_ARGUMENT, "a or b is NULL");
+    void *da = nullptr, *db = nullptr, *dc = nullptr;
+    NRK_TRY(staging(0, a_len * sizeof(T) + 16, &da));
+    NRK_TRY(staging(2, b_len * sizeof(T) + 16, &db));
+    NRK_TRY(staging(1, c_len * sizeof(T) + 16, &dc));
+    hipStream_t s = nullptr;
+    if (a_len) NRK_HIP(hipMemcpyAsync(da, a, a_len * sizeof(T), hipMemcpyHostToDevice, s));
+    if (b_len) NRK_HIP(hipMemcpyAsync(db, b, b_len * sizeof(T), hipMemcpyHostToDevice, s));
+    NRK_TRY(batched_gemm<T>(batch, m, k, n, (const T *)da, m * k, (const T *)db, k * n, (T *)dc, s));
+    NRK_HIP(hipMemcpyAsync(c, dc, c_len * sizeof(T), hipMemcpyDeviceToHost, s));
+    NRK_HIP(hipStreamSynchronize(s));
+    return NRK_OK;
+}
 }  // namespace
 
 extern "C" {
@@ -260,6 +317,22 @@ nrk_status nrk_dev_gemm_f64(size_t m, size_t k, size_t n, const double *a, const
 }
 nrk_status nrk_dev_gemm_f32(size_t m, size_t k, size_t n, const float *a, const float *b, float *c, nrk_stream stream) {
     return gemm<float>(m, k, n, a, b, c, (hipStream_t)stream);
+}
+nrk_status nrk_batched_gemm_f64(size_t batch, size_t m, size_t k, size_t n, const double *a, size_t a_len, const double *b,
+                                size_t b_len, double *c, size_t c_len) {
+    return host_batched_gemm<double>(batch, m, k, n, a, a_len, b, b_len, c, c_len);
+}
+nrk_status nrk_batched_gemm_f32(size_t batch, size_t m, size_t k, size_t n, const float *a, size_t a_len, const float *b,
+                                size_t b_len, float *c, size_t c_len) {
+    return host_batched_gemm<float>(batch, m, k, n, a, a_len, b, b_len, c, c_len);
+}
+nrk_status nrk_dev_batched_gemm_f64(size_t batch, size_t m, size_t k, size_t n, const double *a, size_t stride_a,
+                                    const double *b, size_t stride_b, double *c, nrk_stream stream) {
+    return batched_gemm<double>(batch, m, k, n, a, stride_a, b, stride_b, c, (hipStream_t)stream);
+}
+nrk_status nrk_dev_batched_gemm_f32(size_t batch, size_t m, size_t k, size_t n, const float *a, size_t stride_a,
+                                    const float *b, size_t stride_b, float *c, nrk_stream stream) {
+    return batched_gemm<float>(batch, m, k, n, a, stride_a, b, stride_b, c, (hipStream_t)stream);
 }
 
 }  // extern "C"

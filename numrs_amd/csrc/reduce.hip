@@ -339,10 +339,14 @@ __device__ __forceinline__ void seg_body(const Chain &chain, T (&y)[VPT * Vec16<
     if ((FLAGS & F_M2) && (FULLSEG || cnt > 0.0)) {
         const double mb = FULLSEG ? sb * (1.0 / M) : sb / cnt;
         const T mt = (T)mb;
-        const T qt = pairwise<M, 0, T>([&](int i) {
+        // sum of squared deviations: four interleaved FMA chains, then a pairwise sum of them
+        T q4[4] = {(T)0, (T)0, (T)0, (T)0};
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
             const T d = valid(i) ? rn_sub(y[i], mt) : (T)0;
-            return rn_mul(d, d);
-        });
+            q4[i & 3] = fma(d, d, q4[i & 3]);
+        }
+        const T qt = rn_add(rn_add(q4[0], q4[1]), rn_add(q4[2], q4[3]));
         // qt is taken about mt, the mean rounded to T: Q = M2 + cnt (mt - mean)^2
         const double dm = (double)mt - mb;
         double m2b = (double)qt - cnt * dm * dm;
@@ -363,22 +367,28 @@ __device__ __forceinline__ void seg_body(const Chain &chain, T (&y)[VPT * Vec16<
         a.lo = e - (a.hi - s);
     }
     a.n += cnt;
-    // ---- extremum winners: strict comparison in index order (NaN never compares greater)
+    // ---- extremum winners: strict comparison in index order (NaN never compares greater). The
+    // scan records only the winning slot i of this segment (a literal operand, no register copy
+    // of sl * M + i per element); the code is formed once per segment.
     if (FLAGS & F_MAX) {
+        unsigned ci = NO_CODE;
 #pragma unroll
         for (int i = 0; i < M; ++i) {
             const bool gt = valid(i) && y[i] > a.vmax;
             a.vmax = gt ? y[i] : a.vmax;
-            a.cmax = gt ? sl * M + i : a.cmax;
+            ci = gt ? (unsigned)i : ci;
         }
+        a.cmax = ci != NO_CODE ? sl * M + ci : a.cmax;
     }
     if (FLAGS & F_MIN) {
+        unsigned ci = NO_CODE;
 #pragma unroll
         for (int i = 0; i < M; ++i) {
             const bool lt = valid(i) && y[i] < a.vmin;
             a.vmin = lt ? y[i] : a.vmin;
-            a.cmin = lt ? sl * M + i : a.cmin;
+            ci = lt ? (unsigned)i : ci;
         }
+        a.cmin = ci != NO_CODE ? sl * M + ci : a.cmin;
     }
 }
 

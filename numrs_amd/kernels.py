@@ -6,6 +6,7 @@ Names, argument meaning and error behaviour follow the reference's crate-private
     reduce.rs:207-421     sum_f64 / mean_f64 / sum_sq_dev_f64 / var_f64 / min_f64 / max_f64 /
                           prod_f64 and their _f32 twins -> a Python float
     gemm.rs:246-268       gemm_2d(m, k, n, a, b, c): overwrites c in place
+    array/linalg.rs:56-310 batched_gemm(batch, m, k, n, a, b, c) and Array::matmul's batching
     elementwise.rs:51-258 binary_f64 / binary_f32 / unary_dispatch with op codes instead of closures
     operations.rs:303-318 sum_i64 (Array::<i64>::sum)
 
@@ -154,6 +155,92 @@ def gemm_2d(m: int, k: int, n: int, a, b, c: np.ndarray) -> None:
     fn = {np.dtype(np.float64): "nrk_gemm_2d_f64", np.dtype(np.float32): "nrk_gemm_2d_f32"}[dtype]
     call(fn, C.c_size_t(m), C.c_size_t(k), C.c_size_t(n), _ptr(a), C.c_size_t(a.size), _ptr(b), C.c_size_t(b.size),
          _ptr(c), C.c_size_t(c.size))
+
+
+def batched_gemm(batch: int, m: int, k: int, n: int, a, b, c: np.ndarray) -> None:
+    """batched_gemm (array/linalg.rs:56-136): C[t] := A[t] B[t] over contiguous panels, into the
+    caller's c. batch, m or n = 0, or slice lengths other than batch*m*k / batch*k*n /
+    batch*m*n, leave c untouched; otherwise panel t is gemm_2d's result on that panel."""
+    if not isinstance(c, np.ndarray) or not c.flags.c_contiguous:
+        raise TypeError("c must be a C-contiguous numpy array")
+    dtype = c.dtype
+    a = np.ascontiguousarray(a, dtype=dtype).reshape(-1)
+    b = np.ascontiguousarray(b, dtype=dtype).reshape(-1)
+    fn = {np.dtype(np.float64): "nrk_batched_gemm_f64", np.dtype(np.float32): "nrk_batched_gemm_f32"}[dtype]
+    call(fn, C.c_size_t(batch), C.c_size_t(m), C.c_size_t(k), C.c_size_t(n), _ptr(a), C.c_size_t(a.size), _ptr(b),
+         C.c_size_t(b.size), _ptr(c), C.c_size_t(c.size))
+
+
+def matmul(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """Array::matmul (array/linalg.rs:150-250) for f64 / f32: 2-D x 2-D is gemm_2d; a 1-D operand
+    is promoted to [1, k] (left) or [k, 1] (right) and the promoted axis stays in the result;
+    batch axes (all but the last two) broadcast numpy-style. Instead of broadcast_to copies, an
+    operand whose batch axes are all broadcast is passed once with panel stride 0; batch shapes
+    that broadcast on both sides go panel by panel, each operand indexed in place. Errors (the
+    reference's DimensionMismatch / ShapeMismatch) raise ValueError."""
+    from . import device
+    from ._lib import call as _call
+
+    a = np.asarray(a)
+    b = np.asarray(b)
+    dtype = np.result_type(a.dtype, b.dtype)
+    if dtype not in (np.float64, np.float32):
+        raise TypeError("matmul runs f64 and f32 on the GPU")
+    if a.ndim == 2 and b.ndim == 2:
+        if a.shape[1] != b.shape[0]:
+            raise ValueError(f"shape mismatch: {a.shape} x {b.shape}")
+        c = np.empty((a.shape[0], b.shape[1]), dtype)
+        gemm_2d(a.shape[0], a.shape[1], b.shape[1], a, b, c)
+        return c
+    if a.ndim == 1:
+        a = a.reshape(1, -1)
+    if b.ndim == 1:
+        b = b.reshape(-1, 1)
+    if a.ndim < 2 or b.ndim < 2:
+        raise ValueError("matmul requires operands with at least 1 dimension")
+    m, k = a.shape[-2:]
+    k2, n = b.shape[-2:]
+    if k != k2:
+        raise ValueError(f"shape mismatch: {a.shape} x {b.shape}")
+    ba, bb = a.shape[:-2], b.shape[:-2]
+    try:
+        bshape = tuple(np.broadcast_shapes(ba, bb))
+    except ValueError as e:
+        raise ValueError(f"batch shapes {ba} and {bb} do not broadcast") from e
+    batch = int(np.prod(bshape, dtype=np.int64))
+    out = np.empty(bshape + (m, n), dtype)
+    if batch == 0 or m == 0 or n == 0:
+        return out
+    a = np.ascontiguousarray(a, dtype)
+    b = np.ascontiguousarray(b, dtype)
+    fn = {np.dtype(np.float64): "nrk_dev_batched_gemm_f64", np.dtype(np.float32): "nrk_dev_batched_gemm_f32"}[np.dtype(dtype)]
+    da, db = device.DeviceArray.upload(a.reshape(-1)), device.DeviceArray.upload(b.reshape(-1))
+    dc = device.DeviceArray((out.size,), dtype)
+    isz = np.dtype(dtype).itemsize
+
+    def run(nb, pa, sa, pb, sb, pc):
+        _call(fn, C.c_size_t(nb), C.c_size_t(m), C.c_size_t(k), C.c_size_t(n), C.c_void_p(pa), C.c_size_t(sa),
+              C.c_void_p(pb), C.c_size_t(sb), C.c_void_p(pc), None)
+
+    def stride(sh, panel):  # one call over the whole batch: full batch shape, or a single panel
+        if sh == bshape:
+            return panel
+        if int(np.prod(sh, dtype=np.int64)) == 1:
+            return 0
+        return None
+
+    sa, sb = stride(ba, m * k), stride(bb, k * n)
+    if sa is not None and sb is not None:
+        run(batch, da.ptr, sa, db.ptr, sb, dc.ptr)
+    else:  # both sides broadcast along different axes: panel by panel, operands in place
+        ia = np.broadcast_to(np.arange(int(np.prod(ba, dtype=np.int64))).reshape(ba), bshape).reshape(-1)
+        ib = np.broadcast_to(np.arange(int(np.prod(bb, dtype=np.int64))).reshape(bb), bshape).reshape(-1)
+        for t in range(batch):
+            run(1, da.ptr + int(ia[t]) * m * k * isz, 0, db.ptr + int(ib[t]) * k * n * isz, 0, dc.ptr + t * m * n * isz)
+    device.synchronize()
+    res = dc.download().reshape(out.shape)
+    del da, db, dc
+    return res
 
 
 def gemm_f32_bf16x(a: np.ndarray, b: np.ndarray, products: int = 3) -> np.ndarray:

@@ -86,3 +86,74 @@ def test_large_f64_sampled(gpu, oracle):
     ex = oracle.dd_gemm_entries(a, b, rows, cols)
     err = np.abs(c[rows, cols] - ex) / np.maximum(np.abs(ex), 1.0)
     assert err.max() <= 1e-12
+
+
+# ---- batched_gemm (array/linalg.rs:56-136) and Array::matmul's batch axes (linalg.rs:150-250)
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("shape", [(5, 37, 53, 29), (3, 128, 96, 64), (2, 300, 17, 260)])
+def test_batched_panels_bit_identical_to_gemm_2d(gpu, dtype, shape):
+    """Panel t of a batch is gemm_2d on that panel, bit for bit (same kernel, same k order), and
+    gemm_2d is pinned against the double-double oracle above."""
+    batch, m, k, n = shape
+    a = rng.uniform(rng.BASE_SEED + 1700 + m, batch * m * k).astype(dtype)
+    b = rng.uniform(rng.BASE_SEED + 1800 + n, batch * k * n).astype(dtype)
+    c = np.full(batch * m * n, np.nan, dtype)
+    K.batched_gemm(batch, m, k, n, a, b, c)
+    for t in range(batch):
+        want = gemm(a[t * m * k:(t + 1) * m * k].reshape(m, k), b[t * k * n:(t + 1) * k * n].reshape(k, n))
+        np.testing.assert_array_equal(c[t * m * n:(t + 1) * m * n].reshape(m, n).view(np.uint64 if dtype == np.float64 else np.uint32),
+                                      want.view(np.uint64 if dtype == np.float64 else np.uint32))
+
+
+def test_batched_degenerate_and_length_rules(gpu):
+    """linalg.rs:60-73: batch, m or n = 0, or a slice length mismatch, return without writing C
+    (unlike gemm_2d's zero fill); k = 0 panels are gemm_2d's, i.e. zero-filled."""
+    sentinel = np.full(12, 7.0)
+    for batch, m, k, n, la, lb in [(0, 2, 3, 2, 0, 0), (2, 0, 3, 2, 0, 12), (2, 2, 3, 0, 12, 0),
+                                   (2, 2, 3, 3, 11, 18), (2, 2, 3, 3, 12, 17)]:
+        c = sentinel.copy()
+        K.batched_gemm(batch, m, k, n, np.ones(la), np.ones(lb), c[:batch * m * n] if batch * m * n else c)
+        np.testing.assert_array_equal(c, sentinel)
+    c = np.full(12, np.nan)
+    K.batched_gemm(2, 2, 0, 3, np.ones(0), np.ones(0), c)
+    assert np.all(c == 0.0)
+
+
+def test_batched_integer_panels_exact_and_many_launches(gpu):
+    """Integer-valued panels multiply exactly, so the result equals numpy's exactly. 70 000
+    one-element panels take two launches (at most 65 535 panels per launch)."""
+    r = np.random.default_rng(31)
+    batch, m, k, n = 6, 33, 70, 45
+    a = r.integers(-50, 51, (batch, m, k)).astype(np.float64)
+    b = r.integers(-50, 51, (batch, k, n)).astype(np.float64)
+    c = np.empty(batch * m * n)
+    K.batched_gemm(batch, m, k, n, a, b, c)
+    np.testing.assert_array_equal(c.reshape(batch, m, n), a @ b)
+    big = 70_000
+    x = r.integers(-1000, 1001, big).astype(np.float32)
+    y = r.integers(-1000, 1001, big).astype(np.float32)
+    z = np.empty(big, np.float32)
+    K.batched_gemm(big, 1, 1, 1, x, y, z)
+    np.testing.assert_array_equal(z, x * y)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_matmul_batch_broadcast(gpu, dtype):
+    """matmul's batch broadcast: one side unbatched (panel stride 0), equal batch axes, and axes
+    broadcast on both sides (panel by panel), with 1-D promotion. Integer-valued entries make
+    every product exact, so each case must equal numpy's matmul exactly (same broadcast rules)."""
+    r = np.random.default_rng(41)
+    ints = lambda *s: r.integers(-20, 21, s).astype(dtype)  # noqa: E731
+    cases = [(ints(4, 9, 7), ints(7, 5)), (ints(9, 7), ints(3, 7, 5)), (ints(2, 3, 9, 7), ints(2, 3, 7, 5)),
+             (ints(4, 1, 9, 7), ints(1, 3, 7, 5)), (ints(7), ints(2, 7, 5)), (ints(2, 9, 7), ints(7)),
+             (ints(1, 9, 7), ints(6, 7, 5))]
+    for a, b in cases:
+        got = K.matmul(a, b)
+        want = np.matmul(a.reshape(1, -1) if a.ndim == 1 else a, b.reshape(-1, 1) if b.ndim == 1 else b)
+        assert got.shape == want.shape, (a.shape, b.shape)
+        np.testing.assert_array_equal(got, want)
+    with pytest.raises(ValueError):
+        K.matmul(ints(2, 3, 4), ints(5, 6))
+    with pytest.raises(ValueError):
+        K.matmul(ints(2, 3, 4), ints(3, 4, 6))
