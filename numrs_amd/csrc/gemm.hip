@@ -185,6 +185,21 @@ static nrk_status launch_gemm(int m, int n, int k, const T *a, const T *b, T *c,
     return NRK_OK;
 }
 
+// Default tile: the largest of 128 x 128, 64 x 64 and 32 x 32 that still gives every CU a
+// workgroup over the whole batch (measured at 512^3 f64: 17.8 us with 32 x 32 against 35.3 us
+// with 64 x 64; at 2048^3 128 x 128 is best). The tile never changes a result bit.
+template <typename T>
+static nrk_status launch_default(size_t batch, size_t m, size_t k, size_t n, const T *a, size_t sa, const T *b, size_t sb,
+                                 T *c, size_t sc, hipStream_t stream) {
+    const size_t cus = (size_t)cu_count();
+    const int M = (int)m, N = (int)n, K = (int)k;
+    if (batch * ceil_div(m, 128) * ceil_div(n, 128) >= cus)
+        return launch_gemm<T, 128, 128, 16, 64, 64>(M, N, K, a, b, c, stream, batch, sa, sb, sc);
+    if (batch * ceil_div(m, 64) * ceil_div(n, 64) > cus)
+        return launch_gemm<T, 64, 64, 16, 32, 32>(M, N, K, a, b, c, stream, batch, sa, sb, sc);
+    return launch_gemm<T, 32, 32, 16, 16, 16>(M, N, K, a, b, c, stream, batch, sa, sb, sc);
+}
+
 template <typename T>
 nrk_status gemm(size_t m, size_t k, size_t n, const T *a, const T *b, T *c, hipStream_t stream) {
     NRK_TRY(ensure_device());
@@ -200,11 +215,7 @@ nrk_status gemm(size_t m, size_t k, size_t n, const T *a, const T *b, T *c, hipS
     if (!a || !b) return fail(NRK_ERR_INVALID_ARGUMENT, "A or B is NULL");
     if (m > (size_t)INT32_MAX / 2 || n > (size_t)INT32_MAX / 2 || k > (size_t)INT32_MAX / 2)
         return fail(NRK_ERR_INVALID_ARGUMENT, "dimension exceeds 2^30");
-    const size_t tiles128 = ceil_div(m, 128) * ceil_div(n, 128);
-    if (tiles128 >= (size_t)cu_count()) {
-        return launch_gemm<T, 128, 128, 16, 64, 64>((int)m, (int)n, (int)k, a, b, c, stream);
-    }
-    return launch_gemm<T, 64, 64, 16, 32, 32>((int)m, (int)n, (int)k, a, b, c, stream);
+    return launch_default<T>(1, m, k, n, a, 0, b, 0, c, 0, stream);
 }
 
 template nrk_status gemm<double>(size_t, size_t, size_t, const double *, const double *, double *, hipStream_t);
@@ -212,7 +223,7 @@ template nrk_status gemm<float>(size_t, size_t, size_t, const float *, const flo
 
 // C[t] := A[t] * B[t] for t < batch; C panels are contiguous (sc = m n), A and B panels sit
 // `sa` / `sb` elements apart (0: one panel shared by every t). The tile is chosen as for one
-// gemm_2d but counting the tiles of the whole batch; it does not change any result bit.
+// gemm_2d but counting the tiles of the whole batch.
 template <typename T>
 nrk_status batched_gemm(size_t batch, size_t m, size_t k, size_t n, const T *a, size_t sa, const T *b, size_t sb, T *c,
                         hipStream_t stream) {
@@ -223,10 +234,7 @@ nrk_status batched_gemm(size_t batch, size_t m, size_t k, size_t n, const T *a, 
     if (!a || !b) return fail(NRK_ERR_INVALID_ARGUMENT, "A or B is NULL");
     if (m > (size_t)INT32_MAX / 2 || n > (size_t)INT32_MAX / 2 || k > (size_t)INT32_MAX / 2)
         return fail(NRK_ERR_INVALID_ARGUMENT, "dimension exceeds 2^30");
-    const size_t sc = m * n;
-    if (batch * ceil_div(m, 128) * ceil_div(n, 128) >= (size_t)cu_count())
-        return launch_gemm<T, 128, 128, 16, 64, 64>((int)m, (int)n, (int)k, a, b, c, stream, batch, sa, sb, sc);
-    return launch_gemm<T, 64, 64, 16, 32, 32>((int)m, (int)n, (int)k, a, b, c, stream, batch, sa, sb, sc);
+    return launch_default<T>(batch, m, k, n, a, sa, b, sb, c, m * n, stream);
 }
 
 // Tile configurations the tuner times and the selector chooses from (all 4 wavefronts).

@@ -12,8 +12,9 @@
 //   3. C_j = a_j * b_j^T on v_mfma_i32_16x16x64_i8: int32 accumulation is exact
 //      (k * 127^2 < 2^31 for k <= 133 000), then c_j = C_j mod m_j (int8).
 //   4. CRT: C' = sum_j c_j w_j mod M (w_j = (M/m_j) * ((M/m_j)^-1 mod m_j), M = prod m_j) is the
-//      EXACT integer product A'B'. It is formed with 24-bit limbs whose partial sums are exact
-//      in FP64, reduced mod M, normalised, and evaluated in double-double; C = C' 2^(-s_i - t_j).
+//      EXACT integer product A'B'. It is formed with B-bit limbs (B = 53 - ceil(log2(127 N + 1)),
+//      42 for up to 16 moduli) whose partial sums are exact in FP64, reduced mod M, normalised,
+//      and evaluated in double-double; C = C' 2^(-s_i - t_j).
 // The only error is the rounding of A and B to integers in step 1 (at most 1/2 in units of
 // |row|_2 / X); all integer work is exact.
 //
@@ -29,7 +30,7 @@
 namespace nrk {
 
 constexpr int OZ_MAX_MODULI = 20;
-constexpr int OZ_LIMBS = 7;       // 7 x 24-bit limbs = 168 bits > log2(M) for 20 moduli
+constexpr int OZ_LIMBS = 4;       // 4 x 41-bit limbs = 164 bits > log2(M) for 20 moduli
 constexpr int OZ_DEFAULT_MODULI = 16;
 constexpr int OZ_TILE_CONFIGS = 5;  // INT8 GEMM tiles (workgroup tile, wave tile): 64x64 (32x32),
                                      // 128x128 (64x64), 128x256 (64x64), 256x256 (128x64),
@@ -45,9 +46,10 @@ struct OzConsts {
     int m[OZ_MAX_MODULI];
     double inv_m[OZ_MAX_MODULI];
     double c26[OZ_MAX_MODULI];              // 2^26 mod m_j
-    double w[OZ_MAX_MODULI][OZ_LIMBS];      // CRT weights, 24-bit limbs, least significant first
-    double Mlimb[OZ_LIMBS];                 // M in 24-bit limbs
-    int limbs;                              // 24-bit limbs of M (and of every weight)
+    double w[OZ_MAX_MODULI][OZ_LIMBS];      // CRT weights in B-bit limbs, least significant first
+    double Mlimb[OZ_LIMBS];                 // M in B-bit limbs
+    int limbs;                              // B-bit limbs of M (and of every weight)
+    double base, inv_base;                  // 2^B, 2^-B
     double M_approx;                        // M as a double (rounded)
     double inv_M;                           // 1 / M_approx
 };
@@ -74,6 +76,24 @@ struct Big {  // little-endian base 2^24 limbs, non-negative
     }
 };
 
+// Bits [lo, lo + len) of x (len <= 64).
+static uint64_t big_bits(const Big &x, int lo, int len) {
+    unsigned __int128 acc = 0;
+    const int d0 = lo / 24, sh = lo % 24;
+    for (int i = 3; i >= 0; --i) {
+        const size_t di = (size_t)(d0 + i);
+        acc = (acc << 24) | (unsigned __int128)(di < x.d.size() ? x.d[di] : 0u);
+    }
+    acc >>= sh;  // 96 - 23 >= 64 bits remain
+    return len >= 64 ? (uint64_t)acc : (uint64_t)(acc & ((((unsigned __int128)1) << len) - 1));
+}
+static int big_bitlen(const Big &x) {
+    if (x.d.empty()) return 0;
+    int b = 0;
+    for (uint64_t top = x.d.back(); top; top >>= 1) ++b;
+    return (int)(x.d.size() - 1) * 24 + b;
+}
+
 static int modinv(int a, int m) {  // a^-1 mod m, gcd(a, m) = 1
     int t = 0, nt = 1, r = m, nr = ((a % m) + m) % m;
     while (nr) {
@@ -90,8 +110,23 @@ static nrk_status make_consts(int n, OzConsts *c) {
     c->n = n;
     Big M(1);
     for (int j = 0; j < n; ++j) M.mul_small((uint64_t)kModuli[j]);
-    for (int t = 0; t < OZ_LIMBS; ++t) c->Mlimb[t] = t < (int)M.d.size() ? (double)M.d[t] : 0.0;
-    c->limbs = (int)M.d.size();
+    // Limb width: a sum of n products c_j w_j[t] (|c_j| <= 127, w_j[t] < 2^B) plus a carry stays
+    // below 2^53, and so does a normalised limb minus q Mlimb[t] (|q| <= 127 n).
+    int B = 53;
+    while ((double)(127 * n + 1) * std::ldexp(1.0, B) > 0x1p53) --B;
+    const int bits = big_bitlen(M);
+    int NL = std::max(1, (bits + B - 1) / B);
+    // the top limb carries everything above B (NL - 1) bits: 127 n (2 M_top + 1) + 1 < 2^53
+    while (NL < OZ_LIMBS && 127.0 * n * (2.0 * (double)big_bits(M, B * (NL - 1), 64) + 1.0) + 1.0 >= 0x1p53) ++NL;
+    if (NL > OZ_LIMBS || B * NL < bits) return fail(NRK_ERR_UNSUPPORTED, "CRT limbs do not cover M");
+    c->limbs = NL;
+    c->base = std::ldexp(1.0, B);
+    c->inv_base = std::ldexp(1.0, -B);
+    auto limb = [&](const Big &x, int t) -> double {
+        if (t >= NL) return 0.0;
+        return (double)big_bits(x, B * t, t == NL - 1 ? 64 : B);
+    };
+    for (int t = 0; t < OZ_LIMBS; ++t) c->Mlimb[t] = limb(M, t);
     c->M_approx = M.to_double();
     c->inv_M = 1.0 / c->M_approx;
     for (int j = 0; j < n; ++j) {
@@ -108,7 +143,7 @@ static nrk_status make_consts(int n, OzConsts *c) {
         }
         const int y = modinv(Mj_mod, mj);
         Mj.mul_small((uint64_t)y);  // w_j = M_j * y_j  (< M)
-        for (int t = 0; t < OZ_LIMBS; ++t) c->w[j][t] = t < (int)Mj.d.size() ? (double)Mj.d[t] : 0.0;
+        for (int t = 0; t < OZ_LIMBS; ++t) c->w[j][t] = limb(Mj, t);
     }
     return NRK_OK;
 }
@@ -126,6 +161,7 @@ struct SsqAcc {
     int e;
 };
 constexpr int OZ_NO_EXP = -4000;  // below every double exponent: an empty accumulator
+constexpr int OZ_BATCH = 8;        // loads a thread issues before folding them in
 __device__ __forceinline__ void ssq_add(SsqAcc &a, double x) {
     const double ax = fabs(x);
     if (!(ax > 0.0) || ax == __builtin_inf()) return;  // zero or non-finite (recorded separately)
@@ -163,10 +199,18 @@ __global__ __launch_bounds__(256) void row_scale_kernel(const double *__restrict
     const int r = blockIdx.x;
     SsqAcc acc{0.0, OZ_NO_EXP};
     int bad = 0;
-    for (int c = threadIdx.x; c < Cc; c += 256) {
-        const double v = X[(size_t)r * Cc + c];
-        bad |= !__builtin_isfinite(v);
-        ssq_add(acc, v);
+    for (int c0 = 0; c0 < Cc; c0 += 256 * OZ_BATCH) {  // OZ_BATCH loads in flight, then the updates
+        double v[OZ_BATCH];
+#pragma unroll
+        for (int u = 0; u < OZ_BATCH; ++u) {
+            const int c = c0 + u * 256 + (int)threadIdx.x;
+            v[u] = c < Cc ? X[(size_t)r * Cc + c] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < OZ_BATCH; ++u) {
+            bad |= !__builtin_isfinite(v[u]);
+            ssq_add(acc, v[u]);
+        }
     }
     __shared__ double ss[256];
     __shared__ int se[256], sbad[256];
@@ -198,10 +242,15 @@ __global__ __launch_bounds__(256) void col_ssq_partial_kernel(const double *__re
     const int r0 = blockIdx.y * COL_ROWS, r1 = min(R, r0 + COL_ROWS);
     SsqAcc acc{0.0, OZ_NO_EXP};
     int bad = 0;
-    for (int r = r0; r < r1; ++r) {
-        const double v = X[(size_t)r * Cc + c];
-        bad |= !__builtin_isfinite(v);
-        ssq_add(acc, v);
+    for (int b = r0; b < r1; b += OZ_BATCH) {
+        double v[OZ_BATCH];
+#pragma unroll
+        for (int u = 0; u < OZ_BATCH; ++u) v[u] = b + u < r1 ? X[(size_t)(b + u) * Cc + c] : 0.0;
+#pragma unroll
+        for (int u = 0; u < OZ_BATCH; ++u) {
+            bad |= !__builtin_isfinite(v[u]);
+            ssq_add(acc, v[u]);
+        }
     }
     part[(size_t)blockIdx.y * Cc + c] = bad ? __builtin_inf() : acc.s;
     part[(size_t)(gridDim.y + blockIdx.y) * Cc + c] = (double)acc.e;
@@ -213,10 +262,19 @@ __global__ __launch_bounds__(256) void col_scale_kernel(const double *__restrict
     if (c >= Cc) return;
     SsqAcc acc{0.0, OZ_NO_EXP};
     bool bad = false;
-    for (int h = 0; h < nchunk; ++h) {
-        const double s = part[(size_t)h * Cc + c];
-        bad |= __builtin_isinf(s);
-        acc = ssq_merge(acc, SsqAcc{bad ? 0.0 : s, (int)part[(size_t)(nchunk + h) * Cc + c]});
+    for (int h0 = 0; h0 < nchunk; h0 += OZ_BATCH) {
+        double sv[OZ_BATCH], ev[OZ_BATCH];
+#pragma unroll
+        for (int u = 0; u < OZ_BATCH; ++u) {
+            const int h = h0 + u;
+            sv[u] = h < nchunk ? part[(size_t)h * Cc + c] : 0.0;
+            ev[u] = h < nchunk ? part[(size_t)(nchunk + h) * Cc + c] : (double)OZ_NO_EXP;
+        }
+#pragma unroll
+        for (int u = 0; u < OZ_BATCH; ++u) {
+            bad |= __builtin_isinf(sv[u]);
+            acc = ssq_merge(acc, SsqAcc{bad ? 0.0 : sv[u], (int)ev[u]});
+        }
     }
     scale[c] = bad ? OZ_BAD_SCALE : cs_scale(acc, xeff);
 }
@@ -445,7 +503,7 @@ __device__ __forceinline__ dd_t dd_two_sum(double a, double b) {
 // Thread t reads word t of every modulus' residue array: the residues of block t / 64 (16 x 16
 // blocks, Nb = ceil(N/16) per block row) at lane l = t % 64, i.e. rows 4 (l/16) + 0..3 of column
 // l % 16, and reconstructs those four outputs. A wavefront reads 256 contiguous bytes per modulus.
-// NL >= k.limbs limbs: the weights, M and every intermediate fit (limbs above k.limbs are zero).
+// NL >= k.limbs limbs of B bits: the weights, M and every intermediate fit (limbs above k.limbs are zero).
 template <int NL>
 __global__ __launch_bounds__(256) void crt_kernel(const signed char *__restrict__ Cres, size_t c_stride, int M, int N,
                                                   const int *__restrict__ srow, const int *__restrict__ scol, OzConsts k,
@@ -497,34 +555,33 @@ __global__ __launch_bounds__(256) void crt_kernel(const signed char *__restrict_
         double L[NL];
 #pragma unroll
         for (int t = 0; t < NL; ++t) L[t] = Lx[u][t];
-        // X_raw = sum_t L[t] 2^(24 t); subtract q M with q = round(X_raw / M)
+        // balanced carry normalisation: limbs below the top in [-2^(B-1), 2^(B-1)] (exact)
+        auto normalise = [&]() {
+#pragma unroll
+            for (int t = 0; t < NL - 1; ++t) {
+                const double carry = rint(L[t] * k.inv_base);
+                L[t] = fma(-carry, k.base, L[t]);
+                L[t + 1] += carry;
+            }
+        };
+        normalise();
+        // X_raw = sum_t L[t] 2^(B t); subtract q M with q = round(X_raw / M)
         double approx = 0.0;
 #pragma unroll
-        for (int t = NL - 1; t >= 0; --t) approx = approx * 16777216.0 + L[t];
+        for (int t = NL - 1; t >= 0; --t) approx = fma(approx, k.base, L[t]);
         // |C'| <= (M/2)(1 - 2^-20) by the choice of the scales, and |X_raw| < 2^12 M, so the
         // FP64 quotient approx / M is within 2^-36 of X_raw / M and rounds to the right q
         const double q = rint(approx * k.inv_M);
 #pragma unroll
-        for (int t = 0; t < NL; ++t) L[t] = fma(-q, k.Mlimb[t], L[t]);  // exact
-        // balanced carry normalisation: every limb in [-2^23, 2^23]
+        for (int t = 0; t < NL; ++t) L[t] = fma(-q, k.Mlimb[t], L[t]);  // exact: below 2^53
+        normalise();
+        // double-double Horner from the top limb, base 2^B (every limb is an exact double)
+        dd_t x{L[NL - 1], 0.0};
 #pragma unroll
-        for (int t = 0; t < NL - 1; ++t) {
-            const double carry = rint(L[t] * (1.0 / 16777216.0));
-            L[t] = fma(-carry, 16777216.0, L[t]);
-            L[t + 1] += carry;
-        }
-        // double-double Horner from the top, two limbs per digit: with every limb in
-        // [-2^23, 2^23] a pair L[2i+1] 2^24 + L[2i] is an exact double, so X = sum_i P_i 2^(48 i)
-        constexpr int NP = (NL + 1) / 2;
-        double P[NP];
-#pragma unroll
-        for (int i = 0; i < NP; ++i) P[i] = 2 * i + 1 < NL ? fma(L[2 * i + 1], 16777216.0, L[2 * i]) : L[2 * i];
-        dd_t x{P[NP - 1], 0.0};
-#pragma unroll
-        for (int i = NP - 2; i >= 0; --i) {
-            x.hi *= 281474976710656.0;  // 2^48, exact
-            x.lo *= 281474976710656.0;
-            dd_t s = dd_two_sum(x.hi, P[i]);
+        for (int t = NL - 2; t >= 0; --t) {
+            x.hi *= k.base;  // exact
+            x.lo *= k.base;
+            dd_t s = dd_two_sum(x.hi, L[t]);
             s.lo += x.lo;
             x = dd_two_sum(s.hi, s.lo);
         }
@@ -611,12 +668,14 @@ nrk_status gemm_f64_ozaki(size_t m, size_t k, size_t n, const double *a, const d
         case 3: NRK_TRY((launch_i8<256, 256, 128, 64>((int)m, (int)n, (int)Kp, Ares, sa, Bres, sb, Cres, sc, kc, stream))); break;
         default: NRK_TRY((launch_i8<256, 256, 128, 128>((int)m, (int)n, (int)Kp, Ares, sa, Bres, sb, Cres, sc, kc, stream)));
     }
-    {  // the fewest limbs that hold M: 5 for up to 15 moduli, 6 for 16
+    {  // the limbs that hold M: 3 for 10 to 16 moduli
         const dim3 g((unsigned)ceil_div(sc / 4, 256));
-        if (kc.limbs <= 4) hipLaunchKernelGGL(crt_kernel<4>, g, dim3(256), 0, stream, Cres, sc, (int)m, (int)n, srow, scol, kc, c);
-        else if (kc.limbs == 5) hipLaunchKernelGGL(crt_kernel<5>, g, dim3(256), 0, stream, Cres, sc, (int)m, (int)n, srow, scol, kc, c);
-        else if (kc.limbs == 6) hipLaunchKernelGGL(crt_kernel<6>, g, dim3(256), 0, stream, Cres, sc, (int)m, (int)n, srow, scol, kc, c);
-        else hipLaunchKernelGGL(crt_kernel<OZ_LIMBS>, g, dim3(256), 0, stream, Cres, sc, (int)m, (int)n, srow, scol, kc, c);
+        switch (kc.limbs) {
+            case 1: hipLaunchKernelGGL(crt_kernel<1>, g, dim3(256), 0, stream, Cres, sc, (int)m, (int)n, srow, scol, kc, c); break;
+            case 2: hipLaunchKernelGGL(crt_kernel<2>, g, dim3(256), 0, stream, Cres, sc, (int)m, (int)n, srow, scol, kc, c); break;
+            case 3: hipLaunchKernelGGL(crt_kernel<3>, g, dim3(256), 0, stream, Cres, sc, (int)m, (int)n, srow, scol, kc, c); break;
+            default: hipLaunchKernelGGL(crt_kernel<OZ_LIMBS>, g, dim3(256), 0, stream, Cres, sc, (int)m, (int)n, srow, scol, kc, c);
+        }
     }
     NRK_HIP(hipGetLastError());
     return NRK_OK;

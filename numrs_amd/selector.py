@@ -138,13 +138,16 @@ def call_size(name: str, *args) -> int:
 
 def heuristic(family: int, d0, d1, d2=None) -> np.ndarray:
     """The built-in choice the selector replaces: native GEMM takes 128x128 tiles (config 4) when
-    there are at least as many 128x128 tiles as CUs, else 64x64 (config 1); the bf16 GEMM uses
-    64x64 (config 1); row sums use 8 workgroups per CU (config 3)."""
+    there are at least as many 128x128 tiles as CUs, else 64x64 (config 1) when there are more
+    64x64 tiles than CUs, else 32x32 (config 0); the bf16 GEMM uses 64x64 (config 1); row sums
+    use 8 workgroups per CU (config 3)."""
     d0 = np.asarray(d0)
     if family in (F64, F32):
         cus = device_cus()
-        tiles = np.ceil(d0 / 128) * np.ceil(np.asarray(d1) / 128)
-        return np.where(tiles >= cus, 4, 1)
+        d1 = np.asarray(d1)
+        t128 = np.ceil(d0 / 128) * np.ceil(d1 / 128)
+        t64 = np.ceil(d0 / 64) * np.ceil(d1 / 64)
+        return np.where(t128 >= cus, 4, np.where(t64 > cus, 1, 0))
     if family == BF16:
         return np.full(len(d0), 1)
     if family == ROWSUM_F32:

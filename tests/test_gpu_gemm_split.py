@@ -132,7 +132,7 @@ def test_ozaki_error_within_truncation_bound(gpu, oracle):
     assert errs[16] <= errs[14] <= errs[8]
 
 
-@pytest.mark.parametrize("nm", [8, 16, 20])
+@pytest.mark.parametrize("nm", [3, 8, 10, 16, 20])
 def test_ozaki_products_at_the_edge_of_the_crt_range(gpu, nm):
     """A row of A parallel to a column of B makes Cauchy-Schwarz tight: C'_ii = |A'_i|_2^2 for
     B = A^T, anywhere in (X^2 / 4, X^2] by where |A_i|_2 falls between powers of two. With
@@ -140,13 +140,16 @@ def test_ozaki_products_at_the_edge_of_the_crt_range(gpu, nm):
     to land within 2 % of the top of the CRT range; with 64 rows spread over the octave, a range
     rule too loose by even sqrt(2) would push some products past M/2 and wrap them. With 16 and
     20 moduli (20: the 2^62 cap on X) the scaled entries reach ~2^60, past the 2^53 of an FP64
-    significand, which exercises the residue split of large integers."""
+    significand, which exercises the residue split of large integers. The moduli counts take the
+    CRT through 1, 2, 2, 3 and 4 limbs (3 moduli: entries kept below 39 so that X, ~2^11.5, still
+    scales them by a power of two >= 1)."""
     k = 64
     X = ozaki_row_bound(nm, k)
+    hi = 38 if nm < 6 else 1000
     r = np.random.default_rng(21 + nm)
-    rows = [r.integers(-1000, 1001, k).astype(np.float64) for _ in range(63)]
+    rows = [r.integers(-hi, hi + 1, k).astype(np.float64) for _ in range(63)]
     for _ in range(20000):
-        v = r.integers(-1000, 1001, k).astype(np.float64)
+        v = r.integers(-hi, hi + 1, k).astype(np.float64)
         nrm = math.sqrt(float(v @ v))
         s = math.floor(math.log2(X / nrm))
         if 0.99 * X < nrm * 2.0 ** s < (1 - 1e-6) * X:
