@@ -98,7 +98,31 @@ enum {
     NRK_SOP_ABS = 7,    /* y = |x|                                                */
     NRK_SOP_SQUARE = 8, /* y = x * x                                              */
     NRK_SOP_SQRT = 9,   /* y = sqrt(x), correctly rounded                         */
-    NRK_SOP_COUNT = 10
+    /* ufuncs.rs:702-780, exact: Rust's f64 floor / ceil / trunc / round (half away from
+     * zero) / fract (x - trunc(x)); sign as ufuncs.rs:756 (0 for +-0, 1 for x > 0, else -1,
+     * so NaN gives -1); clamp (ufuncs.rs:774, x.clamp(lo, hi)) as CLAMP_LO s = lo then
+     * CLAMP_HI s = hi (NaN passes through). */
+    NRK_SOP_FLOOR = 10,
+    NRK_SOP_CEIL = 11,
+    NRK_SOP_TRUNC = 12,
+    NRK_SOP_ROUND = 13,
+    NRK_SOP_FRACT = 14,
+    NRK_SOP_SIGN = 15,
+    NRK_SOP_CLAMP_LO = 16, /* y = x < s ? s : x */
+    NRK_SOP_CLAMP_HI = 17, /* y = x > s ? s : x */
+    /* ufuncs.rs:398-475, 639: exp, exp2, expm1, ln, log2, log10, ln_1p, tanh: FP64 evaluation
+     * within 1 ulp of the correctly rounded result (tanh 2 ulp); binary32 inputs are evaluated
+     * in FP64 and rounded once. Elementwise map only (nrk_map_* / nrk_dev_map_*): a fused
+     * map -> reduce chain holding one is rejected with NRK_ERR_UNSUPPORTED. */
+    NRK_SOP_EXP = 18,
+    NRK_SOP_EXP2 = 19,
+    NRK_SOP_EXPM1 = 20,
+    NRK_SOP_LOG = 21,
+    NRK_SOP_LOG2 = 22,
+    NRK_SOP_LOG10 = 23,
+    NRK_SOP_LOG1P = 24,
+    NRK_SOP_TANH = 25,
+    NRK_SOP_COUNT = 26
 };
 typedef struct nrk_scalar_op {
     int32_t op;     /* NRK_SOP_* */

@@ -131,7 +131,7 @@ nrk_status host_axis(int32_t op, const T *x, size_t outer, size_t len, size_t in
 template <typename T>
 nrk_status host_map(const nrk_scalar_op *chain, int32_t chain_len, const T *x, T *y, size_t n) {
     Chain c;
-    NRK_TRY(make_chain(chain, chain_len, &c));
+    NRK_TRY(make_chain(chain, chain_len, &c, true));
     if (n && !y) return fail(NRK_ERR_INVALID_ARGUMENT, "y is NULL");
     return host_call<T, T>(x, n, y, n, [&](const T *d, T *dout, hipStream_t s) { return map<T>(c, d, dout, n, s); });
 }
@@ -164,12 +164,12 @@ nrk_status nrk_map_f64(const nrk_scalar_op *chain, int32_t chain_len, const doub
 nrk_status nrk_map_f32(const nrk_scalar_op *chain, int32_t chain_len, const float *x, float *y, size_t n) { return host_map<float>(chain, chain_len, x, y, n); }
 nrk_status nrk_dev_map_f64(const nrk_scalar_op *chain, int32_t chain_len, const double *x, double *y, size_t n, nrk_stream stream) {
     Chain c;
-    NRK_TRY(make_chain(chain, chain_len, &c));
+    NRK_TRY(make_chain(chain, chain_len, &c, true));
     return map<double>(c, x, y, n, S(stream));
 }
 nrk_status nrk_dev_map_f32(const nrk_scalar_op *chain, int32_t chain_len, const float *x, float *y, size_t n, nrk_stream stream) {
     Chain c;
-    NRK_TRY(make_chain(chain, chain_len, &c));
+    NRK_TRY(make_chain(chain, chain_len, &c, true));
     return map<float>(c, x, y, n, S(stream));
 }
 nrk_status nrk_zip_f64(int32_t op, const double *a, const double *b, double *y, size_t n) { return host_zip<double>(op, a, b, y, n); }

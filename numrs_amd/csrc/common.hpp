@@ -7,6 +7,7 @@
 #include <cstdint>
 
 #include "numrs_amd.h"
+#include "mathfn.hpp"
 
 namespace nrk {
 
@@ -116,10 +117,38 @@ struct Chain {
     double s[NRK_MAX_CHAIN];
 };
 
+__device__ __forceinline__ float dev_floor(float a) { return __builtin_floorf(a); }
+__device__ __forceinline__ double dev_floor(double a) { return __builtin_floor(a); }
+__device__ __forceinline__ float dev_ceil(float a) { return __builtin_ceilf(a); }
+__device__ __forceinline__ double dev_ceil(double a) { return __builtin_ceil(a); }
+__device__ __forceinline__ float dev_trunc(float a) { return __builtin_truncf(a); }
+__device__ __forceinline__ double dev_trunc(double a) { return __builtin_trunc(a); }
+__device__ __forceinline__ float dev_copysign(float a, float b) { return __builtin_copysignf(a, b); }
+__device__ __forceinline__ double dev_copysign(double a, double b) { return __builtin_copysign(a, b); }
+
+// Rust's f64::round: half-way cases away from zero (x - trunc(x) is exact).
+template <typename T>
+__device__ __forceinline__ T round_half_away(T x) {
+    const T t = dev_trunc(x);
+    return dev_fabs(rn_sub(x, t)) >= (T)0.5 ? rn_add(t, dev_copysign((T)1, x)) : t;
+}
+// ufuncs.rs:756: 0 for +-0, 1 above zero, -1 otherwise (NaN included).
+template <typename T>
+__device__ __forceinline__ T sign_ref(T x) {
+    return x == (T)0 ? (T)0 : (x > (T)0 ? (T)1 : (T)-1);
+}
+// Elementary functions: FP64 evaluation, binary32 rounded once.
+template <typename T, typename F>
+__device__ __forceinline__ T via_f64(T x, F f) {
+    return (T)f((double)x);
+}
+
 // Apply the whole chain to N register values, each step (NRK_SOP_*) with one IEEE rounding: one
 // (wavefront-uniform) branch per step, then N straight-line operations, so the op dispatch is
-// not repeated per element.
-template <typename T, int N>
+// not repeated per element. MATH adds the elementary functions (exp .. tanh); kernels built
+// without it (the reductions) keep their register budget, and their chains are validated to
+// hold no such step.
+template <typename T, int N, bool MATH = false>
 __device__ __forceinline__ void apply_chain_n(const Chain &c, T (&v)[N]) {
     for (int k = 0; k < c.len; ++k) {
         const T s = (T)c.s[k];
@@ -141,11 +170,35 @@ __device__ __forceinline__ void apply_chain_n(const Chain &c, T (&v)[N]) {
             NRK_CHAIN_CASE(NRK_SOP_ABS, dev_fabs(x))
             NRK_CHAIN_CASE(NRK_SOP_SQUARE, rn_mul(x, x))
             NRK_CHAIN_CASE(NRK_SOP_SQRT, rn_sqrt(x))
+            NRK_CHAIN_CASE(NRK_SOP_FLOOR, dev_floor(x))
+            NRK_CHAIN_CASE(NRK_SOP_CEIL, dev_ceil(x))
+            NRK_CHAIN_CASE(NRK_SOP_TRUNC, dev_trunc(x))
+            NRK_CHAIN_CASE(NRK_SOP_ROUND, round_half_away(x))
+            NRK_CHAIN_CASE(NRK_SOP_FRACT, rn_sub(x, dev_trunc(x)))
+            NRK_CHAIN_CASE(NRK_SOP_SIGN, sign_ref(x))
+            NRK_CHAIN_CASE(NRK_SOP_CLAMP_LO, x < s ? s : x)
+            NRK_CHAIN_CASE(NRK_SOP_CLAMP_HI, x > s ? s : x)
+            default:
+                if constexpr (MATH) {
+                    switch (c.op[k]) {
+                        NRK_CHAIN_CASE(NRK_SOP_EXP, via_f64(x, [](double d) { return fm::exp(d); }))
+                        NRK_CHAIN_CASE(NRK_SOP_EXP2, via_f64(x, [](double d) { return fm::exp2(d); }))
+                        NRK_CHAIN_CASE(NRK_SOP_EXPM1, via_f64(x, [](double d) { return fm::expm1(d); }))
+                        NRK_CHAIN_CASE(NRK_SOP_LOG, via_f64(x, [](double d) { return fm::log(d); }))
+                        NRK_CHAIN_CASE(NRK_SOP_LOG2, via_f64(x, [](double d) { return fm::log2(d); }))
+                        NRK_CHAIN_CASE(NRK_SOP_LOG10, via_f64(x, [](double d) { return fm::log10(d); }))
+                        NRK_CHAIN_CASE(NRK_SOP_LOG1P, via_f64(x, [](double d) { return fm::log1p(d); }))
+                        NRK_CHAIN_CASE(NRK_SOP_TANH, via_f64(x, [](double d) { return fm::tanh(d); }))
+                        default: break;
+                    }
+                }
+                break;
 #undef NRK_CHAIN_CASE
-            default: break;
         }
     }
 }
-nrk_status make_chain(const nrk_scalar_op *chain, int32_t chain_len, Chain *out);
+// Validates and copies a chain. Without allow_math, elementary-function steps are refused (the
+// fused reductions are built without them).
+nrk_status make_chain(const nrk_scalar_op *chain, int32_t chain_len, Chain *out, bool allow_math = false);
 
 }  // namespace nrk

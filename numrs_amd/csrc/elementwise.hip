@@ -27,6 +27,12 @@ __device__ __forceinline__ T apply_bop(T a, T b) {
     else return a < b ? a : b;                                           // ufuncs.rs:173 (MINIMUM)
 }
 
+// The map applies every NRK_SOP_* step, elementary functions included.
+template <typename T, int N>
+__device__ __forceinline__ void map_chain(const Chain &c, T (&v)[N]) {
+    apply_chain_n<T, N, true>(c, v);
+}
+
 template <typename T>
 __global__ __launch_bounds__(EB) void map_kernel(Chain chain, const T *x, T *y, size_t n, int vec_ok) {
     typedef typename Vec16<T>::type V;
@@ -47,7 +53,7 @@ __global__ __launch_bounds__(EB) void map_kernel(Chain chain, const T *x, T *y, 
 #pragma unroll
                 for (int e = 0; e < VN; ++e) r[u * VN + e] = v[e];
             }
-            apply_chain_n(chain, r);
+            map_chain(chain, r);
 #pragma unroll
             for (int u = 0; u < EUNROLL; ++u) {
                 V v;
@@ -61,7 +67,7 @@ __global__ __launch_bounds__(EB) void map_kernel(Chain chain, const T *x, T *y, 
             T r[VN];
 #pragma unroll
             for (int e = 0; e < VN; ++e) r[e] = v[e];
-            apply_chain_n(chain, r);
+            map_chain(chain, r);
             V w;
 #pragma unroll
             for (int e = 0; e < VN; ++e) w[e] = r[e];
@@ -71,7 +77,7 @@ __global__ __launch_bounds__(EB) void map_kernel(Chain chain, const T *x, T *y, 
     }
     for (size_t i = done + tid; i < n; i += nthreads) {
         T r[1] = {x[i]};
-        apply_chain_n(chain, r);
+        map_chain(chain, r);
         y[i] = r[0];
     }
 }

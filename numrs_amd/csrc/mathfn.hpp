@@ -1,0 +1,269 @@
+// Elementary functions in FP64 for the elementwise map: exp, exp2, expm1, log, log2, log10, log1p
+// and tanh (the reference's ufuncs.rs:398-475 and :639, which call Rust's f64 methods below 64
+// elements and a SIMD library above). Each is written here from a textbook method, named per
+// function; every constant is a mathematical constant (ln 2 and its relatives, split so that
+// products with small integers are exact, the Taylor coefficients 1/n! and 2/(2j+1)), derived
+// with 80-digit decimal arithmetic. Binary32 inputs are evaluated in FP64 and rounded once.
+//
+// Accuracy (measured against correctly rounded results, tests/test_gpu_ufuncs.py): within 1 ulp
+// for exp, exp2, expm1, log, log2, log10 and log1p, within 2 ulp for tanh; exact special values
+// (NaN, +-inf, +-0, the overflow and underflow edges).
+#pragma once
+
+namespace nrk {
+namespace fm {
+
+// ln 2 = LN2_HI + LN2_LO, LN2_HI with 32 significant bits: k * LN2_HI is exact for |k| < 2^21.
+constexpr double LN2 = 0x1.62e42fefa39efp-1;
+constexpr double LN2_TAIL = 0x1.abc9e3b39803fp-56;  // ln 2 - LN2
+constexpr double LN2_HI = 0x1.62e42fee00000p-1;
+constexpr double LN2_LO = 0x1.a39ef35793c76p-33;
+constexpr double INV_LN2 = 0x1.71547652b82fep+0;
+constexpr double INV_LN2_HI = 0x1.7154765200000p+0;
+constexpr double INV_LN2_LO = 0x1.705fc2eefa200p-33;
+constexpr double INV_LN10_HI = 0x1.bcb7b15200000p-2;
+constexpr double INV_LN10_LO = 0x1.b9438ca9aadd5p-36;
+constexpr double LOG10_2_HI = 0x1.3441350800000p-2;  // log10(2), 32 bits
+constexpr double LOG10_2_LO = 0x1.f79fef311f12bp-34;
+constexpr double SQRT_HALF = 0x1.6a09e667f3bcdp-1;
+
+__device__ __forceinline__ double dinf() { return __builtin_inf(); }
+
+// e^r - 1 for |r| <= 0.35: Taylor series to degree 13 (the first omitted term is below 2^-58
+// relative), as r + r^2 (1/2! + r/3! + ... + r^11/13!) so that the leading term is exact.
+__device__ __forceinline__ double expm1_kernel(double r) {
+    double q = 0x1.6124613a86d09p-33;  // 1/13!
+    q = fma(q, r, 0x1.1eed8eff8d898p-29);  // 1/12!
+    q = fma(q, r, 0x1.ae64567f544e4p-26);  // 1/11!
+    q = fma(q, r, 0x1.27e4fb7789f5cp-22);  // 1/10!
+    q = fma(q, r, 0x1.71de3a556c734p-19);  // 1/9!
+    q = fma(q, r, 0x1.a01a01a01a01ap-16);  // 1/8!
+    q = fma(q, r, 0x1.a01a01a01a01ap-13);  // 1/7!
+    q = fma(q, r, 0x1.6c16c16c16c17p-10);  // 1/6!
+    q = fma(q, r, 0x1.1111111111111p-7);   // 1/5!
+    q = fma(q, r, 0x1.5555555555555p-5);   // 1/4!
+    q = fma(q, r, 0x1.5555555555555p-3);   // 1/3!
+    q = fma(q, r, 0.5);                    // 1/2!
+    return fma(r * r, q, r);
+}
+
+// e^x - 1 for |x| <= 0.7: Taylor series to degree 17 (first omitted term below 2^-61 relative),
+// as x + x^2 (1/2! + x/3! + ... + x^15/17!).
+__device__ __forceinline__ double expm1_series(double x) {
+    double q = 0x1.952c77030ad4ap-49;      // 1/17!
+    q = fma(q, x, 0x1.ae7f3e733b81fp-45);  // 1/16!
+    q = fma(q, x, 0x1.ae7f3e733b81fp-41);  // 1/15!
+    q = fma(q, x, 0x1.93974a8c07c9dp-37);  // 1/14!
+    q = fma(q, x, 0x1.6124613a86d09p-33);  // 1/13!
+    q = fma(q, x, 0x1.1eed8eff8d898p-29);  // 1/12!
+    q = fma(q, x, 0x1.ae64567f544e4p-26);  // 1/11!
+    q = fma(q, x, 0x1.27e4fb7789f5cp-22);  // 1/10!
+    q = fma(q, x, 0x1.71de3a556c734p-19);  // 1/9!
+    q = fma(q, x, 0x1.a01a01a01a01ap-16);  // 1/8!
+    q = fma(q, x, 0x1.a01a01a01a01ap-13);  // 1/7!
+    q = fma(q, x, 0x1.6c16c16c16c17p-10);  // 1/6!
+    q = fma(q, x, 0x1.1111111111111p-7);   // 1/5!
+    q = fma(q, x, 0x1.5555555555555p-5);   // 1/4!
+    q = fma(q, x, 0x1.5555555555555p-3);   // 1/3!
+    q = fma(q, x, 0.5);                    // 1/2!
+    return fma(x * x, q, x);
+}
+
+// y 2^k for k in [-1100, 1100] without an intermediate overflow or a premature underflow.
+__device__ __forceinline__ double scale2(double y, int k) {
+    const int h = k / 2;
+    return ldexp(ldexp(y, h), k - h);
+}
+
+// e^(rh + rl) for |rh| <= 0.35, |rl| << ulp(rh): 1 + s + rl (1 + s) with s = e^rh - 1, the
+// rounding error of 1 + s and the rl term added before the one final rounding.
+__device__ __forceinline__ double exp_dd(double rh, double rl) {
+    const double s = expm1_kernel(rh);
+    const double h = 1.0 + s;
+    const double e = (1.0 - h) + s;  // 1 + s = h + e exactly (|s| < 1)
+    return h + fma(rl, h, e);
+}
+
+// e^x: Cody & Waite's reduction x = k ln2 + r (|r| <= ln2 / 2) through the split ln 2, with the
+// rounding error of r carried as rl, then the scaling by 2^k.
+__device__ inline double exp(double x) {
+    if (x != x) return x;
+    if (x > 710.0) return dinf();
+    if (x < -746.0) return 0.0;
+    const double k = rint(x * INV_LN2);
+    const double r1 = fma(-k, LN2_HI, x);  // exact
+    const double r = fma(-k, LN2_LO, r1);
+    const double rl = fma(-k, LN2_LO, r1 - r);
+    return scale2(exp_dd(r, rl), (int)k);
+}
+
+// 2^x = 2^k e^(f ln2), k = rint(x), f = x - k exact.
+__device__ inline double exp2(double x) {
+    if (x != x) return x;
+    if (x > 1025.0) return dinf();
+    if (x < -1076.0) return 0.0;
+    const double k = rint(x);
+    const double f = x - k;
+    const double rh = f * LN2;
+    const double rl = fma(f, LN2, -rh) + f * LN2_TAIL;
+    return scale2(exp_dd(rh, rl), (int)k);
+}
+
+// e^x - 1: the series directly for |x| <= 0.7; otherwise 2^k (1 + s) - 1 with s = e^r - 1,
+// formed as one FMA fma(2^k, s, 2^k - 1) where 2^k - 1 is exact (-53 <= k <= 53), else scaled
+// and then 1 subtracted. (Below 0.7, k = 1 would double the error of s in 2 s + 1.)
+__device__ inline double expm1(double x) {
+    if (x != x || x == 0.0) return x;  // NaN, and +-0 keeping its sign
+    if (x > 710.0) return dinf();
+    if (x < -38.0) return -1.0;  // e^x < 2^-54: -1 + e^x rounds to -1
+    if (__builtin_fabs(x) <= 0.7) return expm1_series(x);
+    const double k = rint(x * INV_LN2);
+    const double r1 = fma(-k, LN2_HI, x);  // exact
+    const double r = fma(-k, LN2_LO, r1);
+    const int ki = (int)k;
+    if (ki >= -53 && ki <= 53) {
+        const double p = ldexp(1.0, ki);
+        return fma(p, expm1_kernel(r), p - 1.0);
+    }
+    // 2^k (h + e) - 1 with h + e = e^r exactly enough: 2^k h - 1 carried exactly, one rounding
+    const double s = expm1_kernel(r);
+    const double h = 1.0 + s;
+    const double e = fma(fma(-k, LN2_LO, r1 - r), h, (1.0 - h) + s);
+    if (ki > 1000) return scale2(h + e, ki);  // the 1 is far below an ulp
+    const double ph = ldexp(h, ki), pe = ldexp(e, ki);  // exact
+    const double sm = ph - 1.0;
+    const double bb = sm - ph;
+    return sm + (((ph - (sm - bb)) + (-1.0 - bb)) + pe);  // TwoSum(ph, -1) + pe
+}
+
+// log(m) for m in [sqrt(1/2), sqrt(2)): with f = m - 1 (exact) and s = f / (2 + f),
+// log(m) = 2 atanh(s) = 2s + 2s^3/3 + 2s^5/5 + ..., rearranged as f - (f^2/2 - s (f^2/2 + R))
+// with R = sum_{j>=1} 2 s^(2j) / (2j + 1) (|s| < 0.1716: 13 terms reach 2^-63 relative).
+// Returned as hi + lo: the last subtraction is carried exactly (|f| > |f^2/2 - ...|).
+__device__ __forceinline__ void log_mant(double f, double &hi, double &lo) {
+    const double s = f / (2.0 + f);
+    const double z = s * s;
+    double R = 0x1.2f684bda12f68p-4;  // 2/27
+    R = fma(R, z, 0x1.47ae147ae147bp-4);  // 2/25
+    R = fma(R, z, 0x1.642c8590b2164p-4);  // 2/23
+    R = fma(R, z, 0x1.8618618618618p-4);  // 2/21
+    R = fma(R, z, 0x1.af286bca1af28p-4);  // 2/19
+    R = fma(R, z, 0x1.e1e1e1e1e1e1ep-4);  // 2/17
+    R = fma(R, z, 0x1.1111111111111p-3);  // 2/15
+    R = fma(R, z, 0x1.3b13b13b13b14p-3);  // 2/13
+    R = fma(R, z, 0x1.745d1745d1746p-3);  // 2/11
+    R = fma(R, z, 0x1.c71c71c71c71cp-3);  // 2/9
+    R = fma(R, z, 0x1.2492492492492p-2);  // 2/7
+    R = fma(R, z, 0x1.999999999999ap-2);  // 2/5
+    R = fma(R, z, 0x1.5555555555555p-1);  // 2/3
+    R *= z;
+    const double hfsq = 0.5 * f * f;
+    const double X = hfsq - s * (hfsq + R);
+    hi = f - X;
+    lo = (f - hi) - X;  // Fast2Sum: exact
+}
+
+// x = 2^e m with m in [sqrt(1/2), sqrt(2)); returns false (and the IEEE result in *special)
+// for x <= 0, +inf and NaN. Subnormals are scaled by 2^54 first.
+__device__ __forceinline__ bool log_split(double x, int &e, double &f, double &special) {
+    if (!(x > 0.0) || x == dinf()) {
+        special = x == 0.0 ? -dinf() : (x == dinf() ? x : __builtin_nan(""));
+        if (x != x) special = x;
+        return false;
+    }
+    int adj = 0;
+    if (x < 0x1p-1022) {
+        x *= 0x1p54;
+        adj = 54;
+    }
+    double m = frexp(x, &e);  // m in [0.5, 1)
+    if (m < SQRT_HALF) {
+        m *= 2.0;
+        e -= 1;
+    }
+    e -= adj;
+    f = m - 1.0;  // exact (Sterbenz)
+    return true;
+}
+
+// a + b + c with a + b carried exactly (TwoSum), the small terms added last.
+__device__ __forceinline__ double sum3(double a, double b, double c) {
+    const double s = a + b;
+    const double bb = s - a;
+    const double t = (a - (s - bb)) + (b - bb);
+    return s + (t + c);
+}
+
+// log x = e ln2 + log(m) (+ c, a small correction folded in before the last rounding).
+__device__ __forceinline__ double log_plus(double x, double c, bool &special_case) {
+    int e;
+    double f, sp;
+    special_case = !log_split(x, e, f, sp);
+    if (special_case) return sp;
+    double hi, lo;
+    log_mant(f, hi, lo);
+    const double de = (double)e;
+    return sum3(de * LN2_HI, hi, lo + fma(de, LN2_LO, c));  // e LN2_HI is exact
+}
+__device__ inline double log(double x) {
+    bool sp;
+    return log_plus(x, 0.0, sp);
+}
+
+// log2 x = e + log(m) / ln2 and log10 x = e log10(2) + log(m) / ln10, the products with the
+// constants carried in two parts.
+__device__ inline double log2(double x) {
+    int e;
+    double f, sp;
+    if (!log_split(x, e, f, sp)) return sp;
+    double hi, lo;
+    log_mant(f, hi, lo);
+    const double p = hi * INV_LN2_HI;
+    const double plo = fma(hi, INV_LN2_HI, -p) + fma(hi, INV_LN2_LO, lo * INV_LN2);
+    return sum3((double)e, p, plo);
+}
+__device__ inline double log10(double x) {
+    int e;
+    double f, sp;
+    if (!log_split(x, e, f, sp)) return sp;
+    double hi, lo;
+    log_mant(f, hi, lo);
+    const double de = (double)e;
+    const double b = hi * INV_LN10_HI;
+    const double blo = fma(hi, INV_LN10_HI, -b) + fma(hi, INV_LN10_LO, fma(lo, INV_LN10_HI, de * LOG10_2_LO));
+    return sum3(de * LOG10_2_HI, b, blo);  // e LOG10_2_HI is exact
+}
+
+// log(1 + x): u = 1 + x rounded, then log(u) plus the first-order correction for that rounding,
+// (x - (u - 1)) / u (Goldberg, "What every computer scientist should know about floating-point
+// arithmetic", 1991).
+__device__ inline double log1p(double x) {
+    if (x != x) return x;
+    if (x == dinf()) return x;
+    if (x < -1.0) return __builtin_nan("");
+    if (x == -1.0) return -dinf();
+    if (__builtin_fabs(x) < 0x1p-54) return x;
+    const double u = 1.0 + x;
+    if (u == 1.0) return x;
+    const double c = u < 0x1p53 ? (x - (u - 1.0)) / u : 0.0;
+    bool sp;
+    return log_plus(u, c, sp);
+}
+
+// tanh x = t / (t + 2) with t = e^(2|x|) - 1, sign restored; +-1 beyond |x| = 22, x below 2^-28.
+// The divisor t + 2 is carried as d + dl exactly and the quotient corrected once by its residual.
+__device__ inline double tanh(double x) {
+    if (x != x) return x;
+    const double ax = __builtin_fabs(x);
+    if (ax > 22.0) return __builtin_copysign(1.0, x);
+    if (ax < 0x1p-28) return x;
+    const double t = expm1(2.0 * ax);
+    const double d = t + 2.0;
+    const double dl = t < 2.0 ? (2.0 - d) + t : (t - d) + 2.0;  // Fast2Sum, larger operand first
+    const double q = t / d;
+    const double res = fma(-q, d, t) - q * dl;  // t - q (d + dl)
+    return __builtin_copysign(q + res / d, x);
+}
+
+}  // namespace fm
+}  // namespace nrk

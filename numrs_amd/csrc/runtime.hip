@@ -164,7 +164,7 @@ nrk_status staging(int slot, size_t bytes, void **ptr) {
     return NRK_OK;
 }
 
-nrk_status make_chain(const nrk_scalar_op *chain, int32_t chain_len, Chain *out) {
+nrk_status make_chain(const nrk_scalar_op *chain, int32_t chain_len, Chain *out, bool allow_math) {
     if (chain_len < 0 || chain_len > NRK_MAX_CHAIN) return fail(NRK_ERR_INVALID_ARGUMENT, "chain_len %d outside [0,%d]", chain_len, NRK_MAX_CHAIN);
     if (chain_len > 0 && chain == nullptr) return fail(NRK_ERR_INVALID_ARGUMENT, "chain is NULL");
     out->len = chain_len;
@@ -174,6 +174,8 @@ nrk_status make_chain(const nrk_scalar_op *chain, int32_t chain_len, Chain *out)
     }
     for (int i = 0; i < chain_len; ++i) {
         if (chain[i].op < 0 || chain[i].op >= NRK_SOP_COUNT) return fail(NRK_ERR_INVALID_ARGUMENT, "chain[%d].op = %d is not an NRK_SOP_* code", i, chain[i].op);
+        if (!allow_math && chain[i].op >= NRK_SOP_EXP)
+            return fail(NRK_ERR_UNSUPPORTED, "chain[%d].op = %d (an elementary function) runs in the elementwise map only", i, chain[i].op);
         out->op[i] = chain[i].op;
         out->s[i] = chain[i].scalar;
     }

@@ -22,7 +22,9 @@ import numpy as np
 from ._lib import ScalarOp, call
 
 # NRK_SOP_* / NRK_BOP_* / NRK_AXIS_* codes of include/numrs_amd.h
-SOP = {"add": 0, "sub": 1, "mul": 2, "div": 3, "rsub": 4, "rdiv": 5, "neg": 6, "abs": 7, "square": 8, "sqrt": 9}
+SOP = {"add": 0, "sub": 1, "mul": 2, "div": 3, "rsub": 4, "rdiv": 5, "neg": 6, "abs": 7, "square": 8, "sqrt": 9,
+       "floor": 10, "ceil": 11, "trunc": 12, "round": 13, "fract": 14, "sign": 15, "clamp_lo": 16, "clamp_hi": 17,
+       "exp": 18, "exp2": 19, "expm1": 20, "log": 21, "log2": 22, "log10": 23, "log1p": 24, "tanh": 25}
 BOP = {"add": 0, "sub": 1, "mul": 2, "div": 3, "rem": 4, "maximum": 5, "minimum": 6}
 AXIS = {"sum": 0, "mean": 1, "var": 2, "min": 3, "max": 4, "argmin": 5, "argmax": 6}
 
@@ -121,6 +123,61 @@ def unary_dispatch(a, steps) -> np.ndarray:
     fn = {np.dtype(np.float64): "nrk_map_f64", np.dtype(np.float32): "nrk_map_f32"}[a.dtype]
     call(fn, C.cast(ops, C.c_void_p), C.c_int32(n_ops), _ptr(a), _ptr(out), C.c_size_t(a.size))
     return out
+
+
+# ---- ufuncs.rs unary functions (f64 in the reference; f32 arrays are accepted too) as op chains.
+# The Rust expression each mirrors is quoted; chains of IEEE steps round exactly like it.
+
+def _ufunc(a, *steps) -> np.ndarray:
+    a = np.asarray(a)
+    if a.dtype not in (np.float32, np.float64):
+        a = a.astype(np.float64)
+    return unary_dispatch(a, list(steps)).reshape(a.shape)
+
+
+def exp(a): return _ufunc(a, ("exp",))                       # ufuncs.rs:398 x.exp()
+def exp2(a): return _ufunc(a, ("exp2",))                     # :408 x.exp2()
+def expm1(a): return _ufunc(a, ("expm1",))                   # :418 x.exp_m1()
+def log(a): return _ufunc(a, ("log",))                       # :428 x.ln()
+def ln(a): return log(a)                                     # :438
+def log2(a): return _ufunc(a, ("log2",))                     # :443 x.log2()
+def log10(a): return _ufunc(a, ("log10",))                   # :453 x.log10()
+def log1p(a): return _ufunc(a, ("log1p",))                   # :463 x.ln_1p()
+def tanh(a): return _ufunc(a, ("tanh",))                     # :639 x.tanh()
+def sigmoid(a): return _ufunc(a, ("neg",), ("exp",), ("add", 1.0), ("rdiv", 1.0))  # :1235 1 / (1 + (-x).exp())
+def reciprocal(a): return _ufunc(a, ("rdiv", 1.0))           # :374 1.0 / x
+def rsqrt(a): return _ufunc(a, ("sqrt",), ("rdiv", 1.0))     # :384 1.0 / x.sqrt()
+def sqrt(a): return _ufunc(a, ("sqrt",))                     # :354 x.sqrt()
+def square(a): return _ufunc(a, ("square",))                 # :344 x * x
+def negative(a): return _ufunc(a, ("neg",))                  # :319 -x
+def absolute(a): return _ufunc(a, ("abs",))                  # :329 x.abs()
+def floor(a): return _ufunc(a, ("floor",))                   # :702 x.floor()
+def ceil(a): return _ufunc(a, ("ceil",))                     # :712 x.ceil()
+def round(a): return _ufunc(a, ("round",))                   # :722 x.round(), half away from zero  # noqa: A001
+def trunc(a): return _ufunc(a, ("trunc",))                   # :732 x.trunc()
+def fract(a): return _ufunc(a, ("fract",))                   # :742 x - x.trunc()
+def sign(a): return _ufunc(a, ("sign",))                     # :756 0 / 1 / -1 (NaN -> -1)
+
+
+def deg2rad(a):
+    """ufuncs.rs:807 x.to_radians() = x * (pi / 180): one multiply by the rounded constant."""
+    return _ufunc(a, ("mul", float(np.pi) / 180.0))
+
+
+def rad2deg(a):
+    """ufuncs.rs:822 x.to_degrees() = x * (180 / pi): one multiply by the rounded constant."""
+    return _ufunc(a, ("mul", 180.0 / float(np.pi)))
+
+
+radians, degrees = deg2rad, rad2deg
+
+
+def clip(a, lo: float, hi: float):
+    """ufuncs.rs:774 x.clamp(min, max): NaN stays NaN; Rust panics when min > max or either bound
+    is NaN, which raises ValueError here."""
+    if not (lo <= hi):
+        raise ValueError(f"clip bounds must satisfy min <= max, got {lo}, {hi}")
+    return _ufunc(a, ("clamp_lo", lo), ("clamp_hi", hi))
 
 
 def binary_f64(a, b, op: str) -> np.ndarray:

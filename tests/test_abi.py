@@ -56,3 +56,18 @@ def test_product_package_has_no_oracle_or_cpu_fallback():
         assert "import oracle" not in text and "from oracle" not in text, f
     for f in (pkg / "csrc").iterdir():
         assert "oracle" not in f.read_text().lower(), f
+
+
+def test_fused_chain_refuses_elementary_functions():
+    """The fused map -> reduce kernels are built without the elementary functions (exp .. tanh):
+    such a chain is refused with NRK_ERR_UNSUPPORTED before any device work (no GPU needed)."""
+    import ctypes as C
+
+    from numrs_amd import kernels as K
+    from numrs_amd._lib import NrkError, ReduceOutputs, call
+
+    ops, n = K.chain(("mul", 2.0), ("exp",))
+    ro, fo = ReduceOutputs(), ReduceOutputs()
+    with pytest.raises(NrkError) as e:
+        call("nrk_dev_map_reduce_f64", C.cast(ops, C.c_void_p), n, C.c_void_p(16), 4, 4, 0, C.byref(ro), C.byref(fo), None)
+    assert "elementwise map only" in str(e.value)

@@ -1,0 +1,184 @@
+"""Elementwise ufuncs (ufuncs.rs:319-1240) through the op-coded map, on the GPU.
+
+The reference's exp / log / tanh ... are Rust's f64 methods below 64 elements and a SIMD library
+above (absent here); neither is bit-reproducible, so the bar is the mathematics: the error
+against the exact value, computed in 80-bit long double (64-bit significand, 11 bits more than
+binary64) on the CPU, measured in units of the last place of the binary64 result. Bounds, written
+here: exp, exp2, expm1, log, log1p <= 1 ulp; log2, log10 <= 1 ulp; tanh <= 2 ulp; sigmoid
+(the reference's 1 / (1 + exp(-x)): exp's relative error passes through two more IEEE roundings,
+and a relative error converts to up to twice as many ulps at a binade edge) <= 3 ulp; binary32
+inputs (FP64 evaluation, one rounding) <= 1 ulp of binary32. Special values (NaN, +-inf, +-0, overflow and underflow edges)
+must match exactly.
+
+Exact ops are bit-exact against an independent computation: floor / ceil / trunc (IEEE), round
+(half away from zero: Python's decimal ROUND_HALF_UP), fract (math.modf; Rust's x - x.trunc()
+gives NaN for +-inf), sign (ufuncs.rs:756), clamp, and the single IEEE steps of reciprocal,
+rsqrt, deg2rad and rad2deg."""
+import decimal
+import math
+
+import numpy as np
+import pytest
+
+from numrs_amd import kernels as K
+
+pytestmark = pytest.mark.gpu
+LD = np.longdouble
+
+
+def ulp_err(got, exact_ld, dtype=np.float64):
+    """|got - exact| in ulps of the correctly rounded result (0 where both are the same special)."""
+    got = np.asarray(got, dtype)
+    ref = exact_ld.astype(dtype)
+    err = np.zeros(got.shape, np.float64)
+    fin = np.isfinite(ref) & np.isfinite(got)
+    tiny = np.finfo(dtype).smallest_subnormal
+    sp = np.maximum(np.spacing(np.abs(ref[fin])).astype(LD), LD(tiny))
+    err[fin] = (np.abs(got[fin].astype(LD) - exact_ld[fin]) / sp).astype(np.float64)
+    bad = ~fin & ~((np.isnan(ref) & np.isnan(got)) | (ref == got))
+    err[bad] = np.inf
+    return err
+
+
+def sample(lo, hi, n, seed):
+    r = np.random.default_rng(seed)
+    return r.uniform(lo, hi, n)
+
+
+SPECIAL = np.array([np.nan, np.inf, -np.inf, 0.0, -0.0, 1.0, -1.0, 5e-324, -5e-324, 2.2250738585072014e-308,
+                    1e-300, -1e-300, 1e300, -1e300, 0.5, 2.0])
+
+CASES = {
+    # name: (gpu fn, long double fn, inputs, bound)
+    "exp": (K.exp, np.exp, np.concatenate([sample(-745.2, 709.8, 60000, 1), sample(-1e-3, 1e-3, 5000, 2),
+                                          [709.782712893383, 709.7827128933841, -745.1332191019411, -745.14,
+                                           -708.39, -720.5, 1e-18, -1e-18], SPECIAL]), 1.0),
+    "exp2": (K.exp2, np.exp2, np.concatenate([sample(-1075.5, 1024.5, 60000, 3), sample(-1, 1, 5000, 4),
+                                             [1023.9999, 1024.0, -1074.0, -1074.5, -1022.5, 0.25], SPECIAL]), 1.0),
+    "expm1": (K.expm1, np.expm1, np.concatenate([sample(-45, 709.7, 40000, 5), sample(-0.4, 0.4, 20000, 6),
+                                                sample(-1e-12, 1e-12, 5000, 7), [-36.7, -37.4, -38.0, 0.3465, -0.3466,
+                                                                                 0.6931, 36.0, 37.0], SPECIAL]), 1.0),
+    "log": (K.log, np.log, np.concatenate([np.exp(sample(-744, 709, 40000, 8)), 1.0 + sample(-1e-3, 1e-3, 10000, 9),
+                                          sample(0.5, 2.0, 10000, 10), [3e-320, 1e-310, 0.7071067811865476,
+                                                                       1.4142135623730951], SPECIAL]), 1.0),
+    "log2": (K.log2, np.log2, np.concatenate([np.exp(sample(-744, 709, 40000, 11)), 1.0 + sample(-1e-3, 1e-3, 10000, 12),
+                                             2.0 ** np.arange(-1074, 1024, 7.0), SPECIAL]), 1.0),
+    "log10": (K.log10, np.log10, np.concatenate([np.exp(sample(-744, 709, 40000, 13)), 1.0 + sample(-1e-3, 1e-3, 10000, 14),
+                                                10.0 ** np.arange(-300, 300, 3.0), SPECIAL]), 1.0),
+    "log1p": (K.log1p, np.log1p, np.concatenate([sample(-1, 1, 30000, 15), np.exp(sample(-700, 700, 20000, 16)),
+                                                sample(-1e-10, 1e-10, 5000, 17), [-0.999999999999, -0.5, 1e-17, -1e-17,
+                                                                                  -2.0], SPECIAL]), 1.0),
+    "tanh": (K.tanh, np.tanh, np.concatenate([sample(-25, 25, 40000, 18), sample(-1, 1, 20000, 19),
+                                             sample(-1e-9, 1e-9, 5000, 20), [22.0, -22.0, 21.99, 0.5493, 3.7e-9], SPECIAL]), 2.0),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_elementary_f64_within_ulp_bound(gpu, name):
+    fn, ref_fn, x, bound = CASES[name]
+    x = np.asarray(x, np.float64)
+    with np.errstate(all="ignore"):
+        exact = ref_fn(x.astype(LD))
+    got = fn(x)
+    err = ulp_err(got, exact)
+    worst = int(np.argmax(err))
+    print(f"{name}: max {err.max():.3f} ulp at x = {x[worst]!r} (got {got[worst]!r}, exact {exact[worst]!r})")
+    assert err.max() <= bound, (name, x[worst], got[worst], exact[worst])
+    # special values exactly, sign of zero included
+    with np.errstate(all="ignore"):
+        ref_sp = ref_fn(SPECIAL.astype(LD)).astype(np.float64)
+    got_sp = fn(SPECIAL)
+    same = (np.isnan(ref_sp) & np.isnan(got_sp)) | ((ref_sp == got_sp) & (np.signbit(ref_sp) == np.signbit(got_sp)))
+    ok = same | (np.isfinite(ref_sp) & (ulp_err(got_sp, ref_fn(SPECIAL.astype(LD))) <= bound))
+    assert np.all(ok), (name, SPECIAL[~ok], got_sp[~ok], ref_sp[~ok])
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_elementary_f32_within_one_ulp(gpu, name):
+    fn, ref_fn, x, _ = CASES[name]
+    x32 = np.asarray(x, np.float64).astype(np.float32)
+    with np.errstate(all="ignore"):
+        exact = ref_fn(x32.astype(LD))
+    got = fn(x32)
+    assert got.dtype == np.float32
+    err = ulp_err(got, exact, np.float32)
+    print(f"{name} f32: max {err.max():.3f} ulp")
+    assert err.max() <= 1.0
+
+
+def test_sigmoid_within_three_ulp(gpu):
+    """ufuncs.rs:1235 computes 1 / (1 + (-x).exp()) in f64: below x = -709.78 (-x).exp() overflows
+    and the result is exactly 0, which the GPU reproduces; elsewhere within 3 ulp of the exact value."""
+    x = np.concatenate([sample(-40, 40, 40000, 21), sample(-709.7, 745, 5000, 22), [0.0, -0.0, np.inf, np.nan]])
+    with np.errstate(all="ignore"):
+        exact = LD(1) / (LD(1) + np.exp(-x.astype(LD)))
+    err = ulp_err(K.sigmoid(x), exact)
+    print(f"sigmoid: max {err.max():.3f} ulp")
+    assert err.max() <= 3.0
+    under = np.array([-709.8, -720.0, -745.5, -1e300, -np.inf])
+    np.testing.assert_array_equal(K.sigmoid(under), np.zeros(5))
+
+
+def bits(a):
+    a = np.asarray(a)
+    return a.view(np.uint64 if a.dtype == np.float64 else np.uint32)
+
+
+def assert_same(got, want):
+    """Bit-identical, except that any NaN matches any NaN (the sign of a NaN produced by an
+    invalid operation differs between x86 and the GPU)."""
+    got, want = np.asarray(got), np.asarray(want)
+    assert got.dtype == want.dtype
+    np.testing.assert_array_equal(np.isnan(got), np.isnan(want))
+    ok = ~np.isnan(want)
+    np.testing.assert_array_equal(bits(got[ok]), bits(want[ok]))
+
+
+EXACT_X = np.concatenate([np.array([0.5, 1.5, 2.5, -0.5, -1.5, -2.5, 0.49999999999999994, -0.49999999999999994,
+                                    4503599627370495.5, -4503599627370495.5, 4503599627370497.0, 1e300, -1e300,
+                                    0.0, -0.0, 5e-324, -5e-324, np.inf, -np.inf, np.nan, 3.0, -3.0, 0.3, -0.7]),
+                          sample(-1e6, 1e6, 20000, 30), sample(-4, 4, 20000, 31)])
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_rounding_ops_bit_exact(gpu, dtype):
+    x = EXACT_X.astype(dtype)
+    assert_same(K.floor(x), np.floor(x))
+    assert_same(K.ceil(x), np.ceil(x))
+    assert_same(K.trunc(x), np.trunc(x))
+    # Rust's round: half away from zero (decimal ROUND_HALF_UP), sign of zero kept
+    want = np.array([v if not math.isfinite(v) else
+                     math.copysign(float(decimal.Decimal(float(v)).to_integral_value(rounding=decimal.ROUND_HALF_UP)), v)
+                     for v in x.astype(np.float64)]).astype(dtype)
+    assert_same(K.round(x), want)
+    # Rust's fract = x - x.trunc(): the C fractional part for finite x, but +0.0 (not modf's -0.0)
+    # for integral x, as x - x is +0 in IEEE arithmetic; NaN for +-inf
+    with np.errstate(invalid="ignore"):
+        want = np.array([math.modf(float(v))[0] + 0.0 if math.isfinite(v) else math.nan
+                         for v in x.astype(np.float64)]).astype(dtype)
+    assert_same(K.fract(x), want)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_sign_clip_and_ieee_steps_bit_exact(gpu, dtype):
+    x = EXACT_X.astype(dtype)
+    # ufuncs.rs:756: 0.0 for +-0, 1 above zero, -1 otherwise (NaN included)
+    want = np.array([0.0 if v == 0 else (1.0 if v > 0 else -1.0) for v in x.astype(np.float64)]).astype(dtype)
+    assert_same(K.sign(x), want)
+    # clamp(-2.5, 7.0): NaN passes, -0.0 stays -0.0
+    got = K.clip(x, -2.5, 7.0)
+    want = np.array([v if v != v else (-2.5 if v < -2.5 else (7.0 if v > 7.0 else v)) for v in x.astype(np.float64)]).astype(dtype)
+    assert_same(got, want)
+    with pytest.raises(ValueError):
+        K.clip(x, 3.0, 1.0)
+    with np.errstate(all="ignore"):
+        assert_same(K.reciprocal(x), dtype(1) / x)
+        assert_same(K.rsqrt(x), dtype(1) / np.sqrt(x))
+    assert_same(K.deg2rad(x), x * dtype(np.pi / 180.0))
+    assert_same(K.rad2deg(x), x * dtype(180.0 / np.pi))
+
+
+def test_elementary_steps_compose_in_one_chain(gpu):
+    """log1p(exp(x)) as one two-step chain equals the two single-step maps, bit for bit."""
+    x = sample(-30, 30, 10000, 40)
+    assert_same(K.unary_dispatch(x, [("exp",), ("log1p",)]), K.log1p(K.exp(x)))
