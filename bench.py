@@ -58,6 +58,7 @@ def parse():
     p.add_argument("--no-gemm", action="store_true")
     p.add_argument("--no-ragged", action="store_true")
     p.add_argument("--no-batched", action="store_true")
+    p.add_argument("--no-ufuncs", action="store_true")
     p.add_argument("--pmc-pass", action="store_true",
                    help="run each hot call once after a plain copy of known size (2 GiB, the counter "
                         "calibration), for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE; prints no JSON line")
@@ -217,6 +218,7 @@ def main() -> int:
     gemm = None if args.no_gemm else bench_gemm(device, rng, call, s, stream)
     ragged = None if args.no_ragged else bench_ragged(device, rng, call, s, stream)
     batched = None if args.no_batched else bench_batched(device, rng, call, s, stream)
+    ufuncs = None if args.no_ufuncs else bench_ufuncs(device, kernels, call, s, stream, d32, d64)
     cpu = None if args.no_cpu_baseline else cpu_baseline(x32, d64, xi, step_bytes)
 
     line = {
@@ -244,6 +246,7 @@ def main() -> int:
         "gemm": gemm,
         "ragged": ragged,
         "batched": batched,
+        "ufuncs": ufuncs,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
@@ -328,6 +331,40 @@ def bench_gemm(device, rng, call, s, stream):
         del dA, dB, dC, A, B
     out["error_measure"] = ("max over 24 sampled entries of |C - C*| / sum_k |A_ik||B_kj|, C* = correctly rounded "
                             "dot product (Dekker products + fsum); inputs sign*2^e*u, e in [-10,10], u in [1,2)")
+    return out
+
+
+# ---------------------------------------------------------------------------------------------
+# elementwise ufuncs (ufuncs.rs exp / log / tanh ...) over the resident 8192x8192 inputs
+
+def bench_ufuncs(device, kernels, call, s, stream, d32, d64):
+    t0 = time.time()
+    n = ROWS * COLS
+    out = {"workload": f"y = f(x) over x ({ROWS},{COLS}) f32 and f64, N(0,1); bytes = read x + write y",
+           "hbm_peak_GB/s": HBM_PEAK_GBS}
+    for dt, dx, tag in ((np.float64, d64, "f64"), (np.float32, d32, "f32")):
+        dy = device.DeviceArray((n,), dt)
+        fn = f"nrk_dev_map_{tag}"
+        row = {}
+        for name, steps in (("copy", ()), ("exp", (("exp",),)), ("log", (("abs",), ("log",))), ("tanh", (("tanh",),)),
+                            ("sigmoid", (("neg",), ("exp",), ("add", 1.0), ("rdiv", 1.0))), ("floor", (("floor",),))):
+            ops, k = kernels.chain(*steps)
+            f = lambda: call(fn, C.cast(ops, C.c_void_p), k, C.c_void_p(dx.ptr), C.c_void_p(dy.ptr), C.c_size_t(n), C.c_void_p(s))
+            f()
+            stream.synchronize()
+            a, b = device.Event(), device.Event()
+            a.record(s)
+            for _ in range(10):
+                f()
+            b.record(s)
+            stream.synchronize()
+            ms = a.elapsed_ms(b) / 10
+            gbs = 2 * n * np.dtype(dt).itemsize / ms / 1e6
+            row[name] = {"ms": round(ms, 4), "GB/s": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 3)}
+        out[tag] = row
+        log(f"ufuncs {tag}: " + ", ".join(f"{k} {v['ms']:.3f} ms ({v['GB/s']:.0f} GB/s)" for k, v in row.items()))
+        del dy
+    out["seconds"] = round(time.time() - t0, 1)
     return out
 
 

@@ -14,7 +14,7 @@
 namespace nrk {
 
 constexpr int EB = 256;  // threads per workgroup
-constexpr int EUNROLL = 4;
+constexpr int EUNROLL = 2;  // 16-byte vectors in flight per thread and pass
 
 template <typename T, int OP>
 __device__ __forceinline__ T apply_bop(T a, T b) {
@@ -121,9 +121,12 @@ __global__ __launch_bounds__(EB) void zip_kernel(const T *a, const T *b, T *y, s
     for (size_t i = done + tid; i < n; i += nthreads) y[i] = apply_bop<T, OP>(a[i], b[i]);
 }
 
+// Up to 64 workgroups per CU: many short-lived workgroups keep more loads in flight than a
+// grid-stride loop over 16 per CU (measured, 8192 x 8192: copy 4.7 -> 5.6 TB/s f64 and 4.6 -> 6.0
+// TB/s f32; exp f64 4.4 -> 5.4 TB/s).
 static int stream_grid(size_t n_vec) {
     size_t want = ceil_div(n_vec, (size_t)EB * EUNROLL);
-    size_t cap = (size_t)cu_count() * 16;
+    size_t cap = (size_t)cu_count() * 64;
     if (want < 1) want = 1;
     return (int)(want < cap ? want : cap);
 }
