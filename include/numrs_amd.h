@@ -122,7 +122,14 @@ enum {
     NRK_SOP_LOG10 = 23,
     NRK_SOP_LOG1P = 24,
     NRK_SOP_TANH = 25,
-    NRK_SOP_COUNT = 26
+    /* ufuncs.rs:619-700: sinh, cosh, asinh, acosh, atanh (within 2 ulp, acosh 3; elementwise
+     * map only) */
+    NRK_SOP_SINH = 26,
+    NRK_SOP_COSH = 27,
+    NRK_SOP_ASINH = 28,
+    NRK_SOP_ACOSH = 29,
+    NRK_SOP_ATANH = 30,
+    NRK_SOP_COUNT = 31
 };
 typedef struct nrk_scalar_op {
     int32_t op;     /* NRK_SOP_* */

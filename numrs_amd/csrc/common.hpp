@@ -189,6 +189,11 @@ __device__ __forceinline__ void apply_chain_n(const Chain &c, T (&v)[N]) {
                         NRK_CHAIN_CASE(NRK_SOP_LOG10, via_f64(x, [](double d) { return fm::log10(d); }))
                         NRK_CHAIN_CASE(NRK_SOP_LOG1P, via_f64(x, [](double d) { return fm::log1p(d); }))
                         NRK_CHAIN_CASE(NRK_SOP_TANH, via_f64(x, [](double d) { return fm::tanh(d); }))
+                        NRK_CHAIN_CASE(NRK_SOP_SINH, via_f64(x, [](double d) { return fm::sinh(d); }))
+                        NRK_CHAIN_CASE(NRK_SOP_COSH, via_f64(x, [](double d) { return fm::cosh(d); }))
+                        NRK_CHAIN_CASE(NRK_SOP_ASINH, via_f64(x, [](double d) { return fm::asinh(d); }))
+                        NRK_CHAIN_CASE(NRK_SOP_ACOSH, via_f64(x, [](double d) { return fm::acosh(d); }))
+                        NRK_CHAIN_CASE(NRK_SOP_ATANH, via_f64(x, [](double d) { return fm::atanh(d); }))
                         default: break;
                     }
                 }

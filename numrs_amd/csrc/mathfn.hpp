@@ -1,12 +1,15 @@
-// Elementary functions in FP64 for the elementwise map: exp, exp2, expm1, log, log2, log10, log1p
-// and tanh (the reference's ufuncs.rs:398-475 and :639, which call Rust's f64 methods below 64
+// Elementary functions in FP64 for the elementwise map: exp, exp2, expm1, log, log2, log10, log1p,
+// tanh, sinh, cosh, asinh, acosh, atanh (the reference's ufuncs.rs:398-475 and :619-700, which
+// call Rust's f64 methods below 64
 // elements and a SIMD library above). Each is written here from a textbook method, named per
 // function; every constant is a mathematical constant (ln 2 and its relatives, split so that
 // products with small integers are exact, the Taylor coefficients 1/n! and 2/(2j+1)), derived
 // with 80-digit decimal arithmetic. Binary32 inputs are evaluated in FP64 and rounded once.
 //
 // Accuracy (measured against correctly rounded results, tests/test_gpu_ufuncs.py): within 1 ulp
-// for exp, exp2, expm1, log, log2, log10 and log1p, within 2 ulp for tanh; exact special values
+// for exp, exp2, expm1, log, log2, log10 and log1p, within 2 ulp for tanh, sinh, cosh, asinh and
+// atanh, 3 ulp for acosh (near 1 the result is ~sqrt(2t): the rounding of 2t + t^2, the square
+// root, t + s and log1p add up); exact special values
 // (NaN, +-inf, +-0, the overflow and underflow edges).
 #pragma once
 
@@ -263,6 +266,87 @@ __device__ inline double tanh(double x) {
     const double q = t / d;
     const double res = fma(-q, d, t) - q * dl;  // t - q (d + dl)
     return __builtin_copysign(q + res / d, x);
+}
+
+// Hyperbolic functions from the exponential and logarithm above (identities of the definitions):
+//   sinh x = (t + t / (t + 1)) / 2, t = e^|x| - 1 (no cancellation), e^|x| / 2 beyond |x| = 22;
+//   cosh x = 1 + t^2 / (2 (1 + t)) below |x| = ln2 / 2, else (e + 1/e) / 2;
+//   atanh x = log1p(2x / (1 - x)) / 2;  asinh x = log1p(|x| + x^2 / (1 + sqrt(1 + x^2)));
+//   acosh x = log1p(t + sqrt(2t + t^2)), t = x - 1 (x <= 2), log(2x - 1/(x + sqrt(x^2 - 1)))
+//   below 2^28 and log(x) + ln2 above. Sign restored for the odd ones.
+__device__ __forceinline__ double half_exp_big(double ax) {  // e^ax / 2 for ax >= 22
+    if (ax <= 709.78) return 0.5 * exp(ax);
+    if (ax <= 710.5) {
+        const double w = exp(0.5 * ax);
+        return (0.5 * w) * w;
+    }
+    return dinf();
+}
+__device__ inline double sinh(double x) {
+    if (x != x) return x;
+    const double ax = __builtin_fabs(x);
+    if (ax < 0x1p-28) return x;
+    double h;
+    if (ax < 22.0) {
+        const double t = expm1(ax);
+        h = 0.5 * (t + t / (t + 1.0));
+    } else {
+        h = half_exp_big(ax);
+    }
+    return __builtin_copysign(h, x);
+}
+__device__ inline double cosh(double x) {
+    if (x != x) return x;
+    const double ax = __builtin_fabs(x);
+    if (ax < 0.3465) {
+        const double t = expm1(ax);
+        return 1.0 + (t * t) / (2.0 * (1.0 + t));
+    }
+    if (ax < 22.0) {
+        const double e = exp(ax);
+        return 0.5 * e + 0.5 / e;
+    }
+    return half_exp_big(ax);
+}
+__device__ inline double atanh(double x) {
+    if (x != x) return x;
+    const double ax = __builtin_fabs(x);
+    if (ax > 1.0) return __builtin_nan("");
+    if (ax == 1.0) return __builtin_copysign(dinf(), x);
+    if (ax < 0x1p-28) return x;
+    double r;
+    if (ax < 0.5) {
+        const double t = ax + ax;
+        r = 0.5 * log1p(t + t * ax / (1.0 - ax));
+    } else {
+        r = 0.5 * log1p((ax + ax) / (1.0 - ax));
+    }
+    return __builtin_copysign(r, x);
+}
+__device__ inline double asinh(double x) {
+    if (x != x || x == dinf() || x == -dinf()) return x;
+    const double ax = __builtin_fabs(x);
+    if (ax < 0x1p-28) return x;
+    double r;
+    if (ax > 0x1p28) {
+        r = log(ax) + LN2;
+    } else if (ax > 2.0) {
+        r = log(2.0 * ax + 1.0 / (__dsqrt_rn(x * x + 1.0) + ax));
+    } else {
+        const double t = x * x;
+        r = log1p(ax + t / (1.0 + __dsqrt_rn(1.0 + t)));
+    }
+    return __builtin_copysign(r, x);
+}
+__device__ inline double acosh(double x) {
+    if (x != x) return x;
+    if (x < 1.0) return __builtin_nan("");
+    if (x == 1.0) return 0.0;
+    if (x == dinf()) return x;
+    if (x > 0x1p28) return log(x) + LN2;
+    if (x > 2.0) return log(2.0 * x - 1.0 / (x + __dsqrt_rn(x * x - 1.0)));
+    const double t = x - 1.0;
+    return log1p(t + __dsqrt_rn(fma(t, t, 2.0 * t)));  // 2t + t^2 rounded once
 }
 
 }  // namespace fm

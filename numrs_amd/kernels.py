@@ -24,7 +24,8 @@ from ._lib import ScalarOp, call
 # NRK_SOP_* / NRK_BOP_* / NRK_AXIS_* codes of include/numrs_amd.h
 SOP = {"add": 0, "sub": 1, "mul": 2, "div": 3, "rsub": 4, "rdiv": 5, "neg": 6, "abs": 7, "square": 8, "sqrt": 9,
        "floor": 10, "ceil": 11, "trunc": 12, "round": 13, "fract": 14, "sign": 15, "clamp_lo": 16, "clamp_hi": 17,
-       "exp": 18, "exp2": 19, "expm1": 20, "log": 21, "log2": 22, "log10": 23, "log1p": 24, "tanh": 25}
+       "exp": 18, "exp2": 19, "expm1": 20, "log": 21, "log2": 22, "log10": 23, "log1p": 24, "tanh": 25,
+       "sinh": 26, "cosh": 27, "asinh": 28, "acosh": 29, "atanh": 30}
 BOP = {"add": 0, "sub": 1, "mul": 2, "div": 3, "rem": 4, "maximum": 5, "minimum": 6}
 AXIS = {"sum": 0, "mean": 1, "var": 2, "min": 3, "max": 4, "argmin": 5, "argmax": 6}
 
@@ -144,6 +145,12 @@ def log2(a): return _ufunc(a, ("log2",))                     # :443 x.log2()
 def log10(a): return _ufunc(a, ("log10",))                   # :453 x.log10()
 def log1p(a): return _ufunc(a, ("log1p",))                   # :463 x.ln_1p()
 def tanh(a): return _ufunc(a, ("tanh",))                     # :639 x.tanh()
+def sinh(a): return _ufunc(a, ("sinh",))                     # :619 x.sinh()
+def cosh(a): return _ufunc(a, ("cosh",))                     # :629 x.cosh()
+def arcsinh(a): return _ufunc(a, ("asinh",))                 # :653 x.asinh()
+def arccosh(a): return _ufunc(a, ("acosh",))                 # :668 x.acosh()
+def arctanh(a): return _ufunc(a, ("atanh",))                 # :683 x.atanh()
+asinh, acosh, atanh = arcsinh, arccosh, arctanh
 def sigmoid(a): return _ufunc(a, ("neg",), ("exp",), ("add", 1.0), ("rdiv", 1.0))  # :1235 1 / (1 + (-x).exp())
 def reciprocal(a): return _ufunc(a, ("rdiv", 1.0))           # :374 1.0 / x
 def rsqrt(a): return _ufunc(a, ("sqrt",), ("rdiv", 1.0))     # :384 1.0 / x.sqrt()

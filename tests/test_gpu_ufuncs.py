@@ -4,7 +4,9 @@ The reference's exp / log / tanh ... are Rust's f64 methods below 64 elements an
 above (absent here); neither is bit-reproducible, so the bar is the mathematics: the error
 against the exact value, computed in 80-bit long double (64-bit significand, 11 bits more than
 binary64) on the CPU, measured in units of the last place of the binary64 result. Bounds, written
-here: exp, exp2, expm1, log, log1p <= 1 ulp; log2, log10 <= 1 ulp; tanh <= 2 ulp; sigmoid
+here: exp, exp2, expm1, log, log1p <= 1 ulp; log2, log10 <= 1 ulp; tanh, sinh, cosh, asinh,
+atanh <= 2 ulp; acosh <= 3 ulp (near 1 its result ~sqrt(2t) collects the roundings of
+2t + t^2, the square root, t + s and log1p); sigmoid
 (the reference's 1 / (1 + exp(-x)): exp's relative error passes through two more IEEE roundings,
 and a relative error converts to up to twice as many ulps at a binade edge) <= 3 ulp; binary32
 inputs (FP64 evaluation, one rounding) <= 1 ulp of binary32. Special values (NaN, +-inf, +-0, overflow and underflow edges)
@@ -70,6 +72,16 @@ CASES = {
                                                                                   -2.0], SPECIAL]), 1.0),
     "tanh": (K.tanh, np.tanh, np.concatenate([sample(-25, 25, 40000, 18), sample(-1, 1, 20000, 19),
                                              sample(-1e-9, 1e-9, 5000, 20), [22.0, -22.0, 21.99, 0.5493, 3.7e-9], SPECIAL]), 2.0),
+    "sinh": (K.sinh, np.sinh, np.concatenate([sample(-711, 711, 20000, 50), sample(-25, 25, 20000, 51),
+                                             sample(-1, 1, 10000, 52), [709.78, 710.4, 710.5, -710.4, 22.0], SPECIAL]), 2.0),
+    "cosh": (K.cosh, np.cosh, np.concatenate([sample(-711, 711, 20000, 53), sample(-25, 25, 20000, 54),
+                                             sample(-1, 1, 10000, 55), [709.78, 710.4, 0.3465, 22.0], SPECIAL]), 2.0),
+    "asinh": (K.arcsinh, np.arcsinh, np.concatenate([sample(-10, 10, 20000, 56), np.exp(sample(-30, 700, 20000, 57)),
+                                                    -np.exp(sample(-30, 700, 5000, 58)), [2.0, 2.0000001, 2.0**28], SPECIAL]), 2.0),
+    "acosh": (K.arccosh, np.arccosh, np.concatenate([1 + np.exp(sample(-40, 2, 20000, 59)), np.exp(sample(0, 700, 20000, 60)),
+                                                    [1.0, 2.0, 2.0**28, 0.999], SPECIAL]), 3.0),
+    "atanh": (K.arctanh, np.arctanh, np.concatenate([sample(-1, 1, 30000, 61), 1 - np.exp(sample(-36, 0, 10000, 62)),
+                                                    sample(-1e-8, 1e-8, 5000, 63), [0.5, -0.5, 1.0, -1.0, 1.5]], ), 2.0),
 }
 
 
