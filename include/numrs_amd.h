@@ -129,7 +129,11 @@ enum {
     NRK_SOP_ASINH = 28,
     NRK_SOP_ACOSH = 29,
     NRK_SOP_ATANH = 30,
-    NRK_SOP_COUNT = 31
+    /* ufuncs.rs:542-590: atan (within 1.5 ulp), asin, acos (within 2 ulp); elementwise map only */
+    NRK_SOP_ATAN = 31,
+    NRK_SOP_ASIN = 32,
+    NRK_SOP_ACOS = 33,
+    NRK_SOP_COUNT = 34
 };
 typedef struct nrk_scalar_op {
     int32_t op;     /* NRK_SOP_* */

@@ -194,6 +194,9 @@ __device__ __forceinline__ void apply_chain_n(const Chain &c, T (&v)[N]) {
                         NRK_CHAIN_CASE(NRK_SOP_ASINH, via_f64(x, [](double d) { return fm::asinh(d); }))
                         NRK_CHAIN_CASE(NRK_SOP_ACOSH, via_f64(x, [](double d) { return fm::acosh(d); }))
                         NRK_CHAIN_CASE(NRK_SOP_ATANH, via_f64(x, [](double d) { return fm::atanh(d); }))
+                        NRK_CHAIN_CASE(NRK_SOP_ATAN, via_f64(x, [](double d) { return fm::atan(d); }))
+                        NRK_CHAIN_CASE(NRK_SOP_ASIN, via_f64(x, [](double d) { return fm::asin(d); }))
+                        NRK_CHAIN_CASE(NRK_SOP_ACOS, via_f64(x, [](double d) { return fm::acos(d); }))
                         default: break;
                     }
                 }

@@ -1,5 +1,6 @@
 // Elementary functions in FP64 for the elementwise map: exp, exp2, expm1, log, log2, log10, log1p,
-// tanh, sinh, cosh, asinh, acosh, atanh (the reference's ufuncs.rs:398-475 and :619-700, which
+// tanh, sinh, cosh, asinh, acosh, atanh, atan, asin, acos (the reference's ufuncs.rs:398-475,
+// :542-590 and :619-700, which
 // call Rust's f64 methods below 64
 // elements and a SIMD library above). Each is written here from a textbook method, named per
 // function; every constant is a mathematical constant (ln 2 and its relatives, split so that
@@ -9,7 +10,7 @@
 // Accuracy (measured against correctly rounded results, tests/test_gpu_ufuncs.py): within 1 ulp
 // for exp, exp2, expm1, log, log2, log10 and log1p, within 2 ulp for tanh, sinh, cosh, asinh and
 // atanh, 3 ulp for acosh (near 1 the result is ~sqrt(2t): the rounding of 2t + t^2, the square
-// root, t + s and log1p add up); exact special values
+// root, t + s and log1p add up); atan 1.5 ulp, asin and acos 2; exact special values
 // (NaN, +-inf, +-0, the overflow and underflow edges).
 #pragma once
 
@@ -347,6 +348,118 @@ __device__ inline double acosh(double x) {
     if (x > 2.0) return log(2.0 * x - 1.0 / (x + __dsqrt_rn(x * x - 1.0)));
     const double t = x - 1.0;
     return log1p(t + __dsqrt_rn(fma(t, t, 2.0 * t)));  // 2t + t^2 rounded once
+}
+
+// ---- inverse trigonometric functions (ufuncs.rs:542-590) ----
+// atan t for t in [0, 1]: with c = j/8 the eighth at or below t, atan t = atan(c) + atan(u),
+// u = (t - c) / (1 + t c) in [0, 1/8] (no cancellation between the two terms); atan(u) by its odd
+// Taylor series to u^21 (the first omitted term is below 2^-60 relative); atan(j/8) in two parts. tl is a correction to t
+// (the rounding error of 1/x when t came from it); the division's residual is carried too.
+constexpr double PIO2_HI = 0x1.921fb54442d18p+0, PIO2_LO = 0x1.1a62633145c07p-54;
+__device__ __forceinline__ double atan_tab_hi(int j) {
+    switch (j) {
+        case 1: return 0x1.fd5ba9aac2f6ep-4;
+        case 2: return 0x1.f5b75f92c80ddp-3;
+        case 3: return 0x1.6f61941e4def1p-2;
+        case 4: return 0x1.dac670561bb4fp-2;
+        case 5: return 0x1.1e00babdefeb4p-1;
+        case 6: return 0x1.4978fa3269ee1p-1;
+        case 7: return 0x1.700a7c5784634p-1;
+        case 8: return 0x1.921fb54442d18p-1;
+        default: return 0.0;
+    }
+}
+__device__ __forceinline__ double atan_tab_lo(int j) {
+    switch (j) {
+        case 1: return -0x1.cd37686760c17p-59;
+        case 2: return 0x1.8ab6e3cf7afbdp-57;
+        case 3: return -0x1.c63aae6f6e918p-56;
+        case 4: return 0x1.a2b7f222f65e2p-56;
+        case 5: return -0x1.928df287a668fp-58;
+        case 6: return 0x1.2419a87f2a458p-56;
+        case 7: return -0x1.8c34d25aadef6p-56;
+        case 8: return 0x1.1a62633145c07p-55;
+        default: return 0.0;
+    }
+}
+__device__ __forceinline__ void two_sum(double a, double b, double &s, double &e) {
+    s = a + b;
+    const double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+__device__ __forceinline__ void atan_core(double t, double tl, double &hi, double &lo) {
+    const int j = (int)(t * 8.0);
+    double u = t, ul = tl;
+    if (j != 0) {
+        const double c = 0.125 * (double)j;
+        const double num = t - c;  // exact (Sterbenz)
+        const double den = fma(t, c, 1.0);
+        u = num / den;
+        ul = fma(-u, den, num) / den + tl * fma(c, c, 1.0) / (den * den);
+    }
+    const double z = u * u;
+    double q = 0x1.8618618618618p-5;      // 1/21
+    q = fma(q, z, -0x1.af286bca1af28p-5); // -1/19
+    q = fma(q, z, 0x1.e1e1e1e1e1e1ep-5);  // 1/17
+    q = fma(q, z, -0x1.1111111111111p-4); // -1/15
+    q = fma(q, z, 0x1.3b13b13b13b14p-4);  // 1/13
+    q = fma(q, z, -0x1.745d1745d1746p-4); // -1/11
+    q = fma(q, z, 0x1.c71c71c71c71cp-4);  // 1/9
+    q = fma(q, z, -0x1.2492492492492p-3); // -1/7
+    q = fma(q, z, 0x1.999999999999ap-3);  // 1/5
+    q = fma(q, z, -0x1.5555555555555p-2); // -1/3
+    const double p = fma(u * z, q, u);
+    double e;
+    two_sum(atan_tab_hi(j), p, hi, e);
+    lo = e + (atan_tab_lo(j) + ul / (1.0 + z));
+}
+// atan t for t >= 0 (+inf included) as hi + lo; above 1 through pi/2 - atan(1/t).
+__device__ __forceinline__ void atan_dd(double t, double &hi, double &lo) {
+    if (t == dinf()) {
+        hi = PIO2_HI;
+        lo = PIO2_LO;
+        return;
+    }
+    if (t > 1.0) {
+        const double r = 1.0 / t;
+        const double rl = fma(-r, t, 1.0) / t;
+        double h, l, e;
+        atan_core(r, rl, h, l);
+        two_sum(PIO2_HI, -h, hi, e);
+        lo = e + (PIO2_LO - l);
+        return;
+    }
+    atan_core(t, 0.0, hi, lo);
+}
+__device__ inline double atan(double x) {
+    if (x != x || x == 0.0) return x;
+    double h, l;
+    atan_dd(__builtin_fabs(x), h, l);
+    return __builtin_copysign(h + l, x);
+}
+// asin x = atan(x / sqrt(1 - x^2)), acos x = atan(sqrt(1 - x^2) / x) (pi minus it for x < 0):
+// 1 - x^2 is one FMA (rounded once, no cancellation), so the argument of atan is within ~1.25 ulp
+// and atan passes a relative error on at most unchanged.
+__device__ inline double asin(double x) {
+    if (x != x || x == 0.0) return x;
+    const double ax = __builtin_fabs(x);
+    if (ax > 1.0) return __builtin_nan("");
+    const double s = __dsqrt_rn(fma(-ax, ax, 1.0));
+    double h, l;
+    atan_dd(s > 0.0 ? ax / s : dinf(), h, l);
+    return __builtin_copysign(h + l, x);
+}
+__device__ inline double acos(double x) {
+    if (x != x) return x;
+    const double ax = __builtin_fabs(x);
+    if (ax > 1.0) return __builtin_nan("");
+    const double s = __dsqrt_rn(fma(-ax, ax, 1.0));
+    double h, l;
+    atan_dd(ax > 0.0 ? s / ax : dinf(), h, l);
+    if (x >= 0.0) return h + l;
+    double a, e;
+    two_sum(2.0 * PIO2_HI, -h, a, e);
+    return a + (e + (2.0 * PIO2_LO - l));
 }
 
 }  // namespace fm

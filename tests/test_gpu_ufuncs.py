@@ -6,7 +6,7 @@ against the exact value, computed in 80-bit long double (64-bit significand, 11 
 binary64) on the CPU, measured in units of the last place of the binary64 result. Bounds, written
 here: exp, exp2, expm1, log, log1p <= 1 ulp; log2, log10 <= 1 ulp; tanh, sinh, cosh, asinh,
 atanh <= 2 ulp; acosh <= 3 ulp (near 1 its result ~sqrt(2t) collects the roundings of
-2t + t^2, the square root, t + s and log1p); sigmoid
+2t + t^2, the square root, t + s and log1p); atan <= 1.5 ulp, asin, acos <= 2 ulp; sigmoid
 (the reference's 1 / (1 + exp(-x)): exp's relative error passes through two more IEEE roundings,
 and a relative error converts to up to twice as many ulps at a binade edge) <= 3 ulp; binary32
 inputs (FP64 evaluation, one rounding) <= 1 ulp of binary32. Special values (NaN, +-inf, +-0, overflow and underflow edges)
@@ -82,6 +82,12 @@ CASES = {
                                                     [1.0, 2.0, 2.0**28, 0.999], SPECIAL]), 3.0),
     "atanh": (K.arctanh, np.arctanh, np.concatenate([sample(-1, 1, 30000, 61), 1 - np.exp(sample(-36, 0, 10000, 62)),
                                                     sample(-1e-8, 1e-8, 5000, 63), [0.5, -0.5, 1.0, -1.0, 1.5]], ), 2.0),
+    "atan": (K.arctan, np.arctan, np.concatenate([sample(-1, 1, 20000, 64), sample(-20, 20, 20000, 65), np.exp(sample(-30, 700, 10000, 66)),
+                                                 -np.exp(sample(-30, 700, 5000, 67)), [1.0, -1.0, 0.0625, 0.1875, 0.9375], SPECIAL]), 1.5),
+    "asin": (K.arcsin, np.arcsin, np.concatenate([sample(-1, 1, 30000, 68), 1 - np.exp(sample(-36, -1, 10000, 69)),
+                                                 sample(-1e-9, 1e-9, 5000, 70), [0.5, -0.5, 1.0000000000000002]], ), 2.0),
+    "acos": (K.arccos, np.arccos, np.concatenate([sample(-1, 1, 30000, 71), 1 - np.exp(sample(-36, -1, 10000, 72)),
+                                                 -1 + np.exp(sample(-36, -1, 10000, 73)), [0.5, -0.5]], ), 2.0),
 }
 
 
