@@ -133,7 +133,12 @@ enum {
     NRK_SOP_ATAN = 31,
     NRK_SOP_ASIN = 32,
     NRK_SOP_ACOS = 33,
-    NRK_SOP_COUNT = 34
+    /* ufuncs.rs:477-510: sin, cos (within 1 ulp), tan (2.5 ulp) for every finite argument
+     * (Payne-Hanek reduction above 2^20); elementwise map only */
+    NRK_SOP_SIN = 34,
+    NRK_SOP_COS = 35,
+    NRK_SOP_TAN = 36,
+    NRK_SOP_COUNT = 37
 };
 typedef struct nrk_scalar_op {
     int32_t op;     /* NRK_SOP_* */

@@ -197,6 +197,9 @@ __device__ __forceinline__ void apply_chain_n(const Chain &c, T (&v)[N]) {
                         NRK_CHAIN_CASE(NRK_SOP_ATAN, via_f64(x, [](double d) { return fm::atan(d); }))
                         NRK_CHAIN_CASE(NRK_SOP_ASIN, via_f64(x, [](double d) { return fm::asin(d); }))
                         NRK_CHAIN_CASE(NRK_SOP_ACOS, via_f64(x, [](double d) { return fm::acos(d); }))
+                        NRK_CHAIN_CASE(NRK_SOP_SIN, via_f64(x, [](double d) { return fm::sin(d); }))
+                        NRK_CHAIN_CASE(NRK_SOP_COS, via_f64(x, [](double d) { return fm::cos(d); }))
+                        NRK_CHAIN_CASE(NRK_SOP_TAN, via_f64(x, [](double d) { return fm::tan(d); }))
                         default: break;
                     }
                 }

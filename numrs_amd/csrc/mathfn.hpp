@@ -1,6 +1,6 @@
 // Elementary functions in FP64 for the elementwise map: exp, exp2, expm1, log, log2, log10, log1p,
-// tanh, sinh, cosh, asinh, acosh, atanh, atan, asin, acos (the reference's ufuncs.rs:398-475,
-// :542-590 and :619-700, which
+// tanh, sinh, cosh, asinh, acosh, atanh, atan, asin, acos, sin, cos, tan (the reference's
+// ufuncs.rs:398-510, :542-590 and :619-700, which
 // call Rust's f64 methods below 64
 // elements and a SIMD library above). Each is written here from a textbook method, named per
 // function; every constant is a mathematical constant (ln 2 and its relatives, split so that
@@ -10,7 +10,8 @@
 // Accuracy (measured against correctly rounded results, tests/test_gpu_ufuncs.py): within 1 ulp
 // for exp, exp2, expm1, log, log2, log10 and log1p, within 2 ulp for tanh, sinh, cosh, asinh and
 // atanh, 3 ulp for acosh (near 1 the result is ~sqrt(2t): the rounding of 2t + t^2, the square
-// root, t + s and log1p add up); atan 1.5 ulp, asin and acos 2; exact special values
+// root, t + s and log1p add up); atan 1.5 ulp, asin and acos 2; sin and cos 1, tan 2.5 (a
+// quotient of the two), for every finite argument; exact special values
 // (NaN, +-inf, +-0, the overflow and underflow edges).
 #pragma once
 
@@ -460,6 +461,156 @@ __device__ inline double acos(double x) {
     double a, e;
     two_sum(2.0 * PIO2_HI, -h, a, e);
     return a + (e + (2.0 * PIO2_LO - l));
+}
+
+// ---- sin, cos, tan (ufuncs.rs:477-510) ----
+// Reduction x = k pi/2 + r, |r| <= pi/4, r as rh + rl:
+//   |x| < 2^20: Cody & Waite with pi/2 in three parts (PIO2_HI exact against k, the rest carried);
+//   larger: Payne & Hanek, "Radian reduction for trigonometric functions" (1983): with x = m 2^E
+//   (m the 53-bit significand), only the bits of 2/pi from 2^-(E-1) on matter mod 4; m times a
+//   192-bit window of them (three 64-bit words) gives k mod 4 and a 128-bit fraction f, and
+//   r = f pi/2. The window's first dropped bit is worth m 2^-190 < 2^-137.
+// The bits of 2/pi, 1536 of them (bit 1 = 2^-1 is the top bit of the first word), from 500-digit
+// arithmetic.
+__device__ __forceinline__ unsigned long long two_over_pi_word(int w) {
+    static constexpr unsigned long long T[24] = {
+        0xa2f9836e4e441529ull, 0xfc2757d1f534ddc0ull, 0xdb6295993c439041ull,
+        0xfe5163abdebbc561ull, 0xb7246e3a424dd2e0ull, 0x06492eea09d1921cull,
+        0xfe1deb1cb129a73eull, 0xe88235f52ebb4484ull, 0xe99c7026b45f7e41ull,
+        0x3991d639835339f4ull, 0x9c845f8bbdf9283bull, 0x1ff897ffde05980full,
+        0xef2f118b5a0a6d1full, 0x6d367ecf27cb09b7ull, 0x4f463f669e5fea2dull,
+        0x7527bac7ebe5f17bull, 0x3d0739f78a5292eaull, 0x6bfb5fb11f8d5d08ull,
+        0x56033046fc7b6babull, 0xf0cfbc209af4361dull, 0xa9e391615ee61b08ull,
+        0x6599855f14a06840ull, 0x8dffd8804d732731ull, 0x06061556ca73a8c9ull};
+    return T[w];
+}
+// 64 bits of 2/pi starting at bit b >= 1.
+__device__ __forceinline__ unsigned long long two_over_pi_bits(int b) {
+    const int o = b - 1, w = o >> 6, sft = o & 63;
+    const unsigned long long hi = two_over_pi_word(w);
+    return sft ? (hi << sft) | (two_over_pi_word(w + 1) >> (64 - sft)) : hi;
+}
+constexpr double PIO2_3 = -0x1.f1976b7ed8fbcp-110;  // pi/2 = PIO2_HI + PIO2_LO + PIO2_3
+constexpr double TWO_OVER_PI = 0x1.45f306dc9c883p-1;
+
+// ax >= 2^20 (finite): quadrant k mod 4 and r = rh + rl.
+__device__ inline int reduce_payne_hanek(double ax, double &rh, double &rl) {
+    int e;
+    const double mf = frexp(ax, &e);
+    const unsigned long long m = (unsigned long long)ldexp(mf, 53);  // ax = m 2^E
+    const int E = e - 53;
+    const int i0 = E - 1 > 1 ? E - 1 : 1;
+    const int p = i0 + 191 - E;  // the binary point of m * window, in [190, 224]
+    const unsigned long long w0 = two_over_pi_bits(i0), w1 = two_over_pi_bits(i0 + 64), w2 = two_over_pi_bits(i0 + 128);
+    // 256-bit product m * (w0:w1:w2), little-endian words R[0..3]
+    unsigned long long R[4];
+    const unsigned long long lo2 = m * w2, hi2 = __umul64hi(m, w2);
+    const unsigned long long lo1 = m * w1, hi1 = __umul64hi(m, w1);
+    const unsigned long long lo0 = m * w0, hi0 = __umul64hi(m, w0);
+    R[0] = lo2;
+    R[1] = hi2 + lo1;
+    const unsigned long long c1 = R[1] < hi2 ? 1ull : 0ull;
+    const unsigned long long t2 = hi1 + lo0;
+    const unsigned long long c2a = t2 < hi1 ? 1ull : 0ull;
+    R[2] = t2 + c1;
+    const unsigned long long c2 = c2a + (R[2] < t2 ? 1ull : 0ull);
+    R[3] = hi0 + c2;
+    auto chunk = [&](int b) -> unsigned long long {  // 64 bits of R from bit b, 62 <= b <= 160
+        const int w = b >> 6, sft = b & 63;
+        return sft ? (R[w] >> sft) | (R[w + 1] << (64 - sft)) : R[w];
+    };
+    const int pw = p >> 6, ps = p & 63;
+    unsigned q = (unsigned)(R[pw] >> ps);
+    if (ps == 63) q |= (unsigned)(R[pw + 1] << 1);  // pw <= 2 here (p <= 224)
+    q &= 3u;
+    unsigned long long fh = chunk(p - 64), fl = chunk(p - 128);
+    double sign = 1.0;
+    if (fh >> 63) {  // f >= 1/2: take f - 1 and the next quadrant
+        q = (q + 1u) & 3u;
+        fl = ~fl + 1ull;
+        fh = ~fh + (fl == 0ull ? 1ull : 0ull);
+        sign = -1.0;
+    }
+    // f = fh 2^-64 + fl 2^-128 as a double-double, then times pi/2
+    const double fd_hi = __ull2double_rn(fh) * 0x1p-64;
+    const long long rem = (long long)(fh - (unsigned long long)(fd_hi * 0x1p64));
+    const double fd_lo = ((double)rem + __ull2double_rn(fl) * 0x1p-64) * 0x1p-64;
+    const double h = fd_hi * PIO2_HI;
+    const double l = fma(fd_hi, PIO2_HI, -h) + fma(fd_hi, PIO2_LO, fd_lo * PIO2_HI);
+    rh = sign * h;
+    rl = sign * l;
+    return (int)q;
+}
+__device__ inline int reduce_pio2(double ax, double &rh, double &rl) {
+    if (ax <= 0.785398163397448) {
+        rh = ax;
+        rl = 0.0;
+        return 0;
+    }
+    if (ax < 0x1p20) {
+        const double k = rint(ax * TWO_OVER_PI);
+        const double r1 = fma(-k, PIO2_HI, ax);  // exact
+        const double th = k * PIO2_LO, tl = fma(k, PIO2_LO, -th);
+        double h, e;
+        two_sum(r1, -th, h, e);
+        const double l = (e - tl) - k * PIO2_3;
+        rh = h + l;
+        rl = l - (rh - h);
+        return (int)k & 3;
+    }
+    return reduce_payne_hanek(ax, rh, rl);
+}
+// sin and cos on |r| <= pi/4 by Taylor series (to r^19 and r^18: first omitted term below
+// 2^-60 relative), the tail rl folded in to first order.
+__device__ __forceinline__ double sin_kernel(double rh, double rl) {
+    const double z = rh * rh;
+    double q = -0x1.2f49b46814157p-57;
+    q = fma(q, z, 0x1.952c77030ad4ap-49);
+    q = fma(q, z, -0x1.ae7f3e733b81fp-41);
+    q = fma(q, z, 0x1.6124613a86d09p-33);
+    q = fma(q, z, -0x1.ae64567f544e4p-26);
+    q = fma(q, z, 0x1.71de3a556c734p-19);
+    q = fma(q, z, -0x1.a01a01a01a01ap-13);
+    q = fma(q, z, 0x1.1111111111111p-7);
+    q = fma(q, z, -0x1.5555555555555p-3);  // -1/3!
+    return rh + fma(rh * z, q, rl * (1.0 - 0.5 * z));
+}
+__device__ __forceinline__ double cos_kernel(double rh, double rl) {
+    const double z = rh * rh, zl = fma(rh, rh, -z);
+    double q = -0x1.6827863b97d97p-53;
+    q = fma(q, z, 0x1.ae7f3e733b81fp-45);
+    q = fma(q, z, -0x1.93974a8c07c9dp-37);
+    q = fma(q, z, 0x1.1eed8eff8d898p-29);
+    q = fma(q, z, -0x1.27e4fb7789f5cp-22);
+    q = fma(q, z, 0x1.a01a01a01a01ap-16);
+    q = fma(q, z, -0x1.6c16c16c16c17p-10);
+    q = fma(q, z, 0x1.5555555555555p-5);  // 1/4!
+    const double hz = 0.5 * z, w = 1.0 - hz;
+    return w + (((1.0 - w) - hz) + (z * z * q - (0.5 * zl + rh * rl)));
+}
+__device__ inline double sin(double x) {
+    if (x != x || x == dinf() || x == -dinf()) return __builtin_nan("");
+    if (x == 0.0) return x;
+    double rh, rl;
+    const int q = reduce_pio2(__builtin_fabs(x), rh, rl);
+    const double v = (q & 1) ? cos_kernel(rh, rl) : sin_kernel(rh, rl);
+    return __builtin_copysign(1.0, x) * ((q & 2) ? -v : v);  // sin(-x) = -sin(x)
+}
+__device__ inline double cos(double x) {
+    if (x != x || x == dinf() || x == -dinf()) return __builtin_nan("");
+    double rh, rl;
+    const int q = reduce_pio2(__builtin_fabs(x), rh, rl);
+    const double v = (q & 1) ? sin_kernel(rh, rl) : cos_kernel(rh, rl);
+    return (q == 1 || q == 2) ? -v : v;
+}
+__device__ inline double tan(double x) {
+    if (x != x || x == dinf() || x == -dinf()) return __builtin_nan("");
+    if (x == 0.0) return x;
+    double rh, rl;
+    const int q = reduce_pio2(__builtin_fabs(x), rh, rl);
+    const double sn = sin_kernel(rh, rl), cs = cos_kernel(rh, rl);
+    const double v = (q & 1) ? -cs / sn : sn / cs;
+    return __builtin_copysign(1.0, x) * v;
 }
 
 }  // namespace fm

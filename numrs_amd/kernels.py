@@ -25,7 +25,8 @@ from ._lib import ScalarOp, call
 SOP = {"add": 0, "sub": 1, "mul": 2, "div": 3, "rsub": 4, "rdiv": 5, "neg": 6, "abs": 7, "square": 8, "sqrt": 9,
        "floor": 10, "ceil": 11, "trunc": 12, "round": 13, "fract": 14, "sign": 15, "clamp_lo": 16, "clamp_hi": 17,
        "exp": 18, "exp2": 19, "expm1": 20, "log": 21, "log2": 22, "log10": 23, "log1p": 24, "tanh": 25,
-       "sinh": 26, "cosh": 27, "asinh": 28, "acosh": 29, "atanh": 30, "atan": 31, "asin": 32, "acos": 33}
+       "sinh": 26, "cosh": 27, "asinh": 28, "acosh": 29, "atanh": 30, "atan": 31, "asin": 32, "acos": 33,
+       "sin": 34, "cos": 35, "tan": 36}
 BOP = {"add": 0, "sub": 1, "mul": 2, "div": 3, "rem": 4, "maximum": 5, "minimum": 6}
 AXIS = {"sum": 0, "mean": 1, "var": 2, "min": 3, "max": 4, "argmin": 5, "argmax": 6}
 
@@ -155,6 +156,10 @@ def arctan(a): return _ufunc(a, ("atan",))                   # :572 x.atan()
 def arcsin(a): return _ufunc(a, ("asin",))                   # :542 x.asin()
 def arccos(a): return _ufunc(a, ("acos",))                   # :557 x.acos()
 atan, asin, acos = arctan, arcsin, arccos
+def sin(a): return _ufunc(a, ("sin",))                       # :477 x.sin()
+def cos(a): return _ufunc(a, ("cos",))                       # :487 x.cos()
+def tan(a): return _ufunc(a, ("tan",))                       # :497 x.tan()
+def sincos(a): return sin(a), cos(a)                         # :507 (x.sin(), x.cos())
 def sigmoid(a): return _ufunc(a, ("neg",), ("exp",), ("add", 1.0), ("rdiv", 1.0))  # :1235 1 / (1 + (-x).exp())
 def reciprocal(a): return _ufunc(a, ("rdiv", 1.0))           # :374 1.0 / x
 def rsqrt(a): return _ufunc(a, ("sqrt",), ("rdiv", 1.0))     # :384 1.0 / x.sqrt()

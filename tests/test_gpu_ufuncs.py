@@ -6,7 +6,8 @@ against the exact value, computed in 80-bit long double (64-bit significand, 11 
 binary64) on the CPU, measured in units of the last place of the binary64 result. Bounds, written
 here: exp, exp2, expm1, log, log1p <= 1 ulp; log2, log10 <= 1 ulp; tanh, sinh, cosh, asinh,
 atanh <= 2 ulp; acosh <= 3 ulp (near 1 its result ~sqrt(2t) collects the roundings of
-2t + t^2, the square root, t + s and log1p); atan <= 1.5 ulp, asin, acos <= 2 ulp; sigmoid
+2t + t^2, the square root, t + s and log1p); atan <= 1.5 ulp, asin, acos <= 2 ulp; sin, cos <= 1 ulp and
+tan <= 2.5 ulp for every finite argument; sigmoid
 (the reference's 1 / (1 + exp(-x)): exp's relative error passes through two more IEEE roundings,
 and a relative error converts to up to twice as many ulps at a binade edge) <= 3 ulp; binary32
 inputs (FP64 evaluation, one rounding) <= 1 ulp of binary32. Special values (NaN, +-inf, +-0, overflow and underflow edges)
@@ -50,6 +51,10 @@ def sample(lo, hi, n, seed):
 SPECIAL = np.array([np.nan, np.inf, -np.inf, 0.0, -0.0, 1.0, -1.0, 5e-324, -5e-324, 2.2250738585072014e-308,
                     1e-300, -1e-300, 1e300, -1e300, 0.5, 2.0])
 
+TRIG_HARD = np.array([6381956970095103 * 2.0 ** 797, 5.319372648326541e+255, 1e22, 2.0 ** 20, 2.0 ** 20 + 1,
+                      2.0 ** 20 - 1, 1.5707963267948966, 3.141592653589793, 0.7853981633974483, 0.785398163397448,
+                      1e300, 1.7976931348623157e308, -1.7976931348623157e308, 355.0, 103993.0])
+
 CASES = {
     # name: (gpu fn, long double fn, inputs, bound)
     "exp": (K.exp, np.exp, np.concatenate([sample(-745.2, 709.8, 60000, 1), sample(-1e-3, 1e-3, 5000, 2),
@@ -88,6 +93,14 @@ CASES = {
                                                  sample(-1e-9, 1e-9, 5000, 70), [0.5, -0.5, 1.0000000000000002]], ), 2.0),
     "acos": (K.arccos, np.arccos, np.concatenate([sample(-1, 1, 30000, 71), 1 - np.exp(sample(-36, -1, 10000, 72)),
                                                  -1 + np.exp(sample(-36, -1, 10000, 73)), [0.5, -0.5]], ), 2.0),
+    # every finite argument: Cody-Waite below 2^20, Payne-Hanek above (incl. 6381956970095103 2^797,
+    # the binary64 closest to a multiple of pi/2, and the largest double)
+    "sin": (K.sin, np.sin, np.concatenate([sample(-10, 10, 20000, 74), sample(-1e6, 1e6, 10000, 75), np.exp(sample(14, 709, 15000, 76)),
+                                         -np.exp(sample(14, 709, 5000, 77)), TRIG_HARD, SPECIAL]), 1.0),
+    "cos": (K.cos, np.cos, np.concatenate([sample(-10, 10, 20000, 78), sample(-1e6, 1e6, 10000, 79), np.exp(sample(14, 709, 15000, 80)),
+                                         -np.exp(sample(14, 709, 5000, 81)), TRIG_HARD, SPECIAL]), 1.0),
+    "tan": (K.tan, np.tan, np.concatenate([sample(-10, 10, 20000, 82), sample(-1e6, 1e6, 10000, 83), np.exp(sample(14, 709, 15000, 84)),
+                                         -np.exp(sample(14, 709, 5000, 85)), TRIG_HARD, SPECIAL]), 2.5),
 }
 
 
