@@ -156,7 +156,11 @@ enum {
     NRK_BOP_REM = 4, /* arithmetic.rs:273-320 Rem (truncated, fmod) */
     NRK_BOP_MAXIMUM = 5, /* ufuncs.rs:169  if a > b {a} else {b} */
     NRK_BOP_MINIMUM = 6, /* ufuncs.rs:173  if a < b {a} else {b} */
-    NRK_BOP_COUNT = 7
+    NRK_BOP_COPYSIGN = 7,  /* ufuncs.rs:784  a.copysign(b), exact                       */
+    NRK_BOP_HYPOT = 8,     /* ufuncs.rs:841  a.hypot(b), within 1 ulp                   */
+    NRK_BOP_ATAN2 = 9,     /* ufuncs.rs:587  a.atan2(b) (a = y, b = x), within 1.5 ulp  */
+    NRK_BOP_LOGADDEXP = 10, /* ufuncs.rs:916 m + ln(e^-|a-m| + e^-|b-m|), within 2 ulp */
+    NRK_BOP_COUNT = 11
 };
 
 /* kernels/elementwise.rs:237-258 unary_serial / unary_dispatch with an op-coded closure:

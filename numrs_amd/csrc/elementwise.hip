@@ -24,7 +24,11 @@ __device__ __forceinline__ T apply_bop(T a, T b) {
     else if constexpr (OP == NRK_BOP_DIV) return rn_div(a, b);
     else if constexpr (OP == NRK_BOP_REM) return __builtin_fmod(a, b);  // exact, truncated like Rust's %
     else if constexpr (OP == NRK_BOP_MAXIMUM) return a > b ? a : b;     // ufuncs.rs:169
-    else return a < b ? a : b;                                           // ufuncs.rs:173 (MINIMUM)
+    else if constexpr (OP == NRK_BOP_MINIMUM) return a < b ? a : b;     // ufuncs.rs:173
+    else if constexpr (OP == NRK_BOP_COPYSIGN) return dev_copysign(a, b);
+    else if constexpr (OP == NRK_BOP_HYPOT) return (T)fm::hypot((double)a, (double)b);
+    else if constexpr (OP == NRK_BOP_ATAN2) return (T)fm::atan2((double)a, (double)b);
+    else return (T)fm::logaddexp((double)a, (double)b);                // LOGADDEXP
 }
 
 // The map applies every NRK_SOP_* step, elementary functions included.
@@ -161,6 +165,10 @@ nrk_status zip(int op, const T *a, const T *b, T *y, size_t n, hipStream_t strea
         NRK_ZIP_CASE(NRK_BOP_REM)
         NRK_ZIP_CASE(NRK_BOP_MAXIMUM)
         NRK_ZIP_CASE(NRK_BOP_MINIMUM)
+        NRK_ZIP_CASE(NRK_BOP_COPYSIGN)
+        NRK_ZIP_CASE(NRK_BOP_HYPOT)
+        NRK_ZIP_CASE(NRK_BOP_ATAN2)
+        NRK_ZIP_CASE(NRK_BOP_LOGADDEXP)
 #undef NRK_ZIP_CASE
     }
     NRK_HIP(hipGetLastError());

@@ -613,5 +613,74 @@ __device__ inline double tan(double x) {
     return __builtin_copysign(1.0, x) * v;
 }
 
+// ---- binary: hypot, atan2, logaddexp (ufuncs.rs:587, :841, :916) ----
+// hypot: the larger magnitude scaled to [1, 2), the sum of squares in double-double, its square
+// root corrected once by the residual; C semantics for infinities (hypot(inf, NaN) = inf).
+__device__ inline double hypot(double a, double b) {
+    double ax = __builtin_fabs(a), ay = __builtin_fabs(b);
+    if (ax == dinf() || ay == dinf()) return dinf();
+    if (ax != ax || ay != ay) return __builtin_nan("");
+    if (ax < ay) {
+        const double t = ax;
+        ax = ay;
+        ay = t;
+    }
+    if (ay == 0.0) return ax;
+    int e;
+    (void)frexp(ax, &e);
+    const double x = ldexp(ax, 1 - e), y = ldexp(ay, 1 - e);  // x in [1, 2)
+    const double xh = x * x, xl = fma(x, x, -xh);
+    const double yh = y * y, yl = fma(y, y, -yh);
+    double sh, sl;
+    two_sum(xh, yh, sh, sl);
+    sl += xl + yl;
+    const double r = __dsqrt_rn(sh);
+    const double rc = r + (fma(-r, r, sh) + sl) / (2.0 * r);
+    return ldexp(rc, e - 1);
+}
+// atan2(y, x) with the C / Rust special cases; otherwise the angle of |y|, |x| from the smaller
+// over the larger magnitude (its division residual carried) and the quadrant in double-double.
+__device__ inline double atan2(double y, double x) {
+    if (x != x || y != y) return __builtin_nan("");
+    const double PI_HI = 2.0 * PIO2_HI, PI_LO = 2.0 * PIO2_LO;
+    const bool xneg = __builtin_signbit(x);
+    if (y == 0.0) return xneg ? __builtin_copysign(PI_HI + PI_LO, y) : y;  // +-0 or +-pi
+    if (x == 0.0) return __builtin_copysign(PIO2_HI + PIO2_LO, y);
+    const double ax = __builtin_fabs(x), ay = __builtin_fabs(y);
+    double h, l;
+    if (ay == dinf()) {
+        if (ax == dinf()) {  // +-pi/4 or +-3pi/4
+            h = xneg ? 3.0 * (0.5 * PIO2_HI) : 0.5 * PIO2_HI;
+            l = xneg ? 3.0 * (0.5 * PIO2_LO) : 0.5 * PIO2_LO;
+            return __builtin_copysign(h + l, y);
+        }
+        return __builtin_copysign(PIO2_HI + PIO2_LO, y);
+    }
+    if (ax == dinf()) return xneg ? __builtin_copysign(PI_HI + PI_LO, y) : __builtin_copysign(0.0, y);
+    if (ay <= ax) {
+        const double t = ay / ax;
+        atan_core(t, fma(-t, ax, ay) / ax, h, l);
+    } else {
+        const double t = ax / ay;
+        double ch, cl, e;
+        atan_core(t, fma(-t, ay, ax) / ay, ch, cl);
+        two_sum(PIO2_HI, -ch, h, e);
+        l = e + (PIO2_LO - cl);
+    }
+    if (xneg) {
+        double e;
+        const double hh = h;
+        two_sum(PI_HI, -hh, h, e);
+        l = e + (PI_LO - l);
+    }
+    return __builtin_copysign(h + l, y);
+}
+// logaddexp as the reference writes it: m + ln(e^-|a - m| + e^-|b - m|), m = max(a, b) with
+// Rust's NaN-ignoring max (so logaddexp(inf, inf) is NaN, as there).
+__device__ inline double logaddexp(double a, double b) {
+    const double m = __builtin_fmax(a, b);
+    return m + log(exp(-__builtin_fabs(a - m)) + exp(-__builtin_fabs(b - m)));
+}
+
 }  // namespace fm
 }  // namespace nrk

@@ -27,7 +27,8 @@ SOP = {"add": 0, "sub": 1, "mul": 2, "div": 3, "rsub": 4, "rdiv": 5, "neg": 6, "
        "exp": 18, "exp2": 19, "expm1": 20, "log": 21, "log2": 22, "log10": 23, "log1p": 24, "tanh": 25,
        "sinh": 26, "cosh": 27, "asinh": 28, "acosh": 29, "atanh": 30, "atan": 31, "asin": 32, "acos": 33,
        "sin": 34, "cos": 35, "tan": 36}
-BOP = {"add": 0, "sub": 1, "mul": 2, "div": 3, "rem": 4, "maximum": 5, "minimum": 6}
+BOP = {"add": 0, "sub": 1, "mul": 2, "div": 3, "rem": 4, "maximum": 5, "minimum": 6, "copysign": 7, "hypot": 8,
+       "atan2": 9, "logaddexp": 10}
 AXIS = {"sum": 0, "mean": 1, "var": 2, "min": 3, "max": 4, "argmin": 5, "argmax": 6}
 
 PARALLEL_MIN_LEN = 10_000  # kernels/mod.rs:74 (kept for callers; the GPU has no CPU tiers)
@@ -173,6 +174,22 @@ def round(a): return _ufunc(a, ("round",))                   # :722 x.round(), h
 def trunc(a): return _ufunc(a, ("trunc",))                   # :732 x.trunc()
 def fract(a): return _ufunc(a, ("fract",))                   # :742 x - x.trunc()
 def sign(a): return _ufunc(a, ("sign",))                     # :756 0 / 1 / -1 (NaN -> -1)
+
+
+def _bufunc(a, b, op: str) -> np.ndarray:
+    a, b = np.asarray(a), np.asarray(b)
+    if a.shape != b.shape:
+        raise ValueError(f"shape mismatch: {a.shape} vs {b.shape}")  # the reference's ShapeMismatch
+    dt = np.float32 if a.dtype == np.float32 and b.dtype == np.float32 else np.float64
+    fn = "nrk_zip_f32" if dt == np.float32 else "nrk_zip_f64"
+    return _binary(fn, a.reshape(-1), b.reshape(-1), op, dt).reshape(a.shape)
+
+
+def copysign(mag, sgn): return _bufunc(mag, sgn, "copysign")   # ufuncs.rs:784 m.copysign(s)
+def hypot(a, b): return _bufunc(a, b, "hypot")                  # :841 a.hypot(b)
+def arctan2(y, x): return _bufunc(y, x, "atan2")                # :587 y.atan2(x)
+def logaddexp(a, b): return _bufunc(a, b, "logaddexp")          # :916
+atan2 = arctan2
 
 
 def deg2rad(a):

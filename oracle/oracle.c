@@ -343,7 +343,7 @@ int64_t ref_sum_i64(const int64_t *a, size_t n) {
 /* ------------------------------------------------------------------------------------------ */
 /* elementwise restatements: op codes match include/numrs_amd.h NRK_SOP_* / NRK_BOP_*          */
 /* ------------------------------------------------------------------------------------------ */
-#define DEFINE_EW(T, SUF, SQRT, FMOD, FABS)                                                 \
+#define DEFINE_EW(T, SUF, SQRT, FMOD, FABS, COPYSIGN)                                               \
     static T sop_##SUF(int op, T s, T x) {                                                  \
         switch (op) {                                                                       \
             case 0: return x + s;  /* Array::add_scalar, operations.rs:448 */              \
@@ -380,13 +380,14 @@ int64_t ref_sum_i64(const int64_t *a, size_t n) {
                 case 4: y[i] = FMOD(p, q); break;                                           \
                 case 5: y[i] = p > q ? p : q; break;                                        \
                 case 6: y[i] = p < q ? p : q; break;                                        \
+                case 7: y[i] = COPYSIGN(p, q); break; /* ufuncs.rs:801 m.copysign(s) */      \
                 default: y[i] = p;                                                          \
             }                                                                               \
         }                                                                                   \
     }
 
-DEFINE_EW(double, f64, sqrt, fmod, fabs)
-DEFINE_EW(float, f32, sqrtf, fmodf, fabsf)
+DEFINE_EW(double, f64, sqrt, fmod, fabs, copysign)
+DEFINE_EW(float, f32, sqrtf, fmodf, fabsf, copysignf)
 
 /* ------------------------------------------------------------------------------------------ */
 /* GEMM restatement                                                                           */

@@ -31,7 +31,9 @@ def test_each_scalar_op_correctly_rounded(gpu, oracle, dtype, step):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-@pytest.mark.parametrize("op", list(K.BOP))
+# hypot, atan2 and logaddexp go through elementary functions: their ulp bounds are tested in
+# test_gpu_ufuncs.py; every other binary op is one IEEE operation and must match bit for bit
+@pytest.mark.parametrize("op", [o for o in K.BOP if o not in ("hypot", "atan2", "logaddexp")])
 def test_zip_bit_exact(gpu, oracle, dtype, op):
     n = 300_007
     a = rng.normal(rng.BASE_SEED + 1000, n, dtype)

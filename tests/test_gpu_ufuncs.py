@@ -213,3 +213,80 @@ def test_elementary_steps_compose_in_one_chain(gpu):
     """log1p(exp(x)) as one two-step chain equals the two single-step maps, bit for bit."""
     x = sample(-30, 30, 10000, 40)
     assert_same(K.unary_dispatch(x, [("exp",), ("log1p",)]), K.log1p(K.exp(x)))
+
+
+# ---- binary ufuncs on the zip kernel (ufuncs.rs:587, 784, 841, 916)
+
+def _wide(n, seed):
+    r = np.random.default_rng(seed)
+    return r.choice([-1.0, 1.0], n) * np.exp(r.uniform(-690, 690, n))
+
+
+def test_copysign_bit_exact(gpu):
+    x = EXACT_X
+    s = EXACT_X[::-1].copy()
+    assert_same(K.copysign(x, s), np.copysign(x, s))
+
+
+@pytest.mark.parametrize("name,bound", [("hypot", 1.0), ("atan2", 1.5), ("logaddexp", 2.0)])
+def test_binary_elementary_within_ulp_bound(gpu, name, bound):
+    r = np.random.default_rng(90)
+    a = np.concatenate([_wide(20000, 91), r.uniform(-10, 10, 20000), r.uniform(-1e-3, 1e-3, 5000)])
+    b = np.concatenate([_wide(20000, 92), r.uniform(-10, 10, 20000), r.uniform(-1e3, 1e3, 5000)])
+    fn = {"hypot": K.hypot, "atan2": K.arctan2, "logaddexp": K.logaddexp}[name]
+    al, bl = a.astype(LD), b.astype(LD)
+    with np.errstate(all="ignore"):
+        if name == "hypot":
+            exact = np.hypot(al, bl)
+        elif name == "atan2":
+            exact = np.arctan2(al, bl)
+        else:
+            exact = None
+    got = fn(a, b)
+    if exact is None:
+        # logaddexp is the reference's f64 expression m + ln(e^-|a-m| + e^-|b-m|) (ufuncs.rs:933-936),
+        # whose sum absorbs the smaller term: compare it with that expression evaluated in f64 with
+        # correctly rounded exp and ln. One ulp of exp can move the rounded sum s in [1, 2] by
+        # 2^-52, so ln s by 2^-52, plus ln's own ulp (<= 2^-53 on [0, ln 2]) and the final rounding.
+        m = np.fmax(a, b)
+        s = (np.exp(-np.abs(a - m).astype(LD)).astype(np.float64) + np.exp(-np.abs(b - m).astype(LD)).astype(np.float64))
+        want = m + np.log(s.astype(LD)).astype(np.float64)
+        tol = 1.5 * 2.0**-52 + np.spacing(np.abs(want))
+        dev = np.abs(got - want) / tol
+        w = int(np.argmax(dev))
+        print(f"{name}: max {dev.max():.3f} of the bound at ({a[w]!r}, {b[w]!r})")
+        assert dev.max() <= 1.0
+        err = ulp_err(got, np.asarray(want, LD))
+        bound = np.inf
+    else:
+        err = ulp_err(got, exact)
+    w = int(np.argmax(err))
+    print(f"{name}: max {err.max():.3f} ulp at ({a[w]!r}, {b[w]!r})")
+    assert err.max() <= bound
+    # special values: C / Rust semantics (numpy's float64 hypot / arctan2 follow C), the
+    # reference's formula for logaddexp
+    sp = np.array([0.0, -0.0, 1.0, -1.0, np.inf, -np.inf, np.nan, 5e-324, 1e308, -2.5])
+    A, B = np.meshgrid(sp, sp)
+    A, B = A.ravel(), B.ravel()
+    with np.errstate(all="ignore"):
+        if name == "hypot":
+            want = np.hypot(A, B)
+        elif name == "atan2":
+            want = np.arctan2(A, B)
+        else:
+            m = np.fmax(A, B)
+            want = m + np.log(np.exp(-np.abs(A - m)) + np.exp(-np.abs(B - m)))
+    got = fn(A, B)
+    same = (np.isnan(want) & np.isnan(got)) | ((want == got) & (np.signbit(want) == np.signbit(got)))
+    close = np.isfinite(want) & np.isfinite(got) & (np.abs(got - want) <= 2 * np.spacing(np.abs(want)))
+    assert np.all(same | close), (name, A[~(same | close)], B[~(same | close)], got[~(same | close)], want[~(same | close)])
+
+
+def test_binary_f32_within_one_ulp(gpu):
+    r = np.random.default_rng(93)
+    a = r.uniform(-100, 100, 20000).astype(np.float32)
+    b = r.uniform(-100, 100, 20000).astype(np.float32)
+    for fn, ref in ((K.hypot, np.hypot), (K.arctan2, np.arctan2)):
+        got = fn(a, b)
+        assert got.dtype == np.float32
+        assert ulp_err(got, ref(a.astype(LD), b.astype(LD)), np.float32).max() <= 1.0

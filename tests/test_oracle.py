@@ -161,6 +161,16 @@ def test_numpy_arithmetic_fixtures(oracle, numpy_ref):
         np.testing.assert_array_equal(got, want, err_msg=name)
 
 
+def test_copysign_matches_ieee(oracle):
+    """Rust's f64::copysign is the IEEE copySign: magnitude of a, sign bit of b, NaN signs too."""
+    a = np.array([0.0, -0.0, 1.5, -2.0, np.inf, -np.inf, np.nan, 5e-324, 3.0, -4.0])
+    b = np.array([-1.0, 1.0, -0.0, 0.0, -np.nan, np.nan, -1.0, -np.inf, np.nan, -np.nan])
+    for t in (np.float64, np.float32):
+        want, got = np.copysign(a.astype(t), b.astype(t)), oracle.ref_zip(7, a.astype(t), b.astype(t))
+        np.testing.assert_array_equal(got.view(np.uint64 if t == np.float64 else np.uint32),
+                                      want.view(np.uint64 if t == np.float64 else np.uint32))
+
+
 def test_two_roundings_not_fma(oracle):
     """y = x*1.5 + (-0.25) is two roundings: bit-equal to numpy's separate ops, and for some x
     different from the fused (single-rounding) value."""
