@@ -138,7 +138,27 @@ enum {
     NRK_SOP_SIN = 34,
     NRK_SOP_COS = 35,
     NRK_SOP_TAN = 36,
-    NRK_SOP_COUNT = 37
+    /* ufuncs.rs:1243-1480 activations, special functions and steps, as the reference's f64
+     * expressions (one IEEE rounding per operation; its exp / ln / tanh / sin within the ulp
+     * bounds above). s is the scalar where named. Elementwise map only. */
+    NRK_SOP_RELU = 37,        /* :1380 x.max(0.0) (NaN -> 0)                                  */
+    NRK_SOP_LEAKY_RELU = 38,  /* :1390 x > 0 ? x : s * x                                      */
+    NRK_SOP_ELU = 39,         /* :1322 x > 0 ? x : s * (exp(x) - 1)                           */
+    NRK_SOP_SELU = 40,        /* :1334 lambda * (x > 0 ? x : alpha * (exp(x) - 1))            */
+    NRK_SOP_HARDSIGMOID = 41, /* :1344 0 / 1 outside [-3, 3], else x / 6 + 0.5                */
+    NRK_SOP_HARDSWISH = 42,   /* :1362 0 / x outside [-3, 3], else x * (x + 3) / 6            */
+    NRK_SOP_SOFTPLUS = 43,    /* :1291 x > 20 ? x : x < -20 ? exp(x) : ln(1 + exp(x))         */
+    NRK_SOP_MISH = 44,        /* :1309 x * tanh(x > 20 ? x : ln(1 + exp(x)))                  */
+    NRK_SOP_SILU = 45,        /* :1275 x / (1 + exp(-x))                                      */
+    NRK_SOP_GELU = 46,        /* :1262 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3)))        */
+    NRK_SOP_LOGIT = 47,       /* :1251 ln(x / (1 - x))                                        */
+    NRK_SOP_SINC = 48,        /* :527  x == 0 ? 1 : sin(pi x) / (pi x)                        */
+    NRK_SOP_SMOOTHSTEP = 49,  /* :1467 t t (3 - 2 t), after SUB e0, DIV (e1 - e0), clamp 0..1 */
+    NRK_SOP_SMOOTHERSTEP = 50, /* :1480 t t t (t (6 t - 15) + 10), the same prefix            */
+    NRK_SOP_ERF = 51,         /* :1081 erf_scalar (Chebyshev fit of erfc, |error| < 1.2e-7)   */
+    NRK_SOP_ERFC = 52,        /* :1118 1 - erf_scalar(x)                                      */
+    NRK_SOP_DIGAMMA = 53,     /* :1202 recurrence to x >= 6, then the asymptotic series       */
+    NRK_SOP_COUNT = 54
 };
 typedef struct nrk_scalar_op {
     int32_t op;     /* NRK_SOP_* */
@@ -160,7 +180,8 @@ enum {
     NRK_BOP_HYPOT = 8,     /* ufuncs.rs:841  a.hypot(b), within 1 ulp                   */
     NRK_BOP_ATAN2 = 9,     /* ufuncs.rs:587  a.atan2(b) (a = y, b = x), within 1.5 ulp  */
     NRK_BOP_LOGADDEXP = 10, /* ufuncs.rs:916 m + ln(e^-|a-m| + e^-|b-m|), within 2 ulp */
-    NRK_BOP_COUNT = 11
+    NRK_BOP_PRELU = 11,    /* ufuncs.rs:1394 a > 0 ? a : b * a (b = alpha)             */
+    NRK_BOP_COUNT = 12
 };
 
 /* kernels/elementwise.rs:237-258 unary_serial / unary_dispatch with an op-coded closure:
@@ -250,6 +271,26 @@ nrk_status nrk_reduce_axis_f64(int32_t op, const double *x, size_t outer, size_t
 nrk_status nrk_reduce_axis_f32(int32_t op, const float *x, size_t outer, size_t len, size_t inner, size_t ddof, void *out);
 nrk_status nrk_dev_reduce_axis_f64(int32_t op, const double *x, size_t outer, size_t len, size_t inner, size_t ddof, void *out, nrk_stream stream);
 nrk_status nrk_dev_reduce_axis_f32(int32_t op, const float *x, size_t outer, size_t len, size_t inner, size_t ddof, void *out, nrk_stream stream);
+
+/* ufuncs.rs:946-979, 1060: dot (sum of the rounded products a[i] * b[i]; 0 for n = 0),
+ * norm_l1 (sum |x|), norm_l2 (sqrt of sum x * x) and std (sqrt of var with divisor n - ddof;
+ * the reference's std is ddof = 0). Sums run in the fixed tree of sum above. */
+nrk_status nrk_dot_f64(const double *a, const double *b, size_t n, double *out);
+nrk_status nrk_dot_f32(const float *a, const float *b, size_t n, float *out);
+nrk_status nrk_dev_dot_f64(const double *a, const double *b, size_t n, double *out, nrk_stream stream);
+nrk_status nrk_dev_dot_f32(const float *a, const float *b, size_t n, float *out, nrk_stream stream);
+nrk_status nrk_norm_l1_f64(const double *a, size_t n, double *out);
+nrk_status nrk_norm_l1_f32(const float *a, size_t n, float *out);
+nrk_status nrk_norm_l2_f64(const double *a, size_t n, double *out);
+nrk_status nrk_norm_l2_f32(const float *a, size_t n, float *out);
+nrk_status nrk_dev_norm_l1_f64(const double *a, size_t n, double *out, nrk_stream stream);
+nrk_status nrk_dev_norm_l1_f32(const float *a, size_t n, float *out, nrk_stream stream);
+nrk_status nrk_dev_norm_l2_f64(const double *a, size_t n, double *out, nrk_stream stream);
+nrk_status nrk_dev_norm_l2_f32(const float *a, size_t n, float *out, nrk_stream stream);
+nrk_status nrk_std_f64(const double *a, size_t n, size_t ddof, double *out);
+nrk_status nrk_std_f32(const float *a, size_t n, size_t ddof, float *out);
+nrk_status nrk_dev_std_f64(const double *a, size_t n, size_t ddof, double *out, nrk_stream stream);
+nrk_status nrk_dev_std_f32(const float *a, size_t n, size_t ddof, float *out, nrk_stream stream);
 
 /* ------------------------------------------------------------------------------------------ */
 /* Fused elementwise -> reduction (one HBM pass)                                               */

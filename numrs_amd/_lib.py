@@ -69,6 +69,14 @@ _SIGS: dict[str, list] = {
     "nrk_mean_f64": [vp, c_size, vp], "nrk_mean_f32": [vp, c_size, vp],
     "nrk_sum_sq_dev_f64": [vp, c_size, C.c_double, vp], "nrk_sum_sq_dev_f32": [vp, c_size, C.c_float, vp],
     "nrk_var_f64": [vp, c_size, c_size, vp], "nrk_var_f32": [vp, c_size, c_size, vp],
+    "nrk_dot_f64": [vp, vp, c_size, vp], "nrk_dot_f32": [vp, vp, c_size, vp],
+    "nrk_dev_dot_f64": [vp, vp, c_size, vp, vp], "nrk_dev_dot_f32": [vp, vp, c_size, vp, vp],
+    "nrk_norm_l1_f64": [vp, c_size, vp], "nrk_norm_l1_f32": [vp, c_size, vp],
+    "nrk_norm_l2_f64": [vp, c_size, vp], "nrk_norm_l2_f32": [vp, c_size, vp],
+    "nrk_dev_norm_l1_f64": [vp, c_size, vp, vp], "nrk_dev_norm_l1_f32": [vp, c_size, vp, vp],
+    "nrk_dev_norm_l2_f64": [vp, c_size, vp, vp], "nrk_dev_norm_l2_f32": [vp, c_size, vp, vp],
+    "nrk_std_f64": [vp, c_size, c_size, vp], "nrk_std_f32": [vp, c_size, c_size, vp],
+    "nrk_dev_std_f64": [vp, c_size, c_size, vp, vp], "nrk_dev_std_f32": [vp, c_size, c_size, vp, vp],
     "nrk_min_f64": [vp, c_size, vp], "nrk_min_f32": [vp, c_size, vp],
     "nrk_max_f64": [vp, c_size, vp], "nrk_max_f32": [vp, c_size, vp],
     "nrk_prod_f64": [vp, c_size, vp], "nrk_prod_f32": [vp, c_size, vp],

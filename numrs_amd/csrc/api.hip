@@ -87,6 +87,33 @@ nrk_status dev_sum_sq_dev(const T *a, size_t n, T mean, T *out, hipStream_t s) {
     return seg_reduce<T>(c, a, 1, n, 0.0, nullptr, &o, s);
 }
 
+// ufuncs.rs:966-979: norm_l2 = sqrt(sum x*x), norm_l1 = sum |x| (one fused pass each);
+// ufuncs.rs:1060 std = var.sqrt().
+template <typename T>
+nrk_status dev_norm(int32_t sop, bool root, const T *a, size_t n, T *out, hipStream_t s) {
+    nrk_scalar_op ops[1] = {{sop, 0, 0.0}};
+    Chain c;
+    NRK_TRY(make_chain(ops, 1, &c));
+    nrk_reduce_outputs o{};
+    o.sum = out;
+    NRK_TRY(seg_reduce<T>(c, a, 1, n, 0.0, nullptr, &o, s));
+    return root ? sqrt_scalar<T>(out, s) : NRK_OK;
+}
+template <typename T>
+nrk_status dev_std(const T *a, size_t n, size_t ddof, T *out, hipStream_t s) {
+    NRK_TRY(dev_var<T>(a, n, ddof, out, s));
+    return sqrt_scalar<T>(out, s);
+}
+template <typename T>
+nrk_status host_dot(const T *a, const T *b, size_t n, T *out) {
+    NRK_TRY(ensure_device());
+    if (n && !b) return fail(NRK_ERR_INVALID_ARGUMENT, "b is NULL");
+    void *db = nullptr;
+    NRK_TRY(staging(2, n * sizeof(T) + 16, &db));
+    if (n) NRK_HIP(hipMemcpyAsync(db, b, n * sizeof(T), hipMemcpyHostToDevice, nullptr));
+    return host_call<T, T>(a, n, out, 1, [&](const T *d, T *dout, hipStream_t s) { return dot<T>(d, (const T *)db, n, dout, s); });
+}
+
 template <typename T>
 nrk_status dev_axis(int32_t op, const T *x, size_t outer, size_t len, size_t inner, size_t ddof, void *out,
                     hipStream_t s) {
@@ -213,6 +240,30 @@ nrk_status nrk_sum_sq_dev_f64(const double *a, size_t n, double mean, double *ou
 nrk_status nrk_sum_sq_dev_f32(const float *a, size_t n, float mean, float *out) {
     return host_call<float, float>(a, n, out, 1, [&](const float *d, float *o, hipStream_t s) { return dev_sum_sq_dev<float>(d, n, mean, o, s); });
 }
+
+// ---- ufuncs.rs reductions: dot, norms, std ----
+nrk_status nrk_dot_f64(const double *a, const double *b, size_t n, double *out) { return host_dot<double>(a, b, n, out); }
+nrk_status nrk_dot_f32(const float *a, const float *b, size_t n, float *out) { return host_dot<float>(a, b, n, out); }
+nrk_status nrk_dev_dot_f64(const double *a, const double *b, size_t n, double *out, nrk_stream stream) { return dot<double>(a, b, n, out, S(stream)); }
+nrk_status nrk_dev_dot_f32(const float *a, const float *b, size_t n, float *out, nrk_stream stream) { return dot<float>(a, b, n, out, S(stream)); }
+#define NRK_NORM(NAME, T, SOP, ROOT)                                                                              \
+    nrk_status nrk_##NAME(const T *a, size_t n, T *out) {                                                         \
+        return host_call<T, T>(a, n, out, 1, [&](const T *d, T *o, hipStream_t s) { return dev_norm<T>(SOP, ROOT, d, n, o, s); }); \
+    }                                                                                                             \
+    nrk_status nrk_dev_##NAME(const T *a, size_t n, T *out, nrk_stream stream) { return dev_norm<T>(SOP, ROOT, a, n, out, S(stream)); }
+NRK_NORM(norm_l1_f64, double, NRK_SOP_ABS, false)
+NRK_NORM(norm_l1_f32, float, NRK_SOP_ABS, false)
+NRK_NORM(norm_l2_f64, double, NRK_SOP_SQUARE, true)
+NRK_NORM(norm_l2_f32, float, NRK_SOP_SQUARE, true)
+#undef NRK_NORM
+nrk_status nrk_std_f64(const double *a, size_t n, size_t ddof, double *out) {
+    return host_call<double, double>(a, n, out, 1, [&](const double *d, double *o, hipStream_t s) { return dev_std<double>(d, n, ddof, o, s); });
+}
+nrk_status nrk_std_f32(const float *a, size_t n, size_t ddof, float *out) {
+    return host_call<float, float>(a, n, out, 1, [&](const float *d, float *o, hipStream_t s) { return dev_std<float>(d, n, ddof, o, s); });
+}
+nrk_status nrk_dev_std_f64(const double *a, size_t n, size_t ddof, double *out, nrk_stream stream) { return dev_std<double>(a, n, ddof, out, S(stream)); }
+nrk_status nrk_dev_std_f32(const float *a, size_t n, size_t ddof, float *out, nrk_stream stream) { return dev_std<float>(a, n, ddof, out, S(stream)); }
 
 // ---- flat reductions, device pointers ----
 nrk_status nrk_dev_sum_f64(const double *a, size_t n, double *out, nrk_stream stream) { return dev_sum<double>(a, n, out, S(stream)); }

@@ -200,6 +200,23 @@ __device__ __forceinline__ void apply_chain_n(const Chain &c, T (&v)[N]) {
                         NRK_CHAIN_CASE(NRK_SOP_SIN, via_f64(x, [](double d) { return fm::sin(d); }))
                         NRK_CHAIN_CASE(NRK_SOP_COS, via_f64(x, [](double d) { return fm::cos(d); }))
                         NRK_CHAIN_CASE(NRK_SOP_TAN, via_f64(x, [](double d) { return fm::tan(d); }))
+                        NRK_CHAIN_CASE(NRK_SOP_RELU, via_f64(x, [](double d) { return fm::relu(d); }))
+                        NRK_CHAIN_CASE(NRK_SOP_LEAKY_RELU, (T)((double)x > 0.0 ? (double)x : (double)c.s[k] * (double)x))
+                        NRK_CHAIN_CASE(NRK_SOP_ELU, (T)fm::elu((double)x, c.s[k]))
+                        NRK_CHAIN_CASE(NRK_SOP_SELU, via_f64(x, [](double d) { return fm::selu(d); }))
+                        NRK_CHAIN_CASE(NRK_SOP_HARDSIGMOID, via_f64(x, [](double d) { return fm::hardsigmoid(d); }))
+                        NRK_CHAIN_CASE(NRK_SOP_HARDSWISH, via_f64(x, [](double d) { return fm::hardswish(d); }))
+                        NRK_CHAIN_CASE(NRK_SOP_SOFTPLUS, via_f64(x, [](double d) { return fm::softplus(d); }))
+                        NRK_CHAIN_CASE(NRK_SOP_MISH, via_f64(x, [](double d) { return fm::mish(d); }))
+                        NRK_CHAIN_CASE(NRK_SOP_SILU, via_f64(x, [](double d) { return fm::silu(d); }))
+                        NRK_CHAIN_CASE(NRK_SOP_GELU, via_f64(x, [](double d) { return fm::gelu(d); }))
+                        NRK_CHAIN_CASE(NRK_SOP_LOGIT, via_f64(x, [](double d) { return fm::logit(d); }))
+                        NRK_CHAIN_CASE(NRK_SOP_SINC, via_f64(x, [](double d) { return fm::sinc(d); }))
+                        NRK_CHAIN_CASE(NRK_SOP_SMOOTHSTEP, via_f64(x, [](double d) { return fm::smoothstep_poly(d); }))
+                        NRK_CHAIN_CASE(NRK_SOP_SMOOTHERSTEP, via_f64(x, [](double d) { return fm::smootherstep_poly(d); }))
+                        NRK_CHAIN_CASE(NRK_SOP_ERF, via_f64(x, [](double d) { return fm::erf_ref(d); }))
+                        NRK_CHAIN_CASE(NRK_SOP_ERFC, via_f64(x, [](double d) { return 1.0 - fm::erf_ref(d); }))
+                        NRK_CHAIN_CASE(NRK_SOP_DIGAMMA, via_f64(x, [](double d) { return fm::digamma(d); }))
                         default: break;
                     }
                 }

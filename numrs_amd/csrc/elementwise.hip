@@ -28,6 +28,7 @@ __device__ __forceinline__ T apply_bop(T a, T b) {
     else if constexpr (OP == NRK_BOP_COPYSIGN) return dev_copysign(a, b);
     else if constexpr (OP == NRK_BOP_HYPOT) return (T)fm::hypot((double)a, (double)b);
     else if constexpr (OP == NRK_BOP_ATAN2) return (T)fm::atan2((double)a, (double)b);
+    else if constexpr (OP == NRK_BOP_PRELU) return a > (T)0 ? a : rn_mul(b, a);  // ufuncs.rs:1394
     else return (T)fm::logaddexp((double)a, (double)b);                // LOGADDEXP
 }
 
@@ -169,6 +170,7 @@ nrk_status zip(int op, const T *a, const T *b, T *y, size_t n, hipStream_t strea
         NRK_ZIP_CASE(NRK_BOP_HYPOT)
         NRK_ZIP_CASE(NRK_BOP_ATAN2)
         NRK_ZIP_CASE(NRK_BOP_LOGADDEXP)
+        NRK_ZIP_CASE(NRK_BOP_PRELU)
 #undef NRK_ZIP_CASE
     }
     NRK_HIP(hipGetLastError());

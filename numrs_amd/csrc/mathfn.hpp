@@ -682,5 +682,82 @@ __device__ inline double logaddexp(double a, double b) {
     return m + log(exp(-__builtin_fabs(a - m)) + exp(-__builtin_fabs(b - m)));
 }
 
+// ---- activations, special functions and steps, as the reference writes them (ufuncs.rs
+// :520-1480). Each is the reference's f64 expression with one IEEE rounding per operation (the
+// library is built without FP contraction) over the elementary functions above, so it differs
+// from the reference only where those differ from correctly rounded values.
+
+__device__ inline double relu(double x) { return x > 0.0 ? x : 0.0; }  // x.max(0.0); NaN -> 0
+__device__ inline double elu(double x, double alpha) { return x > 0.0 ? x : alpha * (exp(x) - 1.0); }
+__device__ inline double selu(double x) {
+    const double lambda = 1.0507009873554805, alpha = 1.6732632423543772;  // the Rust literals, rounded
+    return lambda * (x > 0.0 ? x : alpha * (exp(x) - 1.0));
+}
+__device__ inline double hardsigmoid(double x) {
+    return x <= -3.0 ? 0.0 : (x >= 3.0 ? 1.0 : x / 6.0 + 0.5);
+}
+__device__ inline double hardswish(double x) {
+    return x <= -3.0 ? 0.0 : (x >= 3.0 ? x : x * (x + 3.0) / 6.0);
+}
+__device__ inline double softplus(double x) {
+    return x > 20.0 ? x : (x < -20.0 ? exp(x) : log(1.0 + exp(x)));
+}
+__device__ inline double mish(double x) { return x * tanh(x > 20.0 ? x : log(1.0 + exp(x))); }
+__device__ inline double silu(double x) { return x / (1.0 + exp(-x)); }
+__device__ inline double gelu(double x) {
+    const double c = 0x1.9884533d43651p-1;  // (2.0 / PI).sqrt() in f64: two roundings
+    return 0.5 * x * (1.0 + tanh(c * (x + 0.044715 * (x * x * x))));  // powi(3) = (x x) x
+}
+__device__ inline double logit(double x) { return log(x / (1.0 - x)); }
+__device__ inline double sinc(double x) {
+    if (x == 0.0) return 1.0;
+    const double px = 0x1.921fb54442d18p+1 * x;  // PI * x, rounded
+    return sin(px) / px;
+}
+__device__ inline double smoothstep_poly(double t) { return t * t * (3.0 - 2.0 * t); }
+__device__ inline double smootherstep_poly(double t) { return t * t * t * (t * (t * 6.0 - 15.0) + 10.0); }
+
+// erf_scalar (ufuncs.rs:1081): the Chebyshev fit of erfc from Numerical Recipes (fractional error
+// below 1.2e-7), evaluated in the reference's operation order.
+__device__ inline double erf_ref(double x) {
+    const double t = 1.0 / (1.0 + 0.5 * __builtin_fabs(x));
+    double p = 0.17087277;
+    p = -0.82215223 + t * p;
+    p = 1.48851587 + t * p;
+    p = -1.13520398 + t * p;
+    p = 0.27886807 + t * p;
+    p = -0.18628806 + t * p;
+    p = 0.09678418 + t * p;
+    p = 0.37409196 + t * p;
+    p = 1.00002368 + t * p;
+    const double tau = t * exp(-x * x - 1.26551223 + t * p);
+    return x >= 0.0 ? 1.0 - tau : tau - 1.0;
+}
+
+// digamma (ufuncs.rs:1198-1231): NaN for x <= 0; below 6 the recurrence
+// psi(x) = psi(x + 1) - 1/x, each x + 1 rounded; from 6 on
+// ln x - 1/(2x) - x^-2 (1/12 - x^-2 (1/120 - x^-2 / 252)).
+__device__ inline double digamma_asym(double x) {
+    const double inv_x = 1.0 / x;
+    const double inv_x2 = inv_x * inv_x;
+    return log(x) - 0.5 * inv_x - inv_x2 * (1.0 / 12.0 - inv_x2 * (1.0 / 120.0 - inv_x2 / 252.0));
+}
+__device__ inline double digamma(double x) {
+    if (x <= 0.0) return __builtin_nan("");
+    int k = 0;
+    double xk = x;
+    while (xk < 6.0) {  // at most 6 steps for x > 0
+        xk = xk + 1.0;
+        ++k;
+    }
+    double r = digamma_asym(xk);
+    for (int i = k - 1; i >= 0; --i) {  // unwind: r = psi(x_i + 1) - 1/x_i
+        double xi = x;
+        for (int j = 0; j < i; ++j) xi = xi + 1.0;
+        r = r - 1.0 / xi;
+    }
+    return r;
+}
+
 }  // namespace fm
 }  // namespace nrk

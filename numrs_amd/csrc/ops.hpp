@@ -29,6 +29,10 @@ nrk_status strided_reduce(int op, const T *x, size_t outer, size_t len, size_t i
 nrk_status sum_i64(const int64_t *x, size_t n, int64_t *out, hipStream_t stream);
 template <typename T>
 nrk_status prod(const T *x, size_t n, T *out, hipStream_t stream);
+template <typename T>
+nrk_status dot(const T *a, const T *b, size_t n, T *out, hipStream_t stream);
+template <typename T>
+nrk_status sqrt_scalar(T *p, hipStream_t stream);
 
 template <typename T>
 nrk_status map(const Chain &chain, const T *x, T *y, size_t n, hipStream_t stream);

@@ -985,4 +985,98 @@ nrk_status prod(const T *x, size_t n, T *out, hipStream_t stream) {
 template nrk_status prod<float>(const float *, size_t, float *, hipStream_t);
 template nrk_status prod<double>(const double *, size_t, double *, hipStream_t);
 
+// ---------------- dot: sum of rounded products a[i] * b[i] in a fixed tree ----------------
+// ufuncs.rs:946-963 sums the rounded products in T. Each segment of RB * DOT_PER elements gives
+// one partial (per-thread sums over DOT_PER products, then the wave and workgroup trees), so the
+// tree depends only on n: the result is the same bits for every grid.
+constexpr int DOT_PER = 16;
+
+template <typename T>
+__global__ __launch_bounds__(RB) void dot_kernel(const T *a, const T *b, size_t n, size_t nseg, T *partials) {
+    __shared__ T sred[2][WAVES];
+    const int t = threadIdx.x;
+    int parity = 0;
+    for (size_t seg = blockIdx.x; seg < nseg; seg += gridDim.x, parity ^= 1) {
+        const size_t off = seg * (size_t)(RB * DOT_PER);
+        T acc = (T)0;
+        if (off + (size_t)RB * DOT_PER <= n) {
+            T pa[DOT_PER], pb[DOT_PER];
+#pragma unroll
+            for (int j = 0; j < DOT_PER; ++j) {
+                pa[j] = a[off + (size_t)j * RB + t];
+                pb[j] = b[off + (size_t)j * RB + t];
+            }
+#pragma unroll
+            for (int j = 0; j < DOT_PER; ++j) acc = rn_add(acc, rn_mul(pa[j], pb[j]));
+        } else {
+            for (int j = 0; j < DOT_PER; ++j) {
+                const size_t pos = off + (size_t)j * RB + t;
+                if (pos < n) acc = rn_add(acc, rn_mul(a[pos], b[pos]));
+            }
+        }
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) acc = rn_add(acc, shfl_xor(acc, m));
+        if ((t & 63) == 0) sred[parity][t >> 6] = acc;
+        __syncthreads();
+        if (t == 0) {
+            T s = sred[parity][0];
+            for (int k = 1; k < WAVES; ++k) s = rn_add(s, sred[parity][k]);
+            partials[seg] = s;
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(FB) void sum_finish_kernel(const T *partials, size_t nseg, T *out) {
+    __shared__ T sred[FB / 64];
+    const int t = threadIdx.x;
+    T s = (T)0;
+    for (size_t k = t; k < nseg; k += FB) s = rn_add(s, partials[k]);
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) s = rn_add(s, shfl_xor(s, m));
+    if ((t & 63) == 0) sred[t >> 6] = s;
+    __syncthreads();
+    if (t == 0) {
+        T tot = sred[0];
+        for (int k = 1; k < FB / 64; ++k) tot = rn_add(tot, sred[k]);
+        *out = tot;
+    }
+}
+
+template <typename T>
+nrk_status dot(const T *a, const T *b, size_t n, T *out, hipStream_t stream) {
+    NRK_TRY(ensure_device());
+    if (!out) return fail(NRK_ERR_INVALID_ARGUMENT, "out is NULL");
+    if (n == 0) {  // the empty sum
+        hipLaunchKernelGGL(store_kernel<T>, dim3(1), dim3(1), 0, stream, out, (T)0);
+        NRK_HIP(hipGetLastError());
+        return NRK_OK;
+    }
+    if (!a || !b) return fail(NRK_ERR_INVALID_ARGUMENT, "a or b is NULL");
+    const size_t nseg = ceil_div(n, (size_t)RB * DOT_PER);
+    Scratch s;
+    NRK_TRY(get_scratch(stream, 0, nseg * sizeof(T), &s));
+    const int grid = (int)std::min<size_t>(nseg, (size_t)cu_count() * 8);
+    hipLaunchKernelGGL(dot_kernel<T>, dim3(grid), dim3(RB), 0, stream, a, b, n, nseg, (T *)s.partials);
+    NRK_HIP(hipGetLastError());
+    hipLaunchKernelGGL(sum_finish_kernel<T>, dim3(1), dim3(FB), 0, stream, (const T *)s.partials, nseg, out);
+    NRK_HIP(hipGetLastError());
+    return NRK_OK;
+}
+template nrk_status dot<float>(const float *, const float *, size_t, float *, hipStream_t);
+template nrk_status dot<double>(const double *, const double *, size_t, double *, hipStream_t);
+
+// y = sqrt(y) for one device scalar (norm_l2 = sqrt of the sum of squares, std = sqrt of var).
+template <typename T>
+__global__ void sqrt_scalar_kernel(T *p) { *p = rn_sqrt(*p); }
+
+template <typename T>
+nrk_status sqrt_scalar(T *p, hipStream_t stream) {
+    hipLaunchKernelGGL(sqrt_scalar_kernel<T>, dim3(1), dim3(1), 0, stream, p);
+    NRK_HIP(hipGetLastError());
+    return NRK_OK;
+}
+template nrk_status sqrt_scalar<float>(float *, hipStream_t);
+template nrk_status sqrt_scalar<double>(double *, hipStream_t);
+
 }  // namespace nrk

@@ -26,9 +26,12 @@ SOP = {"add": 0, "sub": 1, "mul": 2, "div": 3, "rsub": 4, "rdiv": 5, "neg": 6, "
        "floor": 10, "ceil": 11, "trunc": 12, "round": 13, "fract": 14, "sign": 15, "clamp_lo": 16, "clamp_hi": 17,
        "exp": 18, "exp2": 19, "expm1": 20, "log": 21, "log2": 22, "log10": 23, "log1p": 24, "tanh": 25,
        "sinh": 26, "cosh": 27, "asinh": 28, "acosh": 29, "atanh": 30, "atan": 31, "asin": 32, "acos": 33,
-       "sin": 34, "cos": 35, "tan": 36}
+       "sin": 34, "cos": 35, "tan": 36, "relu": 37, "leaky_relu": 38, "elu": 39, "selu": 40,
+       "hardsigmoid": 41, "hardswish": 42, "softplus": 43, "mish": 44, "silu": 45, "gelu": 46,
+       "logit": 47, "sinc": 48, "smoothstep": 49, "smootherstep": 50, "erf": 51, "erfc": 52,
+       "digamma": 53}
 BOP = {"add": 0, "sub": 1, "mul": 2, "div": 3, "rem": 4, "maximum": 5, "minimum": 6, "copysign": 7, "hypot": 8,
-       "atan2": 9, "logaddexp": 10}
+       "atan2": 9, "logaddexp": 10, "prelu": 11}
 AXIS = {"sum": 0, "mean": 1, "var": 2, "min": 3, "max": 4, "argmin": 5, "argmax": 6}
 
 PARALLEL_MIN_LEN = 10_000  # kernels/mod.rs:74 (kept for callers; the GPU has no CPU tiers)
@@ -91,6 +94,32 @@ def var_f64(a, ddof: int) -> float:
 
 def var_f32(a, ddof: int) -> float:
     return _scalar_reduce("nrk_var_f32", a, np.float32, C.c_float, C.c_size_t(ddof))
+
+
+# ---- ufuncs.rs:946-979, 1060 ----
+def dot(a, b) -> float:
+    """ufuncs.rs:946 dot: the sum of the rounded products; lengths must match (ShapeMismatch)."""
+    a, b = np.asarray(a), np.asarray(b)
+    if a.size != b.size:
+        raise ValueError(f"shape mismatch: {a.size} vs {b.size}")
+    f32 = a.dtype == np.float32 and b.dtype == np.float32
+    dt, ct = (np.float32, C.c_float) if f32 else (np.float64, C.c_double)
+    av, bv = _vec(a, dt), _vec(b, dt)
+    out = ct()
+    call("nrk_dot_f32" if f32 else "nrk_dot_f64", _ptr(av), _ptr(bv), C.c_size_t(av.size), C.byref(out))
+    return out.value
+
+
+def _float_reduce(name, a, *extra):
+    a = np.asarray(a)
+    f32 = a.dtype == np.float32
+    return _scalar_reduce(f"nrk_{name}_{'f32' if f32 else 'f64'}", a, np.float32 if f32 else np.float64,
+                          C.c_float if f32 else C.c_double, *extra)
+
+
+def norm_l1(a) -> float: return _float_reduce("norm_l1", a)   # :975 sum |x|
+def norm_l2(a) -> float: return _float_reduce("norm_l2", a)   # :966 sqrt(sum x * x)
+def std(a, ddof: int = 0) -> float: return _float_reduce("std", a, C.c_size_t(ddof))  # :1060 var.sqrt()
 
 
 def sum_i64(a) -> int:
@@ -190,6 +219,40 @@ def hypot(a, b): return _bufunc(a, b, "hypot")                  # :841 a.hypot(b
 def arctan2(y, x): return _bufunc(y, x, "atan2")                # :587 y.atan2(x)
 def logaddexp(a, b): return _bufunc(a, b, "logaddexp")          # :916
 atan2 = arctan2
+
+
+def prelu(a, alpha): return _bufunc(a, alpha, "prelu")          # :1394 x > 0 ? x : alpha * x
+
+
+# activations, special functions and steps: the reference's f64 expressions, one rounding per
+# operation (ufuncs.rs:520-1480)
+def relu(a): return _ufunc(a, ("relu",))                     # :1380 x.max(0.0)
+def leaky_relu(a, alpha: float = 0.01): return _ufunc(a, ("leaky_relu", alpha))  # :1390
+def elu(a, alpha: float = 1.0): return _ufunc(a, ("elu", alpha))  # :1322
+def selu(a): return _ufunc(a, ("selu",))                     # :1334
+def hardsigmoid(a): return _ufunc(a, ("hardsigmoid",))       # :1344
+def hardswish(a): return _ufunc(a, ("hardswish",))           # :1362
+def softplus(a): return _ufunc(a, ("softplus",))             # :1291
+def mish(a): return _ufunc(a, ("mish",))                     # :1309
+def swish(a): return _ufunc(a, ("silu",))                    # :1275 x / (1 + (-x).exp())
+silu = swish                                                 # :1279
+def gelu(a): return _ufunc(a, ("gelu",))                     # :1262 (tanh form)
+def logit(a): return _ufunc(a, ("logit",))                   # :1251 (x / (1 - x)).ln()
+def sinc(a): return _ufunc(a, ("sinc",))                     # :527 sin(pi x) / (pi x)
+def erf(a): return _ufunc(a, ("erf",))                       # :1081 erf_scalar
+def erfc(a): return _ufunc(a, ("erfc",))                     # :1118 1 - erf_scalar(x)
+def digamma(a): return _ufunc(a, ("digamma",))               # :1198
+
+
+def _step(op, edge0: float, edge1: float, x):
+    """t = ((v - edge0) / (edge1 - edge0)).clamp(0.0, 1.0) (ufuncs.rs:1467, 1480): edge1 - edge0
+    is rounded once, as the reference computes it; clamp's NaN passes through."""
+    return _ufunc(x, ("sub", edge0), ("div", float(np.float64(edge1) - np.float64(edge0))),
+                  ("clamp_lo", 0.0), ("clamp_hi", 1.0), (op,))
+
+
+def smoothstep(edge0, edge1, x): return _step("smoothstep", edge0, edge1, x)       # :1460
+def smootherstep(edge0, edge1, x): return _step("smootherstep", edge0, edge1, x)   # :1473
 
 
 def deg2rad(a):

@@ -381,6 +381,7 @@ int64_t ref_sum_i64(const int64_t *a, size_t n) {
                 case 5: y[i] = p > q ? p : q; break;                                        \
                 case 6: y[i] = p < q ? p : q; break;                                        \
                 case 7: y[i] = COPYSIGN(p, q); break; /* ufuncs.rs:801 m.copysign(s) */      \
+                case 11: y[i] = p > 0 ? p : q * p; break; /* ufuncs.rs:1405 prelu */          \
                 default: y[i] = p;                                                          \
             }                                                                               \
         }                                                                                   \

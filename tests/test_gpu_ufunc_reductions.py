@@ -1,0 +1,79 @@
+"""dot, norm_l1, norm_l2 and std (ufuncs.rs:946-979, 1060) through the C ABI on the GPU.
+
+The reference sums the rounded products (or |x|, x * x) one after another in f64; the kernels
+sum the same rounded terms in a fixed tree. Parity is therefore a bound, not bits: against the
+exact sum of the same rounded terms (math.fsum), a tree of depth d adds at most d roundings to
+each term, |err| <= d * 2^-53 * sum |term| (2^-24 for binary32), with d = 64 covering the
+in-thread, wave, workgroup and final trees for every n tested here.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from numrs_amd import kernels as K
+
+pytestmark = pytest.mark.gpu
+GAMMA = {np.float64: 64 * 2.0**-53, np.float32: 64 * 2.0**-24}
+SIZES = [1, 7, 255, 4096, 4097, 65_536 + 3, 1_000_003, 1 << 22]
+
+
+def _bound(terms, dt):
+    return GAMMA[dt] * math.fsum(np.abs(terms.astype(np.float64)))
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+@pytest.mark.parametrize("n", SIZES)
+def test_dot_within_tree_bound(gpu, dt, n):
+    r = np.random.default_rng(n)
+    a = r.normal(size=n).astype(dt)
+    b = r.normal(size=n).astype(dt)
+    prods = (a * b).astype(dt)  # the reference's rounded products
+    got = K.dot(a, b)
+    exact = math.fsum(prods.astype(np.float64))
+    assert abs(got - exact) <= _bound(prods, dt) + 1e-300
+    assert K.dot(b, a) == got  # commutative to the bit (test_proptest_array_properties.rs:477)
+    assert K.dot(a, b) == got  # the tree depends only on n
+
+
+def test_dot_known_values_and_edges(gpu):
+    assert K.dot(np.array([1.0, 2.0, 3.0]), np.array([4.0, 5.0, 6.0])) == 32.0
+    assert K.dot(np.array([], np.float64), np.array([], np.float64)) == 0.0
+    with pytest.raises(ValueError):
+        K.dot(np.ones(3), np.ones(4))
+    assert math.isnan(K.dot(np.array([1.0, np.nan]), np.array([1.0, 1.0])))
+    assert K.dot(np.array([np.inf, 1.0]), np.array([2.0, 1.0])) == np.inf
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+@pytest.mark.parametrize("n", SIZES)
+def test_norms_within_tree_bound(gpu, dt, n):
+    r = np.random.default_rng(n + 1)
+    x = r.normal(size=n).astype(dt)
+    l1 = K.norm_l1(x)
+    assert abs(l1 - math.fsum(np.abs(x.astype(np.float64)))) <= _bound(x, dt)
+    sq = (x * x).astype(dt)
+    s = math.fsum(sq.astype(np.float64))
+    l2 = K.norm_l2(x)
+    # sqrt halves the relative error of the sum and adds one rounding
+    eps = 2.0**-53 if dt == np.float64 else 2.0**-24
+    assert abs(l2 - math.sqrt(s)) <= (0.5 * _bound(sq, dt) / s + eps) * math.sqrt(s)
+
+
+def test_norm_and_std_known_values(gpu):
+    assert K.norm_l2(np.array([3.0, 4.0])) == 5.0
+    assert K.norm_l1(np.array([-1.0, 2.0, -3.0])) == 6.0
+    assert K.norm_l2(np.array([], np.float64)) == 0.0
+    assert K.std(np.array([2.0, 4.0, 4.0, 4.0, 5.0, 5.0, 7.0, 9.0])) == 2.0  # the textbook population std
+    assert K.std(np.array([1.0, 2.0, 3.0, 4.0]), ddof=1) == pytest.approx(math.sqrt(5.0 / 3.0), rel=1e-15)
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_std_is_sqrt_of_var(gpu, dt):
+    x = np.random.default_rng(9).normal(3.0, 2.0, 1_000_003).astype(dt)
+    for ddof in (0, 1):
+        v = K.var_f64(x, ddof) if dt == np.float64 else K.var_f32(x, ddof)
+        got = K.std(x, ddof)
+        want = float(np.sqrt(dt(v)))  # var's bits, one correctly rounded sqrt
+        assert got == want
+        assert got == pytest.approx(float(np.std(x.astype(np.float64), ddof=ddof)), rel=1e-12 if dt == np.float64 else 1e-5)

@@ -290,3 +290,168 @@ def test_binary_f32_within_one_ulp(gpu):
         got = fn(a, b)
         assert got.dtype == np.float32
         assert ulp_err(got, ref(a.astype(LD), b.astype(LD)), np.float32).max() <= 1.0
+
+
+# ---- activations, special functions and steps (ufuncs.rs:520-1480)
+#
+# Parity: each is the reference's f64 expression, one IEEE rounding per operation, over exp / ln /
+# tanh / sin. Those differ from the correctly rounded values by at most K ulps (K = 1 for exp, ln
+# and sin, 2 for tanh, from their bounds above), and every expression is monotone in each of their
+# values, so the result must lie between the expression's values with each function's correctly
+# rounded output moved -K and +K ulps (all corners when several appear).
+
+_K = {"exp": 1, "log": 1, "tanh": 2, "sin": 1}
+_CR = {"exp": np.exp, "log": np.log, "tanh": np.tanh, "sin": np.sin}
+_PI = 3.141592653589793
+_GELU_C = np.sqrt(np.float64(2.0) / np.float64(_PI))
+
+
+def _erf_expr(x, F):
+    t = 1.0 / (1.0 + 0.5 * np.abs(x))
+    p = np.float64(0.17087277)
+    for c in (-0.82215223, 1.48851587, -1.13520398, 0.27886807, -0.18628806, 0.09678418, 0.37409196,
+              1.00002368):
+        p = c + t * p
+    tau = t * F["exp"](-x * x - 1.26551223 + t * p)
+    return np.where(x >= 0.0, 1.0 - tau, tau - 1.0)
+
+
+def _digamma_expr(x, F):
+    out = np.full_like(x, np.nan)
+    for i, v in enumerate(x):
+        if not v <= 0.0:
+            xs = [v]
+            while xs[-1] < 6.0:
+                xs.append(xs[-1] + 1.0)
+            ix = 1.0 / xs[-1]
+            ix2 = ix * ix
+            r = F["log"](np.array([xs[-1]]))[0] - 0.5 * ix - ix2 * (1.0 / 12.0 - ix2 * (1.0 / 120.0 - ix2 / 252.0))
+            for xi in reversed(xs[:-1]):
+                r = r - 1.0 / xi
+            out[i] = r
+    return out
+
+
+_SELU_L, _SELU_A = 1.0507009873554805, 1.6732632423543772
+ACT = {  # name: (kernel call, reference expression, functions it calls)
+    "relu": (K.relu, lambda x, F: np.where(x > 0, x, 0.0), ()),
+    "leaky_relu": (lambda x: K.leaky_relu(x, 0.2), lambda x, F: np.where(x > 0, x, 0.2 * x), ()),
+    "elu": (lambda x: K.elu(x, 1.5), lambda x, F: np.where(x > 0, x, 1.5 * (F["exp"](x) - 1.0)), ("exp",)),
+    "selu": (K.selu, lambda x, F: _SELU_L * np.where(x > 0, x, _SELU_A * (F["exp"](x) - 1.0)), ("exp",)),
+    "hardsigmoid": (K.hardsigmoid, lambda x, F: np.where(x <= -3, 0.0, np.where(x >= 3, 1.0, x / 6.0 + 0.5)), ()),
+    "hardswish": (K.hardswish, lambda x, F: np.where(x <= -3, 0.0, np.where(x >= 3, x, x * (x + 3.0) / 6.0)), ()),
+    "softplus": (K.softplus, lambda x, F: np.where(x > 20, x, np.where(x < -20, F["exp"](x), F["log"](1.0 + F["exp"](x)))), ("exp", "log")),
+    "mish": (K.mish, lambda x, F: x * F["tanh"](np.where(x > 20, x, F["log"](1.0 + F["exp"](x)))), ("exp", "log", "tanh")),
+    "silu": (K.silu, lambda x, F: x / (1.0 + F["exp"](-x)), ("exp",)),
+    "gelu": (K.gelu, lambda x, F: 0.5 * x * (1.0 + F["tanh"](_GELU_C * (x + 0.044715 * (x * x * x)))), ("tanh",)),
+    "logit": (K.logit, lambda x, F: F["log"](x / (1.0 - x)), ("log",)),
+    "sinc": (K.sinc, lambda x, F: np.where(x == 0, 1.0, F["sin"](_PI * x) / (_PI * x)), ("sin",)),
+    "erf": (K.erf, _erf_expr, ("exp",)),
+    "erfc": (K.erfc, lambda x, F: 1.0 - _erf_expr(x, F), ("exp",)),
+    "digamma": (K.digamma, _digamma_expr, ("log",)),
+}
+
+
+def _fn(name, d):
+    def f(v):
+        with np.errstate(all="ignore"):
+            r = _CR[name](np.asarray(v, np.float64).astype(LD)).astype(np.float64)
+        for _ in range(abs(d)):
+            r = np.nextafter(r, np.inf if d > 0 else -np.inf)
+        return r
+    return f
+
+
+def _bracket(expr, fns, x):
+    """(centre, lo, hi) of the expression over every corner of the +-K ulp box."""
+    import itertools
+    with np.errstate(all="ignore"):
+        centre = expr(x, {n: _fn(n, 0) for n in _CR})
+        vals = [expr(x, {**{n: _fn(n, 0) for n in _CR}, **{n: _fn(n, s * _K[n]) for n, s in zip(fns, signs)}})
+                for signs in itertools.product((-1, 1), repeat=len(fns))] or [centre]
+    v = np.stack(vals)
+    return centre, v.min(axis=0), v.max(axis=0)
+
+
+def _act_inputs(name):
+    r = np.random.default_rng(hash(name) % 1000)
+    sp = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 20.0, -20.0, 3.0, -3.0, 1e-310, -1e-310, 709.8,
+                   -709.8, 745.5, -745.5, 0.5, 1.0, -1.0, 6.0, 1e-12, 1e300, -1e300])
+    if name == "digamma":
+        return np.concatenate([sp, r.uniform(0, 12, 3000), np.exp(r.uniform(-40, 40, 1000)), r.uniform(-5, 0, 200)])
+    if name == "logit":
+        return np.concatenate([sp, r.uniform(0, 1, 20000), r.uniform(-2, 3, 2000)])
+    return np.concatenate([sp, r.uniform(-8, 8, 20000), r.uniform(-40, 40, 10000), _wide(5000, 95)])
+
+
+@pytest.mark.parametrize("name", list(ACT))
+def test_activation_matches_reference_expression(gpu, name):
+    call, expr, fns = ACT[name]
+    x = _act_inputs(name)
+    got = call(x)
+    centre, lo, hi = _bracket(expr, fns, x)
+    ok = (np.isnan(centre) & np.isnan(got)) | ((got >= lo) & (got <= hi))
+    assert np.all(ok), (name, x[~ok][:5], got[~ok][:5], centre[~ok][:5])
+    if not fns:  # no elementary function: the expression itself, to the bit (zeros by value)
+        assert np.array_equal(got, centre, equal_nan=True)
+    exact_frac = np.mean((got == centre) | (np.isnan(got) & np.isnan(centre)))
+    print(f"{name}: {exact_frac:.4f} equal to the correctly rounded evaluation")
+    # binary32 arrays: the same f64 evaluation, rounded once
+    x32 = x.astype(np.float32)
+    c32, lo32, hi32 = _bracket(expr, fns, x32.astype(np.float64))
+    g32 = call(x32)
+    assert g32.dtype == np.float32
+    ok32 = (np.isnan(c32) & np.isnan(g32)) | ((g32 >= lo32.astype(np.float32)) & (g32 <= hi32.astype(np.float32)))
+    assert np.all(ok32), (name, x32[~ok32][:5], g32[~ok32][:5], c32[~ok32][:5])
+
+
+def test_activations_against_their_mathematics(gpu):
+    """Pins the expressions (kernel and test alike) to what they approximate: a dropped term, a
+    wrong constant or sign fails here."""
+    import math
+    import scipy.special as sps
+    r = np.random.default_rng(96)
+    x = r.uniform(-15, 15, 20000)
+    xl = x.astype(LD)
+    np.testing.assert_array_equal(K.relu(x), np.maximum(x, 0.0))
+    np.testing.assert_array_equal(K.hardsigmoid(x), np.clip(x / 6.0 + 0.5, 0.0, 1.0))
+    np.testing.assert_allclose(K.hardswish(x), x * np.clip(x + 3.0, 0.0, 6.0) / 6.0, rtol=2**-51, atol=0)
+    np.testing.assert_allclose(K.softplus(x), np.logaddexp(0.0, x), rtol=1e-7, atol=0)
+    np.testing.assert_allclose(K.silu(x), x * sps.expit(x), rtol=1e-14, atol=1e-300)
+    np.testing.assert_allclose(K.elu(x, 1.5), np.where(x > 0, x, 1.5 * np.expm1(np.minimum(x, 0))), rtol=0, atol=4e-16)
+    np.testing.assert_allclose(K.selu(x), _SELU_L * np.where(x > 0, x, _SELU_A * np.expm1(np.minimum(x, 0))),
+                               rtol=1e-15, atol=1e-15)
+    mish = (xl * np.tanh(np.log1p(np.exp(xl)))).astype(np.float64)
+    # 1 + e^x rounds away e^x's low bits: ln(1 + e^x) has an absolute error near 2^-53
+    assert np.all(np.abs(K.mish(x) - mish) <= 1e-13 * np.abs(mish) + 4e-16 * np.abs(x))
+    gelu = (0.5 * xl * (1 + np.tanh(np.sqrt(LD(2) / np.pi) * (xl + LD(0.044715) * xl**3)))).astype(np.float64)
+    assert np.all(np.abs(K.gelu(x) - gelu) <= 1e-13 * np.abs(gelu) + 1e-15 * np.abs(x))
+    s = x / 4.0
+    np.testing.assert_allclose(K.sinc(s), np.sinc(s), rtol=0, atol=1e-15)
+    p = r.uniform(1e-6, 1 - 1e-6, 20000)
+    np.testing.assert_allclose(K.logit(p), np.log(p / (1 - p)), rtol=1e-14, atol=1e-15)
+    e = np.array([math.erf(v) for v in x])
+    assert np.max(np.abs(K.erf(x) - e)) <= 1.2e-7  # the fit's stated error
+    ec = np.array([math.erfc(v) for v in x])
+    assert np.all(np.abs(K.erfc(x) - ec) <= 1.2e-7 * ec + 2e-16)
+    d = np.concatenate([r.uniform(1e-3, 30, 20000), [1e-10, 0.5, 1.0, 6.0]])
+    # the series is cut after x^-6: the first dropped term is 1/(240 x^8), 2.5e-9 at x = 6
+    assert np.all(np.abs(K.digamma(d) - sps.digamma(d)) <= 3e-9 + 1e-15 * np.abs(sps.digamma(d)))
+    assert np.isnan(K.digamma(np.array([0.0, -1.0, -2.5]))).all()
+
+
+def test_prelu_and_steps_exact(gpu):
+    r = np.random.default_rng(97)
+    a, al = r.normal(size=30001), r.uniform(0, 1, 30001)
+    a[::97] = np.nan
+    np.testing.assert_array_equal(K.prelu(a, al), np.where(a > 0, a, al * a))
+    with pytest.raises(ValueError):
+        K.prelu(a, al[:-1])
+    t = np.array([0.0, 0.25, 0.5, 1.0, -1.0, 2.0, np.nan])
+    np.testing.assert_array_equal(K.smoothstep(0.0, 1.0, t), [0.0, 0.15625, 0.5, 1.0, 0.0, 1.0, np.nan])
+    np.testing.assert_array_equal(K.smootherstep(0.0, 1.0, t), [0.0, 0.103515625, 0.5, 1.0, 0.0, 1.0, np.nan])
+    x = r.uniform(-3, 5, 50000)
+    e0, e1 = -1.25, 3.5
+    tt = np.clip((x - e0) / (e1 - e0), 0.0, 1.0)
+    np.testing.assert_array_equal(K.smoothstep(e0, e1, x), tt * tt * (3.0 - 2.0 * tt))
+    np.testing.assert_array_equal(K.smootherstep(e0, e1, x), tt * tt * tt * (tt * (tt * 6.0 - 15.0) + 10.0))

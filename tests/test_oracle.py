@@ -171,6 +171,13 @@ def test_copysign_matches_ieee(oracle):
                                       want.view(np.uint64 if t == np.float64 else np.uint32))
 
 
+def test_prelu_matches_numpy(oracle):
+    """prelu (ufuncs.rs:1405): x where x > 0, else alpha * x; NaN x gives NaN."""
+    x = np.array([1.5, -2.0, 0.0, -0.0, np.nan, -np.inf, np.inf, -3.0])
+    al = np.array([0.1, 0.25, 0.5, 0.5, 0.5, 0.5, 0.5, -1.0])
+    np.testing.assert_array_equal(oracle.ref_zip(11, x, al), [1.5, -0.5, 0.0, -0.0, np.nan, -np.inf, np.inf, 3.0])
+
+
 def test_two_roundings_not_fma(oracle):
     """y = x*1.5 + (-0.25) is two roundings: bit-equal to numpy's separate ops, and for some x
     different from the fused (single-rounding) value."""
