@@ -158,7 +158,10 @@ enum {
     NRK_SOP_ERF = 51,         /* :1081 erf_scalar (Chebyshev fit of erfc, |error| < 1.2e-7)   */
     NRK_SOP_ERFC = 52,        /* :1118 1 - erf_scalar(x)                                      */
     NRK_SOP_DIGAMMA = 53,     /* :1202 recurrence to x >= 6, then the asymptotic series       */
-    NRK_SOP_COUNT = 54
+    NRK_SOP_POW = 54,         /* :305 power_scalar, x.powf(s), within 1 ulp, C's special cases */
+    NRK_SOP_GAMMA = 55,       /* :1122 Lanczos g = 7 (reflection below 0.5), with its powf     */
+    NRK_SOP_LGAMMA = 56,      /* :1183 ln |gamma_single(x)| (no reflection, as the reference)  */
+    NRK_SOP_COUNT = 57
 };
 typedef struct nrk_scalar_op {
     int32_t op;     /* NRK_SOP_* */
@@ -181,7 +184,8 @@ enum {
     NRK_BOP_ATAN2 = 9,     /* ufuncs.rs:587  a.atan2(b) (a = y, b = x), within 1.5 ulp  */
     NRK_BOP_LOGADDEXP = 10, /* ufuncs.rs:916 m + ln(e^-|a-m| + e^-|b-m|), within 2 ulp */
     NRK_BOP_PRELU = 11,    /* ufuncs.rs:1394 a > 0 ? a : b * a (b = alpha)             */
-    NRK_BOP_COUNT = 12
+    NRK_BOP_POWER = 12,    /* ufuncs.rs:166 a.powf(b), within 1 ulp                      */
+    NRK_BOP_COUNT = 13
 };
 
 /* kernels/elementwise.rs:237-258 unary_serial / unary_dispatch with an op-coded closure:
@@ -271,6 +275,29 @@ nrk_status nrk_reduce_axis_f64(int32_t op, const double *x, size_t outer, size_t
 nrk_status nrk_reduce_axis_f32(int32_t op, const float *x, size_t outer, size_t len, size_t inner, size_t ddof, void *out);
 nrk_status nrk_dev_reduce_axis_f64(int32_t op, const double *x, size_t outer, size_t len, size_t inner, size_t ddof, void *out, nrk_stream stream);
 nrk_status nrk_dev_reduce_axis_f32(int32_t op, const float *x, size_t outer, size_t len, size_t inner, size_t ddof, void *out, nrk_stream stream);
+
+/* ufuncs.rs:1411-1457 softmax and log_softmax over all n elements (the reference flattens):
+ * m = max, e = exp(x - m), S = sum e (fixed tree), then y = e / S or y = (x - m) - ln S. A NaN
+ * anywhere makes every output NaN, as in the reference. y may alias x only for n = 0. */
+nrk_status nrk_softmax_f64(const double *x, double *y, size_t n);
+nrk_status nrk_softmax_f32(const float *x, float *y, size_t n);
+nrk_status nrk_log_softmax_f64(const double *x, double *y, size_t n);
+nrk_status nrk_log_softmax_f32(const float *x, float *y, size_t n);
+nrk_status nrk_dev_softmax_f64(const double *x, double *y, size_t n, nrk_stream stream);
+nrk_status nrk_dev_softmax_f32(const float *x, float *y, size_t n, nrk_stream stream);
+nrk_status nrk_dev_log_softmax_f64(const double *x, double *y, size_t n, nrk_stream stream);
+nrk_status nrk_dev_log_softmax_f32(const float *x, float *y, size_t n, nrk_stream stream);
+
+/* ufuncs.rs:864 fma: y = a.mul_add(b, c), one rounding; ufuncs.rs:893 lerp: y = a + t * (b - a),
+ * three roundings. Equal lengths n; the reference's ShapeMismatch is the caller's check. */
+nrk_status nrk_fma_f64(const double *a, const double *b, const double *c, double *y, size_t n);
+nrk_status nrk_fma_f32(const float *a, const float *b, const float *c, float *y, size_t n);
+nrk_status nrk_dev_fma_f64(const double *a, const double *b, const double *c, double *y, size_t n, nrk_stream stream);
+nrk_status nrk_dev_fma_f32(const float *a, const float *b, const float *c, float *y, size_t n, nrk_stream stream);
+nrk_status nrk_lerp_f64(const double *a, const double *b, double t, double *y, size_t n);
+nrk_status nrk_lerp_f32(const float *a, const float *b, float t, float *y, size_t n);
+nrk_status nrk_dev_lerp_f64(const double *a, const double *b, double t, double *y, size_t n, nrk_stream stream);
+nrk_status nrk_dev_lerp_f32(const float *a, const float *b, float t, float *y, size_t n, nrk_stream stream);
 
 /* ufuncs.rs:946-979, 1060: dot (sum of the rounded products a[i] * b[i]; 0 for n = 0),
  * norm_l1 (sum |x|), norm_l2 (sqrt of sum x * x) and std (sqrt of var with divisor n - ddof;

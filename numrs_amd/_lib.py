@@ -70,6 +70,14 @@ _SIGS: dict[str, list] = {
     "nrk_sum_sq_dev_f64": [vp, c_size, C.c_double, vp], "nrk_sum_sq_dev_f32": [vp, c_size, C.c_float, vp],
     "nrk_var_f64": [vp, c_size, c_size, vp], "nrk_var_f32": [vp, c_size, c_size, vp],
     "nrk_dot_f64": [vp, vp, c_size, vp], "nrk_dot_f32": [vp, vp, c_size, vp],
+    "nrk_fma_f64": [vp, vp, vp, vp, c_size], "nrk_fma_f32": [vp, vp, vp, vp, c_size],
+    "nrk_softmax_f64": [vp, vp, c_size], "nrk_softmax_f32": [vp, vp, c_size],
+    "nrk_log_softmax_f64": [vp, vp, c_size], "nrk_log_softmax_f32": [vp, vp, c_size],
+    "nrk_dev_softmax_f64": [vp, vp, c_size, vp], "nrk_dev_softmax_f32": [vp, vp, c_size, vp],
+    "nrk_dev_log_softmax_f64": [vp, vp, c_size, vp], "nrk_dev_log_softmax_f32": [vp, vp, c_size, vp],
+    "nrk_dev_fma_f64": [vp, vp, vp, vp, c_size, vp], "nrk_dev_fma_f32": [vp, vp, vp, vp, c_size, vp],
+    "nrk_lerp_f64": [vp, vp, C.c_double, vp, c_size], "nrk_lerp_f32": [vp, vp, C.c_float, vp, c_size],
+    "nrk_dev_lerp_f64": [vp, vp, C.c_double, vp, c_size, vp], "nrk_dev_lerp_f32": [vp, vp, C.c_float, vp, c_size, vp],
     "nrk_dev_dot_f64": [vp, vp, c_size, vp, vp], "nrk_dev_dot_f32": [vp, vp, c_size, vp, vp],
     "nrk_norm_l1_f64": [vp, c_size, vp], "nrk_norm_l1_f32": [vp, c_size, vp],
     "nrk_norm_l2_f64": [vp, c_size, vp], "nrk_norm_l2_f32": [vp, c_size, vp],

@@ -104,6 +104,23 @@ nrk_status dev_std(const T *a, size_t n, size_t ddof, T *out, hipStream_t s) {
     NRK_TRY(dev_var<T>(a, n, ddof, out, s));
     return sqrt_scalar<T>(out, s);
 }
+// ufuncs.rs:1411-1457 softmax / log_softmax over the whole array: max, e = exp(x - max),
+// sum of e, then e / sum or (x - max) - ln(sum). The max and sum stay on the device (two
+// scalars in the stream's workspace), so nothing waits on the host.
+template <typename T>
+nrk_status dev_softmax(bool log_form, const T *x, T *y, size_t n, hipStream_t s) {
+    NRK_TRY(ensure_device());
+    if (n == 0) return NRK_OK;
+    if (!x || !y) return fail(NRK_ERR_INVALID_ARGUMENT, "x or y is NULL");
+    void *ws = nullptr;
+    NRK_TRY(get_workspace(s, 2 * sizeof(T), &ws));
+    T *ms = (T *)ws;
+    NRK_TRY(dev_max<T>(x, n, ms, s));
+    NRK_TRY(softmax_pass<T>(0, x, y, ms, n, s));
+    NRK_TRY(dev_sum<T>(y, n, ms + 1, s));
+    return softmax_pass<T>(log_form ? 2 : 1, x, y, ms, n, s);
+}
+
 template <typename T>
 nrk_status host_dot(const T *a, const T *b, size_t n, T *out) {
     NRK_TRY(ensure_device());
@@ -173,6 +190,21 @@ nrk_status host_zip(int32_t op, const T *a, const T *b, T *y, size_t n) {
     return host_call<T, T>(a, n, y, n, [&](const T *d, T *dout, hipStream_t s) { return zip<T>(op, d, (const T *)db, dout, n, s); });
 }
 
+// fma (op 0) and lerp (op 1) on host slices: a, b (and c) staged, y copied back.
+template <typename T>
+nrk_status host_tri(int op, const T *a, const T *b, const T *c, T t, T *y, size_t n) {
+    NRK_TRY(ensure_device());
+    if (n && (!a || !b || !y || (op == 0 && !c))) return fail(NRK_ERR_INVALID_ARGUMENT, "an input or y is NULL");
+    void *db = nullptr, *dc = nullptr;
+    NRK_TRY(staging(2, n * sizeof(T) + 16, &db));
+    NRK_TRY(staging(3, n * sizeof(T) + 16, &dc));
+    if (n) NRK_HIP(hipMemcpyAsync(db, b, n * sizeof(T), hipMemcpyHostToDevice, nullptr));
+    if (n && op == 0) NRK_HIP(hipMemcpyAsync(dc, c, n * sizeof(T), hipMemcpyHostToDevice, nullptr));
+    return host_call<T, T>(a, n, y, n, [&](const T *d, T *dout, hipStream_t s) {
+        return tri<T>(op, d, (const T *)db, (const T *)dc, t, dout, n, s);
+    });
+}
+
 template <typename T>
 nrk_status dev_map_reduce(const nrk_scalar_op *chain, int32_t chain_len, const T *x, size_t rows, size_t cols, size_t ddof,
                           const nrk_reduce_outputs *row_out, const nrk_reduce_outputs *full_out, hipStream_t s) {
@@ -203,6 +235,34 @@ nrk_status nrk_zip_f64(int32_t op, const double *a, const double *b, double *y, 
 nrk_status nrk_zip_f32(int32_t op, const float *a, const float *b, float *y, size_t n) { return host_zip<float>(op, a, b, y, n); }
 nrk_status nrk_dev_zip_f64(int32_t op, const double *a, const double *b, double *y, size_t n, nrk_stream stream) { return zip<double>(op, a, b, y, n, S(stream)); }
 nrk_status nrk_dev_zip_f32(int32_t op, const float *a, const float *b, float *y, size_t n, nrk_stream stream) { return zip<float>(op, a, b, y, n, S(stream)); }
+
+// ---- softmax, log_softmax ----
+nrk_status nrk_softmax_f64(const double *x, double *y, size_t n) {
+    return host_call<double, double>(x, n, y, n, [&](const double *d, double *o, hipStream_t s) { return dev_softmax<double>(false, d, o, n, s); });
+}
+nrk_status nrk_softmax_f32(const float *x, float *y, size_t n) {
+    return host_call<float, float>(x, n, y, n, [&](const float *d, float *o, hipStream_t s) { return dev_softmax<float>(false, d, o, n, s); });
+}
+nrk_status nrk_log_softmax_f64(const double *x, double *y, size_t n) {
+    return host_call<double, double>(x, n, y, n, [&](const double *d, double *o, hipStream_t s) { return dev_softmax<double>(true, d, o, n, s); });
+}
+nrk_status nrk_log_softmax_f32(const float *x, float *y, size_t n) {
+    return host_call<float, float>(x, n, y, n, [&](const float *d, float *o, hipStream_t s) { return dev_softmax<float>(true, d, o, n, s); });
+}
+nrk_status nrk_dev_softmax_f64(const double *x, double *y, size_t n, nrk_stream stream) { return dev_softmax<double>(false, x, y, n, S(stream)); }
+nrk_status nrk_dev_softmax_f32(const float *x, float *y, size_t n, nrk_stream stream) { return dev_softmax<float>(false, x, y, n, S(stream)); }
+nrk_status nrk_dev_log_softmax_f64(const double *x, double *y, size_t n, nrk_stream stream) { return dev_softmax<double>(true, x, y, n, S(stream)); }
+nrk_status nrk_dev_log_softmax_f32(const float *x, float *y, size_t n, nrk_stream stream) { return dev_softmax<float>(true, x, y, n, S(stream)); }
+
+// ---- fma, lerp ----
+nrk_status nrk_fma_f64(const double *a, const double *b, const double *c, double *y, size_t n) { return host_tri<double>(0, a, b, c, 0.0, y, n); }
+nrk_status nrk_fma_f32(const float *a, const float *b, const float *c, float *y, size_t n) { return host_tri<float>(0, a, b, c, 0.0f, y, n); }
+nrk_status nrk_dev_fma_f64(const double *a, const double *b, const double *c, double *y, size_t n, nrk_stream stream) { return tri<double>(0, a, b, c, 0.0, y, n, S(stream)); }
+nrk_status nrk_dev_fma_f32(const float *a, const float *b, const float *c, float *y, size_t n, nrk_stream stream) { return tri<float>(0, a, b, c, 0.0f, y, n, S(stream)); }
+nrk_status nrk_lerp_f64(const double *a, const double *b, double t, double *y, size_t n) { return host_tri<double>(1, a, b, nullptr, t, y, n); }
+nrk_status nrk_lerp_f32(const float *a, const float *b, float t, float *y, size_t n) { return host_tri<float>(1, a, b, nullptr, t, y, n); }
+nrk_status nrk_dev_lerp_f64(const double *a, const double *b, double t, double *y, size_t n, nrk_stream stream) { return tri<double>(1, a, b, nullptr, t, y, n, S(stream)); }
+nrk_status nrk_dev_lerp_f32(const float *a, const float *b, float t, float *y, size_t n, nrk_stream stream) { return tri<float>(1, a, b, nullptr, t, y, n, S(stream)); }
 
 // ---- flat reductions, host slices ----
 #define NRK_HOST_REDUCE(NAME, T, OUT_T, CALL)                                                  \

@@ -217,6 +217,9 @@ __device__ __forceinline__ void apply_chain_n(const Chain &c, T (&v)[N]) {
                         NRK_CHAIN_CASE(NRK_SOP_ERF, via_f64(x, [](double d) { return fm::erf_ref(d); }))
                         NRK_CHAIN_CASE(NRK_SOP_ERFC, via_f64(x, [](double d) { return 1.0 - fm::erf_ref(d); }))
                         NRK_CHAIN_CASE(NRK_SOP_DIGAMMA, via_f64(x, [](double d) { return fm::digamma(d); }))
+                        NRK_CHAIN_CASE(NRK_SOP_POW, (T)fm::pow((double)x, (double)s))
+                        NRK_CHAIN_CASE(NRK_SOP_GAMMA, via_f64(x, [](double d) { return fm::gamma_ref(d); }))
+                        NRK_CHAIN_CASE(NRK_SOP_LGAMMA, via_f64(x, [](double d) { return fm::lgamma_ref(d); }))
                         default: break;
                     }
                 }

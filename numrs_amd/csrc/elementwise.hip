@@ -29,6 +29,7 @@ __device__ __forceinline__ T apply_bop(T a, T b) {
     else if constexpr (OP == NRK_BOP_HYPOT) return (T)fm::hypot((double)a, (double)b);
     else if constexpr (OP == NRK_BOP_ATAN2) return (T)fm::atan2((double)a, (double)b);
     else if constexpr (OP == NRK_BOP_PRELU) return a > (T)0 ? a : rn_mul(b, a);  // ufuncs.rs:1394
+    else if constexpr (OP == NRK_BOP_POWER) return (T)fm::pow((double)a, (double)b);  // ufuncs.rs:166
     else return (T)fm::logaddexp((double)a, (double)b);                // LOGADDEXP
 }
 
@@ -171,11 +172,99 @@ nrk_status zip(int op, const T *a, const T *b, T *y, size_t n, hipStream_t strea
         NRK_ZIP_CASE(NRK_BOP_ATAN2)
         NRK_ZIP_CASE(NRK_BOP_LOGADDEXP)
         NRK_ZIP_CASE(NRK_BOP_PRELU)
+        NRK_ZIP_CASE(NRK_BOP_POWER)
 #undef NRK_ZIP_CASE
     }
     NRK_HIP(hipGetLastError());
     return NRK_OK;
 }
+
+// ---- three-operand streams: fma (ufuncs.rs:864, a.mul_add(b, c), one rounding) and lerp
+// (ufuncs.rs:893, a + t * (b - a), three roundings). c is unused by lerp.
+enum { TRI_FMA = 0, TRI_LERP = 1 };
+
+template <typename T, int OP>
+__device__ __forceinline__ T apply_tri(T a, T b, T c, T t) {
+    if constexpr (OP == TRI_FMA) {
+        if constexpr (sizeof(T) == 4) return __builtin_fmaf(a, b, c);  // one binary32 rounding
+        else return __builtin_fma(a, b, c);
+    }
+    else return rn_add(a, rn_mul(t, rn_sub(b, a)));
+}
+
+template <typename T, int OP>
+__global__ __launch_bounds__(EB) void tri_kernel(const T *a, const T *b, const T *c, T t, T *y, size_t n, int vec_ok) {
+    typedef typename Vec16<T>::type V;
+    constexpr int VN = Vec16<T>::N;
+    const size_t tid = blockIdx.x * (size_t)EB + threadIdx.x;
+    const size_t nthreads = (size_t)gridDim.x * EB;
+    size_t done = 0;
+    if (vec_ok) {
+        const size_t nv = n / VN;
+        const V *av = reinterpret_cast<const V *>(a);
+        const V *bv = reinterpret_cast<const V *>(b);
+        const V *cv = reinterpret_cast<const V *>(c);
+        V *yv = reinterpret_cast<V *>(y);
+        for (size_t i = tid; i < nv; i += nthreads) {
+            V va = __builtin_nontemporal_load(av + i);
+            const V vb = __builtin_nontemporal_load(bv + i);
+            V vc = va;
+            if constexpr (OP == TRI_FMA) vc = __builtin_nontemporal_load(cv + i);
+#pragma unroll
+            for (int e = 0; e < VN; ++e) va[e] = apply_tri<T, OP>(va[e], vb[e], vc[e], t);
+            __builtin_nontemporal_store(va, yv + i);
+        }
+        done = nv * VN;
+    }
+    for (size_t i = done + tid; i < n; i += nthreads)
+        y[i] = apply_tri<T, OP>(a[i], b[i], OP == TRI_FMA ? c[i] : (T)0, t);
+}
+
+template <typename T>
+nrk_status tri(int op, const T *a, const T *b, const T *c, T t, T *y, size_t n, hipStream_t stream) {
+    NRK_TRY(ensure_device());
+    if (n == 0) return NRK_OK;
+    if (!a || !b || !y || (op == TRI_FMA && !c)) return fail(NRK_ERR_INVALID_ARGUMENT, "an input or y is NULL");
+    auto al = [](const void *p) { return reinterpret_cast<uintptr_t>(p) % 16 == 0; };
+    const int vec_ok = al(a) && al(b) && al(y) && (op != TRI_FMA || al(c));
+    const dim3 grid(stream_grid(n / Vec16<T>::N + 1));
+    if (op == TRI_FMA)
+        hipLaunchKernelGGL((tri_kernel<T, TRI_FMA>), grid, dim3(EB), 0, stream, a, b, c, t, y, n, vec_ok);
+    else
+        hipLaunchKernelGGL((tri_kernel<T, TRI_LERP>), grid, dim3(EB), 0, stream, a, b, c, t, y, n, vec_ok);
+    NRK_HIP(hipGetLastError());
+    return NRK_OK;
+}
+template nrk_status tri<float>(int, const float *, const float *, const float *, float, float *, size_t, hipStream_t);
+template nrk_status tri<double>(int, const double *, const double *, const double *, double, double *, size_t, hipStream_t);
+
+// ---- softmax / log_softmax passes (ufuncs.rs:1411-1457). ms = {max, sum} on the device.
+// mode 0: y = exp(x - max); mode 1: y = y / sum; mode 2: y = (x - max) - ln(sum).
+template <typename T, int MODE>
+__global__ __launch_bounds__(EB) void softmax_pass_kernel(const T *x, T *y, const T *ms, size_t n) {
+    const T m = ms[0];
+    const T s = ms[1];
+    const T lse = MODE == 2 ? (T)fm::log((double)s) : (T)0;
+    const size_t nthreads = (size_t)gridDim.x * EB;
+    for (size_t i = blockIdx.x * (size_t)EB + threadIdx.x; i < n; i += nthreads) {
+        if constexpr (MODE == 0) y[i] = (T)fm::exp((double)rn_sub(x[i], m));
+        else if constexpr (MODE == 1) y[i] = rn_div(y[i], s);
+        else y[i] = rn_sub(rn_sub(x[i], m), lse);
+    }
+}
+
+template <typename T>
+nrk_status softmax_pass(int mode, const T *x, T *y, const T *ms, size_t n, hipStream_t stream) {
+    if (n == 0) return NRK_OK;
+    const dim3 grid((unsigned)std::min<size_t>(ceil_div(n, (size_t)EB * 4), (size_t)cu_count() * 32));
+    if (mode == 0) hipLaunchKernelGGL((softmax_pass_kernel<T, 0>), grid, dim3(EB), 0, stream, x, y, ms, n);
+    else if (mode == 1) hipLaunchKernelGGL((softmax_pass_kernel<T, 1>), grid, dim3(EB), 0, stream, x, y, ms, n);
+    else hipLaunchKernelGGL((softmax_pass_kernel<T, 2>), grid, dim3(EB), 0, stream, x, y, ms, n);
+    NRK_HIP(hipGetLastError());
+    return NRK_OK;
+}
+template nrk_status softmax_pass<float>(int, const float *, float *, const float *, size_t, hipStream_t);
+template nrk_status softmax_pass<double>(int, const double *, double *, const double *, size_t, hipStream_t);
 
 template nrk_status map<float>(const Chain &, const float *, float *, size_t, hipStream_t);
 template nrk_status map<double>(const Chain &, const double *, double *, size_t, hipStream_t);

@@ -759,5 +759,218 @@ __device__ inline double digamma(double x) {
     return r;
 }
 
+
+// ---- pow and the Lanczos gamma (ufuncs.rs:226, 305, 1125-1195) ----
+
+// ln z for z = 2^k m in double-double, relative error near 2^-68 (what pow needs: |y ln x| up to
+// 745 must keep its error below 2^-58). m is reduced to [0.6875, 1.4375) and split as
+// m = c (1 + r) with c from 96 intervals of width 1/128: LOG_TAB[j] = {1/c, ln c hi, ln c lo},
+// 1/c rounded to a double (so it is exactly some c'), ln c' to 106 bits (60-digit decimal
+// arithmetic). The two intervals around 1 take c = 1, keeping ln z relative near z = 1.
+// r = m / c - 1 is formed exactly as a TwoProd, and log1p(r) = r - r^2/2 + r^3 P(r) with the
+// r^2 term in double-double and P to r^11 (|r| < 2^-7.4: truncation below 2^-90).
+__device__ __forceinline__ void log_tab(int j, double &invc, double &lh, double &ll) {
+    static constexpr double LOG_TAB[96][3] = {
+        {0x1.724287f46debcp+0, -0x1.79e26687cfb3dp-2, -0x1.fe977e8bbc0dep-56},
+        {0x1.6e1f76b4337c7p+0, -0x1.6e60ee6af1973p-2, 0x1.56a0f7749e5cdp-56},
+        {0x1.6a13cd1537290p+0, -0x1.630030b3aac48p-2, -0x1.ee0c6728fffccp-56},
+        {0x1.661ec6a5122f9p+0, -0x1.57bf753c8d1fbp-2, 0x1.2908d15f88b63p-57},
+        {0x1.623fa77016240p+0, -0x1.4c9e09e172c3dp-2, 0x1.123615b147a5fp-58},
+        {0x1.5e75bb8d015e7p+0, -0x1.419b423d5e8c6p-2, -0x1.5b7648704e721p-58},
+        {0x1.5ac056b015ac0p+0, -0x1.36b6776be1116p-2, 0x1.324f0e8838590p-58},
+        {0x1.571ed3c506b3ap+0, -0x1.2bef07cdc9355p-2, 0x1.22dad7fd86088p-56},
+        {0x1.5390948f40febp+0, -0x1.214456d0eb8d5p-2, 0x1.50a2dca28b3edp-58},
+        {0x1.5015015015015p+0, -0x1.16b5ccbacfb73p-2, -0x1.56fbd28b40935p-56},
+        {0x1.4cab88725af6ep+0, -0x1.0c42d676162e2p-2, 0x1.5a74e18a8bb85p-56},
+        {0x1.49539e3b2d067p+0, -0x1.01eae5626c691p-2, -0x1.d9f5bd0b5b348p-57},
+        {0x1.460cbc7f5cf9ap+0, -0x1.ef5ade4dcffe5p-3, -0x1.7754d2238f75fp-58},
+        {0x1.42d6625d51f87p+0, -0x1.db13db0d48941p-3, 0x1.8af715b0349a4p-57},
+        {0x1.3fb013fb013fbp+0, -0x1.c6ffbc6f00f71p-3, 0x1.ae58b2c57a4a5p-57},
+        {0x1.3c995a47babe7p+0, -0x1.b31d8575bce3bp-3, 0x1.0d4eace1aa537p-59},
+        {0x1.3991c2c187f63p+0, -0x1.9f6c407089663p-3, 0x1.52979a7e86605p-57},
+        {0x1.3698df3de0748p+0, -0x1.8beafeb38fe8fp-3, 0x1.54aae92cd0b87p-59},
+        {0x1.33ae45b57bcb2p+0, -0x1.7898d85444c74p-3, -0x1.be3dbaf3ec804p-60},
+        {0x1.30d190130d190p+0, -0x1.6574ebe8c1339p-3, -0x1.c5961e173bc82p-57},
+        {0x1.2e025c04b8097p+0, -0x1.527e5e4a1b58dp-3, 0x1.b8d4b411cadffp-60},
+        {0x1.2b404ad012b40p+0, -0x1.3fb45a59928cap-3, 0x1.d87e6a354d057p-57},
+        {0x1.288b01288b013p+0, -0x1.2d1610c86813dp-3, -0x1.d997036941a6dp-60},
+        {0x1.25e22708092f1p+0, -0x1.1aa2b7e23f729p-3, -0x1.6e44389934420p-57},
+        {0x1.23456789abcdfp+0, -0x1.08598b59e3a07p-3, 0x1.fd7009902bf32p-57},
+        {0x1.20b470c67c0d9p+0, -0x1.ec739830a1126p-4, -0x1.eea033743f95bp-58},
+        {0x1.1e2ef3b3fb874p+0, -0x1.c885801bc4b20p-4, 0x1.5c734aa6598fcp-58},
+        {0x1.1bb4a4046ed29p+0, -0x1.a4e7640b1bc38p-4, 0x1.9b5ca203e4259p-58},
+        {0x1.19453808ca29cp+0, -0x1.8197e2f40e3f0p-4, 0x1.230690020895fp-59},
+        {0x1.16e0689427379p+0, -0x1.5e95a4d9791cdp-4, 0x1.4c78ba3a3baf6p-58},
+        {0x1.1485f0e0acd3bp+0, -0x1.3bdf5a7d1ee5ep-4, -0x1.f52eda76b68acp-60},
+        {0x1.12358e75d3033p+0, -0x1.1973bd1465561p-4, 0x1.7aac1b3d35680p-58},
+        {0x1.0fef010fef011p+0, -0x1.eea31c006b87cp-5, 0x1.7c9f9276f6cd8p-60},
+        {0x1.0db20a88f4696p+0, -0x1.aaef2d0fb1108p-5, -0x1.68d4eed0b82aep-59},
+        {0x1.0b7e6ec259dc8p+0, -0x1.67c94f2d4bb65p-5, -0x1.0413e6505e5f9p-59},
+        {0x1.0953f39010954p+0, -0x1.252f32f8d1840p-5, -0x1.ae021b67a9ba8p-61},
+        {0x1.073260a47f7c6p+0, -0x1.c63d2ec14aad7p-6, -0x1.8fe7acbca131dp-63},
+        {0x1.05197f7d73404p+0, -0x1.432a925980cbcp-6, 0x1.8cdaf39004193p-60},
+        {0x1.03091b51f5e1ap+0, -0x1.82448a388a283p-7, -0x1.04b16137f0970p-62},
+        {0x1.0000000000000p+0, 0x0.0p+0, 0x0.0p+0},
+        {0x1.0000000000000p+0, 0x0.0p+0, 0x0.0p+0},
+        {0x1.fa11caa01fa12p-1, 0x1.7dc475f810a69p-7, 0x1.74944bc161072p-61},
+        {0x1.f6310aca0dbb5p-1, 0x1.3cea44346a584p-6, -0x1.865ad48159d00p-61},
+        {0x1.f25f644230ab5p-1, 0x1.b9fc027af919ap-6, -0x1.90ae69229dc86p-60},
+        {0x1.ee9c7f8458e02p-1, 0x1.1b0d98923d97fp-5, -0x1.74d7444dd6241p-59},
+        {0x1.eae807aba01ebp-1, 0x1.58a5bafc8e4d3p-5, -0x1.cab8569c56e40p-64},
+        {0x1.e741aa59750e4p-1, 0x1.95c830ec8e3f2p-5, 0x1.eb41d00a417e9p-60},
+        {0x1.e3a9179dc1a73p-1, 0x1.d276b8adb0b56p-5, 0x1.078f14c95ff53p-59},
+        {0x1.e01e01e01e01ep-1, 0x1.075983598e471p-4, 0x1.006d2999e22dcp-58},
+        {0x1.dca01dca01dcap-1, 0x1.253f62f0a1417p-4, 0x1.1f6d34e01d981p-61},
+        {0x1.d92f2231e7f8ap-1, 0x1.42edcbea646eep-4, -0x1.511583653349bp-58},
+        {0x1.d5cac807572b2p-1, 0x1.60658a93750c4p-4, -0x1.f108b1d8436d3p-59},
+        {0x1.d272ca3fc5b1ap-1, 0x1.7da766d7b12d0p-4, 0x1.a2240644d7da2p-59},
+        {0x1.cf26e5c44bfc6p-1, 0x1.9ab42462033aep-4, -0x1.a099e1c184e8ep-59},
+        {0x1.cbe6d9601cbe7p-1, 0x1.b78c82bb0eda0p-4, -0x1.3ef0e61f9b03cp-58},
+        {0x1.c8b265afb8a42p-1, 0x1.d4313d66cb35dp-4, 0x1.b90dd951d90fap-58},
+        {0x1.c5894d10d4986p-1, 0x1.f0a30c01162a4p-4, 0x1.8be64b8b7759bp-59},
+        {0x1.c26b5392ea01cp-1, 0x1.0671512ca596fp-3, -0x1.2f39b81479b67p-58},
+        {0x1.bf583ee868d8bp-1, 0x1.14785846742acp-3, 0x1.94409f1d3f83ap-60},
+        {0x1.bc4fd65883e7bp-1, 0x1.2266f190a5acdp-3, -0x1.dab840e7f6177p-57},
+        {0x1.b951e2b18ff23p-1, 0x1.303d718e47fd5p-3, -0x1.b5ae71f658247p-57},
+        {0x1.b65e2e3beee05p-1, 0x1.3dfc2b0ecc62ap-3, 0x1.ba62b8c13f7f4p-57},
+        {0x1.b37484ad806cep-1, 0x1.4ba36f39a55e5p-3, -0x1.f767e433c98aap-57},
+        {0x1.b094b31d922a4p-1, 0x1.59338d9982085p-3, 0x1.8d16eaaba9419p-57},
+        {0x1.adbe87f94905ep-1, 0x1.66acd4272ad51p-3, -0x1.9201c9c3d5165p-59},
+        {0x1.aaf1d2f87ebfdp-1, 0x1.740f8f54037a3p-3, 0x1.6d9bf9d57b326p-58},
+        {0x1.a82e65130e159p-1, 0x1.815c0a14357e9p-3, 0x1.141b7f8c5fa9ep-58},
+        {0x1.a574107688a4ap-1, 0x1.8e928de886d41p-3, 0x1.2589eb96a6240p-59},
+        {0x1.a2c2a87c51ca0p-1, 0x1.9bb362e7dfb85p-3, -0x1.51439c1ff83e7p-58},
+        {0x1.a01a01a01a01ap-1, 0x1.a8becfc882f19p-3, -0x1.a8c37918c39ebp-58},
+        {0x1.9d79f176b682dp-1, 0x1.b5b519e8fb5a6p-3, -0x1.d5d8023e61e5fp-57},
+        {0x1.9ae24ea5510dap-1, 0x1.c2968558c18c2p-3, 0x1.6108e3ae024acp-60},
+        {0x1.9852f0d8ec0ffp-1, 0x1.cf6354e09c5ddp-3, 0x1.339a07d55b696p-57},
+        {0x1.95cbb0be377aep-1, 0x1.dc1bca0abec7bp-3, 0x1.c698a33316dfbp-58},
+        {0x1.934c67f9b2ce6p-1, 0x1.e8c0252aa5a60p-3, -0x1.dc074737f9135p-60},
+        {0x1.90d4f120190d5p-1, 0x1.f550a564b7b37p-3, -0x1.13a09202fe73dp-57},
+        {0x1.8e6527af1373fp-1, 0x1.00e6c45ad501dp-2, -0x1.3b9568ff6feadp-57},
+        {0x1.8bfce8062ff3ap-1, 0x1.071b85fcd590dp-2, 0x1.08b83fcbdef40p-57},
+        {0x1.899c0f601899cp-1, 0x1.0d46b579ab74bp-2, 0x1.21f640e1e5ec9p-56},
+        {0x1.87427bcc092b9p-1, 0x1.136870293a8b0p-2, 0x1.86cc531dba494p-57},
+        {0x1.84f00c2780614p-1, 0x1.1980d2dd4236fp-2, -0x1.02c2e4f1b2eb9p-56},
+        {0x1.82a4a0182a4a0p-1, 0x1.1f8ff9e48a2f3p-2, -0x1.93fbf3418960dp-57},
+        {0x1.8060180601806p-1, 0x1.2596010df763ap-2, -0x1.9eed8ae0ebd3cp-59},
+        {0x1.7e225515a4f1dp-1, 0x1.2b9303ab89d25p-2, -0x1.85ad7f614ab51p-58},
+        {0x1.7beb3922e017cp-1, 0x1.31871c9544185p-2, -0x1.ea3598981366fp-57},
+        {0x1.79baa6bb6398bp-1, 0x1.3772662bfd85cp-2, 0x1.02a7589fba088p-57},
+        {0x1.77908119ac60dp-1, 0x1.3d54fa5c1f710p-2, 0x1.53668e578d9cdp-58},
+        {0x1.756cac201756dp-1, 0x1.432ef2a04e813p-2, -0x1.83262e2b59206p-57},
+        {0x1.734f0c541fe8dp-1, 0x1.49006804009d0p-2, -0x1.bff0d07c5df6dp-59},
+        {0x1.713786d9c7c09p-1, 0x1.4ec9732600269p-2, -0x1.1aa87d977dc5ep-56},
+        {0x1.6f26016f26017p-1, 0x1.548a2c3add263p-2, -0x1.58ce7bf1846eep-56},
+        {0x1.6d1a62681c861p-1, 0x1.5a42ab0f4cfe2p-2, -0x1.c6bcb7dee9a3dp-56},
+        {0x1.6b1490aa31a3dp-1, 0x1.5ff3070a793d4p-2, -0x1.063077d7e37b7p-56},
+        {0x1.691473a88d0c0p-1, 0x1.659b57303e1f2p-2, 0x1.db0af8efb83c7p-62},
+        {0x1.6719f3601671ap-1, 0x1.6b3bb2235943dp-2, 0x1.957a93326784dp-56},
+        {0x1.6524f853b4aa3p-1, 0x1.70d42e2789236p-2, 0x1.ee99bf7143954p-56},
+    };
+    invc = LOG_TAB[j][0];
+    lh = LOG_TAB[j][1];
+    ll = LOG_TAB[j][2];
+}
+
+// x > 0 finite (subnormals included).
+__device__ inline void log_dd(double x, double &hi, double &lo) {
+    int e;
+    double m = frexp(x, &e);  // m in [0.5, 1)
+    if (m < 0.6875) {
+        m *= 2.0;
+        --e;
+    }
+    int j = (int)((m - 0.6875) * 128.0);  // m - 0.6875 is exact (Sterbenz)
+    j = j < 0 ? 0 : (j > 95 ? 95 : j);
+    double invc, lh, ll;
+    log_tab(j, invc, lh, ll);
+    const double ph = m * invc, pl = fma(m, invc, -ph);
+    double rh, rl;
+    two_sum(ph - 1.0, pl, rh, rl);  // ph - 1 exact (ph in [0.98, 1.02])
+    const double sqh = rh * rh, sql = fma(rh, rh, -sqh);
+    double p = -0x1.745d1745d1746p-4;    // -1/11
+    p = fma(p, rh, 0x1.999999999999ap-4);  // 1/10
+    p = fma(p, rh, -0x1.c71c71c71c71cp-4);  // -1/9
+    p = fma(p, rh, 0x1.0000000000000p-3);   // 1/8
+    p = fma(p, rh, -0x1.2492492492492p-3);  // -1/7
+    p = fma(p, rh, 0x1.5555555555555p-3);   // 1/6
+    p = fma(p, rh, -0x1.999999999999ap-3);  // -1/5
+    p = fma(p, rh, 0x1.0000000000000p-2);   // 1/4
+    p = fma(p, rh, -0x1.5555555555555p-2);  // -1/3
+    p = -p * (sqh * rh);                    // r^3/3 - r^4/4 + ...
+    const double k = (double)e;
+    double s1, e1, s2, e2, s3, e3;
+    two_sum(k * LN2_HI, lh, s1, e1);       // k LN2_HI exact
+    two_sum(s1, rh, s2, e2);
+    two_sum(s2, -0.5 * sqh, s3, e3);
+    const double tail = e1 + e2 + e3 + ll + k * LN2_LO + rl - 0.5 * sql - rh * rl + p;
+    hi = s3 + tail;
+    lo = tail - (hi - s3);
+}
+
+// e^(xh + xl), |xl| <= 2^-50 |xh|: the reduction of exp with xl joined to the reduced tail.
+__device__ inline double exp_tail(double xh, double xl) {
+    if (xh > 710.0) return dinf();
+    if (xh < -746.0) return 0.0;
+    const double k = rint(xh * INV_LN2);
+    const double r1 = fma(-k, LN2_HI, xh);  // exact
+    const double r = fma(-k, LN2_LO, r1);
+    const double rl = fma(-k, LN2_LO, r1 - r) + xl;
+    const double rh2 = r + rl, rl2 = rl - (rh2 - r);
+    return scale2(exp_dd(rh2, rl2), (int)k);
+}
+
+// x^y with C's (and Rust's powf) special cases: e^(y ln|x|) with ln|x| in double-double and the
+// product y ln|x| carried as a TwoProd.
+__device__ inline double pow(double x, double y) {
+    if (y == 0.0 || x == 1.0) return 1.0;
+    if (x != x || y != y) return x + y;
+    const double ax = __builtin_fabs(x), ay = __builtin_fabs(y);
+    const bool yint = __builtin_trunc(y) == y;
+    const bool yodd = yint && ay < 0x1p53 && __builtin_fmod(ay, 2.0) == 1.0;
+    const bool neg = __builtin_signbit(x) && yodd;  // -0 included: pow(-0, 3) = -0
+    if (ay == dinf()) {
+        if (ax == 1.0) return 1.0;
+        return ((ax < 1.0) == (y < 0.0)) ? dinf() : 0.0;
+    }
+    if (ax == dinf()) {
+        const double v = y < 0.0 ? 0.0 : dinf();
+        return neg ? -v : v;
+    }
+    if (ax == 0.0) {
+        const double v = y < 0.0 ? dinf() : 0.0;
+        return neg ? __builtin_copysign(v, x) : v;
+    }
+    if (x < 0.0 && !yint) return __builtin_nan("");
+    double lh, ll;
+    log_dd(ax, lh, ll);
+    const double ph = y * lh;
+    const double pl = fma(y, lh, -ph) + y * ll;
+    const double v = exp_tail(ph, pl);
+    return neg ? -v : v;
+}
+
+// The reference's Lanczos approximation (g = 7, nine coefficients), in its operation order.
+__device__ inline double lanczos_ref(double x) {
+    const double c[9] = {0.99999999999980993, 676.5203681218851, -1259.1392167224028, 771.32342877765313,
+                         -176.61502916214059, 12.507343278686905, -0.13857109526572012, 9.9843695780195716e-6,
+                         1.5056327351493116e-7};
+    const double xm = x - 1.0;
+    double y = c[0];
+#pragma unroll
+    for (int i = 1; i <= 8; ++i) y += c[i] / (xm + (double)i);
+    const double t = xm + 7.0 + 0.5;
+    const double SQRT_2PI = 0x1.40d931ff62705p+1;  // (2.0 * PI).sqrt(): two roundings
+    return SQRT_2PI * pow(t, xm + 0.5) * exp(-t) * y;
+}
+__device__ inline double gamma_ref(double x) {
+    if (x <= 0.0 && __builtin_floor(x) == x) return dinf();
+    const double PI = 0x1.921fb54442d18p+1;
+    if (x < 0.5) return PI / (sin(PI * x) * lanczos_ref(1.0 - x));
+    return lanczos_ref(x);
+}
+__device__ inline double lgamma_ref(double x) { return log(__builtin_fabs(lanczos_ref(x))); }
+
 }  // namespace fm
 }  // namespace nrk

@@ -38,6 +38,12 @@ template <typename T>
 nrk_status map(const Chain &chain, const T *x, T *y, size_t n, hipStream_t stream);
 template <typename T>
 nrk_status zip(int op, const T *a, const T *b, T *y, size_t n, hipStream_t stream);
+// softmax passes: mode 0 y = exp(x - ms[0]); 1 y = y / ms[1]; 2 y = (x - ms[0]) - ln(ms[1])
+template <typename T>
+nrk_status softmax_pass(int mode, const T *x, T *y, const T *ms, size_t n, hipStream_t stream);
+// op 0: y = fma(a, b, c); op 1: y = a + t * (b - a) (c unused)
+template <typename T>
+nrk_status tri(int op, const T *a, const T *b, const T *c, T t, T *y, size_t n, hipStream_t stream);
 
 // Plain GEMM on MFMA: C(m,n) = A(m,k) * B(k,n), row-major, overwrite (gemm.hip).
 template <typename T>

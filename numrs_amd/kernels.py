@@ -29,9 +29,9 @@ SOP = {"add": 0, "sub": 1, "mul": 2, "div": 3, "rsub": 4, "rdiv": 5, "neg": 6, "
        "sin": 34, "cos": 35, "tan": 36, "relu": 37, "leaky_relu": 38, "elu": 39, "selu": 40,
        "hardsigmoid": 41, "hardswish": 42, "softplus": 43, "mish": 44, "silu": 45, "gelu": 46,
        "logit": 47, "sinc": 48, "smoothstep": 49, "smootherstep": 50, "erf": 51, "erfc": 52,
-       "digamma": 53}
+       "digamma": 53, "pow": 54, "gamma": 55, "lgamma": 56}
 BOP = {"add": 0, "sub": 1, "mul": 2, "div": 3, "rem": 4, "maximum": 5, "minimum": 6, "copysign": 7, "hypot": 8,
-       "atan2": 9, "logaddexp": 10, "prelu": 11}
+       "atan2": 9, "logaddexp": 10, "prelu": 11, "power": 12}
 AXIS = {"sum": 0, "mean": 1, "var": 2, "min": 3, "max": 4, "argmin": 5, "argmax": 6}
 
 PARALLEL_MIN_LEN = 10_000  # kernels/mod.rs:74 (kept for callers; the GPU has no CPU tiers)
@@ -221,7 +221,59 @@ def logaddexp(a, b): return _bufunc(a, b, "logaddexp")          # :916
 atan2 = arctan2
 
 
-def prelu(a, alpha): return _bufunc(a, alpha, "prelu")          # :1394 x > 0 ? x : alpha * x
+def fma(a, b, c):
+    """ufuncs.rs:864 a.mul_add(b, c): one rounding; shapes must match (ShapeMismatch)."""
+    a, b, c = np.asarray(a), np.asarray(b), np.asarray(c)
+    if not (a.shape == b.shape == c.shape):
+        raise ValueError(f"shape mismatch: {a.shape}, {b.shape}, {c.shape}")
+    f32 = a.dtype == b.dtype == c.dtype == np.float32
+    dt = np.float32 if f32 else np.float64
+    av, bv, cv = _vec(a, dt), _vec(b, dt), _vec(c, dt)
+    y = np.empty_like(av)
+    call("nrk_fma_f32" if f32 else "nrk_fma_f64", _ptr(av), _ptr(bv), _ptr(cv), _ptr(y), C.c_size_t(av.size))
+    return y.reshape(a.shape)
+
+
+def lerp(a, b, t: float):
+    """ufuncs.rs:893 a + t * (b - a): three roundings; shapes must match (ShapeMismatch)."""
+    a, b = np.asarray(a), np.asarray(b)
+    if a.shape != b.shape:
+        raise ValueError(f"shape mismatch: {a.shape} vs {b.shape}")
+    f32 = a.dtype == b.dtype == np.float32
+    dt = np.float32 if f32 else np.float64
+    av, bv = _vec(a, dt), _vec(b, dt)
+    y = np.empty_like(av)
+    call("nrk_lerp_f32" if f32 else "nrk_lerp_f64", _ptr(av), _ptr(bv), (C.c_float if f32 else C.c_double)(t), _ptr(y),
+         C.c_size_t(av.size))
+    return y.reshape(a.shape)
+
+
+def _softmax(fn, a):
+    a = np.asarray(a)
+    f32 = a.dtype == np.float32
+    x = _vec(a, np.float32 if f32 else np.float64)
+    y = np.empty_like(x)
+    call(f"nrk_{fn}_{'f32' if f32 else 'f64'}", _ptr(x), _ptr(y), C.c_size_t(x.size))
+    return y.reshape(a.shape)
+
+
+def softmax(a): return _softmax("softmax", a)                  # ufuncs.rs:1434 over all elements
+def log_softmax(a): return _softmax("log_softmax", a)          # ufuncs.rs:1411
+
+
+def prelu(a, alpha): return _bufunc(a, alpha, "prelu")
+
+
+def power(a, b):
+    """ufuncs.rs:226 power through the power ufunc (:166 a.powf(b)), numpy-style broadcasting."""
+    a, b = np.broadcast_arrays(np.asarray(a), np.asarray(b))
+    return _bufunc(np.ascontiguousarray(a), np.ascontiguousarray(b), "power")
+
+
+def power_scalar(a, b: float): return _ufunc(a, ("pow", b))  # :305 x.powf(b)
+def gamma(a): return _ufunc(a, ("gamma",))                    # :1122 Lanczos, reflection below 0.5
+def lgamma(a): return _ufunc(a, ("lgamma",))                  # :1183 gamma_single(x).abs().ln()
+gammaln = lgamma                                              # :1193          # :1394 x > 0 ? x : alpha * x
 
 
 # activations, special functions and steps: the reference's f64 expressions, one rounding per

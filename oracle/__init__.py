@@ -121,6 +121,24 @@ def ref_zip(op: int, a: np.ndarray, b: np.ndarray) -> np.ndarray:
     return y
 
 
+def ref_fma(a: np.ndarray, b: np.ndarray, c: np.ndarray) -> np.ndarray:
+    a, b, c = (np.ascontiguousarray(v) for v in (a, b, c))
+    y = np.empty_like(a)
+    f = getattr(lib(), f"ref_fma_{_suf(a)}")
+    f.argtypes = [C.c_void_p] * 4 + [C.c_size_t]
+    f(_p(a), _p(b), _p(c), _p(y), a.size)
+    return y
+
+
+def ref_lerp(a: np.ndarray, b: np.ndarray, t: float) -> np.ndarray:
+    a, b = np.ascontiguousarray(a), np.ascontiguousarray(b)
+    y = np.empty_like(a)
+    f = getattr(lib(), f"ref_lerp_{_suf(a)}")
+    f.argtypes = [C.c_void_p, C.c_void_p, C.c_double if a.dtype == np.float64 else C.c_float, C.c_void_p, C.c_size_t]
+    f(_p(a), _p(b), t, _p(y), a.size)
+    return y
+
+
 def sum_i64(a: np.ndarray) -> int:
     a = np.ascontiguousarray(a, np.int64)
     return int(lib().ref_sum_i64(_p(a), a.size))

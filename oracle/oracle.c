@@ -387,6 +387,18 @@ int64_t ref_sum_i64(const int64_t *a, size_t n) {
         }                                                                                   \
     }
 
+/* ufuncs.rs:864-891 fma: a.mul_add(b, c), one rounding (C's fma is the IEEE fusedMultiplyAdd);
+ * ufuncs.rs:893-912 lerp: a + t * (b - a), three roundings */
+#define DEFINE_TRI(T, SUF, FMA)                                                             \
+    void ref_fma_##SUF(const T *a, const T *b, const T *c, T *y, size_t n) {                \
+        for (size_t i = 0; i < n; ++i) y[i] = FMA(a[i], b[i], c[i]);                        \
+    }                                                                                       \
+    void ref_lerp_##SUF(const T *a, const T *b, T t, T *y, size_t n) {                      \
+        for (size_t i = 0; i < n; ++i) y[i] = a[i] + t * (b[i] - a[i]);                     \
+    }
+DEFINE_TRI(double, f64, fma)
+DEFINE_TRI(float, f32, fmaf)
+
 DEFINE_EW(double, f64, sqrt, fmod, fabs, copysign)
 DEFINE_EW(float, f32, sqrtf, fmodf, fabsf, copysignf)
 

@@ -31,9 +31,9 @@ def test_each_scalar_op_correctly_rounded(gpu, oracle, dtype, step):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-# hypot, atan2 and logaddexp go through elementary functions: their ulp bounds are tested in
+# hypot, atan2, logaddexp and power go through elementary functions: their ulp bounds are tested in
 # test_gpu_ufuncs.py; every other binary op is one IEEE operation and must match bit for bit
-@pytest.mark.parametrize("op", [o for o in K.BOP if o not in ("hypot", "atan2", "logaddexp")])
+@pytest.mark.parametrize("op", [o for o in K.BOP if o not in ("hypot", "atan2", "logaddexp", "power")])
 def test_zip_bit_exact(gpu, oracle, dtype, op):
     n = 300_007
     a = rng.normal(rng.BASE_SEED + 1000, n, dtype)
@@ -65,3 +65,28 @@ def test_misaligned_views(gpu, oracle):
     x = rng.normal(5, 10_001, np.float32)
     v = x[1:]
     np.testing.assert_array_equal(K.unary_dispatch(v, [("mul", 1.5), ("add", -0.25)]), oracle.ref_map([2, 0], [1.5, -0.25], v))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("n", [1, 3, 1001, 300_007, 1 << 22])
+def test_fma_and_lerp_bit_exact(gpu, oracle, dtype, n):
+    """fma (ufuncs.rs:864, one rounding) and lerp (:893, a + t * (b - a)) against the oracle."""
+    a = rng.normal(rng.BASE_SEED + 2000 + n % 97, n, dtype)
+    b = rng.normal(rng.BASE_SEED + 2001, n, dtype)
+    c = (-(a * b) + rng.normal(rng.BASE_SEED + 2002, n, dtype) * dtype(1e-3)).astype(dtype)
+    if n > 10:
+        a[::1009] = np.nan
+        b[5::1013] = np.inf
+    np.testing.assert_array_equal(K.fma(a, b, c), oracle.ref_fma(a, b, c))
+    for t in (0.0, 0.37, 1.0, -1.5):
+        np.testing.assert_array_equal(K.lerp(a, b, t), oracle.ref_lerp(a, b, t))
+    # an input off 16-byte alignment takes the scalar path and gives the same bits
+    if n > 2:
+        np.testing.assert_array_equal(K.fma(a[1:], b[1:], c[1:]), oracle.ref_fma(a[1:], b[1:], c[1:]))
+
+
+def test_fma_lerp_shape_mismatch(gpu):
+    with pytest.raises(ValueError):
+        K.fma(np.ones(3), np.ones(3), np.ones(4))
+    with pytest.raises(ValueError):
+        K.lerp(np.ones(3), np.ones(2), 0.5)

@@ -77,3 +77,44 @@ def test_std_is_sqrt_of_var(gpu, dt):
         want = float(np.sqrt(dt(v)))  # var's bits, one correctly rounded sqrt
         assert got == want
         assert got == pytest.approx(float(np.std(x.astype(np.float64), ddof=ddof)), rel=1e-12 if dt == np.float64 else 1e-5)
+
+
+LD = np.longdouble
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+@pytest.mark.parametrize("n", [1, 5, 4097, 1_000_003])
+def test_softmax_and_log_softmax(gpu, dt, n):
+    """softmax / log_softmax (ufuncs.rs:1411-1457) against the exact values in long double. Every
+    term of the sum is positive, so the tree's relative error is at most 64 eps; exp adds 1 ulp
+    and the division one rounding."""
+    r = np.random.default_rng(n + 7)
+    x = (r.normal(size=n) * 8).astype(dt)
+    xl = x.astype(LD)
+    e = np.exp(xl - xl.max())
+    s = e.sum()
+    want = (e / s).astype(np.float64)
+    eps = 2.0**-53 if dt == np.float64 else 2.0**-24
+    got = K.softmax(x)
+    assert got.dtype == dt and got.shape == x.shape
+    assert np.all(np.abs(got - want) <= 70 * eps * want + 1e-300)
+    lwant = ((xl - xl.max()) - np.log(s)).astype(np.float64)
+    lgot = K.log_softmax(x)
+    assert np.all(np.abs(lgot - lwant) <= 70 * eps * (1 + np.abs(np.log(np.float64(s)))) + 2 * eps * np.abs(lwant))
+
+
+def test_softmax_special_inputs(gpu):
+    assert K.softmax(np.array([], np.float64)).size == 0
+    np.testing.assert_array_equal(K.softmax(np.array([3.0])), [1.0])
+    np.testing.assert_array_equal(K.softmax(np.array([1.0, 1.0, 1.0, 1.0])), [0.25] * 4)
+    np.testing.assert_array_equal(K.log_softmax(np.array([2.0, 2.0])), [-np.log(2.0)] * 2)
+    # shift by the max: no overflow for large inputs
+    big = np.array([1000.0, 1000.0 + np.log(3.0)])
+    e = np.exp(LD(big[1] - big[0]))  # the difference is exact (Sterbenz); ~3 as rounded
+    np.testing.assert_allclose(K.softmax(big), [float(1 / (1 + e)), float(e / (1 + e))], rtol=1e-15)
+    # a NaN anywhere makes every output NaN, as in the reference (NaN reaches the sum)
+    assert np.isnan(K.softmax(np.array([1.0, np.nan, 2.0]))).all()
+    assert np.isnan(K.log_softmax(np.array([1.0, np.nan, 2.0]))).all()
+    # 2-D input: the reference flattens, and so does this
+    m = np.arange(6.0).reshape(2, 3)
+    np.testing.assert_allclose(K.softmax(m).sum(), 1.0, rtol=1e-15)

@@ -353,9 +353,9 @@ ACT = {  # name: (kernel call, reference expression, functions it calls)
 
 
 def _fn(name, d):
-    def f(v):
+    def f(*v):
         with np.errstate(all="ignore"):
-            r = _CR[name](np.asarray(v, np.float64).astype(LD)).astype(np.float64)
+            r = _CR[name](*[np.asarray(u, np.float64).astype(LD) for u in v]).astype(np.float64)
         for _ in range(abs(d)):
             r = np.nextafter(r, np.inf if d > 0 else -np.inf)
         return r
@@ -455,3 +455,112 @@ def test_prelu_and_steps_exact(gpu):
     tt = np.clip((x - e0) / (e1 - e0), 0.0, 1.0)
     np.testing.assert_array_equal(K.smoothstep(e0, e1, x), tt * tt * (3.0 - 2.0 * tt))
     np.testing.assert_array_equal(K.smootherstep(e0, e1, x), tt * tt * tt * (tt * (tt * 6.0 - 15.0) + 10.0))
+
+
+# ---- power (ufuncs.rs:166, 226, 305) and the Lanczos gamma (:1122-1195)
+
+def _pow_cases():
+    r = np.random.default_rng(98)
+    x = np.exp(r.uniform(-700, 700, 30000))
+    y = r.uniform(-1, 1, 30000) * np.minimum(700 / np.abs(np.log(x)), 1e6)  # |y ln x| < 700
+    x2 = 1.0 + r.uniform(-1, 1, 5000) * 2.0**-30  # near 1 with large y
+    y2 = r.uniform(-1, 1, 5000) * 2.0**38
+    x3 = -np.exp(r.uniform(-5, 5, 5000))  # negative base, integer exponent
+    y3 = np.floor(r.uniform(-60, 60, 5000))
+    x4 = r.uniform(0, 10, 5000)  # subnormal and near-overflow results
+    y4 = np.where(r.random(5000) < 0.5, -1074.0, 1023.0) / np.maximum(np.log2(x4), 0.01) * r.uniform(0.98, 1.0, 5000)
+    return np.concatenate([x, x2, x3, x4]), np.concatenate([y, y2, y3, y4])
+
+
+def test_power_within_one_ulp(gpu):
+    x, y = _pow_cases()
+    got = K.power(x, y)
+    with np.errstate(all="ignore"):
+        exact = np.power(x.astype(LD), y.astype(LD))
+    err = ulp_err(got, exact)
+    w = int(np.argmax(err))
+    print(f"pow: max {err.max():.3f} ulp at ({x[w]!r}, {y[w]!r})")
+    assert err.max() <= 1.0
+    np.testing.assert_array_equal(K.power_scalar(x[:1000], 2.5), K.power(x[:1000], np.full(1000, 2.5)))
+    # C's (and Rust's powf's) special cases, to the bit
+    sp = np.array([0.0, -0.0, 1.0, -1.0, 2.0, -2.0, 0.5, -0.5, 3.0, -3.0, np.inf, -np.inf, np.nan, 5e-324, 1e308])
+    A, B = np.meshgrid(sp, sp)
+    A, B = A.ravel(), B.ravel()
+    with np.errstate(all="ignore"):
+        pg, pw = K.power(A, B), np.power(A, B)
+    # zeros, infinities, NaN and +-1 to the bit; ordinary finite results (0.5^0.5) within 1 ulp
+    near = np.isfinite(pw) & (pw != 0) & (np.abs(pg - pw) <= np.spacing(np.abs(pw)))
+    bad = ~((np.isnan(pg) & np.isnan(pw)) | ((pg == pw) & (np.signbit(pg) == np.signbit(pw))) | near)
+    assert not bad.any(), list(zip(A[bad], B[bad], pg[bad], pw[bad]))
+    # broadcasting, as the power ufunc does
+    np.testing.assert_array_equal(K.power(np.array([[2.0], [3.0]]), np.array([0.0, 1.0, 2.0])), [[1, 2, 4], [1, 3, 9]])
+    # binary32: the FP64 evaluation rounded once
+    x32, y32 = np.abs(x[:20000]).astype(np.float32), (y[:20000] / 50).astype(np.float32)
+    g32 = K.power(x32, y32)
+    with np.errstate(all="ignore"):
+        e32 = np.power(x32.astype(LD), y32.astype(LD))
+    fin = np.isfinite(e32.astype(np.float32)) & (np.abs(e32) > 1e-37)
+    assert ulp_err(g32[fin], e32[fin], np.float32).max() <= 0.5 + 1e-9
+
+
+_SQRT_2PI = float(np.sqrt(np.float64(2.0) * np.float64(np.pi)))
+_LANCZOS = [0.99999999999980993, 676.5203681218851, -1259.1392167224028, 771.32342877765313, -176.61502916214059,
+            12.507343278686905, -0.13857109526572012, 9.9843695780195716e-6, 1.5056327351493116e-7]
+
+
+def _lanczos_expr(x, F):
+    xm = x - 1.0
+    y = np.full_like(x, _LANCZOS[0])
+    for i in range(1, 9):
+        y = y + _LANCZOS[i] / (xm + float(i))
+    t = xm + 7.0 + 0.5
+    return _SQRT_2PI * F["pow"](t, xm + 0.5) * F["exp"](-t) * y
+
+
+def _gamma_expr(x, F):
+    with np.errstate(all="ignore"):
+        refl = _PI / (F["sin"](_PI * x) * _lanczos_expr(1.0 - x, F))
+        main = _lanczos_expr(x, F)
+    out = np.where(x < 0.5, refl, main)
+    return np.where((x <= 0) & (np.floor(x) == x), np.inf, out)
+
+
+_K["pow"] = 1
+_CR["pow"] = np.power
+ACT["gamma"] = (K.gamma, _gamma_expr, ("pow", "exp", "sin"))
+ACT["lgamma"] = (K.lgamma, lambda x, F: F["log"](np.abs(_lanczos_expr(x, F))), ("pow", "exp", "log"))
+
+
+@pytest.mark.parametrize("name", ["gamma", "lgamma"])
+def test_gamma_matches_reference_expression(gpu, name):
+    r = np.random.default_rng(99)
+    x = np.concatenate([[0.5, 1.0, 2.0, 0.0, -1.0, -2.5, 141.0, 150.0, 171.5, np.nan, np.inf, 1e-300],
+                        r.uniform(0.5, 30, 5000), r.uniform(-20, 0.5, 3000), r.uniform(30, 180, 1000)])
+    if name == "lgamma":
+        x = x[(x > 0) | np.isnan(x)]
+    _check_act(name, x)
+
+
+def _check_act(name, x):
+    call, expr, fns = ACT[name]
+    got = call(x)
+    centre, lo, hi = _bracket(expr, fns, x)
+    ok = (np.isnan(centre) & np.isnan(got)) | ((got >= lo) & (got <= hi))
+    assert np.all(ok), (name, x[~ok][:5], got[~ok][:5], centre[~ok][:5])
+    print(f"{name}: {np.mean((got == centre) | (np.isnan(got) & np.isnan(centre))):.4f} equal to the correctly rounded evaluation")
+
+
+def test_gamma_against_its_mathematics(gpu):
+    """The Lanczos fit (g = 7, n = 9) is good to ~1e-15 relative; the reference's powf overflows
+    t^(x + 0.5) past x ~ 141, so the comparison stops there."""
+    import scipy.special as sps
+    r = np.random.default_rng(100)
+    x = r.uniform(0.5, 140, 20000)
+    np.testing.assert_allclose(K.gamma(x), sps.gamma(x), rtol=5e-13)
+    xn = r.uniform(-10, 0.5, 20000)
+    xn = xn[np.abs(xn - np.round(xn)) > 1e-3]
+    np.testing.assert_allclose(K.gamma(xn), sps.gamma(xn), rtol=1e-10)
+    lg = K.lgamma(x)
+    assert np.all(np.abs(lg - sps.gammaln(x)) <= 5e-13 * (1 + np.abs(sps.gammaln(x))))
+    assert K.gamma(np.array([5.0]))[0] == pytest.approx(24.0, rel=1e-14)
+    assert K.gamma(np.array([150.0]))[0] == np.inf  # the reference's overflow, kept

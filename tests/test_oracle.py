@@ -178,6 +178,34 @@ def test_prelu_matches_numpy(oracle):
     np.testing.assert_array_equal(oracle.ref_zip(11, x, al), [1.5, -0.5, 0.0, -0.0, np.nan, -np.inf, np.inf, 3.0])
 
 
+def test_fma_is_one_rounding(oracle):
+    """ufuncs.rs:864 mul_add: the exact a*b + c rounded once, checked with rational arithmetic."""
+    from fractions import Fraction
+    r = np.random.default_rng(77)
+    a, b = r.normal(size=3000), r.normal(size=3000)
+    c = -(a * b) + r.normal(size=3000) * 1e-17  # cancellation: the low half of a*b decides
+    for t in (np.float64, np.float32):
+        A, B, Cc = a.astype(t), b.astype(t), c.astype(t)
+        got = oracle.ref_fma(A, B, Cc)
+        exact = [Fraction(float(x)) * Fraction(float(y)) + Fraction(float(z)) for x, y, z in zip(A, B, Cc)]
+        if t == np.float64:  # float(Fraction) is the correctly rounded double
+            np.testing.assert_array_equal(got, np.array([float(e) for e in exact]))
+        else:  # the nearest binary32: no neighbour is closer to the exact value
+            for g, e in zip(got, exact):
+                lo, hi = np.nextafter(g, np.float32(-np.inf)), np.nextafter(g, np.float32(np.inf))
+                d = abs(Fraction(float(g)) - e)
+                assert d <= abs(Fraction(float(lo)) - e) and d <= abs(Fraction(float(hi)) - e)
+
+
+def test_lerp_is_three_roundings(oracle):
+    r = np.random.default_rng(78)
+    a, b = r.normal(size=5000), r.normal(size=5000)
+    for t in (0.0, 0.3, 1.0, -2.5):
+        np.testing.assert_array_equal(oracle.ref_lerp(a, b, t), a + t * (b - a))
+    a32, b32 = a.astype(np.float32), b.astype(np.float32)
+    np.testing.assert_array_equal(oracle.ref_lerp(a32, b32, 0.3), a32 + np.float32(0.3) * (b32 - a32))
+
+
 def test_two_roundings_not_fma(oracle):
     """y = x*1.5 + (-0.25) is two roundings: bit-equal to numpy's separate ops, and for some x
     different from the fused (single-rounding) value."""
