@@ -346,10 +346,9 @@ def bench_ufuncs(device, kernels, call, s, stream, d32, d64):
         dy = device.DeviceArray((n,), dt)
         fn = f"nrk_dev_map_{tag}"
         row = {}
-        for name, steps in (("copy", ()), ("exp", (("exp",),)), ("log", (("abs",), ("log",))), ("tanh", (("tanh",),)),
-                            ("sigmoid", (("neg",), ("exp",), ("add", 1.0), ("rdiv", 1.0))), ("floor", (("floor",),))):
-            ops, k = kernels.chain(*steps)
-            f = lambda: call(fn, C.cast(ops, C.c_void_p), k, C.c_void_p(dx.ptr), C.c_void_p(dy.ptr), C.c_size_t(n), C.c_void_p(s))
+        isz = np.dtype(dt).itemsize
+
+        def timed(name, f, nbytes):
             f()
             stream.synchronize()
             a, b = device.Event(), device.Event()
@@ -359,8 +358,27 @@ def bench_ufuncs(device, kernels, call, s, stream, d32, d64):
             b.record(s)
             stream.synchronize()
             ms = a.elapsed_ms(b) / 10
-            gbs = 2 * n * np.dtype(dt).itemsize / ms / 1e6
+            gbs = nbytes / ms / 1e6
             row[name] = {"ms": round(ms, 4), "GB/s": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 3)}
+
+        for name, steps in (("copy", ()), ("exp", (("exp",),)), ("log", (("abs",), ("log",))), ("tanh", (("tanh",),)),
+                            ("sigmoid", (("neg",), ("exp",), ("add", 1.0), ("rdiv", 1.0))), ("floor", (("floor",),)),
+                            ("sin", (("sin",),)), ("pow2.5", (("abs",), ("pow", 2.5))), ("gelu", (("gelu",),)),
+                            ("erf", (("erf",),))):
+            ops, k = kernels.chain(*steps)
+            timed(name, lambda: call(fn, C.cast(ops, C.c_void_p), k, C.c_void_p(dx.ptr), C.c_void_p(dy.ptr), C.c_size_t(n),
+                                     C.c_void_p(s)), 2 * n * isz)
+        # two- and three-operand streams, a dot product and softmax (bytes: each input read once,
+        # the output written once; softmax's own passes re-read x and y)
+        dout = device.DeviceArray((16,), dt)
+        timed("dot", lambda: call(f"nrk_dev_dot_{tag}", C.c_void_p(dx.ptr), C.c_void_p(dy.ptr), C.c_size_t(n),
+                                  C.c_void_p(dout.ptr), C.c_void_p(s)), 2 * n * isz)
+        half = n // 2
+        timed("fma", lambda: call(f"nrk_dev_fma_{tag}", C.c_void_p(dx.ptr), C.c_void_p(dx.ptr + half * isz),
+                                  C.c_void_p(dy.ptr), C.c_void_p(dy.ptr), C.c_size_t(half), C.c_void_p(s)), 4 * half * isz)
+        timed("softmax", lambda: call(f"nrk_dev_softmax_{tag}", C.c_void_p(dx.ptr), C.c_void_p(dy.ptr), C.c_size_t(n),
+                                      C.c_void_p(s)), 2 * n * isz)
+        del dout
         out[tag] = row
         log(f"ufuncs {tag}: " + ", ".join(f"{k} {v['ms']:.3f} ms ({v['GB/s']:.0f} GB/s)" for k, v in row.items()))
         del dy

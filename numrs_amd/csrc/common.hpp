@@ -143,16 +143,26 @@ __device__ __forceinline__ T via_f64(T x, F f) {
     return (T)f((double)x);
 }
 
+// Which steps a kernel instantiation compiles: MATH_NONE the IEEE steps (ADD .. CLAMP_HI),
+// MATH_LIGHT also the elementary functions, activations and special functions, MATH_ALL also
+// pow, gamma and lgamma (a table-driven double-double log inside; alone they take the map kernel
+// from ~100 to over 200 VGPRs). The register budget of a kernel is that of its largest case.
+enum { MATH_NONE = 0, MATH_LIGHT = 1, MATH_ALL = 2 };
+__host__ __device__ inline int sop_math_class(int op) {
+    if (op == NRK_SOP_POW || op == NRK_SOP_GAMMA || op == NRK_SOP_LGAMMA) return MATH_ALL;
+    return op >= NRK_SOP_EXP ? MATH_LIGHT : MATH_NONE;
+}
+
 // Apply the whole chain to N register values, each step (NRK_SOP_*) with one IEEE rounding: one
 // (wavefront-uniform) branch per step, then N straight-line operations, so the op dispatch is
-// not repeated per element. MATH adds the elementary functions (exp .. tanh); kernels built
-// without it (the reductions) keep their register budget, and their chains are validated to
-// hold no such step.
-template <typename T, int N, bool MATH = false>
-__device__ __forceinline__ void apply_chain_n(const Chain &c, T (&v)[N]) {
-    for (int k = 0; k < c.len; ++k) {
-        const T s = (T)c.s[k];
-        switch (c.op[k]) {
+// not repeated per element. MATH (above) selects the steps compiled in; kernels built with
+// MATH_NONE (the reductions) keep their register budget, and their chains are validated to hold
+// no other step.
+template <typename T, int N, int MATH = MATH_NONE>
+__device__ __forceinline__ void apply_step_n(int op, double sd, T (&v)[N]) {
+    {
+        const T s = (T)sd;
+        switch (op) {
 #define NRK_CHAIN_CASE(OP, EXPR)                      \
     case OP:                                          \
         _Pragma("unroll") for (int i = 0; i < N; ++i) { \
@@ -179,8 +189,8 @@ __device__ __forceinline__ void apply_chain_n(const Chain &c, T (&v)[N]) {
             NRK_CHAIN_CASE(NRK_SOP_CLAMP_LO, x < s ? s : x)
             NRK_CHAIN_CASE(NRK_SOP_CLAMP_HI, x > s ? s : x)
             default:
-                if constexpr (MATH) {
-                    switch (c.op[k]) {
+                if constexpr (MATH >= MATH_LIGHT) {
+                    switch (op) {
                         NRK_CHAIN_CASE(NRK_SOP_EXP, via_f64(x, [](double d) { return fm::exp(d); }))
                         NRK_CHAIN_CASE(NRK_SOP_EXP2, via_f64(x, [](double d) { return fm::exp2(d); }))
                         NRK_CHAIN_CASE(NRK_SOP_EXPM1, via_f64(x, [](double d) { return fm::expm1(d); }))
@@ -201,8 +211,8 @@ __device__ __forceinline__ void apply_chain_n(const Chain &c, T (&v)[N]) {
                         NRK_CHAIN_CASE(NRK_SOP_COS, via_f64(x, [](double d) { return fm::cos(d); }))
                         NRK_CHAIN_CASE(NRK_SOP_TAN, via_f64(x, [](double d) { return fm::tan(d); }))
                         NRK_CHAIN_CASE(NRK_SOP_RELU, via_f64(x, [](double d) { return fm::relu(d); }))
-                        NRK_CHAIN_CASE(NRK_SOP_LEAKY_RELU, (T)((double)x > 0.0 ? (double)x : (double)c.s[k] * (double)x))
-                        NRK_CHAIN_CASE(NRK_SOP_ELU, (T)fm::elu((double)x, c.s[k]))
+                        NRK_CHAIN_CASE(NRK_SOP_LEAKY_RELU, (T)((double)x > 0.0 ? (double)x : sd * (double)x))
+                        NRK_CHAIN_CASE(NRK_SOP_ELU, (T)fm::elu((double)x, sd))
                         NRK_CHAIN_CASE(NRK_SOP_SELU, via_f64(x, [](double d) { return fm::selu(d); }))
                         NRK_CHAIN_CASE(NRK_SOP_HARDSIGMOID, via_f64(x, [](double d) { return fm::hardsigmoid(d); }))
                         NRK_CHAIN_CASE(NRK_SOP_HARDSWISH, via_f64(x, [](double d) { return fm::hardswish(d); }))
@@ -217,15 +227,36 @@ __device__ __forceinline__ void apply_chain_n(const Chain &c, T (&v)[N]) {
                         NRK_CHAIN_CASE(NRK_SOP_ERF, via_f64(x, [](double d) { return fm::erf_ref(d); }))
                         NRK_CHAIN_CASE(NRK_SOP_ERFC, via_f64(x, [](double d) { return 1.0 - fm::erf_ref(d); }))
                         NRK_CHAIN_CASE(NRK_SOP_DIGAMMA, via_f64(x, [](double d) { return fm::digamma(d); }))
-                        NRK_CHAIN_CASE(NRK_SOP_POW, (T)fm::pow((double)x, (double)s))
-                        NRK_CHAIN_CASE(NRK_SOP_GAMMA, via_f64(x, [](double d) { return fm::gamma_ref(d); }))
-                        NRK_CHAIN_CASE(NRK_SOP_LGAMMA, via_f64(x, [](double d) { return fm::lgamma_ref(d); }))
-                        default: break;
+                        default:
+                            if constexpr (MATH >= MATH_ALL) {
+                                switch (op) {
+                                    NRK_CHAIN_CASE(NRK_SOP_POW, (T)fm::pow((double)x, (double)s))
+                                    NRK_CHAIN_CASE(NRK_SOP_GAMMA, via_f64(x, [](double d) { return fm::gamma_ref(d); }))
+                                    NRK_CHAIN_CASE(NRK_SOP_LGAMMA, via_f64(x, [](double d) { return fm::lgamma_ref(d); }))
+                                    default: break;
+                                }
+                            }
+                            break;
                     }
                 }
                 break;
 #undef NRK_CHAIN_CASE
         }
+    }
+}
+
+template <typename T, int N, int MATH = MATH_NONE>
+__device__ __forceinline__ void apply_chain_n(const Chain &c, T (&v)[N]) {
+    for (int k = 0; k < c.len; ++k) apply_step_n<T, N, MATH>(c.op[k], c.s[k], v);
+}
+
+// The chain with its one elementary-function step fixed at compile time (OP at index `at`), the
+// other steps from the IEEE switch: a kernel built this way holds only OP's registers.
+template <typename T, int N, int OP>
+__device__ __forceinline__ void apply_chain_one_n(const Chain &c, int at, T (&v)[N]) {
+    for (int k = 0; k < c.len; ++k) {
+        if (k == at) apply_step_n<T, N, MATH_ALL>(OP, c.s[k], v);
+        else apply_step_n<T, N, MATH_NONE>(c.op[k], c.s[k], v);
     }
 }
 // Validates and copies a chain. Without allow_math, elementary-function steps are refused (the

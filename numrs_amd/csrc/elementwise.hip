@@ -33,14 +33,19 @@ __device__ __forceinline__ T apply_bop(T a, T b) {
     else return (T)fm::logaddexp((double)a, (double)b);                // LOGADDEXP
 }
 
-// The map applies every NRK_SOP_* step, elementary functions included.
-template <typename T, int N>
-__device__ __forceinline__ void map_chain(const Chain &c, T (&v)[N]) {
-    apply_chain_n<T, N, true>(c, v);
+// The map applies the NRK_SOP_* steps of its class (MATH_NONE / LIGHT / ALL), or, with OP >= 0,
+// the chain whose single elementary-function step OP sits at index `at`. The host launches the
+// one-op kernel when the chain has exactly one such step (every ufunc call), else the smallest
+// class holding every step. One kernel over all cases held ~200 VGPRs (2 waves per SIMD): the
+// loop over steps kept every case's constants live; each op alone needs at most 64.
+template <typename T, int CLS, int OP, int N>
+__device__ __forceinline__ void map_chain(const Chain &c, int at, T (&v)[N]) {
+    if constexpr (OP >= 0) apply_chain_one_n<T, N, OP>(c, at, v);
+    else apply_chain_n<T, N, CLS>(c, v);
 }
 
-template <typename T>
-__global__ __launch_bounds__(EB) void map_kernel(Chain chain, const T *x, T *y, size_t n, int vec_ok) {
+template <typename T, int CLS, int OP = -1>
+__global__ __launch_bounds__(EB) void map_kernel(Chain chain, int at, const T *x, T *y, size_t n, int vec_ok) {
     typedef typename Vec16<T>::type V;
     constexpr int VN = Vec16<T>::N;
     const size_t tid = blockIdx.x * (size_t)EB + threadIdx.x;
@@ -59,7 +64,7 @@ __global__ __launch_bounds__(EB) void map_kernel(Chain chain, const T *x, T *y, 
 #pragma unroll
                 for (int e = 0; e < VN; ++e) r[u * VN + e] = v[e];
             }
-            map_chain(chain, r);
+            map_chain<T, CLS, OP>(chain, at, r);
 #pragma unroll
             for (int u = 0; u < EUNROLL; ++u) {
                 V v;
@@ -73,7 +78,7 @@ __global__ __launch_bounds__(EB) void map_kernel(Chain chain, const T *x, T *y, 
             T r[VN];
 #pragma unroll
             for (int e = 0; e < VN; ++e) r[e] = v[e];
-            map_chain(chain, r);
+            map_chain<T, CLS, OP>(chain, at, r);
             V w;
 #pragma unroll
             for (int e = 0; e < VN; ++e) w[e] = r[e];
@@ -83,7 +88,7 @@ __global__ __launch_bounds__(EB) void map_kernel(Chain chain, const T *x, T *y, 
     }
     for (size_t i = done + tid; i < n; i += nthreads) {
         T r[1] = {x[i]};
-        map_chain(chain, r);
+        map_chain<T, CLS, OP>(chain, at, r);
         y[i] = r[0];
     }
 }
@@ -143,7 +148,37 @@ nrk_status map(const Chain &chain, const T *x, T *y, size_t n, hipStream_t strea
     if (n == 0) return NRK_OK;
     if (!x || !y) return fail(NRK_ERR_INVALID_ARGUMENT, "x or y is NULL");
     const int vec_ok = (reinterpret_cast<uintptr_t>(x) % 16 == 0) && (reinterpret_cast<uintptr_t>(y) % 16 == 0);
-    hipLaunchKernelGGL(map_kernel<T>, dim3(stream_grid(n / Vec16<T>::N + 1)), dim3(EB), 0, stream, chain, x, y, n, vec_ok);
+    int cls = MATH_NONE, n_math = 0, at = -1;
+    for (int k = 0; k < chain.len; ++k) {
+        const int c = sop_math_class(chain.op[k]);
+        cls = std::max(cls, c);
+        if (c != MATH_NONE) {
+            ++n_math;
+            at = k;
+        }
+    }
+    const dim3 grid(stream_grid(n / Vec16<T>::N + 1));
+    static_assert(NRK_SOP_EXP == 18 && NRK_SOP_COUNT == 57, "the one-op kernels below list codes 18 .. 56");
+    if (n_math == 1) {
+        switch (chain.op[at]) {
+#define NRK_MAP_ONE(OP) \
+    case OP: hipLaunchKernelGGL((map_kernel<T, MATH_ALL, OP>), grid, dim3(EB), 0, stream, chain, at, x, y, n, vec_ok); break;
+            NRK_MAP_ONE(18) NRK_MAP_ONE(19) NRK_MAP_ONE(20) NRK_MAP_ONE(21) NRK_MAP_ONE(22) NRK_MAP_ONE(23) NRK_MAP_ONE(24)
+            NRK_MAP_ONE(25) NRK_MAP_ONE(26) NRK_MAP_ONE(27) NRK_MAP_ONE(28) NRK_MAP_ONE(29) NRK_MAP_ONE(30) NRK_MAP_ONE(31)
+            NRK_MAP_ONE(32) NRK_MAP_ONE(33) NRK_MAP_ONE(34) NRK_MAP_ONE(35) NRK_MAP_ONE(36) NRK_MAP_ONE(37) NRK_MAP_ONE(38)
+            NRK_MAP_ONE(39) NRK_MAP_ONE(40) NRK_MAP_ONE(41) NRK_MAP_ONE(42) NRK_MAP_ONE(43) NRK_MAP_ONE(44) NRK_MAP_ONE(45)
+            NRK_MAP_ONE(46) NRK_MAP_ONE(47) NRK_MAP_ONE(48) NRK_MAP_ONE(49) NRK_MAP_ONE(50) NRK_MAP_ONE(51) NRK_MAP_ONE(52)
+            NRK_MAP_ONE(53) NRK_MAP_ONE(54) NRK_MAP_ONE(55) NRK_MAP_ONE(56)
+#undef NRK_MAP_ONE
+            default: return fail(NRK_ERR_INVALID_ARGUMENT, "op %d has no map kernel", chain.op[at]);
+        }
+    } else if (cls == MATH_NONE) {
+        hipLaunchKernelGGL((map_kernel<T, MATH_NONE>), grid, dim3(EB), 0, stream, chain, -1, x, y, n, vec_ok);
+    } else if (cls == MATH_LIGHT) {
+        hipLaunchKernelGGL((map_kernel<T, MATH_LIGHT>), grid, dim3(EB), 0, stream, chain, -1, x, y, n, vec_ok);
+    } else {
+        hipLaunchKernelGGL((map_kernel<T, MATH_ALL>), grid, dim3(EB), 0, stream, chain, -1, x, y, n, vec_ok);
+    }
     NRK_HIP(hipGetLastError());
     return NRK_OK;
 }
