@@ -104,9 +104,10 @@ nrk_status dev_std(const T *a, size_t n, size_t ddof, T *out, hipStream_t s) {
     NRK_TRY(dev_var<T>(a, n, ddof, out, s));
     return sqrt_scalar<T>(out, s);
 }
-// ufuncs.rs:1411-1457 softmax / log_softmax over the whole array: max, e = exp(x - max),
-// sum of e, then e / sum or (x - max) - ln(sum). The max and sum stay on the device (two
-// scalars in the stream's workspace), so nothing waits on the host.
+// ufuncs.rs:1411-1457 softmax / log_softmax over the whole array: (max, sum of e^(x - max)) in
+// one read, then y = e^(x - max) / sum or (x - max) - ln(sum) in a second (read x, write y). The
+// max and sum stay on the device (two scalars in the stream's workspace), so nothing waits on
+// the host.
 template <typename T>
 nrk_status dev_softmax(bool log_form, const T *x, T *y, size_t n, hipStream_t s) {
     NRK_TRY(ensure_device());
@@ -115,10 +116,8 @@ nrk_status dev_softmax(bool log_form, const T *x, T *y, size_t n, hipStream_t s)
     void *ws = nullptr;
     NRK_TRY(get_workspace(s, 2 * sizeof(T), &ws));
     T *ms = (T *)ws;
-    NRK_TRY(dev_max<T>(x, n, ms, s));
-    NRK_TRY(softmax_pass<T>(0, x, y, ms, n, s));
-    NRK_TRY(dev_sum<T>(y, n, ms + 1, s));
-    return softmax_pass<T>(log_form ? 2 : 1, x, y, ms, n, s);
+    NRK_TRY(softmax_stats<T>(x, n, ms, s));
+    return softmax_pass<T>(log_form ? 2 : 3, x, y, ms, n, s);
 }
 
 template <typename T>

@@ -274,7 +274,8 @@ template nrk_status tri<float>(int, const float *, const float *, const float *,
 template nrk_status tri<double>(int, const double *, const double *, const double *, double, double *, size_t, hipStream_t);
 
 // ---- softmax / log_softmax passes (ufuncs.rs:1411-1457). ms = {max, sum} on the device.
-// mode 0: y = exp(x - max); mode 1: y = y / sum; mode 2: y = (x - max) - ln(sum).
+// mode 0: y = exp(x - max); mode 1: y = y / sum; mode 2: y = (x - max) - ln(sum);
+// mode 3: y = exp(x - max) / sum (the reference's per-element expression, one pass).
 template <typename T, int MODE>
 __global__ __launch_bounds__(EB) void softmax_pass_kernel(const T *x, T *y, const T *ms, size_t n) {
     const T m = ms[0];
@@ -284,6 +285,7 @@ __global__ __launch_bounds__(EB) void softmax_pass_kernel(const T *x, T *y, cons
     for (size_t i = blockIdx.x * (size_t)EB + threadIdx.x; i < n; i += nthreads) {
         if constexpr (MODE == 0) y[i] = (T)fm::exp((double)rn_sub(x[i], m));
         else if constexpr (MODE == 1) y[i] = rn_div(y[i], s);
+        else if constexpr (MODE == 3) y[i] = rn_div((T)fm::exp((double)rn_sub(x[i], m)), s);
         else y[i] = rn_sub(rn_sub(x[i], m), lse);
     }
 }
@@ -294,7 +296,8 @@ nrk_status softmax_pass(int mode, const T *x, T *y, const T *ms, size_t n, hipSt
     const dim3 grid((unsigned)std::min<size_t>(ceil_div(n, (size_t)EB * 4), (size_t)cu_count() * 32));
     if (mode == 0) hipLaunchKernelGGL((softmax_pass_kernel<T, 0>), grid, dim3(EB), 0, stream, x, y, ms, n);
     else if (mode == 1) hipLaunchKernelGGL((softmax_pass_kernel<T, 1>), grid, dim3(EB), 0, stream, x, y, ms, n);
-    else hipLaunchKernelGGL((softmax_pass_kernel<T, 2>), grid, dim3(EB), 0, stream, x, y, ms, n);
+    else if (mode == 2) hipLaunchKernelGGL((softmax_pass_kernel<T, 2>), grid, dim3(EB), 0, stream, x, y, ms, n);
+    else hipLaunchKernelGGL((softmax_pass_kernel<T, 3>), grid, dim3(EB), 0, stream, x, y, ms, n);
     NRK_HIP(hipGetLastError());
     return NRK_OK;
 }

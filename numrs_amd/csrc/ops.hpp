@@ -33,6 +33,9 @@ template <typename T>
 nrk_status dot(const T *a, const T *b, size_t n, T *out, hipStream_t stream);
 template <typename T>
 nrk_status sqrt_scalar(T *p, hipStream_t stream);
+// ms[0] = max x (NaN ignored), ms[1] = sum e^(x - max), in one read of x
+template <typename T>
+nrk_status softmax_stats(const T *x, size_t n, T *ms, hipStream_t stream);
 
 template <typename T>
 nrk_status map(const Chain &chain, const T *x, T *y, size_t n, hipStream_t stream);

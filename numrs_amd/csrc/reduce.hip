@@ -1066,6 +1066,106 @@ nrk_status dot(const T *a, const T *b, size_t n, T *out, hipStream_t stream) {
 template nrk_status dot<float>(const float *, const float *, size_t, float *, hipStream_t);
 template nrk_status dot<double>(const double *, const double *, size_t, double *, hipStream_t);
 
+// ---------------- softmax statistics: (max, sum of e^(x - max)) in one read ----------------
+// ufuncs.rs:1411-1457 forms max, then sum exp(x - max). Here each thread takes the max m of its
+// SM_PER values and the sum s of e^(v - m) over them; partials (m, s) merge in a fixed tree as
+// (max(m1, m2), s1 e^(m1 - m) + s2 e^(m2 - m)). -inf values add 0 (as e^(-inf - M) does in the
+// reference), a partial with s = 0 adds nothing, and NaN or +inf values make s NaN, as they
+// make the reference's sum NaN. The max is exact and NaN-ignoring like the reference's fold.
+constexpr int SM_PER = 16;
+
+template <typename T>
+__device__ __forceinline__ T sm_exp(T v) { return (T)fm::exp((double)v); }
+
+template <typename T>
+__device__ __forceinline__ void sm_merge(T &m, T &s, T m2, T s2) {
+    const T mm = m > m2 ? m : (m2 == m2 ? m2 : m);  // max ignoring NaN
+    const T a = s == (T)0 ? (T)0 : rn_mul(s, sm_exp(rn_sub(m, mm)));
+    const T b = s2 == (T)0 ? (T)0 : rn_mul(s2, sm_exp(rn_sub(m2, mm)));
+    m = mm;
+    s = rn_add(a, b);
+}
+
+template <typename T>
+__global__ __launch_bounds__(RB) void softmax_stats_kernel(const T *x, size_t n, size_t nseg, T *partials) {
+    __shared__ T sm[2][WAVES], ss[2][WAVES];
+    const int t = threadIdx.x;
+    const T NEG_INF = -__builtin_huge_valf();
+    int parity = 0;
+    for (size_t seg = blockIdx.x; seg < nseg; seg += gridDim.x, parity ^= 1) {
+        const size_t off = seg * (size_t)(RB * SM_PER);
+        T v[SM_PER];
+#pragma unroll
+        for (int j = 0; j < SM_PER; ++j) {
+            const size_t pos = off + (size_t)j * RB + t;
+            v[j] = pos < n ? x[pos] : NEG_INF;
+        }
+        T m = NEG_INF;
+#pragma unroll
+        for (int j = 0; j < SM_PER; ++j) m = v[j] > m ? v[j] : m;  // NaN never wins
+        T s = (T)0;
+#pragma unroll
+        for (int j = 0; j < SM_PER; ++j) s = rn_add(s, v[j] == NEG_INF ? (T)0 : sm_exp(rn_sub(v[j], m)));
+#pragma unroll
+        for (int k = 32; k >= 1; k >>= 1) {
+            const T m2 = shfl_xor(m, k), s2 = shfl_xor(s, k);
+            sm_merge(m, s, m2, s2);
+        }
+        if ((t & 63) == 0) {
+            sm[parity][t >> 6] = m;
+            ss[parity][t >> 6] = s;
+        }
+        __syncthreads();
+        if (t == 0) {
+            T M = sm[parity][0], S = ss[parity][0];
+            for (int k = 1; k < WAVES; ++k) sm_merge(M, S, sm[parity][k], ss[parity][k]);
+            partials[2 * seg] = M;
+            partials[2 * seg + 1] = S;
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(FB) void softmax_stats_finish(const T *partials, size_t nseg, T *ms) {
+    __shared__ T sm[FB / 64], ss[FB / 64];
+    const int t = threadIdx.x;
+    T m = -__builtin_huge_valf(), s = (T)0;
+    for (size_t k = t; k < nseg; k += FB) sm_merge(m, s, partials[2 * k], partials[2 * k + 1]);
+#pragma unroll
+    for (int k = 32; k >= 1; k >>= 1) {
+        const T m2 = shfl_xor(m, k), s2 = shfl_xor(s, k);
+        sm_merge(m, s, m2, s2);
+    }
+    if ((t & 63) == 0) {
+        sm[t >> 6] = m;
+        ss[t >> 6] = s;
+    }
+    __syncthreads();
+    if (t == 0) {
+        T M = sm[0], S = ss[0];
+        for (int k = 1; k < FB / 64; ++k) sm_merge(M, S, sm[k], ss[k]);
+        ms[0] = M;
+        ms[1] = S;
+    }
+}
+
+template <typename T>
+nrk_status softmax_stats(const T *x, size_t n, T *ms, hipStream_t stream) {
+    NRK_TRY(ensure_device());
+    if (n == 0 || !x || !ms) return fail(NRK_ERR_INVALID_ARGUMENT, "softmax_stats: empty input or NULL pointer");
+    const size_t nseg = ceil_div(n, (size_t)RB * SM_PER);
+    Scratch s;
+    NRK_TRY(get_scratch(stream, 0, 2 * nseg * sizeof(T), &s));
+    const int grid = (int)std::min<size_t>(nseg, (size_t)cu_count() * 8);
+    hipLaunchKernelGGL(softmax_stats_kernel<T>, dim3(grid), dim3(RB), 0, stream, x, n, nseg, (T *)s.partials);
+    NRK_HIP(hipGetLastError());
+    hipLaunchKernelGGL(softmax_stats_finish<T>, dim3(1), dim3(FB), 0, stream, (const T *)s.partials, nseg, ms);
+    NRK_HIP(hipGetLastError());
+    return NRK_OK;
+}
+template nrk_status softmax_stats<float>(const float *, size_t, float *, hipStream_t);
+template nrk_status softmax_stats<double>(const double *, size_t, double *, hipStream_t);
+
 // y = sqrt(y) for one device scalar (norm_l2 = sqrt of the sum of squares, std = sqrt of var).
 template <typename T>
 __global__ void sqrt_scalar_kernel(T *p) { *p = rn_sqrt(*p); }
