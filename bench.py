@@ -378,6 +378,8 @@ def bench_ufuncs(device, kernels, call, s, stream, d32, d64):
                                   C.c_void_p(dy.ptr), C.c_void_p(dy.ptr), C.c_size_t(half), C.c_void_p(s)), 4 * half * isz)
         timed("softmax", lambda: call(f"nrk_dev_softmax_{tag}", C.c_void_p(dx.ptr), C.c_void_p(dy.ptr), C.c_size_t(n),
                                       C.c_void_p(s)), 2 * n * isz)
+        timed("cumsum", lambda: call(f"nrk_dev_accumulate_{tag}", C.c_int32(0), C.c_void_p(dx.ptr), C.c_size_t(1),
+                                     C.c_size_t(n), C.c_void_p(dy.ptr), C.c_void_p(s)), 2 * n * isz)
         del dout
         out[tag] = row
         log(f"ufuncs {tag}: " + ", ".join(f"{k} {v['ms']:.3f} ms ({v['GB/s']:.0f} GB/s)" for k, v in row.items()))

@@ -276,6 +276,17 @@ nrk_status nrk_reduce_axis_f32(int32_t op, const float *x, size_t outer, size_t 
 nrk_status nrk_dev_reduce_axis_f64(int32_t op, const double *x, size_t outer, size_t len, size_t inner, size_t ddof, void *out, nrk_stream stream);
 nrk_status nrk_dev_reduce_axis_f32(int32_t op, const float *x, size_t outer, size_t len, size_t inner, size_t ddof, void *out, nrk_stream stream);
 
+/* ufunc_ops.rs:535-580 ufunc_accumulate along contiguous rows: y[r][j] = x[r][0] op ... op x[r][j]
+ * for `rows` rows of `len` (axis = None is one row; another axis is moved last by the caller).
+ * op: NRK_BOP_ADD (cumsum, statistics/mod.rs:584, running sum from +0), NRK_BOP_MUL (cumprod),
+ * NRK_BOP_MAXIMUM / NRK_BOP_MINIMUM (NaN-propagating, ufunc_ops.rs:156; exact, so the same bits
+ * as the sequential fold), NRK_BOP_SUB (x0 - x1 - ...), NRK_BOP_DIV (x0 / x1 / ..., as a product of
+ * reciprocals). Sums and products run in a fixed tree set by the shape. */
+nrk_status nrk_accumulate_f64(int32_t op, const double *x, size_t rows, size_t len, double *y);
+nrk_status nrk_accumulate_f32(int32_t op, const float *x, size_t rows, size_t len, float *y);
+nrk_status nrk_dev_accumulate_f64(int32_t op, const double *x, size_t rows, size_t len, double *y, nrk_stream stream);
+nrk_status nrk_dev_accumulate_f32(int32_t op, const float *x, size_t rows, size_t len, float *y, nrk_stream stream);
+
 /* ufuncs.rs:1411-1457 softmax and log_softmax over all n elements (the reference flattens):
  * m = max, e = exp(x - m), S = sum e (fixed tree), then y = e / S or y = (x - m) - ln S. A NaN
  * anywhere makes every output NaN, as in the reference. y may alias x only for n = 0. */

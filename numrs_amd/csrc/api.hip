@@ -235,6 +235,16 @@ nrk_status nrk_zip_f32(int32_t op, const float *a, const float *b, float *y, siz
 nrk_status nrk_dev_zip_f64(int32_t op, const double *a, const double *b, double *y, size_t n, nrk_stream stream) { return zip<double>(op, a, b, y, n, S(stream)); }
 nrk_status nrk_dev_zip_f32(int32_t op, const float *a, const float *b, float *y, size_t n, nrk_stream stream) { return zip<float>(op, a, b, y, n, S(stream)); }
 
+// ---- accumulate (ufunc_ops.rs:535) ----
+nrk_status nrk_accumulate_f64(int32_t op, const double *x, size_t rows, size_t len, double *y) {
+    return host_call<double, double>(x, rows * len, y, rows * len, [&](const double *d, double *o, hipStream_t s) { return accumulate<double>(op, d, o, rows, len, s); });
+}
+nrk_status nrk_accumulate_f32(int32_t op, const float *x, size_t rows, size_t len, float *y) {
+    return host_call<float, float>(x, rows * len, y, rows * len, [&](const float *d, float *o, hipStream_t s) { return accumulate<float>(op, d, o, rows, len, s); });
+}
+nrk_status nrk_dev_accumulate_f64(int32_t op, const double *x, size_t rows, size_t len, double *y, nrk_stream stream) { return accumulate<double>(op, x, y, rows, len, S(stream)); }
+nrk_status nrk_dev_accumulate_f32(int32_t op, const float *x, size_t rows, size_t len, float *y, nrk_stream stream) { return accumulate<float>(op, x, y, rows, len, S(stream)); }
+
 // ---- softmax, log_softmax ----
 nrk_status nrk_softmax_f64(const double *x, double *y, size_t n) {
     return host_call<double, double>(x, n, y, n, [&](const double *d, double *o, hipStream_t s) { return dev_softmax<double>(false, d, o, n, s); });

@@ -33,6 +33,9 @@ template <typename T>
 nrk_status dot(const T *a, const T *b, size_t n, T *out, hipStream_t stream);
 template <typename T>
 nrk_status sqrt_scalar(T *p, hipStream_t stream);
+// inclusive scans along `rows` contiguous rows of `len` (op: NRK_BOP_ADD/SUB/MUL/DIV/MAXIMUM/MINIMUM)
+template <typename T>
+nrk_status accumulate(int op, const T *x, T *y, size_t rows, size_t len, hipStream_t stream);
 // ms[0] = max x (NaN ignored), ms[1] = sum e^(x - max), in one read of x
 template <typename T>
 nrk_status softmax_stats(const T *x, size_t n, T *ms, hipStream_t stream);

@@ -1166,6 +1166,167 @@ nrk_status softmax_stats(const T *x, size_t n, T *ms, hipStream_t stream) {
 template nrk_status softmax_stats<float>(const float *, size_t, float *, hipStream_t);
 template nrk_status softmax_stats<double>(const double *, size_t, double *, hipStream_t);
 
+// ---------------- accumulate (ufunc_ops.rs:535-580): inclusive scans along contiguous rows -------
+// op ADD (cumsum, statistics/mod.rs:584), MUL (cumprod, :795), MAXIMUM / MINIMUM (NaN-propagating
+// UfuncOp::apply, ufunc_ops.rs:156), SUB and DIV (a0 - a1 - ... as ADD over a0, -a1, -a2, ...;
+// a0 / a1 / ... as MUL over a0, 1/a1, ...). Rows are cut into tiles of SC_TILE elements: a tile's
+// aggregate (K1), the rows of aggregates scanned by this same routine (recursively), then each
+// tile rescanned from its carry (K3). Every combination order is fixed by the shape.
+constexpr int SC_PER = 8;
+constexpr int SC_TILE = RB * SC_PER;
+
+template <typename T, int OP>
+__device__ __forceinline__ T acc_apply(T a, T b) {
+    if constexpr (OP == 0) return rn_add(a, b);
+    else if constexpr (OP == 2) return rn_mul(a, b);
+    else if constexpr (OP == 5) return (a != a || b != b) ? a + b : (a > b ? a : b);
+    else return (a != a || b != b) ? a + b : (a < b ? a : b);
+}
+template <typename T, int OP>
+__device__ __forceinline__ T acc_identity() {
+    if constexpr (OP == 0) return (T)0;
+    else if constexpr (OP == 2) return (T)1;
+    else if constexpr (OP == 5) return -(T)__builtin_huge_valf();
+    else return (T)__builtin_huge_valf();
+}
+// the element as the scan sees it: SUB negates and DIV inverts every element after a row's first
+template <typename T, int OP, int XF>
+__device__ __forceinline__ T acc_load(const T *x, size_t pos, size_t col) {
+    const T v = x[pos];
+    if constexpr (XF == 1) return col == 0 ? v : -v;
+    else if constexpr (XF == 2) return col == 0 ? v : rn_div((T)1, v);
+    else if constexpr (OP == 0) return rn_add(v, (T)0);  // -0 -> +0: cumsum's running sum starts at +0
+    else return v;
+}
+
+// Workgroup-wide inclusive scan of one value per thread, in a fixed order: the wavefront's
+// Hillis-Steele steps, then the wavefront totals from LDS.
+template <typename T, int OP>
+__device__ __forceinline__ T block_scan(T v, T *lds) {
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const T u = __shfl_up(v, d, 64);
+        if (lane >= d) v = acc_apply<T, OP>(u, v);
+    }
+    if (lane == 63) lds[w] = v;
+    __syncthreads();
+    T pre = acc_identity<T, OP>();
+    for (int k = 0; k < w; ++k) pre = acc_apply<T, OP>(pre, lds[k]);
+    __syncthreads();
+    return w ? acc_apply<T, OP>(pre, v) : v;
+}
+
+// K1: one aggregate per tile (thread folds its SC_PER consecutive elements, then the block scan's
+// last value).
+template <typename T, int OP, int XF>
+__global__ __launch_bounds__(RB) void scan_tile_agg_kernel(const T *x, size_t rows, size_t len, size_t tpr, T *agg) {
+    __shared__ T lds[WAVES];
+    const size_t tiles = rows * tpr;
+    for (size_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const size_t r = tile / tpr, c0 = (tile - r * tpr) * SC_TILE + (size_t)threadIdx.x * SC_PER;
+        T a = acc_identity<T, OP>();
+#pragma unroll
+        for (int j = 0; j < SC_PER; ++j)
+            if (c0 + j < len) a = acc_apply<T, OP>(a, acc_load<T, OP, XF>(x, r * len + c0 + j, c0 + j));
+        a = block_scan<T, OP>(a, lds);
+        if (threadIdx.x == RB - 1) agg[tile] = a;
+    }
+}
+
+// K3: each tile rescanned: y = carry (x) thread prefix (x) in-thread prefix, where carry is the
+// inclusive scan of the previous tiles' aggregates of the row (none for a row's first tile).
+template <typename T, int OP, int XF>
+__global__ __launch_bounds__(RB) void scan_tile_kernel(const T *x, T *y, size_t rows, size_t len, size_t tpr,
+                                                       const T *agg_scan) {
+    __shared__ T lds[WAVES];
+    const size_t tiles = rows * tpr;
+    for (size_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const size_t r = tile / tpr, tr = tile - r * tpr;
+        const size_t c0 = tr * SC_TILE + (size_t)threadIdx.x * SC_PER;
+        T v[SC_PER];
+#pragma unroll
+        for (int j = 0; j < SC_PER; ++j)
+            v[j] = c0 + j < len ? acc_load<T, OP, XF>(x, r * len + c0 + j, c0 + j) : acc_identity<T, OP>();
+#pragma unroll
+        for (int j = 1; j < SC_PER; ++j) v[j] = acc_apply<T, OP>(v[j - 1], v[j]);
+        const T incl = block_scan<T, OP>(v[SC_PER - 1], lds);
+        // the exclusive prefix: the previous thread's inclusive value; for a wavefront's first
+        // thread that is the fold of the earlier wavefronts' totals, in block_scan's own order
+        T pre = __shfl_up(incl, 1, 64);
+        if ((threadIdx.x & 63) == 0 && threadIdx.x) {
+            T p = acc_identity<T, OP>();
+            for (int k = 0; k < (int)(threadIdx.x >> 6); ++k) p = acc_apply<T, OP>(p, lds[k]);
+            pre = p;
+        }
+        const bool has_carry = tr > 0;
+        const T carry = has_carry ? agg_scan[tile - 1] : acc_identity<T, OP>();
+        const bool first = threadIdx.x == 0;
+#pragma unroll
+        for (int j = 0; j < SC_PER; ++j) {
+            if (c0 + j < len) {
+                T o = v[j];
+                if (!first) o = acc_apply<T, OP>(pre, o);
+                if (has_carry) o = acc_apply<T, OP>(carry, o);
+                y[r * len + c0 + j] = o;
+            }
+        }
+        __syncthreads();  // lds is reused by the next tile
+    }
+}
+
+template <typename T, int OP, int XF>
+static nrk_status scan_rows(const T *x, T *y, size_t rows, size_t len, T *ws, hipStream_t stream) {
+    const size_t tpr = ceil_div(len, (size_t)SC_TILE);
+    const size_t tiles = rows * tpr;
+    const int grid = (int)std::min<size_t>(tiles, (size_t)cu_count() * 16);
+    if (tpr > 1) {
+        T *agg = ws;
+        hipLaunchKernelGGL((scan_tile_agg_kernel<T, OP, XF>), dim3(grid), dim3(RB), 0, stream, x, rows, len, tpr, agg);
+        NRK_HIP(hipGetLastError());
+        // the aggregates' own rows (rows x tpr), scanned in place; their transform is applied already
+        NRK_TRY((scan_rows<T, OP, 0>(agg, agg, rows, tpr, ws + tiles, stream)));
+        hipLaunchKernelGGL((scan_tile_kernel<T, OP, XF>), dim3(grid), dim3(RB), 0, stream, x, y, rows, len, tpr,
+                           (const T *)agg);
+    } else {
+        hipLaunchKernelGGL((scan_tile_kernel<T, OP, XF>), dim3(grid), dim3(RB), 0, stream, x, y, rows, len, tpr,
+                           (const T *)nullptr);
+    }
+    NRK_HIP(hipGetLastError());
+    return NRK_OK;
+}
+
+template <typename T>
+nrk_status accumulate(int op, const T *x, T *y, size_t rows, size_t len, hipStream_t stream) {
+    NRK_TRY(ensure_device());
+    if (op != NRK_BOP_ADD && op != NRK_BOP_SUB && op != NRK_BOP_MUL && op != NRK_BOP_DIV && op != NRK_BOP_MAXIMUM &&
+        op != NRK_BOP_MINIMUM)
+        return fail(NRK_ERR_INVALID_ARGUMENT, "accumulate: op %d is not ADD, SUB, MUL, DIV, MAXIMUM or MINIMUM", op);
+    if (rows * len == 0) return NRK_OK;
+    if (!x || !y) return fail(NRK_ERR_INVALID_ARGUMENT, "x or y is NULL");
+    // workspace: every level's aggregates (each level 1/SC_TILE of the one before, at least one per row)
+    size_t need = 0;
+    for (size_t l = len; l > 1;) {
+        const size_t tpr = ceil_div(l, (size_t)SC_TILE);
+        if (tpr <= 1) break;
+        need += rows * tpr;
+        l = tpr;
+    }
+    void *wsv = nullptr;
+    NRK_TRY(get_workspace(stream, (need + 1) * sizeof(T), &wsv));
+    T *ws = (T *)wsv;
+    switch (op) {
+        case NRK_BOP_ADD: return scan_rows<T, 0, 0>(x, y, rows, len, ws, stream);
+        case NRK_BOP_SUB: return scan_rows<T, 0, 1>(x, y, rows, len, ws, stream);
+        case NRK_BOP_MUL: return scan_rows<T, 2, 0>(x, y, rows, len, ws, stream);
+        case NRK_BOP_DIV: return scan_rows<T, 2, 2>(x, y, rows, len, ws, stream);
+        case NRK_BOP_MAXIMUM: return scan_rows<T, 5, 0>(x, y, rows, len, ws, stream);
+        default: return scan_rows<T, 6, 0>(x, y, rows, len, ws, stream);
+    }
+}
+template nrk_status accumulate<float>(int, const float *, float *, size_t, size_t, hipStream_t);
+template nrk_status accumulate<double>(int, const double *, double *, size_t, size_t, hipStream_t);
+
 // y = sqrt(y) for one device scalar (norm_l2 = sqrt of the sum of squares, std = sqrt of var).
 template <typename T>
 __global__ void sqrt_scalar_kernel(T *p) { *p = rn_sqrt(*p); }

@@ -248,6 +248,37 @@ def lerp(a, b, t: float):
     return y.reshape(a.shape)
 
 
+_ACC_OPS = {"add": "add", "subtract": "sub", "multiply": "mul", "divide": "div", "maximum": "maximum",
+            "minimum": "minimum"}
+
+
+def accumulate(op: str, a, axis=None):
+    """ufunc_accumulate (ufunc_ops.rs:535): op in add / subtract / multiply / divide / maximum /
+    minimum; axis None scans the flattened array, otherwise that axis (moved last for the call)."""
+    a = np.asarray(a)
+    f32 = a.dtype == np.float32
+    dt = np.float32 if f32 else np.float64
+    if axis is None:
+        x = _vec(a, dt)
+        rows, length = 1, x.size
+        back = lambda y: y  # noqa: E731 (flat, as the reference returns it)
+    else:
+        ax = axis % a.ndim
+        moved = np.ascontiguousarray(np.moveaxis(a.astype(dt, copy=False), ax, -1))
+        x = moved.reshape(-1)
+        length = moved.shape[-1] if moved.ndim else 1
+        rows = x.size // length if length else 0
+        back = lambda y: np.moveaxis(y.reshape(moved.shape), -1, ax)  # noqa: E731
+    y = np.empty_like(x)
+    call("nrk_accumulate_f32" if f32 else "nrk_accumulate_f64", C.c_int32(BOP[_ACC_OPS[op]]), _ptr(x),
+         C.c_size_t(rows), C.c_size_t(length), _ptr(y))
+    return back(y)
+
+
+def cumsum(a, axis=None): return accumulate("add", a, axis)          # statistics/mod.rs:584
+def cumprod(a, axis=None): return accumulate("multiply", a, axis)    # statistics/mod.rs:795
+
+
 def _softmax(fn, a):
     a = np.asarray(a)
     f32 = a.dtype == np.float32

@@ -118,3 +118,76 @@ def test_softmax_special_inputs(gpu):
     # 2-D input: the reference flattens, and so does this
     m = np.arange(6.0).reshape(2, 3)
     np.testing.assert_allclose(K.softmax(m).sum(), 1.0, rtol=1e-15)
+
+
+# ---- ufunc_accumulate (ufunc_ops.rs:535-580): scans along contiguous rows
+
+ACC_LENS = [1, 7, 2047, 2048, 2049, 70_001, 5_000_003]
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+@pytest.mark.parametrize("n", ACC_LENS)
+def test_cummax_cummin_bit_exact(gpu, dt, n):
+    """Maximum / minimum are exact and associative: the scan gives the sequential fold's bits,
+    NaN propagating from its position on (UfuncOp::apply, ufunc_ops.rs:156)."""
+    r = np.random.default_rng(n)
+    x = r.normal(size=n).astype(dt)
+    if n > 100:
+        x[n // 2] = np.nan
+    np.testing.assert_array_equal(K.accumulate("maximum", x), np.maximum.accumulate(x))
+    np.testing.assert_array_equal(K.accumulate("minimum", x), np.minimum.accumulate(x))
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+@pytest.mark.parametrize("n", ACC_LENS)
+def test_cumsum_and_subtract_within_tree_bound(gpu, dt, n):
+    """The scan's sums run in a fixed tree (in-thread, wavefront, workgroup, carry levels): each
+    prefix is within 128 eps * (prefix of |x|) of the exact prefix sum."""
+    r = np.random.default_rng(n + 1)
+    x = r.normal(size=n).astype(dt)
+    eps = 2.0**-53 if dt == np.float64 else 2.0**-24
+    exact = np.cumsum(x.astype(LD))
+    bound = 128 * eps * np.cumsum(np.abs(x).astype(LD)) + 1e-300
+    got = K.cumsum(x)
+    assert got.dtype == dt
+    assert np.all(np.abs(got.astype(LD) - exact) <= bound)
+    sub = np.concatenate([[x[0]], x[0] - np.cumsum(x[1:].astype(LD))]) if n > 1 else x.astype(LD)
+    assert np.all(np.abs(K.accumulate("subtract", x).astype(LD) - sub) <= bound)
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_cumprod_and_divide(gpu, dt):
+    """Any bracketing of k products has k - 1 roundings: relative error <= k eps (2k for divide,
+    which multiplies by rounded reciprocals)."""
+    r = np.random.default_rng(5)
+    n = 100_003
+    x = r.uniform(0.999, 1.001, n).astype(dt)
+    eps = 2.0**-53 if dt == np.float64 else 2.0**-24
+    k = np.arange(1, n + 1, dtype=np.float64)
+    exact = np.cumprod(x.astype(LD))
+    assert np.all(np.abs(K.cumprod(x).astype(LD) / exact - 1) <= (k + 1) * eps)
+    quot = x[0] / np.concatenate([[LD(1)], np.cumprod(x[1:].astype(LD))])
+    assert np.all(np.abs(K.accumulate("divide", x).astype(LD) / quot - 1) <= (2 * k + 1) * eps)
+
+
+def test_accumulate_axes_and_edges(gpu):
+    r = np.random.default_rng(6)
+    a = r.normal(size=(3, 4100, 5))
+    for ax in (0, 1, 2, -1):
+        got = K.cumsum(a, axis=ax)
+        assert got.shape == a.shape
+        np.testing.assert_allclose(got, np.cumsum(a, axis=ax), rtol=0, atol=1e-12)
+        np.testing.assert_array_equal(K.accumulate("maximum", a, axis=ax), np.maximum.accumulate(a, axis=ax))
+    np.testing.assert_allclose(K.cumsum(a), np.cumsum(a), rtol=0, atol=1e-11)  # axis None: flattened
+    # the running sum starts at +0, so -0 inputs give +0 (statistics/mod.rs:584)
+    z = K.cumsum(np.array([-0.0, -0.0, 1.0]))
+    assert not np.signbit(z[0]) and not np.signbit(z[1])
+    assert K.cumsum(np.array([], np.float64)).size == 0
+    np.testing.assert_array_equal(K.cumsum(np.array([1.0, 2.0, 3.0, 4.0])), [1.0, 3.0, 6.0, 10.0])
+    np.testing.assert_array_equal(K.accumulate("subtract", np.array([10.0, 1.0, 2.0])), [10.0, 9.0, 7.0])
+    from numrs_amd._lib import call
+    import ctypes as C
+    x = np.ones(4)
+    with pytest.raises(Exception):
+        call("nrk_accumulate_f64", C.c_int32(K.BOP["rem"]), x.ctypes.data_as(C.c_void_p), C.c_size_t(1), C.c_size_t(4),
+             x.ctypes.data_as(C.c_void_p))
