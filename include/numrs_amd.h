@@ -276,6 +276,21 @@ nrk_status nrk_reduce_axis_f32(int32_t op, const float *x, size_t outer, size_t 
 nrk_status nrk_dev_reduce_axis_f64(int32_t op, const double *x, size_t outer, size_t len, size_t inner, size_t ddof, void *out, nrk_stream stream);
 nrk_status nrk_dev_reduce_axis_f32(int32_t op, const float *x, size_t outer, size_t len, size_t inner, size_t ddof, void *out, nrk_stream stream);
 
+/* ufunc_ops.rs:583 ufunc_outer: y[i * nb + j] = a[i] op b[j] (shape a.shape + b.shape), and
+ * ufunc_ops.rs:841 ufunc_where: y[i] = mask[i] ? a[i] op b[i] : a[i] (mask bytes 0 / nonzero).
+ * op: NRK_BOP_ADD, SUB, MUL, DIV, MAXIMUM, MINIMUM, with UfuncOp::apply's maximum / minimum
+ * (NaN if either operand is NaN, ufunc_ops.rs:156). One rounding per element. */
+nrk_status nrk_outer_f64(int32_t op, const double *a, size_t na, const double *b, size_t nb, double *y);
+nrk_status nrk_outer_f32(int32_t op, const float *a, size_t na, const float *b, size_t nb, float *y);
+nrk_status nrk_dev_outer_f64(int32_t op, const double *a, size_t na, const double *b, size_t nb, double *y, nrk_stream stream);
+nrk_status nrk_dev_outer_f32(int32_t op, const float *a, size_t na, const float *b, size_t nb, float *y, nrk_stream stream);
+nrk_status nrk_where_f64(int32_t op, const double *a, const double *b, const uint8_t *mask, double *y, size_t n);
+nrk_status nrk_where_f32(int32_t op, const float *a, const float *b, const uint8_t *mask, float *y, size_t n);
+nrk_status nrk_dev_where_f64(int32_t op, const double *a, const double *b, const uint8_t *mask, double *y, size_t n,
+                             nrk_stream stream);
+nrk_status nrk_dev_where_f32(int32_t op, const float *a, const float *b, const uint8_t *mask, float *y, size_t n,
+                             nrk_stream stream);
+
 /* ufunc_ops.rs:535-580 ufunc_accumulate along contiguous rows: y[r][j] = x[r][0] op ... op x[r][j]
  * for `rows` rows of `len` (axis = None is one row; another axis is moved last by the caller).
  * op: NRK_BOP_ADD (cumsum, statistics/mod.rs:584, running sum from +0), NRK_BOP_MUL (cumprod),

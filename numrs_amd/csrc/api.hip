@@ -213,6 +213,28 @@ nrk_status dev_map_reduce(const nrk_scalar_op *chain, int32_t chain_len, const T
     return seg_reduce<T>(c, x, rows, cols, (double)ddof, row_out, full_out, s);
 }
 
+template <typename T>
+nrk_status host_outer(int32_t op, const T *a, size_t na, const T *b, size_t nb, T *y) {
+    NRK_TRY(ensure_device());
+    if (nb && !b) return fail(NRK_ERR_INVALID_ARGUMENT, "b is NULL");
+    void *db = nullptr;
+    NRK_TRY(staging(2, nb * sizeof(T) + 16, &db));
+    if (nb) NRK_HIP(hipMemcpyAsync(db, b, nb * sizeof(T), hipMemcpyHostToDevice, nullptr));
+    return host_call<T, T>(a, na, y, na * nb, [&](const T *d, T *o, hipStream_t s) { return outer<T>(op, d, na, (const T *)db, nb, o, s); });
+}
+template <typename T>
+nrk_status host_where(int32_t op, const T *a, const T *b, const uint8_t *mask, T *y, size_t n) {
+    NRK_TRY(ensure_device());
+    if (n && (!b || !mask)) return fail(NRK_ERR_INVALID_ARGUMENT, "b or mask is NULL");
+    void *db = nullptr, *dm = nullptr;
+    NRK_TRY(staging(2, n * sizeof(T) + 16, &db));
+    NRK_TRY(staging(3, n + 16, &dm));
+    if (n) {
+        NRK_HIP(hipMemcpyAsync(db, b, n * sizeof(T), hipMemcpyHostToDevice, nullptr));
+        NRK_HIP(hipMemcpyAsync(dm, mask, n, hipMemcpyHostToDevice, nullptr));
+    }
+    return host_call<T, T>(a, n, y, n, [&](const T *d, T *o, hipStream_t s) { return where_op<T>(op, d, (const T *)db, (const uint8_t *)dm, o, n, s); });
+}
 }  // namespace
 
 extern "C" {
@@ -234,6 +256,16 @@ nrk_status nrk_zip_f64(int32_t op, const double *a, const double *b, double *y, 
 nrk_status nrk_zip_f32(int32_t op, const float *a, const float *b, float *y, size_t n) { return host_zip<float>(op, a, b, y, n); }
 nrk_status nrk_dev_zip_f64(int32_t op, const double *a, const double *b, double *y, size_t n, nrk_stream stream) { return zip<double>(op, a, b, y, n, S(stream)); }
 nrk_status nrk_dev_zip_f32(int32_t op, const float *a, const float *b, float *y, size_t n, nrk_stream stream) { return zip<float>(op, a, b, y, n, S(stream)); }
+
+// ---- outer, where (ufunc_ops.rs:583, 841) ----
+nrk_status nrk_outer_f64(int32_t op, const double *a, size_t na, const double *b, size_t nb, double *y) { return host_outer<double>(op, a, na, b, nb, y); }
+nrk_status nrk_outer_f32(int32_t op, const float *a, size_t na, const float *b, size_t nb, float *y) { return host_outer<float>(op, a, na, b, nb, y); }
+nrk_status nrk_dev_outer_f64(int32_t op, const double *a, size_t na, const double *b, size_t nb, double *y, nrk_stream stream) { return outer<double>(op, a, na, b, nb, y, S(stream)); }
+nrk_status nrk_dev_outer_f32(int32_t op, const float *a, size_t na, const float *b, size_t nb, float *y, nrk_stream stream) { return outer<float>(op, a, na, b, nb, y, S(stream)); }
+nrk_status nrk_where_f64(int32_t op, const double *a, const double *b, const uint8_t *mask, double *y, size_t n) { return host_where<double>(op, a, b, mask, y, n); }
+nrk_status nrk_where_f32(int32_t op, const float *a, const float *b, const uint8_t *mask, float *y, size_t n) { return host_where<float>(op, a, b, mask, y, n); }
+nrk_status nrk_dev_where_f64(int32_t op, const double *a, const double *b, const uint8_t *mask, double *y, size_t n, nrk_stream stream) { return where_op<double>(op, a, b, mask, y, n, S(stream)); }
+nrk_status nrk_dev_where_f32(int32_t op, const float *a, const float *b, const uint8_t *mask, float *y, size_t n, nrk_stream stream) { return where_op<float>(op, a, b, mask, y, n, S(stream)); }
 
 // ---- accumulate (ufunc_ops.rs:535) ----
 nrk_status nrk_accumulate_f64(int32_t op, const double *x, size_t rows, size_t len, double *y) {

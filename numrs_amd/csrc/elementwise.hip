@@ -273,6 +273,82 @@ nrk_status tri(int op, const T *a, const T *b, const T *c, T t, T *y, size_t n, 
 template nrk_status tri<float>(int, const float *, const float *, const float *, float, float *, size_t, hipStream_t);
 template nrk_status tri<double>(int, const double *, const double *, const double *, double, double *, size_t, hipStream_t);
 
+// ---- ufunc_ops.rs:583 ufunc_outer and :841 ufunc_where with UfuncOp::apply (:156): add, subtract,
+// multiply, divide, and maximum / minimum that return NaN when either operand is NaN.
+template <typename T, int OP>
+__device__ __forceinline__ T ufop_apply(T a, T b) {
+    if constexpr (OP == NRK_BOP_ADD) return rn_add(a, b);
+    else if constexpr (OP == NRK_BOP_SUB) return rn_sub(a, b);
+    else if constexpr (OP == NRK_BOP_MUL) return rn_mul(a, b);
+    else if constexpr (OP == NRK_BOP_DIV) return rn_div(a, b);
+    else if constexpr (OP == NRK_BOP_MAXIMUM) return (a != a || b != b) ? a + b : (a > b ? a : b);
+    else return (a != a || b != b) ? a + b : (a < b ? a : b);
+}
+
+// y[i * nb + j] = a[i] op b[j]: row i per blockIdx.y step, columns coalesced across the x blocks
+template <typename T, int OP>
+__global__ __launch_bounds__(EB) void outer_kernel(const T *a, size_t na, const T *b, size_t nb, T *y) {
+    for (size_t i = blockIdx.y; i < na; i += gridDim.y) {
+        const T ai = a[i];
+        for (size_t j = blockIdx.x * (size_t)EB + threadIdx.x; j < nb; j += (size_t)gridDim.x * EB)
+            y[i * nb + j] = ufop_apply<T, OP>(ai, b[j]);
+    }
+}
+
+template <typename T, int OP>
+__global__ __launch_bounds__(EB) void where_kernel(const T *a, const T *b, const uint8_t *mask, T *y, size_t n) {
+    const size_t nthreads = (size_t)gridDim.x * EB;
+    for (size_t i = blockIdx.x * (size_t)EB + threadIdx.x; i < n; i += nthreads) {
+        const T av = a[i];
+        y[i] = mask[i] ? ufop_apply<T, OP>(av, b[i]) : av;
+    }
+}
+
+static bool ufunc_op_ok(int op) {
+    return op == NRK_BOP_ADD || op == NRK_BOP_SUB || op == NRK_BOP_MUL || op == NRK_BOP_DIV || op == NRK_BOP_MAXIMUM ||
+           op == NRK_BOP_MINIMUM;
+}
+
+template <typename T>
+nrk_status outer(int op, const T *a, size_t na, const T *b, size_t nb, T *y, hipStream_t stream) {
+    NRK_TRY(ensure_device());
+    if (!ufunc_op_ok(op)) return fail(NRK_ERR_INVALID_ARGUMENT, "outer: op %d is not ADD, SUB, MUL, DIV, MAXIMUM or MINIMUM", op);
+    if (na == 0 || nb == 0) return NRK_OK;
+    if (!a || !b || !y) return fail(NRK_ERR_INVALID_ARGUMENT, "a, b or y is NULL");
+    const dim3 grid((unsigned)std::min<size_t>(ceil_div(nb, (size_t)EB), 64), (unsigned)std::min<size_t>(na, 8192));
+    switch (op) {
+#define NRK_OUTER_CASE(OP) \
+    case OP: hipLaunchKernelGGL((outer_kernel<T, OP>), grid, dim3(EB), 0, stream, a, na, b, nb, y); break;
+        NRK_OUTER_CASE(NRK_BOP_ADD) NRK_OUTER_CASE(NRK_BOP_SUB) NRK_OUTER_CASE(NRK_BOP_MUL) NRK_OUTER_CASE(NRK_BOP_DIV)
+        NRK_OUTER_CASE(NRK_BOP_MAXIMUM) NRK_OUTER_CASE(NRK_BOP_MINIMUM)
+#undef NRK_OUTER_CASE
+    }
+    NRK_HIP(hipGetLastError());
+    return NRK_OK;
+}
+template nrk_status outer<float>(int, const float *, size_t, const float *, size_t, float *, hipStream_t);
+template nrk_status outer<double>(int, const double *, size_t, const double *, size_t, double *, hipStream_t);
+
+template <typename T>
+nrk_status where_op(int op, const T *a, const T *b, const uint8_t *mask, T *y, size_t n, hipStream_t stream) {
+    NRK_TRY(ensure_device());
+    if (!ufunc_op_ok(op)) return fail(NRK_ERR_INVALID_ARGUMENT, "where: op %d is not ADD, SUB, MUL, DIV, MAXIMUM or MINIMUM", op);
+    if (n == 0) return NRK_OK;
+    if (!a || !b || !mask || !y) return fail(NRK_ERR_INVALID_ARGUMENT, "a, b, mask or y is NULL");
+    const dim3 grid((unsigned)std::min<size_t>(ceil_div(n, (size_t)EB), (size_t)cu_count() * 32));
+    switch (op) {
+#define NRK_WHERE_CASE(OP) \
+    case OP: hipLaunchKernelGGL((where_kernel<T, OP>), grid, dim3(EB), 0, stream, a, b, mask, y, n); break;
+        NRK_WHERE_CASE(NRK_BOP_ADD) NRK_WHERE_CASE(NRK_BOP_SUB) NRK_WHERE_CASE(NRK_BOP_MUL) NRK_WHERE_CASE(NRK_BOP_DIV)
+        NRK_WHERE_CASE(NRK_BOP_MAXIMUM) NRK_WHERE_CASE(NRK_BOP_MINIMUM)
+#undef NRK_WHERE_CASE
+    }
+    NRK_HIP(hipGetLastError());
+    return NRK_OK;
+}
+template nrk_status where_op<float>(int, const float *, const float *, const uint8_t *, float *, size_t, hipStream_t);
+template nrk_status where_op<double>(int, const double *, const double *, const uint8_t *, double *, size_t, hipStream_t);
+
 // ---- softmax / log_softmax passes (ufuncs.rs:1411-1457). ms = {max, sum} on the device.
 // mode 0: y = exp(x - max); mode 1: y = y / sum; mode 2: y = (x - max) - ln(sum);
 // mode 3: y = exp(x - max) / sum (the reference's per-element expression, one pass).

@@ -44,6 +44,11 @@ template <typename T>
 nrk_status map(const Chain &chain, const T *x, T *y, size_t n, hipStream_t stream);
 template <typename T>
 nrk_status zip(int op, const T *a, const T *b, T *y, size_t n, hipStream_t stream);
+// ufunc_outer / ufunc_where (op: NRK_BOP_ADD/SUB/MUL/DIV/MAXIMUM/MINIMUM, NaN-propagating max/min)
+template <typename T>
+nrk_status outer(int op, const T *a, size_t na, const T *b, size_t nb, T *y, hipStream_t stream);
+template <typename T>
+nrk_status where_op(int op, const T *a, const T *b, const uint8_t *mask, T *y, size_t n, hipStream_t stream);
 // softmax passes: mode 0 y = exp(x - ms[0]); 1 y = y / ms[1]; 2 y = (x - ms[0]) - ln(ms[1])
 template <typename T>
 nrk_status softmax_pass(int mode, const T *x, T *y, const T *ms, size_t n, hipStream_t stream);

@@ -275,6 +275,41 @@ def accumulate(op: str, a, axis=None):
     return back(y)
 
 
+def _ufop_dtype(*arrs):
+    f32 = all(np.asarray(x).dtype == np.float32 for x in arrs)
+    return (np.float32, "f32") if f32 else (np.float64, "f64")
+
+
+def outer(op: str, a, b):
+    """ufunc_outer (ufunc_ops.rs:583): y[i, j] = a[i] op b[j], shape a.shape + b.shape."""
+    a, b = np.asarray(a), np.asarray(b)
+    dt, suf = _ufop_dtype(a, b)
+    av, bv = _vec(a, dt), _vec(b, dt)
+    y = np.empty(av.size * bv.size, dt)
+    call(f"nrk_outer_{suf}", C.c_int32(BOP[_ACC_OPS[op]]), _ptr(av), C.c_size_t(av.size), _ptr(bv),
+         C.c_size_t(bv.size), _ptr(y))
+    return y.reshape(a.shape + b.shape)
+
+
+def where(op: str, a, b, mask):
+    """ufunc_where (ufunc_ops.rs:841): a op b where mask, else a; shapes must match."""
+    a, b, m = np.asarray(a), np.asarray(b), np.asarray(mask)
+    if a.shape != b.shape or a.shape != m.shape:
+        raise ValueError(f"shape mismatch: {a.shape}, {b.shape}, {m.shape}")
+    dt, suf = _ufop_dtype(a, b)
+    av, bv = _vec(a, dt), _vec(b, dt)
+    mv = np.ascontiguousarray(m.reshape(-1), dtype=np.uint8)
+    y = np.empty_like(av)
+    call(f"nrk_where_{suf}", C.c_int32(BOP[_ACC_OPS[op]]), _ptr(av), _ptr(bv), _ptr(mv), _ptr(y), C.c_size_t(av.size))
+    return y.reshape(a.shape)
+
+
+def add_where(a, b, mask): return where("add", a, b, mask)            # ufunc_ops.rs:893
+def subtract_where(a, b, mask): return where("subtract", a, b, mask)  # :902
+def multiply_where(a, b, mask): return where("multiply", a, b, mask)  # :911
+def divide_where(a, b, mask): return where("divide", a, b, mask)      # :920
+
+
 def cumsum(a, axis=None): return accumulate("add", a, axis)          # statistics/mod.rs:584
 def cumprod(a, axis=None): return accumulate("multiply", a, axis)    # statistics/mod.rs:795
 

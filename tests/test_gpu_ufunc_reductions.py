@@ -191,3 +191,38 @@ def test_accumulate_axes_and_edges(gpu):
     with pytest.raises(Exception):
         call("nrk_accumulate_f64", C.c_int32(K.BOP["rem"]), x.ctypes.data_as(C.c_void_p), C.c_size_t(1), C.c_size_t(4),
              x.ctypes.data_as(C.c_void_p))
+
+
+# ---- ufunc_outer / ufunc_where (ufunc_ops.rs:583, 841)
+
+NP_UFUNC = {"add": np.add, "subtract": np.subtract, "multiply": np.multiply, "divide": np.divide,
+            "maximum": np.maximum, "minimum": np.minimum}  # numpy's maximum / minimum propagate NaN too
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+@pytest.mark.parametrize("op", list(NP_UFUNC))
+def test_outer_and_where_bit_exact(gpu, dt, op):
+    r = np.random.default_rng(11)
+    a = r.normal(size=(37, 3)).astype(dt)
+    b = r.normal(size=3001).astype(dt)
+    a[5, 1] = np.nan
+    b[17] = np.nan
+    b[18] = 0.0
+    with np.errstate(all="ignore"):
+        want = NP_UFUNC[op].outer(a, b)
+    got = K.outer(op, a, b)
+    assert got.shape == a.shape + b.shape and got.dtype == dt
+    np.testing.assert_array_equal(got, want)
+    x, y = r.normal(size=100_003).astype(dt), r.normal(size=100_003).astype(dt)
+    x[::101] = np.nan
+    m = r.random(100_003) < 0.5
+    with np.errstate(all="ignore"):
+        np.testing.assert_array_equal(K.where(op, x, y, m), np.where(m, NP_UFUNC[op](x, y), x))
+
+
+def test_outer_where_edges(gpu):
+    assert K.outer("add", np.array([], np.float64), np.ones(3)).shape == (0, 3)
+    np.testing.assert_array_equal(K.add_where(np.array([1.0, 2.0]), np.array([10.0, 20.0]), np.array([True, False])),
+                                  [11.0, 2.0])
+    with pytest.raises(ValueError):
+        K.where("add", np.ones(3), np.ones(3), np.ones(2, bool))
