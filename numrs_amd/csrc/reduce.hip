@@ -1217,19 +1217,40 @@ __device__ __forceinline__ T block_scan(T v, T *lds) {
     return w ? acc_apply<T, OP>(pre, v) : v;
 }
 
+// A tile moves between HBM and the threads through LDS: the global loads and stores are
+// coalesced (lane l takes element j * RB + l), and thread t reads or writes its SC_PER
+// consecutive elements in LDS. The padded index (one slot per SC_PER elements) keeps those
+// per-thread accesses on distinct banks.
+__device__ __forceinline__ int sc_pad(int i) { return i + i / SC_PER; }
+constexpr int SC_LDS = SC_TILE + SC_TILE / SC_PER;
+
+// Loads tile `tr` of row r into buf (identity past the row's end); thread t then owns
+// buf[sc_pad(t * SC_PER + j)].
+template <typename T, int OP, int XF>
+__device__ __forceinline__ void sc_stage_in(const T *x, size_t r, size_t len, size_t tr, T *buf) {
+#pragma unroll
+    for (int j = 0; j < SC_PER; ++j) {
+        const int i = j * RB + threadIdx.x;
+        const size_t col = tr * SC_TILE + i;
+        buf[sc_pad(i)] = col < len ? acc_load<T, OP, XF>(x, r * len + col, col) : acc_identity<T, OP>();
+    }
+    __syncthreads();
+}
+
 // K1: one aggregate per tile (thread folds its SC_PER consecutive elements, then the block scan's
 // last value).
 template <typename T, int OP, int XF>
 __global__ __launch_bounds__(RB) void scan_tile_agg_kernel(const T *x, size_t rows, size_t len, size_t tpr, T *agg) {
     __shared__ T lds[WAVES];
+    __shared__ T buf[SC_LDS];
     const size_t tiles = rows * tpr;
     for (size_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
-        const size_t r = tile / tpr, c0 = (tile - r * tpr) * SC_TILE + (size_t)threadIdx.x * SC_PER;
+        const size_t r = tile / tpr, tr = tile - r * tpr;
+        sc_stage_in<T, OP, XF>(x, r, len, tr, buf);
         T a = acc_identity<T, OP>();
 #pragma unroll
-        for (int j = 0; j < SC_PER; ++j)
-            if (c0 + j < len) a = acc_apply<T, OP>(a, acc_load<T, OP, XF>(x, r * len + c0 + j, c0 + j));
-        a = block_scan<T, OP>(a, lds);
+        for (int j = 0; j < SC_PER; ++j) a = acc_apply<T, OP>(a, buf[sc_pad(threadIdx.x * SC_PER + j)]);
+        a = block_scan<T, OP>(a, lds);  // its barriers also retire this tile's reads of buf
         if (threadIdx.x == RB - 1) agg[tile] = a;
     }
 }
@@ -1240,14 +1261,14 @@ template <typename T, int OP, int XF>
 __global__ __launch_bounds__(RB) void scan_tile_kernel(const T *x, T *y, size_t rows, size_t len, size_t tpr,
                                                        const T *agg_scan) {
     __shared__ T lds[WAVES];
+    __shared__ T buf[SC_LDS];
     const size_t tiles = rows * tpr;
     for (size_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
         const size_t r = tile / tpr, tr = tile - r * tpr;
-        const size_t c0 = tr * SC_TILE + (size_t)threadIdx.x * SC_PER;
+        sc_stage_in<T, OP, XF>(x, r, len, tr, buf);
         T v[SC_PER];
 #pragma unroll
-        for (int j = 0; j < SC_PER; ++j)
-            v[j] = c0 + j < len ? acc_load<T, OP, XF>(x, r * len + c0 + j, c0 + j) : acc_identity<T, OP>();
+        for (int j = 0; j < SC_PER; ++j) v[j] = buf[sc_pad(threadIdx.x * SC_PER + j)];
 #pragma unroll
         for (int j = 1; j < SC_PER; ++j) v[j] = acc_apply<T, OP>(v[j - 1], v[j]);
         const T incl = block_scan<T, OP>(v[SC_PER - 1], lds);
@@ -1262,16 +1283,22 @@ __global__ __launch_bounds__(RB) void scan_tile_kernel(const T *x, T *y, size_t 
         const bool has_carry = tr > 0;
         const T carry = has_carry ? agg_scan[tile - 1] : acc_identity<T, OP>();
         const bool first = threadIdx.x == 0;
+        // block_scan's barriers have retired every read of buf: the results go back through it
 #pragma unroll
         for (int j = 0; j < SC_PER; ++j) {
-            if (c0 + j < len) {
-                T o = v[j];
-                if (!first) o = acc_apply<T, OP>(pre, o);
-                if (has_carry) o = acc_apply<T, OP>(carry, o);
-                y[r * len + c0 + j] = o;
-            }
+            T o = v[j];
+            if (!first) o = acc_apply<T, OP>(pre, o);
+            if (has_carry) o = acc_apply<T, OP>(carry, o);
+            buf[sc_pad(threadIdx.x * SC_PER + j)] = o;
         }
-        __syncthreads();  // lds is reused by the next tile
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < SC_PER; ++j) {
+            const int i = j * RB + threadIdx.x;
+            const size_t col = tr * SC_TILE + i;
+            if (col < len) y[r * len + col] = buf[sc_pad(i)];
+        }
+        __syncthreads();  // lds and buf are reused by the next tile
     }
 }
 
