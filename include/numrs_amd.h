@@ -291,6 +291,17 @@ nrk_status nrk_dev_where_f64(int32_t op, const double *a, const double *b, const
 nrk_status nrk_dev_where_f32(int32_t op, const float *a, const float *b, const uint8_t *mask, float *y, size_t n,
                              nrk_stream stream);
 
+/* ufunc_ops.rs:619-733 ufunc_reduceat along contiguous rows: y[r][k] reduces x[r][j .. end) with
+ * j = indices[k] and end = indices[k + 1] (len for the last k), or is x[r][j] when end <= j. op as
+ * for accumulate below. Host form: an index >= len is NRK_ERR_INVALID_ARGUMENT (the reference's
+ * IndexOutOfBounds); device form (indices on the device): such an output is NaN. */
+nrk_status nrk_reduceat_f64(int32_t op, const double *x, size_t rows, size_t len, const uint64_t *indices, size_t nidx, double *y);
+nrk_status nrk_reduceat_f32(int32_t op, const float *x, size_t rows, size_t len, const uint64_t *indices, size_t nidx, float *y);
+nrk_status nrk_dev_reduceat_f64(int32_t op, const double *x, size_t rows, size_t len, const uint64_t *indices, size_t nidx,
+                                double *y, nrk_stream stream);
+nrk_status nrk_dev_reduceat_f32(int32_t op, const float *x, size_t rows, size_t len, const uint64_t *indices, size_t nidx,
+                                float *y, nrk_stream stream);
+
 /* ufunc_ops.rs:535-580 ufunc_accumulate along contiguous rows: y[r][j] = x[r][0] op ... op x[r][j]
  * for `rows` rows of `len` (axis = None is one row; another axis is moved last by the caller).
  * op: NRK_BOP_ADD (cumsum, statistics/mod.rs:584, running sum from +0), NRK_BOP_MUL (cumprod),

@@ -235,6 +235,20 @@ nrk_status host_where(int32_t op, const T *a, const T *b, const uint8_t *mask, T
     }
     return host_call<T, T>(a, n, y, n, [&](const T *d, T *o, hipStream_t s) { return where_op<T>(op, d, (const T *)db, (const uint8_t *)dm, o, n, s); });
 }
+template <typename T>
+nrk_status host_reduceat(int32_t op, const T *x, size_t rows, size_t len, const uint64_t *idx, size_t nidx, T *y) {
+    NRK_TRY(ensure_device());
+    if (nidx && !idx) return fail(NRK_ERR_INVALID_ARGUMENT, "indices is NULL");
+    for (size_t k = 0; k < nidx; ++k)  // ufunc_ops.rs:627-633
+        if (idx[k] >= len) return fail(NRK_ERR_INVALID_ARGUMENT, "index %llu out of bounds for reduceat axis of size %zu", (unsigned long long)idx[k], len);
+    void *di = nullptr;
+    NRK_TRY(staging(2, nidx * sizeof(uint64_t) + 16, &di));
+    if (nidx) NRK_HIP(hipMemcpyAsync(di, idx, nidx * sizeof(uint64_t), hipMemcpyHostToDevice, nullptr));
+    return host_call<T, T>(x, rows * len, y, rows * nidx, [&](const T *d, T *o, hipStream_t s) {
+        return reduceat<T>(op, d, rows, len, (const uint64_t *)di, nidx, o, s);
+    });
+}
+
 }  // namespace
 
 extern "C" {
@@ -266,6 +280,12 @@ nrk_status nrk_where_f64(int32_t op, const double *a, const double *b, const uin
 nrk_status nrk_where_f32(int32_t op, const float *a, const float *b, const uint8_t *mask, float *y, size_t n) { return host_where<float>(op, a, b, mask, y, n); }
 nrk_status nrk_dev_where_f64(int32_t op, const double *a, const double *b, const uint8_t *mask, double *y, size_t n, nrk_stream stream) { return where_op<double>(op, a, b, mask, y, n, S(stream)); }
 nrk_status nrk_dev_where_f32(int32_t op, const float *a, const float *b, const uint8_t *mask, float *y, size_t n, nrk_stream stream) { return where_op<float>(op, a, b, mask, y, n, S(stream)); }
+
+// ---- reduceat (ufunc_ops.rs:713) ----
+nrk_status nrk_reduceat_f64(int32_t op, const double *x, size_t rows, size_t len, const uint64_t *indices, size_t nidx, double *y) { return host_reduceat<double>(op, x, rows, len, indices, nidx, y); }
+nrk_status nrk_reduceat_f32(int32_t op, const float *x, size_t rows, size_t len, const uint64_t *indices, size_t nidx, float *y) { return host_reduceat<float>(op, x, rows, len, indices, nidx, y); }
+nrk_status nrk_dev_reduceat_f64(int32_t op, const double *x, size_t rows, size_t len, const uint64_t *indices, size_t nidx, double *y, nrk_stream stream) { return reduceat<double>(op, x, rows, len, indices, nidx, y, S(stream)); }
+nrk_status nrk_dev_reduceat_f32(int32_t op, const float *x, size_t rows, size_t len, const uint64_t *indices, size_t nidx, float *y, nrk_stream stream) { return reduceat<float>(op, x, rows, len, indices, nidx, y, S(stream)); }
 
 // ---- accumulate (ufunc_ops.rs:535) ----
 nrk_status nrk_accumulate_f64(int32_t op, const double *x, size_t rows, size_t len, double *y) {

@@ -36,6 +36,9 @@ nrk_status sqrt_scalar(T *p, hipStream_t stream);
 // inclusive scans along `rows` contiguous rows of `len` (op: NRK_BOP_ADD/SUB/MUL/DIV/MAXIMUM/MINIMUM)
 template <typename T>
 nrk_status accumulate(int op, const T *x, T *y, size_t rows, size_t len, hipStream_t stream);
+// segmented reductions at indices along contiguous rows (ufunc_reduceat's segment rule)
+template <typename T>
+nrk_status reduceat(int op, const T *x, size_t rows, size_t len, const uint64_t *idx, size_t nidx, T *y, hipStream_t stream);
 // ms[0] = max x (NaN ignored), ms[1] = sum e^(x - max), in one read of x
 template <typename T>
 nrk_status softmax_stats(const T *x, size_t n, T *ms, hipStream_t stream);

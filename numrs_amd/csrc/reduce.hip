@@ -1354,6 +1354,79 @@ nrk_status accumulate(int op, const T *x, T *y, size_t rows, size_t len, hipStre
 template nrk_status accumulate<float>(int, const float *, float *, size_t, size_t, hipStream_t);
 template nrk_status accumulate<double>(int, const double *, double *, size_t, size_t, hipStream_t);
 
+// ---------------- reduceat (ufunc_ops.rs:619-684) along contiguous rows ----------------
+// Output k of row r reduces x[r][j .. end) with j = idx[k], end = idx[k + 1] (len for the last
+// k), and just x[r][j] when end <= j. One workgroup per (row, segment): thread t folds elements
+// j + t, j + t + RB, ... in order, then the wavefront butterfly and the wavefront totals in
+// order. SUB / DIV fold x[j], -x[k] / 1/x[k] (k > j) with ADD / MUL. ADD starts from -0, the
+// exact additive identity, so a segment of zeros keeps the reference's sign. An index >= len
+// (which the host entry point rejects) yields NaN, never an out-of-bounds read.
+template <typename T, int OP>
+__device__ __forceinline__ T rat_identity() {
+    if constexpr (OP == 0) return -(T)0;
+    else return acc_identity<T, OP>();
+}
+
+template <typename T, int OP, int XF>
+__global__ __launch_bounds__(RB) void reduceat_kernel(const T *x, size_t rows, size_t len, const uint64_t *idx, size_t nidx,
+                                                      T *y) {
+    __shared__ T lds[WAVES];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    for (size_t item = blockIdx.x; item < rows * nidx; item += gridDim.x) {
+        const size_t r = item / nidx, k = item - r * nidx;
+        const uint64_t j = idx[k];
+        const uint64_t nxt = k + 1 < nidx ? idx[k + 1] : (uint64_t)len;
+        const uint64_t e = nxt > j ? (nxt < len ? nxt : (uint64_t)len) : j + 1;
+        T a = rat_identity<T, OP>();
+        if (j < len) {
+            for (uint64_t c = j + t; c < e; c += RB) {
+                T v = x[r * len + c];
+                if constexpr (XF == 1) v = c == j ? v : -v;
+                else if constexpr (XF == 2) v = c == j ? v : rn_div((T)1, v);
+                a = acc_apply<T, OP>(a, v);
+            }
+        }
+#pragma unroll
+        for (int m = 1; m < 64; m <<= 1) {
+            const T o = shfl_xor(a, m);
+            a = (lane & m) ? acc_apply<T, OP>(o, a) : acc_apply<T, OP>(a, o);  // lower lane's value first
+        }
+        if (lane == 0) lds[w] = a;
+        __syncthreads();
+        if (t == 0) {
+            T s = lds[0];
+            for (int q = 1; q < WAVES; ++q) s = acc_apply<T, OP>(s, lds[q]);
+            y[item] = j < len ? s : (T)__builtin_nanf("");
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T>
+nrk_status reduceat(int op, const T *x, size_t rows, size_t len, const uint64_t *idx, size_t nidx, T *y, hipStream_t stream) {
+    NRK_TRY(ensure_device());
+    if (op != NRK_BOP_ADD && op != NRK_BOP_SUB && op != NRK_BOP_MUL && op != NRK_BOP_DIV && op != NRK_BOP_MAXIMUM &&
+        op != NRK_BOP_MINIMUM)
+        return fail(NRK_ERR_INVALID_ARGUMENT, "reduceat: op %d is not ADD, SUB, MUL, DIV, MAXIMUM or MINIMUM", op);
+    if (rows * nidx == 0) return NRK_OK;
+    if (!x || !idx || !y) return fail(NRK_ERR_INVALID_ARGUMENT, "x, indices or y is NULL");
+    const int grid = (int)std::min<size_t>(rows * nidx, (size_t)cu_count() * 16);
+#define NRK_RAT(OPC, XFC) hipLaunchKernelGGL((reduceat_kernel<T, OPC, XFC>), dim3(grid), dim3(RB), 0, stream, x, rows, len, idx, nidx, y)
+    switch (op) {
+        case NRK_BOP_ADD: NRK_RAT(0, 0); break;
+        case NRK_BOP_SUB: NRK_RAT(0, 1); break;
+        case NRK_BOP_MUL: NRK_RAT(2, 0); break;
+        case NRK_BOP_DIV: NRK_RAT(2, 2); break;
+        case NRK_BOP_MAXIMUM: NRK_RAT(5, 0); break;
+        default: NRK_RAT(6, 0); break;
+    }
+#undef NRK_RAT
+    NRK_HIP(hipGetLastError());
+    return NRK_OK;
+}
+template nrk_status reduceat<float>(int, const float *, size_t, size_t, const uint64_t *, size_t, float *, hipStream_t);
+template nrk_status reduceat<double>(int, const double *, size_t, size_t, const uint64_t *, size_t, double *, hipStream_t);
+
 // y = sqrt(y) for one device scalar (norm_l2 = sqrt of the sum of squares, std = sqrt of var).
 template <typename T>
 __global__ void sqrt_scalar_kernel(T *p) { *p = rn_sqrt(*p); }

@@ -310,6 +310,28 @@ def multiply_where(a, b, mask): return where("multiply", a, b, mask)  # :911
 def divide_where(a, b, mask): return where("divide", a, b, mask)      # :920
 
 
+def reduceat(op: str, a, indices, axis=None):
+    """ufunc_reduceat (ufunc_ops.rs:713): numpy's op.reduceat segment rule along axis (None: the
+    flattened array); an index out of bounds raises."""
+    a = np.asarray(a)
+    f32 = a.dtype == np.float32
+    dt = np.float32 if f32 else np.float64
+    idx = np.ascontiguousarray(np.asarray(indices, dtype=np.uint64).reshape(-1))
+    if axis is None:
+        x, rows, length = _vec(a, dt), 1, a.size
+        back = lambda y: y  # noqa: E731
+    else:
+        ax = axis % a.ndim
+        moved = np.ascontiguousarray(np.moveaxis(a.astype(dt, copy=False), ax, -1))
+        x, length = moved.reshape(-1), moved.shape[-1]
+        rows = x.size // length if length else 0
+        back = lambda y: np.moveaxis(y.reshape(moved.shape[:-1] + (idx.size,)), -1, ax)  # noqa: E731
+    y = np.empty(rows * idx.size, dt)
+    call("nrk_reduceat_f32" if f32 else "nrk_reduceat_f64", C.c_int32(BOP[_ACC_OPS[op]]), _ptr(x), C.c_size_t(rows),
+         C.c_size_t(length), _ptr(idx), C.c_size_t(idx.size), _ptr(y))
+    return back(y)
+
+
 def cumsum(a, axis=None): return accumulate("add", a, axis)          # statistics/mod.rs:584
 def cumprod(a, axis=None): return accumulate("multiply", a, axis)    # statistics/mod.rs:795
 

@@ -226,3 +226,39 @@ def test_outer_where_edges(gpu):
                                   [11.0, 2.0])
     with pytest.raises(ValueError):
         K.where("add", np.ones(3), np.ones(3), np.ones(2, bool))
+
+
+# ---- ufunc_reduceat (ufunc_ops.rs:619-733)
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_reduceat(gpu, dt):
+    r = np.random.default_rng(12)
+    x = r.normal(size=200_003).astype(dt)
+    x[777] = np.nan
+    idx = np.array([0, 5, 5, 3, 1000, 70_000, 199_999, 10])  # repeats, decreasing, last segment to the end
+    # maximum / minimum: exact, numpy's reduceat rule and NaN propagation, to the bit
+    np.testing.assert_array_equal(K.reduceat("maximum", x, idx), np.maximum.reduceat(x, idx))
+    np.testing.assert_array_equal(K.reduceat("minimum", x, idx), np.minimum.reduceat(x, idx))
+    # add / subtract / multiply: the fixed tree against the exact segment values
+    eps = 2.0**-53 if dt == np.float64 else 2.0**-24
+    y = x.copy()
+    y[777] = 0.5
+    want = np.add.reduceat(y.astype(LD), idx)
+    absum = np.add.reduceat(np.abs(y).astype(LD), idx)
+    assert np.all(np.abs(K.reduceat("add", y, idx).astype(LD) - want) <= 64 * eps * absum)
+    sub = np.array([y[j] - (np.sum(y[j + 1:e].astype(LD)) if e > j else 0)
+                    for j, e in zip(idx, list(idx[1:]) + [y.size])], LD)
+    assert np.all(np.abs(K.reduceat("subtract", y, idx).astype(LD) - sub) <= 64 * eps * absum)
+    p = r.uniform(0.999, 1.001, 50_000).astype(dt)
+    pidx = np.array([0, 100, 40_000])
+    np.testing.assert_allclose(K.reduceat("multiply", p, pidx).astype(np.float64),
+                               np.multiply.reduceat(p.astype(LD), pidx).astype(np.float64), rtol=50_000 * eps)
+    # along an axis; signed zeros of a one-element and an all-zero segment are kept
+    m = r.normal(size=(6, 9000)).astype(dt)
+    np.testing.assert_array_equal(K.reduceat("maximum", m, [0, 10, 4000], axis=1), np.maximum.reduceat(m, [0, 10, 4000], axis=1))
+    np.testing.assert_array_equal(K.reduceat("maximum", m, [0, 2], axis=0), np.maximum.reduceat(m, [0, 2], axis=0))
+    z = np.array([-0.0, -0.0, 1.0], dt)
+    got = K.reduceat("add", z, [0, 2])
+    assert got[0] == 0 and np.signbit(got[0]) and got[1] == 1.0
+    with pytest.raises(Exception):
+        K.reduceat("add", x, [0, x.size])
