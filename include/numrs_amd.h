@@ -291,6 +291,18 @@ nrk_status nrk_dev_where_f64(int32_t op, const double *a, const double *b, const
 nrk_status nrk_dev_where_f32(int32_t op, const float *a, const float *b, const uint8_t *mask, float *y, size_t n,
                              nrk_stream stream);
 
+/* ufunc_ops.rs:772-838 ufunc_at, in place: for k = 0 .. m-1 in order, row indices[k] of a
+ * (n0 rows of row_len) becomes a[indices[k]][:] op b[k][:], op as for accumulate below. Repeated
+ * indices fold in k order, the reference's sequence of roundings exactly. Host form: an index
+ * >= n0 is NRK_ERR_INVALID_ARGUMENT (IndexOutOfBounds) and a is untouched; device form: indices
+ * must be < n0 (a larger one is skipped). */
+nrk_status nrk_at_f64(int32_t op, double *a, size_t n0, size_t row_len, const uint64_t *indices, size_t m, const double *b);
+nrk_status nrk_at_f32(int32_t op, float *a, size_t n0, size_t row_len, const uint64_t *indices, size_t m, const float *b);
+nrk_status nrk_dev_at_f64(int32_t op, double *a, size_t n0, size_t row_len, const uint64_t *indices, size_t m,
+                          const double *b, nrk_stream stream);
+nrk_status nrk_dev_at_f32(int32_t op, float *a, size_t n0, size_t row_len, const uint64_t *indices, size_t m,
+                          const float *b, nrk_stream stream);
+
 /* ufunc_ops.rs:619-733 ufunc_reduceat along contiguous rows: y[r][k] reduces x[r][j .. end) with
  * j = indices[k] and end = indices[k + 1] (len for the last k), or is x[r][j] when end <= j. op as
  * for accumulate below. Host form: an index >= len is NRK_ERR_INVALID_ARGUMENT (the reference's

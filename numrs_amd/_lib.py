@@ -72,6 +72,9 @@ _SIGS: dict[str, list] = {
     "nrk_dot_f64": [vp, vp, c_size, vp], "nrk_dot_f32": [vp, vp, c_size, vp],
     "nrk_fma_f64": [vp, vp, vp, vp, c_size], "nrk_fma_f32": [vp, vp, vp, vp, c_size],
     "nrk_accumulate_f64": [c_i32, vp, c_size, c_size, vp], "nrk_accumulate_f32": [c_i32, vp, c_size, c_size, vp],
+    "nrk_at_f64": [c_i32, vp, c_size, c_size, vp, c_size, vp], "nrk_at_f32": [c_i32, vp, c_size, c_size, vp, c_size, vp],
+    "nrk_dev_at_f64": [c_i32, vp, c_size, c_size, vp, c_size, vp, vp],
+    "nrk_dev_at_f32": [c_i32, vp, c_size, c_size, vp, c_size, vp, vp],
     "nrk_reduceat_f64": [c_i32, vp, c_size, c_size, vp, c_size, vp], "nrk_reduceat_f32": [c_i32, vp, c_size, c_size, vp, c_size, vp],
     "nrk_dev_reduceat_f64": [c_i32, vp, c_size, c_size, vp, c_size, vp, vp],
     "nrk_dev_reduceat_f32": [c_i32, vp, c_size, c_size, vp, c_size, vp, vp],

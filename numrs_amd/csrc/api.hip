@@ -249,6 +249,28 @@ nrk_status host_reduceat(int32_t op, const T *x, size_t rows, size_t len, const 
     });
 }
 
+template <typename T>
+nrk_status host_at(int32_t op, T *a, size_t n0, size_t row_len, const uint64_t *idx, size_t m, const T *b) {
+    NRK_TRY(ensure_device());
+    const size_t na = n0 * row_len, nb = m * row_len;
+    if ((na && !a) || (m && !idx) || (nb && !b)) return fail(NRK_ERR_INVALID_ARGUMENT, "a, indices or b is NULL");
+    for (size_t k = 0; k < m; ++k)  // ufunc_ops.rs:794-800
+        if (idx[k] >= n0) return fail(NRK_ERR_INVALID_ARGUMENT, "index %llu out of bounds for axis 0 with size %zu", (unsigned long long)idx[k], n0);
+    if (na == 0 || nb == 0) return NRK_OK;
+    void *da = nullptr, *di = nullptr, *db = nullptr;
+    NRK_TRY(staging(0, na * sizeof(T) + 16, &da));
+    NRK_TRY(staging(2, m * sizeof(uint64_t) + 16, &di));
+    NRK_TRY(staging(3, nb * sizeof(T) + 16, &db));
+    hipStream_t s = nullptr;
+    NRK_HIP(hipMemcpyAsync(da, a, na * sizeof(T), hipMemcpyHostToDevice, s));
+    NRK_HIP(hipMemcpyAsync(di, idx, m * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+    NRK_HIP(hipMemcpyAsync(db, b, nb * sizeof(T), hipMemcpyHostToDevice, s));
+    NRK_TRY(ufunc_at<T>(op, (T *)da, n0, row_len, (const uint64_t *)di, m, (const T *)db, s));
+    NRK_HIP(hipMemcpyAsync(a, da, na * sizeof(T), hipMemcpyDeviceToHost, s));
+    NRK_HIP(hipStreamSynchronize(s));
+    return NRK_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -280,6 +302,12 @@ nrk_status nrk_where_f64(int32_t op, const double *a, const double *b, const uin
 nrk_status nrk_where_f32(int32_t op, const float *a, const float *b, const uint8_t *mask, float *y, size_t n) { return host_where<float>(op, a, b, mask, y, n); }
 nrk_status nrk_dev_where_f64(int32_t op, const double *a, const double *b, const uint8_t *mask, double *y, size_t n, nrk_stream stream) { return where_op<double>(op, a, b, mask, y, n, S(stream)); }
 nrk_status nrk_dev_where_f32(int32_t op, const float *a, const float *b, const uint8_t *mask, float *y, size_t n, nrk_stream stream) { return where_op<float>(op, a, b, mask, y, n, S(stream)); }
+
+// ---- at (ufunc_ops.rs:772) ----
+nrk_status nrk_at_f64(int32_t op, double *a, size_t n0, size_t row_len, const uint64_t *indices, size_t m, const double *b) { return host_at<double>(op, a, n0, row_len, indices, m, b); }
+nrk_status nrk_at_f32(int32_t op, float *a, size_t n0, size_t row_len, const uint64_t *indices, size_t m, const float *b) { return host_at<float>(op, a, n0, row_len, indices, m, b); }
+nrk_status nrk_dev_at_f64(int32_t op, double *a, size_t n0, size_t row_len, const uint64_t *indices, size_t m, const double *b, nrk_stream stream) { return ufunc_at<double>(op, a, n0, row_len, indices, m, b, S(stream)); }
+nrk_status nrk_dev_at_f32(int32_t op, float *a, size_t n0, size_t row_len, const uint64_t *indices, size_t m, const float *b, nrk_stream stream) { return ufunc_at<float>(op, a, n0, row_len, indices, m, b, S(stream)); }
 
 // ---- reduceat (ufunc_ops.rs:713) ----
 nrk_status nrk_reduceat_f64(int32_t op, const double *x, size_t rows, size_t len, const uint64_t *indices, size_t nidx, double *y) { return host_reduceat<double>(op, x, rows, len, indices, nidx, y); }

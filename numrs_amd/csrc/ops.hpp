@@ -36,6 +36,10 @@ nrk_status sqrt_scalar(T *p, hipStream_t stream);
 // inclusive scans along `rows` contiguous rows of `len` (op: NRK_BOP_ADD/SUB/MUL/DIV/MAXIMUM/MINIMUM)
 template <typename T>
 nrk_status accumulate(int op, const T *x, T *y, size_t rows, size_t len, hipStream_t stream);
+// a[indices[k]][:] = a[indices[k]][:] op b[k][:] in k order (ufunc_at), a of n0 rows of row_len
+template <typename T>
+nrk_status ufunc_at(int op, T *a, size_t n0, size_t row_len, const uint64_t *indices, size_t m, const T *b,
+                    hipStream_t stream);
 // segmented reductions at indices along contiguous rows (ufunc_reduceat's segment rule)
 template <typename T>
 nrk_status reduceat(int op, const T *x, size_t rows, size_t len, const uint64_t *idx, size_t nidx, T *y, hipStream_t stream);

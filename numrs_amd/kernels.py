@@ -310,6 +310,20 @@ def multiply_where(a, b, mask): return where("multiply", a, b, mask)  # :911
 def divide_where(a, b, mask): return where("divide", a, b, mask)      # :920
 
 
+def at(op: str, a: np.ndarray, indices, b) -> None:
+    """ufunc_at (ufunc_ops.rs:772), in place on the caller's contiguous float array a: for each k in
+    order, a[indices[k]] = a[indices[k]] op b[k]; b has shape (len(indices),) + a.shape[1:]."""
+    if not isinstance(a, np.ndarray) or a.ndim == 0 or not a.flags.c_contiguous or a.dtype not in (np.float32, np.float64):
+        raise ValueError("ufunc_at requires a contiguous float array with at least 1 dimension")
+    idx = np.ascontiguousarray(np.asarray(indices, dtype=np.uint64).reshape(-1))
+    b = np.ascontiguousarray(np.asarray(b, dtype=a.dtype))
+    if b.shape != (idx.size,) + a.shape[1:]:
+        raise ValueError(f"shape mismatch: b is {b.shape}, expected {(idx.size,) + a.shape[1:]}")
+    row_len = int(np.prod(a.shape[1:], dtype=np.int64))
+    call("nrk_at_f32" if a.dtype == np.float32 else "nrk_at_f64", C.c_int32(BOP[_ACC_OPS[op]]), _ptr(a),
+         C.c_size_t(a.shape[0]), C.c_size_t(row_len), _ptr(idx), C.c_size_t(idx.size), _ptr(b))
+
+
 def reduceat(op: str, a, indices, axis=None):
     """ufunc_reduceat (ufunc_ops.rs:713): numpy's op.reduceat segment rule along axis (None: the
     flattened array); an index out of bounds raises."""

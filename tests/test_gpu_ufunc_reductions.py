@@ -262,3 +262,40 @@ def test_reduceat(gpu, dt):
     assert got[0] == 0 and np.signbit(got[0]) and got[1] == 1.0
     with pytest.raises(Exception):
         K.reduceat("add", x, [0, x.size])
+
+
+# ---- ufunc_at (ufunc_ops.rs:772-838)
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+@pytest.mark.parametrize("op", list(NP_UFUNC))
+def test_ufunc_at_bit_exact(gpu, dt, op):
+    """Repeated indices fold in k order, as numpy's unbuffered op.at does: the same bits."""
+    r = np.random.default_rng(13)
+    a = r.uniform(0.5, 2.0, size=(300, 7)).astype(dt)
+    idx = r.integers(0, 300, 5000)
+    idx[:50] = 17  # a long run of one row
+    b = r.uniform(0.5, 2.0, size=(5000, 7)).astype(dt)
+    b[3, 2] = np.nan
+    want = a.copy()
+    with np.errstate(all="ignore"):
+        NP_UFUNC[op].at(want, idx, b)
+    got = a.copy()
+    K.at(op, got, idx, b)
+    np.testing.assert_array_equal(got, want)
+    v = r.normal(size=1000).astype(dt)  # 1-D
+    vi = r.integers(0, 1000, 100_000)
+    vb = r.normal(size=100_000).astype(dt)
+    wv = v.copy()
+    with np.errstate(all="ignore"):
+        NP_UFUNC[op].at(wv, vi, vb)
+    K.at(op, v, vi, vb)
+    np.testing.assert_array_equal(v, wv)
+
+
+def test_ufunc_at_errors(gpu):
+    a = np.zeros((4, 2))
+    with pytest.raises(Exception):
+        K.at("add", a, [0, 4], np.ones((2, 2)))
+    assert not a.any()  # untouched after the rejected call
+    with pytest.raises(ValueError):
+        K.at("add", a, [0], np.ones((1, 3)))
