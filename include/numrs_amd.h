@@ -79,6 +79,7 @@ nrk_status nrk_device_synchronize(void);
 nrk_status nrk_event_create(void **event);
 nrk_status nrk_event_destroy(void *event);
 nrk_status nrk_event_record(void *event, nrk_stream stream);
+nrk_status nrk_stream_wait_event(nrk_stream stream, void *event); /* later work on stream waits for event */
 nrk_status nrk_event_elapsed_ms(float *ms, void *start, void *stop);
 
 /* ------------------------------------------------------------------------------------------ */

@@ -54,6 +54,7 @@ _SIGS: dict[str, list] = {
     "nrk_event_create": [C.POINTER(vp)],
     "nrk_event_destroy": [vp],
     "nrk_event_record": [vp, vp],
+    "nrk_stream_wait_event": [vp, vp],
     "nrk_event_elapsed_ms": [C.POINTER(C.c_float), vp, vp],
     # elementwise
     "nrk_map_f64": [vp, c_i32, vp, vp, c_size],

@@ -324,6 +324,11 @@ nrk_status nrk_event_record(void *event, nrk_stream stream) {
     return NRK_OK;
 }
 
+nrk_status nrk_stream_wait_event(nrk_stream stream, void *event) {
+    NRK_HIP(hipStreamWaitEvent((hipStream_t)stream, (hipEvent_t)event, 0));
+    return NRK_OK;
+}
+
 nrk_status nrk_event_elapsed_ms(float *ms, void *start, void *stop) {
     if (!ms) return fail(NRK_ERR_INVALID_ARGUMENT, "ms is NULL");
     NRK_HIP(hipEventSynchronize((hipEvent_t)stop));

@@ -73,6 +73,10 @@ class Stream:
         call("nrk_stream_create", C.byref(s))
         self.handle = s.value
 
+    def wait(self, event: "Event") -> None:
+        """Later work on this stream waits for `event` (recorded on any stream)."""
+        call("nrk_stream_wait_event", C.c_void_p(self.handle), C.c_void_p(event.handle))
+
     def synchronize(self) -> None:
         call("nrk_stream_synchronize", C.c_void_p(self.handle))
 
