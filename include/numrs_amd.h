@@ -162,7 +162,8 @@ enum {
     NRK_SOP_POW = 54,         /* :305 power_scalar, x.powf(s), within 1 ulp, C's special cases */
     NRK_SOP_GAMMA = 55,       /* :1122 Lanczos g = 7 (reflection below 0.5), with its powf     */
     NRK_SOP_LGAMMA = 56,      /* :1183 ln |gamma_single(x)| (no reflection, as the reference)  */
-    NRK_SOP_COUNT = 57
+    NRK_SOP_CBRT = 57,        /* :367 x.cbrt(), within 1 ulp                                   */
+    NRK_SOP_COUNT = 58
 };
 typedef struct nrk_scalar_op {
     int32_t op;     /* NRK_SOP_* */

@@ -227,6 +227,7 @@ __device__ __forceinline__ void apply_step_n(int op, double sd, T (&v)[N]) {
                         NRK_CHAIN_CASE(NRK_SOP_ERF, via_f64(x, [](double d) { return fm::erf_ref(d); }))
                         NRK_CHAIN_CASE(NRK_SOP_ERFC, via_f64(x, [](double d) { return 1.0 - fm::erf_ref(d); }))
                         NRK_CHAIN_CASE(NRK_SOP_DIGAMMA, via_f64(x, [](double d) { return fm::digamma(d); }))
+                        NRK_CHAIN_CASE(NRK_SOP_CBRT, via_f64(x, [](double d) { return fm::cbrt(d); }))
                         default:
                             if constexpr (MATH >= MATH_ALL) {
                                 switch (op) {

@@ -158,7 +158,7 @@ nrk_status map(const Chain &chain, const T *x, T *y, size_t n, hipStream_t strea
         }
     }
     const dim3 grid(stream_grid(n / Vec16<T>::N + 1));
-    static_assert(NRK_SOP_EXP == 18 && NRK_SOP_COUNT == 57, "the one-op kernels below list codes 18 .. 56");
+    static_assert(NRK_SOP_EXP == 18 && NRK_SOP_COUNT == 58, "the one-op kernels below list codes 18 .. 57");
     if (n_math == 1) {
         switch (chain.op[at]) {
 #define NRK_MAP_ONE(OP) \
@@ -168,7 +168,7 @@ nrk_status map(const Chain &chain, const T *x, T *y, size_t n, hipStream_t strea
             NRK_MAP_ONE(32) NRK_MAP_ONE(33) NRK_MAP_ONE(34) NRK_MAP_ONE(35) NRK_MAP_ONE(36) NRK_MAP_ONE(37) NRK_MAP_ONE(38)
             NRK_MAP_ONE(39) NRK_MAP_ONE(40) NRK_MAP_ONE(41) NRK_MAP_ONE(42) NRK_MAP_ONE(43) NRK_MAP_ONE(44) NRK_MAP_ONE(45)
             NRK_MAP_ONE(46) NRK_MAP_ONE(47) NRK_MAP_ONE(48) NRK_MAP_ONE(49) NRK_MAP_ONE(50) NRK_MAP_ONE(51) NRK_MAP_ONE(52)
-            NRK_MAP_ONE(53) NRK_MAP_ONE(54) NRK_MAP_ONE(55) NRK_MAP_ONE(56)
+            NRK_MAP_ONE(53) NRK_MAP_ONE(54) NRK_MAP_ONE(55) NRK_MAP_ONE(56) NRK_MAP_ONE(57)
 #undef NRK_MAP_ONE
             default: return fail(NRK_ERR_INVALID_ARGUMENT, "op %d has no map kernel", chain.op[at]);
         }

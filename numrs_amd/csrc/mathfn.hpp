@@ -760,6 +760,31 @@ __device__ inline double digamma(double x) {
 }
 
 
+// cbrt (ufuncs.rs:367, x.cbrt()): |x| = m 2^(3q) with m in [0.5, 4); a binary32 start from the
+// hardware exp2 / log2 (about 22 bits), two Newton steps y <- y - (y^3 - m) / (3 y^2), then one
+// more with the residual y^3 - m formed in double-double, so the last step's own rounding is the
+// only one left. Signed zeros, infinities and NaN return themselves.
+__device__ inline double cbrt(double x) {
+    if (x == 0.0 || x != x || __builtin_fabs(x) == dinf()) return x;
+    int e;
+    double m = frexp(__builtin_fabs(x), &e);  // [0.5, 1)
+    int q = e / 3, r = e - 3 * q;
+    if (r < 0) {
+        r += 3;
+        --q;
+    }
+    m = ldexp(m, r);  // [0.5, 4)
+    const float mf = (float)m;
+    double y = (double)__builtin_amdgcn_exp2f(__builtin_amdgcn_logf(mf) * (1.0f / 3.0f));
+    y = y - (y * y * y - m) / (3.0 * y * y);
+    y = y - (y * y * y - m) / (3.0 * y * y);
+    const double y2h = y * y, y2l = fma(y, y, -y2h);
+    const double y3h = y2h * y, y3l = fma(y2h, y, -y3h) + y2l * y;
+    const double res = (y3h - m) + y3l;  // y3h - m is exact (Sterbenz: y^3 within 2^-40 of m)
+    y = y - res / (3.0 * y2h);
+    return __builtin_copysign(ldexp(y, q), x);
+}
+
 // ---- pow and the Lanczos gamma (ufuncs.rs:226, 305, 1125-1195) ----
 
 // ln z for z = 2^k m in double-double, relative error near 2^-68 (what pow needs: |y ln x| up to

@@ -29,7 +29,7 @@ SOP = {"add": 0, "sub": 1, "mul": 2, "div": 3, "rsub": 4, "rdiv": 5, "neg": 6, "
        "sin": 34, "cos": 35, "tan": 36, "relu": 37, "leaky_relu": 38, "elu": 39, "selu": 40,
        "hardsigmoid": 41, "hardswish": 42, "softplus": 43, "mish": 44, "silu": 45, "gelu": 46,
        "logit": 47, "sinc": 48, "smoothstep": 49, "smootherstep": 50, "erf": 51, "erfc": 52,
-       "digamma": 53, "pow": 54, "gamma": 55, "lgamma": 56}
+       "digamma": 53, "pow": 54, "gamma": 55, "lgamma": 56, "cbrt": 57}
 BOP = {"add": 0, "sub": 1, "mul": 2, "div": 3, "rem": 4, "maximum": 5, "minimum": 6, "copysign": 7, "hypot": 8,
        "atan2": 9, "logaddexp": 10, "prelu": 11, "power": 12}
 AXIS = {"sum": 0, "mean": 1, "var": 2, "min": 3, "max": 4, "argmin": 5, "argmax": 6}
@@ -195,6 +195,7 @@ def reciprocal(a): return _ufunc(a, ("rdiv", 1.0))           # :374 1.0 / x
 def rsqrt(a): return _ufunc(a, ("sqrt",), ("rdiv", 1.0))     # :384 1.0 / x.sqrt()
 def sqrt(a): return _ufunc(a, ("sqrt",))                     # :354 x.sqrt()
 def square(a): return _ufunc(a, ("square",))                 # :344 x * x
+def cbrt(a): return _ufunc(a, ("cbrt",))                     # :367 x.cbrt()
 def negative(a): return _ufunc(a, ("neg",))                  # :319 -x
 def absolute(a): return _ufunc(a, ("abs",))                  # :329 x.abs()
 def floor(a): return _ufunc(a, ("floor",))                   # :702 x.floor()
@@ -363,7 +364,25 @@ def softmax(a): return _softmax("softmax", a)                  # ufuncs.rs:1434 
 def log_softmax(a): return _softmax("log_softmax", a)          # ufuncs.rs:1411
 
 
-def prelu(a, alpha): return _bufunc(a, alpha, "prelu")
+def prelu(a, alpha): return _bufunc(a, alpha, "prelu")  # :1394 x > 0 ? x : alpha * x
+
+
+# ufuncs.rs:182-303 arithmetic by name (the ufunc closures :150-173, one rounding each), their
+# scalar forms (Array::*_scalar, operations.rs:448-474), and the flat f64 reductions (:990-1060)
+def add(a, b): return _bufunc(a, b, "add")
+def subtract(a, b): return _bufunc(a, b, "sub")
+def multiply(a, b): return _bufunc(a, b, "mul")
+def divide(a, b): return _bufunc(a, b, "div")
+def maximum(a, b): return _bufunc(a, b, "maximum")            # :170 if a > b {a} else {b}
+def minimum(a, b): return _bufunc(a, b, "minimum")            # :174
+def add_scalar(a, s: float): return _ufunc(a, ("add", s))
+def subtract_scalar(a, s: float): return _ufunc(a, ("sub", s))
+def multiply_scalar(a, s: float): return _ufunc(a, ("mul", s))
+def divide_scalar(a, s: float): return _ufunc(a, ("div", s))
+abs = absolute  # noqa: A001
+def sum(a): return sum_f64(np.asarray(a, np.float64).reshape(-1))    # noqa: A001
+def mean(a): return mean_f64(np.asarray(a, np.float64).reshape(-1))
+def var(a): return var_f64(np.asarray(a, np.float64).reshape(-1), 0)
 
 
 def power(a, b):

@@ -564,3 +564,32 @@ def test_gamma_against_its_mathematics(gpu):
     assert np.all(np.abs(lg - sps.gammaln(x)) <= 5e-13 * (1 + np.abs(sps.gammaln(x))))
     assert K.gamma(np.array([5.0]))[0] == pytest.approx(24.0, rel=1e-14)
     assert K.gamma(np.array([150.0]))[0] == np.inf  # the reference's overflow, kept
+
+
+def test_cbrt_within_one_ulp(gpu):
+    """ufuncs.rs:367 x.cbrt(): every binade, subnormals, negatives; special values exact."""
+    r = np.random.default_rng(101)
+    x = np.concatenate([r.choice([-1.0, 1.0], 60000) * np.exp2(r.uniform(-1074, 1023, 60000)),
+                        r.uniform(-10, 10, 20000), np.arange(1.0, 2000.0) ** 3])
+    got = K.cbrt(x)
+    err = ulp_err(got, np.cbrt(x.astype(LD)))
+    print(f"cbrt: max {err.max():.3f} ulp")
+    assert err.max() <= 1.0
+    sp = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 8.0, -27.0, 5e-324])
+    assert_same(K.cbrt(sp)[:7], np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 2.0, -3.0]))
+    x32 = r.uniform(-1e6, 1e6, 20000).astype(np.float32)
+    assert ulp_err(K.cbrt(x32), np.cbrt(x32.astype(LD)), np.float32).max() <= 0.5 + 1e-9
+
+
+def test_reference_named_arithmetic(gpu):
+    """ufuncs.rs:182-303 by name, one IEEE rounding each, and the flat reductions."""
+    r = np.random.default_rng(102)
+    a, b = r.normal(size=10001), r.normal(size=10001)
+    for name, ref in (("add", np.add), ("subtract", np.subtract), ("multiply", np.multiply), ("divide", np.divide)):
+        np.testing.assert_array_equal(getattr(K, name)(a, b), ref(a, b))
+        np.testing.assert_array_equal(getattr(K, name + "_scalar")(a, 0.37), ref(a, 0.37))
+    np.testing.assert_array_equal(K.maximum(a, b), np.where(a > b, a, b))
+    np.testing.assert_array_equal(K.abs(a), np.abs(a))
+    assert K.sum(a) == pytest.approx(math.fsum(a), abs=1e-12)
+    assert K.mean(a) == pytest.approx(math.fsum(a) / a.size, abs=1e-15)
+    assert K.var(a) == pytest.approx(np.var(a), rel=1e-13)
