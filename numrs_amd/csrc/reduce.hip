@@ -988,18 +988,34 @@ template nrk_status prod<double>(const double *, size_t, double *, hipStream_t);
 // ---------------- dot: sum of rounded products a[i] * b[i] in a fixed tree ----------------
 // ufuncs.rs:946-963 sums the rounded products in T. Each segment of RB * DOT_PER elements gives
 // one partial (per-thread sums over DOT_PER products, then the wave and workgroup trees), so the
-// tree depends only on n: the result is the same bits for every grid.
+// tree depends only on n and on whether both operands are 16-byte aligned (vector loads): the
+// result is the same bits for every grid.
 constexpr int DOT_PER = 16;
 
 template <typename T>
-__global__ __launch_bounds__(RB) void dot_kernel(const T *a, const T *b, size_t n, size_t nseg, T *partials) {
+__global__ __launch_bounds__(RB) void dot_kernel(const T *a, const T *b, size_t n, size_t nseg, int vec_ok, T *partials) {
     __shared__ T sred[2][WAVES];
     const int t = threadIdx.x;
     int parity = 0;
     for (size_t seg = blockIdx.x; seg < nseg; seg += gridDim.x, parity ^= 1) {
         const size_t off = seg * (size_t)(RB * DOT_PER);
         T acc = (T)0;
-        if (off + (size_t)RB * DOT_PER <= n) {
+        if (vec_ok && off + (size_t)RB * DOT_PER <= n) {
+            // 16-byte non-temporal loads: vector j of thread t holds elements off + (j RB + t) VN ..
+            typedef typename Vec16<T>::type V;
+            constexpr int VN = Vec16<T>::N, NV = DOT_PER / VN;
+            const V *av = reinterpret_cast<const V *>(a + off), *bv = reinterpret_cast<const V *>(b + off);
+            V pa[NV], pb[NV];
+#pragma unroll
+            for (int j = 0; j < NV; ++j) {
+                pa[j] = __builtin_nontemporal_load(av + j * RB + t);
+                pb[j] = __builtin_nontemporal_load(bv + j * RB + t);
+            }
+#pragma unroll
+            for (int j = 0; j < NV; ++j)
+#pragma unroll
+                for (int e = 0; e < VN; ++e) acc = rn_add(acc, rn_mul(pa[j][e], pb[j][e]));
+        } else if (off + (size_t)RB * DOT_PER <= n) {
             T pa[DOT_PER], pb[DOT_PER];
 #pragma unroll
             for (int j = 0; j < DOT_PER; ++j) {
@@ -1057,7 +1073,10 @@ nrk_status dot(const T *a, const T *b, size_t n, T *out, hipStream_t stream) {
     Scratch s;
     NRK_TRY(get_scratch(stream, 0, nseg * sizeof(T), &s));
     const int grid = (int)std::min<size_t>(nseg, (size_t)cu_count() * 8);
-    hipLaunchKernelGGL(dot_kernel<T>, dim3(grid), dim3(RB), 0, stream, a, b, n, nseg, (T *)s.partials);
+    // the vector path's per-thread grouping differs from the scalar one: it is chosen by the
+    // operands' alignment only, so a given pair of buffers always gets the same bits
+    const int vec_ok = (reinterpret_cast<uintptr_t>(a) % 16 == 0) && (reinterpret_cast<uintptr_t>(b) % 16 == 0);
+    hipLaunchKernelGGL(dot_kernel<T>, dim3(grid), dim3(RB), 0, stream, a, b, n, nseg, vec_ok, (T *)s.partials);
     NRK_HIP(hipGetLastError());
     hipLaunchKernelGGL(sum_finish_kernel<T>, dim3(1), dim3(FB), 0, stream, (const T *)s.partials, nseg, out);
     NRK_HIP(hipGetLastError());
