@@ -297,7 +297,7 @@ nrk_status nrk_dev_where_f32(int32_t op, const float *a, const float *b, const u
  * (n0 rows of row_len) becomes a[indices[k]][:] op b[k][:], op as for accumulate below. Repeated
  * indices fold in k order, the reference's sequence of roundings exactly. Host form: an index
  * >= n0 is NRK_ERR_INVALID_ARGUMENT (IndexOutOfBounds) and a is untouched; device form: indices
- * must be < n0 (a larger one is skipped). */
+ * >= n0 are skipped (their updates are dropped; the valid ones are applied as above). */
 nrk_status nrk_at_f64(int32_t op, double *a, size_t n0, size_t row_len, const uint64_t *indices, size_t m, const double *b);
 nrk_status nrk_at_f32(int32_t op, float *a, size_t n0, size_t row_len, const uint64_t *indices, size_t m, const float *b);
 nrk_status nrk_dev_at_f64(int32_t op, double *a, size_t n0, size_t row_len, const uint64_t *indices, size_t m,
@@ -491,10 +491,13 @@ nrk_status nrk_dev_gemm_bf16(const nrk_gemm_desc *desc, nrk_stream stream);
  *   NRK_DT_F32_BF16X3 FP32 GEMM as 3 BF16 products (d0=m, d1=n, d2=k): configs 0..4 = tiles
  *   (workgroup / wave) 64x64 / 32x32, 128x128 / 64x64, 128x64 / 64x32, 256x128 / 64x64,
  *   128x256 / 64x64 (results identical across configs);
+ *   NRK_DT_F32_BF16X6 FP32 GEMM as 6 BF16 products: the same tile table (3 and 4 do not fit the
+ *   LDS with six terms and run as 1);
  *   NRK_DT_F64_OZAKI FP64 GEMM by the Ozaki scheme II, 16 moduli: configs 0..4 = INT8 tiles
  *   (workgroup / wave) 64x64 / 32x32, 128x128 / 64x64, 128x256 / 64x64, 256x256 / 128x64,
  *   256x256 / 128x128 (results identical across configs). */
-enum { NRK_DT_F64 = 0, NRK_DT_F32 = 1, NRK_DT_BF16 = 2, NRK_DT_ROWSUM_F32 = 3, NRK_DT_F32_BF16X3 = 4, NRK_DT_F64_OZAKI = 5 };
+enum { NRK_DT_F64 = 0, NRK_DT_F32 = 1, NRK_DT_BF16 = 2, NRK_DT_ROWSUM_F32 = 3, NRK_DT_F32_BF16X3 = 4, NRK_DT_F64_OZAKI = 5,
+       NRK_DT_F32_BF16X6 = 6 };
 nrk_status nrk_config_count(int32_t family, int32_t *count);
 /* Run one configuration (device pointers; c receives the result). */
 nrk_status nrk_dev_run_config(int32_t family, int32_t config, int32_t d0, int32_t d1, int32_t d2, const void *a,

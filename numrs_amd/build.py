@@ -1,6 +1,6 @@
 """Build recipe for libnumrs_amd.so (gfx950).
 
-    python -m numrs_amd.build
+    python -m numrs_amd.build [--changed]
 
 The library is compiled with hipcc, one object per source, in-tree: the shared object lands at
 numrs_amd/libnumrs_amd.so so that it travels with the repository snapshot. Every device pass
@@ -50,32 +50,45 @@ def _run(cmd: list[str]) -> None:
         print(out, file=sys.stderr)
 
 
-def build_lib(verbose: bool = True) -> Path:
-    """Compile every csrc/*.hip and link libnumrs_amd.so. Always recompiles."""
+def _stale(src: Path, obj: Path) -> bool:
+    """True if obj is missing or older than src or than any header it may include."""
+    if not obj.exists():
+        return True
+    t = obj.stat().st_mtime
+    deps = [src, *CSRC.glob("*.hpp"), *INCLUDE.glob("*.h"), Path(__file__)]
+    return any(d.stat().st_mtime > t for d in deps)
+
+
+def build_lib(verbose: bool = True, changed_only: bool = False) -> Path:
+    """Compile every csrc/*.hip and link libnumrs_amd.so. Always recompiles everything unless
+    changed_only, which recompiles only objects older than their source or any header."""
     if not Path(HIPCC).exists() and shutil.which(HIPCC) is None:
         raise RuntimeError(f"hipcc not found at {HIPCC}")
     BUILD.mkdir(exist_ok=True)
-    sources = sorted(CSRC.glob("*.hip"))
-    objs = [BUILD / (s.stem + ".o") for s in sources]
+    all_sources = sorted(CSRC.glob("*.hip"))
+    objs_all = [BUILD / (s.stem + ".o") for s in all_sources]
+    pairs = [(s, o) for s, o in zip(all_sources, objs_all) if not changed_only or _stale(s, o)]
+    sources = [s for s, _ in pairs]
+    objs = objs_all
 
     def compile_one(src_obj):
         src, obj = src_obj
         _run([HIPCC, *HIP_FLAGS, "-c", str(src), "-o", str(obj)])
         return obj
 
-    workers = min(len(sources), max(1, min(8, os.cpu_count() or 1)))
+    workers = min(max(1, len(sources)), max(1, min(8, os.cpu_count() or 1)))
     with cf.ThreadPoolExecutor(workers) as ex:
-        list(ex.map(compile_one, zip(sources, objs)))
+        list(ex.map(compile_one, pairs))
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
     os.replace(tmp, LIB)
     if verbose:
-        print(f"built {LIB.relative_to(ROOT)} from {len(sources)} sources", file=sys.stderr)
+        print(f"built {LIB.relative_to(ROOT)} ({len(sources)} of {len(all_sources)} sources compiled)", file=sys.stderr)
     return LIB
 
 
 def main(argv: list[str]) -> int:
-    build_lib()
+    build_lib(changed_only="--changed" in argv)
     return 0
 
 

@@ -987,9 +987,10 @@ template nrk_status prod<double>(const double *, size_t, double *, hipStream_t);
 
 // ---------------- dot: sum of rounded products a[i] * b[i] in a fixed tree ----------------
 // ufuncs.rs:946-963 sums the rounded products in T. Each segment of RB * DOT_PER elements gives
-// one partial (per-thread sums over DOT_PER products, then the wave and workgroup trees), so the
-// tree depends only on n and on whether both operands are 16-byte aligned (vector loads): the
-// result is the same bits for every grid.
+// one partial (per-thread sums over DOT_PER products, then the wave and workgroup trees). Thread
+// t of a segment sums the elements off + (j RB + t) VN + e, j < DOT_PER / VN, e < VN, in (j, e)
+// order, whether they arrive as 16-byte vectors (both operands aligned) or one by one: the tree
+// depends only on n, so the result bits are the same for every grid and every alignment.
 constexpr int DOT_PER = 16;
 
 template <typename T>
@@ -1000,10 +1001,10 @@ __global__ __launch_bounds__(RB) void dot_kernel(const T *a, const T *b, size_t 
     for (size_t seg = blockIdx.x; seg < nseg; seg += gridDim.x, parity ^= 1) {
         const size_t off = seg * (size_t)(RB * DOT_PER);
         T acc = (T)0;
+        typedef typename Vec16<T>::type V;
+        constexpr int VN = Vec16<T>::N, NV = DOT_PER / VN;
         if (vec_ok && off + (size_t)RB * DOT_PER <= n) {
             // 16-byte non-temporal loads: vector j of thread t holds elements off + (j RB + t) VN ..
-            typedef typename Vec16<T>::type V;
-            constexpr int VN = Vec16<T>::N, NV = DOT_PER / VN;
             const V *av = reinterpret_cast<const V *>(a + off), *bv = reinterpret_cast<const V *>(b + off);
             V pa[NV], pb[NV];
 #pragma unroll
@@ -1016,19 +1017,24 @@ __global__ __launch_bounds__(RB) void dot_kernel(const T *a, const T *b, size_t 
 #pragma unroll
                 for (int e = 0; e < VN; ++e) acc = rn_add(acc, rn_mul(pa[j][e], pb[j][e]));
         } else if (off + (size_t)RB * DOT_PER <= n) {
+            // the same elements in the same order as the vector path, one by one
             T pa[DOT_PER], pb[DOT_PER];
 #pragma unroll
-            for (int j = 0; j < DOT_PER; ++j) {
-                pa[j] = a[off + (size_t)j * RB + t];
-                pb[j] = b[off + (size_t)j * RB + t];
-            }
+            for (int j = 0; j < NV; ++j)
 #pragma unroll
-            for (int j = 0; j < DOT_PER; ++j) acc = rn_add(acc, rn_mul(pa[j], pb[j]));
+                for (int e = 0; e < VN; ++e) {
+                    const size_t pos = off + ((size_t)j * RB + t) * VN + e;
+                    pa[j * VN + e] = a[pos];
+                    pb[j * VN + e] = b[pos];
+                }
+#pragma unroll
+            for (int i = 0; i < DOT_PER; ++i) acc = rn_add(acc, rn_mul(pa[i], pb[i]));
         } else {
-            for (int j = 0; j < DOT_PER; ++j) {
-                const size_t pos = off + (size_t)j * RB + t;
-                if (pos < n) acc = rn_add(acc, rn_mul(a[pos], b[pos]));
-            }
+            for (int j = 0; j < NV; ++j)
+                for (int e = 0; e < VN; ++e) {
+                    const size_t pos = off + ((size_t)j * RB + t) * VN + e;
+                    if (pos < n) acc = rn_add(acc, rn_mul(a[pos], b[pos]));
+                }
         }
 #pragma unroll
         for (int m = 32; m >= 1; m >>= 1) acc = rn_add(acc, shfl_xor(acc, m));
@@ -1073,8 +1079,8 @@ nrk_status dot(const T *a, const T *b, size_t n, T *out, hipStream_t stream) {
     Scratch s;
     NRK_TRY(get_scratch(stream, 0, nseg * sizeof(T), &s));
     const int grid = (int)std::min<size_t>(nseg, (size_t)cu_count() * 8);
-    // the vector path's per-thread grouping differs from the scalar one: it is chosen by the
-    // operands' alignment only, so a given pair of buffers always gets the same bits
+    // vector loads where both operands are 16-byte aligned; the scalar path sums the same
+    // elements in the same order, so alignment changes only the speed, never the bits
     const int vec_ok = (reinterpret_cast<uintptr_t>(a) % 16 == 0) && (reinterpret_cast<uintptr_t>(b) % 16 == 0);
     hipLaunchKernelGGL(dot_kernel<T>, dim3(grid), dim3(RB), 0, stream, a, b, n, nseg, vec_ok, (T *)s.partials);
     NRK_HIP(hipGetLastError());
@@ -1198,8 +1204,8 @@ template <typename T, int OP>
 __device__ __forceinline__ T acc_apply(T a, T b) {
     if constexpr (OP == 0) return rn_add(a, b);
     else if constexpr (OP == 2) return rn_mul(a, b);
-    else if constexpr (OP == 5) return (a != a || b != b) ? a + b : (a > b ? a : b);
-    else return (a != a || b != b) ? a + b : (a < b ? a : b);
+    else if constexpr (OP == 5) return (a != a || b != b) ? dev_nan<T>() : (a > b ? a : b);
+    else return (a != a || b != b) ? dev_nan<T>() : (a < b ? a : b);
 }
 template <typename T, int OP>
 __device__ __forceinline__ T acc_identity() {
@@ -1398,7 +1404,12 @@ __global__ __launch_bounds__(RB) void reduceat_kernel(const T *x, size_t rows, s
         const uint64_t e = nxt > j ? (nxt < len ? nxt : (uint64_t)len) : j + 1;
         T a = rat_identity<T, OP>();
         if (j < len) {
-            for (uint64_t c = j + t; c < e; c += RB) {
+            // thread t folds the t-th contiguous run of the segment: every combine below keeps the
+            // earlier elements on the left, so maximum / minimum return the element the reference's
+            // sequential fold returns (a tie goes to the later operand, which fixes a zero's sign)
+            const uint64_t per = (e - j + RB - 1) / RB;
+            const uint64_t c0 = j + (uint64_t)t * per, c1 = c0 + per < e ? c0 + per : e;
+            for (uint64_t c = c0; c < c1; ++c) {
                 T v = x[r * len + c];
                 if constexpr (XF == 1) v = c == j ? v : -v;
                 else if constexpr (XF == 2) v = c == j ? v : rn_div((T)1, v);

@@ -24,6 +24,7 @@ static int config_count(int dtype) {
         case NRK_DT_BF16: return 5;
         case NRK_DT_ROWSUM_F32: return 4;
         case NRK_DT_F32_BF16X3:
+        case NRK_DT_F32_BF16X6:
         case NRK_DT_F64_OZAKI: return 5;
         default: return 0;
     }
@@ -53,6 +54,9 @@ static nrk_status run_config(int dtype, int config, int d0, int d1, int d2, cons
         case NRK_DT_F32_BF16X3:
             if (config < 0 || config > 4) return fail(NRK_ERR_INVALID_ARGUMENT, "config %d outside [0,4]", config);
             return gemm_f32_bf16x(d0, d2, d1, (const float *)a, (const float *)b, (float *)c, 3, config, s);
+        case NRK_DT_F32_BF16X6:
+            if (config < 0 || config > 4) return fail(NRK_ERR_INVALID_ARGUMENT, "config %d outside [0,4]", config);
+            return gemm_f32_bf16x(d0, d2, d1, (const float *)a, (const float *)b, (float *)c, 6, config, s);
         case NRK_DT_F64_OZAKI:
             if (config < 0 || config >= config_count(dtype)) return fail(NRK_ERR_INVALID_ARGUMENT, "config %d outside [0,%d)", config, config_count(dtype));
             return gemm_f64_ozaki(d0, d2, d1, (const double *)a, (const double *)b, (double *)c, 16, config, s);

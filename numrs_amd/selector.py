@@ -33,7 +33,7 @@ from . import device
 from ._lib import call, load
 
 # operation families (include/numrs_amd.h)
-F64, F32, BF16, ROWSUM_F32, F32_BF16X3, F64_OZAKI = 0, 1, 2, 3, 4, 5
+F64, F32, BF16, ROWSUM_F32, F32_BF16X3, F64_OZAKI, F32_BF16X6 = 0, 1, 2, 3, 4, 5, 6
 GEMM_FAMILY_DTYPE = {F64: 0, F32: 1, BF16: 2}
 IN, H = 6, 64
 

@@ -6,6 +6,7 @@ and only as the checker or the timed CPU baseline -- never as a product code pat
 from __future__ import annotations
 
 import ctypes as C
+import math
 from pathlib import Path
 
 import numpy as np
@@ -215,3 +216,27 @@ def abs_gemm_entries(a: np.ndarray, b: np.ndarray, rows: np.ndarray, cols: np.nd
     lib().abs_gemm_entries_f64(k, n, _p(np.ascontiguousarray(a, np.float64)), _p(np.ascontiguousarray(b, np.float64)),
                                rows.size, _p(rows), _p(cols), _p(out))
     return out
+
+
+def ref_ufunc_fold(op: str, seq) -> float:
+    """ufunc_ops.rs:240-257 reduce_lane without an initial value: fold from the first element with
+    UfuncOp::apply (ufunc_ops.rs:156-180). maximum / minimum: NaN in either operand gives NaN,
+    otherwise `a > b ? a : b` (`a < b ? a : b`), so an equal later operand wins, which fixes the
+    sign of a zero result. Pure Python: for short segments only."""
+    it = iter(seq)
+    acc = next(it)
+    for x in it:
+        if op in ("maximum", "minimum"):
+            if math.isnan(acc) or math.isnan(x):
+                acc = math.nan
+            elif (acc > x) if op == "maximum" else (acc < x):
+                pass
+            else:
+                acc = x
+        elif op == "add":
+            acc = acc + x
+        elif op == "multiply":
+            acc = acc * x
+        else:
+            raise ValueError(op)
+    return acc

@@ -6,10 +6,9 @@
   sum and var within the stated bounds of the double-double statistics of y; the whole-array sum
   within the same bound of the double-double sum of the row sums (a checksum of checksums).
 - i64 sum of 2^28 values uniform in [-2^20, 2^20]: exact.
-- Ozaki-II FP64 GEMM at 4096^3: on sampled entries, never less accurate than the native FP64
-  GEMM against correctly rounded dot products (error scale sum_k |A_ik||B_kj|)."""
-import math
-
+- Ozaki-II FP64 GEMM (14 and 16 moduli) at 4096^3 and 8192^3, and the FP32 split GEMMs at
+  8192^3: on 512 sampled entries against the double-double dot products of the oracle (error scale
+  sum_k |A_ik||B_kj|), never less accurate than the native FP64 GEMM, resp. within the split bars."""
 import numpy as np
 import pytest
 
@@ -52,34 +51,48 @@ def test_i64_sum_2_28_exact(gpu):
     assert K.sum_i64(x) == int(x.sum(dtype=np.int64))
 
 
-def _split(x):
-    c = x * 134217729.0
-    hi = c - (c - x)
-    return hi, x - hi
+def _sampled(oracle, a, b, cs, count, seed):
+    """max over `count` sampled entries of |C - C*| / sum_k |A_ik||B_kj| for each C in cs, C* the
+    double-double dot product of the oracle."""
+    n, m = b.shape[1], a.shape[0]
+    pick = np.random.default_rng(seed)
+    rows, cols = pick.integers(0, m, count), pick.integers(0, n, count)
+    exact = oracle.dd_gemm_entries(a, b, rows, cols)
+    scale = oracle.abs_gemm_entries(a, b, rows, cols)
+    return [float((np.abs(c[rows, cols].astype(np.float64) - exact) / scale).max()) for c in cs]
 
 
-def _exact_dot(a, b):
-    ah, al = _split(a)
-    bh, bl = _split(b)
-    p = a * b
-    e = al * bl - (((p - ah * bh) - al * bh) - ah * bl)  # Dekker: p + e = a*b exactly
-    return math.fsum(np.concatenate([p, e]).tolist())
-
-
-def test_ozaki_4096_not_less_accurate_than_native(gpu):
-    n = 4096
-    a = rng.scaled_pow2(rng.BASE_SEED + 2400, n * n).reshape(n, n)
-    b = rng.scaled_pow2(rng.BASE_SEED + 2401, n * n).reshape(n, n)
+@pytest.mark.parametrize("n", [4096, 8192])
+def test_ozaki14_not_less_accurate_than_native(gpu, oracle, n):
+    """The benchmark's choice at native-or-better error, on the benchmark's own inputs (same seeds):
+    Ozaki-II with 14 moduli against the native FP64 MFMA GEMM on 512 sampled entries; 16 moduli
+    (the default) likewise."""
+    a = rng.scaled_pow2(rng.BASE_SEED + 40 + n, n * n).reshape(n, n)
+    b = rng.scaled_pow2(rng.BASE_SEED + 41 + n, n * n).reshape(n, n)
     c_nat = np.empty((n, n))
     K.gemm_2d(n, n, n, a, b, c_nat)
-    c_oz = K.gemm_f64_ozaki(a, b, 15)
-    pick = np.random.default_rng(5)
-    worst_nat = worst_oz = 0.0
-    for _ in range(32):
-        i, j = int(pick.integers(n)), int(pick.integers(n))
-        ex = _exact_dot(a[i], b[:, j])
-        scale = float(np.abs(a[i]) @ np.abs(b[:, j]))
-        worst_nat = max(worst_nat, abs(c_nat[i, j] - ex) / scale)
-        worst_oz = max(worst_oz, abs(c_oz[i, j] - ex) / scale)
-    print(f"4096^3 sampled max error: native {worst_nat:.2e}, ozaki15 {worst_oz:.2e}")
-    assert worst_oz <= worst_nat
+    c14 = K.gemm_f64_ozaki(a, b, 14)
+    c16 = K.gemm_f64_ozaki(a, b, 16)
+    e_nat, e14, e16 = _sampled(oracle, a, b, [c_nat, c14, c16], 512, 5 + n)
+    print(f"{n}^3 max error over 512 entries: native {e_nat:.3e}, ozaki14 {e14:.3e}, ozaki16 {e16:.3e}")
+    assert e14 <= e_nat and e16 <= e_nat
+
+
+def test_split_gemm_8192_f32(gpu, oracle):
+    """FP32 at 8192^3 on the benchmark's inputs: native FP32 (128 x 128 tile), BF16x3 (the
+    128 x 256 split tile) and BF16x6 (128 x 128) on 512 sampled entries against the double-double
+    oracle. Bars: native within gamma_k = k u / (1 - k u) of sum |A||B| (the classical dot-product
+    bound, u = 2^-24); BF16x6 <= 4x native + 2^-22; BF16x3 <= 2^-13."""
+    n = 8192
+    a = rng.scaled_pow2(rng.BASE_SEED + 40 + n, n * n, dtype=np.float32).reshape(n, n)
+    b = rng.scaled_pow2(rng.BASE_SEED + 41 + n, n * n, dtype=np.float32).reshape(n, n)
+    c_nat = np.empty((n, n), np.float32)
+    K.gemm_2d(n, n, n, a, b, c_nat)
+    c3 = K.gemm_f32_bf16x(a, b, 3)
+    c6 = K.gemm_f32_bf16x(a, b, 6)
+    e_nat, e3, e6 = _sampled(oracle, a, b, [c_nat, c3, c6], 512, 17)
+    print(f"8192^3 f32 max error over 512 entries: native {e_nat:.3e}, bf16x3 {e3:.3e}, bf16x6 {e6:.3e}")
+    u = 2.0 ** -24
+    assert e_nat <= n * u / (1 - n * u)
+    assert e6 <= 4 * e_nat + 2.0 ** -22
+    assert e3 <= 2.0 ** -13

@@ -157,3 +157,29 @@ def test_matmul_batch_broadcast(gpu, dtype):
         K.matmul(ints(2, 3, 4), ints(5, 6))
     with pytest.raises(ValueError):
         K.matmul(ints(2, 3, 4), ints(3, 4, 6))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_matmul_non_contiguous_operands(gpu, oracle, dtype):
+    """tests/test_matmul_dispatch.rs:283-325: operands that are transposed views (not C-contiguous)
+    are read in logical order (borrow::operand, borrow.rs:59-64, copies them), so the product is
+    the contiguous product: bit-identical to the call on contiguous copies, and within the
+    reference's 1e-9 (f64) / 1e-4 (f32) of the naive product. Also for a batch of transposed
+    panels (matmul's batch axes) and a column-sliced (strided) left operand."""
+    m, k, n = 33, 65, 31
+    a = rng.uniform(rng.BASE_SEED + 1900, k * m).reshape(k, m).astype(dtype).T
+    b = rng.uniform(rng.BASE_SEED + 1901, n * k).reshape(n, k).astype(dtype).T
+    assert not a.flags.c_contiguous and not b.flags.c_contiguous
+    ex = oracle.dd_gemm(a.astype(np.float64), b.astype(np.float64))
+    ac, bc = np.ascontiguousarray(a), np.ascontiguousarray(b)
+    want = K.matmul(ac, bc)
+    for x, y in ((a, b), (a, bc), (ac, b)):
+        got = K.matmul(x, y)
+        np.testing.assert_array_equal(got, want)
+        assert np.all(np.abs(got - ex) <= (1e-9 if dtype == np.float64 else 1e-4) * np.maximum(np.abs(ex), 1.0))
+    wide = rng.uniform(rng.BASE_SEED + 1902, m * 2 * k).reshape(m, 2 * k).astype(dtype)[:, ::2]
+    np.testing.assert_array_equal(K.matmul(wide, b), K.matmul(np.ascontiguousarray(wide), bc))
+    pa = rng.uniform(rng.BASE_SEED + 1903, 4 * k * m).reshape(4, k, m).astype(dtype).transpose(0, 2, 1)
+    got = K.matmul(pa, b)
+    for t in range(4):
+        np.testing.assert_array_equal(got[t], K.matmul(np.ascontiguousarray(pa[t]), bc))

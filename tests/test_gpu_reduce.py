@@ -106,6 +106,26 @@ def test_sum_mean_var_vs_double_double(gpu, oracle, dtype):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_sum_sq_dev_vs_exact_sum_of_terms(gpu, dtype):
+    """sum_sq_dev (reduce.rs:239-260): the reference sums the terms d * d with d = x - mean, each
+    operation rounded in T. numpy forms the same rounded terms (IEEE arithmetic in T); the kernel
+    may only differ in the order of the sum: |got - fsum(terms)| <= SUM_C eps sum(terms) over the
+    whole size grid, with the mean both exact-ish (the FP64 mean) and deliberately off."""
+    worst = 0.0
+    for i, n in enumerate(SIZES):
+        x = rng.normal(rng.BASE_SEED + 150 + i, n, dtype) + dtype(3.0)
+        for mean in (dtype(float(np.mean(x.astype(np.float64)))), dtype(2.5)):
+            d = (x - mean).astype(dtype)
+            terms = (d * d).astype(dtype)
+            exact = math.fsum(terms.astype(np.float64).tolist())
+            got = kfn("sum_sq_dev", dtype)(x, float(mean))
+            err = abs(got - exact)
+            assert err <= SUM_C * EPS[dtype] * exact + EPS[dtype] * exact, (n, float(mean), got, exact)
+            worst = max(worst, err / exact if exact else err)
+    print(f"{SUF[dtype]}: sum_sq_dev max rel err {worst:.3e}")
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_min_max_argmax_bit_exact(gpu, oracle, dtype):
     for i, n in enumerate(SIZES):
         x = rng.normal(rng.BASE_SEED + 200 + i, n, dtype)

@@ -299,3 +299,82 @@ def test_ufunc_at_errors(gpu):
     assert not a.any()  # untouched after the rejected call
     with pytest.raises(ValueError):
         K.at("add", a, [0], np.ones((1, 3)))
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_reduceat_max_min_zero_sign_follows_reference_fold(gpu, oracle, dt):
+    """Segments longer than a workgroup (256 threads) of zeros of mixed signs: the reference's fold
+    (reduce_lane, ufunc_ops.rs:240-257) lets an equal later operand win, so maximum / minimum return
+    the segment's last zero. Also NaN results are the canonical NaN (UfuncOp::apply's T::nan())."""
+    r = np.random.default_rng(51)
+    x = np.where(r.integers(0, 2, 5000) == 1, -0.0, 0.0).astype(dt)
+    idx = np.array([0, 300, 1299, 1300, 4000])
+    x[1299], x[3999], x[4999] = -0.0, 0.0, -0.0
+    for op in ("maximum", "minimum"):
+        got = K.reduceat(op, x, idx)
+        ends = list(idx[1:]) + [x.size]
+        want = np.array([oracle.ref_ufunc_fold(op, x[j:e].tolist()) for j, e in zip(idx, ends)], dt)
+        np.testing.assert_array_equal(np.signbit(got), np.signbit(want), err_msg=op)
+        np.testing.assert_array_equal(got, want)
+    y = np.ones(1000, dt)
+    y[500] = -np.nan
+    got = K.reduceat("maximum", y, [0, 600])
+    nan_bits = np.array([np.nan], dt).view(np.uint64 if dt == np.float64 else np.uint32)[0]
+    assert got.view(nan_bits.dtype)[0] == nan_bits and got[1] == 1.0
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_dot_bits_do_not_depend_on_alignment(gpu, dt):
+    """nrk_dev_dot on operands offset by one element (8-byte f64, 4-byte f32: the scalar path)
+    gives the same bits as on 16-byte aligned copies (the vector path), for full and tail
+    segments."""
+    import ctypes as C
+
+    from numrs_amd import device
+    from numrs_amd._lib import call
+
+    fn = "nrk_dev_dot_f64" if dt == np.float64 else "nrk_dev_dot_f32"
+    isz = np.dtype(dt).itemsize
+    st = device.Stream()
+    for n in (4096, 3 * 4096 + 100, 1_000_003):
+        r = np.random.default_rng(n)
+        a = r.normal(size=n).astype(dt)
+        b = r.normal(size=n).astype(dt)
+        da, db = device.DeviceArray.upload(a), device.DeviceArray.upload(b)
+        pa = device.DeviceArray.upload(np.concatenate([np.zeros(1, dt), a]))
+        pb = device.DeviceArray.upload(np.concatenate([np.zeros(1, dt), b]))
+        out = device.DeviceArray((2,), dt)
+        print(f"dot alignment n={n} {np.dtype(dt).name}", flush=True)
+        call(fn, C.c_void_p(da.ptr), C.c_void_p(db.ptr), C.c_size_t(n), C.c_void_p(out.ptr), C.c_void_p(st.handle))
+        call(fn, C.c_void_p(pa.ptr + isz), C.c_void_p(pb.ptr + isz), C.c_size_t(n), C.c_void_p(out.ptr + isz),
+             C.c_void_p(st.handle))
+        st.synchronize()
+        got = out.download()
+        assert got[0].tobytes() == got[1].tobytes(), (n, got)
+        assert got[0] == K.dot(a, b)
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_at_device_form_skips_out_of_range_indices(gpu, dt):
+    """Device form of ufunc_at: indices >= n0 are dropped, including ones whose low bits equal a
+    valid row (10 and 2^40 + 2 against n0 = 5 would share the sort key of row 2 if only the low
+    bits were sorted); the valid updates apply in k order, as numpy's op.at over the valid ones."""
+    import ctypes as C
+
+    from numrs_amd import device
+    from numrs_amd._lib import call
+
+    n0, row = 5, 3
+    a = np.arange(n0 * row, dtype=dt).reshape(n0, row) + 1
+    idx = np.array([2, 10, 2, (1 << 40) + 2, 4, 5, 7, 2], np.uint64)
+    b = (np.arange(idx.size * row, dtype=dt).reshape(idx.size, row) + 1) * dt(0.5)
+    valid = idx < n0
+    want = a.copy()
+    np.multiply.at(want, idx[valid].astype(np.int64), b[valid])
+    st = device.Stream()
+    da, di, dbb = device.DeviceArray.upload(a), device.DeviceArray.upload(idx), device.DeviceArray.upload(b)
+    fn = "nrk_dev_at_f64" if dt == np.float64 else "nrk_dev_at_f32"
+    call(fn, C.c_int32(2), C.c_void_p(da.ptr), C.c_size_t(n0), C.c_size_t(row), C.c_void_p(di.ptr), C.c_size_t(idx.size),
+         C.c_void_p(dbb.ptr), C.c_void_p(st.handle))
+    st.synchronize()
+    np.testing.assert_array_equal(da.download(), want)

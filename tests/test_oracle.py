@@ -356,3 +356,25 @@ def test_mlp_oracle_adam_closed_form():
     g = np.array([0.5, -2.0, 1e-3, -7.0])
     p1, m, v = mlp.adam(np.zeros(4), g, np.zeros(4), np.zeros(4), 1, 0.01)
     np.testing.assert_allclose(p1, -0.01 * np.sign(g), rtol=1e-4)
+
+
+def test_ufunc_fold_oracle_pins():
+    """ref_ufunc_fold (reduce_lane + UfuncOp::apply): maximum / minimum equal numpy's reductions on
+    tie-free data; on an all-zero segment the result is the last element (a tie goes to the later
+    operand), so its sign is the last zero's; NaN anywhere gives NaN; add of small integers is the
+    closed-form n(n-1)/2."""
+    import math as _m
+
+    import oracle
+
+    r = np.random.default_rng(41)
+    x = r.normal(size=300)
+    assert oracle.ref_ufunc_fold("maximum", x) == x.max()
+    assert oracle.ref_ufunc_fold("minimum", x) == x.min()
+    for last in (0.0, -0.0):
+        z = [(-0.0 if r.integers(2) else 0.0) for _ in range(50)] + [last]
+        assert _m.copysign(1.0, oracle.ref_ufunc_fold("maximum", z)) == _m.copysign(1.0, last)
+        assert _m.copysign(1.0, oracle.ref_ufunc_fold("minimum", z)) == _m.copysign(1.0, last)
+    assert _m.isnan(oracle.ref_ufunc_fold("maximum", [1.0, _m.nan, 3.0]))
+    assert _m.isnan(oracle.ref_ufunc_fold("minimum", [_m.nan, 1.0]))
+    assert oracle.ref_ufunc_fold("add", [float(i) for i in range(1000)]) == 1000 * 999 / 2
