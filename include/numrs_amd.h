@@ -397,6 +397,20 @@ nrk_status nrk_dev_map_reduce_f32(const nrk_scalar_op *chain, int32_t chain_len,
                                   size_t ddof, const nrk_reduce_outputs *row_out, const nrk_reduce_outputs *full_out,
                                   nrk_stream stream);
 
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under
+ * hipGraph capture, tight serving loops, harnesses that forbid hipMalloc): nrk_dev_map_reduce_f64
+ * with the same arguments and results, whose scratch (chunk records, record-tree buffers, the
+ * finish counter) is a fixed per-device array of NRK_NOALLOC_SCRATCH_BYTES, set up by nrk_init.
+ * It calls no hipMalloc / hipFree. Whole-array results alone fit for every shape (about 8192 chunk
+ * records); row results need about 272 B of records per row, so up to ~61 000 rows. A shape whose
+ * records exceed the array is NRK_ERR_INVALID_ARGUMENT, and nothing is launched.
+ * Every call on one device shares that device's array: calls that overlap on one device (e.g. on
+ * two streams) race and must be serialised by the caller. */
+#define NRK_NOALLOC_SCRATCH_BYTES ((size_t)16 << 20)
+nrk_status nrk_dev_map_reduce_f64_noalloc(const nrk_scalar_op *chain, int32_t chain_len, const double *x, size_t rows,
+                                          size_t cols, size_t ddof, const nrk_reduce_outputs *row_out,
+                                          const nrk_reduce_outputs *full_out, nrk_stream stream);
+
 /* ------------------------------------------------------------------------------------------ */
 /* GEMM: C := A * B, row-major, overwrite                                                      */
 /* ------------------------------------------------------------------------------------------ */

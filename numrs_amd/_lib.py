@@ -121,6 +121,7 @@ _SIGS: dict[str, list] = {
     "nrk_dev_reduce_axis_f32": [c_i32, vp, c_size, c_size, c_size, c_size, vp, vp],
     # fused
     "nrk_dev_map_reduce_f64": [vp, c_i32, vp, c_size, c_size, c_size, vp, vp, vp],
+    "nrk_dev_map_reduce_f64_noalloc": [vp, c_i32, vp, c_size, c_size, c_size, vp, vp, vp],
     "nrk_dev_map_reduce_f32": [vp, c_i32, vp, c_size, c_size, c_size, vp, vp, vp],
     # gemm
     "nrk_gemm_2d_f64": [c_size, c_size, c_size, vp, c_size, vp, c_size, vp, c_size],

@@ -206,11 +206,12 @@ nrk_status host_tri(int op, const T *a, const T *b, const T *c, T t, T *y, size_
 
 template <typename T>
 nrk_status dev_map_reduce(const nrk_scalar_op *chain, int32_t chain_len, const T *x, size_t rows, size_t cols, size_t ddof,
-                          const nrk_reduce_outputs *row_out, const nrk_reduce_outputs *full_out, hipStream_t s) {
+                          const nrk_reduce_outputs *row_out, const nrk_reduce_outputs *full_out, hipStream_t s,
+                          const ExtScratch *ext = nullptr) {
     Chain c;
     NRK_TRY(make_chain(chain, chain_len, &c));
     if (row_out && row_out->var && cols <= ddof) return fail(NRK_ERR_INVALID_ARGUMENT, "Degrees of freedom %zu >= axis size %zu", ddof, cols);
-    return seg_reduce<T>(c, x, rows, cols, (double)ddof, row_out, full_out, s);
+    return seg_reduce<T>(c, x, rows, cols, (double)ddof, row_out, full_out, s, ext);
 }
 
 template <typename T>
@@ -447,6 +448,14 @@ nrk_status nrk_dev_map_reduce_f64(const nrk_scalar_op *chain, int32_t chain_len,
 nrk_status nrk_dev_map_reduce_f32(const nrk_scalar_op *chain, int32_t chain_len, const float *x, size_t rows, size_t cols, size_t ddof,
                                   const nrk_reduce_outputs *row_out, const nrk_reduce_outputs *full_out, nrk_stream stream) {
     return dev_map_reduce<float>(chain, chain_len, x, rows, cols, ddof, row_out, full_out, S(stream));
+}
+// The same kernels on the device's fixed scratch: no hipMalloc / hipFree, ever.
+nrk_status nrk_dev_map_reduce_f64_noalloc(const nrk_scalar_op *chain, int32_t chain_len, const double *x, size_t rows,
+                                          size_t cols, size_t ddof, const nrk_reduce_outputs *row_out,
+                                          const nrk_reduce_outputs *full_out, nrk_stream stream) {
+    ExtScratch ext;
+    NRK_TRY(static_scratch(&ext));
+    return dev_map_reduce<double>(chain, chain_len, x, rows, cols, ddof, row_out, full_out, S(stream), &ext);
 }
 
 }  // extern "C"

@@ -45,6 +45,13 @@ struct Scratch {
 };
 nrk_status get_scratch(hipStream_t stream, size_t n_counters, size_t partial_bytes, Scratch *out);
 
+// Caller-provided device scratch (device address and size): the reducer then allocates nothing.
+// Its first 256 bytes hold the finish counter, cleared by a kernel at every call.
+struct ExtScratch {
+    void *base;
+    size_t bytes;
+};
+
 // Per-stream workspace for multi-pass algorithms (split operands, residues). Contents undefined
 // at entry; grows on demand (growing synchronises the stream first).
 nrk_status get_workspace(hipStream_t stream, size_t bytes, void **out);

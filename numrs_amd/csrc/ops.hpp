@@ -6,10 +6,15 @@
 namespace nrk {
 
 // Fused chain -> {sum, mean, var, min, max, argmin, argmax} along axis 1 of a (rows, cols)
-// matrix and/or over all of it, in one pass (reduce.hip).
+// matrix and/or over all of it, in one pass (reduce.hip). ext: NULL for the stream's own
+// scratch (allocated on demand), else the caller's.
 template <typename T>
 nrk_status seg_reduce(const Chain &chain, const T *x, size_t rows, size_t cols, double ddof,
-                      const nrk_reduce_outputs *row_out, const nrk_reduce_outputs *full_out, hipStream_t stream);
+                      const nrk_reduce_outputs *row_out, const nrk_reduce_outputs *full_out, hipStream_t stream,
+                      const ExtScratch *ext = nullptr);
+
+// The fixed device scratch of the calling thread's device (set up by nrk_init, runtime.hip).
+nrk_status static_scratch(ExtScratch *out);
 
 // Workgroups per CU of the segment reducer's persistent grid (result bits are independent of it).
 int reduce_blocks_per_cu();

@@ -749,9 +749,12 @@ static nrk_status launch_all(SegParams<T> &p, bool want_row, bool want_full, con
 }
 
 // Host launcher for the fused map -> reduce over a (rows, cols) matrix.
+__global__ void clear_counter_kernel(unsigned int *c) { *c = 0u; }
+
 template <typename T>
 nrk_status seg_reduce(const Chain &chain, const T *x, size_t rows, size_t cols, double ddof,
-                      const nrk_reduce_outputs *row_out, const nrk_reduce_outputs *full_out, hipStream_t stream) {
+                      const nrk_reduce_outputs *row_out, const nrk_reduce_outputs *full_out, hipStream_t stream,
+                      const ExtScratch *ext) {
     constexpr int M = NRK_REDUCE_M;
     constexpr int VPT = M / Vec16<T>::N;
     static_assert(VPT * Vec16<T>::N == M, "elements per lane must be whole 16-byte vectors");
@@ -804,7 +807,19 @@ nrk_status seg_reduce(const Chain &chain, const T *x, size_t rows, size_t cols, 
     const size_t tmp_count = std::max(rows * ceil_div(p.cpr, (size_t)TREE_SPAN), ceil_div(std::max(rows, p.nchunk), (size_t)TREE_SPAN)) + 1;
     const size_t pbytes = parts_bytes(p.nchunk) + parts_bytes(rows) + 2 * parts_bytes(tmp_count) + 256;
     Scratch s;
-    NRK_TRY(get_scratch(stream, 1, pbytes, &s));
+    if (ext) {  // the caller's scratch: counter in the first 256 bytes, cleared here; records after
+        if (pbytes + 256 > ext->bytes)
+            return fail(NRK_ERR_INVALID_ARGUMENT, "(%zu, %zu) needs %zu bytes of scratch; the fixed scratch holds %zu", rows, cols,
+                        pbytes + 256, ext->bytes);
+        s.counters = (unsigned int *)ext->base;
+        s.n_counters = 1;
+        s.partials = (char *)ext->base + 256;
+        s.partial_bytes = ext->bytes - 256;
+        hipLaunchKernelGGL(clear_counter_kernel, dim3(1), dim3(1), 0, stream, s.counters);
+        NRK_HIP(hipGetLastError());
+    } else {
+        NRK_TRY(get_scratch(stream, 1, pbytes, &s));
+    }
     char *b = (char *)s.partials;
     p.l1 = carve_parts(b, p.nchunk);
     const Parts rowrec = carve_parts(b, rows);
@@ -830,9 +845,9 @@ nrk_status seg_reduce(const Chain &chain, const T *x, size_t rows, size_t cols, 
 }
 
 template nrk_status seg_reduce<float>(const Chain &, const float *, size_t, size_t, double, const nrk_reduce_outputs *,
-                                      const nrk_reduce_outputs *, hipStream_t);
+                                      const nrk_reduce_outputs *, hipStream_t, const ExtScratch *);
 template nrk_status seg_reduce<double>(const Chain &, const double *, size_t, size_t, double, const nrk_reduce_outputs *,
-                                       const nrk_reduce_outputs *, hipStream_t);
+                                       const nrk_reduce_outputs *, hipStream_t, const ExtScratch *);
 
 // ---------------- i64 sum: wrapping integer adds, exact and order-independent ----------------
 constexpr int I64_VPT = 8;                 // 16-byte vectors per thread per segment

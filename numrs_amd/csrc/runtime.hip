@@ -40,7 +40,14 @@ struct DeviceInfo {
     bool checked = false;
     bool ok = false;
     int cus = 0;
+    void *scratch = nullptr;  // this device's copy of `scratch` (below)
 };
+
+// Fixed device scratch for nrk_dev_map_reduce_f64_noalloc, whose callers cannot allocate. Each
+// device holds its own copy until the process ends. In host code `scratch` is only a handle: its
+// device address comes from hipGetSymbolAddress with that device current (nrk_init), and only that
+// address is ever passed to kernels.
+static __device__ __align__(256) unsigned char scratch[NRK_NOALLOC_SCRATCH_BYTES];
 static std::mutex g_dev_mutex;
 static std::map<int, DeviceInfo> g_devices;
 
@@ -60,6 +67,28 @@ static nrk_status check_device(int device) {
     info.ok = std::strncmp(prop.gcnArchName, "gfx950", 6) == 0;
     info.cus = prop.multiProcessorCount;
     if (!info.ok) return fail(NRK_ERR_NO_DEVICE, "device %d is %s, this library is built for gfx950", device, prop.gcnArchName);
+    return NRK_OK;
+}
+
+// Looks up this device's address of `scratch` once (the device must be current).
+static nrk_status init_static_scratch(int device) {
+    std::lock_guard<std::mutex> lock(g_dev_mutex);
+    DeviceInfo &info = g_devices[device];
+    if (info.scratch) return NRK_OK;
+    void *p = nullptr;
+    NRK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(NRK_ERR_HIP, "hipGetSymbolAddress gave no address for the fixed scratch of device %d", device);
+    info.scratch = p;
+    return NRK_OK;
+}
+
+nrk_status static_scratch(ExtScratch *out) {
+    NRK_TRY(ensure_device());
+    std::lock_guard<std::mutex> lock(g_dev_mutex);
+    auto it = g_devices.find(g_thread_device);
+    if (it == g_devices.end() || !it->second.scratch) return fail(NRK_ERR_HIP, "fixed scratch of device %d is not set up", g_thread_device);
+    out->base = it->second.scratch;
+    out->bytes = NRK_NOALLOC_SCRATCH_BYTES;
     return NRK_OK;
 }
 
@@ -191,6 +220,7 @@ extern "C" {
 nrk_status nrk_init(int device) {
     NRK_TRY(check_device(device));
     NRK_HIP(hipSetDevice(device));
+    NRK_TRY(init_static_scratch(device));
     g_thread_device = device;
     return NRK_OK;
 }
