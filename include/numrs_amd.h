@@ -187,7 +187,14 @@ enum {
     NRK_BOP_LOGADDEXP = 10, /* ufuncs.rs:916 m + ln(e^-|a-m| + e^-|b-m|), within 2 ulp */
     NRK_BOP_PRELU = 11,    /* ufuncs.rs:1394 a > 0 ? a : b * a (b = alpha)             */
     NRK_BOP_POWER = 12,    /* ufuncs.rs:166 a.powf(b), within 1 ulp                      */
-    NRK_BOP_COUNT = 13
+    NRK_BOP_COUNT = 13,    /* the floating-point ops end here; the rest are integer-only  */
+    /* integer-only (bitwise_ops.rs:30-243): &, |, ^, and shifts whose amount is masked to the
+     * width (Rust's release arithmetic); >> is arithmetic for signed types, logical otherwise */
+    NRK_BOP_BITAND = 13,
+    NRK_BOP_BITOR = 14,
+    NRK_BOP_BITXOR = 15,
+    NRK_BOP_SHL = 16,
+    NRK_BOP_SHR = 17
 };
 
 /* kernels/elementwise.rs:237-258 unary_serial / unary_dispatch with an op-coded closure:
@@ -203,6 +210,63 @@ nrk_status nrk_zip_f64(int32_t op, const double *a, const double *b, double *y, 
 nrk_status nrk_zip_f32(int32_t op, const float *a, const float *b, float *y, size_t n);
 nrk_status nrk_dev_zip_f64(int32_t op, const double *a, const double *b, double *y, size_t n, nrk_stream stream);
 nrk_status nrk_dev_zip_f32(int32_t op, const float *a, const float *b, float *y, size_t n, nrk_stream stream);
+
+/* kernels/elementwise.rs:217-258 binary_dispatch / unary_dispatch for the integer T (i64, i32,
+ * u64, u32) with array/arithmetic.rs's operators and bitwise_ops.rs's. Rust's release arithmetic:
+ * + - * and negation wrap; / and % truncate; shift amounts are masked to the width. Rust panics on
+ * x / 0, x % 0 and MIN / -1: the host forms check first and return NRK_ERR_INVALID_ARGUMENT (or
+ * NRK_ERR_UNSUPPORTED where the check would need the data of a scalar-divisor chain) without
+ * writing; the device forms give x / 0 = x % 0 = 0, MIN / -1 = MIN, MIN % -1 = 0.
+ * zip ops: NRK_BOP_ADD SUB MUL DIV REM MAXIMUM MINIMUM BITAND BITOR BITXOR SHL SHR.
+ * map steps: NRK_SOP_ADD SUB MUL DIV RSUB RDIV NEG ABS SQUARE SIGN CLAMP_LO CLAMP_HI (NEG, ABS
+ * and SIGN only for the signed types), with an integer scalar. */
+typedef struct nrk_int_op {
+    int32_t op;       /* NRK_SOP_* */
+    int32_t reserved; /* 0 */
+    int64_t scalar;   /* the step's integer operand (converted to the array's type) */
+} nrk_int_op;
+nrk_status nrk_zip_i64(int32_t op, const int64_t *a, const int64_t *b, int64_t *y, size_t n);
+nrk_status nrk_zip_i32(int32_t op, const int32_t *a, const int32_t *b, int32_t *y, size_t n);
+nrk_status nrk_zip_u64(int32_t op, const uint64_t *a, const uint64_t *b, uint64_t *y, size_t n);
+nrk_status nrk_zip_u32(int32_t op, const uint32_t *a, const uint32_t *b, uint32_t *y, size_t n);
+nrk_status nrk_dev_zip_i64(int32_t op, const int64_t *a, const int64_t *b, int64_t *y, size_t n, nrk_stream stream);
+nrk_status nrk_dev_zip_i32(int32_t op, const int32_t *a, const int32_t *b, int32_t *y, size_t n, nrk_stream stream);
+nrk_status nrk_dev_zip_u64(int32_t op, const uint64_t *a, const uint64_t *b, uint64_t *y, size_t n, nrk_stream stream);
+nrk_status nrk_dev_zip_u32(int32_t op, const uint32_t *a, const uint32_t *b, uint32_t *y, size_t n, nrk_stream stream);
+nrk_status nrk_map_i64(const nrk_int_op *chain, int32_t chain_len, const int64_t *x, int64_t *y, size_t n);
+nrk_status nrk_map_i32(const nrk_int_op *chain, int32_t chain_len, const int32_t *x, int32_t *y, size_t n);
+nrk_status nrk_map_u64(const nrk_int_op *chain, int32_t chain_len, const uint64_t *x, uint64_t *y, size_t n);
+nrk_status nrk_map_u32(const nrk_int_op *chain, int32_t chain_len, const uint32_t *x, uint32_t *y, size_t n);
+nrk_status nrk_dev_map_i64(const nrk_int_op *chain, int32_t chain_len, const int64_t *x, int64_t *y, size_t n,
+                           nrk_stream stream);
+nrk_status nrk_dev_map_i32(const nrk_int_op *chain, int32_t chain_len, const int32_t *x, int32_t *y, size_t n,
+                           nrk_stream stream);
+nrk_status nrk_dev_map_u64(const nrk_int_op *chain, int32_t chain_len, const uint64_t *x, uint64_t *y, size_t n,
+                           nrk_stream stream);
+nrk_status nrk_dev_map_u32(const nrk_int_op *chain, int32_t chain_len, const uint32_t *x, uint32_t *y, size_t n,
+                           nrk_stream stream);
+
+/* Element types of the dtype-generic entry points below. */
+enum { NRK_DTYPE_F64 = 0, NRK_DTYPE_F32 = 1, NRK_DTYPE_I64 = 2, NRK_DTYPE_I32 = 3, NRK_DTYPE_U64 = 4, NRK_DTYPE_U32 = 5 };
+#define NRK_MAX_NDIM 8
+
+/* operations.rs:370-440 add/subtract/multiply/divide_broadcast (Array::broadcast_op + binary_serial):
+ * y = a op b over numpy's broadcast of a's and b's shapes (right-aligned; extents equal or 1; at
+ * most NRK_MAX_NDIM axes), without materialising either operand at the common shape. y is
+ * contiguous at the broadcast shape; a, b and y are host pointers; shapes that do not broadcast
+ * are NRK_ERR_INVALID_ARGUMENT (the reference's ShapeMismatch). Floating-point ops: NRK_BOP_ADD ..
+ * NRK_BOP_COPYSIGN and PRELU; integer ops as for nrk_zip_i64 (divisors checked as there). */
+nrk_status nrk_zip_broadcast(int32_t dtype, int32_t op, const void *a, int32_t a_ndim, const size_t *a_shape, const void *b,
+                             int32_t b_ndim, const size_t *b_shape, void *y);
+/* The device form, over any strides: y (contiguous, `shape`) = a op b with a[i] at
+ * sum_d i_d a_strides[d] (elements; 0 broadcasts an axis, negative strides are allowed), likewise
+ * b. This serves broadcasting, non-contiguous views (no borrow::operand copy) and binary_serial's
+ * heterogeneous operand types (kernels/elementwise.rs:51-66): for NRK_BOP_SHL / SHR the shift
+ * amounts b may have any integer dtype b_dtype (bitwise_ops.rs:190-243 left_shift<T, U>); every
+ * other op needs b_dtype == dtype. */
+nrk_status nrk_dev_zip_strided(int32_t dtype, int32_t op, int32_t ndim, const size_t *shape, const void *a,
+                               const int64_t *a_strides, const void *b, int32_t b_dtype, const int64_t *b_strides, void *y,
+                               nrk_stream stream);
 
 /* ------------------------------------------------------------------------------------------ */
 /* Reductions over a flat slice                                                                */
@@ -425,6 +489,20 @@ nrk_status nrk_gemm_2d_f32(size_t m, size_t k, size_t n, const float *a, size_t 
                            float *c, size_t c_len);
 /* Device forms: plain FP64 / FP32 on MFMA (v_mfma_f64_16x16x4 / v_mfma_f32_16x16x4). */
 nrk_status nrk_dev_gemm_f64(size_t m, size_t k, size_t n, const double *a, const double *b, double *c, nrk_stream stream);
+/* kernels/gemm.rs:408-445 gemm_generic<T> for the integer types (Array<i32/i64>::matmul): every
+ * output accumulated from 0 in ascending k as acc + a * b, wrapping like Rust's release
+ * arithmetic, so the result is exact modulo 2^32 / 2^64 (the same bits for u32 / u64 data).
+ * Host form: the reference's slice contract -- C is zeroed over c_len; a, b or c shorter than
+ * m*k, k*n, m*n leaves it zeroed; otherwise the leading parts are multiplied. Device form: `batch`
+ * panels (C panels of m*n; a panel stride of 0 broadcasts that operand, as matmul's batch axes). */
+nrk_status nrk_gemm_generic_i64(size_t m, size_t k, size_t n, const int64_t *a, size_t a_len, const int64_t *b,
+                                size_t b_len, int64_t *c, size_t c_len);
+nrk_status nrk_gemm_generic_i32(size_t m, size_t k, size_t n, const int32_t *a, size_t a_len, const int32_t *b,
+                                size_t b_len, int32_t *c, size_t c_len);
+nrk_status nrk_dev_gemm_generic_i64(size_t batch, size_t m, size_t k, size_t n, const int64_t *a, size_t stride_a,
+                                    const int64_t *b, size_t stride_b, int64_t *c, nrk_stream stream);
+nrk_status nrk_dev_gemm_generic_i32(size_t batch, size_t m, size_t k, size_t n, const int32_t *a, size_t stride_a,
+                                    const int32_t *b, size_t stride_b, int32_t *c, nrk_stream stream);
 nrk_status nrk_dev_gemm_f32(size_t m, size_t k, size_t n, const float *a, const float *b, float *c, nrk_stream stream);
 
 /* array/linalg.rs:56-136 batched_gemm<T>(batch, m, k, n, a, b, c) for T = f64 / f32 (the

@@ -122,6 +122,20 @@ _SIGS: dict[str, list] = {
     # fused
     "nrk_dev_map_reduce_f64": [vp, c_i32, vp, c_size, c_size, c_size, vp, vp, vp],
     "nrk_dev_map_reduce_f64_noalloc": [vp, c_i32, vp, c_size, c_size, c_size, vp, vp, vp],
+    "nrk_zip_i64": [c_i32, vp, vp, vp, c_size], "nrk_dev_zip_i64": [c_i32, vp, vp, vp, c_size, vp],
+    "nrk_map_i64": [vp, c_i32, vp, vp, c_size], "nrk_dev_map_i64": [vp, c_i32, vp, vp, c_size, vp],
+    "nrk_zip_i32": [c_i32, vp, vp, vp, c_size], "nrk_dev_zip_i32": [c_i32, vp, vp, vp, c_size, vp],
+    "nrk_map_i32": [vp, c_i32, vp, vp, c_size], "nrk_dev_map_i32": [vp, c_i32, vp, vp, c_size, vp],
+    "nrk_zip_u64": [c_i32, vp, vp, vp, c_size], "nrk_dev_zip_u64": [c_i32, vp, vp, vp, c_size, vp],
+    "nrk_map_u64": [vp, c_i32, vp, vp, c_size], "nrk_dev_map_u64": [vp, c_i32, vp, vp, c_size, vp],
+    "nrk_zip_u32": [c_i32, vp, vp, vp, c_size], "nrk_dev_zip_u32": [c_i32, vp, vp, vp, c_size, vp],
+    "nrk_map_u32": [vp, c_i32, vp, vp, c_size], "nrk_dev_map_u32": [vp, c_i32, vp, vp, c_size, vp],
+    "nrk_zip_broadcast": [c_i32, c_i32, vp, c_i32, vp, vp, c_i32, vp, vp],
+    "nrk_dev_zip_strided": [c_i32, c_i32, c_i32, vp, vp, vp, vp, c_i32, vp, vp, vp],
+    "nrk_gemm_generic_i64": [c_size, c_size, c_size, vp, c_size, vp, c_size, vp, c_size],
+    "nrk_gemm_generic_i32": [c_size, c_size, c_size, vp, c_size, vp, c_size, vp, c_size],
+    "nrk_dev_gemm_generic_i64": [c_size, c_size, c_size, c_size, vp, c_size, vp, c_size, vp, vp],
+    "nrk_dev_gemm_generic_i32": [c_size, c_size, c_size, c_size, vp, c_size, vp, c_size, vp, vp],
     "nrk_dev_map_reduce_f32": [vp, c_i32, vp, c_size, c_size, c_size, vp, vp, vp],
     # gemm
     "nrk_gemm_2d_f64": [c_size, c_size, c_size, vp, c_size, vp, c_size, vp, c_size],

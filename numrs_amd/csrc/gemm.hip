@@ -266,11 +266,9 @@ nrk_status host_gemm(size_t m, size_t k, size_t n, const T *a, size_t a_len, con
     NRK_TRY(ensure_device());
     if (c_len > 0 && !c) return fail(NRK_ERR_INVALID_ARGUMENT, "c is NULL");
     const bool lens_ok = a_len == m * k && b_len == k * n && c_len == m * n;
-    if (!lens_ok) {
-        // gemm.rs:250-253 -> gemm_generic's defensive branch (gemm.rs:408-423): zero C, read nothing
-        for (size_t i = 0; i < c_len; ++i) c[i] = (T)0;
-        return NRK_OK;
-    }
+    // gemm.rs:250-253: any other length goes to gemm_generic (gemm.rs:408-445), which zeroes C and
+    // then, if every slice is long enough, multiplies the leading parts in its own k order
+    if (!lens_ok) return host_gemm_generic<T>(m, k, n, a, a_len, b, b_len, c, c_len);
     if (m == 0 || n == 0) return NRK_OK;
     void *da = nullptr, *db = nullptr, *dc = nullptr;
     NRK_TRY(staging(0, a_len * sizeof(T) + 16, &da));

@@ -72,6 +72,17 @@ nrk_status tri(int op, const T *a, const T *b, const T *c, T t, T *y, size_t n, 
 template <typename T>
 nrk_status gemm(size_t m, size_t k, size_t n, const T *a, const T *b, T *c, hipStream_t stream);
 
+// gemm_generic's arithmetic (ascending k, acc + a * b in T; integers wrap) over `batch` panels with
+// element strides sa, sb (0 broadcasts) and C panels of m*n (gemm_generic.hip); T = f32, f64, i32,
+// i64. host_gemm_generic is the reference's host-slice contract: C zeroed over c_len, computed
+// only when every slice is long enough.
+template <typename T>
+nrk_status gemm_generic(size_t batch, size_t m, size_t k, size_t n, const T *a, size_t sa, const T *b, size_t sb, T *c,
+                        hipStream_t stream);
+template <typename T>
+nrk_status host_gemm_generic(size_t m, size_t k, size_t n, const T *a, size_t a_len, const T *b, size_t b_len, T *c,
+                             size_t c_len);
+
 // Split-precision GEMMs with an explicit tile configuration (-1: built-in choice):
 // FP32 as BF16 split products (gemm_bf16.hip), FP64 by the Ozaki scheme II (gemm_ozaki.hip).
 nrk_status gemm_f32_bf16x(size_t m, size_t k, size_t n, const float *a, const float *b, float *c, int products, int tile,

@@ -15,6 +15,7 @@ goes to the GPU through libnumrs_amd.so; there is no CPU path here.
 """
 from __future__ import annotations
 
+import builtins
 import ctypes as C
 
 import numpy as np
@@ -31,7 +32,11 @@ SOP = {"add": 0, "sub": 1, "mul": 2, "div": 3, "rsub": 4, "rdiv": 5, "neg": 6, "
        "logit": 47, "sinc": 48, "smoothstep": 49, "smootherstep": 50, "erf": 51, "erfc": 52,
        "digamma": 53, "pow": 54, "gamma": 55, "lgamma": 56, "cbrt": 57}
 BOP = {"add": 0, "sub": 1, "mul": 2, "div": 3, "rem": 4, "maximum": 5, "minimum": 6, "copysign": 7, "hypot": 8,
-       "atan2": 9, "logaddexp": 10, "prelu": 11, "power": 12}
+       "atan2": 9, "logaddexp": 10, "prelu": 11, "power": 12,
+       "bitand": 13, "bitor": 14, "bitxor": 15, "shl": 16, "shr": 17}  # the last five: integers only
+DTYPE = {np.dtype(np.float64): 0, np.dtype(np.float32): 1, np.dtype(np.int64): 2, np.dtype(np.int32): 3,
+         np.dtype(np.uint64): 4, np.dtype(np.uint32): 5}
+_INT_SUF = {np.dtype(np.int64): "i64", np.dtype(np.int32): "i32", np.dtype(np.uint64): "u64", np.dtype(np.uint32): "u32"}
 AXIS = {"sum": 0, "mean": 1, "var": 2, "min": 3, "max": 4, "argmin": 5, "argmax": 6}
 
 PARALLEL_MIN_LEN = 10_000  # kernels/mod.rs:74 (kept for callers; the GPU has no CPU tiers)
@@ -459,6 +464,105 @@ def binary_f32(a, b, op: str) -> np.ndarray:
     return _binary("nrk_zip_f32", a, b, op, np.float32)
 
 
+class IntOp(C.Structure):
+    """nrk_int_op"""
+    _fields_ = [("op", C.c_int32), ("reserved", C.c_int32), ("scalar", C.c_int64)]
+
+
+def binary_dispatch(a, b, op: str) -> np.ndarray:
+    """binary_dispatch (elementwise.rs:217-235) for any T the library runs: f64 / f32 through the
+    float zip, i64 / i32 / u64 / u32 through the integer zip (wrapping, truncating division,
+    masked shifts; a zero divisor raises like the reference's panic). Equal lengths."""
+    a, b = np.ascontiguousarray(a), np.ascontiguousarray(b)
+    if a.dtype in (np.float64, np.float32):
+        return _binary("nrk_zip_f64" if a.dtype == np.float64 else "nrk_zip_f32", a, b, op, a.dtype)
+    suf = _INT_SUF.get(a.dtype)
+    if suf is None:
+        raise TypeError(f"no integer kernel for {a.dtype}")
+    return _binary(f"nrk_zip_{suf}", a, b, op, a.dtype)
+
+
+def unary_dispatch_int(a, steps) -> np.ndarray:
+    """unary_dispatch (elementwise.rs:237-258) for the integer T: steps like ("mul", 3), ("neg",)
+    with integer scalars; + - * and negation wrap as Rust's release arithmetic does."""
+    a = np.ascontiguousarray(a)
+    suf = _INT_SUF.get(a.dtype)
+    if suf is None:
+        raise TypeError(f"no integer kernel for {a.dtype}")
+    arr = (IntOp * max(1, len(steps)))()
+    for i, st in enumerate(steps):
+        arr[i].op = SOP[st[0]]
+        arr[i].reserved = 0
+        arr[i].scalar = int(st[1]) if len(st) > 1 else 0
+    out = np.empty_like(a)
+    call(f"nrk_map_{suf}", C.cast(arr, C.c_void_p), C.c_int32(len(steps)), _ptr(a), _ptr(out), C.c_size_t(a.size))
+    return out
+
+
+def _broadcast_op(a, b, op: str) -> np.ndarray:
+    """Array::broadcast_op + binary_serial (operations.rs:260-284, 370-440): y = a op b over the
+    numpy broadcast of the two shapes, one GPU pass (no broadcast_to copies)."""
+    a, b = np.ascontiguousarray(a), np.ascontiguousarray(b)
+    if a.dtype != b.dtype:
+        raise TypeError("operands of a broadcast op share one dtype")
+    if a.dtype not in DTYPE:
+        raise TypeError(f"no kernel for {a.dtype}")
+    try:
+        shape = np.broadcast_shapes(a.shape, b.shape)
+    except ValueError as e:
+        raise ValueError(f"shape mismatch: {a.shape} vs {b.shape}") from e
+    out = np.empty(shape, a.dtype)
+    sa = (C.c_size_t * max(1, a.ndim))(*a.shape)
+    sb = (C.c_size_t * max(1, b.ndim))(*b.shape)
+    call("nrk_zip_broadcast", C.c_int32(DTYPE[a.dtype]), C.c_int32(BOP[op]), _ptr(a), C.c_int32(a.ndim), sa, _ptr(b),
+         C.c_int32(b.ndim), sb, _ptr(out))
+    return out
+
+
+def add_broadcast(a, b): return _broadcast_op(a, b, "add")          # operations.rs:370
+def subtract_broadcast(a, b): return _broadcast_op(a, b, "sub")     # operations.rs:392
+def multiply_broadcast(a, b): return _broadcast_op(a, b, "mul")     # operations.rs:414
+def divide_broadcast(a, b): return _broadcast_op(a, b, "div")       # operations.rs:436
+
+
+def zip_strided(op: str, a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """nrk_dev_zip_strided on numpy views as they are (any strides, 0 for broadcast axes), so a
+    transposed or sliced operand is read in place; b may be another integer dtype for shifts
+    (bitwise_ops.rs:190-243 left_shift<T, U> / right_shift). The result has a's dtype."""
+    from . import device
+
+    a, b = np.asarray(a), np.asarray(b)
+    shape = np.broadcast_shapes(a.shape, b.shape)
+    av, bv = np.broadcast_to(a, shape), np.broadcast_to(b, shape)
+    out = np.empty(shape, a.dtype)
+    if out.size == 0:
+        return out
+
+    def upload(v):  # the smallest buffer that holds every element the view can address
+        lo = builtins.sum(builtins.min(0, st * (n - 1)) for st, n in zip(v.strides, v.shape))
+        hi = builtins.sum(builtins.max(0, st * (n - 1)) for st, n in zip(v.strides, v.shape)) + v.itemsize
+        raw = np.ctypeslib.as_array((C.c_uint8 * (hi - lo)).from_address(v.ctypes.data + lo)).copy()
+        d = device.DeviceArray((raw.size,), np.uint8)
+        d.fill_from(raw)
+        return d, -lo
+
+    da, offa = upload(av)
+    db, offb = upload(bv)
+    dy = device.DeviceArray(shape, a.dtype)
+    nd = len(shape)
+    shp = (C.c_size_t * max(1, nd))(*shape)
+    stra = (C.c_int64 * max(1, nd))(*[st // av.itemsize for st in av.strides])
+    strb = (C.c_int64 * max(1, nd))(*[st // bv.itemsize for st in bv.strides])
+    call("nrk_dev_zip_strided", C.c_int32(DTYPE[a.dtype]), C.c_int32(BOP[op]), C.c_int32(nd), shp,
+         C.c_void_p(da.ptr + offa), stra, C.c_void_p(db.ptr + offb), C.c_int32(DTYPE[b.dtype]), strb, C.c_void_p(dy.ptr), None)
+    device.synchronize()
+    return dy.download()
+
+
+def left_shift(a, b): return zip_strided("shl", a, b)     # bitwise_ops.rs:190
+def right_shift(a, b): return zip_strided("shr", a, b)    # bitwise_ops.rs:228
+
+
 def _binary(fn, a, b, op, dtype):
     a = _vec(a, dtype)
     b = _vec(b, dtype)
@@ -498,7 +602,8 @@ def batched_gemm(batch: int, m: int, k: int, n: int, a, b, c: np.ndarray) -> Non
 
 
 def matmul(a: np.ndarray, b: np.ndarray) -> np.ndarray:
-    """Array::matmul (array/linalg.rs:150-250) for f64 / f32: 2-D x 2-D is gemm_2d; a 1-D operand
+    """Array::matmul (array/linalg.rs:150-250) for f64 / f32 (MFMA) and the integer types
+    (gemm_generic's wrapping kernel, gemm.rs:408-445): 2-D x 2-D is gemm_2d; a 1-D operand
     is promoted to [1, k] (left) or [k, 1] (right) and the promoted axis stays in the result;
     batch axes (all but the last two) broadcast numpy-style. Instead of broadcast_to copies, an
     operand whose batch axes are all broadcast is passed once with panel stride 0; batch shapes
@@ -510,9 +615,10 @@ def matmul(a: np.ndarray, b: np.ndarray) -> np.ndarray:
     a = np.asarray(a)
     b = np.asarray(b)
     dtype = np.result_type(a.dtype, b.dtype)
-    if dtype not in (np.float64, np.float32):
-        raise TypeError("matmul runs f64 and f32 on the GPU")
-    if a.ndim == 2 and b.ndim == 2:
+    cdt = _compute_dtype(dtype)  # integers: gemm_generic in the 32- or 64-bit wrapping kernel
+    if cdt != dtype:
+        return matmul(a.astype(cdt), b.astype(cdt)).astype(dtype)
+    if a.ndim == 2 and b.ndim == 2 and dtype in (np.float64, np.float32):
         if a.shape[1] != b.shape[0]:
             raise ValueError(f"shape mismatch: {a.shape} x {b.shape}")
         c = np.empty((a.shape[0], b.shape[1]), dtype)
@@ -539,7 +645,8 @@ def matmul(a: np.ndarray, b: np.ndarray) -> np.ndarray:
         return out
     a = np.ascontiguousarray(a, dtype)
     b = np.ascontiguousarray(b, dtype)
-    fn = {np.dtype(np.float64): "nrk_dev_batched_gemm_f64", np.dtype(np.float32): "nrk_dev_batched_gemm_f32"}[np.dtype(dtype)]
+    fn = {np.dtype(np.float64): "nrk_dev_batched_gemm_f64", np.dtype(np.float32): "nrk_dev_batched_gemm_f32",
+          np.dtype(np.int64): "nrk_dev_gemm_generic_i64", np.dtype(np.int32): "nrk_dev_gemm_generic_i32"}[np.dtype(dtype)]
     da, db = device.DeviceArray.upload(a.reshape(-1)), device.DeviceArray.upload(b.reshape(-1))
     dc = device.DeviceArray((out.size,), dtype)
     isz = np.dtype(dtype).itemsize
@@ -567,6 +674,34 @@ def matmul(a: np.ndarray, b: np.ndarray) -> np.ndarray:
     res = dc.download().reshape(out.shape)
     del da, db, dc
     return res
+
+
+def _compute_dtype(dtype) -> np.dtype:
+    """The kernel a matmul of `dtype` runs in: f64 / f32 natively; integers in the wrapping 64-bit
+    (64-bit types) or 32-bit (narrower types) gemm_generic kernel -- the product modulo 2^32 taken
+    modulo 2^w is the product modulo 2^w, so casting back gives the reference's wrapped w-bit
+    result. Anything else is refused."""
+    dtype = np.dtype(dtype)
+    if dtype in (np.float64, np.float32):
+        return dtype
+    if np.issubdtype(dtype, np.integer):
+        return np.dtype(np.int64) if dtype.itemsize == 8 else np.dtype(np.int32)
+    raise TypeError(f"matmul has no GPU kernel for {dtype}")
+
+
+def gemm_generic(m: int, k: int, n: int, a, b, c: np.ndarray) -> None:
+    """gemm_generic (gemm.rs:408-445) for integer c (i64 / u64 in the 64-bit kernel, i32 / u32
+    in the 32-bit one): c is zeroed over its own length; if a, b or c is shorter than m*k, k*n,
+    m*n it stays zero, otherwise the leading parts are multiplied, wrapping."""
+    if not isinstance(c, np.ndarray) or not c.flags.c_contiguous:
+        raise TypeError("c must be a C-contiguous numpy array")
+    if c.dtype.itemsize not in (4, 8) or not np.issubdtype(c.dtype, np.integer):
+        raise TypeError("gemm_generic takes 32- or 64-bit integers")
+    w = np.int64 if c.dtype.itemsize == 8 else np.int32
+    a = np.ascontiguousarray(np.asarray(a).astype(c.dtype, copy=False)).reshape(-1).view(w)
+    b = np.ascontiguousarray(np.asarray(b).astype(c.dtype, copy=False)).reshape(-1).view(w)
+    call("nrk_gemm_generic_i64" if w == np.int64 else "nrk_gemm_generic_i32", C.c_size_t(m), C.c_size_t(k), C.c_size_t(n),
+         _ptr(a), C.c_size_t(a.size), _ptr(b), C.c_size_t(b.size), _ptr(c), C.c_size_t(c.size))
 
 
 def gemm_f32_bf16x(a: np.ndarray, b: np.ndarray, products: int = 3) -> np.ndarray:

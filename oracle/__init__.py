@@ -60,8 +60,9 @@ def lib() -> C.CDLL:
             f = getattr(L, f"ref_gemm_generic_{suf}")
             f.argtypes = [sz, sz, sz, vp, vp, vp]
             f.restype = None
-        L.ref_gemm_generic_i64.argtypes = [sz, sz, sz, vp, vp, vp]
-        L.ref_gemm_generic_i64.restype = None
+        for f in (L.ref_gemm_generic_i64, L.ref_gemm_generic_i32):
+            f.argtypes = [sz, sz, sz, vp, vp, vp]
+            f.restype = None
         L.ref_gemm_rowsplit_f64.argtypes = [sz, sz, sz, vp, vp, vp]
         L.ref_gemm_rowsplit_f64.restype = None
         L.dd_gemm_f64.argtypes = [sz, sz, sz, vp, vp, vp]
@@ -122,6 +123,27 @@ def ref_zip(op: int, a: np.ndarray, b: np.ndarray) -> np.ndarray:
     return y
 
 
+_ISUF = {np.dtype(np.int64): "i64", np.dtype(np.int32): "i32", np.dtype(np.uint64): "u64", np.dtype(np.uint32): "u32"}
+
+
+def ref_zip_int(op: int, a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """Integer binary_serial (elementwise.rs:51-66) under Rust's release arithmetic (oracle.c)."""
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b, a.dtype)
+    y = np.empty_like(a)
+    getattr(lib(), f"ref_zip_int_{_ISUF[a.dtype]}")(op, _p(a), _p(b), _p(y), C.c_size_t(a.size))
+    return y
+
+
+def ref_map_int(ops: list[int], scalars: list[int], x: np.ndarray) -> np.ndarray:
+    x = np.ascontiguousarray(x)
+    o = np.asarray(ops, np.int32)
+    s = np.asarray(scalars, np.int64)
+    y = np.empty_like(x)
+    getattr(lib(), f"ref_map_int_{_ISUF[x.dtype]}")(_p(o), _p(s), len(ops), _p(x), _p(y), C.c_size_t(x.size))
+    return y
+
+
 def ref_fma(a: np.ndarray, b: np.ndarray, c: np.ndarray) -> np.ndarray:
     a, b, c = (np.ascontiguousarray(v) for v in (a, b, c))
     y = np.empty_like(a)
@@ -174,7 +196,8 @@ def gemm_generic(a: np.ndarray, b: np.ndarray) -> np.ndarray:
     a = np.ascontiguousarray(a)
     b = np.ascontiguousarray(b)
     c = np.empty((m, n), a.dtype)
-    suf = {np.dtype(np.float64): "f64", np.dtype(np.float32): "f32", np.dtype(np.int64): "i64"}[a.dtype]
+    suf = {np.dtype(np.float64): "f64", np.dtype(np.float32): "f32", np.dtype(np.int64): "i64",
+           np.dtype(np.int32): "i32"}[a.dtype]
     getattr(lib(), f"ref_gemm_generic_{suf}")(m, k, n, _p(a), _p(b), _p(c))
     return c
 

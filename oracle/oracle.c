@@ -387,6 +387,62 @@ int64_t ref_sum_i64(const int64_t *a, size_t n) {
         }                                                                                   \
     }
 
+/* Integer restatement of binary_serial / unary_dispatch with arithmetic.rs's and bitwise_ops.rs's
+ * operators under Rust's release arithmetic: + - * and negation wrap (formed in the unsigned type
+ * of the same width); / and % truncate toward zero (C99 does too); a shift amount is masked to
+ * the width; >> is arithmetic for the signed types. The callers never pass a zero divisor or
+ * MIN / -1 (Rust panics on both). */
+#define DEFINE_INT_ELEM(T, U, SUF, BITS)                                                    \
+    static T iop_##SUF(int op, T p, T q) {                                                  \
+        switch (op) {                                                                       \
+            case 0: return (T)((U)p + (U)q);                                                \
+            case 1: return (T)((U)p - (U)q);                                                \
+            case 2: return (T)((U)p * (U)q);                                                \
+            case 3: return p / q;                                                           \
+            case 4: return p % q;                                                           \
+            case 5: return p > q ? p : q;                                                   \
+            case 6: return p < q ? p : q;                                                   \
+            case 13: return p & q;                                                          \
+            case 14: return p | q;                                                          \
+            case 15: return p ^ q;                                                          \
+            case 16: return (T)((U)p << ((U)q & (BITS - 1)));                               \
+            case 17: return (T)(p >> ((U)q & (BITS - 1)));                                  \
+            default: return p;                                                              \
+        }                                                                                   \
+    }                                                                                       \
+    void ref_zip_int_##SUF(int op, const T *a, const T *b, T *y, size_t n) {               \
+        for (size_t i = 0; i < n; ++i) y[i] = iop_##SUF(op, a[i], b[i]);                    \
+    }                                                                                       \
+    void ref_map_int_##SUF(const int32_t *ops, const int64_t *scalars, int len, const T *x, T *y, \
+                           size_t n) {                                                      \
+        for (size_t i = 0; i < n; ++i) {                                                    \
+            T v = x[i];                                                                     \
+            for (int k = 0; k < len; ++k) {                                                 \
+                const T s = (T)scalars[k];                                                  \
+                switch (ops[k]) {                                                           \
+                    case 0: v = iop_##SUF(0, v, s); break;                                  \
+                    case 1: v = iop_##SUF(1, v, s); break;                                  \
+                    case 2: v = iop_##SUF(2, v, s); break;                                  \
+                    case 3: v = v / s; break;                                               \
+                    case 4: v = iop_##SUF(1, s, v); break;                                  \
+                    case 5: v = s / v; break;                                               \
+                    case 6: v = (T)(0 - (U)v); break;                                       \
+                    case 7: v = v < 0 ? (T)(0 - (U)v) : v; break;                           \
+                    case 8: v = iop_##SUF(2, v, v); break;                                  \
+                    case 15: v = (T)(v > 0 ? 1 : (v == 0 ? 0 : -1)); break;                 \
+                    case 16: v = v < s ? s : v; break;                                      \
+                    case 17: v = v > s ? s : v; break;                                      \
+                    default: break;                                                         \
+                }                                                                           \
+            }                                                                               \
+            y[i] = v;                                                                       \
+        }                                                                                   \
+    }
+DEFINE_INT_ELEM(int64_t, uint64_t, i64, 64)
+DEFINE_INT_ELEM(int32_t, uint32_t, i32, 32)
+DEFINE_INT_ELEM(uint64_t, uint64_t, u64, 64)
+DEFINE_INT_ELEM(uint32_t, uint32_t, u32, 32)
+
 /* ufuncs.rs:864-891 fma: a.mul_add(b, c), one rounding (C's fma is the IEEE fusedMultiplyAdd);
  * ufuncs.rs:893-912 lerp: a + t * (b - a), three roundings */
 #define DEFINE_TRI(T, SUF, FMA)                                                             \
@@ -428,7 +484,31 @@ DEFINE_EW(float, f32, sqrtf, fmodf, fabsf, copysignf)
     }
 DEFINE_GEMM_GENERIC(double, f64)
 DEFINE_GEMM_GENERIC(float, f32)
-DEFINE_GEMM_GENERIC(int64_t, i64)
+/* Integers: Rust's release arithmetic wraps, so the restatement forms every product and sum in the
+ * unsigned type of the same width (defined wrap-around in C), with the same loop order. */
+#define DEFINE_GEMM_GENERIC_INT(T, U, SUF)                                                   \
+    void ref_gemm_generic_##SUF(size_t m, size_t k, size_t n, const T *a, const T *b, T *c) { \
+        U *cu = (U *)c;                                                                      \
+        for (size_t i = 0; i < m * n; ++i) cu[i] = 0;                                        \
+        const size_t BS = 64;                                                                \
+        for (size_t ib = 0; ib < m; ib += BS) {                                              \
+            size_t ie = ib + BS < m ? ib + BS : m;                                           \
+            for (size_t kb = 0; kb < k; kb += BS) {                                          \
+                size_t ke = kb + BS < k ? kb + BS : k;                                       \
+                for (size_t jb = 0; jb < n; jb += BS) {                                      \
+                    size_t je = jb + BS < n ? jb + BS : n;                                   \
+                    for (size_t i = ib; i < ie; ++i)                                         \
+                        for (size_t kk = kb; kk < ke; ++kk) {                                \
+                            U aik = (U)a[i * k + kk];                                        \
+                            for (size_t j = jb; j < je; ++j)                                 \
+                                cu[i * n + j] = (U)(cu[i * n + j] + (U)(aik * (U)b[kk * n + j])); \
+                        }                                                                    \
+                }                                                                            \
+            }                                                                                \
+        }                                                                                    \
+    }
+DEFINE_GEMM_GENERIC_INT(int64_t, uint64_t, i64)
+DEFINE_GEMM_GENERIC_INT(int32_t, uint32_t, i32)
 
 /* CPU-baseline stand-in for the reference's f64/f32 tier (gemm.rs:290-381): rows split into
  * ceil(m / threads) chunks across the pool, K never split; each chunk runs a packed, cache-

@@ -264,6 +264,14 @@ def test_gemm_generic_known_and_brute_force(oracle, known):
     ai = rng.integers(-50, 50, (70, 130)).astype(np.int64)
     bi = rng.integers(-50, 50, (130, 65)).astype(np.int64)
     np.testing.assert_array_equal(oracle.gemm_generic(ai, bi), ai @ bi)
+    # wrapping (Rust release arithmetic): numpy's integer matmul wraps the same way
+    r2 = np.random.default_rng(18)
+    for dt in (np.int64, np.int32):
+        info = np.iinfo(dt)
+        aw = r2.integers(info.min, info.max, (9, 70), dtype=dt, endpoint=True)
+        bw = r2.integers(info.min, info.max, (70, 11), dtype=dt, endpoint=True)
+        with np.errstate(over="ignore"):
+            np.testing.assert_array_equal(oracle.gemm_generic(aw, bw), aw @ bw)
     # f64: ordered k-sum == explicit loop
     af = rng.standard_normal((5, 70))
     bf = rng.standard_normal((70, 4))
@@ -378,3 +386,47 @@ def test_ufunc_fold_oracle_pins():
     assert _m.isnan(oracle.ref_ufunc_fold("maximum", [1.0, _m.nan, 3.0]))
     assert _m.isnan(oracle.ref_ufunc_fold("minimum", [_m.nan, 1.0]))
     assert oracle.ref_ufunc_fold("add", [float(i) for i in range(1000)]) == 1000 * 999 / 2
+
+
+def test_integer_elementwise_oracle_pins():
+    """ref_zip_int / ref_map_int (Rust release arithmetic): the reference's doc examples
+    (bitwise_ops.rs:25-28 and, or, xor; :186-189 left_shift; :224-227 right_shift), numpy's
+    wrapping + - * & | ^ on full-range values, Python's exact integers for truncated / and % and
+    for masked shifts."""
+    import fractions
+
+    import oracle
+
+    a = np.array([13, 17, 21], np.int64)
+    b = np.array([9, 7, 15], np.int64)
+    assert list(oracle.ref_zip_int(13, a, b)) == [9, 1, 5]
+    assert list(oracle.ref_zip_int(14, a, b)) == [13, 23, 31]
+    assert list(oracle.ref_zip_int(15, a, b)) == [4, 22, 26]
+    assert list(oracle.ref_zip_int(16, np.array([5, 10, 15], np.int32), np.array([1, 2, 3], np.int32))) == [10, 40, 120]
+    assert list(oracle.ref_zip_int(17, np.array([40, 80, 120], np.int32), np.array([1, 2, 3], np.int32))) == [20, 20, 15]
+    r = np.random.default_rng(61)
+    for dt, bits in ((np.int64, 64), (np.int32, 32), (np.uint64, 64), (np.uint32, 32)):
+        info = np.iinfo(dt)
+        x = r.integers(info.min, info.max, 2000, dtype=dt, endpoint=True)
+        y = r.integers(info.min, info.max, 2000, dtype=dt, endpoint=True)
+        with np.errstate(over="ignore"):
+            for op, f in ((0, np.add), (1, np.subtract), (2, np.multiply), (13, np.bitwise_and), (14, np.bitwise_or),
+                          (15, np.bitwise_xor), (5, np.maximum), (6, np.minimum)):
+                np.testing.assert_array_equal(oracle.ref_zip_int(op, x, y), f(x, y))
+        y[y == 0] = 3
+        if info.min < 0:
+            y[(y == -1) & (x == info.min)] = 5
+        q, rm = oracle.ref_zip_int(3, x, y), oracle.ref_zip_int(4, x, y)
+        for i in range(200):
+            tq = int(fractions.Fraction(int(x[i]), int(y[i])))  # truncates toward zero
+            assert int(q[i]) == tq and int(rm[i]) == int(x[i]) - int(y[i]) * tq
+        s = r.integers(-200, 200, 2000).astype(dt)
+        sh = oracle.ref_zip_int(16, x, s)
+        for i in range(200):
+            amt = int(s[i]) & (bits - 1)
+            want = (int(x[i]) << amt) & ((1 << bits) - 1)
+            assert int(sh[i].astype(np.uint64 if bits == 64 else np.uint32)) == want
+    v = np.array([-5, 0, 7, np.iinfo(np.int64).min], np.int64)
+    got = oracle.ref_map_int([2, 0, 6, 7], [3, 1, 0, 0], v)  # |-(3v + 1)|, wrapping
+    with np.errstate(over="ignore"):
+        np.testing.assert_array_equal(got, np.abs(-(v * 3 + 1)))
