@@ -475,6 +475,46 @@ nrk_status nrk_dev_map_reduce_f64_noalloc(const nrk_scalar_op *chain, int32_t ch
                                           size_t cols, size_t ddof, const nrk_reduce_outputs *row_out,
                                           const nrk_reduce_outputs *full_out, nrk_stream stream);
 
+/* expr/fused_eval.rs:421-697 expression fusion: an ExprNode tree (owned.rs:204-223) over up to
+ * NRK_MAX_LEAVES equally long operands, as a postfix program (children before their parent):
+ *   NRK_EXPR_LEAF   op = operand index                  (ExprNode::Leaf)
+ *   NRK_EXPR_STEP   op = NRK_SOP_ADD/SUB/MUL/DIV with `scalar` (ScalarRhs: x op s), RSUB / RDIV
+ *                   (ScalarLhs s - x, s / x; s + x and s * x are ADD / MUL), NEG, ABS, SQRT, EXP,
+ *                   LOG (UnaryOp::Neg/Abs/Sqrt/Exp/Ln)        -- applied to the top of the stack
+ *   NRK_EXPR_BINARY op = NRK_BOP_ADD/SUB/MUL/DIV (BinOp), also REM, MAXIMUM, MINIMUM, COPYSIGN,
+ *                   PRELU: pops r then l, pushes l op r    (ExprNode::Binary)
+ *   NRK_EXPR_FMA    pops c, b, a, pushes (a * b) + c with two roundings (ExprNode::Fma)
+ * Every node rounds once and nothing is contracted, so the result is eval_eager's to the bit;
+ * the whole tree is one pass (each operand read once, y written once). Limits: NRK_MAX_EXPR nodes,
+ * a stack of 4. */
+enum { NRK_EXPR_LEAF = 0, NRK_EXPR_STEP = 1, NRK_EXPR_BINARY = 2, NRK_EXPR_FMA = 3 };
+#define NRK_MAX_EXPR 16
+#define NRK_MAX_LEAVES 4
+typedef struct nrk_expr_node {
+    int32_t kind;  /* NRK_EXPR_* */
+    int32_t op;    /* see above */
+    double scalar; /* NRK_EXPR_STEP's operand */
+} nrk_expr_node;
+nrk_status nrk_expr_eval_f64(const nrk_expr_node *nodes, int32_t len, const double *const *leaves, int32_t n_leaves,
+                             double *y, size_t n);
+nrk_status nrk_expr_eval_f32(const nrk_expr_node *nodes, int32_t len, const float *const *leaves, int32_t n_leaves,
+                             float *y, size_t n);
+nrk_status nrk_dev_expr_eval_f64(const nrk_expr_node *nodes, int32_t len, const double *const *leaves, int32_t n_leaves,
+                                 double *y, size_t n, nrk_stream stream);
+nrk_status nrk_dev_expr_eval_f32(const nrk_expr_node *nodes, int32_t len, const float *const *leaves, int32_t n_leaves,
+                                 float *y, size_t n, nrk_stream stream);
+/* The expression feeding the fused reduction (as nrk_dev_map_reduce_*, leaves are (rows, cols)
+ * row-major): one pass over the operands when the tree is one operand with scalar steps, or two
+ * operands (each with scalar steps) combined by one binary op, followed by scalar steps; any other
+ * tree is evaluated into a workspace first. EXP / LOG steps are not allowed here
+ * (NRK_ERR_UNSUPPORTED), as for the map -> reduce chains. */
+nrk_status nrk_dev_expr_reduce_f64(const nrk_expr_node *nodes, int32_t len, const double *const *leaves, int32_t n_leaves,
+                                   size_t rows, size_t cols, size_t ddof, const nrk_reduce_outputs *row_out,
+                                   const nrk_reduce_outputs *full_out, nrk_stream stream);
+nrk_status nrk_dev_expr_reduce_f32(const nrk_expr_node *nodes, int32_t len, const float *const *leaves, int32_t n_leaves,
+                                   size_t rows, size_t cols, size_t ddof, const nrk_reduce_outputs *row_out,
+                                   const nrk_reduce_outputs *full_out, nrk_stream stream);
+
 /* ------------------------------------------------------------------------------------------ */
 /* GEMM: C := A * B, row-major, overwrite                                                      */
 /* ------------------------------------------------------------------------------------------ */

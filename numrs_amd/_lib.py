@@ -130,6 +130,11 @@ _SIGS: dict[str, list] = {
     "nrk_map_u64": [vp, c_i32, vp, vp, c_size], "nrk_dev_map_u64": [vp, c_i32, vp, vp, c_size, vp],
     "nrk_zip_u32": [c_i32, vp, vp, vp, c_size], "nrk_dev_zip_u32": [c_i32, vp, vp, vp, c_size, vp],
     "nrk_map_u32": [vp, c_i32, vp, vp, c_size], "nrk_dev_map_u32": [vp, c_i32, vp, vp, c_size, vp],
+    "nrk_expr_eval_f64": [vp, c_i32, vp, c_i32, vp, c_size], "nrk_expr_eval_f32": [vp, c_i32, vp, c_i32, vp, c_size],
+    "nrk_dev_expr_eval_f64": [vp, c_i32, vp, c_i32, vp, c_size, vp],
+    "nrk_dev_expr_eval_f32": [vp, c_i32, vp, c_i32, vp, c_size, vp],
+    "nrk_dev_expr_reduce_f64": [vp, c_i32, vp, c_i32, c_size, c_size, c_size, vp, vp, vp],
+    "nrk_dev_expr_reduce_f32": [vp, c_i32, vp, c_i32, c_size, c_size, c_size, vp, vp, vp],
     "nrk_zip_broadcast": [c_i32, c_i32, vp, c_i32, vp, vp, c_i32, vp, vp],
     "nrk_dev_zip_strided": [c_i32, c_i32, c_i32, vp, vp, vp, vp, c_i32, vp, vp, vp],
     "nrk_gemm_generic_i64": [c_size, c_size, c_size, vp, c_size, vp, c_size, vp, c_size],

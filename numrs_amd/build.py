@@ -50,13 +50,28 @@ def _run(cmd: list[str]) -> None:
         print(out, file=sys.stderr)
 
 
+def _includes(path: Path, seen: set) -> None:
+    """The project headers `path` includes, recursively (quoted includes found in csrc/ or include/)."""
+    for line in path.read_text().splitlines():
+        line = line.strip()
+        if not line.startswith('#include "'):
+            continue
+        name = line.split('"')[1]
+        for d in (CSRC, INCLUDE):
+            h = d / name
+            if h.exists() and h not in seen:
+                seen.add(h)
+                _includes(h, seen)
+
+
 def _stale(src: Path, obj: Path) -> bool:
-    """True if obj is missing or older than src or than any header it may include."""
+    """True if obj is missing or older than src, this recipe, or any project header src includes."""
     if not obj.exists():
         return True
     t = obj.stat().st_mtime
-    deps = [src, *CSRC.glob("*.hpp"), *INCLUDE.glob("*.h"), Path(__file__)]
-    return any(d.stat().st_mtime > t for d in deps)
+    deps: set = set()
+    _includes(src, deps)
+    return any(d.stat().st_mtime > t for d in [src, Path(__file__), *deps])
 
 
 def build_lib(verbose: bool = True, changed_only: bool = False) -> Path:

@@ -449,6 +449,22 @@ nrk_status nrk_dev_map_reduce_f32(const nrk_scalar_op *chain, int32_t chain_len,
                                   const nrk_reduce_outputs *row_out, const nrk_reduce_outputs *full_out, nrk_stream stream) {
     return dev_map_reduce<float>(chain, chain_len, x, rows, cols, ddof, row_out, full_out, S(stream));
 }
+nrk_status nrk_dev_expr_reduce_f64(const nrk_expr_node *nodes, int32_t len, const double *const *leaves, int32_t n_leaves,
+                                   size_t rows, size_t cols, size_t ddof, const nrk_reduce_outputs *row_out,
+                                   const nrk_reduce_outputs *full_out, nrk_stream stream) {
+    ExprProg pg;
+    NRK_TRY(compile_expr(nodes, len, &pg));
+    if (n_leaves < pg.n_leaves) return fail(NRK_ERR_INVALID_ARGUMENT, "the expression reads %d operands, %d given", pg.n_leaves, n_leaves);
+    return expr_reduce<double>(pg, leaves, rows, cols, (double)ddof, row_out, full_out, S(stream));
+}
+nrk_status nrk_dev_expr_reduce_f32(const nrk_expr_node *nodes, int32_t len, const float *const *leaves, int32_t n_leaves,
+                                   size_t rows, size_t cols, size_t ddof, const nrk_reduce_outputs *row_out,
+                                   const nrk_reduce_outputs *full_out, nrk_stream stream) {
+    ExprProg pg;
+    NRK_TRY(compile_expr(nodes, len, &pg));
+    if (n_leaves < pg.n_leaves) return fail(NRK_ERR_INVALID_ARGUMENT, "the expression reads %d operands, %d given", pg.n_leaves, n_leaves);
+    return expr_reduce<float>(pg, leaves, rows, cols, (double)ddof, row_out, full_out, S(stream));
+}
 // The same kernels on the device's fixed scratch: no hipMalloc / hipFree, ever.
 nrk_status nrk_dev_map_reduce_f64_noalloc(const nrk_scalar_op *chain, int32_t chain_len, const double *x, size_t rows,
                                           size_t cols, size_t ddof, const nrk_reduce_outputs *row_out,

@@ -16,6 +16,27 @@ nrk_status seg_reduce(const Chain &chain, const T *x, size_t rows, size_t cols, 
 // The fixed device scratch of the calling thread's device (set up by nrk_init, runtime.hip).
 nrk_status static_scratch(ExtScratch *out);
 
+// A compiled expression (expr.hip): postfix instructions over up to NRK_MAX_LEAVES operands with a
+// stack of at most EXPR_DEPTH values; FMA is (a * b) + c with two roundings.
+constexpr int EXPR_DEPTH = 4, EXPR_MAX_INS = NRK_MAX_EXPR;
+enum { EXPR_I_LEAF = 0, EXPR_I_STEP = 1, EXPR_I_BIN = 2, EXPR_I_FMA = 3 };
+struct ExprProg {
+    int len, n_leaves, math;  // math: an EXP or LOG step is present
+    int kind[EXPR_MAX_INS];
+    int op[EXPR_MAX_INS];
+    double s[EXPR_MAX_INS];
+};
+nrk_status compile_expr(const nrk_expr_node *nodes, int32_t len, ExprProg *out);
+template <typename T>
+nrk_status expr_eval(const ExprProg &pg, const T *const *leaves, T *y, size_t n, hipStream_t stream);
+// The fused reduction of an expression (reduce.hip): one pass when the program is one operand
+// with scalar steps, or two operands combined by one binary op, each with scalar steps (any
+// order of pushes), on the lean path; otherwise the expression is evaluated into the stream's
+// workspace first.
+template <typename T>
+nrk_status expr_reduce(const ExprProg &pg, const T *const *leaves, size_t rows, size_t cols, double ddof,
+                       const nrk_reduce_outputs *row_out, const nrk_reduce_outputs *full_out, hipStream_t stream);
+
 // Workgroups per CU of the segment reducer's persistent grid (result bits are independent of it).
 int reduce_blocks_per_cu();
 void set_reduce_blocks_per_cu(int b);

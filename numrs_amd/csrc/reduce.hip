@@ -35,7 +35,7 @@
 // the earlier element), then the larger value with ties to the lower index, and the value
 // written is the element itself, so even the sign of a zero maximum matches the reference's
 // left-to-right fold.
-#include "common.hpp"
+#include "ops.hpp"
 
 #include <algorithm>
 
@@ -748,13 +748,399 @@ static nrk_status launch_all(SegParams<T> &p, bool want_row, bool want_full, con
     return NRK_OK;
 }
 
+// ---------------- the lean kernel: full segments, contiguous chunk ranges ----------------
+// Used when every segment is whole (cols, or rows * cols for a whole-array reduction, is a
+// multiple of the segment), there are at least LEAN_MIN_CHUNKS chunks, and a chunk is a whole row
+// when row results are wanted. Then every chunk holds exactly cl full segments, and:
+//   - chunks are consecutive in memory (chunk c is elements [c cl WSEG, (c + 1) cl WSEG)), so a
+//     wavefront owns the contiguous range of its cpw chunks and walks it with one pointer,
+//     loading the next segment while it folds the current one;
+//   - every lane of a chunk has the same count (16 per segment), so the Chan updates need no
+//     division: the weight of the j-th segment's merge, 1 / (16 j (j + 1)), comes from a constant
+//     table, and the butterfly's levels merge equal counts with weight 1 / (2 n);
+//   - binary32 lanes add their segment sums into plain FP64 (exact to 2^-29 of a binary32 ulp);
+//     binary64 lanes keep the double word;
+//   - row results stay in registers (lane k holds the wavefront's k-th chunk) until the loop
+//     ends, so the loop issues loads only; the workgroup's 4 cpw chunks are merged in chunk order
+//     into one group record, and one finish kernel merges the group records in order.
+// The chunking depends only on the shape, so results are bit-identical across launches, grids
+// and devices. The input is an expression of one or two operands (see LeanProg).
+constexpr int LEAN_WTAB = 1024;  // chunks of at most this many segments
+struct LeanWTab {
+    double w[LEAN_WTAB];
+    constexpr LeanWTab() : w() {
+        for (int j = 1; j < LEAN_WTAB; ++j) w[j] = 1.0 / (16.0 * j * (j + 1));
+    }
+};
+static_assert(NRK_REDUCE_M == 16, "the Chan weight table assumes 16 elements per lane and segment");
+__constant__ LeanWTab kLeanW = LeanWTab();
+
+// y = post(pre0(x0) op pre1(x1)) with two operands, post(pre0(x0)) with one (post empty then).
+struct LeanProg {
+    Chain pre0, pre1, post;
+    int op;
+};
+
+template <typename T>
+struct LeanParams {
+    const T *x0, *x1;
+    size_t nchunk, cl, cols;
+    int cpw;          // chunks per wavefront; a workgroup (4 wavefronts) owns 4 cpw consecutive chunks
+    LeanProg prog;
+    double ddof;
+    int row_direct;   // a chunk is a row and row results are wanted
+    OutPtrs<T> row;
+    int need_group;   // whole-array results wanted: store one record per workgroup
+    Parts grp;
+};
+
+template <typename T, int N>
+__device__ __forceinline__ void bop_n(int op, T (&a)[N], const T (&b)[N]) {
+    switch (op) {
+#define NRK_BOPN(OP, EXPR)                                         \
+    case OP:                                                       \
+        _Pragma("unroll") for (int i = 0; i < N; ++i) a[i] = EXPR; \
+        break;
+        NRK_BOPN(NRK_BOP_ADD, rn_add(a[i], b[i]))
+        NRK_BOPN(NRK_BOP_SUB, rn_sub(a[i], b[i]))
+        NRK_BOPN(NRK_BOP_MUL, rn_mul(a[i], b[i]))
+        NRK_BOPN(NRK_BOP_DIV, rn_div(a[i], b[i]))
+        NRK_BOPN(NRK_BOP_REM, __builtin_fmod(a[i], b[i]))
+        NRK_BOPN(NRK_BOP_MAXIMUM, a[i] > b[i] ? a[i] : b[i])
+        NRK_BOPN(NRK_BOP_MINIMUM, a[i] < b[i] ? a[i] : b[i])
+        NRK_BOPN(NRK_BOP_COPYSIGN, dev_copysign(a[i], b[i]))
+        NRK_BOPN(NRK_BOP_PRELU, a[i] > (T)0 ? a[i] : rn_mul(b[i], a[i]))
+#undef NRK_BOPN
+        default: break;
+    }
+}
+
+// Issue the loads of one full segment: lane l takes elements (j * 64 + l) VN + e, as 16-byte
+// vectors when the operand is aligned (VEC), else one by one -- the same elements either way, so
+// alignment changes the speed only, never the bits.
+template <typename T, int VPT, bool VEC>
+__device__ __forceinline__ void lean_load(const T *base, T (&y)[VPT * Vec16<T>::N]) {
+    typedef typename Vec16<T>::type V;
+    constexpr int VN = Vec16<T>::N;
+    const unsigned lane = threadIdx.x & 63;
+    if constexpr (VEC) {
+        const V *bv = reinterpret_cast<const V *>(base) + lane;
+#pragma unroll
+        for (int j = 0; j < VPT; ++j) {
+            const V v = __builtin_nontemporal_load(bv + j * 64);
+#pragma unroll
+            for (int e = 0; e < VN; ++e) y[j * VN + e] = v[e];
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < VPT; ++j)
+#pragma unroll
+            for (int e = 0; e < VN; ++e) y[j * VN + e] = base[(j * 64 + lane) * VN + e];
+    }
+}
+
+// Fold segment j of the current chunk (loaded into y, and z for the second operand).
+template <typename T, int VPT, int FLAGS, int LEAVES>
+__device__ __forceinline__ void lean_fold(const LeanProg &pg, T (&y)[VPT * Vec16<T>::N], T (&z)[VPT * Vec16<T>::N],
+                                          unsigned j, LaneAcc<T> &a, unsigned &fnan) {
+    constexpr int M = VPT * Vec16<T>::N;
+    apply_chain_n(pg.pre0, y);
+    if constexpr (LEAVES == 2) {
+        apply_chain_n(pg.pre1, z);
+        bop_n(pg.op, y, z);
+        apply_chain_n(pg.post, y);
+    }
+    if (j == 0) fnan = __builtin_isnan(y[0]) ? 1u : 0u;  // lane 0's value is the chunk's first element
+    const T st = pairwise<M, 0, T>([&](int i) { return y[i]; });
+    if (__builtin_isnan(st)) {  // rare: tell a NaN element apart from inf - inf
+#pragma unroll
+        for (int i = 0; i < M; ++i) a.nan |= __builtin_isnan(y[i]) ? 1u : 0u;
+    }
+    const double sb = (double)st;
+    if (FLAGS & F_M2) {
+        const T mt = rn_mul(st, (T)(1.0 / M));  // the segment mean (exact: a power of two)
+        T q4[4] = {(T)0, (T)0, (T)0, (T)0};
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            const T d = rn_sub(y[i], mt);
+            q4[i & 3] = fma(d, d, q4[i & 3]);
+        }
+        const double qt = (double)rn_add(rn_add(q4[0], q4[1]), rn_add(q4[2], q4[3]));
+        if (j == 0) {
+            a.m2 = qt;
+        } else {  // Chan with n_a = 16 j, n_b = 16: (S_a - j S_b)^2 / (16 j (j + 1))
+            const double d = (a.hi + a.lo) - (double)j * sb;
+            a.m2 = a.m2 + qt + d * d * kLeanW.w[j];
+        }
+    }
+    if constexpr (sizeof(T) == 8) {
+        double s, e;
+        two_sum(a.hi, sb, s, e);
+        e = e + a.lo;
+        a.hi = s + e;
+        a.lo = e - (a.hi - s);
+    } else {
+        a.hi += sb;
+    }
+    if (FLAGS & F_MAX) {
+        unsigned ci = NO_CODE;
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            const bool gt = y[i] > a.vmax;
+            a.vmax = gt ? y[i] : a.vmax;
+            ci = gt ? (unsigned)i : ci;
+        }
+        a.cmax = ci != NO_CODE ? j * M + ci : a.cmax;
+    }
+    if (FLAGS & F_MIN) {
+        unsigned ci = NO_CODE;
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            const bool lt = y[i] < a.vmin;
+            a.vmin = lt ? y[i] : a.vmin;
+            ci = lt ? (unsigned)i : ci;
+        }
+        a.cmin = ci != NO_CODE ? j * M + ci : a.cmin;
+    }
+}
+
+// Butterfly of records of equal counts n (every lane of a chunk): M2 += (S_a - S_b)^2 / (2 n).
+template <typename T, int FLAGS>
+__device__ __forceinline__ void lean_wave_merge(Part &a) {
+    double w = 0.5 / a.n;
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        const Part b = part_shfl<FLAGS>(a, m);
+        a.nan |= b.nan;
+        if (FLAGS & F_MAX) take_max(a.vmax, a.imax, b.vmax, b.imax);
+        if (FLAGS & F_MIN) take_min(a.vmin, a.imin, b.vmin, b.imin);
+        if (FLAGS & F_M2) {
+            const double d = (a.hi + a.lo) - (b.hi + b.lo);
+            a.m2 = a.m2 + b.m2 + d * d * w;
+        }
+        if constexpr (sizeof(T) == 8) dw_add(a.hi, a.lo, b.hi, b.lo);
+        else a.hi += b.hi;
+        a.n += b.n;
+        w *= 0.5;
+    }
+}
+
+template <typename T, int VPT, int FLAGS, int LEAVES, bool VEC>
+__global__ __launch_bounds__(RB) void lean_kernel(LeanParams<T> p) {
+    constexpr int VN = Vec16<T>::N;
+    constexpr int M = VPT * VN;
+    constexpr int WSEG = 64 * M;
+    __shared__ Part grp_lds[WAVES];
+    __shared__ unsigned grp_fn[WAVES];
+    const unsigned lane = threadIdx.x & 63;
+    const unsigned wave = lane0((unsigned)(threadIdx.x >> 6));
+    const size_t gchunks = (size_t)WAVES * p.cpw;
+    const size_t c_begin = blockIdx.x * gchunks + (size_t)wave * p.cpw;
+    const size_t c_end = min(p.nchunk, c_begin + (size_t)p.cpw);
+    const size_t nseg = c_end > c_begin ? (c_end - c_begin) * p.cl : 0;
+    Part wp = part_empty();  // this wavefront's chunks, merged in chunk order
+    unsigned wp_fn = 0;
+    Part keep = part_empty();  // lane k: the record of the wavefront's k-th chunk
+    unsigned keep_fn = 0;
+    if (nseg > 0) {
+        const T *b0 = p.x0 + c_begin * p.cl * WSEG;
+        const T *b1 = LEAVES == 2 ? p.x1 + c_begin * p.cl * WSEG : nullptr;
+        LaneAcc<T> a;
+        acc_reset(a);
+        unsigned fnan = 0, j = 0, k = 0;
+        T ya[M], yb[M], za[M], zb[M];
+        auto load = [&](T(&y)[M], T(&z)[M], size_t s) {
+            lean_load<T, VPT, VEC>(b0 + s * WSEG, y);
+            if constexpr (LEAVES == 2) lean_load<T, VPT, VEC>(b1 + s * WSEG, z);
+        };
+        // after segment j of chunk k: close the chunk when it is the last
+        auto step = [&](T(&y)[M], T(&z)[M]) {
+            lean_fold<T, VPT, FLAGS, LEAVES>(p.prog, y, z, j, a, fnan);
+            if (++j == p.cl) {
+                const size_t c = c_begin + k;
+                const uint64_t fbase = (uint64_t)c * p.cl * WSEG;
+                auto flat = [&](unsigned code) -> uint64_t {
+                    const unsigned sl = code / M, i = code % M;
+                    return fbase + (uint64_t)sl * WSEG + (uint64_t)(((i / VN) * 64 + lane) * VN + (i % VN));
+                };
+                Part pa;
+                pa.n = (double)M * (double)p.cl;
+                pa.hi = a.hi; pa.lo = a.lo; pa.m2 = a.m2; pa.nan = a.nan;
+                pa.vmax = a.cmax == NO_CODE ? -__builtin_inf() : (double)a.vmax;
+                pa.imax = a.cmax == NO_CODE ? NO_INDEX : flat(a.cmax);
+                pa.vmin = a.cmin == NO_CODE ? __builtin_inf() : (double)a.vmin;
+                pa.imin = a.cmin == NO_CODE ? NO_INDEX : flat(a.cmin);
+                lean_wave_merge<T, FLAGS>(pa);
+                const unsigned fn0 = (unsigned)__shfl(fnan, 0, 64);
+                if (lane == k) {
+                    keep = pa;
+                    keep_fn = fn0;
+                }
+                if (k == 0) wp_fn = fn0;
+                part_merge<FLAGS>(wp, pa);
+                acc_reset(a);
+                fnan = 0;
+                j = 0;
+                ++k;
+            }
+        };
+        load(ya, za, 0);
+        // the next segment's loads are issued before the current one is folded; past the last
+        // segment they reread it (clamped address), so there is no branch around a load
+        for (size_t s = 0;;) {
+            load(yb, zb, s + 1 < nseg ? s + 1 : s);
+            step(ya, za);
+            if (++s == nseg) break;
+            load(ya, za, s + 1 < nseg ? s + 1 : s);
+            step(yb, zb);
+            if (++s == nseg) break;
+        }
+        if (p.row_direct && lane < (unsigned)(c_end - c_begin)) {
+            const size_t r = c_begin + lane;  // a chunk is a row
+            write_outputs<T>(p.row, r, keep, p.ddof, keep_fn, (uint64_t)r * p.cols);
+        }
+    }
+    if (!p.need_group) return;  // uniform over the grid
+    if (lane == 0) {
+        grp_lds[wave] = wp;
+        grp_fn[wave] = wp_fn;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Part g = grp_lds[0];
+        for (int w = 1; w < WAVES; ++w) part_merge<FLAGS>(g, grp_lds[w]);
+        part_store<FLAGS>(p.grp, blockIdx.x, g, grp_fn[0]);
+    }
+}
+
+// The whole-array result from the group records, merged in a fixed order: thread t folds the
+// contiguous records [t per, (t + 1) per), then the wavefront butterflies, then wavefront order.
+template <typename T, int FLAGS>
+__global__ __launch_bounds__(FB) void lean_finish_kernel(Parts in, size_t count, OutPtrs<T> o, double ddof) {
+    __shared__ Part lds[FB / 64];
+    const unsigned t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const size_t per = (count + FB - 1) / FB;
+    Part a = part_empty();
+    for (size_t i = t * per; i < min(count, (size_t)(t + 1) * per); ++i) part_merge<FLAGS>(a, part_load<FLAGS>(in, i));
+    wave_merge<FLAGS>(a);
+    if (lane == 0) lds[w] = a;
+    __syncthreads();
+    if (t == 0) {
+        Part r = lds[0];
+        for (int k = 1; k < FB / 64; ++k) part_merge<FLAGS>(r, lds[k]);
+        write_outputs<T>(o, 0, r, ddof, (in.flags[0] >> 1) & 1u, 0);
+    }
+}
+
+// Chunk length for a whole-array lean reduction of spr full segments: the smallest divisor of spr
+// that leaves at most CHUNK_TARGET chunks (0 if none up to the weight table's length).
+static size_t lean_chunk_len(size_t spr) {
+    const size_t lo = std::max<size_t>(1, ceil_div(spr, CHUNK_TARGET));
+    for (size_t cl = lo; cl < (size_t)LEAN_WTAB && cl <= spr; ++cl)
+        if (spr % cl == 0) return cl;
+    return 0;
+}
+
+// Chunks per wavefront, from the shape only (it fixes the group records, hence the whole-array
+// result's bits): enough workgroups to fill a 256-CU chip several times over.
+constexpr size_t LEAN_MIN_CHUNKS = 1024;
+static int lean_cpw(size_t nchunk) {
+    return nchunk >= 65536 ? 4 : (nchunk >= 16384 ? 2 : 1);
+}
+
+template <typename T, int VPT, int FLAGS, int LEAVES>
+static nrk_status launch_lean(const LeanParams<T> &p, size_t ngroups, const OutPtrs<T> &full, hipStream_t stream) {
+    const bool vec = (uintptr_t)p.x0 % 16 == 0 && (LEAVES == 1 || (uintptr_t)p.x1 % 16 == 0);
+    if (vec)
+        hipLaunchKernelGGL((lean_kernel<T, VPT, FLAGS, LEAVES, true>), dim3((unsigned)ngroups), dim3(RB), 0, stream, p);
+    else
+        hipLaunchKernelGGL((lean_kernel<T, VPT, FLAGS, LEAVES, false>), dim3((unsigned)ngroups), dim3(RB), 0, stream, p);
+    NRK_HIP(hipGetLastError());
+    if (p.need_group) {
+        hipLaunchKernelGGL((lean_finish_kernel<T, FLAGS>), dim3(1), dim3(FB), 0, stream, p.grp, ngroups, full, p.ddof);
+        NRK_HIP(hipGetLastError());
+    }
+    return NRK_OK;
+}
+
+// The lean path, if the shape and operands allow it (else returns false in *taken). x1 != NULL:
+// two operands (prog.op and the three chains), else one (prog.pre0).
+template <typename T>
+static nrk_status try_lean(const LeanProg &prog, const T *x0, const T *x1, size_t rows, size_t cols, double ddof,
+                           const OutPtrs<T> &row, const OutPtrs<T> &full, bool want_row, bool want_full, hipStream_t stream,
+                           const ExtScratch *ext, bool *taken) {
+    constexpr int M = NRK_REDUCE_M;
+    constexpr int VPT = M / Vec16<T>::N;
+    constexpr size_t WSEG = 64 * M;
+    *taken = false;
+    // the choice depends on the shape only (never on alignment or the device), so a given shape
+    // always gets the same summation tree
+    size_t cl, nchunk;
+    if (want_row) {  // a chunk is a row
+        if (cols % WSEG != 0 || cols / WSEG >= (size_t)LEAN_WTAB || rows < LEAN_MIN_CHUNKS) return NRK_OK;
+        cl = cols / WSEG;
+        nchunk = rows;
+    } else {
+        const size_t n = rows * cols;
+        if (n % WSEG != 0) return NRK_OK;
+        cl = lean_chunk_len(n / WSEG);
+        if (cl == 0) return NRK_OK;
+        nchunk = n / WSEG / cl;
+        if (nchunk < LEAN_MIN_CHUNKS) return NRK_OK;
+    }
+    LeanParams<T> p{};
+    p.x0 = x0;
+    p.x1 = x1;
+    p.nchunk = nchunk;
+    p.cl = cl;
+    p.cols = cols;
+    p.cpw = lean_cpw(nchunk);
+    p.prog = prog;
+    p.ddof = ddof;
+    p.row_direct = want_row;
+    p.row = want_row ? row : OutPtrs<T>{};
+    p.need_group = want_full;
+    const size_t ngroups = ceil_div(nchunk, (size_t)WAVES * p.cpw);
+    if (ngroups > 0x7fffffffu) return NRK_OK;
+    if (want_full) {
+        const size_t pbytes = parts_bytes(ngroups) + 256;
+        char *b = nullptr;
+        if (ext) {
+            if (pbytes + 256 > ext->bytes)
+                return fail(NRK_ERR_INVALID_ARGUMENT, "(%zu, %zu) needs %zu bytes of scratch; the fixed scratch holds %zu", rows, cols,
+                            pbytes + 256, ext->bytes);
+            b = (char *)ext->base + 256;
+        } else {
+            Scratch s;
+            NRK_TRY(get_scratch(stream, 1, pbytes, &s));
+            b = (char *)s.partials;
+        }
+        p.grp = carve_parts(b, ngroups);
+    }
+    const bool m2 = (row.var || full.var);
+    const bool mx = (row.max || row.argmax || full.max || full.argmax);
+    const bool mn = (row.min || row.argmin || full.min || full.argmin);
+    const int flags = (m2 ? F_M2 : 0) | (mx ? F_MAX : 0) | (mn ? F_MIN : 0);
+    *taken = true;
+#define NRK_LEAN(F)                                                                             \
+    case F:                                                                                     \
+        return x1 ? launch_lean<T, VPT, F, 2>(p, ngroups, full, stream) : launch_lean<T, VPT, F, 1>(p, ngroups, full, stream);
+    switch (flags) {
+        NRK_LEAN(0) NRK_LEAN(1) NRK_LEAN(2) NRK_LEAN(3) NRK_LEAN(4) NRK_LEAN(5) NRK_LEAN(6)
+        default: NRK_LEAN(7)
+    }
+#undef NRK_LEAN
+}
+
 // Host launcher for the fused map -> reduce over a (rows, cols) matrix.
 __global__ void clear_counter_kernel(unsigned int *c) { *c = 0u; }
 
+// x1 != NULL (with prog2): the two-operand lean form; *no_lean is set (and nothing launched)
+// when the lean path cannot take it.
 template <typename T>
-nrk_status seg_reduce(const Chain &chain, const T *x, size_t rows, size_t cols, double ddof,
-                      const nrk_reduce_outputs *row_out, const nrk_reduce_outputs *full_out, hipStream_t stream,
-                      const ExtScratch *ext) {
+static nrk_status seg_reduce_impl(const Chain &chain, const T *x, size_t rows, size_t cols, double ddof,
+                                  const nrk_reduce_outputs *row_out, const nrk_reduce_outputs *full_out, hipStream_t stream,
+                                  const ExtScratch *ext, const LeanProg *prog2, const T *x1, bool *no_lean) {
     constexpr int M = NRK_REDUCE_M;
     constexpr int VPT = M / Vec16<T>::N;
     static_assert(VPT * Vec16<T>::N == M, "elements per lane must be whole 16-byte vectors");
@@ -769,7 +1155,7 @@ nrk_status seg_reduce(const Chain &chain, const T *x, size_t rows, size_t cols, 
         return fail(NRK_ERR_INVALID_ARGUMENT, "max/min/argmax/argmin along an empty axis");
     if (want_full && (full.argmax || full.argmin) && rows * cols == 0)
         return fail(NRK_ERR_INVALID_ARGUMENT, "argmax/argmin of an empty array");
-    if (!x && rows * cols > 0) return fail(NRK_ERR_INVALID_ARGUMENT, "x is NULL");
+    if ((!x || (prog2 && !x1)) && rows * cols > 0) return fail(NRK_ERR_INVALID_ARGUMENT, "x is NULL");
     if (rows * cols == 0) {
         // reference: empty -> 0 for sum/mean/min/max/var (kernels/reduce.rs:207-421)
         if (want_full) {
@@ -781,6 +1167,18 @@ nrk_status seg_reduce(const Chain &chain, const T *x, size_t rows, size_t cols, 
             NRK_HIP(hipGetLastError());
         }
         return NRK_OK;
+    }
+    {
+        LeanProg prog{};
+        prog.pre0 = chain;
+        bool taken = false;
+        NRK_TRY(try_lean<T>(prog2 ? *prog2 : prog, x, prog2 ? x1 : nullptr, rows, cols, ddof, row, full, want_row, want_full,
+                            stream, ext, &taken));
+        if (taken) return NRK_OK;
+        if (prog2) {
+            *no_lean = true;
+            return NRK_OK;
+        }
     }
     if (!want_row) {  // a whole-array reduction: the rows are contiguous, so take them as one
         cols *= rows;
@@ -843,6 +1241,83 @@ nrk_status seg_reduce(const Chain &chain, const T *x, size_t rows, size_t cols, 
     }
 #undef NRK_LAUNCH_FLAGS
 }
+
+template <typename T>
+nrk_status seg_reduce(const Chain &chain, const T *x, size_t rows, size_t cols, double ddof,
+                      const nrk_reduce_outputs *row_out, const nrk_reduce_outputs *full_out, hipStream_t stream,
+                      const ExtScratch *ext) {
+    return seg_reduce_impl<T>(chain, x, rows, cols, ddof, row_out, full_out, stream, ext, nullptr, nullptr, nullptr);
+}
+
+// LEAF a STEP* [LEAF b STEP* BIN STEP*] with IEEE steps only: the lean kernel's program form.
+static bool lean_form(const ExprProg &pg, LeanProg *lp, int *first, int *second) {
+    int k = 0;
+    auto steps = [&](Chain &c) -> bool {
+        c.len = 0;
+        for (int i = 0; i < NRK_MAX_CHAIN; ++i) {
+            c.op[i] = -1;
+            c.s[i] = 0.0;
+        }
+        while (k < pg.len && pg.kind[k] == EXPR_I_STEP) {
+            if (sop_math_class(pg.op[k]) != MATH_NONE || c.len == NRK_MAX_CHAIN) return false;
+            c.op[c.len] = pg.op[k];
+            c.s[c.len] = pg.s[k];
+            ++c.len;
+            ++k;
+        }
+        return true;
+    };
+    *lp = LeanProg{};
+    if (pg.len < 1 || pg.kind[0] != EXPR_I_LEAF) return false;
+    *first = pg.op[0];
+    *second = -1;
+    k = 1;
+    if (!steps(lp->pre0)) return false;
+    if (k == pg.len) {
+        steps(lp->pre1);
+        steps(lp->post);
+        return true;
+    }
+    if (pg.kind[k] != EXPR_I_LEAF) return false;
+    *second = pg.op[k++];
+    if (!steps(lp->pre1)) return false;
+    if (k >= pg.len || pg.kind[k] != EXPR_I_BIN) return false;
+    lp->op = pg.op[k++];
+    if (!steps(lp->post)) return false;
+    return k == pg.len;
+}
+
+template <typename T>
+nrk_status expr_reduce(const ExprProg &pg, const T *const *leaves, size_t rows, size_t cols, double ddof,
+                       const nrk_reduce_outputs *row_out, const nrk_reduce_outputs *full_out, hipStream_t stream) {
+    NRK_TRY(ensure_device());
+    if (!leaves) return fail(NRK_ERR_INVALID_ARGUMENT, "leaves is NULL");
+    for (int j = 0; j < pg.n_leaves; ++j)
+        if (!leaves[j] && rows * cols > 0) return fail(NRK_ERR_INVALID_ARGUMENT, "leaf %d is NULL", j);
+    if (row_out && row_out->var && cols <= (size_t)ddof)
+        return fail(NRK_ERR_INVALID_ARGUMENT, "Degrees of freedom %zu >= axis size %zu", (size_t)ddof, cols);
+    LeanProg lp;
+    int a = 0, b = -1;
+    if (lean_form(pg, &lp, &a, &b)) {
+        if (b < 0) return seg_reduce<T>(lp.pre0, leaves[a], rows, cols, ddof, row_out, full_out, stream);
+        bool no_lean = false;
+        NRK_TRY(seg_reduce_impl<T>(lp.pre0, leaves[a], rows, cols, ddof, row_out, full_out, stream, nullptr, &lp, leaves[b],
+                                   &no_lean));
+        if (!no_lean) return NRK_OK;
+    }
+    // any other tree (or shape): evaluate it into the stream's workspace, then reduce that
+    const size_t n = rows * cols;
+    void *ws = nullptr;
+    NRK_TRY(get_workspace(stream, std::max<size_t>(n, 1) * sizeof(T), &ws));
+    NRK_TRY(expr_eval<T>(pg, leaves, (T *)ws, n, stream));
+    Chain none;
+    NRK_TRY(make_chain(nullptr, 0, &none));
+    return seg_reduce<T>(none, (const T *)ws, rows, cols, ddof, row_out, full_out, stream);
+}
+template nrk_status expr_reduce<float>(const ExprProg &, const float *const *, size_t, size_t, double,
+                                       const nrk_reduce_outputs *, const nrk_reduce_outputs *, hipStream_t);
+template nrk_status expr_reduce<double>(const ExprProg &, const double *const *, size_t, size_t, double,
+                                        const nrk_reduce_outputs *, const nrk_reduce_outputs *, hipStream_t);
 
 template nrk_status seg_reduce<float>(const Chain &, const float *, size_t, size_t, double, const nrk_reduce_outputs *,
                                       const nrk_reduce_outputs *, hipStream_t, const ExtScratch *);

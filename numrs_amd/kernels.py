@@ -724,3 +724,98 @@ def gemm_f64_ozaki(a: np.ndarray, b: np.ndarray, moduli: int = 0) -> np.ndarray:
     c = np.empty((m, n), np.float64)
     call("nrk_gemm_f64_ozaki", C.c_size_t(m), C.c_size_t(k), C.c_size_t(n), _ptr(a), _ptr(b), _ptr(c), C.c_int32(moduli))
     return c
+
+
+# ---- expr/owned.rs ExprNode trees and expr/fused_eval.rs eval ----
+class ExprNodeC(C.Structure):
+    """nrk_expr_node"""
+    _fields_ = [("kind", C.c_int32), ("op", C.c_int32), ("scalar", C.c_double)]
+
+
+_EXPR_BIN = {"add": 0, "sub": 1, "mul": 2, "div": 3}
+_EXPR_RHS = {"add": 0, "sub": 1, "mul": 2, "div": 3}       # x op s
+_EXPR_LHS = {"add": 0, "sub": 4, "mul": 2, "div": 5}       # s op x (s + x = x + s, s * x = x * s)
+_EXPR_UNARY = {"neg": 6, "abs": 7, "sqrt": 9, "exp": 18, "ln": 21}
+
+
+class Expr:
+    """An ExprNode tree (expr/owned.rs:204-223): Leaf, Binary, ScalarRhs, ScalarLhs, Unary, Fma,
+    built with the operators (Expr.leaf(a) * Expr.leaf(b) + Expr.leaf(c), e * 2.0, 1.0 - e, -e,
+    e.sqrt(), e.exp(), e.ln(), Expr.fma(a, b, c)) and evaluated in one GPU pass by eval()."""
+
+    def __init__(self, kind, op=None, kids=(), scalar=0.0, array=None):
+        self.kind, self.op, self.kids, self.scalar, self.array = kind, op, tuple(kids), scalar, array
+
+    @staticmethod
+    def leaf(a) -> "Expr":
+        return Expr("leaf", array=np.ascontiguousarray(a))
+
+    @staticmethod
+    def fma(a, b, c) -> "Expr":
+        return Expr("fma", kids=(_as_expr(a), _as_expr(b), _as_expr(c)))
+
+    def _bin(self, other, op, reverse=False):
+        if isinstance(other, Expr):
+            return Expr("bin", op, (other, self) if reverse else (self, other))
+        return Expr("lhs" if reverse else "rhs", op, (self,), float(other))
+
+    def __add__(self, o): return self._bin(o, "add")
+    def __radd__(self, o): return self._bin(o, "add", True)
+    def __sub__(self, o): return self._bin(o, "sub")
+    def __rsub__(self, o): return self._bin(o, "sub", True)
+    def __mul__(self, o): return self._bin(o, "mul")
+    def __rmul__(self, o): return self._bin(o, "mul", True)
+    def __truediv__(self, o): return self._bin(o, "div")
+    def __rtruediv__(self, o): return self._bin(o, "div", True)
+    def __neg__(self): return Expr("unary", "neg", (self,))
+    def abs(self): return Expr("unary", "abs", (self,))
+    def sqrt(self): return Expr("unary", "sqrt", (self,))
+    def exp(self): return Expr("unary", "exp", (self,))
+    def ln(self): return Expr("unary", "ln", (self,))
+
+    def program(self):
+        """Postfix nodes [(kind, op, scalar)] and the distinct operand arrays, in first-use order."""
+        leaves, nodes = [], []
+
+        def walk(e):
+            if e.kind == "leaf":
+                for j, a in enumerate(leaves):
+                    if a is e.array:
+                        break
+                else:
+                    j = len(leaves)
+                    leaves.append(e.array)
+                nodes.append((0, j, 0.0))
+                return
+            for k in e.kids:
+                walk(k)
+            if e.kind == "bin":
+                nodes.append((2, _EXPR_BIN[e.op], 0.0))
+            elif e.kind == "rhs":
+                nodes.append((1, _EXPR_RHS[e.op], e.scalar))
+            elif e.kind == "lhs":
+                nodes.append((1, _EXPR_LHS[e.op], e.scalar))
+            elif e.kind == "unary":
+                nodes.append((1, _EXPR_UNARY[e.op], 0.0))
+            else:
+                nodes.append((3, 0, 0.0))
+
+        walk(self)
+        return nodes, leaves
+
+    def eval(self) -> np.ndarray:
+        """fused_eval.rs:635-697 eval: the whole tree in one pass, each node rounded once."""
+        nodes, leaves = self.program()
+        dt = leaves[0].dtype
+        if dt not in (np.float64, np.float32) or any(a.dtype != dt or a.shape != leaves[0].shape for a in leaves):
+            raise ValueError("expression operands must share one float dtype and one shape")
+        arr = (ExprNodeC * len(nodes))(*[ExprNodeC(k, o, s) for k, o, s in nodes])
+        ptrs = (C.c_void_p * len(leaves))(*[a.ctypes.data for a in leaves])
+        out = np.empty_like(leaves[0])
+        call("nrk_expr_eval_f64" if dt == np.float64 else "nrk_expr_eval_f32", C.cast(arr, C.c_void_p), C.c_int32(len(nodes)),
+             C.cast(ptrs, C.c_void_p), C.c_int32(len(leaves)), _ptr(out), C.c_size_t(out.size))
+        return out
+
+
+def _as_expr(v):
+    return v if isinstance(v, Expr) else Expr.leaf(v)

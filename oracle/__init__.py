@@ -144,6 +144,20 @@ def ref_map_int(ops: list[int], scalars: list[int], x: np.ndarray) -> np.ndarray
     return y
 
 
+def ref_expr(nodes, leaves) -> np.ndarray:
+    """eval_eager (fused_eval.rs:605-658) of a postfix program [(kind, op, scalar), ...] over equally
+    long leaves, one rounded operation per node (oracle.c ref_expr)."""
+    leaves = [np.ascontiguousarray(v) for v in leaves]
+    dt = leaves[0].dtype
+    kind = np.array([nd[0] for nd in nodes], np.int32)
+    op = np.array([nd[1] for nd in nodes], np.int32)
+    sc = np.array([nd[2] if len(nd) > 2 else 0.0 for nd in nodes], np.float64)
+    ptrs = (C.c_void_p * len(leaves))(*[v.ctypes.data for v in leaves])
+    y = np.empty_like(leaves[0])
+    getattr(lib(), f"ref_expr_{_suf(leaves[0])}")(_p(kind), _p(op), _p(sc), len(nodes), ptrs, _p(y), C.c_size_t(y.size))
+    return y
+
+
 def ref_fma(a: np.ndarray, b: np.ndarray, c: np.ndarray) -> np.ndarray:
     a, b, c = (np.ascontiguousarray(v) for v in (a, b, c))
     y = np.empty_like(a)

@@ -443,6 +443,60 @@ DEFINE_INT_ELEM(int32_t, uint32_t, i32, 32)
 DEFINE_INT_ELEM(uint64_t, uint64_t, u64, 64)
 DEFINE_INT_ELEM(uint32_t, uint32_t, u32, 32)
 
+/* expr/fused_eval.rs:605-658 eval_eager: the tree evaluated node by node, each node one rounded
+ * operation (Binary -> the broadcast arithmetic, ScalarRhs / ScalarLhs -> the scalar ops, Unary ->
+ * neg / abs / sqrt, Fma -> multiply then add), as a postfix program of (kind, op, scalar) nodes
+ * (kinds 0 leaf, 1 step, 2 binary, 3 fma; ops the library's NRK_SOP_* / NRK_BOP_* codes). IEEE
+ * operations only (the elementary functions are not restated). One element at a time. */
+#define DEFINE_EXPR(T, SUF, SQRT, FABS, FMOD, COPYSIGN)                                      \
+    void ref_expr_##SUF(const int32_t *kind, const int32_t *op, const double *sc, int len,   \
+                        const T *const *leaves, T *y, size_t n) {                             \
+        for (size_t i = 0; i < n; ++i) {                                                    \
+            T st[16];                                                                       \
+            int sp = 0;                                                                     \
+            for (int k = 0; k < len; ++k) {                                                 \
+                if (kind[k] == 0) { st[sp++] = leaves[op[k]][i]; continue; }                \
+                if (kind[k] == 1) {                                                         \
+                    T x = st[sp - 1], s = (T)sc[k];                                         \
+                    switch (op[k]) {                                                        \
+                        case 0: x = x + s; break;                                           \
+                        case 1: x = x - s; break;                                           \
+                        case 2: x = x * s; break;                                           \
+                        case 3: x = x / s; break;                                           \
+                        case 4: x = s - x; break;                                           \
+                        case 5: x = s / x; break;                                           \
+                        case 6: x = -x; break;                                              \
+                        case 7: x = FABS(x); break;                                         \
+                        case 9: x = SQRT(x); break;                                         \
+                        default: break;                                                     \
+                    }                                                                       \
+                    st[sp - 1] = x;                                                         \
+                    continue;                                                               \
+                }                                                                           \
+                if (kind[k] == 2) {                                                         \
+                    T r = st[--sp], l = st[sp - 1], v;                                      \
+                    switch (op[k]) {                                                        \
+                        case 0: v = l + r; break;                                           \
+                        case 1: v = l - r; break;                                           \
+                        case 2: v = l * r; break;                                           \
+                        case 3: v = l / r; break;                                           \
+                        case 4: v = FMOD(l, r); break;                                      \
+                        case 5: v = l > r ? l : r; break;                                   \
+                        case 6: v = l < r ? l : r; break;                                   \
+                        case 7: v = COPYSIGN(l, r); break;                                  \
+                        default: v = l > 0 ? l : r * l; break; /* 11: prelu */              \
+                    }                                                                       \
+                    st[sp - 1] = v;                                                         \
+                    continue;                                                               \
+                }                                                                           \
+                { T c = st[--sp], b = st[--sp], a = st[sp - 1]; T p = a * b; st[sp - 1] = p + c; } \
+            }                                                                               \
+            y[i] = st[0];                                                                   \
+        }                                                                                   \
+    }
+DEFINE_EXPR(double, f64, sqrt, fabs, fmod, copysign)
+DEFINE_EXPR(float, f32, sqrtf, fabsf, fmodf, copysignf)
+
 /* ufuncs.rs:864-891 fma: a.mul_add(b, c), one rounding (C's fma is the IEEE fusedMultiplyAdd);
  * ufuncs.rs:893-912 lerp: a + t * (b - a), three roundings */
 #define DEFINE_TRI(T, SUF, FMA)                                                             \

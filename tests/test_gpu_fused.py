@@ -46,7 +46,8 @@ def run_fused(x: np.ndarray, steps, ddof=0, want_rows=True, want_full=True, offs
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-@pytest.mark.parametrize("shape", [(1, 1), (4, 7), (16, 8192), (9, 8193), (3, 40_000), (257, 100)])
+@pytest.mark.parametrize("shape", [(1, 1), (4, 7), (16, 8192), (9, 8193), (3, 40_000), (257, 100),
+                                   (1024, 1024), (2048, 3072), (1100, 2048)])  # the last three: the lean kernel
 def test_fused_vs_oracle(gpu, oracle, dtype, shape):
     rows, cols = shape
     x = rng.normal(rng.BASE_SEED + 1100 + rows * 7 + cols, rows * cols, dtype).reshape(rows, cols)
@@ -100,3 +101,43 @@ def test_fused_nan_rows(gpu, oracle):
     assert got["row_argmax"][1] == 0 and np.isnan(got["row_max"][1]) and np.isnan(got["row_max"][2])
     np.testing.assert_array_equal(got["row_argmax"], oracle.ref_axis(6, x, 6, 9000, 1))
     assert got["full_argmax"][0] == oracle.ref("argmax", x) and np.isnan(got["full_max"][0])
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("shape", [(1024, 2048), (1, 1 << 21)])
+def test_lean_path_alignment_nan_and_ties(gpu, oracle, dtype, shape):
+    """The lean kernel (whole segments, >= 1024 chunks): a pointer off 16-byte alignment reads the
+    same elements one by one, so every output is bit-identical; NaN at a row's first element pins
+    argmax to 0, an interior NaN is skipped by argmax and poisons max; equal maxima go to the first
+    index, the sign of a zero maximum is the first zero's (row and whole array, against the
+    oracle's sequential folds)."""
+    rows, cols = shape
+    x = rng.normal(rng.BASE_SEED + 1500 + rows, rows * cols, dtype).reshape(rows, cols)
+    want_rows = rows > 1
+    a = run_fused(x, [("mul", 1.5), ("add", -0.25)], want_rows=want_rows)
+    b = run_fused(x, [("mul", 1.5), ("add", -0.25)], want_rows=want_rows, offset_elems=1)
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+    z = x.copy()
+    flat = z.reshape(-1)
+    if want_rows:
+        z[3, 0] = np.nan
+        z[5, 777] = np.nan
+        z[7, :] = -0.0
+        z[7, 1000] = 0.0
+        z[9, 100] = z[9, 1500] = 50.0
+    else:
+        flat[123_457] = np.nan
+        flat[:] = np.where(np.arange(flat.size) % 4096 == 17, 80.0, flat)
+    got = run_fused(z, [], want_rows=want_rows)
+    assert got["full_argmax"][0] == oracle.ref("argmax", z)
+    assert np.isnan(got["full_max"][0]) == np.isnan(oracle.ref("max", z))
+    if want_rows:
+        np.testing.assert_array_equal(got["row_argmax"], oracle.ref_axis(6, z, rows, cols, 1))
+        np.testing.assert_array_equal(got["row_argmin"], oracle.ref_axis(5, z, rows, cols, 1))
+        rm, om = got["row_max"], oracle.ref_axis(4, z, rows, cols, 1)
+        np.testing.assert_array_equal(np.isnan(rm), np.isnan(om))
+        ok = ~np.isnan(om)
+        np.testing.assert_array_equal(rm[ok], om[ok])
+        np.testing.assert_array_equal(np.signbit(rm[ok]), np.signbit(om[ok]))
+        assert got["row_argmax"][3] == 0 and got["row_argmax"][9] == 100
