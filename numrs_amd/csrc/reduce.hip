@@ -35,27 +35,9 @@
 // the earlier element), then the larger value with ties to the lower index, and the value
 // written is the element itself, so even the sign of a zero maximum matches the reference's
 // left-to-right fold.
-#include "ops.hpp"
-
-#include <algorithm>
+#include "reduce_common.hpp"
 
 namespace nrk {
-
-constexpr int RB = 256;  // threads per streaming workgroup
-constexpr int WAVES = RB / 64;
-constexpr uint64_t NO_INDEX = ~0ull;
-constexpr unsigned NO_CODE = 0xffffffffu;
-constexpr int FB = 1024;                // threads of the single-workgroup finish kernels
-constexpr size_t CHUNK_TARGET = 8192;   // chunks per matrix when rows are few (about one per wavefront)
-
-enum : int { F_M2 = 1, F_MAX = 2, F_MIN = 4 };
-
-// Elements per lane in one wavefront segment of the f32/f64 streaming kernel (16: 4 16-byte
-// vectors of f32 or 8 of f64 per lane; a segment is 4 KiB of f32 or 8 KiB of f64). It fixes the
-// segment boundaries and hence the summation tree, so results are reproducible for a given build.
-#ifndef NRK_REDUCE_M
-#define NRK_REDUCE_M 16
-#endif
 
 // Persistent-grid size of the streaming kernels (workgroups per CU). Result bits do not depend on
 // it, only speed, so the tuner may choose it per shape.
@@ -63,216 +45,10 @@ static thread_local int g_blocks_per_cu = 8;
 int reduce_blocks_per_cu() { return g_blocks_per_cu; }
 void set_reduce_blocks_per_cu(int b) { g_blocks_per_cu = b < 1 ? 1 : (b > 16 ? 16 : b); }
 
-template <typename T>
-struct OutPtrs {
-    T *sum, *mean, *var, *min, *max;
-    uint64_t *argmin, *argmax;
-};
-
-// A set of partial records, structure of arrays. Indices are flat (row * cols + col).
-struct Parts {
-    double *n, *hi, *lo, *m2, *vmax, *vmin;
-    uint64_t *imax, *imin;
-    unsigned int *flags;  // bit0: a NaN was seen; bit1: the set's first element is NaN
-};
-
-template <typename T>
-struct SegParams {
-    const T *x;
-    size_t rows, cols;
-    size_t spr;     // segments per row
-    size_t cl;      // segments per chunk (the last chunk of a row may hold fewer)
-    size_t cpr;     // chunks per row
-    size_t nchunk;  // rows * cpr
-    Chain chain;
-    double ddof;
-    int row_direct;  // cpr == 1 and row results wanted: the stream writes them
-    int need_l1;     // chunk records are read by a tree level
-    Parts l1;        // chunk records, chunk c = row * cpr + q
-    OutPtrs<T> row;
-};
-
-// One level of the record tree: `nsets` sets of `per` consecutive input records; a wavefront
-// merges up to TREE_SPAN consecutive records of one set into one output record.
-constexpr int TREE_LOADS = 4;               // records per lane per level, loaded together
-constexpr int TREE_SPAN = 64 * TREE_LOADS;  // records per wavefront per level
-template <typename T>
-struct TreeParams {
-    Parts in, out;
-    size_t nsets, per, per_out;  // per_out = ceil(per / TREE_SPAN)
-    int results;                 // per_out == 1: write the set's results to `o`
-    int keep;                    // write output records (a later level reads them)
-    int rowmode;                 // set r is row r: flat index base r * cols
-    size_t cols;
-    double ddof;
-    OutPtrs<T> o;
-    // fused last level (one set, per_out <= TREE_SPAN): the workgroup that finishes last merges
-    // the level's output records and writes the results; `counter` is zero at launch and is
-    // reset to zero by that workgroup
-    int fuse_last;
-    OutPtrs<T> last_o;
-    unsigned int *counter;
-};
-
-// ---------------- small device utilities ----------------
-template <typename T>
-__device__ __forceinline__ void two_sum(T a, T b, T &s, T &e) {
-    s = rn_add(a, b);
-    T v = rn_sub(s, a);
-    e = rn_add(rn_sub(a, rn_sub(s, v)), rn_sub(b, v));
-}
-
-// double-word addition (hi + lo) += (bhi + blo)
-template <typename T>
-__device__ __forceinline__ void dw_add(T &hi, T &lo, T bhi, T blo) {
-    T s, e;
-    two_sum(hi, bhi, s, e);
-    e = rn_add(e, rn_add(lo, blo));
-    hi = rn_add(s, e);
-    lo = rn_sub(e, rn_sub(hi, s));
-}
-
-__device__ __forceinline__ float shfl_xor(float v, int m) { return __shfl_xor(v, m, 64); }
-__device__ __forceinline__ double shfl_xor(double v, int m) { return __shfl_xor(v, m, 64); }
-__device__ __forceinline__ uint64_t shfl_xor(uint64_t v, int m) {
-    unsigned lo = __shfl_xor((unsigned)(v & 0xffffffffu), m, 64);
-    unsigned hi = __shfl_xor((unsigned)(v >> 32), m, 64);
-    return ((uint64_t)hi << 32) | lo;
-}
-
-// Lane 0's value, for every lane (the wavefront is fully active where this is used).
-__device__ __forceinline__ unsigned lane0(unsigned v) { return (unsigned)__builtin_amdgcn_readfirstlane((int)v); }
-
-// (value, index) winners: larger (smaller) value wins, equal values go to the lower index.
-// Values are never NaN here (NaN elements never become winners).
-template <typename T>
-__device__ __forceinline__ void take_max(T &v, uint64_t &i, T bv, uint64_t bi) {
-    if (bv > v || (bv == v && bi < i)) { v = bv; i = bi; }
-}
-template <typename T>
-__device__ __forceinline__ void take_min(T &v, uint64_t &i, T bv, uint64_t bi) {
-    if (bv < v || (bv == v && bi < i)) { v = bv; i = bi; }
-}
-
-// Pairwise (balanced binary tree) sum of f(OFF) .. f(OFF+N-1), written depth-first.
-template <int N, int OFF, typename T, typename F>
-__device__ __forceinline__ T pairwise(const F &f) {
-    if constexpr (N == 1) {
-        return f(OFF);
-    } else {
-        return rn_add(pairwise<N / 2, OFF, T>(f), pairwise<N / 2, OFF + N / 2, T>(f));
-    }
-}
-
-// A partial record in registers; merge() = Chan, Golub & LeVeque (1979) for the second moment:
-// M2 = M2a + M2b + (na nb / n) (mean_a - mean_b)^2, in FP64; double-word sums.
-struct Part {
-    double n, hi, lo, m2, vmax, vmin;
-    uint64_t imax, imin;
-    unsigned nan;
-};
-
-__device__ __forceinline__ Part part_empty() {
-    Part a;
-    a.n = 0.0; a.hi = 0.0; a.lo = 0.0; a.m2 = 0.0;
-    a.vmax = -__builtin_inf(); a.vmin = __builtin_inf();
-    a.imax = NO_INDEX; a.imin = NO_INDEX; a.nan = 0;
-    return a;
-}
-
-template <int FLAGS>
-__device__ __forceinline__ void part_merge(Part &a, const Part &b) {
-    a.nan |= b.nan;
-    if (FLAGS & F_MAX) take_max(a.vmax, a.imax, b.vmax, b.imax);
-    if (FLAGS & F_MIN) take_min(a.vmin, a.imin, b.vmin, b.imin);
-    if (b.n == 0.0) return;
-    if (a.n == 0.0) {
-        a.n = b.n; a.hi = b.hi; a.lo = b.lo; a.m2 = b.m2;
-        return;
-    }
-    if (FLAGS & F_M2) {
-        // (na nb / n)(mean_a - mean_b)^2 = (nb Sa - na Sb)^2 / (na nb n): one division
-        const double d = b.n * (a.hi + a.lo) - a.n * (b.hi + b.lo);
-        a.m2 = a.m2 + b.m2 + d * d / (a.n * b.n * (a.n + b.n));
-    }
-    dw_add(a.hi, a.lo, b.hi, b.lo);
-    a.n += b.n;
-}
-
-template <int FLAGS>
-__device__ __forceinline__ Part part_shfl(const Part &a, int m) {
-    Part b;
-    b.n = shfl_xor(a.n, m);
-    b.hi = shfl_xor(a.hi, m);
-    b.lo = shfl_xor(a.lo, m);
-    b.m2 = (FLAGS & F_M2) ? shfl_xor(a.m2, m) : 0.0;
-    b.vmax = (FLAGS & F_MAX) ? shfl_xor(a.vmax, m) : -__builtin_inf();
-    b.imax = (FLAGS & F_MAX) ? shfl_xor(a.imax, m) : NO_INDEX;
-    b.vmin = (FLAGS & F_MIN) ? shfl_xor(a.vmin, m) : __builtin_inf();
-    b.imin = (FLAGS & F_MIN) ? shfl_xor(a.imin, m) : NO_INDEX;
-    b.nan = (unsigned)__shfl_xor((int)a.nan, m, 64);
-    return b;
-}
-
-template <int FLAGS>
-__device__ __forceinline__ void wave_merge(Part &a) {
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) {
-        const Part b = part_shfl<FLAGS>(a, m);
-        part_merge<FLAGS>(a, b);
-    }
-}
-
-// Record I/O touches only the fields the op set uses.
-template <int FLAGS>
-__device__ __forceinline__ Part part_load(const Parts &P, size_t s) {
-    Part a = part_empty();
-    a.n = P.n[s]; a.hi = P.hi[s]; a.lo = P.lo[s];
-    if (FLAGS & F_M2) a.m2 = P.m2[s];
-    if (FLAGS & F_MAX) { a.vmax = P.vmax[s]; a.imax = P.imax[s]; }
-    if (FLAGS & F_MIN) { a.vmin = P.vmin[s]; a.imin = P.imin[s]; }
-    a.nan = P.flags[s] & 1u;
-    return a;
-}
-
-template <int FLAGS>
-__device__ __forceinline__ void part_store(const Parts &P, size_t s, const Part &a, unsigned first_nan) {
-    P.n[s] = a.n; P.hi[s] = a.hi; P.lo[s] = a.lo;
-    if (FLAGS & F_M2) P.m2[s] = a.m2;
-    if (FLAGS & F_MAX) { P.vmax[s] = a.vmax; P.imax[s] = a.imax; }
-    if (FLAGS & F_MIN) { P.vmin[s] = a.vmin; P.imin[s] = a.imin; }
-    P.flags[s] = a.nan | (first_nan << 1);
-}
-
-// Write one set of results. `index_base` is subtracted from the flat winner indices (a row's
-// first flat index for row results, 0 for whole-matrix results). No winner means every element
-// is -inf (+inf) or NaN; the first one is then the answer, and index 0 is its position.
-template <typename T>
-__device__ __forceinline__ void write_outputs(const OutPtrs<T> &o, size_t at, const Part &a, double ddof, unsigned first_nan,
-                                              uint64_t index_base) {
-    const double s = a.hi + a.lo;
-    if (o.sum) o.sum[at] = (T)s;
-    if (o.mean) o.mean[at] = (T)(s / a.n);
-    if (o.var) o.var[at] = (T)(a.m2 / (a.n - ddof));
-    if (o.max) o.max[at] = a.nan ? dev_nan<T>() : (T)a.vmax;
-    if (o.min) o.min[at] = a.nan ? dev_nan<T>() : (T)a.vmin;
-    if (o.argmax) o.argmax[at] = first_nan ? 0 : (a.imax == NO_INDEX ? 0 : a.imax - index_base);
-    if (o.argmin) o.argmin[at] = first_nan ? 0 : (a.imin == NO_INDEX ? 0 : a.imin - index_base);
-}
-
 // ---------------- the streaming kernel ----------------
 // A segment is WSEG = 64 lanes x VPT 16-byte vectors of one row: lane l holds element
 // i = j*VN + e at segment position (j*64 + l)*VN + e, so every load instruction of the
 // wavefront reads 1 KiB of consecutive memory.
-
-// Per-lane running state over the segments of one chunk.
-template <typename T>
-struct LaneAcc {
-    double n, hi, lo, m2;
-    T vmax, vmin;
-    unsigned cmax, cmin;  // winner codes: segment-in-chunk * M + element slot; NO_CODE: none yet
-    unsigned nan;
-};
 
 // Issue the loads of the segment at `base` (len > 0 elements of the row remain there) into y and
 // record in vmask which elements exist (the others hold copies of the segment's first element;
@@ -407,12 +183,6 @@ __device__ __forceinline__ ChunkPos chunk_pos(const SegParams<T> &p, size_t c) {
     return k;
 }
 
-template <typename T>
-__device__ __forceinline__ void acc_reset(LaneAcc<T> &a) {
-    a.n = 0.0; a.hi = 0.0; a.lo = 0.0; a.m2 = 0.0;
-    a.vmax = -dev_inf<T>(); a.vmin = dev_inf<T>();
-    a.cmax = NO_CODE; a.cmin = NO_CODE; a.nan = 0;
-}
 
 // Per-wavefront batch of up to 64 chunk records, kept in LDS (slot k: the k-th chunk of the
 // batch) until they are stored together. Only the owning wavefront touches its slots.
@@ -630,44 +400,6 @@ __global__ void fill_outputs_kernel(OutPtrs<T> o, size_t n, T value) {
     if (o.argmax) o.argmax[i] = 0;
 }
 
-template <typename T>
-static OutPtrs<T> to_ptrs(const nrk_reduce_outputs *o) {
-    OutPtrs<T> p{};
-    if (!o) return p;
-    p.sum = (T *)o->sum;
-    p.mean = (T *)o->mean;
-    p.var = (T *)o->var;
-    p.min = (T *)o->min;
-    p.max = (T *)o->max;
-    p.argmin = o->argmin;
-    p.argmax = o->argmax;
-    return p;
-}
-
-template <typename T>
-static bool any_out(const OutPtrs<T> &o) {
-    return o.sum || o.mean || o.var || o.min || o.max || o.argmin || o.argmax;
-}
-
-static Parts carve_parts(char *&b, size_t count) {
-    auto take = [&](size_t bytes) {
-        void *r = b;
-        b += align_up(bytes, 256);
-        return r;
-    };
-    Parts P;
-    P.n = (double *)take(count * 8);
-    P.hi = (double *)take(count * 8);
-    P.lo = (double *)take(count * 8);
-    P.m2 = (double *)take(count * 8);
-    P.vmax = (double *)take(count * 8);
-    P.vmin = (double *)take(count * 8);
-    P.imax = (uint64_t *)take(count * 8);
-    P.imin = (uint64_t *)take(count * 8);
-    P.flags = (unsigned *)take(count * 4);
-    return P;
-}
-static size_t parts_bytes(size_t count) { return 8 * align_up(count * 8, 256) + align_up(count * 4, 256); }
 
 // Workgroups of `kernel` that fit on one CU at once (register / LDS limited), cached per kernel.
 template <typename K>
@@ -748,391 +480,6 @@ static nrk_status launch_all(SegParams<T> &p, bool want_row, bool want_full, con
     return NRK_OK;
 }
 
-// ---------------- the lean kernel: full segments, contiguous chunk ranges ----------------
-// Used when every segment is whole (cols, or rows * cols for a whole-array reduction, is a
-// multiple of the segment), there are at least LEAN_MIN_CHUNKS chunks, and a chunk is a whole row
-// when row results are wanted. Then every chunk holds exactly cl full segments, and:
-//   - chunks are consecutive in memory (chunk c is elements [c cl WSEG, (c + 1) cl WSEG)), so a
-//     wavefront owns the contiguous range of its cpw chunks and walks it with one pointer,
-//     loading the next segment while it folds the current one;
-//   - every lane of a chunk has the same count (16 per segment), so the Chan updates need no
-//     division: the weight of the j-th segment's merge, 1 / (16 j (j + 1)), comes from a constant
-//     table, and the butterfly's levels merge equal counts with weight 1 / (2 n);
-//   - binary32 lanes add their segment sums into plain FP64 (exact to 2^-29 of a binary32 ulp);
-//     binary64 lanes keep the double word;
-//   - row results stay in registers (lane k holds the wavefront's k-th chunk) until the loop
-//     ends, so the loop issues loads only; the workgroup's 4 cpw chunks are merged in chunk order
-//     into one group record, and one finish kernel merges the group records in order.
-// The chunking depends only on the shape, so results are bit-identical across launches, grids
-// and devices. The input is an expression of one or two operands (see LeanProg).
-constexpr int LEAN_WTAB = 1024;  // chunks of at most this many segments
-struct LeanWTab {
-    double w[LEAN_WTAB];
-    constexpr LeanWTab() : w() {
-        for (int j = 1; j < LEAN_WTAB; ++j) w[j] = 1.0 / (16.0 * j * (j + 1));
-    }
-};
-static_assert(NRK_REDUCE_M == 16, "the Chan weight table assumes 16 elements per lane and segment");
-__constant__ LeanWTab kLeanW = LeanWTab();
-
-// y = post(pre0(x0) op pre1(x1)) with two operands, post(pre0(x0)) with one (post empty then).
-struct LeanProg {
-    Chain pre0, pre1, post;
-    int op;
-};
-
-template <typename T>
-struct LeanParams {
-    const T *x0, *x1;
-    size_t nchunk, cl, cols;
-    int cpw;          // chunks per wavefront; a workgroup (4 wavefronts) owns 4 cpw consecutive chunks
-    LeanProg prog;
-    double ddof;
-    int row_direct;   // a chunk is a row and row results are wanted
-    OutPtrs<T> row;
-    int need_group;   // whole-array results wanted: store one record per workgroup
-    Parts grp;
-};
-
-template <typename T, int N>
-__device__ __forceinline__ void bop_n(int op, T (&a)[N], const T (&b)[N]) {
-    switch (op) {
-#define NRK_BOPN(OP, EXPR)                                         \
-    case OP:                                                       \
-        _Pragma("unroll") for (int i = 0; i < N; ++i) a[i] = EXPR; \
-        break;
-        NRK_BOPN(NRK_BOP_ADD, rn_add(a[i], b[i]))
-        NRK_BOPN(NRK_BOP_SUB, rn_sub(a[i], b[i]))
-        NRK_BOPN(NRK_BOP_MUL, rn_mul(a[i], b[i]))
-        NRK_BOPN(NRK_BOP_DIV, rn_div(a[i], b[i]))
-        NRK_BOPN(NRK_BOP_REM, __builtin_fmod(a[i], b[i]))
-        NRK_BOPN(NRK_BOP_MAXIMUM, a[i] > b[i] ? a[i] : b[i])
-        NRK_BOPN(NRK_BOP_MINIMUM, a[i] < b[i] ? a[i] : b[i])
-        NRK_BOPN(NRK_BOP_COPYSIGN, dev_copysign(a[i], b[i]))
-        NRK_BOPN(NRK_BOP_PRELU, a[i] > (T)0 ? a[i] : rn_mul(b[i], a[i]))
-#undef NRK_BOPN
-        default: break;
-    }
-}
-
-// Issue the loads of one full segment: lane l takes elements (j * 64 + l) VN + e, as 16-byte
-// vectors when the operand is aligned (VEC), else one by one -- the same elements either way, so
-// alignment changes the speed only, never the bits.
-template <typename T, int VPT, bool VEC>
-__device__ __forceinline__ void lean_load(const T *base, T (&y)[VPT * Vec16<T>::N]) {
-    typedef typename Vec16<T>::type V;
-    constexpr int VN = Vec16<T>::N;
-    const unsigned lane = threadIdx.x & 63;
-    if constexpr (VEC) {
-        const V *bv = reinterpret_cast<const V *>(base) + lane;
-#pragma unroll
-        for (int j = 0; j < VPT; ++j) {
-            const V v = __builtin_nontemporal_load(bv + j * 64);
-#pragma unroll
-            for (int e = 0; e < VN; ++e) y[j * VN + e] = v[e];
-        }
-    } else {
-#pragma unroll
-        for (int j = 0; j < VPT; ++j)
-#pragma unroll
-            for (int e = 0; e < VN; ++e) y[j * VN + e] = base[(j * 64 + lane) * VN + e];
-    }
-}
-
-// Fold segment j of the current chunk (loaded into y, and z for the second operand).
-template <typename T, int VPT, int FLAGS, int LEAVES>
-__device__ __forceinline__ void lean_fold(const LeanProg &pg, T (&y)[VPT * Vec16<T>::N], T (&z)[VPT * Vec16<T>::N],
-                                          unsigned j, LaneAcc<T> &a, unsigned &fnan) {
-    constexpr int M = VPT * Vec16<T>::N;
-    apply_chain_n(pg.pre0, y);
-    if constexpr (LEAVES == 2) {
-        apply_chain_n(pg.pre1, z);
-        bop_n(pg.op, y, z);
-        apply_chain_n(pg.post, y);
-    }
-    if (j == 0) fnan = __builtin_isnan(y[0]) ? 1u : 0u;  // lane 0's value is the chunk's first element
-    const T st = pairwise<M, 0, T>([&](int i) { return y[i]; });
-    if (__builtin_isnan(st)) {  // rare: tell a NaN element apart from inf - inf
-#pragma unroll
-        for (int i = 0; i < M; ++i) a.nan |= __builtin_isnan(y[i]) ? 1u : 0u;
-    }
-    const double sb = (double)st;
-    if (FLAGS & F_M2) {
-        const T mt = rn_mul(st, (T)(1.0 / M));  // the segment mean (exact: a power of two)
-        T q4[4] = {(T)0, (T)0, (T)0, (T)0};
-#pragma unroll
-        for (int i = 0; i < M; ++i) {
-            const T d = rn_sub(y[i], mt);
-            q4[i & 3] = fma(d, d, q4[i & 3]);
-        }
-        const double qt = (double)rn_add(rn_add(q4[0], q4[1]), rn_add(q4[2], q4[3]));
-        if (j == 0) {
-            a.m2 = qt;
-        } else {  // Chan with n_a = 16 j, n_b = 16: (S_a - j S_b)^2 / (16 j (j + 1))
-            const double d = (a.hi + a.lo) - (double)j * sb;
-            a.m2 = a.m2 + qt + d * d * kLeanW.w[j];
-        }
-    }
-    if constexpr (sizeof(T) == 8) {
-        double s, e;
-        two_sum(a.hi, sb, s, e);
-        e = e + a.lo;
-        a.hi = s + e;
-        a.lo = e - (a.hi - s);
-    } else {
-        a.hi += sb;
-    }
-    if (FLAGS & F_MAX) {
-        unsigned ci = NO_CODE;
-#pragma unroll
-        for (int i = 0; i < M; ++i) {
-            const bool gt = y[i] > a.vmax;
-            a.vmax = gt ? y[i] : a.vmax;
-            ci = gt ? (unsigned)i : ci;
-        }
-        a.cmax = ci != NO_CODE ? j * M + ci : a.cmax;
-    }
-    if (FLAGS & F_MIN) {
-        unsigned ci = NO_CODE;
-#pragma unroll
-        for (int i = 0; i < M; ++i) {
-            const bool lt = y[i] < a.vmin;
-            a.vmin = lt ? y[i] : a.vmin;
-            ci = lt ? (unsigned)i : ci;
-        }
-        a.cmin = ci != NO_CODE ? j * M + ci : a.cmin;
-    }
-}
-
-// Butterfly of records of equal counts n (every lane of a chunk): M2 += (S_a - S_b)^2 / (2 n).
-template <typename T, int FLAGS>
-__device__ __forceinline__ void lean_wave_merge(Part &a) {
-    double w = 0.5 / a.n;
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) {
-        const Part b = part_shfl<FLAGS>(a, m);
-        a.nan |= b.nan;
-        if (FLAGS & F_MAX) take_max(a.vmax, a.imax, b.vmax, b.imax);
-        if (FLAGS & F_MIN) take_min(a.vmin, a.imin, b.vmin, b.imin);
-        if (FLAGS & F_M2) {
-            const double d = (a.hi + a.lo) - (b.hi + b.lo);
-            a.m2 = a.m2 + b.m2 + d * d * w;
-        }
-        if constexpr (sizeof(T) == 8) dw_add(a.hi, a.lo, b.hi, b.lo);
-        else a.hi += b.hi;
-        a.n += b.n;
-        w *= 0.5;
-    }
-}
-
-template <typename T, int VPT, int FLAGS, int LEAVES, bool VEC>
-__global__ __launch_bounds__(RB) void lean_kernel(LeanParams<T> p) {
-    constexpr int VN = Vec16<T>::N;
-    constexpr int M = VPT * VN;
-    constexpr int WSEG = 64 * M;
-    __shared__ Part grp_lds[WAVES];
-    __shared__ unsigned grp_fn[WAVES];
-    const unsigned lane = threadIdx.x & 63;
-    const unsigned wave = lane0((unsigned)(threadIdx.x >> 6));
-    const size_t gchunks = (size_t)WAVES * p.cpw;
-    const size_t c_begin = blockIdx.x * gchunks + (size_t)wave * p.cpw;
-    const size_t c_end = min(p.nchunk, c_begin + (size_t)p.cpw);
-    const size_t nseg = c_end > c_begin ? (c_end - c_begin) * p.cl : 0;
-    Part wp = part_empty();  // this wavefront's chunks, merged in chunk order
-    unsigned wp_fn = 0;
-    Part keep = part_empty();  // lane k: the record of the wavefront's k-th chunk
-    unsigned keep_fn = 0;
-    if (nseg > 0) {
-        const T *b0 = p.x0 + c_begin * p.cl * WSEG;
-        const T *b1 = LEAVES == 2 ? p.x1 + c_begin * p.cl * WSEG : nullptr;
-        LaneAcc<T> a;
-        acc_reset(a);
-        unsigned fnan = 0, j = 0, k = 0;
-        T ya[M], yb[M], za[M], zb[M];
-        auto load = [&](T(&y)[M], T(&z)[M], size_t s) {
-            lean_load<T, VPT, VEC>(b0 + s * WSEG, y);
-            if constexpr (LEAVES == 2) lean_load<T, VPT, VEC>(b1 + s * WSEG, z);
-        };
-        // after segment j of chunk k: close the chunk when it is the last
-        auto step = [&](T(&y)[M], T(&z)[M]) {
-            lean_fold<T, VPT, FLAGS, LEAVES>(p.prog, y, z, j, a, fnan);
-            if (++j == p.cl) {
-                const size_t c = c_begin + k;
-                const uint64_t fbase = (uint64_t)c * p.cl * WSEG;
-                auto flat = [&](unsigned code) -> uint64_t {
-                    const unsigned sl = code / M, i = code % M;
-                    return fbase + (uint64_t)sl * WSEG + (uint64_t)(((i / VN) * 64 + lane) * VN + (i % VN));
-                };
-                Part pa;
-                pa.n = (double)M * (double)p.cl;
-                pa.hi = a.hi; pa.lo = a.lo; pa.m2 = a.m2; pa.nan = a.nan;
-                pa.vmax = a.cmax == NO_CODE ? -__builtin_inf() : (double)a.vmax;
-                pa.imax = a.cmax == NO_CODE ? NO_INDEX : flat(a.cmax);
-                pa.vmin = a.cmin == NO_CODE ? __builtin_inf() : (double)a.vmin;
-                pa.imin = a.cmin == NO_CODE ? NO_INDEX : flat(a.cmin);
-                lean_wave_merge<T, FLAGS>(pa);
-                const unsigned fn0 = (unsigned)__shfl(fnan, 0, 64);
-                if (lane == k) {
-                    keep = pa;
-                    keep_fn = fn0;
-                }
-                if (k == 0) wp_fn = fn0;
-                part_merge<FLAGS>(wp, pa);
-                acc_reset(a);
-                fnan = 0;
-                j = 0;
-                ++k;
-            }
-        };
-        load(ya, za, 0);
-        // the next segment's loads are issued before the current one is folded; past the last
-        // segment they reread it (clamped address), so there is no branch around a load
-        for (size_t s = 0;;) {
-            load(yb, zb, s + 1 < nseg ? s + 1 : s);
-            step(ya, za);
-            if (++s == nseg) break;
-            load(ya, za, s + 1 < nseg ? s + 1 : s);
-            step(yb, zb);
-            if (++s == nseg) break;
-        }
-        if (p.row_direct && lane < (unsigned)(c_end - c_begin)) {
-            const size_t r = c_begin + lane;  // a chunk is a row
-            write_outputs<T>(p.row, r, keep, p.ddof, keep_fn, (uint64_t)r * p.cols);
-        }
-    }
-    if (!p.need_group) return;  // uniform over the grid
-    if (lane == 0) {
-        grp_lds[wave] = wp;
-        grp_fn[wave] = wp_fn;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        Part g = grp_lds[0];
-        for (int w = 1; w < WAVES; ++w) part_merge<FLAGS>(g, grp_lds[w]);
-        part_store<FLAGS>(p.grp, blockIdx.x, g, grp_fn[0]);
-    }
-}
-
-// The whole-array result from the group records, merged in a fixed order: thread t folds the
-// contiguous records [t per, (t + 1) per), then the wavefront butterflies, then wavefront order.
-template <typename T, int FLAGS>
-__global__ __launch_bounds__(FB) void lean_finish_kernel(Parts in, size_t count, OutPtrs<T> o, double ddof) {
-    __shared__ Part lds[FB / 64];
-    const unsigned t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const size_t per = (count + FB - 1) / FB;
-    Part a = part_empty();
-    for (size_t i = t * per; i < min(count, (size_t)(t + 1) * per); ++i) part_merge<FLAGS>(a, part_load<FLAGS>(in, i));
-    wave_merge<FLAGS>(a);
-    if (lane == 0) lds[w] = a;
-    __syncthreads();
-    if (t == 0) {
-        Part r = lds[0];
-        for (int k = 1; k < FB / 64; ++k) part_merge<FLAGS>(r, lds[k]);
-        write_outputs<T>(o, 0, r, ddof, (in.flags[0] >> 1) & 1u, 0);
-    }
-}
-
-// Chunk length for a whole-array lean reduction of spr full segments: the smallest divisor of spr
-// that leaves at most CHUNK_TARGET chunks (0 if none up to the weight table's length).
-static size_t lean_chunk_len(size_t spr) {
-    const size_t lo = std::max<size_t>(1, ceil_div(spr, CHUNK_TARGET));
-    for (size_t cl = lo; cl < (size_t)LEAN_WTAB && cl <= spr; ++cl)
-        if (spr % cl == 0) return cl;
-    return 0;
-}
-
-// Chunks per wavefront, from the shape only (it fixes the group records, hence the whole-array
-// result's bits): enough workgroups to fill a 256-CU chip several times over.
-constexpr size_t LEAN_MIN_CHUNKS = 1024;
-static int lean_cpw(size_t nchunk) {
-    return nchunk >= 65536 ? 4 : (nchunk >= 16384 ? 2 : 1);
-}
-
-template <typename T, int VPT, int FLAGS, int LEAVES>
-static nrk_status launch_lean(const LeanParams<T> &p, size_t ngroups, const OutPtrs<T> &full, hipStream_t stream) {
-    const bool vec = (uintptr_t)p.x0 % 16 == 0 && (LEAVES == 1 || (uintptr_t)p.x1 % 16 == 0);
-    if (vec)
-        hipLaunchKernelGGL((lean_kernel<T, VPT, FLAGS, LEAVES, true>), dim3((unsigned)ngroups), dim3(RB), 0, stream, p);
-    else
-        hipLaunchKernelGGL((lean_kernel<T, VPT, FLAGS, LEAVES, false>), dim3((unsigned)ngroups), dim3(RB), 0, stream, p);
-    NRK_HIP(hipGetLastError());
-    if (p.need_group) {
-        hipLaunchKernelGGL((lean_finish_kernel<T, FLAGS>), dim3(1), dim3(FB), 0, stream, p.grp, ngroups, full, p.ddof);
-        NRK_HIP(hipGetLastError());
-    }
-    return NRK_OK;
-}
-
-// The lean path, if the shape and operands allow it (else returns false in *taken). x1 != NULL:
-// two operands (prog.op and the three chains), else one (prog.pre0).
-template <typename T>
-static nrk_status try_lean(const LeanProg &prog, const T *x0, const T *x1, size_t rows, size_t cols, double ddof,
-                           const OutPtrs<T> &row, const OutPtrs<T> &full, bool want_row, bool want_full, hipStream_t stream,
-                           const ExtScratch *ext, bool *taken) {
-    constexpr int M = NRK_REDUCE_M;
-    constexpr int VPT = M / Vec16<T>::N;
-    constexpr size_t WSEG = 64 * M;
-    *taken = false;
-    // the choice depends on the shape only (never on alignment or the device), so a given shape
-    // always gets the same summation tree
-    size_t cl, nchunk;
-    if (want_row) {  // a chunk is a row
-        if (cols % WSEG != 0 || cols / WSEG >= (size_t)LEAN_WTAB || rows < LEAN_MIN_CHUNKS) return NRK_OK;
-        cl = cols / WSEG;
-        nchunk = rows;
-    } else {
-        const size_t n = rows * cols;
-        if (n % WSEG != 0) return NRK_OK;
-        cl = lean_chunk_len(n / WSEG);
-        if (cl == 0) return NRK_OK;
-        nchunk = n / WSEG / cl;
-        if (nchunk < LEAN_MIN_CHUNKS) return NRK_OK;
-    }
-    LeanParams<T> p{};
-    p.x0 = x0;
-    p.x1 = x1;
-    p.nchunk = nchunk;
-    p.cl = cl;
-    p.cols = cols;
-    p.cpw = lean_cpw(nchunk);
-    p.prog = prog;
-    p.ddof = ddof;
-    p.row_direct = want_row;
-    p.row = want_row ? row : OutPtrs<T>{};
-    p.need_group = want_full;
-    const size_t ngroups = ceil_div(nchunk, (size_t)WAVES * p.cpw);
-    if (ngroups > 0x7fffffffu) return NRK_OK;
-    if (want_full) {
-        const size_t pbytes = parts_bytes(ngroups) + 256;
-        char *b = nullptr;
-        if (ext) {
-            if (pbytes + 256 > ext->bytes)
-                return fail(NRK_ERR_INVALID_ARGUMENT, "(%zu, %zu) needs %zu bytes of scratch; the fixed scratch holds %zu", rows, cols,
-                            pbytes + 256, ext->bytes);
-            b = (char *)ext->base + 256;
-        } else {
-            Scratch s;
-            NRK_TRY(get_scratch(stream, 1, pbytes, &s));
-            b = (char *)s.partials;
-        }
-        p.grp = carve_parts(b, ngroups);
-    }
-    const bool m2 = (row.var || full.var);
-    const bool mx = (row.max || row.argmax || full.max || full.argmax);
-    const bool mn = (row.min || row.argmin || full.min || full.argmin);
-    const int flags = (m2 ? F_M2 : 0) | (mx ? F_MAX : 0) | (mn ? F_MIN : 0);
-    *taken = true;
-#define NRK_LEAN(F)                                                                             \
-    case F:                                                                                     \
-        return x1 ? launch_lean<T, VPT, F, 2>(p, ngroups, full, stream) : launch_lean<T, VPT, F, 1>(p, ngroups, full, stream);
-    switch (flags) {
-        NRK_LEAN(0) NRK_LEAN(1) NRK_LEAN(2) NRK_LEAN(3) NRK_LEAN(4) NRK_LEAN(5) NRK_LEAN(6)
-        default: NRK_LEAN(7)
-    }
-#undef NRK_LEAN
-}
-
-// Host launcher for the fused map -> reduce over a (rows, cols) matrix.
 __global__ void clear_counter_kernel(unsigned int *c) { *c = 0u; }
 
 // x1 != NULL (with prog2): the two-operand lean form; *no_lean is set (and nothing launched)
