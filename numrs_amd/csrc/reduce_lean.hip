@@ -8,9 +8,9 @@ namespace nrk {
 // Used when every segment is whole (cols, or rows * cols for a whole-array reduction, is a
 // multiple of the segment), there are at least LEAN_MIN_CHUNKS chunks, and a chunk is a whole row
 // when row results are wanted. Then every chunk holds exactly cl full segments, and:
-//   - chunk c is the contiguous elements [c cl WSEG, (c + 1) cl WSEG); wavefront v of the W in
-//     the grid reduces chunks v, v + W, ... with one pointer, loading the next segment while it
-//     folds the current one;
+//   - chunk c is the contiguous elements [c cl WSEG, (c + 1) cl WSEG); each wavefront reduces an
+//     arithmetic sequence of chunks (round robin or a contiguous block, see lean_layout) with one
+//     pointer, loading the next segment while it folds the current one;
 //   - every lane of a chunk has the same count (16 per segment), so the Chan updates need no
 //     division: the weight of the j-th segment's merge, 1 / (16 j (j + 1)), comes from a constant
 //     table, and the butterfly's levels merge equal counts with weight 1 / (2 n);
@@ -42,7 +42,8 @@ struct LeanParams {
     const T *x0, *x1;
     double s0, s1;    // CH_MULADD's scalars
     size_t nchunk, cl, cols;
-    size_t W;         // wavefronts of the grid (4 per workgroup); wavefront v reduces chunks v, v + W, ...
+    size_t A, B;      // wavefront v reduces chunks v A + k B: A = 1, B = W (round robin over the W
+                      // wavefronts of the grid) or A = chunks per wavefront, B = 1 (contiguous blocks)
     LeanProg prog;
     double ddof;
     int row_direct;   // a chunk is a row and row results are wanted
@@ -197,9 +198,10 @@ __global__ __launch_bounds__(RB) void lean_kernel(LeanParams<T> p) {
     const unsigned lane = threadIdx.x & 63;
     const unsigned wave = lane0((unsigned)(threadIdx.x >> 6));
     const size_t v = blockIdx.x * (size_t)WAVES + wave;
-    const size_t nck = v < p.nchunk ? (p.nchunk - v + p.W - 1) / p.W : 0;  // chunks v, v + W, ...
+    const size_t c0 = v * p.A;  // chunks c0, c0 + B, ...
+    const size_t nck = c0 >= p.nchunk ? 0 : (p.A == 1 ? (p.nchunk - c0 + p.B - 1) / p.B : min(p.A, p.nchunk - c0));
     const size_t nseg = nck * p.cl;
-    const size_t cstride = p.W * p.cl * WSEG;  // elements from one of this wavefront's chunks to the next
+    const size_t cstride = p.B * p.cl * WSEG;  // elements from one of this wavefront's chunks to the next
     Part wp = part_empty();  // this wavefront's chunks, merged in chunk order
     unsigned wp_fn = 0;
     if (nseg > 0) {
@@ -208,9 +210,9 @@ __global__ __launch_bounds__(RB) void lean_kernel(LeanParams<T> p) {
         unsigned fnan = 0, j = 0;
         size_t k = 0;
         T ya[M], yb[M], za[M], zb[M];
-        // segment t of this wavefront: chunk v + (t / cl) W, segment t % cl of it
-        const T *n0 = p.x0 + v * p.cl * WSEG;  // the next segment to load
-        const T *n1 = LEAVES == 2 ? p.x1 + v * p.cl * WSEG : nullptr;
+        // segment t of this wavefront: chunk c0 + (t / cl) B, segment t % cl of it
+        const T *n0 = p.x0 + c0 * p.cl * WSEG;  // the next segment to load
+        const T *n1 = LEAVES == 2 ? p.x1 + c0 * p.cl * WSEG : nullptr;
         unsigned nj = 0;
         size_t left = nseg;  // segments not yet loaded
         auto load = [&](T(&y)[M], T(&z)[M]) {
@@ -231,8 +233,8 @@ __global__ __launch_bounds__(RB) void lean_kernel(LeanParams<T> p) {
         const T s0 = (T)p.s0, s1 = (T)p.s1;
         auto step = [&](T(&y)[M], T(&z)[M]) {
             lean_fold<T, VPT, FLAGS, LEAVES, CH>(p.prog, s0, s1, y, z, j, a, fnan);
-            if (++j == p.cl) {  // chunk c = v + k W is complete
-                const size_t c = v + k * p.W;
+            if (++j == p.cl) {  // chunk c = c0 + k B is complete
+                const size_t c = c0 + k * p.B;
                 const uint64_t fbase = (uint64_t)c * p.cl * WSEG;
                 auto flat = [&](unsigned code) -> uint64_t {
                     const unsigned sl = code / M, i = code % M;
@@ -308,15 +310,29 @@ static size_t lean_chunk_len(size_t spr) {
     return 0;
 }
 
-// Workgroups of the lean grid, from the shape only (it fixes which chunks a group record holds,
-// hence the whole-array result's bits): at most 512 (two per CU of a 256-CU chip; the finish
-// merges one record per thread), at least 4 chunks per workgroup. Wavefront v reduces chunks v,
-// v + W, ... (W = 4 workgroups), so at any moment the grid reads one compact window of the array:
-// measured faster than contiguous blocks of chunks per wavefront (f32 8192 x 8192: 41-45 against
-// 49-50 us), which keep far more DRAM pages open at once.
+// The chunk-to-wavefront layout, from the dtype and the shape only (it fixes which chunks a group
+// record holds, hence the whole-array result's bits). Measured on 8192 x 8192 (stream kernel
+// alone): binary32 reads faster when wavefront v takes chunks v, v + W, ... (W the grid's
+// wavefronts; the grid reads one compact window at a time): 41-45 us against 49-50 with
+// contiguous blocks; binary64 the other way round (8 consecutive chunks per wavefront: 79-80 us
+// against 84-86). At most 512 workgroups (the finish merges one record per thread).
 constexpr size_t LEAN_MIN_CHUNKS = 1024;
-static size_t lean_groups(size_t nchunk) {
-    return std::min<size_t>(512, ceil_div(nchunk, (size_t)4 * WAVES));
+struct LeanLayout {
+    size_t groups, A, B;
+};
+template <typename T>
+static LeanLayout lean_layout(size_t nchunk) {
+    LeanLayout L;
+    if (sizeof(T) == 4) {
+        L.groups = std::min<size_t>(512, ceil_div(nchunk, (size_t)4 * WAVES));
+        L.A = 1;
+        L.B = L.groups * WAVES;
+    } else {
+        L.A = std::max<size_t>(8, ceil_div(nchunk, (size_t)512 * WAVES));
+        L.groups = ceil_div(nchunk, L.A * WAVES);
+        L.B = 1;
+    }
+    return L;
 }
 
 template <typename T, int VPT, int FLAGS, int LEAVES, int CH>
@@ -380,8 +396,10 @@ nrk_status try_lean(const LeanProg &prog, const T *x0, const T *x1, size_t rows,
     p.nchunk = nchunk;
     p.cl = cl;
     p.cols = cols;
-    const size_t ngroups = lean_groups(nchunk);
-    p.W = ngroups * WAVES;
+    const LeanLayout L = lean_layout<T>(nchunk);
+    const size_t ngroups = L.groups;
+    p.A = L.A;
+    p.B = L.B;
     p.prog = prog;
     if (!x1 && prog.pre0.len == 2) {  // CH_MULADD: y = (x * s0) + s1, with x - b as x + (-b)
         p.s0 = prog.pre0.s[0];
