@@ -209,6 +209,9 @@ def main() -> int:
     kernels_out["fused_map_reduce_f64"]["errors"] = fused_errors(x64, k64)
     log("fused errors: f32 " + json.dumps(kernels_out["fused_map_reduce_f32"]["errors"]) + " f64 "
         + json.dumps(kernels_out["fused_map_reduce_f64"]["errors"]))
+    kernels_out["plain_vs_compensated"] = plain_vs_compensated(device, call, ReduceOutputs, ops_p, n_ops, s, stream,
+                                                               ((x32, d32), (x64, d64)), args.steps)
+    log("plain vs compensated: " + json.dumps(kernels_out["plain_vs_compensated"]))
     del x64
 
     # host-pointer boundary (the reference's &[T] slices): sum of x (f32, 256 MiB) through
@@ -258,6 +261,61 @@ def main() -> int:
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def _exact_stats(y):
+    """Row and whole-array sums and population variances of y in 80-bit long double (pairwise)."""
+    LD = np.longdouble
+    yl = y.astype(LD)
+    n = y.shape[1]
+    row_sum = yl.sum(axis=1)
+    row_var = ((yl - (row_sum / n)[:, None]) ** 2).sum(axis=1) / n
+    full_sum = row_sum.sum()
+    full_var = ((yl - full_sum / y.size) ** 2).sum() / y.size
+    return row_sum, row_var, full_sum, full_var
+
+
+def plain_vs_compensated(device, call, ReduceOutputs, ops_p, n_ops, s, stream, data, steps) -> dict:
+    """The same y = 1.5 x - 0.25 -> row and whole-array sum and var (ddof 0) through the plain call
+    (the reference's own order of rounded additions: chunked simd_sum + left-to-right folds, several
+    passes) and the compensated one (one pass, double-word merges): time per call and max relative
+    error of each against long double."""
+    out = {}
+    for x, dx in data:
+        dt = x.dtype.type
+        suf = "f32" if dt == np.float32 else "f64"
+        bufs = {k: device.DeviceArray((ROWS if k.startswith("row") else 1,), dt)
+                for k in ("row_sum", "row_var", "full_sum", "full_var")}
+        ro, fo = ReduceOutputs(), ReduceOutputs()
+        ro.sum, ro.var = bufs["row_sum"].ptr, bufs["row_var"].ptr
+        fo.sum, fo.var = bufs["full_sum"].ptr, bufs["full_var"].ptr
+        y = ((x * dt(1.5)).astype(dt) + dt(-0.25)).astype(dt)
+        ex = _exact_stats(y)
+        del y
+        entry = {}
+        for mode, fn in (("plain", f"nrk_dev_map_reduce_plain_{suf}"), ("compensated", f"nrk_dev_map_reduce_{suf}")):
+            def once():
+                call(fn, ops_p, n_ops, C.c_void_p(dx.ptr), ROWS, COLS, 0, C.byref(ro), C.byref(fo), C.c_void_p(s))
+            once()
+            stream.synchronize()
+            a, b = device.Event(), device.Event()
+            a.record(s)
+            for _ in range(steps):
+                once()
+            b.record(s)
+            stream.synchronize()
+            ms = a.elapsed_ms(b) / steps
+            got = [bufs[k].download() for k in ("row_sum", "row_var", "full_sum", "full_var")]
+            LD = np.longdouble
+
+            def rel(g, e):
+                return float(f"{float(np.max(np.abs(np.asarray(g, LD) - e) / np.abs(e))):.3e}")
+            entry[mode] = {"ms": round(ms, 5), "GB/s_one_read": round(x.nbytes / (ms * 1e-3) / 1e9, 1),
+                           "row_sum_max_rel_err": rel(got[0], ex[0]), "row_var_max_rel_err": rel(got[1], ex[1]),
+                           "full_sum_rel_err": rel(got[2][0], ex[2]), "full_var_rel_err": rel(got[3][0], ex[3])}
+        entry["fields"] = "row and whole-array sum, var (ddof 0)"
+        out[suf] = entry
+    return out
 
 
 def fused_errors(x, keep) -> dict:

@@ -461,6 +461,23 @@ nrk_status nrk_dev_map_reduce_f32(const nrk_scalar_op *chain, int32_t chain_len,
                                   size_t ddof, const nrk_reduce_outputs *row_out, const nrk_reduce_outputs *full_out,
                                   nrk_stream stream);
 
+/* The same call with every sum, mean and variance computed in the reference's own order of
+ * rounded additions, so each is bit-identical to the reference's CPU result on y:
+ *   full sum/mean: kernels/reduce.rs:207-232 (simd_sum below 10000 elements; else simd_sum of
+ *     each 8192-element chunk, folded left to right), mean = sum / n;
+ *   full var: kernels/reduce.rs:284-314 (one tier decision at 40000 elements for both passes);
+ *   row sum/mean/var: math/aggregation.rs:548-561, math/statistics/mod.rs:1064-1071 and
+ *     :1403-1494 (left-to-right folds along the row; var = sum((y - mean)^2) / (cols - ddof)).
+ * min/max/argmin/argmax do not depend on an order of additions and come from the fused kernel.
+ * Several passes over x (two per requested sum-and-variance); the compensated call above is the
+ * faster and more accurate one. */
+nrk_status nrk_dev_map_reduce_plain_f64(const nrk_scalar_op *chain, int32_t chain_len, const double *x, size_t rows,
+                                        size_t cols, size_t ddof, const nrk_reduce_outputs *row_out,
+                                        const nrk_reduce_outputs *full_out, nrk_stream stream);
+nrk_status nrk_dev_map_reduce_plain_f32(const nrk_scalar_op *chain, int32_t chain_len, const float *x, size_t rows,
+                                        size_t cols, size_t ddof, const nrk_reduce_outputs *row_out,
+                                        const nrk_reduce_outputs *full_out, nrk_stream stream);
+
 /* No counterpart in the reference. For callers that cannot allocate device memory (code under
  * hipGraph capture, tight serving loops, harnesses that forbid hipMalloc): nrk_dev_map_reduce_f64
  * with the same arguments and results, whose scratch (chunk records, record-tree buffers, the

@@ -142,6 +142,8 @@ _SIGS: dict[str, list] = {
     "nrk_dev_gemm_generic_i64": [c_size, c_size, c_size, c_size, vp, c_size, vp, c_size, vp, vp],
     "nrk_dev_gemm_generic_i32": [c_size, c_size, c_size, c_size, vp, c_size, vp, c_size, vp, vp],
     "nrk_dev_map_reduce_f32": [vp, c_i32, vp, c_size, c_size, c_size, vp, vp, vp],
+    "nrk_dev_map_reduce_plain_f64": [vp, c_i32, vp, c_size, c_size, c_size, vp, vp, vp],
+    "nrk_dev_map_reduce_plain_f32": [vp, c_i32, vp, c_size, c_size, c_size, vp, vp, vp],
     # gemm
     "nrk_gemm_2d_f64": [c_size, c_size, c_size, vp, c_size, vp, c_size, vp, c_size],
     "nrk_gemm_2d_f32": [c_size, c_size, c_size, vp, c_size, vp, c_size, vp, c_size],

@@ -215,6 +215,14 @@ nrk_status dev_map_reduce(const nrk_scalar_op *chain, int32_t chain_len, const T
 }
 
 template <typename T>
+nrk_status dev_map_reduce_plain(const nrk_scalar_op *chain, int32_t chain_len, const T *x, size_t rows, size_t cols, size_t ddof,
+                                const nrk_reduce_outputs *row_out, const nrk_reduce_outputs *full_out, hipStream_t s) {
+    Chain c;
+    NRK_TRY(make_chain(chain, chain_len, &c));
+    return plain_reduce<T>(c, x, rows, cols, ddof, row_out, full_out, s);
+}
+
+template <typename T>
 nrk_status host_outer(int32_t op, const T *a, size_t na, const T *b, size_t nb, T *y) {
     NRK_TRY(ensure_device());
     if (nb && !b) return fail(NRK_ERR_INVALID_ARGUMENT, "b is NULL");
@@ -448,6 +456,16 @@ nrk_status nrk_dev_map_reduce_f64(const nrk_scalar_op *chain, int32_t chain_len,
 nrk_status nrk_dev_map_reduce_f32(const nrk_scalar_op *chain, int32_t chain_len, const float *x, size_t rows, size_t cols, size_t ddof,
                                   const nrk_reduce_outputs *row_out, const nrk_reduce_outputs *full_out, nrk_stream stream) {
     return dev_map_reduce<float>(chain, chain_len, x, rows, cols, ddof, row_out, full_out, S(stream));
+}
+nrk_status nrk_dev_map_reduce_plain_f64(const nrk_scalar_op *chain, int32_t chain_len, const double *x, size_t rows, size_t cols,
+                                        size_t ddof, const nrk_reduce_outputs *row_out, const nrk_reduce_outputs *full_out,
+                                        nrk_stream stream) {
+    return dev_map_reduce_plain<double>(chain, chain_len, x, rows, cols, ddof, row_out, full_out, S(stream));
+}
+nrk_status nrk_dev_map_reduce_plain_f32(const nrk_scalar_op *chain, int32_t chain_len, const float *x, size_t rows, size_t cols,
+                                        size_t ddof, const nrk_reduce_outputs *row_out, const nrk_reduce_outputs *full_out,
+                                        nrk_stream stream) {
+    return dev_map_reduce_plain<float>(chain, chain_len, x, rows, cols, ddof, row_out, full_out, S(stream));
 }
 nrk_status nrk_dev_expr_reduce_f64(const nrk_expr_node *nodes, int32_t len, const double *const *leaves, int32_t n_leaves,
                                    size_t rows, size_t cols, size_t ddof, const nrk_reduce_outputs *row_out,

@@ -13,6 +13,12 @@ nrk_status seg_reduce(const Chain &chain, const T *x, size_t rows, size_t cols, 
                       const nrk_reduce_outputs *row_out, const nrk_reduce_outputs *full_out, hipStream_t stream,
                       const ExtScratch *ext = nullptr);
 
+// The same outputs with every sum, mean and variance in the reference's own order of rounded
+// additions (reduce_plain.hip): bit-identical to the reference's CPU kernels.
+template <typename T>
+nrk_status plain_reduce(const Chain &chain, const T *x, size_t rows, size_t cols, size_t ddof,
+                        const nrk_reduce_outputs *row_out, const nrk_reduce_outputs *full_out, hipStream_t stream);
+
 // The fixed device scratch of the calling thread's device (set up by nrk_init, runtime.hip).
 nrk_status static_scratch(ExtScratch *out);
 

@@ -19,7 +19,7 @@ EPS = {np.float64: 2.0 ** -53, np.float32: 2.0 ** -24}
 FIELDS = ("sum", "mean", "var", "min", "max", "argmin", "argmax")
 
 
-def run_fused(x: np.ndarray, steps, ddof=0, want_rows=True, want_full=True, offset_elems=0):
+def run_fused(x: np.ndarray, steps, ddof=0, want_rows=True, want_full=True, offset_elems=0, plain=False, fields=FIELDS):
     rows, cols = x.shape
     dt = x.dtype
     dx = DeviceArray((x.size + offset_elems,), dt)
@@ -29,7 +29,7 @@ def run_fused(x: np.ndarray, steps, ddof=0, want_rows=True, want_full=True, offs
     xptr = dx.ptr + offset_elems * dt.itemsize
     outs = {}
     ro, fo = ReduceOutputs(), ReduceOutputs()
-    for f in FIELDS:
+    for f in fields:
         odt = np.uint64 if f.startswith("arg") else dt
         if want_rows:
             outs["row_" + f] = DeviceArray((rows,), odt)
@@ -38,7 +38,7 @@ def run_fused(x: np.ndarray, steps, ddof=0, want_rows=True, want_full=True, offs
             outs["full_" + f] = DeviceArray((1,), odt)
             setattr(fo, f, outs["full_" + f].ptr)
     ops, n_ops = K.chain(*steps)
-    fn = "nrk_dev_map_reduce_f64" if dt == np.float64 else "nrk_dev_map_reduce_f32"
+    fn = ("nrk_dev_map_reduce_plain_" if plain else "nrk_dev_map_reduce_") + ("f64" if dt == np.float64 else "f32")
     call(fn, C.cast(ops, C.c_void_p), C.c_int32(n_ops), C.c_void_p(xptr), C.c_size_t(rows), C.c_size_t(cols),
          C.c_size_t(ddof), C.byref(ro) if want_rows else None, C.byref(fo) if want_full else None, None)
     call("nrk_device_synchronize")
