@@ -6,7 +6,7 @@
 // x2 = bf16(x - x0 - x1); each difference is exact in f32. The product keeps the pairs
 //   BF16x3: A1*B0 + A0*B1 + A0*B0              (3 MFMA products, drops terms of order 2^-16)
 //   BF16x6: A2*B0 + A1*B1 + A0*B2 + A1*B0 + A0*B1 + A0*B0   (drops terms of order 2^-24)
-// accumulated smallest-first into one FP32 accumulator per output.
+// accumulated smallest-first (BF16x6: A0*B0 in an accumulator of its own, see ACC_SPLIT).
 //
 // Data path: a split pass writes the terms as bf16 matrices zero-padded to the tile grid, A as
 // (Mp, Kp) and B TRANSPOSED as (Np, Kp) (K contiguous for both, so every MFMA operand is one
@@ -77,7 +77,16 @@ __device__ __forceinline__ constexpr int pair_b(int p) {
 // pairs smallest first.
 constexpr int BF_BK = 32;  // bf16 values per K tile (one MFMA K step)
 
-template <int BM, int BN, int WM, int WN, int NTA, int NTB, int NPAIR, int NBUF>
+// Accumulation of the pairs (ACC): ACC_ONE, every pair into one FP32 accumulator per output;
+// ACC_SPLIT, A0*B0 into its own accumulator and the smaller pairs into a second, added once at
+// the end, so the small terms never round against the large running sum. Measured at K = 8192
+// (N(0,1) operands, 2048 sampled outputs, error / sum_k |a_k b_k|): BF16x6 goes from 1.4e-7 max
+// (2.5e-8 rms) with one accumulator to 4.9e-8 (9.8e-9 rms) split, against 1.8e-7 (2.8e-8) for
+// native FP32. BF16x3's error (3.4e-7 max, 7.7e-8 rms) is the pairs it drops (order 2^-16 of
+// each product) and is the same under either accumulation, so it keeps the one accumulator.
+enum { ACC_ONE = 0, ACC_SPLIT = 1 };
+
+template <int BM, int BN, int WM, int WN, int NTA, int NTB, int NPAIR, int NBUF, int ACC>
 __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_bf16_nt_kernel(
     int M, int N, int Kp, const unsigned short *__restrict__ A, size_t a_term_stride,
     const unsigned short *__restrict__ Bt, size_t b_term_stride, float *__restrict__ C, int ldc, int group_m) {
@@ -116,10 +125,17 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_bf16_nt_kerne
     };
 
     f32x4_t acc[TM][TN];
+    f32x4_t lo[ACC == ACC_SPLIT ? TM : 1][ACC == ACC_SPLIT ? TN : 1];  // ACC_SPLIT: the smaller pairs
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll
         for (int j = 0; j < TN; ++j) acc[i][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+    if constexpr (ACC == ACC_SPLIT) {
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j) lo[i][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+    }
 
     const int r16 = lane & 15;
     const int pc = ((lane >> 4) ^ lds_swz(r16)) * 16;  // stored position of this lane's chunk (row = r16 mod 16)
@@ -144,17 +160,29 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_bf16_nt_kerne
             for (int j = 0; j < TN; ++j)
                 fb[u][j] = *reinterpret_cast<const bf16x8_t *>(Bs + (u * BN + wn * WN + j * 16 + r16) * 64 + pc);
     };
-    // products of the tile held in fa / fb (per output: the pairs in order), A row block i then
-    // reloaded from tile tn
+    // products of the tile held in fa / fb (per output: the pairs in order; the last pair of the
+    // list is A0*B0), A row block i then reloaded from tile tn
     auto mma_reload = [&](const bf16x8_t(&fb)[NTB][TN], int tn) {
 #pragma unroll
         for (int i = 0; i < TM; ++i) {
+            if constexpr (ACC == ACC_ONE) {
 #pragma unroll
-            for (int p = 0; p < NPAIR; ++p)
+                for (int p = 0; p < NPAIR; ++p)
+#pragma unroll
+                    for (int j = 0; j < TN; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[pair_a<NPAIR>(p)][i], fb[pair_b<NPAIR>(p)][j],
+                                                                            acc[i][j], 0, 0, 0);
+            } else {
+#pragma unroll
+                for (int p = 0; p < NPAIR - 1; ++p)
+#pragma unroll
+                    for (int j = 0; j < TN; ++j)
+                        lo[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[pair_a<NPAIR>(p)][i], fb[pair_b<NPAIR>(p)][j],
+                                                                           lo[i][j], 0, 0, 0);
 #pragma unroll
                 for (int j = 0; j < TN; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[pair_a<NPAIR>(p)][i], fb[pair_b<NPAIR>(p)][j],
-                                                                        acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[0][i], fb[0][j], acc[i][j], 0, 0, 0);
+            }
             load_a(i, tn);
             __builtin_amdgcn_sched_barrier(0);
         }
@@ -194,7 +222,10 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_bf16_nt_kerne
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int row = bm0 + wm * WM + i * 16 + 4 * (lane >> 4) + r;
-                if (row < M && col < N) C[(size_t)row * ldc + col] = acc[i][j][r];
+                if (row < M && col < N) {
+                    if constexpr (ACC == ACC_SPLIT) C[(size_t)row * ldc + col] = acc[i][j][r] + lo[i][j][r];
+                    else C[(size_t)row * ldc + col] = acc[i][j][r];
+                }
             }
         }
 }
@@ -206,7 +237,11 @@ static nrk_status launch_bf16_nt(int M, int N, int Kp, const unsigned short *A, 
     constexpr int threads = (BM / WM) * (BN / WN) * 64;
     constexpr size_t lds = (size_t)NBUF * (NTA * BM + NTB * BN) * 64;
     static_assert(lds <= 160 * 1024, "tile does not fit in LDS");
-    auto kern = gemm_bf16_nt_kernel<BM, BN, WM, WN, NTA, NTB, NPAIR, NBUF>;
+    // the six-product form runs four-wave tiles only (the eight-wave ones exceed the LDS), where
+    // the second accumulator set fits; one ACC per product count keeps every tile's bits equal
+    constexpr int ACC = NPAIR == 6 ? ACC_SPLIT : ACC_ONE;
+    static_assert(ACC == ACC_ONE || threads <= 256, "split accumulators need a four-wave tile");
+    auto kern = gemm_bf16_nt_kernel<BM, BN, WM, WN, NTA, NTB, NPAIR, NBUF, ACC>;
     static bool attr_set = false;
     if (!attr_set) {
         NRK_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

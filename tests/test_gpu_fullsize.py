@@ -82,7 +82,8 @@ def test_split_gemm_8192_f32(gpu, oracle):
     """FP32 at 8192^3 on the benchmark's inputs: native FP32 (128 x 128 tile), BF16x3 (the
     128 x 256 split tile) and BF16x6 (128 x 128) on 512 sampled entries against the double-double
     oracle. Bars: native within gamma_k = k u / (1 - k u) of sum |A||B| (the classical dot-product
-    bound, u = 2^-24); BF16x6 <= 4x native + 2^-22; BF16x3 <= 2^-13."""
+    bound, u = 2^-24); BF16x6 (A0*B0 accumulated apart from the five smaller pairs) no less
+    accurate than native; BF16x3 <= 2^-13 (its dropped pairs are of order 2^-16 per product)."""
     n = 8192
     a = rng.scaled_pow2(rng.BASE_SEED + 40 + n, n * n, dtype=np.float32).reshape(n, n)
     b = rng.scaled_pow2(rng.BASE_SEED + 41 + n, n * n, dtype=np.float32).reshape(n, n)
@@ -94,5 +95,5 @@ def test_split_gemm_8192_f32(gpu, oracle):
     print(f"8192^3 f32 max error over 512 entries: native {e_nat:.3e}, bf16x3 {e3:.3e}, bf16x6 {e6:.3e}")
     u = 2.0 ** -24
     assert e_nat <= n * u / (1 - n * u)
-    assert e6 <= 4 * e_nat + 2.0 ** -22
+    assert e6 <= e_nat
     assert e3 <= 2.0 ** -13
