@@ -60,6 +60,7 @@ def parse():
     p.add_argument("--no-ragged", action="store_true")
     p.add_argument("--no-batched", action="store_true")
     p.add_argument("--no-ufuncs", action="store_true")
+    p.add_argument("--no-selector", action="store_true")
     p.add_argument("--pmc-pass", action="store_true",
                    help="run each hot call once after a plain copy of known size (2 GiB, the counter "
                         "calibration), for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE; prints no JSON line")
@@ -229,6 +230,7 @@ def main() -> int:
     ragged = None if args.no_ragged else bench_ragged(device, rng, call, s, stream)
     batched = None if args.no_batched else bench_batched(device, rng, call, s, stream)
     ufuncs = None if args.no_ufuncs else bench_ufuncs(device, kernels, call, s, stream, d32, d64)
+    sel = None if args.no_selector else bench_selector()
     cpu = None if args.no_cpu_baseline else cpu_baseline(x32, d64, xi, step_bytes)
 
     line = {
@@ -257,6 +259,7 @@ def main() -> int:
         "ragged": ragged,
         "batched": batched,
         "ufuncs": ufuncs,
+        "selector": sel,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
@@ -388,31 +391,50 @@ def _sampled_error(Cm, idx, refs) -> float:
     return float((np.abs(Cm[rows, cols].astype(np.float64) - exact) / scale).max())
 
 
-def _library_gemm(A, B, reps):
-    """The installed BLAS (torch.matmul: rocBLAS / hipBLASLt) on the same operands, for the
-    'native' column: returns (ms per call, result) or None when torch sees no GPU."""
-    try:
-        import torch
-    except ImportError as e:
-        log(f"library GEMM skipped: {e}")
+_ROCBLAS = None
+
+
+def _rocblas():
+    """rocBLAS from the ROCm install (the same HIP runtime as the library, so it runs on the
+    library's device buffers and stream): a handle, or None when it cannot be loaded."""
+    global _ROCBLAS
+    if _ROCBLAS is None:
+        try:
+            lib = C.CDLL("/opt/rocm/lib/librocblas.so")
+            h = C.c_void_p()
+            if lib.rocblas_create_handle(C.byref(h)) != 0:
+                raise OSError("rocblas_create_handle failed")
+            for fn, ct in (("rocblas_sgemm", C.c_float), ("rocblas_dgemm", C.c_double)):
+                getattr(lib, fn).argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(ct),
+                                             C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.POINTER(ct), C.c_void_p, C.c_int]
+            lib.rocblas_set_stream.argtypes = [C.c_void_p, C.c_void_p]
+            _ROCBLAS = (lib, h)
+        except OSError as e:
+            log(f"library GEMM skipped: {e}")
+            _ROCBLAS = False
+    return _ROCBLAS or None
+
+
+def _library_gemm(n, dt, pa, pb, pc, s):
+    """C = A B (row-major n x n) by rocBLAS's sgemm/dgemm, as the column-major C^T = B^T A^T:
+    a callable for the timing loop, or None when rocBLAS is unavailable."""
+    rb = _rocblas()
+    if rb is None:
         return None
-    if not torch.cuda.is_available():
-        log("library GEMM skipped: torch sees no GPU")
+    lib, h = rb
+    if lib.rocblas_set_stream(h, C.c_void_p(s)) != 0:
+        log("library GEMM skipped: rocblas_set_stream failed")
         return None
-    ta, tb = torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()
-    tc = torch.matmul(ta, tb)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(reps):
-        tc = torch.matmul(ta, tb)
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / reps
-    out = tc.cpu().numpy()
-    del ta, tb, tc
-    torch.cuda.empty_cache()
-    return ms, out
+    ct = C.c_float if dt == np.float32 else C.c_double
+    fn = lib.rocblas_sgemm if dt == np.float32 else lib.rocblas_dgemm
+    one, zero = ct(1.0), ct(0.0)
+    none = 111  # rocblas_operation_none
+
+    def run():
+        st = fn(h, none, none, n, n, n, C.byref(one), pb, n, pa, n, C.byref(zero), pc, n)
+        if st != 0:
+            raise RuntimeError(f"rocBLAS gemm returned status {st}")
+    return run
 
 
 def bench_gemm(device, rng, call, s, stream):
@@ -454,12 +476,13 @@ def bench_gemm(device, rng, call, s, stream):
             err = _sampled_error(dC.download(), idx, refs)
             res[name] = {"ms": round(ms, 3), "TFLOP/s": round(flop / ms / 1e9, 2), "max_rel_err": float(f"{err:.3e}"),
                          "mfma_products": products}
-        lib = _library_gemm(A, B, 3)
+        lib = _library_gemm(n, dt, pa, pb, pc, s)
         libname = "library_fp32" if dt == np.float32 else "library_fp64"
         if lib is not None:
-            err = _sampled_error(lib[1], idx, refs)
-            res[libname] = {"ms": round(lib[0], 3), "TFLOP/s": round(flop / lib[0] / 1e9, 2), "max_rel_err": float(f"{err:.3e}"),
-                            "what": "torch.matmul (rocBLAS / hipBLASLt) on the same operands"}
+            ms = timeit(lib, 3)
+            err = _sampled_error(dC.download(), idx, refs)
+            res[libname] = {"ms": round(ms, 3), "TFLOP/s": round(flop / ms / 1e9, 2), "max_rel_err": float(f"{err:.3e}"),
+                            "what": "rocBLAS sgemm/dgemm (ROCm install) on the same device operands"}
         natives = [runs[0][0]] + ([libname] if lib is not None else [])
         native = res[runs[0][0]]
         # a split engine qualifies at the error of the own native kernel; the speed-up is taken
@@ -584,6 +607,33 @@ def bench_batched(device, rng, call, s, stream):
 
 # ---------------------------------------------------------------------------------------------
 # ragged batch: 4096 bf16 GEMMs + 65536 f32 row sums in one persistent launch
+
+def bench_selector() -> dict:
+    """The learned tile selector on a held-out stream: 20000 GEMM + 4000 row-sum training shapes
+    tuned exhaustively, the two cost models trained on the GPU, then 100000 held-out calls (80%
+    GEMMs of m, n, k log-uniform in [16, 512] over f64 / f32 / bf16, 20% row sums) chosen by the
+    models with timing fallback, against exhaustive search and the built-in rules; plus the shipped
+    model (numrs_amd/selector_model.npz, the one the GEMM dispatch consults) on the same calls."""
+    from numrs_amd import selector as S
+
+    t0 = time.time()
+    res = S.run_evaluation(20000, 4000, 100000, log=log)
+    st = res["stream"]
+    out = {"train_shapes": res["train_shapes"], "eval_calls": res["eval_calls"], "margin_log2": res["margin_log2"],
+           "stream_regret": round(st["regret_model_with_fallback"], 5), "stream_regret_builtin_rules": round(st["regret_heuristic"], 5),
+           "fallback_rate": round(st["fallback_rate"], 5), "stream_ms": {k: round(st[k], 3) for k in
+                                                                          ("exhaustive_ms", "selected_ms", "heuristic_ms")},
+           "gemm": {k: round(res["gemm"][k], 5) for k in ("regret_model", "regret_model_with_fallback", "fallback_rate",
+                                                           "regret_heuristic", "top1_agreement")},
+           "rows": {k: round(res["rows"][k], 5) for k in ("regret_model", "regret_model_with_fallback", "fallback_rate",
+                                                           "regret_heuristic")},
+           "mlp_inferences_per_s": round(res["mlp_inference_per_s"]), "seconds": round(time.time() - t0, 1)}
+    if "shipped_model_gemm" in res:
+        out["shipped_model_gemm"] = {k: round(res["shipped_model_gemm"][k], 5)
+                                     for k in ("regret_model", "regret_model_with_fallback", "fallback_rate", "top1_agreement")}
+    log("selector: " + json.dumps(out))
+    return out
+
 
 def bench_ragged(device, rng, call, s, stream):
     from numrs_amd.grouped import Graph, GroupedPlan, to_bf16_bits

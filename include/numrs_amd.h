@@ -655,6 +655,34 @@ nrk_status nrk_dev_run_config(int32_t family, int32_t config, int32_t d0, int32_
  * `reps` launches, HIP events) on library-owned operands; times_ns is count x n_configs. */
 nrk_status nrk_tune(int32_t family, int32_t count, const int32_t *d0, const int32_t *d1, const int32_t *d2, int32_t reps,
                     float *times_ns);
+
+/* The selector on the dispatch path (no reference counterpart: it replaces the compile-time tiers
+ * of kernels/mod.rs:63-106 and kernels/gemm.rs:290-312). Every call that runs a GEMM of these
+ * families without a forced tile -- nrk_dev_gemm_f64/f32 (NRK_DT_F64/F32), nrk_dev_gemm_bf16
+ * (NRK_DT_BF16), nrk_dev_gemm_f32_bf16x (NRK_DT_F32_BF16X3/X6), nrk_dev_gemm_f64_ozaki
+ * (NRK_DT_F64_OZAKI) -- takes its tile from, in order: a recorded decision for the shape; the
+ * family's cost model (the MLP above, evaluated on the host) when the shape lies within half a
+ * binade of the shapes it was trained on, with the exhaustive timing of nrk_tune when its two best
+ * predictions are closer than `margin` (log2 units; not while the stream is captured into a
+ * graph); else the built-in rule. Decisions are recorded per (family, d0, d1, d2). Every
+ * configuration of a family gives the same result bits.
+ *
+ * set_model: host arrays; params as nrk_mlp_param_count(K) floats (K = nrk_config_count), inputs
+ * normalised as (feature - x_mean) / x_scale over the features log2 d0, log2 d1, log2 d2,
+ * dtype_feature, 0, 0; predictions are log2(ns) - y_offset; domain_lo/hi[3] bound log2 d0..d2.
+ * clear: the model, the recorded decisions and the counters of one family (-1: all).
+ * table_put / table_size / table_get: the timed decisions, to persist and restore them.
+ * choose: the configuration a dispatch of this shape takes now (-1: the built-in rule).
+ * stats: dispatch decisions so far by source. */
+nrk_status nrk_selector_set_model(int32_t family, float dtype_feature, const float *params, int32_t K, const float *x_mean,
+                                  const float *x_scale, const float *domain_lo, const float *domain_hi, float y_offset,
+                                  float margin);
+nrk_status nrk_selector_clear(int32_t family);
+nrk_status nrk_selector_table_put(int32_t family, int32_t d0, int32_t d1, int32_t d2, int32_t config);
+nrk_status nrk_selector_table_size(int32_t *count);
+nrk_status nrk_selector_table_get(int32_t index, int32_t *family, int32_t *d0, int32_t *d1, int32_t *d2, int32_t *config);
+nrk_status nrk_selector_choose(int32_t family, int32_t d0, int32_t d1, int32_t d2, int32_t *config);
+nrk_status nrk_selector_stats(int32_t family, int64_t *by_rule, int64_t *by_model, int64_t *by_timing, int64_t *by_table);
 /* MLP cost model 6 -> 64 -> 64 -> K (ReLU, fp32), K <= 8. Parameters (device, contiguous):
  * W1[64][6] b1[64] W2[64][64] b2[64] W3[K][64] b3[K]; nrk_mlp_param_count(K) floats. */
 size_t nrk_mlp_param_count(int32_t K);

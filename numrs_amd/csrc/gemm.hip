@@ -215,6 +215,8 @@ nrk_status gemm(size_t m, size_t k, size_t n, const T *a, const T *b, T *c, hipS
     if (!a || !b) return fail(NRK_ERR_INVALID_ARGUMENT, "A or B is NULL");
     if (m > (size_t)INT32_MAX / 2 || n > (size_t)INT32_MAX / 2 || k > (size_t)INT32_MAX / 2)
         return fail(NRK_ERR_INVALID_ARGUMENT, "dimension exceeds 2^30");
+    const int cfg = select_config(sizeof(T) == 8 ? NRK_DT_F64 : NRK_DT_F32, m, n, k, stream);
+    if (cfg >= 0) return gemm_config<T>(cfg, m, k, n, a, b, c, stream);
     return launch_default<T>(1, m, k, n, a, 0, b, 0, c, 0, stream);
 }
 

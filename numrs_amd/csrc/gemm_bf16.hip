@@ -327,6 +327,7 @@ nrk_status gemm_f32_bf16x(size_t m, size_t k, size_t n, const float *a, const fl
         NRK_TRY(split<1>(b, (int)k, (int)n, true, Bts, (int)Np, (int)Kp, stream));
     }
     const size_t sa = Mp * Kp, sb = Np * Kp;
+    if (tile < 0 && products != 1) tile = select_config(products == 6 ? NRK_DT_F32_BF16X6 : NRK_DT_F32_BF16X3, m, n, k, stream);
     if (tile < 0) {  // the largest tiles that still give every CU a workgroup
         if (ceil_div(m, 128) * ceil_div(n, 256) >= (size_t)cu_count()) tile = 4;
         else if (ceil_div(m, 128) * ceil_div(n, 128) >= (size_t)cu_count()) tile = 1;

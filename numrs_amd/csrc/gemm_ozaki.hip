@@ -656,6 +656,7 @@ nrk_status gemm_f64_ozaki(size_t m, size_t k, size_t n, const double *a, const d
     hipLaunchKernelGGL((residue_kernel<true>), dim3((unsigned)(ceil_div(Np, 16) * ceil_div(nkc, 16))), dim3(256), 0, stream,
                        b, (int)k, (int)n, scol, kc, Bres, (int)Np, (int)Kp);
     NRK_HIP(hipGetLastError());
+    if (tile < 0) tile = select_config(NRK_DT_F64_OZAKI, m, n, k, stream);
     if (tile < 0) {  // default: the largest tiles that still give every CU a workgroup
         if (ceil_div(m, 256) * ceil_div(n, 256) * moduli >= (size_t)cu_count()) tile = 3;
         else if (ceil_div(m, 128) * ceil_div(n, 128) * moduli >= (size_t)cu_count() * 2) tile = 1;

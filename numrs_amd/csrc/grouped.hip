@@ -491,7 +491,9 @@ nrk_status nrk_grouped_plan_destroy(nrk_grouped_plan *p) {
 nrk_status nrk_dev_gemm_bf16(const nrk_gemm_desc *desc, nrk_stream stream) {
     NRK_TRY(ensure_device());
     if (!desc) return fail(NRK_ERR_INVALID_ARGUMENT, "desc is NULL");
-    return gemm_bf16_config(1, *desc, (hipStream_t)stream);  // the 64 x 64 tile the grouped launch uses
+    const int cfg = select_config(NRK_DT_BF16, (size_t)std::max(desc->m, 0), (size_t)std::max(desc->n, 0),
+                                  (size_t)std::max(desc->k, 0), (hipStream_t)stream);
+    return gemm_bf16_config(cfg >= 0 ? cfg : 1, *desc, (hipStream_t)stream);  // built-in: 64 x 64
 }
 
 // ---- hipGraph capture helpers ----

@@ -43,6 +43,12 @@ template <typename T>
 nrk_status expr_reduce(const ExprProg &pg, const T *const *leaves, size_t rows, size_t cols, double ddof,
                        const nrk_reduce_outputs *row_out, const nrk_reduce_outputs *full_out, hipStream_t stream);
 
+// The tile configuration the selector picks for a dispatch with no forced tile (selector.hip):
+// a recorded decision, the registered cost model's choice inside its trained shapes (timed when
+// the model is unsure), or -1 when neither applies and the caller keeps its fixed rule.
+// family: NRK_DT_*; GEMMs pass (m, n, k).
+int select_config(int family, size_t d0, size_t d1, size_t d2, hipStream_t stream);
+
 // Workgroups per CU of the segment reducer's persistent grid (result bits are independent of it).
 int reduce_blocks_per_cu();
 void set_reduce_blocks_per_cu(int b);

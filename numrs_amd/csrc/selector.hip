@@ -8,6 +8,10 @@
 // timing. Families: 0 = f64 GEMM, 1 = f32 GEMM (MFMA tile sizes), 2 = bf16 -> f32 NN GEMM
 // (tile sizes), 3 = f32 row sums (persistent-grid sizes; result bits do not depend on it).
 #include <algorithm>
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "ops.hpp"
@@ -228,6 +232,116 @@ __global__ void sum_kernel(const float *__restrict__ x, int n, float *__restrict
     if (threadIdx.x == 0) *out = s[0];
 }
 
+
+// ---------------- the selector on the dispatch path ----------------
+// Per operation family: an optional cost model (the MLP above, evaluated on the host, a few
+// thousand multiply-adds) and a decision table per shape. A dispatch with no forced tile asks
+// select_config: a recorded decision wins; else, inside the shapes the model was trained on, the
+// model's fastest predicted configuration, or, when its two best predictions are closer than the
+// margin, the configuration the exhaustive timing finds (not while the stream is being captured
+// into a graph, where the model's choice stands); else -1, and the caller keeps its fixed rule.
+// Every configuration of a family gives the same result bits, so the choice only moves time.
+struct SelModel {
+    bool set = false;
+    int K = 0;
+    float dtype_feature = 0.f;
+    std::vector<float> params;
+    float x_mean[MLP_IN], x_scale[MLP_IN];
+    float lo[3], hi[3];  // log2 d0, d1, d2 range of the training shapes
+    float y_offset = 0.f, margin = 0.f;
+};
+enum { SEL_RULE = 0, SEL_MODEL = 1, SEL_TUNED = 2, SEL_TABLE = 3 };
+struct SelDecision {
+    int config, source;
+};
+struct SelStats {
+    long long n[4] = {0, 0, 0, 0};  // decisions by source, counted per dispatch
+};
+constexpr int SEL_FAMILIES = 7;
+static std::mutex g_sel_mutex;
+static SelModel g_sel_model[SEL_FAMILIES];
+static SelStats g_sel_stats[SEL_FAMILIES];
+static std::map<std::tuple<int, int, int, int>, SelDecision> g_sel_table;
+
+static void host_mlp(const SelModel &m, const float (&f)[MLP_IN], float *y) {
+    const float *W1 = m.params.data(), *b1 = W1 + MLP_H * MLP_IN, *W2 = b1 + MLP_H, *b2 = W2 + MLP_H * MLP_H;
+    const float *W3 = b2 + MLP_H, *b3 = W3 + m.K * MLP_H;
+    float x[MLP_IN], h1[MLP_H], h2[MLP_H];
+    for (int i = 0; i < MLP_IN; ++i) x[i] = (f[i] - m.x_mean[i]) / m.x_scale[i];
+    for (int j = 0; j < MLP_H; ++j) {
+        float s = b1[j];
+        for (int i = 0; i < MLP_IN; ++i) s = std::fma(W1[j * MLP_IN + i], x[i], s);
+        h1[j] = s > 0.f ? s : 0.f;
+    }
+    for (int j = 0; j < MLP_H; ++j) {
+        float s = b2[j];
+        for (int i = 0; i < MLP_H; ++i) s = std::fma(W2[j * MLP_H + i], h1[i], s);
+        h2[j] = s > 0.f ? s : 0.f;
+    }
+    for (int o = 0; o < m.K; ++o) {
+        float s = b3[o];
+        for (int i = 0; i < MLP_H; ++i) s = std::fma(W3[o * MLP_H + i], h2[i], s);
+        y[o] = s + m.y_offset;
+    }
+}
+
+// The decision for one shape (source: SEL_*), without the table; the caller holds no lock.
+static SelDecision decide(int family, int d0, int d1, int d2, hipStream_t stream) {
+    SelModel m;
+    {
+        std::lock_guard<std::mutex> lock(g_sel_mutex);
+        if (!g_sel_model[family].set) return {-1, SEL_RULE};
+        m = g_sel_model[family];
+    }
+    const float f[MLP_IN] = {std::log2((float)d0), std::log2((float)d1), std::log2((float)std::max(d2, 1)), m.dtype_feature, 0.f,
+                             0.f};
+    for (int i = 0; i < 3; ++i)
+        if (f[i] < m.lo[i] - 0.5f || f[i] > m.hi[i] + 0.5f) return {-1, SEL_RULE};  // outside the trained shapes
+    float y[MLP_MAXK];
+    host_mlp(m, f, y);
+    int best = 0, second = -1;
+    for (int o = 1; o < m.K; ++o) {
+        if (y[o] < y[best]) {
+            second = best;
+            best = o;
+        } else if (second < 0 || y[o] < y[second]) {
+            second = o;
+        }
+    }
+    if (second >= 0 && y[second] - y[best] < m.margin) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(stream, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone) {
+            std::vector<float> t(m.K);
+            const int32_t a0 = d0, a1 = d1, a2 = std::max(d2, 1);
+            if (nrk_tune(family, 1, &a0, &a1, &a2, 3, t.data()) == NRK_OK)
+                return {(int)(std::min_element(t.begin(), t.end()) - t.begin()), SEL_TUNED};
+        }
+    }
+    return {best, SEL_MODEL};
+}
+
+int select_config(int family, size_t d0, size_t d1, size_t d2, hipStream_t stream) {
+    if (family < 0 || family >= SEL_FAMILIES || d0 > (1u << 30) || d1 > (1u << 30) || d2 > (1u << 30)) return -1;
+    const auto key = std::make_tuple(family, (int)d0, (int)d1, (int)d2);
+    {
+        std::lock_guard<std::mutex> lock(g_sel_mutex);
+        auto it = g_sel_table.find(key);
+        if (it != g_sel_table.end()) {
+            ++g_sel_stats[family].n[it->second.source == SEL_TUNED ? SEL_TABLE : it->second.source];
+            return it->second.config;
+        }
+        if (!g_sel_model[family].set) {
+            ++g_sel_stats[family].n[SEL_RULE];
+            return -1;
+        }
+    }
+    const SelDecision d = decide(family, (int)d0, (int)d1, (int)d2, stream);
+    std::lock_guard<std::mutex> lock(g_sel_mutex);
+    ++g_sel_stats[family].n[d.source];
+    if (d.config >= 0) g_sel_table[key] = d;
+    return d.config;
+}
+
 }  // namespace nrk
 
 using namespace nrk;
@@ -365,6 +479,105 @@ nrk_status nrk_mlp_train_step(float *params, float *adam_m, float *adam_v, int32
                            (int)P, lr, b1, b2, 1e-8f, c1, c2);
     }
     NRK_HIP(hipGetLastError());
+    return NRK_OK;
+}
+
+
+nrk_status nrk_selector_set_model(int32_t family, float dtype_feature, const float *params, int32_t K, const float *x_mean,
+                                  const float *x_scale, const float *domain_lo, const float *domain_hi, float y_offset,
+                                  float margin) {
+    if (family < 0 || family >= SEL_FAMILIES || config_count(family) == 0)
+        return fail(NRK_ERR_INVALID_ARGUMENT, "unknown operation family %d", family);
+    if (K != config_count(family)) return fail(NRK_ERR_INVALID_ARGUMENT, "family %d has %d configurations, the model %d", family, config_count(family), K);
+    if (!params || !x_mean || !x_scale || !domain_lo || !domain_hi) return fail(NRK_ERR_INVALID_ARGUMENT, "NULL pointer");
+    SelModel m;
+    m.set = true;
+    m.K = K;
+    m.dtype_feature = dtype_feature;
+    m.params.assign(params, params + mlp_param_count(K));
+    for (int i = 0; i < MLP_IN; ++i) {
+        if (!(x_scale[i] > 0.f)) return fail(NRK_ERR_INVALID_ARGUMENT, "x_scale[%d] must be positive", i);
+        m.x_mean[i] = x_mean[i];
+        m.x_scale[i] = x_scale[i];
+    }
+    for (int i = 0; i < 3; ++i) {
+        m.lo[i] = domain_lo[i];
+        m.hi[i] = domain_hi[i];
+    }
+    m.y_offset = y_offset;
+    m.margin = margin;
+    std::lock_guard<std::mutex> lock(g_sel_mutex);
+    g_sel_model[family] = std::move(m);
+    for (auto it = g_sel_table.begin(); it != g_sel_table.end();)  // the new model's choices replace the old ones'
+        it = (std::get<0>(it->first) == family && it->second.source == SEL_MODEL) ? g_sel_table.erase(it) : std::next(it);
+    return NRK_OK;
+}
+
+nrk_status nrk_selector_clear(int32_t family) {
+    if (family < -1 || family >= SEL_FAMILIES) return fail(NRK_ERR_INVALID_ARGUMENT, "unknown operation family %d", family);
+    std::lock_guard<std::mutex> lock(g_sel_mutex);
+    for (int f = 0; f < SEL_FAMILIES; ++f) {
+        if (family >= 0 && f != family) continue;
+        g_sel_model[f] = SelModel();
+        g_sel_stats[f] = SelStats();
+    }
+    for (auto it = g_sel_table.begin(); it != g_sel_table.end();)
+        it = (family < 0 || std::get<0>(it->first) == family) ? g_sel_table.erase(it) : std::next(it);
+    return NRK_OK;
+}
+
+nrk_status nrk_selector_table_put(int32_t family, int32_t d0, int32_t d1, int32_t d2, int32_t config) {
+    if (family < 0 || family >= SEL_FAMILIES || config_count(family) == 0)
+        return fail(NRK_ERR_INVALID_ARGUMENT, "unknown operation family %d", family);
+    if (config < 0 || config >= config_count(family)) return fail(NRK_ERR_INVALID_ARGUMENT, "config %d outside [0,%d)", config, config_count(family));
+    if (d0 <= 0 || d1 <= 0 || d2 < 0) return fail(NRK_ERR_INVALID_ARGUMENT, "non-positive dimension");
+    std::lock_guard<std::mutex> lock(g_sel_mutex);
+    g_sel_table[std::make_tuple((int)family, (int)d0, (int)d1, (int)d2)] = SelDecision{config, SEL_TUNED};
+    return NRK_OK;
+}
+
+nrk_status nrk_selector_table_size(int32_t *count) {
+    if (!count) return fail(NRK_ERR_INVALID_ARGUMENT, "count is NULL");
+    std::lock_guard<std::mutex> lock(g_sel_mutex);
+    int32_t n = 0;
+    for (const auto &e : g_sel_table) n += e.second.source == SEL_TUNED;
+    *count = n;
+    return NRK_OK;
+}
+
+nrk_status nrk_selector_table_get(int32_t index, int32_t *family, int32_t *d0, int32_t *d1, int32_t *d2, int32_t *config) {
+    if (!family || !d0 || !d1 || !d2 || !config) return fail(NRK_ERR_INVALID_ARGUMENT, "NULL pointer");
+    std::lock_guard<std::mutex> lock(g_sel_mutex);
+    int32_t i = 0;
+    for (const auto &e : g_sel_table) {
+        if (e.second.source != SEL_TUNED) continue;
+        if (i++ == index) {
+            std::tie(*family, *d0, *d1, *d2) = e.first;
+            *config = e.second.config;
+            return NRK_OK;
+        }
+    }
+    return fail(NRK_ERR_INVALID_ARGUMENT, "table index %d out of range", index);
+}
+
+nrk_status nrk_selector_choose(int32_t family, int32_t d0, int32_t d1, int32_t d2, int32_t *config) {
+    NRK_TRY(ensure_device());
+    if (!config) return fail(NRK_ERR_INVALID_ARGUMENT, "config is NULL");
+    if (family < 0 || family >= SEL_FAMILIES || config_count(family) == 0)
+        return fail(NRK_ERR_INVALID_ARGUMENT, "unknown operation family %d", family);
+    if (d0 <= 0 || d1 <= 0 || d2 < 0) return fail(NRK_ERR_INVALID_ARGUMENT, "non-positive dimension");
+    *config = select_config(family, (size_t)d0, (size_t)d1, (size_t)d2, nullptr);
+    return NRK_OK;
+}
+
+nrk_status nrk_selector_stats(int32_t family, int64_t *by_rule, int64_t *by_model, int64_t *by_timing, int64_t *by_table) {
+    if (family < 0 || family >= SEL_FAMILIES) return fail(NRK_ERR_INVALID_ARGUMENT, "unknown operation family %d", family);
+    std::lock_guard<std::mutex> lock(g_sel_mutex);
+    const SelStats &st = g_sel_stats[family];
+    if (by_rule) *by_rule = st.n[SEL_RULE];
+    if (by_model) *by_model = st.n[SEL_MODEL];
+    if (by_timing) *by_timing = st.n[SEL_TUNED];
+    if (by_table) *by_table = st.n[SEL_TABLE];
     return NRK_OK;
 }
 

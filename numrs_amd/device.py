@@ -111,8 +111,13 @@ def synchronize() -> None:
 
 
 def init(device: int = 0) -> None:
+    """Select the device, and register the persisted tile-selection model (if present) with the
+    library's GEMM dispatch."""
     load()
     call("nrk_init", C.c_int(device))
+    from . import selector
+
+    selector.install()
 
 
 def call_memset(buf: DeviceBuffer, stream=None) -> None:

@@ -26,6 +26,7 @@ import json
 import math
 import sys
 import time
+from pathlib import Path
 
 import numpy as np
 
@@ -36,6 +37,9 @@ from ._lib import call, load
 F64, F32, BF16, ROWSUM_F32, F32_BF16X3, F64_OZAKI, F32_BF16X6 = 0, 1, 2, 3, 4, 5, 6
 GEMM_FAMILY_DTYPE = {F64: 0, F32: 1, BF16: 2}
 IN, H = 6, 64
+# the trained GEMM cost model, persisted (numpy .npz, no pickles) and registered with the library's
+# dispatch by install()
+MODEL_PATH = Path(__file__).resolve().parent / "selector_model.npz"
 
 
 def config_count(family: int) -> int:
@@ -129,6 +133,60 @@ class CostModel:
         call("nrk_mlp_forward", C.c_void_p(self.params.ptr), self.K, C.c_void_p(dx.ptr), len(X), C.c_void_p(dy.ptr),
              C.c_void_p(stream))
         return dy.download() + self.y_offset
+
+
+def save_model(path, model: "CostModel", X: np.ndarray, margin: float) -> None:
+    """Persist a trained GEMM cost model: parameters, input normalisation, target offset, the
+    log2 range of the training shapes (the library consults the model only inside it) and the
+    fallback margin."""
+    X = np.asarray(X, np.float32)
+    np.savez(path, params=model.params.download(), K=np.int32(model.K), x_mean=model.x_mean, x_scale=model.x_scale,
+             y_offset=np.float32(model.y_offset), domain_lo=X[:, :3].min(axis=0), domain_hi=X[:, :3].max(axis=0),
+             margin=np.float32(margin))
+
+
+def install(path=None, families=(F64, F32, BF16)) -> list[int]:
+    """Register the persisted GEMM cost model with the library's dispatch for `families` (one model
+    for all three GEMM dtypes, told apart by the dtype feature). Returns the families installed:
+    none when the file is absent (the built-in rules then stay)."""
+    path = MODEL_PATH if path is None else Path(path)
+    if not path.exists():
+        return []
+    z = np.load(path, allow_pickle=False)
+    K = int(z["K"])
+    arrs = {k: np.ascontiguousarray(z[k], np.float32) for k in ("params", "x_mean", "x_scale", "domain_lo", "domain_hi")}
+    ptr = {k: v.ctypes.data_as(C.c_void_p) for k, v in arrs.items()}
+    done = []
+    for fam in families:
+        call("nrk_selector_set_model", fam, C.c_float(GEMM_FAMILY_DTYPE[fam]), ptr["params"], K, ptr["x_mean"], ptr["x_scale"],
+             ptr["domain_lo"], ptr["domain_hi"], C.c_float(float(z["y_offset"])), C.c_float(float(z["margin"])))
+        done.append(fam)
+    return done
+
+
+def choose(family: int, d0: int, d1: int, d2: int) -> int:
+    """The tile configuration a dispatch of this shape takes now (-1: the built-in rule)."""
+    c = C.c_int32()
+    call("nrk_selector_choose", family, d0, d1, d2, C.byref(c))
+    return c.value
+
+
+def stats(family: int) -> dict:
+    v = [C.c_int64() for _ in range(4)]
+    call("nrk_selector_stats", family, *[C.byref(x) for x in v])
+    return dict(zip(("rule", "model", "timing", "table"), (x.value for x in v)))
+
+
+def table() -> list[tuple[int, int, int, int, int]]:
+    """The timed decisions (family, d0, d1, d2, config), e.g. to persist them."""
+    n = C.c_int32()
+    call("nrk_selector_table_size", C.byref(n))
+    out = []
+    for i in range(n.value):
+        v = [C.c_int32() for _ in range(5)]
+        call("nrk_selector_table_get", i, *[C.byref(x) for x in v])
+        out.append(tuple(x.value for x in v))
+    return out
 
 
 def call_size(name: str, *args) -> int:
@@ -246,6 +304,65 @@ def evaluate(model: CostModel, ts: TunedSet, margin: float) -> dict:
                           "heuristic": float(ts.times[np.arange(len(best)), ts.heur].sum() / 1e6)}}
 
 
+def run_evaluation(n_train: int = 20000, n_train_rows: int = 4000, n_eval: int = 100000, steps: int = 1500, reps: int = 3,
+                   margin: float = 0.05, seed: int = 0x4E554D5253, save=None, log=None) -> dict:
+    """Tune n_train GEMM and n_train_rows row-sum shapes, train the two cost models on the GPU, and
+    evaluate model-based selection (with timing fallback) against exhaustive search and the
+    built-in rules on a held-out stream of n_eval calls (80% GEMMs, 20% row sums). When the
+    persisted GEMM model exists, it is scored on the same held-out GEMM calls too."""
+    log = log or (lambda s: print(s, file=sys.stderr, flush=True))
+    r = np.random.default_rng(seed)
+    t0 = time.time()
+    tr_g = tune_gemm_set(*gemm_shapes(r, n_train), reps)
+    tr_r = tune_row_set(*row_shapes(r, n_train_rows), reps)
+    log(f"tuned training shapes in {time.time() - t0:.1f}s")
+    n_eval_g = int(round(n_eval * 0.8))
+    t1 = time.time()
+    ev_g = tune_gemm_set(*gemm_shapes(r, n_eval_g), reps)
+    ev_r = tune_row_set(*row_shapes(r, n_eval - n_eval_g), reps)
+    log(f"tuned held-out stream in {time.time() - t1:.1f}s")
+    t2 = time.time()
+    mg = CostModel(config_count(F64), seed=1)
+    hist_g = mg.fit(tr_g.X, np.log2(tr_g.times), steps=steps)
+    mr = CostModel(config_count(ROWSUM_F32), seed=2)
+    hist_r = mr.fit(tr_r.X, np.log2(tr_r.times), steps=steps)
+    log(f"trained in {time.time() - t2:.1f}s; loss gemm {hist_g[0]:.3f} -> {hist_g[-1]:.4f}, rows {hist_r[0]:.3f} -> {hist_r[-1]:.4f}")
+    if save:
+        save_model(save, mg, tr_g.X, margin)
+        log(f"saved the GEMM model to {save}")
+    eg, er = evaluate(mg, ev_g, margin), evaluate(mr, ev_r, margin)
+    times = np.concatenate([ev_g.times.min(axis=1), ev_r.times.min(axis=1)])
+    # the mixed stream: every held-out call with the model's (or the fallback's) choice
+    cg, fg = Selector(mg, margin).choose(ev_g.X)
+    cr, fr = Selector(mr, margin).choose(ev_r.X)
+    sel_g = np.where(fg, ev_g.times.argmin(1), cg)
+    sel_r = np.where(fr, ev_r.times.argmin(1), cr)
+    t_sel = ev_g.times[np.arange(len(sel_g)), sel_g].sum() + ev_r.times[np.arange(len(sel_r)), sel_r].sum()
+    t_heur = ev_g.times[np.arange(len(sel_g)), ev_g.heur].sum() + ev_r.times[np.arange(len(sel_r)), ev_r.heur].sum()
+    big = np.tile(ev_g.X, (max(1, 100000 // max(1, len(ev_g.X))), 1))[:100000]
+    mg.predict(big)
+    ti = time.perf_counter()
+    mg.predict(big)
+    infer = time.perf_counter() - ti
+    out = {"train_shapes": {"gemm": n_train, "rows": n_train_rows}, "eval_calls": n_eval, "margin_log2": margin,
+           "stream": {"regret_model_with_fallback": float(t_sel / times.sum() - 1.0),
+                      "regret_heuristic": float(t_heur / times.sum() - 1.0),
+                      "fallback_rate": float((fg.sum() + fr.sum()) / n_eval),
+                      "exhaustive_ms": float(times.sum() / 1e6), "selected_ms": float(t_sel / 1e6),
+                      "heuristic_ms": float(t_heur / 1e6)},
+           "gemm": eg, "rows": er, "mlp_inference_per_s": len(big) / infer,
+           "loss": {"gemm": [hist_g[0], hist_g[-1]], "rows": [hist_r[0], hist_r[-1]]},
+           "seconds": {"tune": round(t2 - t0, 1), "train": round(time.time() - t2, 1)}}
+    if MODEL_PATH.exists():
+        z = np.load(MODEL_PATH, allow_pickle=False)
+        shipped = CostModel(int(z["K"]), seed=0)
+        shipped.params = device.DeviceArray.upload(np.ascontiguousarray(z["params"], np.float32))
+        shipped.x_mean, shipped.x_scale = z["x_mean"].astype(np.float32), z["x_scale"].astype(np.float32)
+        shipped.y_offset = float(z["y_offset"])
+        out["shipped_model_gemm"] = evaluate(shipped, ev_g, float(z["margin"]))
+    return out
+
+
 def main(argv=None) -> int:
     import argparse
 
@@ -257,45 +374,10 @@ def main(argv=None) -> int:
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--margin", type=float, default=0.05, help="fallback when the top-2 predictions are closer (log2)")
     ap.add_argument("--seed", type=int, default=0x4E554D5253)
+    ap.add_argument("--save", default=None, help="write the trained GEMM model here (.npz)")
     a = ap.parse_args(argv)
     device.init(0)
-    r = np.random.default_rng(a.seed)
-    log = lambda s: print(s, file=sys.stderr, flush=True)  # noqa: E731
-    t0 = time.time()
-    tr_g = tune_gemm_set(*gemm_shapes(r, a.train), a.reps)
-    tr_r = tune_row_set(*row_shapes(r, a.train_rows), a.reps)
-    log(f"tuned training shapes in {time.time() - t0:.1f}s")
-    n_eval_g = int(round(a.eval * 0.8))
-    t1 = time.time()
-    ev_g = tune_gemm_set(*gemm_shapes(r, n_eval_g), a.reps)
-    ev_r = tune_row_set(*row_shapes(r, a.eval - n_eval_g), a.reps)
-    log(f"tuned held-out stream in {time.time() - t1:.1f}s")
-    t2 = time.time()
-    mg = CostModel(config_count(F64), seed=1)
-    hist_g = mg.fit(tr_g.X, np.log2(tr_g.times), steps=a.steps)
-    mr = CostModel(config_count(ROWSUM_F32), seed=2)
-    hist_r = mr.fit(tr_r.X, np.log2(tr_r.times), steps=a.steps)
-    log(f"trained in {time.time() - t2:.1f}s; loss gemm {hist_g[0]:.3f} -> {hist_g[-1]:.4f}, rows {hist_r[0]:.3f} -> {hist_r[-1]:.4f}")
-    eg, er = evaluate(mg, ev_g, a.margin), evaluate(mr, ev_r, a.margin)
-    times = np.concatenate([ev_g.times.min(axis=1), ev_r.times.min(axis=1)])
-    # the mixed stream: every held-out call with the model's (or the fallback's) choice
-    sel_g = np.where(Selector(mg, a.margin).choose(ev_g.X)[1], ev_g.times.argmin(1), Selector(mg, a.margin).choose(ev_g.X)[0])
-    sel_r = np.where(Selector(mr, a.margin).choose(ev_r.X)[1], ev_r.times.argmin(1), Selector(mr, a.margin).choose(ev_r.X)[0])
-    t_sel = ev_g.times[np.arange(len(sel_g)), sel_g].sum() + ev_r.times[np.arange(len(sel_r)), sel_r].sum()
-    t_heur = ev_g.times[np.arange(len(sel_g)), ev_g.heur].sum() + ev_r.times[np.arange(len(sel_r)), ev_r.heur].sum()
-    big = np.tile(ev_g.X, (max(1, 100000 // max(1, len(ev_g.X))), 1))[:100000]
-    mg.predict(big)
-    ti = time.perf_counter()
-    mg.predict(big)
-    infer = time.perf_counter() - ti
-    out = {"train_shapes": {"gemm": a.train, "rows": a.train_rows}, "eval_calls": a.eval, "margin_log2": a.margin,
-           "stream": {"regret_model_with_fallback": float(t_sel / times.sum() - 1.0),
-                      "regret_heuristic": float(t_heur / times.sum() - 1.0),
-                      "exhaustive_ms": float(times.sum() / 1e6), "selected_ms": float(t_sel / 1e6),
-                      "heuristic_ms": float(t_heur / 1e6)},
-           "gemm": eg, "rows": er, "mlp_inference_per_s": len(big) / infer,
-           "loss": {"gemm": [hist_g[0], hist_g[-1]], "rows": [hist_r[0], hist_r[-1]]},
-           "seconds": {"tune": round(t2 - t0, 1), "train": round(time.time() - t2, 1)}}
+    out = run_evaluation(a.train, a.train_rows, a.eval, a.steps, a.reps, a.margin, a.seed, a.save)
     print(json.dumps(out), flush=True)
     return 0
 
