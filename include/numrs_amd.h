@@ -627,6 +627,12 @@ nrk_status nrk_grouped_plan_info(const nrk_grouped_plan *plan, int32_t *n_items,
 nrk_status nrk_grouped_plan_destroy(nrk_grouped_plan *plan);
 /* One bf16 GEMM per launch (the unbatched per-call path the grouped launch is compared to). */
 nrk_status nrk_dev_gemm_bf16(const nrk_gemm_desc *desc, nrk_stream stream);
+/* The same batch without a plan: one nrk_dev_gemm_bf16 launch per GEMM, then one nrk_dev_sum_f32
+ * launch per row, issued in a loop on `stream` (descs and row_offsets are HOST arrays). The
+ * per-call baseline a caller without the grouped launch runs (no reference counterpart as one
+ * call; the reference runs one task per panel, array/linalg.rs:56-136). */
+nrk_status nrk_dev_ragged_per_call(int32_t gemm_count, const nrk_gemm_desc *descs, int32_t row_count, const float *rows_x,
+                                   const int64_t *row_offsets, float *rows_out, nrk_stream stream);
 
 /* ------------------------------------------------------------------------------------------ */
 /* Autotuner and learned selector                                                             */

@@ -131,3 +131,58 @@ def test_empty_and_rows_only(gpu, oracle):
         seg = x[offs[rr]:offs[rr + 1]]
         assert abs(s[rr] - oracle.dd_stats(seg)["sum"]) <= 8 * 2.0 ** -24 * float(np.abs(seg).sum()) + 1e-30
     plan.close()
+
+
+def test_packed_unaligned_operands_and_per_call_loop(gpu, oracle, batch):
+    """Operands packed back to back in one arena each (bases and rows on 2-byte boundaries only:
+    the aligned-block loads and their shifts): the grouped result is bit-identical to each GEMM
+    run alone on its own 16-byte-aligned copy, and nrk_dev_ragged_per_call (one launch per GEMM
+    and per row from C) gives the same GEMM bits and row sums within the bar."""
+    import ctypes as C
+
+    from numrs_amd._lib import GemmDesc, call
+
+    probs, offs, x = batch
+    probs = probs[:60]
+    a_off = np.concatenate([[1], 1 + np.cumsum([p[0] * p[2] for p in probs])])  # start one element in
+    b_off = np.concatenate([[3], 3 + np.cumsum([p[2] * p[1] for p in probs])])
+    arena_a = np.zeros(int(a_off[-1]) + 8, np.uint16)
+    arena_b = np.zeros(int(b_off[-1]) + 8, np.uint16)
+    for g, (m, n, k, a, b) in enumerate(probs):
+        arena_a[a_off[g]:a_off[g] + m * k] = a
+        arena_b[b_off[g]:b_off[g] + k * n] = b
+    dA, dB = device.DeviceArray.upload(arena_a), device.DeviceArray.upload(arena_b)
+    outs = [device.DeviceArray((m, n), np.float32) for (m, n, *_r) in probs]
+    descs = [(m, n, k, dA.ptr + 2 * int(a_off[g]), dB.ptr + 2 * int(b_off[g]), outs[g].ptr)
+             for g, (m, n, k, *_r) in enumerate(probs)]
+    dx = device.DeviceArray.upload(x)
+    dout = device.DeviceArray((len(offs) - 1,), np.float32)
+    plan = GroupedPlan(descs, dx.ptr, offs, dout.ptr)
+    plan.run()
+    device.synchronize()
+    grouped = [o.download() for o in outs]
+    sums = dout.download()
+    plan.close()
+    aligned = upload(probs)
+    for (m, n, k, da, db, dc) in aligned:
+        gemm_bf16(m, n, k, da.ptr, db.ptr, dc.ptr)
+    device.synchronize()
+    for g, (cg, (*_r, dc)) in enumerate(zip(grouped, aligned)):
+        np.testing.assert_array_equal(cg, dc.download(), err_msg=f"problem {g}")
+    # the per-call loop over the same packed operands
+    for o in outs:
+        device.call_memset(o)
+    dout2 = device.DeviceArray((len(offs) - 1,), np.float32)
+    arr = (GemmDesc * len(descs))(*[GemmDesc(m, n, k, 0, a, b, c) for (m, n, k, a, b, c) in descs])
+    roff = np.ascontiguousarray(offs, np.int64)
+    call("nrk_dev_ragged_per_call", len(descs), C.cast(arr, C.c_void_p), len(offs) - 1, C.c_void_p(dx.ptr),
+         roff.ctypes.data_as(C.c_void_p), C.c_void_p(dout2.ptr), None)
+    device.synchronize()
+    for g, (o, cg) in enumerate(zip(outs, grouped)):
+        np.testing.assert_array_equal(o.download(), cg, err_msg=f"per-call problem {g}")
+    s2 = dout2.download()
+    for rr in range(len(offs) - 1):
+        seg = x[offs[rr]:offs[rr + 1]]
+        ex = oracle.dd_stats(seg)["sum"]
+        bar = 8 * 2.0 ** -24 * float(np.abs(seg).astype(np.float64).sum()) + 2.0 ** -24 * abs(ex)
+        assert abs(s2[rr] - ex) <= bar and abs(sums[rr] - ex) <= bar, rr
