@@ -198,14 +198,14 @@ __device__ __forceinline__ void apply_step_n(int op, double sd, T (&v)[N]) {
             default:
                 if constexpr (MATH >= MATH_LIGHT) {
                     switch (op) {
-                        NRK_CHAIN_CASE(NRK_SOP_EXP, via_f64(x, [](double d) { return fm::exp(d); }))
+                        NRK_CHAIN_CASE(NRK_SOP_EXP, fm::exp_t(x))
                         NRK_CHAIN_CASE(NRK_SOP_EXP2, via_f64(x, [](double d) { return fm::exp2(d); }))
                         NRK_CHAIN_CASE(NRK_SOP_EXPM1, via_f64(x, [](double d) { return fm::expm1(d); }))
                         NRK_CHAIN_CASE(NRK_SOP_LOG, via_f64(x, [](double d) { return fm::log(d); }))
                         NRK_CHAIN_CASE(NRK_SOP_LOG2, via_f64(x, [](double d) { return fm::log2(d); }))
                         NRK_CHAIN_CASE(NRK_SOP_LOG10, via_f64(x, [](double d) { return fm::log10(d); }))
                         NRK_CHAIN_CASE(NRK_SOP_LOG1P, via_f64(x, [](double d) { return fm::log1p(d); }))
-                        NRK_CHAIN_CASE(NRK_SOP_TANH, via_f64(x, [](double d) { return fm::tanh(d); }))
+                        NRK_CHAIN_CASE(NRK_SOP_TANH, fm::tanh_t(x))
                         NRK_CHAIN_CASE(NRK_SOP_SINH, via_f64(x, [](double d) { return fm::sinh(d); }))
                         NRK_CHAIN_CASE(NRK_SOP_COSH, via_f64(x, [](double d) { return fm::cosh(d); }))
                         NRK_CHAIN_CASE(NRK_SOP_ASINH, via_f64(x, [](double d) { return fm::asinh(d); }))
@@ -238,7 +238,7 @@ __device__ __forceinline__ void apply_step_n(int op, double sd, T (&v)[N]) {
                         default:
                             if constexpr (MATH >= MATH_ALL) {
                                 switch (op) {
-                                    NRK_CHAIN_CASE(NRK_SOP_POW, (T)fm::pow((double)x, (double)s))
+                                    NRK_CHAIN_CASE(NRK_SOP_POW, fm::pow_t(x, s))
                                     NRK_CHAIN_CASE(NRK_SOP_GAMMA, via_f64(x, [](double d) { return fm::gamma_ref(d); }))
                                     NRK_CHAIN_CASE(NRK_SOP_LGAMMA, via_f64(x, [](double d) { return fm::lgamma_ref(d); }))
                                     default: break;

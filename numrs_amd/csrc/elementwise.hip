@@ -29,7 +29,7 @@ __device__ __forceinline__ T apply_bop(T a, T b) {
     else if constexpr (OP == NRK_BOP_HYPOT) return (T)fm::hypot((double)a, (double)b);
     else if constexpr (OP == NRK_BOP_ATAN2) return (T)fm::atan2((double)a, (double)b);
     else if constexpr (OP == NRK_BOP_PRELU) return a > (T)0 ? a : rn_mul(b, a);  // ufuncs.rs:1394
-    else if constexpr (OP == NRK_BOP_POWER) return (T)fm::pow((double)a, (double)b);  // ufuncs.rs:166
+    else if constexpr (OP == NRK_BOP_POWER) return fm::pow_t(a, b);  // ufuncs.rs:166
     else return (T)fm::logaddexp((double)a, (double)b);                // LOGADDEXP
 }
 
@@ -352,28 +352,67 @@ template nrk_status where_op<double>(int, const double *, const double *, const 
 // ---- softmax / log_softmax passes (ufuncs.rs:1411-1457). ms = {max, sum} on the device.
 // mode 0: y = exp(x - max); mode 1: y = y / sum; mode 2: y = (x - max) - ln(sum);
 // mode 3: y = exp(x - max) / sum (the reference's per-element expression, one pass).
+// inv_s: 1 / sum in binary64 (binary32 only): e^(x - max) / sum evaluated unrounded in binary64
+// and rounded once, instead of two binary32 roundings and a binary64 division per element
 template <typename T, int MODE>
-__global__ __launch_bounds__(EB) void softmax_pass_kernel(const T *x, T *y, const T *ms, size_t n) {
+__device__ __forceinline__ T softmax_elem(T xv, T yv, T m, T s, T lse, double inv_s) {
+    if constexpr (MODE == 0) return fm::exp_t(rn_sub(xv, m));
+    else if constexpr (MODE == 1) return rn_div(yv, s);
+    else if constexpr (MODE == 3) {
+        if constexpr (sizeof(T) == 4) return (T)(fm::f32::exp_d(rn_sub(xv, m)) * inv_s);
+        else return rn_div(fm::exp_t(rn_sub(xv, m)), s);
+    } else return rn_sub(rn_sub(xv, m), lse);
+}
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(EB) void softmax_pass_kernel(const T *x, T *y, const T *ms, size_t n, int vec_ok) {
+    typedef typename Vec16<T>::type V;
+    constexpr int VN = Vec16<T>::N;
     const T m = ms[0];
     const T s = ms[1];
     const T lse = MODE == 2 ? (T)fm::log((double)s) : (T)0;
+    const double inv_s = 1.0 / (double)s;
+    const size_t tid = blockIdx.x * (size_t)EB + threadIdx.x;
     const size_t nthreads = (size_t)gridDim.x * EB;
-    for (size_t i = blockIdx.x * (size_t)EB + threadIdx.x; i < n; i += nthreads) {
-        if constexpr (MODE == 0) y[i] = (T)fm::exp((double)rn_sub(x[i], m));
-        else if constexpr (MODE == 1) y[i] = rn_div(y[i], s);
-        else if constexpr (MODE == 3) y[i] = rn_div((T)fm::exp((double)rn_sub(x[i], m)), s);
-        else y[i] = rn_sub(rn_sub(x[i], m), lse);
+    size_t done = 0;
+    if (vec_ok) {  // x and y 16-byte aligned: EUNROLL vectors per thread in flight
+        const size_t nv = n / VN;
+        const V *src = reinterpret_cast<const V *>(MODE == 1 ? y : x);
+        V *yv = reinterpret_cast<V *>(y);
+        size_t i = tid;
+        for (; i + (EUNROLL - 1) * nthreads < nv; i += EUNROLL * nthreads) {
+            V v[EUNROLL];
+#pragma unroll
+            for (int u = 0; u < EUNROLL; ++u) v[u] = src[i + u * nthreads];  // cached: x was just read
+#pragma unroll
+            for (int u = 0; u < EUNROLL; ++u) {
+                V w;
+#pragma unroll
+                for (int e = 0; e < VN; ++e) w[e] = softmax_elem<T, MODE>(v[u][e], v[u][e], m, s, lse, inv_s);
+                __builtin_nontemporal_store(w, yv + i + u * nthreads);
+            }
+        }
+        for (; i < nv; i += nthreads) {
+            const V v = src[i];
+            V w;
+#pragma unroll
+            for (int e = 0; e < VN; ++e) w[e] = softmax_elem<T, MODE>(v[e], v[e], m, s, lse, inv_s);
+            __builtin_nontemporal_store(w, yv + i);
+        }
+        done = nv * VN;
     }
+    for (size_t i = done + tid; i < n; i += nthreads) y[i] = softmax_elem<T, MODE>(x[i], MODE == 1 ? y[i] : (T)0, m, s, lse, inv_s);
 }
 
 template <typename T>
 nrk_status softmax_pass(int mode, const T *x, T *y, const T *ms, size_t n, hipStream_t stream) {
     if (n == 0) return NRK_OK;
-    const dim3 grid((unsigned)std::min<size_t>(ceil_div(n, (size_t)EB * 4), (size_t)cu_count() * 32));
-    if (mode == 0) hipLaunchKernelGGL((softmax_pass_kernel<T, 0>), grid, dim3(EB), 0, stream, x, y, ms, n);
-    else if (mode == 1) hipLaunchKernelGGL((softmax_pass_kernel<T, 1>), grid, dim3(EB), 0, stream, x, y, ms, n);
-    else if (mode == 2) hipLaunchKernelGGL((softmax_pass_kernel<T, 2>), grid, dim3(EB), 0, stream, x, y, ms, n);
-    else hipLaunchKernelGGL((softmax_pass_kernel<T, 3>), grid, dim3(EB), 0, stream, x, y, ms, n);
+    const int vec_ok = ((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(y)) & 15) == 0;
+    const dim3 grid(stream_grid(n / Vec16<T>::N + 1));
+    if (mode == 0) hipLaunchKernelGGL((softmax_pass_kernel<T, 0>), grid, dim3(EB), 0, stream, x, y, ms, n, vec_ok);
+    else if (mode == 1) hipLaunchKernelGGL((softmax_pass_kernel<T, 1>), grid, dim3(EB), 0, stream, x, y, ms, n, vec_ok);
+    else if (mode == 2) hipLaunchKernelGGL((softmax_pass_kernel<T, 2>), grid, dim3(EB), 0, stream, x, y, ms, n, vec_ok);
+    else hipLaunchKernelGGL((softmax_pass_kernel<T, 3>), grid, dim3(EB), 0, stream, x, y, ms, n, vec_ok);
     NRK_HIP(hipGetLastError());
     return NRK_OK;
 }

@@ -105,7 +105,7 @@ __device__ __forceinline__ void expr_step(int op, double sd, T (&v)[N]) {
         default:
             if constexpr (MATH) {
                 switch (op) {
-                    NRK_XSTEP(NRK_SOP_EXP, (T)fm::exp((double)x))
+                    NRK_XSTEP(NRK_SOP_EXP, fm::exp_t(x))
                     NRK_XSTEP(NRK_SOP_LOG, (T)fm::log((double)x))
                     default: break;
                 }

@@ -997,5 +997,135 @@ __device__ inline double gamma_ref(double x) {
 }
 __device__ inline double lgamma_ref(double x) { return log(__builtin_fabs(lanczos_ref(x))); }
 
+// ---------------- binary32 inputs ----------------
+// exp, tanh and pow of binary32 arguments, evaluated in binary64 only as far as binary32 needs:
+// short Taylor series after the usual reductions, with a relative error below 2^-36 (exp), 2^-41
+// (tanh) or 2^-30 (pow, |y| <= 2^10), then rounded once to binary32, so within 0.52 ulp of it -- the
+// full-precision binary64 routines above cost 2-5x as much, which kept these maps below the HBM
+// rate. Special values as in the binary64 routines.
+namespace f32 {
+
+// 1 / i!, folded at compile time
+template <int I>
+struct InvFact {
+    static constexpr double v = InvFact<I - 1>::v / (double)I;
+};
+template <>
+struct InvFact<0> {
+    static constexpr double v = 1.0;
+};
+template <int D, int I = D>
+__device__ __forceinline__ double exp_horner(double r, double p) {
+    if constexpr (I == 0) return p;
+    else return exp_horner<D, I - 1>(r, __builtin_fma(p, r, InvFact<I - 1>::v));
+}
+// e^r for |r| <= ln2/2 + a little: Taylor of degree D in Horner form (relative error below
+// |r|^(D+1) / (D+1)! / e^-|r|).
+template <int D>
+__device__ __forceinline__ double exp_taylor(double r) {
+    return exp_horner<D>(r, InvFact<D>::v);
+}
+// 2^k e^r, k = rint(x log2 e): for |x| <= 1100
+template <int D>
+__device__ __forceinline__ double exp_reduced(double x) {
+    const double k = __builtin_rint(x * 0x1.71547652b82fep0);
+    const double r = __builtin_fma(-k, 0x1.62e42fefa39efp-1, x);  // |k ln2 - fl(k ln2)| <= 2^-43
+    return __builtin_ldexp(exp_taylor<D>(r), (int)k);
+}
+
+// 1 / d for the positive normal d here: the hardware reciprocal refined by one Newton step
+// (relative error below 2^-50), where a correctly rounded quotient is not needed
+__device__ __forceinline__ double recip(double d) {
+    double r = __builtin_amdgcn_rcp(d);
+    const double e = __builtin_fma(-d, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+
+// e^x unrounded, degree 9: (0.3466)^10 / 10! / 0.707 < 2^-36 relative (NaN propagates)
+__device__ __forceinline__ double exp_d(float xf) {
+    return exp_reduced<9>(__builtin_fmin(__builtin_fmax((double)xf, -110.0), 100.0));  // 0 / inf once rounded
+}
+__device__ inline float exp(float xf) {
+    if (xf != xf) return xf + xf;
+    return (float)exp_d(xf);
+}
+
+// tanh: x for |x| < 2^-12 (the cubic term is below half an ulp); else (1 - t) / (1 + t) with
+// t = e^-2|x| to 2^-47: 1 - t cancels for small |x|, but its relative error stays below
+// 2^-52 / (2|x|) <= 2^-41 there, well inside the 29 bits binary64 carries beyond binary32; the
+// sign restored. No branches: every lane of a wavefront runs the same instructions.
+__device__ inline float tanh(float xf) {
+    const double x = (double)xf, a = __builtin_fmin(__builtin_fabs(x), 600.0);
+    const double t = exp_reduced<11>(-2.0 * a);
+    const double v = (1.0 - t) * recip(1.0 + t);
+    return (a < 0x1p-12 || xf != xf) ? xf : (float)__builtin_copysign(v, x);  // NaN: fmin dropped it
+}
+
+// log2 of a positive normal binary64 (a binary32 value): x = 2^e m, m in [sqrt(1/2), sqrt(2)),
+// ln m = 2 atanh(s), s = (m - 1) / (m + 1), |s| <= 0.1716, to s^13 (relative error < 2^-40)
+__device__ __forceinline__ double log2_pos(double x) {
+    int e;
+    const double f = __builtin_frexp(x, &e);  // [0.5, 1)
+    const bool lo = f < 0x1.6a09e667f3bcdp-1;
+    const double m = lo ? 2.0 * f : f;
+    const double s = (m - 1.0) * recip(m + 1.0), z = s * s;
+    double p = 1.0 / 13.0;
+#pragma unroll
+    for (int k = 11; k >= 1; k -= 2) p = __builtin_fma(p, z, 1.0 / (double)k);
+    return __builtin_fma(2.0 * s * p, 0x1.71547652b82fep0, (double)(lo ? e - 1 : e));
+}
+
+// The special cases of pow (ufuncs.rs:166 semantics, as the binary64 routine), for the arguments
+// the fast path below does not take.
+__device__ __noinline__ float pow_special(float xf, float yf) {
+    const double x = (double)xf, y = (double)yf;
+    if (y == 0.0 || x == 1.0) return 1.0f;
+    if (x != x || y != y) return (float)(x + y);
+    const double ax = __builtin_fabs(x), ay = __builtin_fabs(y);
+    const bool yint = __builtin_trunc(y) == y;
+    const bool yodd = yint && ay < 0x1p53 && __builtin_trunc(ay * 0.5) != ay * 0.5;  // ay / 2 is exact
+    const bool neg = __builtin_signbit(x) && yodd;
+    if (ay == __builtin_inf()) {
+        if (ax == 1.0) return 1.0f;
+        return ((ax < 1.0) == (y < 0.0)) ? __builtin_inff() : 0.0f;
+    }
+    if (ax == __builtin_inf()) {
+        const float v = y < 0.0 ? 0.0f : __builtin_inff();
+        return neg ? -v : v;
+    }
+    if (ax == 0.0) {
+        const float v = y < 0.0 ? __builtin_inff() : 0.0f;
+        return neg ? __builtin_copysignf(v, xf) : v;
+    }
+    if (x < 0.0 && !yint) return __builtin_nanf("");
+    const double t = __builtin_fmin(__builtin_fmax(y * log2_pos(ax), -1100.0), 1100.0);
+    const double k = __builtin_rint(t);
+    const double v = __builtin_ldexp(exp_taylor<10>((t - k) * 0x1.62e42fefa39efp-1), (int)k);
+    return (float)(neg ? -v : v);
+}
+
+// pow: 2^(y log2 x) for finite x > 0 and finite y, the exponent's error |y| 2^-40 + 2^-53 |y log2 x|
+// carried into the result's relative error (degree 10 for 2^f: (0.347)^11 / 11! < 2^-42); every
+// other argument through pow_special.
+__device__ inline float pow(float xf, float yf) {
+    const double x = (double)xf, y = (double)yf;
+    if (!(x > 0.0 && x < __builtin_inf() && __builtin_fabs(y) < __builtin_inf()) || y == 0.0 || x == 1.0)
+        return pow_special(xf, yf);
+    const double t = __builtin_fmin(__builtin_fmax(y * log2_pos(x), -1100.0), 1100.0);
+    const double k = __builtin_rint(t);
+    return (float)__builtin_ldexp(exp_taylor<10>((t - k) * 0x1.62e42fefa39efp-1), (int)k);
+}
+
+}  // namespace f32
+
+// The elementary functions by argument type: binary64 at full precision, binary32 through the
+// routines above. Every kernel of the library that evaluates exp, tanh or pow of a T calls these.
+__device__ __forceinline__ double exp_t(double x) { return exp(x); }
+__device__ __forceinline__ float exp_t(float x) { return f32::exp(x); }
+__device__ __forceinline__ double tanh_t(double x) { return tanh(x); }
+__device__ __forceinline__ float tanh_t(float x) { return f32::tanh(x); }
+__device__ __forceinline__ double pow_t(double x, double y) { return pow(x, y); }
+__device__ __forceinline__ float pow_t(float x, float y) { return f32::pow(x, y); }
+
 }  // namespace fm
 }  // namespace nrk

@@ -929,15 +929,25 @@ template nrk_status dot<float>(const float *, const float *, size_t, float *, hi
 template nrk_status dot<double>(const double *, const double *, size_t, double *, hipStream_t);
 
 // ---------------- softmax statistics: (max, sum of e^(x - max)) in one read ----------------
-// ufuncs.rs:1411-1457 forms max, then sum exp(x - max). Here each thread takes the max m of its
-// SM_PER values and the sum s of e^(v - m) over them; partials (m, s) merge in a fixed tree as
-// (max(m1, m2), s1 e^(m1 - m) + s2 e^(m2 - m)). -inf values add 0 (as e^(-inf - M) does in the
-// reference), a partial with s = 0 adds nothing, and NaN or +inf values make s NaN, as they
-// make the reference's sum NaN. The max is exact and NaN-ignoring like the reference's fold.
-constexpr int SM_PER = 16;
+// ufuncs.rs:1411-1457 forms max, then sum exp(x - max). Here each wavefront takes the max m of
+// its 64 x sm_per values of a segment and the sum s of e^(v - m) over them; the partials (m, s)
+// of the wavefronts, of a workgroup's segments (in order) and of the workgroups merge as
+// (max(m1, m2), s1 e^(m1 - m) + s2 e^(m2 - m)). -inf values add 0 (as e^(-inf - M) does in the reference), a partial with
+// s = 0 adds nothing, and NaN or +inf values make s NaN, as they make the reference's sum NaN.
+// The max is exact and NaN-ignoring like the reference's fold.
+// values per thread per segment: 16 whole 16-byte vectors of T (256 bytes in flight per lane)
+template <typename T>
+constexpr int sm_per() { return 16 * Vec16<T>::N; }
 
 template <typename T>
-__device__ __forceinline__ T sm_exp(T v) { return (T)fm::exp((double)v); }
+__device__ __forceinline__ T sm_exp(T v) { return fm::exp_t(v); }
+// the terms of the sum in binary32: e^v (v <= 0) to 2^-22 relative (degree 6 after the
+// reduction), ample for a sum whose own bound is tens of ulps; binary64 keeps the full routine
+__device__ __forceinline__ float sm_term(float v) {
+    const double d = (double)v;
+    return v != v ? v : (float)fm::f32::exp_reduced<6>(d < -110.0 ? -110.0 : d);  // NaN reaches the sum
+}
+__device__ __forceinline__ double sm_term(double v) { return fm::exp(v); }
 
 template <typename T>
 __device__ __forceinline__ void sm_merge(T &m, T &s, T m2, T s2) {
@@ -948,42 +958,76 @@ __device__ __forceinline__ void sm_merge(T &m, T &s, T m2, T s2) {
     s = rn_add(a, b);
 }
 
-template <typename T>
+// VEC (x 16-byte aligned): a thread's SM_PER values are SM_PER / VN whole vectors of the segment.
+template <typename T, bool VEC>
 __global__ __launch_bounds__(RB) void softmax_stats_kernel(const T *x, size_t n, size_t nseg, T *partials) {
+    typedef typename Vec16<T>::type V;
+    constexpr int VN = Vec16<T>::N;
     __shared__ T sm[2][WAVES], ss[2][WAVES];
     const int t = threadIdx.x;
     const T NEG_INF = -__builtin_huge_valf();
     int parity = 0;
-    for (size_t seg = blockIdx.x; seg < nseg; seg += gridDim.x, parity ^= 1) {
+    constexpr int SM_PER = sm_per<T>();
+    T run_m = NEG_INF, run_s = (T)0;  // thread 0: the merged pair of this workgroup's segments
+    // the segment's values of this thread; the next segment's are loaded while this one is reduced
+    auto load = [&](size_t seg, T (&v)[SM_PER]) {
         const size_t off = seg * (size_t)(RB * SM_PER);
-        T v[SM_PER];
+        if constexpr (VEC) {
 #pragma unroll
-        for (int j = 0; j < SM_PER; ++j) {
-            const size_t pos = off + (size_t)j * RB + t;
-            v[j] = pos < n ? x[pos] : NEG_INF;
+            for (int j = 0; j < SM_PER / VN; ++j) {
+                const size_t pos = off + ((size_t)j * RB + t) * VN;
+                if (pos + VN <= n) {  // a plain (cached) load: the output pass reads x again next
+                    const V w = *reinterpret_cast<const V *>(x + pos);
+#pragma unroll
+                    for (int e = 0; e < VN; ++e) v[j * VN + e] = w[e];
+                } else {
+#pragma unroll
+                    for (int e = 0; e < VN; ++e) v[j * VN + e] = pos + e < n ? x[pos + e] : NEG_INF;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < SM_PER; ++j) {
+                const size_t pos = off + (size_t)j * RB + t;
+                v[j] = pos < n ? x[pos] : NEG_INF;
+            }
         }
+    };
+    T v[SM_PER], vn[SM_PER];
+    if (blockIdx.x < nseg) load(blockIdx.x, v);
+    for (size_t seg = blockIdx.x; seg < nseg; seg += gridDim.x, parity ^= 1) {
+        if (seg + gridDim.x < nseg) load(seg + gridDim.x, vn);
+        // the wavefront's max first (exact, NaN never wins), then every lane's sum of e^(v - max)
+        // against that common max, summed with plain additions: one exp per element
         T m = NEG_INF;
 #pragma unroll
-        for (int j = 0; j < SM_PER; ++j) m = v[j] > m ? v[j] : m;  // NaN never wins
-        T s = (T)0;
-#pragma unroll
-        for (int j = 0; j < SM_PER; ++j) s = rn_add(s, v[j] == NEG_INF ? (T)0 : sm_exp(rn_sub(v[j], m)));
+        for (int j = 0; j < SM_PER; ++j) m = v[j] > m ? v[j] : m;
 #pragma unroll
         for (int k = 32; k >= 1; k >>= 1) {
-            const T m2 = shfl_xor(m, k), s2 = shfl_xor(s, k);
-            sm_merge(m, s, m2, s2);
+            const T m2 = shfl_xor(m, k);
+            m = m2 > m ? m2 : m;
         }
+        T s = (T)0;
+#pragma unroll
+        for (int j = 0; j < SM_PER; ++j) s = rn_add(s, v[j] == NEG_INF ? (T)0 : sm_term(rn_sub(v[j], m)));
+#pragma unroll
+        for (int k = 32; k >= 1; k >>= 1) s = rn_add(s, shfl_xor(s, k));
         if ((t & 63) == 0) {
             sm[parity][t >> 6] = m;
             ss[parity][t >> 6] = s;
         }
         __syncthreads();
-        if (t == 0) {
+        if (t == 0) {  // the segment's waves, then into the workgroup's running pair
             T M = sm[parity][0], S = ss[parity][0];
             for (int k = 1; k < WAVES; ++k) sm_merge(M, S, sm[parity][k], ss[parity][k]);
-            partials[2 * seg] = M;
-            partials[2 * seg + 1] = S;
+            sm_merge(run_m, run_s, M, S);
         }
+#pragma unroll
+        for (int j = 0; j < SM_PER; ++j) v[j] = vn[j];
+    }
+    if (t == 0) {  // one partial per workgroup
+        partials[2 * blockIdx.x] = run_m;
+        partials[2 * blockIdx.x + 1] = run_s;
     }
 }
 
@@ -1015,13 +1059,23 @@ template <typename T>
 nrk_status softmax_stats(const T *x, size_t n, T *ms, hipStream_t stream) {
     NRK_TRY(ensure_device());
     if (n == 0 || !x || !ms) return fail(NRK_ERR_INVALID_ARGUMENT, "softmax_stats: empty input or NULL pointer");
-    const size_t nseg = ceil_div(n, (size_t)RB * SM_PER);
+    const size_t nseg = ceil_div(n, (size_t)RB * sm_per<T>());
+    // one partial per workgroup; as many workgroups as are resident at once
+    const bool vec = reinterpret_cast<uintptr_t>(x) % 16 == 0;
+    int resident = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, vec ? (const void *)softmax_stats_kernel<T, true>
+                                                                    : (const void *)softmax_stats_kernel<T, false>,
+                                                     RB, 0) != hipSuccess || resident < 1)
+        resident = 2;
+    const int grid = (int)std::min<size_t>(nseg, (size_t)cu_count() * resident);
     Scratch s;
-    NRK_TRY(get_scratch(stream, 0, 2 * nseg * sizeof(T), &s));
-    const int grid = (int)std::min<size_t>(nseg, (size_t)cu_count() * 8);
-    hipLaunchKernelGGL(softmax_stats_kernel<T>, dim3(grid), dim3(RB), 0, stream, x, n, nseg, (T *)s.partials);
+    NRK_TRY(get_scratch(stream, 0, 2 * (size_t)grid * sizeof(T), &s));
+    if (vec)
+        hipLaunchKernelGGL((softmax_stats_kernel<T, true>), dim3(grid), dim3(RB), 0, stream, x, n, nseg, (T *)s.partials);
+    else
+        hipLaunchKernelGGL((softmax_stats_kernel<T, false>), dim3(grid), dim3(RB), 0, stream, x, n, nseg, (T *)s.partials);
     NRK_HIP(hipGetLastError());
-    hipLaunchKernelGGL(softmax_stats_finish<T>, dim3(1), dim3(FB), 0, stream, (const T *)s.partials, nseg, ms);
+    hipLaunchKernelGGL(softmax_stats_finish<T>, dim3(1), dim3(FB), 0, stream, (const T *)s.partials, (size_t)grid, ms);
     NRK_HIP(hipGetLastError());
     return NRK_OK;
 }

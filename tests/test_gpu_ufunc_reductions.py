@@ -378,3 +378,13 @@ def test_at_device_form_skips_out_of_range_indices(gpu, dt):
          C.c_void_p(dbb.ptr), C.c_void_p(st.handle))
     st.synchronize()
     np.testing.assert_array_equal(da.download(), want)
+
+
+def test_softmax_f32_special_inputs(gpu):
+    """binary32: a NaN anywhere makes every output NaN, +inf likewise (inf - inf); -inf entries give
+    exactly 0; equal entries give exactly 1/n."""
+    x = np.array([1.0, np.nan, 2.0] * 5000, np.float32)
+    assert np.isnan(K.softmax(x)).all()
+    assert np.isnan(K.softmax(np.array([1.0, np.inf, 2.0] * 5000, np.float32))).all()
+    y = K.softmax(np.array([0.0, -np.inf, 0.0, 0.0] * 4096, np.float32))
+    assert np.all(y[1::4] == 0) and np.all(y[0::4] == np.float32(1 / 12288))
