@@ -65,6 +65,11 @@ const char *nrk_version(void);
 nrk_status nrk_device_cu_count(int32_t *count);
 
 nrk_status nrk_device_malloc(void **ptr, size_t bytes);
+/* Test hook for the out-of-memory paths (no reference counterpart): the library's allocation
+ * number `after` from now on (0: the next one) -- scratch, workspace, staging or
+ * nrk_device_malloc -- fails with NRK_ERR_OUT_OF_MEMORY, as a real one does; the hook then
+ * disarms. after < 0 disarms it. */
+nrk_status nrk_debug_fail_allocations(int32_t after);
 nrk_status nrk_device_free(void *ptr);
 nrk_status nrk_memcpy_htod(void *dst, const void *src, size_t bytes, nrk_stream stream);
 nrk_status nrk_memcpy_dtoh(void *dst, const void *src, size_t bytes, nrk_stream stream);

@@ -175,6 +175,7 @@ _SIGS: dict[str, list] = {
     "nrk_mlp_train_step": [vp, vp, vp, c_i32, c_i32, vp, vp, vp, c_i32, C.c_float, vp, vp, vp],
     "nrk_selector_set_model": [c_i32, C.c_float, vp, c_i32, vp, vp, vp, vp, C.c_float, C.c_float],
     "nrk_selector_clear": [c_i32],
+    "nrk_debug_fail_allocations": [c_i32],
     "nrk_dev_ragged_per_call": [c_i32, vp, c_i32, vp, vp, vp, vp],
     "nrk_selector_table_put": [c_i32, c_i32, c_i32, c_i32, c_i32],
     "nrk_selector_table_size": [vp],

@@ -1,5 +1,6 @@
 // Runtime part of the C ABI: device selection, errors, memory, streams, events, and the
 // per-stream scratch that the reduction kernels use for their cross-workgroup finish.
+#include <atomic>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -104,6 +105,29 @@ int cu_count() {
     return (it != g_devices.end() && it->second.cus > 0) ? it->second.cus : 256;
 }
 
+// ---- the library's device allocations ----
+// Every buffer the library allocates for itself (per-stream scratch and workspace, host-call
+// staging, nrk_device_malloc) goes through lib_alloc: an out-of-memory result becomes
+// NRK_ERR_OUT_OF_MEMORY with the size and purpose, and nrk_debug_fail_allocations can make
+// the n-th next one fail (a test hook for the error paths; one failure, then it disarms).
+static std::atomic<long long> g_fail_after{-1};
+
+static nrk_status lib_alloc(void **p, size_t bytes, const char *what) {
+    *p = nullptr;
+    long long k = g_fail_after.load();
+    while (k >= 0 && !g_fail_after.compare_exchange_weak(k, k - 1)) {
+    }
+    if (k == 0) return fail(NRK_ERR_OUT_OF_MEMORY, "device allocation of %zu bytes for %s failed (injected)", bytes, what);
+    const hipError_t e = hipMalloc(p, bytes);
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return fail(NRK_ERR_OUT_OF_MEMORY, "device allocation of %zu bytes for %s failed: out of memory", bytes, what);
+    }
+    NRK_HIP(e);
+    return NRK_OK;
+}
+
 // ---- per-(device, stream) scratch ----
 struct StreamScratch {
     unsigned int *counters = nullptr;
@@ -125,7 +149,7 @@ nrk_status get_scratch(hipStream_t stream, size_t n_counters, size_t partial_byt
             s.counters = nullptr;
             s.n_counters = 0;
         }
-        NRK_HIP(hipMalloc(&s.counters, want * sizeof(unsigned int)));
+        NRK_TRY(lib_alloc((void **)&s.counters, want * sizeof(unsigned int), "reduction counters"));
         // Establishes the zero invariant once; kernels keep it.
         NRK_HIP(hipMemset(s.counters, 0, want * sizeof(unsigned int)));
         s.n_counters = want;
@@ -138,7 +162,7 @@ nrk_status get_scratch(hipStream_t stream, size_t n_counters, size_t partial_byt
             s.partials = nullptr;
             s.partial_bytes = 0;
         }
-        NRK_HIP(hipMalloc(&s.partials, want));
+        NRK_TRY(lib_alloc(&s.partials, want, "reduction scratch"));
         s.partial_bytes = want;
     }
     out->counters = s.counters;
@@ -166,7 +190,7 @@ nrk_status get_workspace(hipStream_t stream, size_t bytes, void **out) {
             w.bytes = 0;
         }
         const size_t want = align_up(bytes, 1u << 21);
-        NRK_HIP(hipMalloc(&w.ptr, want));
+        NRK_TRY(lib_alloc(&w.ptr, want, "stream workspace"));
         w.bytes = want;
     }
     *out = w.ptr;
@@ -186,7 +210,7 @@ nrk_status staging(int slot, size_t bytes, void **ptr) {
             s.bytes = 0;
         }
         size_t want = align_up(bytes < 4096 ? 4096 : bytes, 4096);
-        NRK_HIP(hipMalloc(&s.ptr, want));
+        NRK_TRY(lib_alloc(&s.ptr, want, "host-call staging"));
         s.bytes = want;
     }
     *ptr = s.ptr;
@@ -252,7 +276,12 @@ nrk_status nrk_device_malloc(void **ptr, size_t bytes) {
     NRK_TRY(ensure_device());
     *ptr = nullptr;
     if (bytes == 0) bytes = 16;
-    NRK_HIP(hipMalloc(ptr, bytes));
+    NRK_TRY(lib_alloc(ptr, bytes, "nrk_device_malloc"));
+    return NRK_OK;
+}
+
+nrk_status nrk_debug_fail_allocations(int32_t after) {
+    g_fail_after.store(after < 0 ? -1 : (long long)after);
     return NRK_OK;
 }
 

@@ -1,6 +1,7 @@
 """Build recipe for the oracle shared library (test infrastructure only).
 
-    python -m oracle.build
+    python -m oracle.build              # oracle/liboracle.so
+    python -m oracle.build --sanitize   # oracle/_san/sanitize_check (ASan + UBSan, host only)
 
 Compiles oracle/oracle.c with gcc into oracle/liboracle.so. -ffp-contract=off keeps every
 `a * b + c` of the restatement as two roundings (Rust never contracts); -O3 -march=x86-64-v3 (AVX2; the library also runs on the GPU host) for the
@@ -34,5 +35,28 @@ def build(verbose: bool = True) -> Path:
     return LIB
 
 
+SAN_DIR = HERE / "_san"
+SAN_BIN = SAN_DIR / "sanitize_check"
+# oracle.c with a driver that calls every function on exactly-sized buffers (test infrastructure)
+SAN_FLAGS = ["-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
+             "-ffp-contract=off", "-std=gnu11", "-Wall", "-Wno-unused-variable", "-pthread"]
+
+
+def build_sanitized(verbose: bool = True) -> Path:
+    """AddressSanitizer + UndefinedBehaviorSanitizer build of the oracle and its driver
+    (oracle/sanitize_check.c) into oracle/_san/sanitize_check: host code only, gcc."""
+    SAN_DIR.mkdir(exist_ok=True)
+    cmd = ["gcc", *SAN_FLAGS, str(SRC), str(HERE / "sanitize_check.c"), "-o", str(SAN_BIN), "-lm"]
+    proc = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if proc.returncode != 0:
+        raise RuntimeError(f"sanitized oracle build failed: {' '.join(cmd)}\n{proc.stdout}")
+    if verbose:
+        print(f"built {SAN_BIN.relative_to(HERE.parent)}", file=sys.stderr)
+    return SAN_BIN
+
+
 if __name__ == "__main__":
-    build()
+    if "--sanitize" in sys.argv[1:]:
+        build_sanitized()
+    else:
+        build()
