@@ -210,6 +210,9 @@ def main() -> int:
     kernels_out["fused_map_reduce_f64"]["errors"] = fused_errors(x64, k64)
     log("fused errors: f32 " + json.dumps(kernels_out["fused_map_reduce_f32"]["errors"]) + " f64 "
         + json.dumps(kernels_out["fused_map_reduce_f64"]["errors"]))
+    kernels_out["unfused_reference_style"] = unfused_reference_style(device, call, ReduceOutputs, ops_p, n_ops, s, stream,
+                                                                     ((np.float32, d32), (np.float64, d64)), args.steps)
+    log("unfused reference-style sequence: " + json.dumps(kernels_out["unfused_reference_style"]))
     kernels_out["plain_vs_compensated"] = plain_vs_compensated(device, call, ReduceOutputs, ops_p, n_ops, s, stream,
                                                                ((x32, d32), (x64, d64)), args.steps)
     log("plain vs compensated: " + json.dumps(kernels_out["plain_vs_compensated"]))
@@ -276,6 +279,47 @@ def _exact_stats(y):
     full_sum = row_sum.sum()
     full_var = ((yl - full_sum / y.size) ** 2).sum() / y.size
     return row_sum, row_var, full_sum, full_var
+
+
+def unfused_reference_style(device, call, ReduceOutputs, ops_p, n_ops, s, stream, data, steps) -> dict:
+    """The reference's unfused sequence on this GPU, for comparison with the fused pass: y = 1.5 x - 0.25
+    written to HBM (one map launch; the reference's two maps round the same way), then each of the
+    eight reductions (sum, max, argmax, var along axis 1 and over the whole array) as a pass of its
+    own over y. Bytes counted: write y once, read x once and y eight times."""
+    out = {}
+    for dt, dx in data:
+        suf = "f32" if dt == np.float32 else "f64"
+        isz = np.dtype(dt).itemsize
+        n = ROWS * COLS
+        dy = device.DeviceArray((n,), dt)
+        bufs = []
+        outs = []
+        for full in (False, True):
+            for f in ("sum", "max", "argmax", "var"):
+                o = ReduceOutputs()
+                b = device.DeviceArray((1 if full else ROWS,), np.uint64 if f == "argmax" else dt)
+                bufs.append(b)
+                setattr(o, f, b.ptr)
+                outs.append((full, o))
+
+        def once():
+            call(f"nrk_dev_map_{suf}", ops_p, n_ops, C.c_void_p(dx.ptr), C.c_void_p(dy.ptr), C.c_size_t(n), C.c_void_p(s))
+            for full, o in outs:
+                call(f"nrk_dev_map_reduce_{suf}", None, 0, C.c_void_p(dy.ptr), ROWS, COLS, 0,
+                     None if full else C.byref(o), C.byref(o) if full else None, C.c_void_p(s))
+        once()
+        stream.synchronize()
+        a, b = device.Event(), device.Event()
+        a.record(s)
+        for _ in range(steps):
+            once()
+        b.record(s)
+        stream.synchronize()
+        ms = a.elapsed_ms(b) / steps
+        moved = (1 + 1 + 8) * n * isz
+        out[suf] = {"ms": round(ms, 5), "launches": 9, "bytes_moved": moved, "GB/s_moved": round(moved / (ms * 1e-3) / 1e9, 1)}
+        del dy, bufs
+    return out
 
 
 def plain_vs_compensated(device, call, ReduceOutputs, ops_p, n_ops, s, stream, data, steps) -> dict:
