@@ -10,15 +10,17 @@ namespace nrk {
 // when row results are wanted. Then every chunk holds exactly cl full segments, and:
 //   - chunk c is the contiguous elements [c cl WSEG, (c + 1) cl WSEG); each wavefront reduces an
 //     arithmetic sequence of chunks (round robin or a contiguous block, see lean_layout) with one
-//     pointer, loading the next segment while it folds the current one;
+//     pointer, keeping the next segment's loads in flight while it folds the current one;
 //   - every lane of a chunk has the same count (16 per segment), so the Chan updates need no
 //     division: the weight of the j-th segment's merge, 1 / (16 j (j + 1)), comes from a constant
 //     table, and the butterfly's levels merge equal counts with weight 1 / (2 n);
 //   - binary32 lanes add their segment sums into plain FP64 (exact to 2^-29 of a binary32 ulp);
 //     binary64 lanes keep the double word;
-//   - lane 0 writes a row's results as soon as its chunk is merged; each wavefront merges its
-//     chunks in order, the workgroup its 4 wavefronts into one group record, and one finish kernel
-//     merges the group records in a fixed order.
+//   - a row's results wait in LDS and go out after the segment loop; each wavefront merges its
+//     chunks in order, the workgroup its 4 wavefronts into one group record, and the last
+//     workgroup to finish (counted with agent-scope atomics, no fence) merges the group records
+//     in a fixed order: no second launch. Where every record holds the same count, these merges
+//     take no division.
 // The chunking depends only on the shape, so results are bit-identical across launches, grids
 // and devices. The input is an expression of one or two operands (see LeanProg).
 constexpr int LEAN_WTAB = 1024;  // chunks of at most this many segments
@@ -49,7 +51,10 @@ struct LeanParams {
     int row_direct;   // a chunk is a row and row results are wanted
     OutPtrs<T> row;
     int need_group;   // whole-array results wanted: store one record per workgroup
+    int uniform;      // every wavefront reduces the same number of chunks (equal record counts)
     Parts grp;
+    OutPtrs<T> full;         // whole-array results (need_group)
+    unsigned int *counter;   // zero at launch; the last workgroup to finish resets it
 };
 
 template <typename T, int N>
@@ -97,10 +102,12 @@ __device__ __forceinline__ void lean_load(const T *base, T (&y)[VPT * Vec16<T>::
     }
 }
 
-// Fold segment j of the current chunk (loaded into y, and z for the second operand).
+// Fold the j-th segment of the current chunk to be processed, segment sj of the chunk (loaded
+// into y, and z for the second operand). Every segment holds 16 elements per lane, so the order
+// of the Chan merges needs only j; the winners' codes and the chunk's first element use sj.
 template <typename T, int VPT, int FLAGS, int LEAVES, int CH>
 __device__ __forceinline__ void lean_fold(const LeanProg &pg, T s0, T s1, T (&y)[VPT * Vec16<T>::N], T (&z)[VPT * Vec16<T>::N],
-                                          unsigned j, LaneAcc<T> &a, unsigned &fnan) {
+                                          unsigned j, unsigned sj, LaneAcc<T> &a, unsigned &fnan) {
     constexpr int M = VPT * Vec16<T>::N;
     if constexpr (CH == CH_MULADD) {
 #pragma unroll
@@ -113,7 +120,7 @@ __device__ __forceinline__ void lean_fold(const LeanProg &pg, T s0, T s1, T (&y)
         bop_n(pg.op, y, z);
         apply_chain_n(pg.post, y);
     }
-    if (j == 0) fnan = __builtin_isnan(y[0]) ? 1u : 0u;  // lane 0's value is the chunk's first element
+    if (sj == 0) fnan = __builtin_isnan(y[0]) ? 1u : 0u;  // lane 0's value is the chunk's first element
     const T st = pairwise<M, 0, T>([&](int i) { return y[i]; });
     if (__builtin_isnan(st)) {  // rare: tell a NaN element apart from inf - inf
 #pragma unroll
@@ -145,56 +152,206 @@ __device__ __forceinline__ void lean_fold(const LeanProg &pg, T s0, T s1, T (&y)
     } else {
         a.hi += sb;
     }
+    // extrema: the segment's first strict winner, then the lane's running winner (equal values go
+    // to the lower code: segments need not come in index order)
     if (FLAGS & F_MAX) {
+        T sv = -dev_inf<T>();
         unsigned ci = NO_CODE;
 #pragma unroll
         for (int i = 0; i < M; ++i) {
-            const bool gt = y[i] > a.vmax;
-            a.vmax = gt ? y[i] : a.vmax;
+            const bool gt = y[i] > sv;
+            sv = gt ? y[i] : sv;
             ci = gt ? (unsigned)i : ci;
         }
-        a.cmax = ci != NO_CODE ? j * M + ci : a.cmax;
+        const unsigned code = sj * M + ci;
+        const bool take = ci != NO_CODE && (sv > a.vmax || (sv == a.vmax && code < a.cmax));
+        a.vmax = take ? sv : a.vmax;
+        a.cmax = take ? code : a.cmax;
     }
     if (FLAGS & F_MIN) {
+        T sv = dev_inf<T>();
         unsigned ci = NO_CODE;
 #pragma unroll
         for (int i = 0; i < M; ++i) {
-            const bool lt = y[i] < a.vmin;
-            a.vmin = lt ? y[i] : a.vmin;
+            const bool lt = y[i] < sv;
+            sv = lt ? y[i] : sv;
             ci = lt ? (unsigned)i : ci;
         }
-        a.cmin = ci != NO_CODE ? j * M + ci : a.cmin;
+        const unsigned code = sj * M + ci;
+        const bool take = ci != NO_CODE && (sv < a.vmin || (sv == a.vmin && code < a.cmin));
+        a.vmin = take ? sv : a.vmin;
+        a.cmin = take ? code : a.cmin;
     }
 }
 
-// Butterfly of records of equal counts n (every lane of a chunk): M2 += (S_a - S_b)^2 / (2 n).
+// A lane's record of one chunk, in the butterfly: the count is the same on every lane (16 per
+// segment), so it is not carried; winners are chunk-relative element offsets (< 2^32); binary32
+// sums have no low word. Only the fields the op set uses cross lanes.
+template <typename T>
+struct LeanRec {
+    double hi, lo, m2;
+    T vmax, vmin;
+    unsigned imax, imin;  // chunk-relative offsets, NO_CODE: no winner
+    unsigned nan;
+};
+
+__device__ __forceinline__ unsigned shfl_xor_u(unsigned v, unsigned m) { return (unsigned)__shfl_xor((int)v, (int)m, 64); }
+
+// (value, offset) winners: larger (smaller) value wins, equal values go to the lower offset; NO_CODE
+// (no winner) loses to every winner.
+template <typename T>
+__device__ __forceinline__ void lean_take_max(T &v, unsigned &i, T bv, unsigned bi) {
+    const bool take = bi != NO_CODE && (i == NO_CODE || bv > v || (bv == v && bi < i));
+    v = take ? bv : v;
+    i = take ? bi : i;
+}
+template <typename T>
+__device__ __forceinline__ void lean_take_min(T &v, unsigned &i, T bv, unsigned bi) {
+    const bool take = bi != NO_CODE && (i == NO_CODE || bv < v || (bv == v && bi < i));
+    v = take ? bv : v;
+    i = take ? bi : i;
+}
+
+// Butterfly of records of equal counts (n per lane): M2 += (S_a - S_b)^2 / (2 n) at the first
+// level, the weight halving with every level as the counts double.
 template <typename T, int FLAGS>
-__device__ __forceinline__ void lean_wave_merge(Part &a) {
-    double w = 0.5 / a.n;
+__device__ __forceinline__ void lean_wave_merge(LeanRec<T> &a, double n) {
+    double w = 0.5 / n;
 #pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) {
-        const Part b = part_shfl<FLAGS>(a, m);
+    for (unsigned m = 32; m >= 1; m >>= 1) {
+        LeanRec<T> b;
+        b.hi = shfl_xor(a.hi, (int)m);
+        if constexpr (sizeof(T) == 8) b.lo = shfl_xor(a.lo, (int)m);
+        if (FLAGS & F_M2) b.m2 = shfl_xor(a.m2, (int)m);
+        if (FLAGS & F_MAX) { b.vmax = shfl_xor(a.vmax, (int)m); b.imax = shfl_xor_u(a.imax, m); }
+        if (FLAGS & F_MIN) { b.vmin = shfl_xor(a.vmin, (int)m); b.imin = shfl_xor_u(a.imin, m); }
+        b.nan = shfl_xor_u(a.nan, m);
         a.nan |= b.nan;
-        if (FLAGS & F_MAX) take_max(a.vmax, a.imax, b.vmax, b.imax);
-        if (FLAGS & F_MIN) take_min(a.vmin, a.imin, b.vmin, b.imin);
-        if (FLAGS & F_M2) {
-            const double d = (a.hi + a.lo) - (b.hi + b.lo);
-            a.m2 = a.m2 + b.m2 + d * d * w;
+        if (FLAGS & F_MAX) lean_take_max(a.vmax, a.imax, b.vmax, b.imax);
+        if (FLAGS & F_MIN) lean_take_min(a.vmin, a.imin, b.vmin, b.imin);
+        if constexpr (sizeof(T) == 8) {
+            if (FLAGS & F_M2) {
+                const double d = (a.hi + a.lo) - (b.hi + b.lo);
+                a.m2 = a.m2 + b.m2 + d * d * w;
+            }
+            dw_add(a.hi, a.lo, b.hi, b.lo);
+        } else {
+            if (FLAGS & F_M2) {
+                const double d = a.hi - b.hi;
+                a.m2 = a.m2 + b.m2 + d * d * w;
+            }
+            a.hi += b.hi;
         }
-        if constexpr (sizeof(T) == 8) dw_add(a.hi, a.lo, b.hi, b.lo);
-        else a.hi += b.hi;
-        a.n += b.n;
         w *= 0.5;
     }
 }
+
+// a += b for records of equal counts (w = 1 / (2 n), n either count): M2 += (S_a - S_b)^2 w.
+template <int FLAGS>
+__device__ __forceinline__ void part_merge_eq(Part &a, const Part &b, double w) {
+    a.nan |= b.nan;
+    if (FLAGS & F_MAX) take_max(a.vmax, a.imax, b.vmax, b.imax);
+    if (FLAGS & F_MIN) take_min(a.vmin, a.imin, b.vmin, b.imin);
+    if (FLAGS & F_M2) {
+        const double d = (a.hi + a.lo) - (b.hi + b.lo);
+        a.m2 = a.m2 + b.m2 + d * d * w;
+    }
+    dw_add(a.hi, a.lo, b.hi, b.lo);
+    a.n += b.n;
+}
+
+// wp (k chunks of n elements) += one more chunk of n: Chan with n_a = k n, n_b = n, whose term
+// (n_b S_a - n_a S_b)^2 / (n_a n_b (n_a + n_b)) is (S_a - k S_b)^2 / (k (k + 1) n) -- no division
+// (1 / (k (k + 1)) = 16 w[k] from the weight table, k < LEAN_WTAB; else divided).
+template <int FLAGS>
+__device__ __forceinline__ void lean_merge_next(Part &wp, const Part &b, size_t k, double inv_n) {
+    if (k == 0) {
+        wp = b;
+        return;
+    }
+    wp.nan |= b.nan;
+    if (FLAGS & F_MAX) take_max(wp.vmax, wp.imax, b.vmax, b.imax);
+    if (FLAGS & F_MIN) take_min(wp.vmin, wp.imin, b.vmin, b.imin);
+    if (FLAGS & F_M2) {
+        const double d = (wp.hi + wp.lo) - (double)k * (b.hi + b.lo);
+        const double wk = k < (size_t)LEAN_WTAB ? 16.0 * kLeanW.w[k] : 1.0 / ((double)k * (double)(k + 1));
+        wp.m2 = wp.m2 + b.m2 + d * d * (wk * inv_n);
+    }
+    dw_add(wp.hi, wp.lo, b.hi, b.lo);
+    wp.n += b.n;
+}
+
+// Record fields stored and loaded coherently across the chip's L2 caches: relaxed agent-scope
+// atomic accesses (the stores write through to the point of coherence, the loads read there).
+template <typename U>
+__device__ __forceinline__ void st_agent(U *a, U v) {
+    __hip_atomic_store(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename U>
+__device__ __forceinline__ U ld_agent(U *a) {
+    return __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <int FLAGS>
+__device__ __forceinline__ void part_store_agent(const Parts &P, size_t s, const Part &a, unsigned first_nan) {
+    st_agent(P.n + s, a.n); st_agent(P.hi + s, a.hi); st_agent(P.lo + s, a.lo);
+    if (FLAGS & F_M2) st_agent(P.m2 + s, a.m2);
+    if (FLAGS & F_MAX) { st_agent(P.vmax + s, a.vmax); st_agent(P.imax + s, a.imax); }
+    if (FLAGS & F_MIN) { st_agent(P.vmin + s, a.vmin); st_agent(P.imin + s, a.imin); }
+    st_agent(P.flags + s, a.nan | (first_nan << 1));
+}
+template <int FLAGS>
+__device__ __forceinline__ Part part_load_agent(const Parts &P, size_t s) {
+    Part a = part_empty();
+    a.n = ld_agent(P.n + s); a.hi = ld_agent(P.hi + s); a.lo = ld_agent(P.lo + s);
+    if (FLAGS & F_M2) a.m2 = ld_agent(P.m2 + s);
+    if (FLAGS & F_MAX) { a.vmax = ld_agent(P.vmax + s); a.imax = ld_agent(P.imax + s); }
+    if (FLAGS & F_MIN) { a.vmin = ld_agent(P.vmin + s); a.imin = ld_agent(P.imin + s); }
+    a.nan = ld_agent(P.flags + s) & 1u;
+    return a;
+}
+
+// s_waitcnt vmcnt(0), expcnt and lgkmcnt unconstrained (gfx9 encoding: vmcnt in bits 3:0 and
+// 15:14, expcnt 6:4, lgkmcnt 11:8): every vector memory access of this wavefront has completed.
+#define NRK_WAIT_VMEM0() __builtin_amdgcn_s_waitcnt((0x7 << 4) | (0xf << 8))
+
+// The whole-array finish, at most LEAN_MAX_GROUPS group records; arrival counters (the top one,
+// then one per set of LEAN_SET groups) LEAN_CTR_STRIDE words apart.
+constexpr int LEAN_FIN_LOADS = 4;
+constexpr size_t LEAN_MAX_GROUPS = (size_t)RB * LEAN_FIN_LOADS;
+constexpr unsigned LEAN_SET = 32;
+constexpr unsigned LEAN_CTR_STRIDE = 64;
+constexpr size_t LEAN_CTR_WORDS = LEAN_CTR_STRIDE * (1 + LEAN_MAX_GROUPS / LEAN_SET);
+static_assert(1 + LEAN_MAX_GROUPS / LEAN_SET <= (size_t)RB, "the last workgroup resets every counter, one per thread");
+
+// s_waitcnt lgkmcnt(0) then a bare s_barrier: this wavefront's LDS accesses are complete; its
+// global loads and stores may still be in flight (the fence of __syncthreads would wait for them).
+// The empty asm statements keep the compiler from moving memory accesses across the barrier.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_s_waitcnt(0xf | (0x7 << 4) | (0x3 << 14));
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
+// Row results wait in LDS (no global store inside the segment loop, so the compiler's load
+// counting stays exact and the next segment really is in flight) and go out ROWBUF rows at a time.
+constexpr int LEAN_ROWBUF = 16;
+struct RowRec {
+    Part part;
+    uint64_t chunk;
+    unsigned first_nan;
+};
 
 template <typename T, int VPT, int FLAGS, int LEAVES, bool VEC, int CH>
 __global__ __launch_bounds__(RB) void lean_kernel(LeanParams<T> p) {
     constexpr int VN = Vec16<T>::N;
     constexpr int M = VPT * VN;
     constexpr int WSEG = 64 * M;
-    __shared__ Part grp_lds[WAVES];
+    constexpr int DZ = LEAVES == 2 ? 2 : 1;
+    __shared__ Part grp_lds[WAVES], grp_lds2[WAVES];
     __shared__ unsigned grp_fn[WAVES];
+    __shared__ unsigned s_last;
+    __shared__ RowRec rowbuf[WAVES][LEAN_ROWBUF];
     const unsigned lane = threadIdx.x & 63;
     const unsigned wave = lane0((unsigned)(threadIdx.x >> 6));
     const size_t v = blockIdx.x * (size_t)WAVES + wave;
@@ -204,20 +361,29 @@ __global__ __launch_bounds__(RB) void lean_kernel(LeanParams<T> p) {
     const size_t cstride = p.B * p.cl * WSEG;  // elements from one of this wavefront's chunks to the next
     Part wp = part_empty();  // this wavefront's chunks, merged in chunk order
     unsigned wp_fn = 0;
+    unsigned nbuf = 0;  // row results waiting in rowbuf[wave]
+    // lanes r < nbuf write the waiting row results (one row per lane)
+    auto flush_rows = [&]() {
+        if (lane < nbuf) {
+            const RowRec &r = rowbuf[wave][lane];
+            write_outputs<T>(p.row, r.chunk, r.part, p.ddof, r.first_nan, (uint64_t)r.chunk * p.cols);
+        }
+        nbuf = 0;
+    };
     if (nseg > 0) {
         LaneAcc<T> a;
         acc_reset(a);
         unsigned fnan = 0, j = 0;
         size_t k = 0;
-        T ya[M], yb[M], za[M], zb[M];
+        T y[2][M], z[DZ][M];
         // segment t of this wavefront: chunk c0 + (t / cl) B, segment t % cl of it
         const T *n0 = p.x0 + c0 * p.cl * WSEG;  // the next segment to load
         const T *n1 = LEAVES == 2 ? p.x1 + c0 * p.cl * WSEG : nullptr;
         unsigned nj = 0;
         size_t left = nseg;  // segments not yet loaded
-        auto load = [&](T(&y)[M], T(&z)[M]) {
-            lean_load<T, VPT, VEC>(n0, y);
-            if constexpr (LEAVES == 2) lean_load<T, VPT, VEC>(n1, z);
+        auto load = [&](T(&yy)[M], T(&zz)[M]) {
+            lean_load<T, VPT, VEC>(n0, yy);
+            if constexpr (LEAVES == 2) lean_load<T, VPT, VEC>(n1, zz);
             if (left > 1) {  // advance (past the last segment the pointer stays: a harmless reload)
                 --left;
                 if (++nj == p.cl) {
@@ -231,74 +397,147 @@ __global__ __launch_bounds__(RB) void lean_kernel(LeanParams<T> p) {
             }
         };
         const T s0 = (T)p.s0, s1 = (T)p.s1;
-        auto step = [&](T(&y)[M], T(&z)[M]) {
-            lean_fold<T, VPT, FLAGS, LEAVES, CH>(p.prog, s0, s1, y, z, j, a, fnan);
+        const double inv_n = 1.0 / ((double)M * (double)p.cl * 64.0);  // 1 / (elements per chunk)
+        auto step = [&](T(&yy)[M], T(&zz)[M]) {
+            lean_fold<T, VPT, FLAGS, LEAVES, CH>(p.prog, s0, s1, yy, zz, j, j, a, fnan);
             if (++j == p.cl) {  // chunk c = c0 + k B is complete
                 const size_t c = c0 + k * p.B;
                 const uint64_t fbase = (uint64_t)c * p.cl * WSEG;
-                auto flat = [&](unsigned code) -> uint64_t {
+                auto flat = [&](unsigned code) -> unsigned {  // chunk-relative offset of a lane's code
                     const unsigned sl = code / M, i = code % M;
-                    return fbase + (uint64_t)sl * WSEG + (uint64_t)(((i / VN) * 64 + lane) * VN + (i % VN));
+                    return sl * WSEG + ((i / VN) * 64 + lane) * VN + (i % VN);
                 };
+                LeanRec<T> r;
+                r.hi = a.hi; r.lo = a.lo; r.m2 = a.m2; r.nan = a.nan;
+                r.vmax = a.vmax; r.imax = a.cmax == NO_CODE ? NO_CODE : flat(a.cmax);
+                r.vmin = a.vmin; r.imin = a.cmin == NO_CODE ? NO_CODE : flat(a.cmin);
+                lean_wave_merge<T, FLAGS>(r, (double)M * (double)p.cl);
                 Part pa;
-                pa.n = (double)M * (double)p.cl;
-                pa.hi = a.hi; pa.lo = a.lo; pa.m2 = a.m2; pa.nan = a.nan;
-                pa.vmax = a.cmax == NO_CODE ? -__builtin_inf() : (double)a.vmax;
-                pa.imax = a.cmax == NO_CODE ? NO_INDEX : flat(a.cmax);
-                pa.vmin = a.cmin == NO_CODE ? __builtin_inf() : (double)a.vmin;
-                pa.imin = a.cmin == NO_CODE ? NO_INDEX : flat(a.cmin);
-                lean_wave_merge<T, FLAGS>(pa);
+                pa.n = (double)M * (double)p.cl * 64.0;
+                pa.hi = r.hi; pa.lo = sizeof(T) == 8 ? r.lo : 0.0; pa.m2 = r.m2; pa.nan = r.nan;
+                pa.vmax = r.imax == NO_CODE ? -__builtin_inf() : (double)r.vmax;
+                pa.imax = r.imax == NO_CODE ? NO_INDEX : fbase + r.imax;
+                pa.vmin = r.imin == NO_CODE ? __builtin_inf() : (double)r.vmin;
+                pa.imin = r.imin == NO_CODE ? NO_INDEX : fbase + r.imin;
                 const unsigned fn0 = (unsigned)__shfl(fnan, 0, 64);
-                if (p.row_direct && lane == 0) write_outputs<T>(p.row, c, pa, p.ddof, fn0, (uint64_t)c * p.cols);
+                if (p.row_direct) {
+                    if (lane == 0) rowbuf[wave][nbuf] = RowRec{pa, (uint64_t)c, fn0};
+                    if (++nbuf == LEAN_ROWBUF) flush_rows();
+                }
                 if (k == 0) wp_fn = fn0;
-                part_merge<FLAGS>(wp, pa);
+                lean_merge_next<FLAGS>(wp, pa, k, inv_n);
                 acc_reset(a);
                 fnan = 0;
                 j = 0;
                 ++k;
             }
         };
-        load(ya, za);
-        // the next segment's loads are issued before the current one is folded
-        for (size_t s = 0;;) {
-            load(yb, zb);
-            step(ya, za);
-            if (++s == nseg) break;
-            load(ya, za);
-            step(yb, zb);
-            if (++s == nseg) break;
+        // two segments in turn: the next one's loads are in flight while one is folded. The steady
+        // state issues every load and fold unconditionally (exact load counting); an odd last
+        // segment is already loaded when it ends.
+        load(y[0], z[0]);
+        const size_t npair = nseg / 2;
+        for (size_t it = 0; it < npair; ++it) {
+            load(y[1], z[LEAVES == 2 ? 1 : 0]);
+            step(y[0], z[0]);
+            load(y[0], z[0]);
+            step(y[1], z[LEAVES == 2 ? 1 : 0]);
         }
+        if (nseg & 1) step(y[0], z[0]);
     }
-    if (!p.need_group) return;  // uniform over the grid
+    if (!p.need_group) {  // uniform over the grid
+        if (p.row_direct) flush_rows();
+        return;
+    }
     if (lane == 0) {
         grp_lds[wave] = wp;
         grp_fn[wave] = wp_fn;
     }
-    __syncthreads();
+    // LDS-only barriers from here: nothing waits for the row results' global stores, which go
+    // out after this group's arrival is counted
+    lds_barrier();
     if (threadIdx.x == 0) {
-        Part g = grp_lds[0];
-        for (int w = 1; w < WAVES; ++w) part_merge<FLAGS>(g, grp_lds[w]);
-        part_store<FLAGS>(p.grp, blockIdx.x, g, grp_fn[0]);
+        Part g;
+        if (p.uniform) {  // equal counts: pairwise, without division
+            static_assert(WAVES == 4, "the pairwise group merge takes four wavefronts");
+            Part g23 = grp_lds[2];
+            g = grp_lds[0];
+            const double w = 0.5 / g.n;
+            part_merge_eq<FLAGS>(g, grp_lds[1], w);
+            part_merge_eq<FLAGS>(g23, grp_lds[3], w);
+            part_merge_eq<FLAGS>(g, g23, 0.5 * w);
+        } else {
+            g = grp_lds[0];
+            for (int w = 1; w < WAVES; ++w) part_merge<FLAGS>(g, grp_lds[w]);
+        }
+        part_store_agent<FLAGS>(p.grp, blockIdx.x, g, grp_fn[0]);
+        // the record has reached the point of coherence before it is counted. Arrivals are
+        // counted in two levels (a counter per set of LEAN_SET groups, each on its own 256-byte
+        // line, then one for the sets), so no counter takes more than LEAN_SET atomics.
+        NRK_WAIT_VMEM0();
+        const unsigned set = blockIdx.x / LEAN_SET, nset = (gridDim.x + LEAN_SET - 1) / LEAN_SET;
+        const unsigned in_set = min(LEAN_SET, gridDim.x - set * LEAN_SET);
+        bool last = false;
+        if (__hip_atomic_fetch_add(p.counter + LEAN_CTR_STRIDE * (1 + set), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+            in_set - 1)
+            last = __hip_atomic_fetch_add(p.counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == nset - 1;
+        s_last = last;
     }
-}
-
-// The whole-array result from the group records, merged in a fixed order: thread t folds records
-// t, t + LFB, ..., then the wavefront butterflies, then wavefront order.
-constexpr int LFB = 512;
-template <typename T, int FLAGS>
-__global__ __launch_bounds__(LFB) void lean_finish_kernel(Parts in, size_t count, OutPtrs<T> o, double ddof) {
-    __shared__ Part lds[LFB / 64];
-    const unsigned t = threadIdx.x, lane = t & 63, w = t >> 6;
-    Part a = part_empty();
-    for (size_t i = t; i < count; i += LFB) part_merge<FLAGS>(a, part_load<FLAGS>(in, i));
-    wave_merge<FLAGS>(a);
-    if (lane == 0) lds[w] = a;
-    __syncthreads();
-    if (t == 0) {
-        Part r = lds[0];
-        for (int k = 1; k < LFB / 64; ++k) part_merge<FLAGS>(r, lds[k]);
-        write_outputs<T>(o, 0, r, ddof, (in.flags[0] >> 1) & 1u, 0);
+    lds_barrier();
+    if (p.row_direct) flush_rows();
+    if (!s_last) return;  // uniform over the workgroup
+    // The last workgroup to arrive merges the group records in a fixed order, whichever workgroup
+    // it is: thread t folds records t, t + RB, ..., then the wavefront butterflies, then
+    // wavefront order. (Every load is issued before the first is used: indices past the last
+    // record reload it and are then replaced by empty records.) With equal counts and RB or 2 RB
+    // records, every merge is between equal counts and needs no division; the order is then
+    // pairwise: (t, t + RB), the butterfly, (w0 w1)(w2 w3).
+    Part r[LEAN_FIN_LOADS];
+    const bool eq = p.uniform && (gridDim.x == RB || gridDim.x == 2 * RB);
+    if (eq) {
+        r[0] = part_load_agent<FLAGS>(p.grp, threadIdx.x);
+        r[1] = part_load_agent<FLAGS>(p.grp, gridDim.x == RB ? threadIdx.x : threadIdx.x + RB);
+        double w = 0.5 / r[0].n;
+        if (gridDim.x == 2 * RB) {
+            part_merge_eq<FLAGS>(r[0], r[1], w);
+            w *= 0.5;
+        }
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) {
+            const Part b = part_shfl<FLAGS>(r[0], m);
+            part_merge_eq<FLAGS>(r[0], b, w);
+            w *= 0.5;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < LEAN_FIN_LOADS; ++q)
+            r[q] = part_load_agent<FLAGS>(p.grp, min(threadIdx.x + (size_t)q * RB, (size_t)gridDim.x - 1));
+#pragma unroll
+        for (int q = 1; q < LEAN_FIN_LOADS; ++q)
+            if (threadIdx.x + (size_t)q * RB >= gridDim.x) r[q] = part_empty();
+        if (threadIdx.x >= gridDim.x) r[0] = part_empty();
+#pragma unroll
+        for (int q = 1; q < LEAN_FIN_LOADS; ++q) part_merge<FLAGS>(r[0], r[q]);
+        wave_merge<FLAGS>(r[0]);
     }
+    if (lane == 0) grp_lds2[wave] = r[0];
+    lds_barrier();
+    if (threadIdx.x == 0) {
+        Part g = grp_lds2[0];
+        if (eq) {
+            Part g23 = grp_lds2[2];
+            const double w = 0.5 / g.n;
+            part_merge_eq<FLAGS>(g, grp_lds2[1], w);
+            part_merge_eq<FLAGS>(g23, grp_lds2[3], w);
+            part_merge_eq<FLAGS>(g, g23, 0.5 * w);
+        } else {
+            for (int w = 1; w < WAVES; ++w) part_merge<FLAGS>(g, grp_lds2[w]);
+        }
+        const unsigned fn = (ld_agent(p.grp.flags) >> 1) & 1u;  // group 0 holds the first element
+        write_outputs<T>(p.full, 0, g, p.ddof, fn, 0);
+    }
+    // the zero-at-launch invariant for the next launch on this stream
+    if (threadIdx.x <= (gridDim.x + LEAN_SET - 1) / LEAN_SET) st_agent(p.counter + LEAN_CTR_STRIDE * threadIdx.x, 0u);
 }
 
 // Chunk length for a whole-array lean reduction of spr full segments: the smallest divisor of spr
@@ -311,12 +550,12 @@ static size_t lean_chunk_len(size_t spr) {
 }
 
 // The chunk-to-wavefront layout, from the dtype and the shape only (it fixes which chunks a group
-// record holds, hence the whole-array result's bits). Measured on 8192 x 8192 (stream kernel
-// alone): binary32 reads faster when wavefront v takes chunks v, v + W, ... (W the grid's
-// wavefronts; the grid reads one compact window at a time): 41-45 us against 49-50 with
-// contiguous blocks; binary64 the other way round (8 consecutive chunks per wavefront: 79-80 us
-// against 84-86). At most 512 workgroups (the finish merges one record per thread).
+// record holds, hence the whole-array result's bits): binary32 wavefront v takes chunks v, v + W,
+// ... (W the grid's wavefronts), binary64 8 consecutive chunks per wavefront.
 constexpr size_t LEAN_MIN_CHUNKS = 1024;
+constexpr size_t LEAN_GROUPS_F32 = 512;  // round robin over at most this many workgroups
+constexpr size_t LEAN_BLOCK_F64 = 8;     // consecutive chunks per binary64 wavefront (at least)
+static_assert(LEAN_GROUPS_F32 <= LEAN_MAX_GROUPS, "the finish merges at most LEAN_MAX_GROUPS group records");
 struct LeanLayout {
     size_t groups, A, B;
 };
@@ -324,11 +563,11 @@ template <typename T>
 static LeanLayout lean_layout(size_t nchunk) {
     LeanLayout L;
     if (sizeof(T) == 4) {
-        L.groups = std::min<size_t>(512, ceil_div(nchunk, (size_t)4 * WAVES));
+        L.groups = std::min<size_t>(LEAN_GROUPS_F32, ceil_div(nchunk, (size_t)4 * WAVES));
         L.A = 1;
         L.B = L.groups * WAVES;
     } else {
-        L.A = std::max<size_t>(8, ceil_div(nchunk, (size_t)512 * WAVES));
+        L.A = std::max<size_t>(LEAN_BLOCK_F64, ceil_div(nchunk, LEAN_MAX_GROUPS * WAVES));
         L.groups = ceil_div(nchunk, L.A * WAVES);
         L.B = 1;
     }
@@ -344,7 +583,7 @@ static void launch_lean_kernel(const LeanParams<T> &p, size_t ngroups, bool vec,
 }
 
 template <typename T, int VPT, int FLAGS, int LEAVES>
-static nrk_status launch_lean(const LeanParams<T> &p, size_t ngroups, const OutPtrs<T> &full, hipStream_t stream) {
+static nrk_status launch_lean(const LeanParams<T> &p, size_t ngroups, hipStream_t stream) {
     const bool vec = (uintptr_t)p.x0 % 16 == 0 && (LEAVES == 1 || (uintptr_t)p.x1 % 16 == 0);
     if constexpr (LEAVES == 2) {
         launch_lean_kernel<T, VPT, FLAGS, 2, CH_ANY>(p, ngroups, vec, stream);
@@ -358,12 +597,10 @@ static nrk_status launch_lean(const LeanParams<T> &p, size_t ngroups, const OutP
             launch_lean_kernel<T, VPT, FLAGS, 1, CH_ANY>(p, ngroups, vec, stream);
     }
     NRK_HIP(hipGetLastError());
-    if (p.need_group) {
-        hipLaunchKernelGGL((lean_finish_kernel<T, FLAGS>), dim3(1), dim3(LFB), 0, stream, p.grp, ngroups, full, p.ddof);
-        NRK_HIP(hipGetLastError());
-    }
     return NRK_OK;
 }
+
+__global__ void lean_clear_counter_kernel(unsigned int *c) { c[threadIdx.x * LEAN_CTR_STRIDE] = 0u; }
 
 // The lean path, if the shape and operands allow it (else returns false in *taken). x1 != NULL:
 // two operands (prog.op and the three chains), else one (prog.pre0).
@@ -400,6 +637,9 @@ nrk_status try_lean(const LeanProg &prog, const T *x0, const T *x1, size_t rows,
     const size_t ngroups = L.groups;
     p.A = L.A;
     p.B = L.B;
+    // equal chunk counts on every wavefront of the grid: round robin over a multiple of the
+    // wavefront count, or contiguous blocks that fill every wavefront
+    p.uniform = L.A == 1 ? nchunk % (L.groups * WAVES) == 0 : nchunk == L.A * L.groups * WAVES;
     p.prog = prog;
     if (!x1 && prog.pre0.len == 2) {  // CH_MULADD: y = (x * s0) + s1, with x - b as x + (-b)
         p.s0 = prog.pre0.s[0];
@@ -409,17 +649,23 @@ nrk_status try_lean(const LeanProg &prog, const T *x0, const T *x1, size_t rows,
     p.row_direct = want_row;
     p.row = want_row ? row : OutPtrs<T>{};
     p.need_group = want_full;
+    p.full = want_full ? full : OutPtrs<T>{};
     if (want_full) {
         const size_t pbytes = parts_bytes(ngroups) + 256;
         char *b = nullptr;
-        if (ext) {
-            if (pbytes + 256 > ext->bytes)
+        constexpr size_t cbytes = LEAN_CTR_WORDS * sizeof(unsigned);
+        if (ext) {  // the caller's scratch: the counters first, cleared here; records after
+            if (pbytes + cbytes > ext->bytes)
                 return fail(NRK_ERR_INVALID_ARGUMENT, "(%zu, %zu) needs %zu bytes of scratch; the fixed scratch holds %zu", rows, cols,
-                            pbytes + 256, ext->bytes);
-            b = (char *)ext->base + 256;
+                            pbytes + cbytes, ext->bytes);
+            p.counter = (unsigned int *)ext->base;
+            b = (char *)ext->base + cbytes;
+            hipLaunchKernelGGL(lean_clear_counter_kernel, dim3(1), dim3(LEAN_CTR_WORDS / LEAN_CTR_STRIDE), 0, stream, p.counter);
+            NRK_HIP(hipGetLastError());
         } else {
             Scratch s;
-            NRK_TRY(get_scratch(stream, 1, pbytes, &s));
+            NRK_TRY(get_scratch(stream, LEAN_CTR_WORDS, pbytes, &s));
+            p.counter = s.counters;
             b = (char *)s.partials;
         }
         p.grp = carve_parts(b, ngroups);
@@ -431,7 +677,7 @@ nrk_status try_lean(const LeanProg &prog, const T *x0, const T *x1, size_t rows,
     *taken = true;
 #define NRK_LEAN(F)                                                                             \
     case F:                                                                                     \
-        return x1 ? launch_lean<T, VPT, F, 2>(p, ngroups, full, stream) : launch_lean<T, VPT, F, 1>(p, ngroups, full, stream);
+        return x1 ? launch_lean<T, VPT, F, 2>(p, ngroups, stream) : launch_lean<T, VPT, F, 1>(p, ngroups, stream);
     switch (flags) {
         NRK_LEAN(0) NRK_LEAN(1) NRK_LEAN(2) NRK_LEAN(3) NRK_LEAN(4) NRK_LEAN(5) NRK_LEAN(6)
         default: NRK_LEAN(7)
