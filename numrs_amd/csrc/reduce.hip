@@ -929,12 +929,14 @@ template nrk_status dot<float>(const float *, const float *, size_t, float *, hi
 template nrk_status dot<double>(const double *, const double *, size_t, double *, hipStream_t);
 
 // ---------------- softmax statistics: (max, sum of e^(x - max)) in one read ----------------
-// ufuncs.rs:1411-1457 forms max, then sum exp(x - max). Here each wavefront takes the max m of
-// its 64 x sm_per values of a segment and the sum s of e^(v - m) over them; the partials (m, s)
-// of the wavefronts, of a workgroup's segments (in order) and of the workgroups merge as
-// (max(m1, m2), s1 e^(m1 - m) + s2 e^(m2 - m)). -inf values add 0 (as e^(-inf - M) does in the reference), a partial with
-// s = 0 adds nothing, and NaN or +inf values make s NaN, as they make the reference's sum NaN.
-// The max is exact and NaN-ignoring like the reference's fold.
+// ufuncs.rs:1411-1457 forms max, then sum exp(x - max). Here every lane keeps a running pair
+// (m, s): per segment its values' max m' (exact, NaN never wins), the running sum rescaled once by
+// e^(m - m') when m' > m, then e^(v - m') added for each value -- one exp per element, no
+// communication inside the loop. Pairs merge as (max(m1, m2), s1 e^(m1 - m) + s2 e^(m2 - m)):
+// lanes by the xor butterfly, wavefronts in order, then the workgroups' pairs in a fixed order by
+// the last workgroup to finish (no second launch). -inf values add 0 (as e^(-inf - M) does in the
+// reference), a pair with s = 0 adds nothing, and NaN or +inf values make s NaN, as they make the
+// reference's sum NaN.
 // values per thread per segment: 16 whole 16-byte vectors of T (256 bytes in flight per lane)
 template <typename T>
 constexpr int sm_per() { return 16 * Vec16<T>::N; }
@@ -958,17 +960,19 @@ __device__ __forceinline__ void sm_merge(T &m, T &s, T m2, T s2) {
     s = rn_add(a, b);
 }
 
+constexpr int SM_MAX_GROUPS = 4 * RB;  // the finish merges at most this many workgroup pairs
+
 // VEC (x 16-byte aligned): a thread's SM_PER values are SM_PER / VN whole vectors of the segment.
 template <typename T, bool VEC>
-__global__ __launch_bounds__(RB) void softmax_stats_kernel(const T *x, size_t n, size_t nseg, T *partials) {
+__global__ __launch_bounds__(RB) void softmax_stats_kernel(const T *x, size_t n, size_t nseg, T *partials, unsigned *counter,
+                                                           T *ms) {
     typedef typename Vec16<T>::type V;
     constexpr int VN = Vec16<T>::N;
-    __shared__ T sm[2][WAVES], ss[2][WAVES];
-    const int t = threadIdx.x;
+    __shared__ T sm[WAVES], ss[WAVES];
+    __shared__ unsigned s_last;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const T NEG_INF = -__builtin_huge_valf();
-    int parity = 0;
     constexpr int SM_PER = sm_per<T>();
-    T run_m = NEG_INF, run_s = (T)0;  // thread 0: the merged pair of this workgroup's segments
     // the segment's values of this thread; the next segment's are loaded while this one is reduced
     auto load = [&](size_t seg, T (&v)[SM_PER]) {
         const size_t off = seg * (size_t)(RB * SM_PER);
@@ -993,65 +997,66 @@ __global__ __launch_bounds__(RB) void softmax_stats_kernel(const T *x, size_t n,
             }
         }
     };
+    T lm = NEG_INF, ls = (T)0;  // this lane's running pair
     T v[SM_PER], vn[SM_PER];
     if (blockIdx.x < nseg) load(blockIdx.x, v);
-    for (size_t seg = blockIdx.x; seg < nseg; seg += gridDim.x, parity ^= 1) {
+    for (size_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
         if (seg + gridDim.x < nseg) load(seg + gridDim.x, vn);
-        // the wavefront's max first (exact, NaN never wins), then every lane's sum of e^(v - max)
-        // against that common max, summed with plain additions: one exp per element
-        T m = NEG_INF;
+        T m = lm;
 #pragma unroll
         for (int j = 0; j < SM_PER; ++j) m = v[j] > m ? v[j] : m;
-#pragma unroll
-        for (int k = 32; k >= 1; k >>= 1) {
-            const T m2 = shfl_xor(m, k);
-            m = m2 > m ? m2 : m;
+        if (m > lm) {  // a larger max: rescale the running sum once
+            ls = ls == (T)0 ? (T)0 : rn_mul(ls, sm_exp(rn_sub(lm, m)));
+            lm = m;
         }
-        T s = (T)0;
 #pragma unroll
-        for (int j = 0; j < SM_PER; ++j) s = rn_add(s, v[j] == NEG_INF ? (T)0 : sm_term(rn_sub(v[j], m)));
-#pragma unroll
-        for (int k = 32; k >= 1; k >>= 1) s = rn_add(s, shfl_xor(s, k));
-        if ((t & 63) == 0) {
-            sm[parity][t >> 6] = m;
-            ss[parity][t >> 6] = s;
-        }
-        __syncthreads();
-        if (t == 0) {  // the segment's waves, then into the workgroup's running pair
-            T M = sm[parity][0], S = ss[parity][0];
-            for (int k = 1; k < WAVES; ++k) sm_merge(M, S, sm[parity][k], ss[parity][k]);
-            sm_merge(run_m, run_s, M, S);
-        }
+        for (int j = 0; j < SM_PER; ++j) ls = rn_add(ls, v[j] == NEG_INF ? (T)0 : sm_term(rn_sub(v[j], lm)));
 #pragma unroll
         for (int j = 0; j < SM_PER; ++j) v[j] = vn[j];
     }
-    if (t == 0) {  // one partial per workgroup
-        partials[2 * blockIdx.x] = run_m;
-        partials[2 * blockIdx.x + 1] = run_s;
-    }
-}
-
-template <typename T>
-__global__ __launch_bounds__(FB) void softmax_stats_finish(const T *partials, size_t nseg, T *ms) {
-    __shared__ T sm[FB / 64], ss[FB / 64];
-    const int t = threadIdx.x;
-    T m = -__builtin_huge_valf(), s = (T)0;
-    for (size_t k = t; k < nseg; k += FB) sm_merge(m, s, partials[2 * k], partials[2 * k + 1]);
 #pragma unroll
-    for (int k = 32; k >= 1; k >>= 1) {
-        const T m2 = shfl_xor(m, k), s2 = shfl_xor(s, k);
-        sm_merge(m, s, m2, s2);
+    for (int k = 32; k >= 1; k >>= 1) sm_merge(lm, ls, shfl_xor(lm, k), shfl_xor(ls, k));
+    if (lane == 0) {
+        sm[wave] = lm;
+        ss[wave] = ls;
     }
-    if ((t & 63) == 0) {
-        sm[t >> 6] = m;
-        ss[t >> 6] = s;
-    }
-    __syncthreads();
-    if (t == 0) {
+    lds_barrier();
+    if (t == 0) {  // the workgroup's pair, then its arrival
         T M = sm[0], S = ss[0];
-        for (int k = 1; k < FB / 64; ++k) sm_merge(M, S, sm[k], ss[k]);
-        ms[0] = M;
-        ms[1] = S;
+        for (int k = 1; k < WAVES; ++k) sm_merge(M, S, sm[k], ss[k]);
+        st_agent(partials + 2 * blockIdx.x, M);
+        st_agent(partials + 2 * blockIdx.x + 1, S);
+        NRK_WAIT_VMEM0();  // the pair has reached the point of coherence before it is counted
+        s_last = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1;
+    }
+    lds_barrier();
+    if (!s_last) return;  // uniform over the workgroup
+    // the last workgroup: thread t merges pairs t, t + RB, ... in order, then the butterfly,
+    // then the wavefronts in order
+    T pm[4], ps[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const size_t i = min(t + (size_t)q * RB, (size_t)gridDim.x - 1);
+        pm[q] = ld_agent(partials + 2 * i);
+        ps[q] = ld_agent(partials + 2 * i + 1);
+    }
+    T M = NEG_INF, S = (T)0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (t + (size_t)q * RB < gridDim.x) sm_merge(M, S, pm[q], ps[q]);
+#pragma unroll
+    for (int k = 32; k >= 1; k >>= 1) sm_merge(M, S, shfl_xor(M, k), shfl_xor(S, k));
+    if (lane == 0) {
+        sm[wave] = M;
+        ss[wave] = S;
+    }
+    lds_barrier();
+    if (t == 0) {
+        T Mf = sm[0], Sf = ss[0];
+        for (int k = 1; k < WAVES; ++k) sm_merge(Mf, Sf, sm[k], ss[k]);
+        ms[0] = Mf;
+        ms[1] = Sf;
+        st_agent(counter, 0u);  // the zero-at-launch invariant for the next launch on this stream
     }
 }
 
@@ -1060,22 +1065,20 @@ nrk_status softmax_stats(const T *x, size_t n, T *ms, hipStream_t stream) {
     NRK_TRY(ensure_device());
     if (n == 0 || !x || !ms) return fail(NRK_ERR_INVALID_ARGUMENT, "softmax_stats: empty input or NULL pointer");
     const size_t nseg = ceil_div(n, (size_t)RB * sm_per<T>());
-    // one partial per workgroup; as many workgroups as are resident at once
+    // one pair per workgroup; as many workgroups as are resident at once (at most SM_MAX_GROUPS)
     const bool vec = reinterpret_cast<uintptr_t>(x) % 16 == 0;
     int resident = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, vec ? (const void *)softmax_stats_kernel<T, true>
                                                                     : (const void *)softmax_stats_kernel<T, false>,
                                                      RB, 0) != hipSuccess || resident < 1)
         resident = 2;
-    const int grid = (int)std::min<size_t>(nseg, (size_t)cu_count() * resident);
+    const int grid = (int)std::min<size_t>({nseg, (size_t)cu_count() * resident, (size_t)SM_MAX_GROUPS});
     Scratch s;
-    NRK_TRY(get_scratch(stream, 0, 2 * (size_t)grid * sizeof(T), &s));
+    NRK_TRY(get_scratch(stream, 1, 2 * (size_t)grid * sizeof(T), &s));
     if (vec)
-        hipLaunchKernelGGL((softmax_stats_kernel<T, true>), dim3(grid), dim3(RB), 0, stream, x, n, nseg, (T *)s.partials);
+        hipLaunchKernelGGL((softmax_stats_kernel<T, true>), dim3(grid), dim3(RB), 0, stream, x, n, nseg, (T *)s.partials, s.counters, ms);
     else
-        hipLaunchKernelGGL((softmax_stats_kernel<T, false>), dim3(grid), dim3(RB), 0, stream, x, n, nseg, (T *)s.partials);
-    NRK_HIP(hipGetLastError());
-    hipLaunchKernelGGL(softmax_stats_finish<T>, dim3(1), dim3(FB), 0, stream, (const T *)s.partials, (size_t)grid, ms);
+        hipLaunchKernelGGL((softmax_stats_kernel<T, false>), dim3(grid), dim3(RB), 0, stream, x, n, nseg, (T *)s.partials, s.counters, ms);
     NRK_HIP(hipGetLastError());
     return NRK_OK;
 }

@@ -78,6 +78,30 @@ struct TreeParams {
 };
 
 // ---------------- small device utilities ----------------
+// Record fields stored and loaded coherently across the chip's L2 caches: relaxed agent-scope
+// atomic accesses (the stores write through to the point of coherence, the loads read there).
+template <typename U>
+__device__ __forceinline__ void st_agent(U *a, U v) {
+    __hip_atomic_store(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename U>
+__device__ __forceinline__ U ld_agent(U *a) {
+    return __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// s_waitcnt vmcnt(0), expcnt and lgkmcnt unconstrained (gfx9 encoding: vmcnt in bits 3:0 and
+// 15:14, expcnt 6:4, lgkmcnt 11:8): every vector memory access of this wavefront has completed.
+#define NRK_WAIT_VMEM0() __builtin_amdgcn_s_waitcnt((0x7 << 4) | (0xf << 8))
+
+// s_waitcnt lgkmcnt(0) then a bare s_barrier: this wavefront's LDS accesses are complete; its
+// global loads and stores may still be in flight (the fence of __syncthreads would wait for them).
+// The empty asm statements keep the compiler from moving memory accesses across the barrier.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_s_waitcnt(0xf | (0x7 << 4) | (0x3 << 14));
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
 template <typename T>
 __device__ __forceinline__ void two_sum(T a, T b, T &s, T &e) {
     s = rn_add(a, b);

@@ -281,16 +281,6 @@ __device__ __forceinline__ void lean_merge_next(Part &wp, const Part &b, size_t 
     wp.n += b.n;
 }
 
-// Record fields stored and loaded coherently across the chip's L2 caches: relaxed agent-scope
-// atomic accesses (the stores write through to the point of coherence, the loads read there).
-template <typename U>
-__device__ __forceinline__ void st_agent(U *a, U v) {
-    __hip_atomic_store(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <typename U>
-__device__ __forceinline__ U ld_agent(U *a) {
-    return __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 template <int FLAGS>
 __device__ __forceinline__ void part_store_agent(const Parts &P, size_t s, const Part &a, unsigned first_nan) {
     st_agent(P.n + s, a.n); st_agent(P.hi + s, a.hi); st_agent(P.lo + s, a.lo);
@@ -310,10 +300,6 @@ __device__ __forceinline__ Part part_load_agent(const Parts &P, size_t s) {
     return a;
 }
 
-// s_waitcnt vmcnt(0), expcnt and lgkmcnt unconstrained (gfx9 encoding: vmcnt in bits 3:0 and
-// 15:14, expcnt 6:4, lgkmcnt 11:8): every vector memory access of this wavefront has completed.
-#define NRK_WAIT_VMEM0() __builtin_amdgcn_s_waitcnt((0x7 << 4) | (0xf << 8))
-
 // The whole-array finish, at most LEAN_MAX_GROUPS group records; arrival counters (the top one,
 // then one per set of LEAN_SET groups) LEAN_CTR_STRIDE words apart.
 constexpr int LEAN_FIN_LOADS = 4;
@@ -322,16 +308,6 @@ constexpr unsigned LEAN_SET = 32;
 constexpr unsigned LEAN_CTR_STRIDE = 64;
 constexpr size_t LEAN_CTR_WORDS = LEAN_CTR_STRIDE * (1 + LEAN_MAX_GROUPS / LEAN_SET);
 static_assert(1 + LEAN_MAX_GROUPS / LEAN_SET <= (size_t)RB, "the last workgroup resets every counter, one per thread");
-
-// s_waitcnt lgkmcnt(0) then a bare s_barrier: this wavefront's LDS accesses are complete; its
-// global loads and stores may still be in flight (the fence of __syncthreads would wait for them).
-// The empty asm statements keep the compiler from moving memory accesses across the barrier.
-__device__ __forceinline__ void lds_barrier() {
-    __builtin_amdgcn_s_waitcnt(0xf | (0x7 << 4) | (0x3 << 14));
-    asm volatile("" ::: "memory");
-    __builtin_amdgcn_s_barrier();
-    asm volatile("" ::: "memory");
-}
 
 // Row results wait in LDS (no global store inside the segment loop, so the compiler's load
 // counting stays exact and the next segment really is in flight) and go out ROWBUF rows at a time.
