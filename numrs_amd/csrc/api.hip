@@ -19,9 +19,9 @@ nrk_status host_call(const T *a, size_t n, OutT *out, size_t out_count, F dev) {
     NRK_TRY(staging(0, n * sizeof(T) + 16, &da));
     NRK_TRY(staging(1, out_count * sizeof(OutT) + 16, &dout));
     hipStream_t s = nullptr;
-    if (n) NRK_HIP(hipMemcpyAsync(da, a, n * sizeof(T), hipMemcpyHostToDevice, s));
+    if (n) NRK_TRY(copy_h2d(da, a, n * sizeof(T), s));
     NRK_TRY(dev((const T *)da, (OutT *)dout, s));
-    if (out_count) NRK_HIP(hipMemcpyAsync(out, dout, out_count * sizeof(OutT), hipMemcpyDeviceToHost, s));
+    if (out_count) NRK_TRY(copy_d2h(out, dout, out_count * sizeof(OutT), s));
     NRK_HIP(hipStreamSynchronize(s));
     return NRK_OK;
 }
@@ -126,7 +126,7 @@ nrk_status host_dot(const T *a, const T *b, size_t n, T *out) {
     if (n && !b) return fail(NRK_ERR_INVALID_ARGUMENT, "b is NULL");
     void *db = nullptr;
     NRK_TRY(staging(2, n * sizeof(T) + 16, &db));
-    if (n) NRK_HIP(hipMemcpyAsync(db, b, n * sizeof(T), hipMemcpyHostToDevice, nullptr));
+    if (n) NRK_TRY(copy_h2d(db, b, n * sizeof(T), nullptr));
     return host_call<T, T>(a, n, out, 1, [&](const T *d, T *dout, hipStream_t s) { return dot<T>(d, (const T *)db, n, dout, s); });
 }
 
@@ -185,7 +185,7 @@ nrk_status host_zip(int32_t op, const T *a, const T *b, T *y, size_t n) {
     if (n && (!a || !b || !y)) return fail(NRK_ERR_INVALID_ARGUMENT, "a, b or y is NULL");
     void *db = nullptr;
     NRK_TRY(staging(2, n * sizeof(T) + 16, &db));
-    if (n) NRK_HIP(hipMemcpyAsync(db, b, n * sizeof(T), hipMemcpyHostToDevice, nullptr));
+    if (n) NRK_TRY(copy_h2d(db, b, n * sizeof(T), nullptr));
     return host_call<T, T>(a, n, y, n, [&](const T *d, T *dout, hipStream_t s) { return zip<T>(op, d, (const T *)db, dout, n, s); });
 }
 
@@ -197,8 +197,8 @@ nrk_status host_tri(int op, const T *a, const T *b, const T *c, T t, T *y, size_
     void *db = nullptr, *dc = nullptr;
     NRK_TRY(staging(2, n * sizeof(T) + 16, &db));
     NRK_TRY(staging(3, n * sizeof(T) + 16, &dc));
-    if (n) NRK_HIP(hipMemcpyAsync(db, b, n * sizeof(T), hipMemcpyHostToDevice, nullptr));
-    if (n && op == 0) NRK_HIP(hipMemcpyAsync(dc, c, n * sizeof(T), hipMemcpyHostToDevice, nullptr));
+    if (n) NRK_TRY(copy_h2d(db, b, n * sizeof(T), nullptr));
+    if (n && op == 0) NRK_TRY(copy_h2d(dc, c, n * sizeof(T), nullptr));
     return host_call<T, T>(a, n, y, n, [&](const T *d, T *dout, hipStream_t s) {
         return tri<T>(op, d, (const T *)db, (const T *)dc, t, dout, n, s);
     });
@@ -228,7 +228,7 @@ nrk_status host_outer(int32_t op, const T *a, size_t na, const T *b, size_t nb, 
     if (nb && !b) return fail(NRK_ERR_INVALID_ARGUMENT, "b is NULL");
     void *db = nullptr;
     NRK_TRY(staging(2, nb * sizeof(T) + 16, &db));
-    if (nb) NRK_HIP(hipMemcpyAsync(db, b, nb * sizeof(T), hipMemcpyHostToDevice, nullptr));
+    if (nb) NRK_TRY(copy_h2d(db, b, nb * sizeof(T), nullptr));
     return host_call<T, T>(a, na, y, na * nb, [&](const T *d, T *o, hipStream_t s) { return outer<T>(op, d, na, (const T *)db, nb, o, s); });
 }
 template <typename T>
@@ -239,8 +239,8 @@ nrk_status host_where(int32_t op, const T *a, const T *b, const uint8_t *mask, T
     NRK_TRY(staging(2, n * sizeof(T) + 16, &db));
     NRK_TRY(staging(3, n + 16, &dm));
     if (n) {
-        NRK_HIP(hipMemcpyAsync(db, b, n * sizeof(T), hipMemcpyHostToDevice, nullptr));
-        NRK_HIP(hipMemcpyAsync(dm, mask, n, hipMemcpyHostToDevice, nullptr));
+        NRK_TRY(copy_h2d(db, b, n * sizeof(T), nullptr));
+        NRK_TRY(copy_h2d(dm, mask, n, nullptr));
     }
     return host_call<T, T>(a, n, y, n, [&](const T *d, T *o, hipStream_t s) { return where_op<T>(op, d, (const T *)db, (const uint8_t *)dm, o, n, s); });
 }
@@ -252,7 +252,7 @@ nrk_status host_reduceat(int32_t op, const T *x, size_t rows, size_t len, const 
         if (idx[k] >= len) return fail(NRK_ERR_INVALID_ARGUMENT, "index %llu out of bounds for reduceat axis of size %zu", (unsigned long long)idx[k], len);
     void *di = nullptr;
     NRK_TRY(staging(2, nidx * sizeof(uint64_t) + 16, &di));
-    if (nidx) NRK_HIP(hipMemcpyAsync(di, idx, nidx * sizeof(uint64_t), hipMemcpyHostToDevice, nullptr));
+    if (nidx) NRK_TRY(copy_h2d(di, idx, nidx * sizeof(uint64_t), nullptr));
     return host_call<T, T>(x, rows * len, y, rows * nidx, [&](const T *d, T *o, hipStream_t s) {
         return reduceat<T>(op, d, rows, len, (const uint64_t *)di, nidx, o, s);
     });
@@ -271,11 +271,11 @@ nrk_status host_at(int32_t op, T *a, size_t n0, size_t row_len, const uint64_t *
     NRK_TRY(staging(2, m * sizeof(uint64_t) + 16, &di));
     NRK_TRY(staging(3, nb * sizeof(T) + 16, &db));
     hipStream_t s = nullptr;
-    NRK_HIP(hipMemcpyAsync(da, a, na * sizeof(T), hipMemcpyHostToDevice, s));
-    NRK_HIP(hipMemcpyAsync(di, idx, m * sizeof(uint64_t), hipMemcpyHostToDevice, s));
-    NRK_HIP(hipMemcpyAsync(db, b, nb * sizeof(T), hipMemcpyHostToDevice, s));
+    NRK_TRY(copy_h2d(da, a, na * sizeof(T), s));
+    NRK_TRY(copy_h2d(di, idx, m * sizeof(uint64_t), s));
+    NRK_TRY(copy_h2d(db, b, nb * sizeof(T), s));
     NRK_TRY(ufunc_at<T>(op, (T *)da, n0, row_len, (const uint64_t *)di, m, (const T *)db, s));
-    NRK_HIP(hipMemcpyAsync(a, da, na * sizeof(T), hipMemcpyDeviceToHost, s));
+    NRK_TRY(copy_d2h(a, da, na * sizeof(T), s));
     NRK_HIP(hipStreamSynchronize(s));
     return NRK_OK;
 }

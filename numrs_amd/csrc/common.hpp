@@ -63,6 +63,12 @@ struct Staging {
 };
 nrk_status staging(int slot, size_t bytes, void **ptr);
 
+// Copies between a host pointer (pageable) and device memory for the host-pointer entry points,
+// through pinned buffers and a small host-copy pool. copy_h2d returns once the source has been
+// read (the DMAs are queued on s); copy_d2h returns with the data in dst.
+nrk_status copy_h2d(void *dst, const void *src, size_t bytes, hipStream_t s);
+nrk_status copy_d2h(void *dst, const void *src, size_t bytes, hipStream_t s);
+
 inline size_t ceil_div(size_t a, size_t b) { return (a + b - 1) / b; }
 inline size_t align_up(size_t a, size_t b) { return ceil_div(a, b) * b; }
 

@@ -383,10 +383,10 @@ static nrk_status host_int_zip(int op, const T *a, const T *b, T *y, size_t n) {
     NRK_TRY(staging(2, n * sizeof(T) + 16, &db));
     NRK_TRY(staging(1, n * sizeof(T) + 16, &dy));
     hipStream_t s = nullptr;
-    NRK_HIP(hipMemcpyAsync(da, a, n * sizeof(T), hipMemcpyHostToDevice, s));
-    NRK_HIP(hipMemcpyAsync(db, b, n * sizeof(T), hipMemcpyHostToDevice, s));
+    NRK_TRY(copy_h2d(da, a, n * sizeof(T), s));
+    NRK_TRY(copy_h2d(db, b, n * sizeof(T), s));
     NRK_TRY(int_zip<T>(op, (const T *)da, (const T *)db, (T *)dy, n, s));
-    NRK_HIP(hipMemcpyAsync(y, dy, n * sizeof(T), hipMemcpyDeviceToHost, s));
+    NRK_TRY(copy_d2h(y, dy, n * sizeof(T), s));
     NRK_HIP(hipStreamSynchronize(s));
     return NRK_OK;
 }
@@ -407,9 +407,9 @@ static nrk_status host_int_map(const nrk_int_op *chain, int32_t chain_len, const
     NRK_TRY(staging(0, n * sizeof(T) + 16, &dx));
     NRK_TRY(staging(1, n * sizeof(T) + 16, &dy));
     hipStream_t s = nullptr;
-    NRK_HIP(hipMemcpyAsync(dx, x, n * sizeof(T), hipMemcpyHostToDevice, s));
+    NRK_TRY(copy_h2d(dx, x, n * sizeof(T), s));
     NRK_TRY(int_map<T>(chain, chain_len, (const T *)dx, (T *)dy, n, s));
-    NRK_HIP(hipMemcpyAsync(y, dy, n * sizeof(T), hipMemcpyDeviceToHost, s));
+    NRK_TRY(copy_d2h(y, dy, n * sizeof(T), s));
     NRK_HIP(hipStreamSynchronize(s));
     return NRK_OK;
 }
@@ -470,10 +470,10 @@ static nrk_status host_zip_broadcast(int dtype, int op, const void *a, int a_ndi
     NRK_TRY(staging(2, nb * es + 16, &db));
     NRK_TRY(staging(1, no * es + 16, &dy));
     hipStream_t s = nullptr;
-    NRK_HIP(hipMemcpyAsync(da, a, na * es, hipMemcpyHostToDevice, s));
-    NRK_HIP(hipMemcpyAsync(db, b, nb * es, hipMemcpyHostToDevice, s));
+    NRK_TRY(copy_h2d(da, a, na * es, s));
+    NRK_TRY(copy_h2d(db, b, nb * es, s));
     NRK_TRY(zip_strided(dtype, op, on, osh, da, sa, db, dtype, sb, dy, s));
-    NRK_HIP(hipMemcpyAsync(y, dy, no * es, hipMemcpyDeviceToHost, s));
+    NRK_TRY(copy_d2h(y, dy, no * es, s));
     NRK_HIP(hipStreamSynchronize(s));
     return NRK_OK;
 }

@@ -278,11 +278,11 @@ static nrk_status host_expr_eval(const nrk_expr_node *nodes, int32_t len, const 
     for (int j = 0; j < pg.n_leaves; ++j) {
         if (!leaves[j]) return fail(NRK_ERR_INVALID_ARGUMENT, "leaf %d is NULL", j);
         T *d = (T *)((char *)ws + span * (j + 1));
-        NRK_HIP(hipMemcpyAsync(d, leaves[j], n * sizeof(T), hipMemcpyHostToDevice, s));
+        NRK_TRY(copy_h2d(d, leaves[j], n * sizeof(T), s));
         dl[j] = d;
     }
     NRK_TRY(expr_eval<T>(pg, dl, (T *)ws, n, s));
-    NRK_HIP(hipMemcpyAsync(y, ws, n * sizeof(T), hipMemcpyDeviceToHost, s));
+    NRK_TRY(copy_d2h(y, ws, n * sizeof(T), s));
     NRK_HIP(hipStreamSynchronize(s));
     return NRK_OK;
 }

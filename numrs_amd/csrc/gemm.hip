@@ -277,10 +277,10 @@ nrk_status host_gemm(size_t m, size_t k, size_t n, const T *a, size_t a_len, con
     NRK_TRY(staging(2, b_len * sizeof(T) + 16, &db));
     NRK_TRY(staging(1, c_len * sizeof(T) + 16, &dc));
     hipStream_t s = nullptr;
-    if (a_len) NRK_HIP(hipMemcpyAsync(da, a, a_len * sizeof(T), hipMemcpyHostToDevice, s));
-    if (b_len) NRK_HIP(hipMemcpyAsync(db, b, b_len * sizeof(T), hipMemcpyHostToDevice, s));
+    if (a_len) NRK_TRY(copy_h2d(da, a, a_len * sizeof(T), s));
+    if (b_len) NRK_TRY(copy_h2d(db, b, b_len * sizeof(T), s));
     NRK_TRY(gemm<T>(m, k, n, (const T *)da, (const T *)db, (T *)dc, s));
-    NRK_HIP(hipMemcpyAsync(c, dc, c_len * sizeof(T), hipMemcpyDeviceToHost, s));
+    NRK_TRY(copy_d2h(c, dc, c_len * sizeof(T), s));
     NRK_HIP(hipStreamSynchronize(s));
     return NRK_OK;
 }
@@ -301,10 +301,10 @@ nrk_status host_batched_gemm(size_t batch, size_t m, size_t k, size_t n, const T
     NRK_TRY(staging(2, b_len * sizeof(T) + 16, &db));
     NRK_TRY(staging(1, c_len * sizeof(T) + 16, &dc));
     hipStream_t s = nullptr;
-    if (a_len) NRK_HIP(hipMemcpyAsync(da, a, a_len * sizeof(T), hipMemcpyHostToDevice, s));
-    if (b_len) NRK_HIP(hipMemcpyAsync(db, b, b_len * sizeof(T), hipMemcpyHostToDevice, s));
+    if (a_len) NRK_TRY(copy_h2d(da, a, a_len * sizeof(T), s));
+    if (b_len) NRK_TRY(copy_h2d(db, b, b_len * sizeof(T), s));
     NRK_TRY(batched_gemm<T>(batch, m, k, n, (const T *)da, m * k, (const T *)db, k * n, (T *)dc, s));
-    NRK_HIP(hipMemcpyAsync(c, dc, c_len * sizeof(T), hipMemcpyDeviceToHost, s));
+    NRK_TRY(copy_d2h(c, dc, c_len * sizeof(T), s));
     NRK_HIP(hipStreamSynchronize(s));
     return NRK_OK;
 }

@@ -357,10 +357,10 @@ nrk_status nrk_gemm_f32_bf16x(size_t m, size_t k, size_t n, const float *a, cons
     NRK_TRY(staging(2, k * n * 4 + 16, &db));
     NRK_TRY(staging(1, m * n * 4 + 16, &dc));
     hipStream_t s = nullptr;
-    if (m * k) NRK_HIP(hipMemcpyAsync(da, a, m * k * 4, hipMemcpyHostToDevice, s));
-    if (k * n) NRK_HIP(hipMemcpyAsync(db, b, k * n * 4, hipMemcpyHostToDevice, s));
+    if (m * k) NRK_TRY(copy_h2d(da, a, m * k * 4, s));
+    if (k * n) NRK_TRY(copy_h2d(db, b, k * n * 4, s));
     NRK_TRY(gemm_f32_bf16x(m, k, n, (const float *)da, (const float *)db, (float *)dc, products, -1, s));
-    NRK_HIP(hipMemcpyAsync(c, dc, m * n * 4, hipMemcpyDeviceToHost, s));
+    NRK_TRY(copy_d2h(c, dc, m * n * 4, s));
     NRK_HIP(hipStreamSynchronize(s));
     return NRK_OK;
 }

@@ -158,10 +158,10 @@ nrk_status host_gemm_generic(size_t m, size_t k, size_t n, const T *a, size_t a_
     NRK_TRY(staging(2, k * n * sizeof(T) + 16, &db));
     NRK_TRY(staging(1, m * n * sizeof(T) + 16, &dc));
     hipStream_t s = nullptr;
-    NRK_HIP(hipMemcpyAsync(da, a, m * k * sizeof(T), hipMemcpyHostToDevice, s));
-    NRK_HIP(hipMemcpyAsync(db, b, k * n * sizeof(T), hipMemcpyHostToDevice, s));
+    NRK_TRY(copy_h2d(da, a, m * k * sizeof(T), s));
+    NRK_TRY(copy_h2d(db, b, k * n * sizeof(T), s));
     NRK_TRY(gemm_generic<T>(1, m, k, n, (const T *)da, 0, (const T *)db, 0, (T *)dc, s));
-    NRK_HIP(hipMemcpyAsync(c, dc, m * n * sizeof(T), hipMemcpyDeviceToHost, s));
+    NRK_TRY(copy_d2h(c, dc, m * n * sizeof(T), s));
     NRK_HIP(hipStreamSynchronize(s));
     return NRK_OK;
 }

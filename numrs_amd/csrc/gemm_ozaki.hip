@@ -701,10 +701,10 @@ nrk_status nrk_gemm_f64_ozaki(size_t m, size_t k, size_t n, const double *a, con
     NRK_TRY(staging(2, k * n * 8 + 16, &db));
     NRK_TRY(staging(1, m * n * 8 + 16, &dc));
     hipStream_t s = nullptr;
-    if (m * k) NRK_HIP(hipMemcpyAsync(da, a, m * k * 8, hipMemcpyHostToDevice, s));
-    if (k * n) NRK_HIP(hipMemcpyAsync(db, b, k * n * 8, hipMemcpyHostToDevice, s));
+    if (m * k) NRK_TRY(copy_h2d(da, a, m * k * 8, s));
+    if (k * n) NRK_TRY(copy_h2d(db, b, k * n * 8, s));
     NRK_TRY(gemm_f64_ozaki(m, k, n, (const double *)da, (const double *)db, (double *)dc, moduli, -1, s));
-    NRK_HIP(hipMemcpyAsync(c, dc, m * n * 8, hipMemcpyDeviceToHost, s));
+    NRK_TRY(copy_d2h(c, dc, m * n * 8, s));
     NRK_HIP(hipStreamSynchronize(s));
     return NRK_OK;
 }

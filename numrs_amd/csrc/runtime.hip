@@ -1,11 +1,15 @@
 // Runtime part of the C ABI: device selection, errors, memory, streams, events, and the
 // per-stream scratch that the reduction kernels use for their cross-workgroup finish.
+#include <algorithm>
 #include <atomic>
+#include <condition_variable>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <thread>
+#include <vector>
 
 #include "common.hpp"
 
@@ -197,6 +201,166 @@ nrk_status get_workspace(hipStream_t stream, size_t bytes, void **out) {
     return NRK_OK;
 }
 
+// ---- host <-> device copies of the host-pointer entry points ----
+// A copy from or to pageable memory goes through a ring of pinned buffers: the host side of each
+// chunk (pageable <-> pinned) is copied by a small pool of threads while the DMA of the previous
+// chunk runs. Measured on the MI355X host: pageable hipMemcpy 27-28 GB/s to the device and 17 GB/s
+// back, pinned DMA 57 GB/s either way, pageable -> pinned memcpy 29 GB/s on one thread and 87 GB/s
+// on four.
+constexpr size_t STAGE_MIN = (size_t)1 << 20;     // below this, one plain hipMemcpyAsync
+constexpr size_t STAGE_CHUNK = (size_t)8 << 20;   // bytes per ring buffer
+constexpr int STAGE_NBUF = 4;                      // ring buffers per host thread and device
+constexpr int COPY_PARTS = 4;                      // host copy: the caller and three pool threads
+
+class CopyPool {
+  public:
+    static CopyPool &get() {
+        static CopyPool pool;
+        return pool;
+    }
+    // memcpy(d, s, n), split over the pool; returns when every part is done
+    void copy(void *d, const void *s, size_t n) {
+        std::lock_guard<std::mutex> one(run_mu_);  // one parallel copy at a time
+        if (threads_.empty())
+            for (int i = 0; i < COPY_PARTS - 1; ++i) threads_.emplace_back([this, i] { worker(i); });
+        {
+            std::lock_guard<std::mutex> l(mu_);
+            dst_ = (char *)d;
+            src_ = (const char *)s;
+            n_ = n;
+            pending_ = COPY_PARTS - 1;
+            ++gen_;
+        }
+        work_cv_.notify_all();
+        part(COPY_PARTS - 1);
+        std::unique_lock<std::mutex> l(mu_);
+        done_cv_.wait(l, [this] { return pending_ == 0; });
+    }
+    ~CopyPool() {
+        {
+            std::lock_guard<std::mutex> l(mu_);
+            stop_ = true;
+        }
+        work_cv_.notify_all();
+        for (auto &t : threads_) t.join();
+    }
+
+  private:
+    void part(int i) {  // part i of COPY_PARTS, cut on 64-byte boundaries
+        const size_t lo = (n_ * i / COPY_PARTS) & ~(size_t)63, hi = i + 1 == COPY_PARTS ? n_ : (n_ * (i + 1) / COPY_PARTS) & ~(size_t)63;
+        if (hi > lo) std::memcpy(dst_ + lo, src_ + lo, hi - lo);
+    }
+    void worker(int i) {
+        unsigned seen = 0;
+        for (;;) {
+            std::unique_lock<std::mutex> l(mu_);
+            work_cv_.wait(l, [&] { return stop_ || gen_ != seen; });
+            if (stop_) return;
+            seen = gen_;
+            l.unlock();
+            part(i);
+            l.lock();
+            if (--pending_ == 0) done_cv_.notify_one();
+        }
+    }
+    std::mutex run_mu_, mu_;
+    std::condition_variable work_cv_, done_cv_;
+    std::vector<std::thread> threads_;
+    char *dst_ = nullptr;
+    const char *src_ = nullptr;
+    size_t n_ = 0;
+    unsigned gen_ = 0;
+    int pending_ = 0;
+    bool stop_ = false;
+};
+
+static void host_copy(void *d, const void *s, size_t n) {
+    if (n < STAGE_MIN) std::memcpy(d, s, n);
+    else CopyPool::get().copy(d, s, n);
+}
+
+struct StageRing {
+    bool tried = false;  // set up once
+    bool ok = false;     // pinned buffers and events exist; else copies take plain hipMemcpy
+    char *buf[STAGE_NBUF] = {};
+    hipEvent_t ev[STAGE_NBUF] = {};
+    bool used[STAGE_NBUF] = {};
+};
+static thread_local std::map<int, StageRing> g_rings;
+
+// This thread's ring on its current device, created on first use (no pinned memory: ok = false).
+static StageRing &stage_ring() {
+    StageRing &r = g_rings[g_thread_device];
+    if (r.tried) return r;
+    r.tried = true;
+    bool good = true;
+    for (int b = 0; b < STAGE_NBUF && good; ++b) {
+        good = hipHostMalloc((void **)&r.buf[b], STAGE_CHUNK, hipHostMallocDefault) == hipSuccess &&
+               hipEventCreateWithFlags(&r.ev[b], hipEventDisableTiming) == hipSuccess;
+    }
+    if (!good) {
+        (void)hipGetLastError();
+        for (int b = 0; b < STAGE_NBUF; ++b) {
+            if (r.buf[b]) (void)hipHostFree(r.buf[b]);
+            if (r.ev[b]) (void)hipEventDestroy(r.ev[b]);
+            r.buf[b] = nullptr;
+            r.ev[b] = nullptr;
+        }
+        return r;  // plain copies from now on
+    }
+    r.ok = true;
+    return r;
+}
+
+nrk_status copy_h2d(void *dst, const void *src, size_t bytes, hipStream_t s) {
+    if (bytes == 0) return NRK_OK;
+    StageRing *r = bytes < STAGE_MIN ? nullptr : &stage_ring();
+    if (!r || !r->ok) {
+        NRK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+        return NRK_OK;
+    }
+    for (size_t off = 0, c = 0; off < bytes; ++c) {
+        const size_t len = std::min(STAGE_CHUNK, bytes - off);
+        const int b = (int)(c % STAGE_NBUF);
+        if (r->used[b]) NRK_HIP(hipEventSynchronize(r->ev[b]));  // its previous DMA is done
+        host_copy(r->buf[b], (const char *)src + off, len);
+        NRK_HIP(hipMemcpyAsync((char *)dst + off, r->buf[b], len, hipMemcpyHostToDevice, s));
+        NRK_HIP(hipEventRecord(r->ev[b], s));
+        r->used[b] = true;
+        off += len;
+    }
+    return NRK_OK;
+}
+
+nrk_status copy_d2h(void *dst, const void *src, size_t bytes, hipStream_t s) {
+    if (bytes == 0) return NRK_OK;
+    StageRing *r = bytes < STAGE_MIN ? nullptr : &stage_ring();
+    if (!r || !r->ok) {
+        NRK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+        NRK_HIP(hipStreamSynchronize(s));
+        return NRK_OK;
+    }
+    const size_t nchunk = ceil_div(bytes, STAGE_CHUNK);
+    auto issue = [&](size_t c) -> nrk_status {  // DMA of chunk c into its ring buffer
+        const int b = (int)(c % STAGE_NBUF);
+        if (r->used[b]) NRK_HIP(hipEventSynchronize(r->ev[b]));
+        const size_t off = c * STAGE_CHUNK, len = std::min(STAGE_CHUNK, bytes - off);
+        NRK_HIP(hipMemcpyAsync(r->buf[b], (const char *)src + off, len, hipMemcpyDeviceToHost, s));
+        NRK_HIP(hipEventRecord(r->ev[b], s));
+        r->used[b] = true;
+        return NRK_OK;
+    };
+    for (size_t c = 0; c < std::min(nchunk, (size_t)STAGE_NBUF); ++c) NRK_TRY(issue(c));
+    for (size_t c = 0; c < nchunk; ++c) {
+        const int b = (int)(c % STAGE_NBUF);
+        NRK_HIP(hipEventSynchronize(r->ev[b]));
+        const size_t off = c * STAGE_CHUNK, len = std::min(STAGE_CHUNK, bytes - off);
+        host_copy((char *)dst + off, r->buf[b], len);
+        if (c + STAGE_NBUF < nchunk) NRK_TRY(issue(c + STAGE_NBUF));
+    }
+    return NRK_OK;
+}
+
 // ---- host staging buffers (per thread, per slot) ----
 static thread_local Staging g_staging[4];
 
@@ -294,7 +458,7 @@ nrk_status nrk_device_free(void *ptr) {
 nrk_status nrk_memcpy_htod(void *dst, const void *src, size_t bytes, nrk_stream stream) {
     if (bytes == 0) return NRK_OK;
     NRK_TRY(ensure_device());
-    NRK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+    NRK_TRY(copy_h2d(dst, src, bytes, (hipStream_t)stream));
     NRK_HIP(hipStreamSynchronize((hipStream_t)stream));
     return NRK_OK;
 }
@@ -302,9 +466,7 @@ nrk_status nrk_memcpy_htod(void *dst, const void *src, size_t bytes, nrk_stream 
 nrk_status nrk_memcpy_dtoh(void *dst, const void *src, size_t bytes, nrk_stream stream) {
     if (bytes == 0) return NRK_OK;
     NRK_TRY(ensure_device());
-    NRK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
-    NRK_HIP(hipStreamSynchronize((hipStream_t)stream));
-    return NRK_OK;
+    return copy_d2h(dst, src, bytes, (hipStream_t)stream);
 }
 
 nrk_status nrk_memcpy_dtod(void *dst, const void *src, size_t bytes, nrk_stream stream) {
