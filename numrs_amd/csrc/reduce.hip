@@ -1108,18 +1108,9 @@ __device__ __forceinline__ T acc_identity() {
     else if constexpr (OP == 5) return -(T)__builtin_huge_valf();
     else return (T)__builtin_huge_valf();
 }
-// the element as the scan sees it: SUB negates and DIV inverts every element after a row's first
-template <typename T, int OP, int XF>
-__device__ __forceinline__ T acc_load(const T *x, size_t pos, size_t col) {
-    const T v = x[pos];
-    if constexpr (XF == 1) return col == 0 ? v : -v;
-    else if constexpr (XF == 2) return col == 0 ? v : rn_div((T)1, v);
-    else if constexpr (OP == 0) return rn_add(v, (T)0);  // -0 -> +0: cumsum's running sum starts at +0
-    else return v;
-}
-
 // Workgroup-wide inclusive scan of one value per thread, in a fixed order: the wavefront's
-// Hillis-Steele steps, then the wavefront totals from LDS.
+// Hillis-Steele steps, then the wavefront totals from LDS. Its barriers are LDS-only: global loads
+// of the next tile stay in flight across them.
 template <typename T, int OP>
 __device__ __forceinline__ T block_scan(T v, T *lds) {
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -1129,43 +1120,88 @@ __device__ __forceinline__ T block_scan(T v, T *lds) {
         if (lane >= d) v = acc_apply<T, OP>(u, v);
     }
     if (lane == 63) lds[w] = v;
-    __syncthreads();
+    lds_barrier();
     T pre = acc_identity<T, OP>();
     for (int k = 0; k < w; ++k) pre = acc_apply<T, OP>(pre, lds[k]);
-    __syncthreads();
+    lds_barrier();
     return w ? acc_apply<T, OP>(pre, v) : v;
 }
 
 // A tile moves between HBM and the threads through LDS: the global loads and stores are
-// coalesced (lane l takes element j * RB + l), and thread t reads or writes its SC_PER
-// consecutive elements in LDS. The padded index (one slot per SC_PER elements) keeps those
-// per-thread accesses on distinct banks.
+// coalesced, and thread t reads or writes its SC_PER consecutive elements in LDS. The padded
+// index (one slot per SC_PER elements) keeps those per-thread accesses on distinct banks.
 __device__ __forceinline__ int sc_pad(int i) { return i + i / SC_PER; }
 constexpr int SC_LDS = SC_TILE + SC_TILE / SC_PER;
 
-// Loads tile `tr` of row r into buf (identity past the row's end); thread t then owns
-// buf[sc_pad(t * SC_PER + j)].
-template <typename T, int OP, int XF>
-__device__ __forceinline__ void sc_stage_in(const T *x, size_t r, size_t len, size_t tr, T *buf) {
+// The global side of a tile, held in registers between its loads and its LDS writes, so the next
+// tile's loads are in flight while this one is scanned. VEC (operands 16-byte aligned, rows a
+// whole number of vectors): thread t holds 16-byte vectors j RB + t of the tile; else elements
+// j RB + t. Elements past the row's end hold the identity.
+template <typename T, bool VEC>
+struct ScRegs {
+    static constexpr int VN = VEC ? Vec16<T>::N : 1;
+    static constexpr int NL = SC_PER / VN;  // loads per thread and tile
+    T v[SC_PER];
+};
+
+template <typename T, int OP, int XF, bool VEC>
+__device__ __forceinline__ void sc_load(const T *x, size_t r, size_t len, size_t tr, ScRegs<T, VEC> &g) {
+    constexpr int VN = ScRegs<T, VEC>::VN, NL = ScRegs<T, VEC>::NL;
+    const size_t c0 = tr * SC_TILE;
 #pragma unroll
-    for (int j = 0; j < SC_PER; ++j) {
-        const int i = j * RB + threadIdx.x;
-        const size_t col = tr * SC_TILE + i;
-        buf[sc_pad(i)] = col < len ? acc_load<T, OP, XF>(x, r * len + col, col) : acc_identity<T, OP>();
+    for (int j = 0; j < NL; ++j) {
+        const size_t col = c0 + ((size_t)j * RB + threadIdx.x) * VN;  // first element of this load
+        if constexpr (VEC) {
+            typedef typename Vec16<T>::type V;
+            if (col + VN <= len) {
+                const V w = __builtin_nontemporal_load(reinterpret_cast<const V *>(x + r * len + col));
+#pragma unroll
+                for (int e = 0; e < VN; ++e) g.v[j * VN + e] = w[e];
+            } else {
+#pragma unroll
+                for (int e = 0; e < VN; ++e) g.v[j * VN + e] = (T)0;  // past the end (rows are whole vectors)
+            }
+        } else {
+            g.v[j] = col < len ? x[r * len + col] : (T)0;
+        }
     }
-    __syncthreads();
+}
+
+// registers -> LDS, with the element as the scan sees it (SUB negates and DIV inverts every element
+// after a row's first; identity past the row's end)
+template <typename T, int OP, int XF, bool VEC>
+__device__ __forceinline__ void sc_put(const ScRegs<T, VEC> &g, size_t len, size_t tr, T *buf) {
+    constexpr int VN = ScRegs<T, VEC>::VN, NL = ScRegs<T, VEC>::NL;
+    const size_t c0 = tr * SC_TILE;
+#pragma unroll
+    for (int j = 0; j < NL; ++j)
+#pragma unroll
+        for (int e = 0; e < VN; ++e) {
+            const int i = (j * RB + (int)threadIdx.x) * VN + e;  // position in the tile
+            const size_t col = c0 + i;
+            T v = g.v[j * VN + e];
+            if constexpr (XF == 1) v = col == 0 ? v : -v;
+            else if constexpr (XF == 2) v = col == 0 ? v : rn_div((T)1, v);
+            else if constexpr (OP == 0) v = rn_add(v, (T)0);  // -0 -> +0: cumsum's running sum starts at +0
+            buf[sc_pad(i)] = col < len ? v : acc_identity<T, OP>();
+        }
 }
 
 // K1: one aggregate per tile (thread folds its SC_PER consecutive elements, then the block scan's
 // last value).
-template <typename T, int OP, int XF>
+template <typename T, int OP, int XF, bool VEC>
 __global__ __launch_bounds__(RB) void scan_tile_agg_kernel(const T *x, size_t rows, size_t len, size_t tpr, T *agg) {
     __shared__ T lds[WAVES];
     __shared__ T buf[SC_LDS];
     const size_t tiles = rows * tpr;
+    ScRegs<T, VEC> g;
+    if (blockIdx.x < tiles) sc_load<T, OP, XF, VEC>(x, blockIdx.x / tpr, len, blockIdx.x % tpr, g);
     for (size_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
         const size_t r = tile / tpr, tr = tile - r * tpr;
-        sc_stage_in<T, OP, XF>(x, r, len, tr, buf);
+        sc_put<T, OP, XF, VEC>(g, len, tr, buf);
+        lds_barrier();
+        const size_t nt = tile + gridDim.x;  // the next tile's loads, in flight while this one folds
+        if (nt < tiles) sc_load<T, OP, XF, VEC>(x, nt / tpr, len, nt % tpr, g);
         T a = acc_identity<T, OP>();
 #pragma unroll
         for (int j = 0; j < SC_PER; ++j) a = acc_apply<T, OP>(a, buf[sc_pad(threadIdx.x * SC_PER + j)]);
@@ -1176,15 +1212,21 @@ __global__ __launch_bounds__(RB) void scan_tile_agg_kernel(const T *x, size_t ro
 
 // K3: each tile rescanned: y = carry (x) thread prefix (x) in-thread prefix, where carry is the
 // inclusive scan of the previous tiles' aggregates of the row (none for a row's first tile).
-template <typename T, int OP, int XF>
+template <typename T, int OP, int XF, bool VEC>
 __global__ __launch_bounds__(RB) void scan_tile_kernel(const T *x, T *y, size_t rows, size_t len, size_t tpr,
                                                        const T *agg_scan) {
+    constexpr int VN = ScRegs<T, VEC>::VN, NL = ScRegs<T, VEC>::NL;
     __shared__ T lds[WAVES];
     __shared__ T buf[SC_LDS];
     const size_t tiles = rows * tpr;
+    ScRegs<T, VEC> g;
+    if (blockIdx.x < tiles) sc_load<T, OP, XF, VEC>(x, blockIdx.x / tpr, len, blockIdx.x % tpr, g);
     for (size_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
         const size_t r = tile / tpr, tr = tile - r * tpr;
-        sc_stage_in<T, OP, XF>(x, r, len, tr, buf);
+        sc_put<T, OP, XF, VEC>(g, len, tr, buf);
+        lds_barrier();
+        const size_t nt = tile + gridDim.x;  // the next tile's loads, in flight while this one scans
+        if (nt < tiles) sc_load<T, OP, XF, VEC>(x, nt / tpr, len, nt % tpr, g);
         T v[SC_PER];
 #pragma unroll
         for (int j = 0; j < SC_PER; ++j) v[j] = buf[sc_pad(threadIdx.x * SC_PER + j)];
@@ -1210,14 +1252,25 @@ __global__ __launch_bounds__(RB) void scan_tile_kernel(const T *x, T *y, size_t 
             if (has_carry) o = acc_apply<T, OP>(carry, o);
             buf[sc_pad(threadIdx.x * SC_PER + j)] = o;
         }
-        __syncthreads();
+        lds_barrier();
+        const size_t c0 = tr * SC_TILE;
 #pragma unroll
-        for (int j = 0; j < SC_PER; ++j) {
-            const int i = j * RB + threadIdx.x;
-            const size_t col = tr * SC_TILE + i;
-            if (col < len) y[r * len + col] = buf[sc_pad(i)];
+        for (int j = 0; j < NL; ++j) {
+            const int i0 = (j * RB + (int)threadIdx.x) * VN;
+            const size_t col = c0 + i0;
+            if constexpr (VEC) {
+                typedef typename Vec16<T>::type V;
+                if (col + VN <= len) {
+                    V w;
+#pragma unroll
+                    for (int e = 0; e < VN; ++e) w[e] = buf[sc_pad(i0 + e)];
+                    __builtin_nontemporal_store(w, reinterpret_cast<V *>(y + r * len + col));
+                }
+            } else {
+                if (col < len) y[r * len + col] = buf[sc_pad(i0)];
+            }
         }
-        __syncthreads();  // lds and buf are reused by the next tile
+        lds_barrier();  // lds and buf are reused by the next tile
     }
 }
 
@@ -1226,18 +1279,21 @@ static nrk_status scan_rows(const T *x, T *y, size_t rows, size_t len, T *ws, hi
     const size_t tpr = ceil_div(len, (size_t)SC_TILE);
     const size_t tiles = rows * tpr;
     const int grid = (int)std::min<size_t>(tiles, (size_t)cu_count() * 16);
+    // 16-byte loads and stores when every row starts on a 16-byte boundary (the summation tree is
+    // the same either way)
+    const bool vec = ((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(y)) & 15) == 0 && len % Vec16<T>::N == 0;
+    const T *agg = nullptr;
     if (tpr > 1) {
-        T *agg = ws;
-        hipLaunchKernelGGL((scan_tile_agg_kernel<T, OP, XF>), dim3(grid), dim3(RB), 0, stream, x, rows, len, tpr, agg);
+        T *a = ws;
+        if (vec) hipLaunchKernelGGL((scan_tile_agg_kernel<T, OP, XF, true>), dim3(grid), dim3(RB), 0, stream, x, rows, len, tpr, a);
+        else hipLaunchKernelGGL((scan_tile_agg_kernel<T, OP, XF, false>), dim3(grid), dim3(RB), 0, stream, x, rows, len, tpr, a);
         NRK_HIP(hipGetLastError());
         // the aggregates' own rows (rows x tpr), scanned in place; their transform is applied already
-        NRK_TRY((scan_rows<T, OP, 0>(agg, agg, rows, tpr, ws + tiles, stream)));
-        hipLaunchKernelGGL((scan_tile_kernel<T, OP, XF>), dim3(grid), dim3(RB), 0, stream, x, y, rows, len, tpr,
-                           (const T *)agg);
-    } else {
-        hipLaunchKernelGGL((scan_tile_kernel<T, OP, XF>), dim3(grid), dim3(RB), 0, stream, x, y, rows, len, tpr,
-                           (const T *)nullptr);
+        NRK_TRY((scan_rows<T, OP, 0>(a, a, rows, tpr, ws + tiles, stream)));
+        agg = a;
     }
+    if (vec) hipLaunchKernelGGL((scan_tile_kernel<T, OP, XF, true>), dim3(grid), dim3(RB), 0, stream, x, y, rows, len, tpr, agg);
+    else hipLaunchKernelGGL((scan_tile_kernel<T, OP, XF, false>), dim3(grid), dim3(RB), 0, stream, x, y, rows, len, tpr, agg);
     NRK_HIP(hipGetLastError());
     return NRK_OK;
 }
