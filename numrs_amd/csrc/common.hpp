@@ -207,10 +207,10 @@ __device__ __forceinline__ void apply_step_n(int op, double sd, T (&v)[N]) {
                         NRK_CHAIN_CASE(NRK_SOP_EXP, fm::exp_t(x))
                         NRK_CHAIN_CASE(NRK_SOP_EXP2, via_f64(x, [](double d) { return fm::exp2(d); }))
                         NRK_CHAIN_CASE(NRK_SOP_EXPM1, via_f64(x, [](double d) { return fm::expm1(d); }))
-                        NRK_CHAIN_CASE(NRK_SOP_LOG, via_f64(x, [](double d) { return fm::log(d); }))
-                        NRK_CHAIN_CASE(NRK_SOP_LOG2, via_f64(x, [](double d) { return fm::log2(d); }))
-                        NRK_CHAIN_CASE(NRK_SOP_LOG10, via_f64(x, [](double d) { return fm::log10(d); }))
-                        NRK_CHAIN_CASE(NRK_SOP_LOG1P, via_f64(x, [](double d) { return fm::log1p(d); }))
+                        NRK_CHAIN_CASE(NRK_SOP_LOG, fm::log_t(x))
+                        NRK_CHAIN_CASE(NRK_SOP_LOG2, fm::log2_t(x))
+                        NRK_CHAIN_CASE(NRK_SOP_LOG10, fm::log10_t(x))
+                        NRK_CHAIN_CASE(NRK_SOP_LOG1P, fm::log1p_t(x))
                         NRK_CHAIN_CASE(NRK_SOP_TANH, fm::tanh_t(x))
                         NRK_CHAIN_CASE(NRK_SOP_SINH, via_f64(x, [](double d) { return fm::sinh(d); }))
                         NRK_CHAIN_CASE(NRK_SOP_COSH, via_f64(x, [](double d) { return fm::cosh(d); }))
@@ -220,8 +220,8 @@ __device__ __forceinline__ void apply_step_n(int op, double sd, T (&v)[N]) {
                         NRK_CHAIN_CASE(NRK_SOP_ATAN, via_f64(x, [](double d) { return fm::atan(d); }))
                         NRK_CHAIN_CASE(NRK_SOP_ASIN, via_f64(x, [](double d) { return fm::asin(d); }))
                         NRK_CHAIN_CASE(NRK_SOP_ACOS, via_f64(x, [](double d) { return fm::acos(d); }))
-                        NRK_CHAIN_CASE(NRK_SOP_SIN, via_f64(x, [](double d) { return fm::sin(d); }))
-                        NRK_CHAIN_CASE(NRK_SOP_COS, via_f64(x, [](double d) { return fm::cos(d); }))
+                        NRK_CHAIN_CASE(NRK_SOP_SIN, fm::sin_t(x))
+                        NRK_CHAIN_CASE(NRK_SOP_COS, fm::cos_t(x))
                         NRK_CHAIN_CASE(NRK_SOP_TAN, via_f64(x, [](double d) { return fm::tan(d); }))
                         NRK_CHAIN_CASE(NRK_SOP_RELU, via_f64(x, [](double d) { return fm::relu(d); }))
                         NRK_CHAIN_CASE(NRK_SOP_LEAKY_RELU, (T)((double)x > 0.0 ? (double)x : sd * (double)x))

@@ -1116,16 +1116,107 @@ __device__ inline float pow(float xf, float yf) {
     return (float)__builtin_ldexp(exp_taylor<10>((t - k) * 0x1.62e42fefa39efp-1), (int)k);
 }
 
+// Logarithms of binary32 values: log2_pos (relative error below 2^-40 of the log2 of the mantissa,
+// and the exponent added exactly) scaled once by a binary64 constant, then rounded to binary32 --
+// within 0.5 ulp + 2^-15 ulp. log1p forms 1 + x exactly in binary64 (x has 24 significant bits);
+// |x| < 2^-25 returns x (log1p(x) = x (1 - x/2 + ...) is within half an ulp of it).
+__device__ __forceinline__ bool log_special(float xf, float &r) {
+    if (xf > 0.0f && xf < __builtin_inff()) return false;
+    r = xf != xf ? xf : (xf == 0.0f ? -__builtin_inff() : (xf < 0.0f ? __builtin_nanf("") : xf));
+    return true;
+}
+__device__ inline float log(float xf) {
+    float r;
+    if (log_special(xf, r)) return r;
+    return (float)(log2_pos((double)xf) * 0x1.62e42fefa39efp-1);  // ln 2
+}
+__device__ inline float log2(float xf) {
+    float r;
+    if (log_special(xf, r)) return r;
+    return (float)log2_pos((double)xf);
+}
+__device__ inline float log10(float xf) {
+    float r;
+    if (log_special(xf, r)) return r;
+    return (float)(log2_pos((double)xf) * 0x1.34413509f79ffp-2);  // log10 2
+}
+__device__ inline float log1p(float xf) {
+    if (xf != xf || xf == __builtin_inff()) return xf;
+    if (xf == -1.0f) return -__builtin_inff();
+    if (xf < -1.0f) return __builtin_nanf("");
+    if (__builtin_fabsf(xf) < 0x1p-25f) return xf;
+    return (float)(log2_pos(1.0 + (double)xf) * 0x1.62e42fefa39efp-1);
+}
+
+// sin and cos of binary32 values with |x| < 2^20: k = rint(|x| 2/pi), r = |x| - k pi/2 from the
+// two leading parts of pi/2 (the product with the first exact, the second rounded once: |r| is at
+// least ~2^-30 for binary32 inputs there, so r keeps 2^-36 relative), then Taylor series to r^11
+// and r^12 on |r| <= pi/4 (first omitted terms below 6e-12 and 5e-13). Larger, infinite and NaN
+// arguments take the binary64 routine.
+__device__ __forceinline__ double sin_r11(double r) {
+    const double z = r * r;
+    double q = -InvFact<11>::v;
+    q = __builtin_fma(q, z, InvFact<9>::v);
+    q = __builtin_fma(q, z, -InvFact<7>::v);
+    q = __builtin_fma(q, z, InvFact<5>::v);
+    q = __builtin_fma(q, z, -InvFact<3>::v);
+    return __builtin_fma(r * z, q, r);
+}
+__device__ __forceinline__ double cos_r12(double r) {
+    const double z = r * r;
+    double q = InvFact<12>::v;
+    q = __builtin_fma(q, z, -InvFact<10>::v);
+    q = __builtin_fma(q, z, InvFact<8>::v);
+    q = __builtin_fma(q, z, -InvFact<6>::v);
+    q = __builtin_fma(q, z, InvFact<4>::v);
+    q = __builtin_fma(q, z, -0.5);
+    return __builtin_fma(z, q, 1.0);
+}
+__device__ __forceinline__ double reduce_f32(double ax, int &q) {
+    const double k = __builtin_rint(ax * 0x1.45f306dc9c883p-1);  // 2/pi
+    q = (int)k & 3;
+    return __builtin_fma(-k, 0x1.1a62633145c07p-54, __builtin_fma(-k, 0x1.921fb54442d18p+0, ax));
+}
+__device__ inline float sin(float xf) {
+    const double x = (double)xf, ax = __builtin_fabs(x);
+    if (!(ax < 0x1p20)) return (float)nrk::fm::sin(x);
+    int q;
+    const double r = reduce_f32(ax, q);
+    const double v = (q & 1) ? cos_r12(r) : sin_r11(r);
+    return (float)(__builtin_copysign(1.0, x) * ((q & 2) ? -v : v));  // sin(-x) = -sin(x)
+}
+__device__ inline float cos(float xf) {
+    const double x = (double)xf, ax = __builtin_fabs(x);
+    if (!(ax < 0x1p20)) return (float)nrk::fm::cos(x);
+    int q;
+    const double r = reduce_f32(ax, q);
+    const double v = (q & 1) ? sin_r11(r) : cos_r12(r);
+    return (float)((q == 1 || q == 2) ? -v : v);
+}
+
 }  // namespace f32
 
 // The elementary functions by argument type: binary64 at full precision, binary32 through the
-// routines above. Every kernel of the library that evaluates exp, tanh or pow of a T calls these.
+// routines above. Every map kernel of the library that evaluates exp, tanh, pow, a logarithm, sin
+// or cos of a T calls these.
 __device__ __forceinline__ double exp_t(double x) { return exp(x); }
 __device__ __forceinline__ float exp_t(float x) { return f32::exp(x); }
 __device__ __forceinline__ double tanh_t(double x) { return tanh(x); }
 __device__ __forceinline__ float tanh_t(float x) { return f32::tanh(x); }
 __device__ __forceinline__ double pow_t(double x, double y) { return pow(x, y); }
 __device__ __forceinline__ float pow_t(float x, float y) { return f32::pow(x, y); }
+__device__ __forceinline__ double log_t(double x) { return log(x); }
+__device__ __forceinline__ float log_t(float x) { return f32::log(x); }
+__device__ __forceinline__ double log2_t(double x) { return log2(x); }
+__device__ __forceinline__ float log2_t(float x) { return f32::log2(x); }
+__device__ __forceinline__ double log10_t(double x) { return log10(x); }
+__device__ __forceinline__ float log10_t(float x) { return f32::log10(x); }
+__device__ __forceinline__ double log1p_t(double x) { return log1p(x); }
+__device__ __forceinline__ float log1p_t(float x) { return f32::log1p(x); }
+__device__ __forceinline__ double sin_t(double x) { return sin(x); }
+__device__ __forceinline__ float sin_t(float x) { return f32::sin(x); }
+__device__ __forceinline__ double cos_t(double x) { return cos(x); }
+__device__ __forceinline__ float cos_t(float x) { return f32::cos(x); }
 
 }  // namespace fm
 }  // namespace nrk
