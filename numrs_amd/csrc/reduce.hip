@@ -681,10 +681,16 @@ constexpr int I64_BLOCKS_PER_CU = 2;
 
 // Each workgroup adds all of its segments (integer adds are exact, so the grouping cannot change
 // the result) and writes one partial.
+// The i64 sum: a persistent grid walks 32 KiB segments (non-temporal 16-byte loads, wrapping
+// unsigned adds: exact and order-independent, so any grouping gives the same bits); one partial
+// per workgroup, stored agent-scope and counted with one agent-scope atomic; the last workgroup to
+// arrive adds the partials and writes the result (no second launch).
+constexpr int I64_MAX_GROUPS = 4 * RB;
 __global__ __launch_bounds__(RB) void sum_i64_kernel(const int64_t *x, size_t n, size_t nseg, int vec_ok,
-                                                     unsigned long long *partials) {
+                                                     unsigned long long *partials, unsigned *counter, int64_t *out) {
     typedef long long V2 __attribute__((ext_vector_type(2)));
     __shared__ unsigned long long sred[WAVES];
+    __shared__ unsigned s_last;
     const int t = threadIdx.x;
     unsigned long long acc = 0;  // unsigned: wrapping is defined, bits equal two's complement i64
     for (size_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
@@ -706,27 +712,33 @@ __global__ __launch_bounds__(RB) void sum_i64_kernel(const int64_t *x, size_t n,
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) acc += shfl_xor((uint64_t)acc, m);
     if ((t & 63) == 0) sred[t >> 6] = acc;
-    __syncthreads();
+    lds_barrier();
     if (t == 0) {
-        unsigned long long s = 0;
-        for (int k = 0; k < WAVES; ++k) s += sred[k];
-        partials[blockIdx.x] = s;
+        unsigned long long g = 0;
+        for (int k = 0; k < WAVES; ++k) g += sred[k];
+        st_agent(partials + blockIdx.x, g);
+        NRK_WAIT_VMEM0();  // the partial has reached the point of coherence before it is counted
+        s_last = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1;
     }
-}
-
-__global__ __launch_bounds__(FB) void sum_u64_finish_kernel(const unsigned long long *partials, size_t count, int64_t *out) {
-    __shared__ unsigned long long sred[FB / 64];
-    const int t = threadIdx.x;
-    unsigned long long s = 0;
-    for (size_t k = t; k < count; k += FB) s += partials[k];
+    lds_barrier();
+    if (!s_last) return;  // uniform over the workgroup
+    unsigned long long v[4];
 #pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) s += shfl_xor((uint64_t)s, m);
-    if ((t & 63) == 0) sred[t >> 6] = s;
-    __syncthreads();
+    for (int q = 0; q < 4; ++q) {
+        const size_t i = t + (size_t)q * RB;
+        v[q] = ld_agent(partials + min(i, (size_t)gridDim.x - 1));
+        if (i >= gridDim.x) v[q] = 0;
+    }
+    unsigned long long tot = (v[0] + v[1]) + (v[2] + v[3]);
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) tot += shfl_xor((uint64_t)tot, m);
+    if ((t & 63) == 0) sred[t >> 6] = tot;
+    lds_barrier();
     if (t == 0) {
-        unsigned long long tot = 0;
-        for (int k = 0; k < FB / 64; ++k) tot += sred[k];
-        *out = (int64_t)tot;
+        unsigned long long r = 0;
+        for (int k = 0; k < WAVES; ++k) r += sred[k];
+        *out = (int64_t)r;
+        st_agent(counter, 0u);  // the zero-at-launch invariant for the next launch on this stream
     }
 }
 
@@ -741,14 +753,12 @@ nrk_status sum_i64(const int64_t *x, size_t n, int64_t *out, hipStream_t stream)
     }
     if (!x) return fail(NRK_ERR_INVALID_ARGUMENT, "x is NULL");
     const size_t nseg = ceil_div(n, I64_SEG);
-    const int grid = (int)std::min<size_t>(nseg, (size_t)cu_count() * I64_BLOCKS_PER_CU);
+    const int grid = (int)std::min<size_t>({nseg, (size_t)cu_count() * I64_BLOCKS_PER_CU, (size_t)I64_MAX_GROUPS});
     Scratch s;
-    NRK_TRY(get_scratch(stream, 0, (size_t)grid * 8, &s));
+    NRK_TRY(get_scratch(stream, 1, (size_t)grid * 8, &s));
     const int vec_ok = (reinterpret_cast<uintptr_t>(x) % 16) == 0;
-    hipLaunchKernelGGL(sum_i64_kernel, dim3(grid), dim3(RB), 0, stream, x, n, nseg, vec_ok, (unsigned long long *)s.partials);
-    NRK_HIP(hipGetLastError());
-    hipLaunchKernelGGL(sum_u64_finish_kernel, dim3(1), dim3(FB), 0, stream, (const unsigned long long *)s.partials, (size_t)grid,
-                       out);
+    hipLaunchKernelGGL(sum_i64_kernel, dim3(grid), dim3(RB), 0, stream, x, n, nseg, vec_ok, (unsigned long long *)s.partials,
+                       s.counters, out);
     NRK_HIP(hipGetLastError());
     return NRK_OK;
 }
