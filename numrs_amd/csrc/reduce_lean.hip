@@ -52,6 +52,10 @@ struct LeanParams {
     OutPtrs<T> row;
     int need_group;   // whole-array results wanted: store one record per workgroup
     int uniform;      // every wavefront reduces the same number of chunks (equal record counts)
+    // whole-array mode of any length: elements [tail0, n_total) after the last whole chunk are
+    // folded by one more workgroup, tail_group (none: ~0u), whose record joins the finish
+    size_t tail0, n_total;
+    unsigned tail_group;
     Parts grp;
     OutPtrs<T> full;         // whole-array results (need_group)
     unsigned int *counter;   // zero at launch; the last workgroup to finish resets it
@@ -318,6 +322,58 @@ struct RowRec {
     unsigned first_nan;
 };
 
+// The tail of a whole-array reduction (what follows the last whole chunk: fewer than a chunk and
+// a segment of elements): lane l of wavefront w folds elements tail0 + 64 w + l, + RB, ... in
+// index order as one-element records, then the wavefront's butterfly; wp receives the result.
+// Loads go out LEAN_TAIL_BATCH at a time (values past the end are loaded as zeros and skipped).
+constexpr int LEAN_TAIL_BATCH = 16;
+template <typename T, int FLAGS, int LEAVES, int CH>
+__device__ void lean_tail(const LeanParams<T> &p, unsigned wave, unsigned lane, Part &wp) {
+    constexpr int NB = LEAN_TAIL_BATCH;
+    Part a = part_empty();
+    const T s0 = (T)p.s0, s1 = (T)p.s1;
+    for (size_t i0 = p.tail0 + wave * 64 + lane; i0 < p.n_total; i0 += (size_t)NB * RB) {
+        T y[NB], z[LEAVES == 2 ? NB : 1];
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            const size_t i = i0 + (size_t)k * RB;
+            y[k] = i < p.n_total ? p.x0[i] : (T)0;
+            if constexpr (LEAVES == 2) z[k] = i < p.n_total ? p.x1[i] : (T)0;
+        }
+        if constexpr (CH == CH_MULADD) {
+#pragma unroll
+            for (int k = 0; k < NB; ++k) y[k] = rn_add(rn_mul(y[k], s0), s1);
+        } else if constexpr (CH == CH_ANY) {
+            apply_chain_n(p.prog.pre0, y);
+        }
+        if constexpr (LEAVES == 2) {
+            apply_chain_n(p.prog.pre1, z);
+            bop_n(p.prog.op, y, z);
+            apply_chain_n(p.prog.post, y);
+        }
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            const size_t i = i0 + (size_t)k * RB;
+            if (i >= p.n_total) break;
+            const T v = y[k];
+            Part e;
+            e.n = 1.0;
+            e.hi = (double)v;
+            e.lo = 0.0;
+            e.m2 = 0.0;
+            e.nan = v != v ? 1u : 0u;
+            const bool up = v > -dev_inf<T>(), dn = v < dev_inf<T>();  // both false for NaN
+            e.vmax = up ? (double)v : -__builtin_inf();
+            e.imax = up ? (uint64_t)i : NO_INDEX;  // -inf and NaN never win
+            e.vmin = dn ? (double)v : __builtin_inf();
+            e.imin = dn ? (uint64_t)i : NO_INDEX;
+            part_merge<FLAGS>(a, e);
+        }
+    }
+    wave_merge<FLAGS>(a);
+    wp = a;
+}
+
 template <typename T, int VPT, int FLAGS, int LEAVES, bool VEC, int CH>
 __global__ __launch_bounds__(RB) void lean_kernel(LeanParams<T> p) {
     constexpr int VN = Vec16<T>::N;
@@ -332,7 +388,8 @@ __global__ __launch_bounds__(RB) void lean_kernel(LeanParams<T> p) {
     const unsigned wave = lane0((unsigned)(threadIdx.x >> 6));
     const size_t v = blockIdx.x * (size_t)WAVES + wave;
     const size_t c0 = v * p.A;  // chunks c0, c0 + B, ...
-    const size_t nck = c0 >= p.nchunk ? 0 : (p.A == 1 ? (p.nchunk - c0 + p.B - 1) / p.B : min(p.A, p.nchunk - c0));
+    const bool is_tail = blockIdx.x == p.tail_group;  // uniform over the workgroup
+    const size_t nck = (is_tail || c0 >= p.nchunk) ? 0 : (p.A == 1 ? (p.nchunk - c0 + p.B - 1) / p.B : min(p.A, p.nchunk - c0));
     const size_t nseg = nck * p.cl;
     const size_t cstride = p.B * p.cl * WSEG;  // elements from one of this wavefront's chunks to the next
     Part wp = part_empty();  // this wavefront's chunks, merged in chunk order
@@ -421,6 +478,7 @@ __global__ __launch_bounds__(RB) void lean_kernel(LeanParams<T> p) {
         }
         if (nseg & 1) step(y[0], z[0]);
     }
+    if (is_tail) lean_tail<T, FLAGS, LEAVES, CH>(p, wave, lane, wp);
     if (!p.need_group) {  // uniform over the grid
         if (p.row_direct) flush_rows();
         return;
@@ -536,14 +594,14 @@ struct LeanLayout {
     size_t groups, A, B;
 };
 template <typename T>
-static LeanLayout lean_layout(size_t nchunk) {
+static LeanLayout lean_layout(size_t nchunk, size_t max_groups) {
     LeanLayout L;
     if (sizeof(T) == 4) {
         L.groups = std::min<size_t>(LEAN_GROUPS_F32, ceil_div(nchunk, (size_t)4 * WAVES));
         L.A = 1;
         L.B = L.groups * WAVES;
     } else {
-        L.A = std::max<size_t>(LEAN_BLOCK_F64, ceil_div(nchunk, LEAN_MAX_GROUPS * WAVES));
+        L.A = std::max<size_t>(LEAN_BLOCK_F64, ceil_div(nchunk, max_groups * WAVES));
         L.groups = ceil_div(nchunk, L.A * WAVES);
         L.B = 1;
     }
@@ -590,17 +648,22 @@ nrk_status try_lean(const LeanProg &prog, const T *x0, const T *x1, size_t rows,
     *taken = false;
     // the choice depends on the shape only (never on alignment or the device), so a given shape
     // always gets the same summation tree
-    size_t cl, nchunk;
+    size_t cl, nchunk, tail0 = rows * cols;
     if (want_row) {  // a chunk is a row
         if (cols % WSEG != 0 || cols / WSEG >= (size_t)LEAN_WTAB || rows < LEAN_MIN_CHUNKS) return NRK_OK;
         cl = cols / WSEG;
         nchunk = rows;
     } else {
-        const size_t n = rows * cols;
-        if (n % WSEG != 0) return NRK_OK;
-        cl = lean_chunk_len(n / WSEG);
-        if (cl == 0) return NRK_OK;
-        nchunk = n / WSEG / cl;
+        // whole segments in chunks of an exact divisor when there is one; else (any length)
+        // chunks of ceil(segments / CHUNK_TARGET) segments and the rest as a tail
+        const size_t n = rows * cols, spr = n / WSEG;
+        cl = n % WSEG == 0 ? lean_chunk_len(spr) : 0;
+        if (cl == 0) {
+            cl = std::max<size_t>(1, ceil_div(spr, CHUNK_TARGET));
+            if (cl >= (size_t)LEAN_WTAB) return NRK_OK;
+            tail0 = spr / cl * cl * WSEG;
+        }
+        nchunk = spr / cl;
         if (nchunk < LEAN_MIN_CHUNKS) return NRK_OK;
     }
     LeanParams<T> p{};
@@ -609,13 +672,17 @@ nrk_status try_lean(const LeanProg &prog, const T *x0, const T *x1, size_t rows,
     p.nchunk = nchunk;
     p.cl = cl;
     p.cols = cols;
-    const LeanLayout L = lean_layout<T>(nchunk);
-    const size_t ngroups = L.groups;
+    const bool tail = tail0 < rows * cols;
+    const LeanLayout L = lean_layout<T>(nchunk, LEAN_MAX_GROUPS - (tail ? 1 : 0));
+    const size_t ngroups = L.groups + (tail ? 1 : 0);
     p.A = L.A;
     p.B = L.B;
+    p.tail0 = tail0;
+    p.n_total = rows * cols;
+    p.tail_group = tail ? (unsigned)L.groups : ~0u;
     // equal chunk counts on every wavefront of the grid: round robin over a multiple of the
     // wavefront count, or contiguous blocks that fill every wavefront
-    p.uniform = L.A == 1 ? nchunk % (L.groups * WAVES) == 0 : nchunk == L.A * L.groups * WAVES;
+    p.uniform = !tail && (L.A == 1 ? nchunk % (L.groups * WAVES) == 0 : nchunk == L.A * L.groups * WAVES);
     p.prog = prog;
     if (!x1 && prog.pre0.len == 2) {  // CH_MULADD: y = (x * s0) + s1, with x - b as x + (-b)
         p.s0 = prog.pre0.s[0];

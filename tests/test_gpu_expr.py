@@ -112,7 +112,7 @@ def run_expr_reduce(nodes, leaves, rows, cols, ddof=0, want_rows=True):
 
 
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
-@pytest.mark.parametrize("shape", [(2048, 1024), (1024, 4096), (37, 1000), (1, 1 << 20)])
+@pytest.mark.parametrize("shape", [(2048, 1024), (1024, 4096), (37, 1000), (1, 1 << 20), (1, (1 << 20) + 333)])
 def test_two_operand_tree_feeds_the_reduction(gpu, oracle, dt, shape):
     """y = (a * 1.5) - (b + 0.25), reduced along rows and over the whole array in one pass over a
     and b: every output bit-identical to the fused reduction of the materialised y (the same

@@ -141,3 +141,45 @@ def test_lean_path_alignment_nan_and_ties(gpu, oracle, dtype, shape):
         np.testing.assert_array_equal(rm[ok], om[ok])
         np.testing.assert_array_equal(np.signbit(rm[ok]), np.signbit(om[ok]))
         assert got["row_argmax"][3] == 0 and got["row_argmax"][9] == 100
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("shape", [(1, (1 << 21) + 777), (1, 1100 * 1024 + 5), (1, (1 << 24) + 3 * 1024 + 17),
+                                   (4097, 4099)])
+def test_lean_whole_array_with_tail(gpu, oracle, dtype, shape):
+    """Whole-array reductions of any length: whole chunks on the lean kernel plus one workgroup for
+    the elements after the last whole chunk. Against the oracle (extrema and indices bit-exact,
+    sums and variances within the reduction bars); a pointer off 16-byte alignment and a repeated
+    call give the same bits; a NaN or a tied maximum placed in the tail follows the sequential
+    rules (NaN poisons max and is skipped by argmax; the first of equal maxima wins)."""
+    rows, cols = shape
+    n = rows * cols
+    x = rng.normal(rng.BASE_SEED + 1600 + cols, n, dtype).reshape(rows, cols)
+    steps = [("mul", 1.5), ("add", -0.25)]
+    a = run_fused(x, steps, want_rows=False)
+    b = run_fused(x, steps, want_rows=False, offset_elems=1)
+    c = run_fused(x, steps, want_rows=False)
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+        np.testing.assert_array_equal(a[k], c[k], err_msg=k)
+    y = oracle.ref_map([2, 0], [1.5, -0.25], x).reshape(rows, cols)
+    for op in ("max", "min", "argmax", "argmin"):
+        assert a["full_" + op][0] == oracle.ref(op, y), op
+    ex = oracle.dd_stats(y, 0)
+    e = EPS[dtype]
+    s_abs = float(np.abs(y.astype(np.float64)).sum())
+    assert abs(a["full_sum"][0] - ex["sum"]) <= 8 * e * s_abs + e * abs(ex["sum"])
+    assert abs(a["full_var"][0] - ex["var"]) <= 64 * e * ex["var"]
+    assert abs(a["full_mean"][0] - ex["mean"]) <= 8 * e * s_abs / n + 2 * e * abs(ex["mean"])
+    z = x.copy()
+    flat = z.reshape(-1)
+    big = dtype(1000.0)
+    flat[n // 3] = big       # ties: the first index wins, the tied copy sits in the tail
+    flat[n - 2] = big
+    flat[n - 5] = -big       # the minimum in the tail
+    got = run_fused(z, [], want_rows=False)
+    assert got["full_argmax"][0] == oracle.ref("argmax", z) == n // 3
+    assert got["full_argmin"][0] == oracle.ref("argmin", z) == n - 5
+    flat[n - 1] = np.nan     # a NaN in the tail: max poisoned, argmax unchanged
+    got = run_fused(z, [], want_rows=False)
+    assert np.isnan(got["full_max"][0]) and got["full_argmax"][0] == oracle.ref("argmax", z) == n // 3
