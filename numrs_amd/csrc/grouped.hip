@@ -209,8 +209,9 @@ __device__ __forceinline__ void wave_dw(float &hi, float &lo) {
         float s, e;
         two_sum_f(hi, bh, s, e);
         e = __fadd_rn(e, __fadd_rn(lo, bl));
-        hi = __fadd_rn(s, e);
-        lo = __fsub_rn(e, __fsub_rn(hi, s));
+        const float h = __fadd_rn(s, e);  // a sum that is not finite stays the plain sum
+        lo = __builtin_isfinite(h) ? __fsub_rn(e, __fsub_rn(h, s)) : 0.f;
+        hi = __builtin_isfinite(s) ? h : s;
     }
 }
 
@@ -227,7 +228,7 @@ __device__ __forceinline__ void wave_row_sum(const float *x, int64_t len, float 
         float s, e;
         two_sum_f(acc, p, s, e);
         acc = s;
-        comp = __fadd_rn(comp, e);
+        comp = __fadd_rn(comp, __builtin_isfinite(s) ? e : 0.f);  // inf stays inf (its error term is NaN)
     };
     const int64_t head = min(len, (int64_t)(((16u - ((unsigned)(uintptr_t)x & 15u)) & 15u) >> 2));
     if (lane < head) add(x[lane]);

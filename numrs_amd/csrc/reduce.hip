@@ -139,8 +139,7 @@ __device__ __forceinline__ void seg_body(const Chain &chain, T (&y)[VPT * Vec16<
         double s, e;
         two_sum(a.hi, sb, s, e);
         e = e + a.lo;
-        a.hi = s + e;
-        a.lo = e - (a.hi - s);
+        dw_fix(s, e, a.hi, a.lo);
     }
     a.n += cnt;
     // ---- extremum winners: strict comparison in index order (NaN never compares greater). The

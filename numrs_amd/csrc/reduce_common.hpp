@@ -109,14 +109,22 @@ __device__ __forceinline__ void two_sum(T a, T b, T &s, T &e) {
     e = rn_add(rn_sub(a, rn_sub(s, v)), rn_sub(b, v));
 }
 
+// (hi, lo) = s + e renormalised. A sum that is not finite stays the plain sum: hi = s (inf, or NaN
+// for inf - inf; TwoSum's error term is NaN there) or the overflowed s + e, and lo = 0.
+template <typename T>
+__device__ __forceinline__ void dw_fix(T s, T e, T &hi, T &lo) {
+    const T h = rn_add(s, e);
+    hi = __builtin_isfinite(s) ? h : s;
+    lo = __builtin_isfinite(h) ? rn_sub(e, rn_sub(h, s)) : (T)0;
+}
+
 // double-word addition (hi + lo) += (bhi + blo)
 template <typename T>
 __device__ __forceinline__ void dw_add(T &hi, T &lo, T bhi, T blo) {
     T s, e;
     two_sum(hi, bhi, s, e);
     e = rn_add(e, rn_add(lo, blo));
-    hi = rn_add(s, e);
-    lo = rn_sub(e, rn_sub(hi, s));
+    dw_fix(s, e, hi, lo);
 }
 
 __device__ __forceinline__ float shfl_xor(float v, int m) { return __shfl_xor(v, m, 64); }

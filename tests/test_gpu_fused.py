@@ -183,3 +183,33 @@ def test_lean_whole_array_with_tail(gpu, oracle, dtype, shape):
     flat[n - 1] = np.nan     # a NaN in the tail: max poisoned, argmax unchanged
     got = run_fused(z, [], want_rows=False)
     assert np.isnan(got["full_max"][0]) and got["full_argmax"][0] == oracle.ref("argmax", z) == n // 3
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("shape", [(1024, 2048), (16, 8192), (9, 8193), (3000, 100), (1, (1 << 21) + 777)])
+def test_infinite_elements_give_binary64_sums(gpu, dtype, shape):
+    """An infinite element makes its row's and the whole array's sum and mean infinite (not the NaN
+    of a TwoSum error term); +inf and -inf together make NaN; the variance is NaN; on every path
+    (row chunks, split rows, misaligned rows, short rows, a whole array with a tail)."""
+    rows, cols = shape
+    x = rng.normal(rng.BASE_SEED + 2100 + cols, rows * cols, dtype).reshape(rows, cols)
+    x[0, 5] = -np.inf
+    if rows > 2:
+        x[2, 1] = np.inf
+        x[2, cols - 1] = -np.inf
+    want_rows = rows > 1
+    got = run_fused(x, [], want_rows=want_rows)
+    y = x.astype(np.float64)
+    with np.errstate(invalid="ignore"):
+        if want_rows:
+            rs = y.sum(axis=1)
+            np.testing.assert_array_equal(np.isnan(got["row_sum"]), np.isnan(rs))
+            np.testing.assert_array_equal(np.isinf(got["row_sum"]), np.isinf(rs))
+            assert got["row_sum"][0] == -np.inf and got["row_mean"][0] == -np.inf and np.isnan(got["row_var"][0])
+            if rows > 2:
+                assert np.isnan(got["row_sum"][2])
+        fs = y.sum()
+    assert np.isnan(got["full_sum"][0]) == np.isnan(fs)
+    if not np.isnan(fs):
+        assert got["full_sum"][0] == -np.inf and got["full_mean"][0] == -np.inf
+    assert np.isnan(got["full_var"][0])

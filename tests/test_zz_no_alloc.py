@@ -35,7 +35,7 @@ def _run(fn, dx, rows, cols, ddof, st, with_rows=True):
     return {k: v.download() for k, v in keep.items() if with_rows or k.startswith("full_")}
 
 
-@pytest.mark.parametrize("shapes", [[(300, 5000), (40, 777)], [(8192, 1024), (3, 100_003)]])
+@pytest.mark.parametrize("shapes", [[(300, 5000), (40, 777)], [(8192, 1024), (3, 100_003)], [(20_000, 300), (50, 7)]])
 def test_noalloc_matches_allocating_entry_point(gpu, shapes):
     st = device.Stream()
     for i, (rows, cols) in enumerate(shapes):  # the second call is smaller than the first
@@ -57,10 +57,11 @@ def test_noalloc_whole_array_only_and_size_limit(gpu):
     got = _run("nrk_dev_map_reduce_f64_noalloc", dx, 1, n, 0, st, with_rows=False)
     for k in want:
         np.testing.assert_array_equal(got[k].view(np.uint64), want[k].view(np.uint64), err_msg=k)
-    # row results for 100 000 rows need ~27 MB of records: refused, nothing launched
-    big = device.DeviceArray((100_000,), np.float64)
+    # row results for 100 000 rows of 513 (the record tree's path) need ~27 MB of records:
+    # refused, nothing launched
+    big = device.DeviceArray((100_000 * 513,), np.float64)
     with pytest.raises(NrkError) as e:
-        _run("nrk_dev_map_reduce_f64_noalloc", big, 100_000, 1, 0, st)
+        _run("nrk_dev_map_reduce_f64_noalloc", big, 100_000, 513, 0, st)
     assert "fixed scratch" in str(e.value)
     # and the entry point still works after the refusal
     got2 = _run("nrk_dev_map_reduce_f64_noalloc", dx, 1, n, 0, st, with_rows=False)
