@@ -1,0 +1,279 @@
+// The short-row reduction kernel and its launcher, instantiated per element type in
+// reduce_rows_f32.hip and reduce_rows_f64.hip.
+#pragma once
+#include "reduce_lean.hpp"
+
+#include <type_traits>
+
+namespace nrk {
+
+// ---------------- short rows ----------------
+// Rows of at most 64 K elements, K slots per lane (K = 16 or 32 binary32, 8 or 16 binary64: the
+// larger where the smaller would take 32 or more lanes per row): G lanes per row, the smallest
+// power of two with G K >= cols, so a wavefront reduces a block of 64 / G rows at a time and
+// wavefront v of the W in the grid takes blocks v, v + W, ... Lane j of a row's G lanes holds slot
+// s = element ((s / VN) G + j) VN + s % VN of the row (VN-element vectors, the G lanes' side by
+// side), its first nv slots being inside the row. The whole row is in registers: the sum is
+// pairwise per lane (in T), then a butterfly over the row's lanes (double, double-word for
+// binary64); the variance is two-pass, sum((y - m)^2) with m the mean rounded to T, less
+// n (mean - m)^2. Extrema: strict winners per lane in slot (= element) order, then the butterfly
+// with equal values going to the lower element. Whole-array results: every lane folds the records
+// of its rows in turn (equal counts, no division), the wavefront merges its lanes' records once at
+// the end, then the group finish.
+constexpr size_t ROWS_GROUPS = 1024;  // round robin over at most this many workgroups
+
+template <typename T>
+struct RowsParams {
+    const T *x;
+    double s0, s1;  // CH_MULADD's scalars
+    Chain chain;
+    size_t rows, cols, nblk;
+    unsigned G, lg;  // lanes per row and its log2
+    double ddof;
+    OutPtrs<T> row;
+    int need_group, uniform;
+    Parts grp;
+    OutPtrs<T> full;
+    unsigned int *counter;
+};
+
+template <typename T, int K, int FLAGS, bool VEC, int CH>
+__global__ __launch_bounds__(RB) void rows_kernel(RowsParams<T> p) {
+    typedef typename Vec16<T>::type V;
+    constexpr int VN = Vec16<T>::N;
+    constexpr int VPT = K / VN;
+    __shared__ FinishLds fin;
+    const unsigned lane = threadIdx.x & 63;
+    const unsigned wave = lane0((unsigned)(threadIdx.x >> 6));
+    const unsigned G = p.G, R = 64u >> p.lg;
+    const unsigned j = lane & (G - 1), rr = lane >> p.lg;
+    const unsigned cols = (unsigned)p.cols;
+    const size_t W = (size_t)gridDim.x * WAVES;
+    const size_t v = blockIdx.x * (size_t)WAVES + wave;
+    const size_t nit = v < p.nblk ? (p.nblk - 1 - v) / W + 1 : 0;
+    auto off = [&](unsigned s) -> unsigned { return ((s / VN) * G + j) * VN + s % VN; };
+    unsigned nv = 0;  // slots inside the row (a prefix)
+#pragma unroll
+    for (int s = 0; s < K; ++s) nv += off(s) < cols ? 1u : 0u;
+    const bool every = cols == G * (unsigned)K;  // every slot of every lane is inside the row (uniform)
+    const T s0 = (T)p.s0, s1 = (T)p.s1;
+    const double ncols = (double)cols, inv_cols = 1.0 / ncols;
+    Part lp = part_empty();  // this lane's rows so far (whole-array results)
+    size_t lk = 0;
+    unsigned wp_fn = 0;
+    // block b's row for this lane (rows past the end read the last row: a harmless reload)
+    auto load = [&](T(&yy)[K], size_t b) {
+        const size_t r = min(b * R + rr, p.rows - 1);
+        const T *base = p.x + r * p.cols;
+#pragma unroll
+        for (int q = 0; q < VPT; ++q) {
+            const unsigned e0 = (q * G + j) * VN;
+            if constexpr (VEC) {
+                const V t = __builtin_nontemporal_load(reinterpret_cast<const V *>(base + min(e0, cols - VN)));
+#pragma unroll
+                for (int e = 0; e < VN; ++e) yy[q * VN + e] = t[e];
+            } else {
+#pragma unroll
+                for (int e = 0; e < VN; ++e) yy[q * VN + e] = base[min(e0 + e, cols - 1)];
+            }
+        }
+    };
+    // the row's totals from this lane's slots (the first n of them, or all K)
+    struct RowTot {
+        double hi, lo, m2;
+        T vmax, vmin;
+        unsigned imax, imin, nan;
+    };
+    auto row_totals = [&](const T(&y)[K], auto masked) -> RowTot {
+        constexpr bool MASK = decltype(masked)::value;
+        auto in = [&](int i) -> bool { return MASK ? (unsigned)i < nv : true; };
+        RowTot t;
+        const T st = pairwise<K, 0, T>([&](int i) { return in(i) ? y[i] : (T)0; });
+        t.nan = 0;
+        if (__builtin_isnan(st)) {
+#pragma unroll
+            for (int i = 0; i < K; ++i) t.nan |= (in(i) && __builtin_isnan(y[i])) ? 1u : 0u;
+        }
+        t.vmax = -dev_inf<T>();
+        t.vmin = dev_inf<T>();
+        t.imax = NO_CODE;
+        t.imin = NO_CODE;
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            if (FLAGS & F_MAX) {
+                const bool w = in(i) && y[i] > t.vmax;
+                t.vmax = w ? y[i] : t.vmax;
+                t.imax = w ? (unsigned)i : t.imax;
+            }
+            if (FLAGS & F_MIN) {
+                const bool w = in(i) && y[i] < t.vmin;
+                t.vmin = w ? y[i] : t.vmin;
+                t.imin = w ? (unsigned)i : t.imin;
+            }
+        }
+        if (FLAGS & F_MAX) t.imax = t.imax == NO_CODE ? NO_CODE : off(t.imax);
+        if (FLAGS & F_MIN) t.imin = t.imin == NO_CODE ? NO_CODE : off(t.imin);
+        t.hi = (double)st;
+        t.lo = 0.0;
+        for (unsigned m = G >> 1; m >= 1; m >>= 1) {  // uniform: G is the same for every lane
+            if constexpr (sizeof(T) == 8) {
+                const double bh = shfl_xor(t.hi, (int)m), bl = shfl_xor(t.lo, (int)m);
+                dw_add(t.hi, t.lo, bh, bl);
+            } else {
+                t.hi += shfl_xor(t.hi, (int)m);
+            }
+            if (FLAGS & F_MAX) lean_take_max(t.vmax, t.imax, shfl_xor(t.vmax, (int)m), shfl_xor_u(t.imax, m));
+            if (FLAGS & F_MIN) lean_take_min(t.vmin, t.imin, shfl_xor(t.vmin, (int)m), shfl_xor_u(t.imin, m));
+            t.nan |= shfl_xor_u(t.nan, m);
+        }
+        t.m2 = 0.0;
+        if (FLAGS & F_M2) {
+            const double mu = (t.hi + t.lo) / ncols;  // divided: a constant row has m = its value, variance 0
+            const T mt = (T)mu;
+            T q4[4] = {(T)0, (T)0, (T)0, (T)0};
+#pragma unroll
+            for (int i = 0; i < K; ++i) {
+                const T d = in(i) ? rn_sub(y[i], mt) : (T)0;
+                q4[i & 3] = fma(d, d, q4[i & 3]);
+            }
+            double q = (double)rn_add(rn_add(q4[0], q4[1]), rn_add(q4[2], q4[3]));
+            for (unsigned m = G >> 1; m >= 1; m >>= 1) q += shfl_xor(q, (int)m);
+            const double dm = mu - (double)mt;
+            const double m2 = q - ncols * dm * dm;
+            t.m2 = m2 < 0.0 ? 0.0 : m2;  // (NaN stays NaN)
+        }
+        return t;
+    };
+    auto step = [&](T(&y)[K], size_t it) {
+        const size_t r = (v + it * W) * R + rr;
+        if constexpr (CH == CH_MULADD) {
+#pragma unroll
+            for (int i = 0; i < K; ++i) y[i] = rn_add(rn_mul(y[i], s0), s1);
+        } else if constexpr (CH == CH_ANY) {
+            apply_chain_n(p.chain, y);
+        }
+        const unsigned fn = (unsigned)__shfl(__builtin_isnan(y[0]) ? 1 : 0, (int)(lane & ~(G - 1)), 64);  // element 0
+        const RowTot t = every ? row_totals(y, std::integral_constant<bool, false>{}) : row_totals(y, std::integral_constant<bool, true>{});
+        if (r >= p.rows) return;  // past the last row (the last block only)
+        Part a;
+        a.n = ncols;
+        a.hi = t.hi;
+        a.lo = t.lo;
+        a.m2 = t.m2;
+        a.nan = t.nan;
+        a.vmax = t.imax != NO_CODE ? (double)t.vmax : -__builtin_inf();
+        a.imax = t.imax != NO_CODE ? (uint64_t)t.imax : NO_INDEX;
+        a.vmin = t.imin != NO_CODE ? (double)t.vmin : __builtin_inf();
+        a.imin = t.imin != NO_CODE ? (uint64_t)t.imin : NO_INDEX;
+        if (j == 0) write_outputs<T>(p.row, r, a, p.ddof, fn, 0);
+        if (!p.need_group) return;  // uniform over the grid
+        const uint64_t rb = (uint64_t)r * p.cols;
+        if (a.imax != NO_INDEX) a.imax += rb;
+        if (a.imin != NO_INDEX) a.imin += rb;
+        if (it == 0) wp_fn = fn;
+        lean_merge_next<FLAGS>(lp, a, lk, inv_cols);
+        ++lk;
+    };
+    // two blocks in turn: the next one's loads are in flight while one is folded
+    if (nit > 0) {
+        T y[2][K];
+        load(y[0], v);
+        const size_t npair = nit / 2;
+        for (size_t it = 0; it < npair; ++it) {
+            load(y[1], v + (2 * it + 1) * W);
+            step(y[0], 2 * it);
+            load(y[0], v + (2 * it + 2) * W);
+            step(y[1], 2 * it + 1);
+        }
+        if (nit & 1) step(y[0], nit - 1);
+    }
+    if (!p.need_group) return;
+    // the wavefront's rows: lanes G, 2G, ... apart hold different rows (equal counts when uniform)
+    if (p.uniform) {
+        double w = 0.5 / lp.n;
+        for (unsigned m = G; m < 64; m <<= 1) {
+            const Part o = part_shfl<FLAGS>(lp, (int)m);
+            part_merge_eq<FLAGS>(lp, o, w);
+            w *= 0.5;
+        }
+    } else {
+        for (unsigned m = G; m < 64; m <<= 1) {
+            const Part o = part_shfl<FLAGS>(lp, (int)m);
+            part_merge<FLAGS>(lp, o);
+        }
+    }
+    wp_fn = (unsigned)__shfl((int)wp_fn, 0, 64);
+    lean_group_finish<T, FLAGS>(fin, p.grp, p.counter, p.full, p.ddof, p.uniform, lp, wp_fn, []() {});
+}
+
+template <typename T, int FLAGS, int CH>
+static void launch_rows_kernel(const RowsParams<T> &p, size_t ngroups, bool vec, bool big, hipStream_t stream) {
+    constexpr int K1 = rows_k<T>(), K2 = 2 * rows_k<T>();
+    const dim3 g((unsigned)ngroups), b(RB);
+    if (big) {
+        if (vec) hipLaunchKernelGGL((rows_kernel<T, K2, FLAGS, true, CH>), g, b, 0, stream, p);
+        else hipLaunchKernelGGL((rows_kernel<T, K2, FLAGS, false, CH>), g, b, 0, stream, p);
+    } else {
+        if (vec) hipLaunchKernelGGL((rows_kernel<T, K1, FLAGS, true, CH>), g, b, 0, stream, p);
+        else hipLaunchKernelGGL((rows_kernel<T, K1, FLAGS, false, CH>), g, b, 0, stream, p);
+    }
+}
+
+template <typename T, int FLAGS>
+static nrk_status launch_rows(const RowsParams<T> &p, size_t ngroups, bool big, hipStream_t stream) {
+    constexpr int VN = Vec16<T>::N;
+    const bool vec = (uintptr_t)p.x % 16 == 0 && p.cols % VN == 0;
+    const Chain &c = p.chain;
+    if (c.len == 0)
+        launch_rows_kernel<T, FLAGS, CH_NONE>(p, ngroups, vec, big, stream);
+    else if (c.len == 2 && c.op[0] == NRK_SOP_MUL && (c.op[1] == NRK_SOP_ADD || c.op[1] == NRK_SOP_SUB))
+        launch_rows_kernel<T, FLAGS, CH_MULADD>(p, ngroups, vec, big, stream);
+    else
+        launch_rows_kernel<T, FLAGS, CH_ANY>(p, ngroups, vec, big, stream);
+    NRK_HIP(hipGetLastError());
+    return NRK_OK;
+}
+
+template <typename T>
+nrk_status run_rows(const Chain &chain, const T *x, size_t rows, size_t cols, double ddof, const OutPtrs<T> &row,
+                           const OutPtrs<T> &full, bool want_full, hipStream_t stream, const ExtScratch *ext) {
+    RowsParams<T> p{};
+    p.x = x;
+    p.chain = chain;
+    p.rows = rows;
+    p.cols = cols;
+    p.ddof = ddof;
+    // K slots per lane, doubled where the smaller K would take 32 or more lanes per row
+    const bool big = cols > 16 * (size_t)rows_k<T>();
+    const size_t K = big ? 2 * rows_k<T>() : rows_k<T>();
+    unsigned lg = 0;
+    while ((K << lg) < cols) ++lg;
+    p.G = 1u << lg;
+    p.lg = lg;
+    const size_t R = (size_t)64 >> lg;
+    p.nblk = ceil_div(rows, R);
+    const size_t ngroups = std::min<size_t>(ROWS_GROUPS, ceil_div(p.nblk, (size_t)WAVES));
+    // every wavefront reduces the same number of whole blocks: the finish merges equal counts
+    p.uniform = rows % R == 0 && p.nblk % (ngroups * WAVES) == 0;
+    if (chain.len == 2) {  // CH_MULADD: y = (x * s0) + s1, with x - b as x + (-b)
+        p.s0 = chain.s[0];
+        p.s1 = chain.op[1] == NRK_SOP_SUB ? -chain.s[1] : chain.s[1];
+    }
+    p.row = row;
+    p.need_group = want_full;
+    p.full = want_full ? full : OutPtrs<T>{};
+    if (want_full) NRK_TRY(group_scratch(ngroups, rows, cols, ext, stream, &p.counter, &p.grp));
+    const bool m2 = (row.var || full.var);
+    const bool mx = (row.max || row.argmax || full.max || full.argmax);
+    const bool mn = (row.min || row.argmin || full.min || full.argmin);
+    const int flags = (m2 ? F_M2 : 0) | (mx ? F_MAX : 0) | (mn ? F_MIN : 0);
+#define NRK_ROWS(F) \
+    case F: return launch_rows<T, F>(p, ngroups, big, stream);
+    switch (flags) {
+        NRK_ROWS(0) NRK_ROWS(1) NRK_ROWS(2) NRK_ROWS(3) NRK_ROWS(4) NRK_ROWS(5) NRK_ROWS(6)
+        default: NRK_ROWS(7)
+    }
+#undef NRK_ROWS
+}
+
+}  // namespace nrk

@@ -57,11 +57,11 @@ def test_noalloc_whole_array_only_and_size_limit(gpu):
     got = _run("nrk_dev_map_reduce_f64_noalloc", dx, 1, n, 0, st, with_rows=False)
     for k in want:
         np.testing.assert_array_equal(got[k].view(np.uint64), want[k].view(np.uint64), err_msg=k)
-    # row results for 100 000 rows of 513 (the record tree's path) need ~27 MB of records:
+    # row results for 100 000 rows of 1025 (the record tree's path) need ~27 MB of records:
     # refused, nothing launched
-    big = device.DeviceArray((100_000 * 513,), np.float64)
+    big = device.DeviceArray((100_000 * 1025,), np.float64)
     with pytest.raises(NrkError) as e:
-        _run("nrk_dev_map_reduce_f64_noalloc", big, 100_000, 513, 0, st)
+        _run("nrk_dev_map_reduce_f64_noalloc", big, 100_000, 1025, 0, st)
     assert "fixed scratch" in str(e.value)
     # and the entry point still works after the refusal
     got2 = _run("nrk_dev_map_reduce_f64_noalloc", dx, 1, n, 0, st, with_rows=False)
