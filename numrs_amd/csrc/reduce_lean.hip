@@ -180,10 +180,12 @@ static size_t lean_chunk_len(size_t spr) {
 
 // The chunk-to-wavefront layout, from the dtype and the shape only (it fixes which chunks a group
 // record holds, hence the whole-array result's bits): binary32 wavefront v takes chunks v, v + W,
-// ... (W the grid's wavefronts), binary64 8 consecutive chunks per wavefront.
+// ... (W the grid's wavefronts), binary64 8 consecutive chunks per wavefront (fewer below
+// 8 x 256 x 4 chunks, for 256 workgroups); below LEAN_MIN_CHUNKS chunks, wavefront v takes chunk v.
 constexpr size_t LEAN_MIN_CHUNKS = 1024;
 constexpr size_t LEAN_GROUPS_F32 = 512;  // round robin over at most this many workgroups
-constexpr size_t LEAN_BLOCK_F64 = 8;     // consecutive chunks per binary64 wavefront (at least)
+constexpr size_t LEAN_BLOCK_F64 = 8;     // consecutive chunks per binary64 wavefront (at least) ...
+constexpr size_t LEAN_GROUPS_F64 = 256;  // ... where that leaves at least this many workgroups
 static_assert(LEAN_GROUPS_F32 <= LEAN_MAX_GROUPS, "the finish merges at most LEAN_MAX_GROUPS group records");
 struct LeanLayout {
     size_t groups, A, B;
@@ -191,14 +193,22 @@ struct LeanLayout {
 template <typename T>
 static LeanLayout lean_layout(size_t nchunk, size_t max_groups) {
     LeanLayout L;
-    if (sizeof(T) == 4) {
+    if (nchunk < LEAN_MIN_CHUNKS) {  // a small array: one chunk per wavefront, the fewest dependent loads
+        L.groups = ceil_div(nchunk, (size_t)WAVES);
+        L.A = 1;
+        L.B = L.groups * WAVES;
+    } else if (sizeof(T) == 4) {
         L.groups = std::min<size_t>(LEAN_GROUPS_F32, ceil_div(nchunk, (size_t)4 * WAVES));
         L.A = 1;
         L.B = L.groups * WAVES;
     } else {
-        L.A = std::max<size_t>(LEAN_BLOCK_F64, ceil_div(nchunk, max_groups * WAVES));
+        // LEAN_BLOCK_F64 (or more) consecutive chunks per wavefront where that leaves a workgroup
+        // per CU (LEAN_GROUPS_F64), fewer where it would not
+        const size_t full = LEAN_BLOCK_F64 * LEAN_GROUPS_F64 * WAVES;
+        L.A = nchunk >= full ? std::max<size_t>(LEAN_BLOCK_F64, ceil_div(nchunk, max_groups * WAVES))
+                             : ceil_div(nchunk, LEAN_GROUPS_F64 * WAVES);
         L.groups = ceil_div(nchunk, L.A * WAVES);
-        L.B = 1;
+        L.B = L.A == 1 ? L.groups * WAVES : 1;  // A = 1 is read as round robin (one chunk per wavefront)
     }
     return L;
 }
@@ -288,8 +298,7 @@ nrk_status try_lean(const LeanProg &prog, const T *x0, const T *x1, size_t rows,
             if (cl >= (size_t)LEAN_WTAB) return NRK_OK;
             tail0 = spr / cl * cl * WSEG;
         }
-        nchunk = spr / cl;
-        if (nchunk < LEAN_MIN_CHUNKS) return NRK_OK;
+        nchunk = spr / cl;  // any count: fewer than a segment is all tail (one workgroup)
     }
     LeanParams<T> p{};
     p.x0 = x0;

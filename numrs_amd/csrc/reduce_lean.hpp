@@ -7,9 +7,10 @@
 namespace nrk {
 
 // ---------------- the lean kernel: full segments, contiguous chunk ranges ----------------
-// Used when every segment is whole (cols, or rows * cols for a whole-array reduction, is a
-// multiple of the segment), there are at least LEAN_MIN_CHUNKS chunks, and a chunk is a whole row
-// when row results are wanted. Then every chunk holds exactly cl full segments, and:
+// Used for row results when cols is a multiple of the segment and there are at least
+// LEAN_MIN_CHUNKS rows (a chunk is then a row), and for every whole-array reduction (chunks of cl
+// whole segments, and the elements after the last whole chunk folded by one more workgroup).
+// Every chunk holds exactly cl full segments, and:
 //   - chunk c is the contiguous elements [c cl WSEG, (c + 1) cl WSEG); each wavefront reduces an
 //     arithmetic sequence of chunks (round robin or a contiguous block, see lean_layout) with one
 //     pointer, keeping the next segment's loads in flight while it folds the current one;
@@ -358,18 +359,23 @@ __device__ __forceinline__ void lean_group_finish(FinishLds &L, const Parts &grp
             g = L.grp[0];
             for (int w = 1; w < WAVES; ++w) part_merge<FLAGS>(g, L.grp[w]);
         }
-        part_store_agent<FLAGS>(grp, blockIdx.x, g, L.fn[0]);
-        // the record has reached the point of coherence before it is counted. Arrivals are
-        // counted in two levels (a counter per set of LEAN_SET groups, each on its own 256-byte
-        // line, then one for the sets), so no counter takes more than LEAN_SET atomics.
-        NRK_WAIT_VMEM0();
-        const unsigned set = blockIdx.x / LEAN_SET, nset = (gridDim.x + LEAN_SET - 1) / LEAN_SET;
-        const unsigned in_set = min(LEAN_SET, gridDim.x - set * LEAN_SET);
-        bool last = false;
-        if (__hip_atomic_fetch_add(counter + LEAN_CTR_STRIDE * (1 + set), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
-            in_set - 1)
-            last = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == nset - 1;
-        L.last = last;
+        if (gridDim.x == 1) {  // the only group: its record is the result (no counting, no reload)
+            write_outputs<T>(full, 0, g, ddof, L.fn[0], 0);
+            L.last = 0;
+        } else {
+            part_store_agent<FLAGS>(grp, blockIdx.x, g, L.fn[0]);
+            // the record has reached the point of coherence before it is counted. Arrivals are
+            // counted in two levels (a counter per set of LEAN_SET groups, each on its own 256-byte
+            // line, then one for the sets), so no counter takes more than LEAN_SET atomics.
+            NRK_WAIT_VMEM0();
+            const unsigned set = blockIdx.x / LEAN_SET, nset = (gridDim.x + LEAN_SET - 1) / LEAN_SET;
+            const unsigned in_set = min(LEAN_SET, gridDim.x - set * LEAN_SET);
+            bool last = false;
+            if (__hip_atomic_fetch_add(counter + LEAN_CTR_STRIDE * (1 + set), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+                in_set - 1)
+                last = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == nset - 1;
+            L.last = last;
+        }
     }
     lds_barrier();
     after_count();

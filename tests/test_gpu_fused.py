@@ -47,7 +47,7 @@ def run_fused(x: np.ndarray, steps, ddof=0, want_rows=True, want_full=True, offs
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("shape", [(1, 1), (4, 7), (16, 8192), (9, 8193), (3, 40_000), (257, 100),
-                                   (1024, 1024), (2048, 3072), (1100, 2048)])  # the last three: the lean kernel
+                                   (1024, 1024), (2048, 3072), (1100, 2048), (4096, 1024)])  # the last four: the lean kernel
 def test_fused_vs_oracle(gpu, oracle, dtype, shape):
     rows, cols = shape
     x = rng.normal(rng.BASE_SEED + 1100 + rows * 7 + cols, rows * cols, dtype).reshape(rows, cols)
