@@ -61,6 +61,7 @@ def parse():
     p.add_argument("--no-batched", action="store_true")
     p.add_argument("--no-ufuncs", action="store_true")
     p.add_argument("--no-selector", action="store_true")
+    p.add_argument("--no-shapes", action="store_true")
     p.add_argument("--pmc-pass", action="store_true",
                    help="run each hot call once after a plain copy of known size (2 GiB, the counter "
                         "calibration), for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE; prints no JSON line")
@@ -233,6 +234,7 @@ def main() -> int:
     ragged = None if args.no_ragged else bench_ragged(device, rng, call, s, stream)
     batched = None if args.no_batched else bench_batched(device, rng, call, s, stream)
     ufuncs = None if args.no_ufuncs else bench_ufuncs(device, kernels, call, s, stream, d32, d64)
+    shapes = None if args.no_shapes else bench_shapes(device, kernels, call, ReduceOutputs, s, stream, d32, d64)
     sel = None if args.no_selector else bench_selector()
     cpu = None if args.no_cpu_baseline else cpu_baseline(x32, d64, xi, step_bytes)
 
@@ -262,6 +264,7 @@ def main() -> int:
         "ragged": ragged,
         "batched": batched,
         "ufuncs": ufuncs,
+        "general_shapes": shapes,
         "selector": sel,
         "cpu_baseline": cpu,
     }
@@ -596,6 +599,68 @@ def bench_ufuncs(device, kernels, call, s, stream, d32, d64):
         out[tag] = row
         log(f"ufuncs {tag}: " + ", ".join(f"{k} {v['ms']:.3f} ms ({v['GB/s']:.0f} GB/s)" for k, v in row.items()))
         del dy
+    out["seconds"] = round(time.time() - t0, 1)
+    return out
+
+
+# ---------------------------------------------------------------------------------------------
+# the fused reduction on shapes other than the headline's: the first rows * cols elements of the
+# headline's x (2^26 elements), the same y = 1.5 x - 0.25 and sum / max / argmax / var
+
+SHAPES = (  # (rows, cols, row results, whole-array results)
+    (8191, 8191, True, True),      # long rows off the 16-byte grid (general kernel)
+    (8192, 8188, True, False),     # long aligned rows, not a multiple of the segment (general kernel)
+    (512, 131072, True, False),    # few long rows (general kernel, rows split)
+    (1 << 20, 64, True, False),    # short rows (short-row kernel, 4 lanes per row)
+    (1 << 20, 64, True, True),
+    (262144, 256, True, False),    # 16 lanes per row
+    (100000, 671, True, False),    # 32 lanes per row, rows off the grid
+    (1 << 23, 8, True, False),     # one lane per row
+    (1, (1 << 26) - 1, False, True),  # whole array, not a whole number of chunks (lean + tail)
+    (1, 1000003, False, True),     # whole array, under 1024 chunks
+    (1, 1000, False, True),        # whole array, one workgroup
+)
+
+
+def bench_shapes(device, kernels, call, ReduceOutputs, s, stream, d32, d64):
+    t0 = time.time()
+    ops, n_ops = kernels.chain(("mul", 1.5), ("add", -0.25))
+    ops_p = C.cast(ops, C.c_void_p)
+    out = {"workload": "fused y = 1.5x - 0.25 -> sum/max/argmax/var, row and/or whole-array results, on the first "
+                       "rows*cols elements of the headline x; GB/s counts the input bytes once",
+           "hbm_peak_GB/s": HBM_PEAK_GBS}
+    for dt, dx, tag in ((np.float32, d32, "f32"), (np.float64, d64, "f64")):
+        fn = f"nrk_dev_map_reduce_{tag}"
+        row = {}
+        for rows, cols, want_rows, want_full in SHAPES:
+            ro, fo, keep = ReduceOutputs(), ReduceOutputs(), []
+            for f in ("sum", "max", "argmax", "var"):
+                odt = np.uint64 if f == "argmax" else dt
+                if want_rows:
+                    keep.append(device.DeviceArray((rows,), odt))
+                    setattr(ro, f, keep[-1].ptr)
+                if want_full:
+                    keep.append(device.DeviceArray((1,), odt))
+                    setattr(fo, f, keep[-1].ptr)
+            args_ = (ops_p, n_ops, C.c_void_p(dx.ptr), rows, cols, 0, C.byref(ro) if want_rows else None,
+                     C.byref(fo) if want_full else None, C.c_void_p(s))
+            f = lambda: call(fn, *args_)
+            f()
+            stream.synchronize()
+            a, b = device.Event(), device.Event()
+            a.record(s)
+            for _ in range(10):
+                f()
+            b.record(s)
+            stream.synchronize()
+            ms = a.elapsed_ms(b) / 10
+            nbytes = rows * cols * np.dtype(dt).itemsize
+            key = f"{rows}x{cols}" + (":rows" if want_rows else "") + (":full" if want_full else "")
+            row[key] = {"us": round(ms * 1e3, 2), "GB/s": round(nbytes / ms / 1e6, 1),
+                        "frac_hbm": round(nbytes / ms / 1e6 / HBM_PEAK_GBS, 3)}
+            del keep
+        out[tag] = row
+        log(f"general shapes {tag}: " + ", ".join(f"{k} {v['us']:.1f} us ({v['frac_hbm']:.2f})" for k, v in row.items()))
     out["seconds"] = round(time.time() - t0, 1)
     return out
 

@@ -50,58 +50,70 @@ void set_reduce_blocks_per_cu(int b) { g_blocks_per_cu = b < 1 ? 1 : (b > 16 ? 1
 // i = j*VN + e at segment position (j*64 + l)*VN + e, so every load instruction of the
 // wavefront reads 1 KiB of consecutive memory.
 
-// Issue the loads of the segment at `base` (len > 0 elements of the row remain there) into y and
-// record in vmask which elements exist (the others hold copies of the segment's first element;
-// the fold reads them only through vmask). VEC: len % VN == 0, so whole vectors. Every lane issues
-// every load, straight into y, so a segment is always exactly VPT (or VPT * VN) loads with no
+// Issue the loads of the segment at `base` into y; its positions [lo, len) belong to the row (lo > 0
+// only in a phased row's first segment; len <= lo: none, and base is then a safe address). The
+// others hold values of other elements; the fold reads them only through seg_mask's bits. VEC: base
+// is 16-byte aligned, so whole vectors. Every lane issues every load,
+// straight into y, so a segment is always exactly VPT (or VPT * VN) loads with no
 // branch around them and no copy after them: the compiler can then wait for one segment's loads
 // while the next segment's are in flight, where a data-dependent branch around a load (or a
 // select that frees the load registers for reuse) makes it wait for all of them.
 template <typename T, int VPT, bool VEC>
-__device__ __forceinline__ void seg_load(const T *base, size_t len, T (&y)[VPT * Vec16<T>::N], unsigned &vmask) {
+__device__ __forceinline__ void seg_load(const T *base, int64_t len, unsigned lo, T (&y)[VPT * Vec16<T>::N]) {
     typedef typename Vec16<T>::type V;
     constexpr int VN = Vec16<T>::N;
-    constexpr int M = VPT * VN;  // elements per lane (<= 32: one validity bit each)
-    static_assert(M <= 32, "validity mask holds 32 elements");
     const unsigned lane = threadIdx.x & 63;
-    vmask = 0u;
-    if (VEC) {
-        const V *bv = reinterpret_cast<const V *>(base);
-        const size_t nvec = len / VN;
 #pragma unroll
-        for (int j = 0; j < VPT; ++j) {
-            const unsigned jv = j * 64 + lane;
-            const bool ok = jv < nvec;
-            const V v = __builtin_nontemporal_load(bv + (ok ? jv : 0u));
+    for (int j = 0; j < VPT; ++j) {
+        const unsigned jv = j * 64 + lane;
+        const int64_t p0 = (int64_t)jv * VN;
+        if (VEC) {
+            // a vector starting inside the row lies in the row's 16-byte blocks
+            const V v = __builtin_nontemporal_load(reinterpret_cast<const V *>(base) + (p0 < len ? jv : 0u));
 #pragma unroll
             for (int e = 0; e < VN; ++e) y[j * VN + e] = v[e];
-            vmask |= (ok ? ((1u << VN) - 1u) : 0u) << (j * VN);
-        }
-    } else {
+        } else {
 #pragma unroll
-        for (int j = 0; j < VPT; ++j) {
-#pragma unroll
-            for (int e = 0; e < VN; ++e) {
-                const unsigned pos = (j * 64 + lane) * VN + e;
-                const bool ok = pos < len;
-                y[j * VN + e] = base[ok ? pos : 0u];
-                vmask |= (ok ? 1u : 0u) << (j * VN + e);
-            }
+            for (int e = 0; e < VN; ++e) y[j * VN + e] = base[(p0 + e >= (int64_t)lo && p0 + e < len) ? p0 + e : (int64_t)lo];
         }
     }
+}
+
+// Which of a segment's positions belong to the row: [lo, len), one bit per element slot.
+template <typename T, int VPT>
+__device__ __forceinline__ unsigned seg_mask(int64_t len, unsigned lo) {
+    constexpr int VN = Vec16<T>::N;
+    static_assert(VPT * VN <= 32, "validity mask holds 32 elements");
+    const unsigned lane = threadIdx.x & 63;
+    unsigned vmask = 0u;
+#pragma unroll
+    for (int j = 0; j < VPT; ++j) {
+        const int64_t p0 = (int64_t)(j * 64 + lane) * VN;
+        const int64_t dh = len - p0, dl = (int64_t)lo - p0;
+        const unsigned nhi = dh <= 0 ? 0u : (dh >= VN ? (unsigned)VN : (unsigned)dh);
+        const unsigned nlo = dl <= 0 ? 0u : (dl >= VN ? (unsigned)VN : (unsigned)dl);
+        vmask |= (((1u << nhi) - 1u) & ~((1u << nlo) - 1u)) << (j * VN);
+    }
+    return vmask;
 }
 
 // Fold one loaded segment (segment `sl` of its chunk) into the lane accumulators. FULLSEG: every
 // element exists (vmask is all ones).
 template <typename T, int VPT, int FLAGS, bool FULLSEG>
 __device__ __forceinline__ void seg_body(const Chain &chain, T (&y)[VPT * Vec16<T>::N], unsigned vmask, unsigned sl,
-                                         LaneAcc<T> &a, unsigned &fnan, bool row_start) {
+                                         LaneAcc<T> &a, unsigned &fnan, bool row_start, unsigned ph) {
     constexpr int M = VPT * Vec16<T>::N;
+    constexpr int VN = Vec16<T>::N;
     auto valid = [&](int i) -> bool { return FULLSEG || ((vmask >> i) & 1u); };
 
     // ---- elementwise chain, one rounding per step
     apply_chain_n(chain, y);
-    if (row_start) fnan = __builtin_isnan(y[0]) ? 1u : 0u;  // lane 0's y[0] is the row's element 0
+    if (row_start) {  // lane 0's y[ph] is the row's element 0
+        T y0 = y[0];
+#pragma unroll
+        for (int i = 1; i < VN; ++i) y0 = ph == (unsigned)i ? y[i] : y0;
+        fnan = __builtin_isnan(y0) ? 1u : 0u;
+    }
 
     // ---- sum: pairwise in T over the lane's elements, then into the lane's FP64 double word
     const T st = pairwise<M, 0, T>([&](int i) { return valid(i) ? y[i] : (T)0; });
@@ -167,6 +179,12 @@ __device__ __forceinline__ void seg_body(const Chain &chain, T (&y)[VPT * Vec16<
     }
 }
 
+// Row r's phase: its segments start ph = (r cols) mod VN elements before it (0 unless phased).
+template <typename T>
+__device__ __forceinline__ unsigned row_phase(const SegParams<T> &p, size_t r) {
+    return p.phased ? (unsigned)((r * p.cols) & (size_t)(Vec16<T>::N - 1)) : 0u;
+}
+
 // Position of a wavefront in its work: chunk c = row r, chunk-of-row q, segments [s0, s1).
 struct ChunkPos {
     size_t c, r, q, s0, s1;
@@ -228,7 +246,7 @@ __device__ __forceinline__ void chunk_end(const SegParams<T> &p, const ChunkPos 
     constexpr int M = VPT * VN;
     constexpr int WSEG = 64 * M;
     const unsigned lane = threadIdx.x & 63;
-    const uint64_t fbase = (uint64_t)k.r * p.cols + (uint64_t)k.s0 * WSEG;
+    const uint64_t fbase = (uint64_t)k.r * p.cols - row_phase<T>(p, k.r) + (uint64_t)k.s0 * WSEG;
     auto flat = [&](unsigned code) -> uint64_t {
         const unsigned sl = code / M, i = code % M;
         return fbase + (uint64_t)sl * WSEG + (uint64_t)(((i / VN) * 64 + lane) * VN + (i % VN));
@@ -261,13 +279,15 @@ __device__ __forceinline__ void chunk_end(const SegParams<T> &p, const ChunkPos 
 // Fold segment s of chunk k (already loaded into y) and close the chunk after its last segment.
 template <typename T, int VPT, int FLAGS>
 __device__ __forceinline__ void seg_step(const SegParams<T> &p, const ChunkPos &k, size_t s, T (&y)[VPT * Vec16<T>::N],
-                                         unsigned vmask, LaneAcc<T> &a, unsigned &fnan, Batch &bt, size_t W) {
+                                         LaneAcc<T> &a, unsigned &fnan, Batch &bt, size_t W) {
     constexpr int WSEG = 64 * VPT * Vec16<T>::N;
     const unsigned sl = (unsigned)(s - k.s0);
-    if (s * WSEG + WSEG <= p.cols) {
-        seg_body<T, VPT, FLAGS, true>(p.chain, y, vmask, sl, a, fnan, s == 0);
+    const unsigned ph = row_phase<T>(p, k.r);
+    if ((s > 0 || ph == 0) && s * WSEG + WSEG <= p.cols + ph) {
+        seg_body<T, VPT, FLAGS, true>(p.chain, y, ~0u, sl, a, fnan, s == 0, ph);
     } else {
-        seg_body<T, VPT, FLAGS, false>(p.chain, y, vmask, sl, a, fnan, s == 0);
+        const unsigned vmask = seg_mask<T, VPT>((int64_t)(p.cols + ph) - (int64_t)(s * WSEG), s == 0 ? ph : 0u);
+        seg_body<T, VPT, FLAGS, false>(p.chain, y, vmask, sl, a, fnan, s == 0, ph);
     }
     if (s + 1 == k.s1) chunk_end<T, VPT, FLAGS>(p, k, a, fnan, bt, W);
 }
@@ -291,8 +311,16 @@ __global__ __launch_bounds__(RB) void stream_kernel(SegParams<T> p) {
     LaneAcc<T> a;
     acc_reset(a);
     unsigned fnan = 0;
-    auto base_of = [&](const ChunkPos &k, size_t s) { return p.x + k.r * p.cols + s * WSEG; };
-    auto len_of = [&](size_t s) { return p.cols - s * WSEG; };
+    // segment s of row r: positions from element r cols - ph + s WSEG; the row's own are [lo, len).
+    // A segment past a phased row's end (len <= 0) reads from the row's first vector instead.
+    auto len_of = [&](const ChunkPos &k, size_t s) {
+        return (int64_t)(p.cols + row_phase<T>(p, k.r)) - (int64_t)(s * WSEG);
+    };
+    auto base_of = [&](const ChunkPos &k, size_t s) {
+        const T *b = p.x + k.r * p.cols - row_phase<T>(p, k.r);
+        return len_of(k, s) > 0 ? b + s * WSEG : b;
+    };
+    auto lo_of = [&](const ChunkPos &k, size_t s) { return s == 0 ? row_phase<T>(p, k.r) : 0u; };
     // advance (k, s) to the wavefront's next segment
     auto next = [&](ChunkPos &k, size_t &s) {
         if (s + 1 < k.s1) {
@@ -305,8 +333,7 @@ __global__ __launch_bounds__(RB) void stream_kernel(SegParams<T> p) {
     ChunkPos ka = chunk_pos(p, c0);
     size_t sa = ka.s0;
     T ya[M], yb[M];
-    unsigned ma, mb;
-    seg_load<T, VPT, VEC>(base_of(ka, sa), len_of(sa), ya, ma);
+    seg_load<T, VPT, VEC>(base_of(ka, sa), len_of(ka, sa), lo_of(ka, sa), ya);
     // The next segment's loads are issued unconditionally (past the wavefront's last segment they
     // reread the current one and are never used): no branch around them.
     for (;;) {
@@ -314,15 +341,17 @@ __global__ __launch_bounds__(RB) void stream_kernel(SegParams<T> p) {
         size_t sb = sa;
         next(kb, sb);
         const bool hb = kb.c < p.nchunk;
-        seg_load<T, VPT, VEC>(hb ? base_of(kb, sb) : base_of(ka, sa), hb ? len_of(sb) : len_of(sa), yb, mb);
-        seg_step<T, VPT, FLAGS>(p, ka, sa, ya, ma, a, fnan, bt, W);
+        seg_load<T, VPT, VEC>(hb ? base_of(kb, sb) : base_of(ka, sa), hb ? len_of(kb, sb) : len_of(ka, sa),
+                              hb ? lo_of(kb, sb) : lo_of(ka, sa), yb);
+        seg_step<T, VPT, FLAGS>(p, ka, sa, ya, a, fnan, bt, W);
         if (!hb) break;
         ka = kb;
         sa = sb;
         next(ka, sa);
         const bool ha = ka.c < p.nchunk;
-        seg_load<T, VPT, VEC>(ha ? base_of(ka, sa) : base_of(kb, sb), ha ? len_of(sa) : len_of(sb), ya, ma);
-        seg_step<T, VPT, FLAGS>(p, kb, sb, yb, mb, a, fnan, bt, W);
+        seg_load<T, VPT, VEC>(ha ? base_of(ka, sa) : base_of(kb, sb), ha ? len_of(ka, sa) : len_of(kb, sb),
+                              ha ? lo_of(ka, sa) : lo_of(kb, sb), ya);
+        seg_step<T, VPT, FLAGS>(p, kb, sb, yb, a, fnan, bt, W);
         if (!ha) break;
     }
     flush_batch<T, FLAGS>(p, bt, W);
@@ -415,7 +444,7 @@ static int resident_blocks(K kernel, int *cache) {
 // tuner's workgroups per CU), each wavefront then looping over its chunks.
 template <typename T, int VPT, int FLAGS>
 static nrk_status launch_stream(const SegParams<T> &p, hipStream_t stream) {
-    const bool vec = ((reinterpret_cast<uintptr_t>(p.x) % 16) == 0) && (p.cols % Vec16<T>::N == 0);
+    const bool vec = (reinterpret_cast<uintptr_t>(p.x) % 16) == 0;  // phased rows start on the grid too
     static int occ[2] = {0, 0};
     const int res = vec ? resident_blocks(stream_kernel<T, VPT, FLAGS, true>, &occ[1])
                         : resident_blocks(stream_kernel<T, VPT, FLAGS, false>, &occ[0]);
@@ -536,7 +565,10 @@ static nrk_status seg_reduce_impl(const Chain &chain, const T *x, size_t rows, s
     p.cols = cols;
     p.chain = chain;
     p.ddof = ddof;
-    p.spr = ceil_div(cols, (size_t)WSEG);
+    // rows whose length is not a multiple of the vector are phased: a row's positions run over
+    // cols + ph < cols + VN, so up to one more segment per row
+    p.phased = rows > 1 && cols % Vec16<T>::N != 0;
+    p.spr = ceil_div(cols + (p.phased ? Vec16<T>::N - 1 : 0), (size_t)WSEG);
     // chunks: whole rows when there are enough of them, else rows split into ~CHUNK_TARGET
     // chunks in total; depends on the shape only.
     size_t cpr = std::min(p.spr, std::max<size_t>(1, ceil_div(CHUNK_TARGET, rows)));

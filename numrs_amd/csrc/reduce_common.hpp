@@ -47,6 +47,8 @@ struct SegParams {
     size_t cl;      // segments per chunk (the last chunk of a row may hold fewer)
     size_t cpr;     // chunks per row
     size_t nchunk;  // rows * cpr
+    int phased;     // cols is not a multiple of the vector: row r's segments start at the vector-aligned
+                    // element r cols - ph, ph = (r cols) mod VN, its first ph positions masked
     Chain chain;
     double ddof;
     int row_direct;  // cpr == 1 and row results wanted: the stream writes them
