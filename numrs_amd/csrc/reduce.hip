@@ -381,34 +381,41 @@ __global__ __launch_bounds__(256) void tree_kernel(TreeParams<T> t) {
         if (lane == 0) {
             const unsigned fn = (t.in.flags[lo] >> 1) & 1u;
             if (t.results) write_outputs<T>(t.o, set, a, t.ddof, fn, t.rowmode ? (uint64_t)set * t.cols : 0);
-            if (t.keep) part_store<FLAGS>(t.out, g, a, fn);
+            if (t.keep) {
+                if (t.fuse_last)
+                    part_store_agent<FLAGS>(t.out, g, a, fn);
+                else
+                    part_store<FLAGS>(t.out, g, a, fn);
+            }
         }
     }
     if (!t.fuse_last) return;  // uniform over the grid
     // The last workgroup to finish merges the per_out records of this level (one set) in a fixed
     // order -- lane l folds records l, l+64, ..., then the butterfly -- whichever workgroup it is.
+    // Records go out at agent scope and are acknowledged before the arrival is counted (relaxed
+    // agent-scope atomic); the last workgroup reads them at agent scope: no fence.
     __shared__ unsigned is_last;
-    __threadfence();  // release this workgroup's records (device scope)
-    __syncthreads();
-    if (threadIdx.x == 0) is_last = atomicAdd(t.counter, 1u) == gridDim.x - 1;
-    __syncthreads();
+    NRK_WAIT_VMEM0();
+    lds_barrier();
+    if (threadIdx.x == 0)
+        is_last = __hip_atomic_fetch_add(t.counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1;
+    lds_barrier();
     if (!is_last) return;  // uniform over the workgroup
-    __threadfence();       // acquire the other workgroups' records
     if (threadIdx.x < 64) {
         Part v[TREE_LOADS];
 #pragma unroll
         for (int k = 0; k < TREE_LOADS; ++k) {
             const size_t s = lane + 64 * k;
-            v[k] = s < t.per_out ? part_load<FLAGS>(t.out, s) : part_empty();
+            v[k] = s < t.per_out ? part_load_agent<FLAGS>(t.out, s) : part_empty();
         }
         Part a = v[0];
 #pragma unroll
         for (int k = 1; k < TREE_LOADS; ++k) part_merge<FLAGS>(a, v[k]);
         wave_merge<FLAGS>(a);
         if (lane == 0) {
-            const unsigned fn = (t.out.flags[0] >> 1) & 1u;
+            const unsigned fn = (ld_agent(t.out.flags) >> 1) & 1u;
             write_outputs<T>(t.last_o, 0, a, t.ddof, fn, 0);
-            *t.counter = 0u;  // the zero-at-launch invariant for the next kernel on this stream
+            st_agent(t.counter, 0u);  // the zero-at-launch invariant for the next kernel on this stream
         }
     }
 }

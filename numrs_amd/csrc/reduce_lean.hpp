@@ -287,25 +287,6 @@ __device__ __forceinline__ void lean_merge_next(Part &wp, const Part &b, size_t 
     wp.n += b.n;
 }
 
-template <int FLAGS>
-__device__ __forceinline__ void part_store_agent(const Parts &P, size_t s, const Part &a, unsigned first_nan) {
-    st_agent(P.n + s, a.n); st_agent(P.hi + s, a.hi); st_agent(P.lo + s, a.lo);
-    if (FLAGS & F_M2) st_agent(P.m2 + s, a.m2);
-    if (FLAGS & F_MAX) { st_agent(P.vmax + s, a.vmax); st_agent(P.imax + s, a.imax); }
-    if (FLAGS & F_MIN) { st_agent(P.vmin + s, a.vmin); st_agent(P.imin + s, a.imin); }
-    st_agent(P.flags + s, a.nan | (first_nan << 1));
-}
-template <int FLAGS>
-__device__ __forceinline__ Part part_load_agent(const Parts &P, size_t s) {
-    Part a = part_empty();
-    a.n = ld_agent(P.n + s); a.hi = ld_agent(P.hi + s); a.lo = ld_agent(P.lo + s);
-    if (FLAGS & F_M2) a.m2 = ld_agent(P.m2 + s);
-    if (FLAGS & F_MAX) { a.vmax = ld_agent(P.vmax + s); a.imax = ld_agent(P.imax + s); }
-    if (FLAGS & F_MIN) { a.vmin = ld_agent(P.vmin + s); a.imin = ld_agent(P.imin + s); }
-    a.nan = ld_agent(P.flags + s) & 1u;
-    return a;
-}
-
 // The whole-array finish, at most LEAN_MAX_GROUPS group records; arrival counters (the top one,
 // then one per set of LEAN_SET groups) LEAN_CTR_STRIDE words apart.
 constexpr int LEAN_FIN_LOADS = 4;
