@@ -41,13 +41,13 @@ ROWS = COLS = 8192
 N_I64 = 1 << 28
 # HBM bytes per launch measured with `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE` over
 # `python3 bench.py --pmc-pass`, each counter scaled by its reading for a plain 2 GiB copy
-# (FETCH factor 2097152 / 1048718.688 KiB, WRITE factor 2097152 / 2097167.875 KiB). The run and its
+# (FETCH factor 2097152 / 1048730.438 KiB, WRITE factor 2097152 / 2097212.25 KiB). The run and its
 # figures are kept in profiles/hbm_traffic_reduce_step.md; the i64 sum fetches 0.9999x its
 # algorithmic bytes (no re-reads).
-_FETCH, _WRITE = 2097152 / 1048718.688, 2097152 / 2097167.875
-MEASURED_TRAFFIC = {"sum_i64": (1048607.375 * _FETCH + 16.0 * _WRITE) * 1024,
-                    "fused_map_reduce_f32": (132539.125 * _FETCH + 385.156 * _WRITE) * 1024,
-                    "fused_map_reduce_f64": (263766.188 * _FETCH + 320.656 * _WRITE) * 1024}
+_FETCH, _WRITE = 2097152 / 1048730.438, 2097152 / 2097212.25
+MEASURED_TRAFFIC = {"sum_i64": (1048619.062 * _FETCH + 32.062 * _WRITE) * 1024,
+                    "fused_map_reduce_f32": (132561.5 * _FETCH + 385.156 * _WRITE) * 1024,
+                    "fused_map_reduce_f64": (263728.375 * _FETCH + 320.656 * _WRITE) * 1024}
 
 
 def parse():
