@@ -11,10 +11,11 @@ namespace nrk {
 // Loads go out LEAN_TAIL_BATCH at a time (values past the end are loaded as zeros and skipped).
 constexpr int LEAN_TAIL_BATCH = 16;
 template <typename T, int FLAGS, int LEAVES, int CH>
-__device__ void lean_tail(const LeanParams<T> &p, unsigned wave, unsigned lane, Part &wp) {
+__device__ void lean_tail(const LeanParams<T> &p, unsigned wave, unsigned lane, Part &wp, unsigned &wp_fn) {
     constexpr int NB = LEAN_TAIL_BATCH;
     Part a = part_empty();
     const T s0 = (T)p.s0, s1 = (T)p.s1;
+    unsigned fn = 0;  // the array's element 0 is NaN (lane 0 of wavefront 0 holds it when the tail starts at 0)
     for (size_t i0 = p.tail0 + wave * 64 + lane; i0 < p.n_total; i0 += (size_t)NB * RB) {
         T y[NB], z[LEAVES == 2 ? NB : 1];
 #pragma unroll
@@ -34,6 +35,7 @@ __device__ void lean_tail(const LeanParams<T> &p, unsigned wave, unsigned lane, 
             bop_n(p.prog.op, y, z);
             apply_chain_n(p.prog.post, y);
         }
+        if (i0 == 0) fn = y[0] != y[0] ? 1u : 0u;
 #pragma unroll
         for (int k = 0; k < NB; ++k) {
             const size_t i = i0 + (size_t)k * RB;
@@ -55,6 +57,7 @@ __device__ void lean_tail(const LeanParams<T> &p, unsigned wave, unsigned lane, 
     }
     wave_merge<FLAGS>(a);
     wp = a;
+    wp_fn = (unsigned)__shfl((int)fn, 0, 64);
 }
 
 template <typename T, int VPT, int FLAGS, int LEAVES, bool VEC, int CH>
@@ -159,7 +162,7 @@ __global__ __launch_bounds__(RB) void lean_kernel(LeanParams<T> p) {
         }
         if (nseg & 1) step(y[0], z[0]);
     }
-    if (is_tail) lean_tail<T, FLAGS, LEAVES, CH>(p, wave, lane, wp);
+    if (is_tail) lean_tail<T, FLAGS, LEAVES, CH>(p, wave, lane, wp, wp_fn);
     if (!p.need_group) {  // uniform over the grid
         if (p.row_direct) flush_rows();
         return;

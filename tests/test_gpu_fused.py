@@ -213,3 +213,21 @@ def test_infinite_elements_give_binary64_sums(gpu, dtype, shape):
     if not np.isnan(fs):
         assert got["full_sum"][0] == -np.inf and got["full_mean"][0] == -np.inf
     assert np.isnan(got["full_var"][0])
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("n", [1, 5, 1000, 1023, 1024 + 5, 3000, 1_000_003])
+def test_whole_array_nan_first_and_interior(gpu, oracle, dtype, n):
+    """Whole arrays of every size class (one workgroup, a few chunks, many): a NaN at element 0
+    pins argmax and argmin to 0; an interior NaN is skipped by them; max and min are NaN either
+    way (against the oracle's sequential folds)."""
+    x = rng.normal(rng.BASE_SEED + 2200 + n, n, dtype).reshape(1, n)
+    for pos in sorted({0, n // 2, n - 1}):
+        z = x.copy()
+        z[0, pos] = np.nan
+        got = run_fused(z, [], want_rows=False)
+        assert np.isnan(got["full_max"][0]) and np.isnan(got["full_min"][0]), pos
+        assert got["full_argmax"][0] == oracle.ref("argmax", z), pos
+        assert got["full_argmin"][0] == oracle.ref("argmin", z), pos
+        if pos == 0:
+            assert got["full_argmax"][0] == 0 and got["full_argmin"][0] == 0

@@ -14,7 +14,8 @@ pytestmark = pytest.mark.gpu
 
 SHAPES = [(1, 1), (3, 2), (64, 1), (777, 3), (1000, 16), (100_003, 7), (50_000, 8), (4097, 12), (4099, 17), (5000, 64),
           (2048, 100), (1000, 255), (333, 256),
-          (20_000, 64), (3000, 511), (2500, 512), (1500, 513), (1200, 671), (1100, 1000), (1030, 1024)]
+          (20_000, 64), (3000, 511), (2500, 512), (1500, 513), (1200, 671), (1100, 1000), (1030, 1024),
+          (999, 1025), (1500, 1500), (700, 2048)]
 
 
 def _check_rows(got, y, oracle, ddof=0):
