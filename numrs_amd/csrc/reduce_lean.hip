@@ -283,7 +283,10 @@ nrk_status try_lean(const LeanProg &prog, const T *x0, const T *x1, size_t rows,
     // always gets the same summation tree
     size_t cl, nchunk, tail0 = rows * cols;
     const bool row_chunks = cols % WSEG == 0 && cols / WSEG < (size_t)LEAN_WTAB && rows >= LEAN_MIN_CHUNKS;
-    if (want_row && !row_chunks && !x1 && cols <= 128 * (size_t)rows_k<T>()) {  // short rows
+    // short rows: positions (cols, plus up to VN - 1 of phase for binary32) fit 64 lanes of
+    // 2 rows_k slots
+    const size_t span = cols + (sizeof(T) == 4 && rows > 1 && cols % Vec16<T>::N != 0 ? Vec16<T>::N - 1 : 0);
+    if (want_row && !row_chunks && !x1 && span <= 128 * (size_t)rows_k<T>()) {
         *taken = true;
         return run_rows<T>(prog.pre0, x0, rows, cols, ddof, row, full, want_full, stream, ext);
     }

@@ -12,8 +12,11 @@ namespace nrk {
 // larger where the smaller would take 32 or more lanes per row): G lanes per row, the smallest
 // power of two with G K >= cols, so a wavefront reduces a block of 64 / G rows at a time and
 // wavefront v of the W in the grid takes blocks v, v + W, ... Lane j of a row's G lanes holds slot
-// s = element ((s / VN) G + j) VN + s % VN of the row (VN-element vectors, the G lanes' side by
-// side), its first nv slots being inside the row. The whole row is in registers: the sum is
+// s = position ((s / VN) G + j) VN + s % VN of the row (VN-element vectors, the G lanes' side by
+// side), its first nv slots being inside the row. Binary32 rows whose length is not a multiple of
+// the vector are phased: row r's positions start at the aligned element r cols - ph, ph = (r cols)
+// mod VN, positions [ph, ph + cols) are the row's (so every load is a 16-byte vector), and G K
+// covers cols + VN - 1 positions. The whole row is in registers: the sum is
 // pairwise per lane (in T), then a butterfly over the row's lanes (double, double-word for
 // binary64); the variance is two-pass, sum((y - m)^2) with m the mean rounded to T, less
 // n (mean - m)^2. Extrema: strict winners per lane in slot (= element) order, then the butterfly
@@ -29,6 +32,8 @@ struct RowsParams {
     Chain chain;
     size_t rows, cols, nblk;
     unsigned G, lg;  // lanes per row and its log2
+    int phased;      // binary32, cols not a multiple of the vector: row r's positions start at the
+                     // aligned element r cols - ph, ph = (r cols) mod VN; [ph, ph + cols) are the row's
     double ddof;
     OutPtrs<T> row;
     int need_group, uniform;
@@ -55,26 +60,47 @@ __global__ __launch_bounds__(RB) void rows_kernel(RowsParams<T> p) {
     unsigned nv = 0;  // slots inside the row (a prefix)
 #pragma unroll
     for (int s = 0; s < K; ++s) nv += off(s) < cols ? 1u : 0u;
-    const bool every = cols == G * (unsigned)K;  // every slot of every lane is inside the row (uniform)
+    const bool every = !p.phased && cols == G * (unsigned)K;  // every slot of every lane is inside the row (uniform)
+    auto row_phase = [&](size_t r) -> unsigned { return (unsigned)((r * p.cols) & (size_t)(VN - 1)); };
     const T s0 = (T)p.s0, s1 = (T)p.s1;
     const double ncols = (double)cols, inv_cols = 1.0 / ncols;
     Part lp = part_empty();  // this lane's rows so far (whole-array results)
     size_t lk = 0;
     unsigned wp_fn = 0;
-    // block b's row for this lane (rows past the end read the last row: a harmless reload)
-    auto load = [&](T(&yy)[K], size_t b) {
+    // block b's row for this lane (rows past the end read the last row: a harmless reload). PH:
+    // phased rows (positions from the aligned element before the row).
+    auto load = [&](T(&yy)[K], size_t b, auto ph_tag) {
+        constexpr bool PH = decltype(ph_tag)::value;
         const size_t r = min(b * R + rr, p.rows - 1);
-        const T *base = p.x + r * p.cols;
+        if constexpr (PH) {
+            const unsigned ph = row_phase(r);
+            const T *base = p.x + r * p.cols - ph;  // position 0 (16-byte aligned when x is)
 #pragma unroll
-        for (int q = 0; q < VPT; ++q) {
-            const unsigned e0 = (q * G + j) * VN;
-            if constexpr (VEC) {
-                const V t = __builtin_nontemporal_load(reinterpret_cast<const V *>(base + min(e0, cols - VN)));
+            for (int q = 0; q < VPT; ++q) {
+                const unsigned e0 = (q * G + j) * VN;
+                if constexpr (VEC) {
+                    // a vector starting inside the row lies in the row's 16-byte blocks; else position 0's
+                    const V t = __builtin_nontemporal_load(reinterpret_cast<const V *>(base + (e0 < cols + ph ? e0 : 0u)));
 #pragma unroll
-                for (int e = 0; e < VN; ++e) yy[q * VN + e] = t[e];
-            } else {
+                    for (int e = 0; e < VN; ++e) yy[q * VN + e] = t[e];
+                } else {
 #pragma unroll
-                for (int e = 0; e < VN; ++e) yy[q * VN + e] = base[min(e0 + e, cols - 1)];
+                    for (int e = 0; e < VN; ++e) yy[q * VN + e] = base[e0 + e - ph < cols ? e0 + e : ph];
+                }
+            }
+        } else {
+            const T *base = p.x + r * p.cols;
+#pragma unroll
+            for (int q = 0; q < VPT; ++q) {
+                const unsigned e0 = (q * G + j) * VN;
+                if constexpr (VEC) {
+                    const V t = __builtin_nontemporal_load(reinterpret_cast<const V *>(base + min(e0, cols - VN)));
+#pragma unroll
+                    for (int e = 0; e < VN; ++e) yy[q * VN + e] = t[e];
+                } else {
+#pragma unroll
+                    for (int e = 0; e < VN; ++e) yy[q * VN + e] = base[min(e0 + e, cols - 1)];
+                }
             }
         }
     };
@@ -84,9 +110,14 @@ __global__ __launch_bounds__(RB) void rows_kernel(RowsParams<T> p) {
         T vmax, vmin;
         unsigned imax, imin, nan;
     };
-    auto row_totals = [&](const T(&y)[K], auto masked) -> RowTot {
-        constexpr bool MASK = decltype(masked)::value;
-        auto in = [&](int i) -> bool { return MASK ? (unsigned)i < nv : true; };
+    // MASK 0: every slot is the row's; 1: the first nv slots; 2: positions [ph, ph + cols)
+    auto row_totals = [&](const T(&y)[K], auto masked, unsigned ph) -> RowTot {
+        constexpr int MASK = decltype(masked)::value;
+        auto in = [&](int i) -> bool {
+            if constexpr (MASK == 0) return true;
+            else if constexpr (MASK == 1) return (unsigned)i < nv;
+            else return off((unsigned)i) - ph < cols;  // (unsigned: positions before ph wrap high)
+        };
         RowTot t;
         const T st = pairwise<K, 0, T>([&](int i) { return in(i) ? y[i] : (T)0; });
         t.nan = 0;
@@ -111,8 +142,8 @@ __global__ __launch_bounds__(RB) void rows_kernel(RowsParams<T> p) {
                 t.imin = w ? (unsigned)i : t.imin;
             }
         }
-        if (FLAGS & F_MAX) t.imax = t.imax == NO_CODE ? NO_CODE : off(t.imax);
-        if (FLAGS & F_MIN) t.imin = t.imin == NO_CODE ? NO_CODE : off(t.imin);
+        if (FLAGS & F_MAX) t.imax = t.imax == NO_CODE ? NO_CODE : off(t.imax) - ph;
+        if (FLAGS & F_MIN) t.imin = t.imin == NO_CODE ? NO_CODE : off(t.imin) - ph;
         t.hi = (double)st;
         t.lo = 0.0;
         for (unsigned m = G >> 1; m >= 1; m >>= 1) {  // uniform: G is the same for every lane
@@ -144,7 +175,8 @@ __global__ __launch_bounds__(RB) void rows_kernel(RowsParams<T> p) {
         }
         return t;
     };
-    auto step = [&](T(&y)[K], size_t it) {
+    auto step = [&](T(&y)[K], size_t it, auto ph_tag) {
+        constexpr bool PH = decltype(ph_tag)::value;
         const size_t r = (v + it * W) * R + rr;
         if constexpr (CH == CH_MULADD) {
 #pragma unroll
@@ -152,8 +184,19 @@ __global__ __launch_bounds__(RB) void rows_kernel(RowsParams<T> p) {
         } else if constexpr (CH == CH_ANY) {
             apply_chain_n(p.chain, y);
         }
-        const unsigned fn = (unsigned)__shfl(__builtin_isnan(y[0]) ? 1 : 0, (int)(lane & ~(G - 1)), 64);  // element 0
-        const RowTot t = every ? row_totals(y, std::integral_constant<bool, false>{}) : row_totals(y, std::integral_constant<bool, true>{});
+        unsigned fn;  // the row's element 0 is NaN: slot ph (0 unphased) of its lane 0
+        RowTot t;
+        if constexpr (PH) {
+            const unsigned ph = row_phase(min(r, p.rows - 1));
+            T y0 = y[0];
+#pragma unroll
+            for (int i = 1; i < VN; ++i) y0 = ph == (unsigned)i ? y[i] : y0;
+            fn = (unsigned)__shfl(__builtin_isnan(y0) ? 1 : 0, (int)(lane & ~(G - 1)), 64);
+            t = row_totals(y, std::integral_constant<int, 2>{}, ph);
+        } else {
+            fn = (unsigned)__shfl(__builtin_isnan(y[0]) ? 1 : 0, (int)(lane & ~(G - 1)), 64);
+            t = every ? row_totals(y, std::integral_constant<int, 0>{}, 0u) : row_totals(y, std::integral_constant<int, 1>{}, 0u);
+        }
         if (r >= p.rows) return;  // past the last row (the last block only)
         Part a;
         a.n = ncols;
@@ -175,17 +218,24 @@ __global__ __launch_bounds__(RB) void rows_kernel(RowsParams<T> p) {
         ++lk;
     };
     // two blocks in turn: the next one's loads are in flight while one is folded
-    if (nit > 0) {
+    auto run = [&](auto ph_tag) {
+        if (nit == 0) return;
         T y[2][K];
-        load(y[0], v);
+        load(y[0], v, ph_tag);
         const size_t npair = nit / 2;
         for (size_t it = 0; it < npair; ++it) {
-            load(y[1], v + (2 * it + 1) * W);
-            step(y[0], 2 * it);
-            load(y[0], v + (2 * it + 2) * W);
-            step(y[1], 2 * it + 1);
+            load(y[1], v + (2 * it + 1) * W, ph_tag);
+            step(y[0], 2 * it, ph_tag);
+            load(y[0], v + (2 * it + 2) * W, ph_tag);
+            step(y[1], 2 * it + 1, ph_tag);
         }
-        if (nit & 1) step(y[0], nit - 1);
+        if (nit & 1) step(y[0], nit - 1, ph_tag);
+    };
+    if constexpr (sizeof(T) == 4) {  // uniform: one loop or the other
+        if (p.phased) run(std::true_type{});
+        else run(std::false_type{});
+    } else {
+        run(std::false_type{});  // binary64 rows are not phased (two 8-byte loads cost less than the masks)
     }
     if (!p.need_group) return;
     // the wavefront's rows: lanes G, 2G, ... apart hold different rows (equal counts when uniform)
@@ -222,7 +272,7 @@ static void launch_rows_kernel(const RowsParams<T> &p, size_t ngroups, bool vec,
 template <typename T, int FLAGS>
 static nrk_status launch_rows(const RowsParams<T> &p, size_t ngroups, bool big, hipStream_t stream) {
     constexpr int VN = Vec16<T>::N;
-    const bool vec = (uintptr_t)p.x % 16 == 0 && p.cols % VN == 0;
+    const bool vec = (uintptr_t)p.x % 16 == 0 && (p.cols % VN == 0 || p.phased);  // phased rows start on the grid
     const Chain &c = p.chain;
     if (c.len == 0)
         launch_rows_kernel<T, FLAGS, CH_NONE>(p, ngroups, vec, big, stream);
@@ -243,11 +293,15 @@ nrk_status run_rows(const Chain &chain, const T *x, size_t rows, size_t cols, do
     p.rows = rows;
     p.cols = cols;
     p.ddof = ddof;
-    // K slots per lane, doubled where the smaller K would take 32 or more lanes per row
-    const bool big = cols > 16 * (size_t)rows_k<T>();
+    // binary32 rows whose length is not a multiple of the vector are phased: a row's positions run
+    // over cols + ph < cols + VN. K slots per lane, doubled where the smaller K would take 32 or
+    // more lanes per row.
+    p.phased = sizeof(T) == 4 && rows > 1 && cols % Vec16<T>::N != 0;
+    const size_t span = cols + (p.phased ? Vec16<T>::N - 1 : 0);
+    const bool big = span > 16 * (size_t)rows_k<T>();
     const size_t K = big ? 2 * rows_k<T>() : rows_k<T>();
     unsigned lg = 0;
-    while ((K << lg) < cols) ++lg;
+    while ((K << lg) < span) ++lg;
     p.G = 1u << lg;
     p.lg = lg;
     const size_t R = (size_t)64 >> lg;
