@@ -231,3 +231,37 @@ def test_whole_array_nan_first_and_interior(gpu, oracle, dtype, n):
         assert got["full_argmin"][0] == oracle.ref("argmin", z), pos
         if pos == 0:
             assert got["full_argmax"][0] == 0 and got["full_argmin"][0] == 0
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("shape", [(4096, 3001), (4100, 2500), (5000, 1030)])
+def test_general_rows_with_whole_array(gpu, oracle, dtype, shape):
+    """Long rows (not a multiple of the segment) from 4096 rows: a chunk is a row (row results
+    straight from the stream) and the whole-array results come from the record tree over the row
+    records. Against the oracle; a pointer off 16-byte alignment and a repeated call give the same
+    bits, and the row results equal the rows-only call's."""
+    rows, cols = shape
+    x = rng.normal(rng.BASE_SEED + 2300 + cols, rows * cols, dtype).reshape(rows, cols)
+    steps = [("mul", 1.5), ("add", -0.25)]
+    a = run_fused(x, steps)
+    b = run_fused(x, steps, offset_elems=1)
+    c = run_fused(x, steps)
+    d = run_fused(x, steps, want_full=False)
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+        np.testing.assert_array_equal(a[k], c[k], err_msg=k)
+        if k.startswith("row_"):
+            np.testing.assert_array_equal(a[k], d[k], err_msg=k)
+    y = oracle.ref_map([2, 0], [1.5, -0.25], x).reshape(rows, cols)
+    for op in ("max", "min", "argmax", "argmin"):
+        assert a["full_" + op][0] == oracle.ref(op, y), op
+        np.testing.assert_array_equal(a["row_" + op], oracle.ref_axis(K.AXIS[op], y, rows, cols, 1), err_msg=op)
+    ex = oracle.dd_stats(y, 0)
+    e = EPS[dtype]
+    s_abs = float(np.abs(y.astype(np.float64)).sum())
+    assert abs(a["full_sum"][0] - ex["sum"]) <= 8 * e * s_abs + e * abs(ex["sum"])
+    assert abs(a["full_var"][0] - ex["var"]) <= 64 * e * ex["var"]
+    z = x.copy()
+    z[0, 0] = np.nan  # the first element: whole-array argmax / argmin pinned to 0
+    got = run_fused(z, [])
+    assert got["full_argmax"][0] == 0 and got["full_argmin"][0] == 0 and np.isnan(got["full_max"][0])
