@@ -576,10 +576,10 @@ static nrk_status seg_reduce_impl(const Chain &chain, const T *x, size_t rows, s
     // cols + ph < cols + VN, so up to one more segment per row
     p.phased = rows > 1 && cols % Vec16<T>::N != 0;
     p.spr = ceil_div(cols + (p.phased ? Vec16<T>::N - 1 : 0), (size_t)WSEG);
-    // chunks: whole rows when there are enough of them (CHUNK_TARGET / 2 rows: at least one per
-    // wavefront of the grid, and the row results then come straight from the stream), else rows
-    // split into ~CHUNK_TARGET chunks in total; depends on the shape only.
-    size_t cpr = rows >= CHUNK_TARGET / 2 ? 1 : std::min(p.spr, std::max<size_t>(1, ceil_div(CHUNK_TARGET, rows)));
+    // chunks: whole rows when there are enough of them (CHUNK_TARGET / 4 rows; the row results then
+    // come straight from the stream, no record tree; measured against split rows from 1024 to 4096
+    // rows), else rows split into ~CHUNK_TARGET chunks in total; depends on the shape only.
+    size_t cpr = rows >= CHUNK_TARGET / 4 ? 1 : std::min(p.spr, std::max<size_t>(1, ceil_div(CHUNK_TARGET, rows)));
     p.cl = ceil_div(p.spr, cpr);
     p.cpr = ceil_div(p.spr, p.cl);
     p.nchunk = rows * p.cpr;
