@@ -236,7 +236,7 @@ def test_whole_array_nan_first_and_interior(gpu, oracle, dtype, n):
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("shape", [(4096, 3001), (4100, 2500), (5000, 1030)])
 def test_general_rows_with_whole_array(gpu, oracle, dtype, shape):
-    """Long rows (not a multiple of the segment) from 4096 rows: a chunk is a row (row results
+    """Long rows (not a multiple of the segment) from 2048 rows: a chunk is a row (row results
     straight from the stream) and the whole-array results come from the record tree over the row
     records. Against the oracle; a pointer off 16-byte alignment and a repeated call give the same
     bits, and the row results equal the rows-only call's."""
