@@ -306,7 +306,10 @@ nrk_status run_rows(const Chain &chain, const T *x, size_t rows, size_t cols, do
     p.lg = lg;
     const size_t R = (size_t)64 >> lg;
     p.nblk = ceil_div(rows, R);
-    const size_t ngroups = std::min<size_t>(ROWS_GROUPS, ceil_div(p.nblk, (size_t)WAVES));
+    // rows only with the doubled K (wide rows, few per wavefront): twice the workgroups (the row
+    // results do not depend on the grid; the whole-array finish keeps ROWS_GROUPS)
+    const size_t gmax = (!want_full && big) ? 2 * ROWS_GROUPS : ROWS_GROUPS;
+    const size_t ngroups = std::min<size_t>(gmax, ceil_div(p.nblk, (size_t)WAVES));
     // every wavefront reduces the same number of whole blocks: the finish merges equal counts
     p.uniform = rows % R == 0 && p.nblk % (ngroups * WAVES) == 0;
     if (chain.len == 2) {  // CH_MULADD: y = (x * s0) + s1, with x - b as x + (-b)
