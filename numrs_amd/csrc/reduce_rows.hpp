@@ -65,7 +65,6 @@ __global__ __launch_bounds__(RB) void rows_kernel(RowsParams<T> p) {
     const T s0 = (T)p.s0, s1 = (T)p.s1;
     const double ncols = (double)cols, inv_cols = 1.0 / ncols;
     Part lp = part_empty();  // this lane's rows so far (whole-array results)
-    size_t lk = 0;
     unsigned wp_fn = 0;
     // block b's row for this lane (rows past the end read the last row: a harmless reload). PH:
     // phased rows (positions from the aligned element before the row).
@@ -214,8 +213,9 @@ __global__ __launch_bounds__(RB) void rows_kernel(RowsParams<T> p) {
         if (a.imax != NO_INDEX) a.imax += rb;
         if (a.imin != NO_INDEX) a.imin += rb;
         if (it == 0) wp_fn = fn;
-        lean_merge_next<FLAGS>(lp, a, lk, inv_cols);
-        ++lk;
+        // this lane's it-th row (every earlier block was whole): the block index is uniform, so the
+        // weight table is read with a scalar load
+        lean_merge_next<FLAGS>(lp, a, it, inv_cols);
     };
     // two blocks in turn: the next one's loads are in flight while one is folded
     auto run = [&](auto ph_tag) {
