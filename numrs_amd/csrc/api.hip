@@ -130,6 +130,8 @@ nrk_status host_dot(const T *a, const T *b, size_t n, T *out) {
     return host_call<T, T>(a, n, out, 1, [&](const T *d, T *dout, hipStream_t s) { return dot<T>(d, (const T *)db, n, dout, s); });
 }
 
+constexpr size_t AXIS_DIRECT_OUTPUTS = (size_t)1 << 18;  // outputs from which one thread per output fills the chip
+
 template <typename T>
 nrk_status dev_axis(int32_t op, const T *x, size_t outer, size_t len, size_t inner, size_t ddof, void *out,
                     hipStream_t s) {
@@ -141,7 +143,19 @@ nrk_status dev_axis(int32_t op, const T *x, size_t outer, size_t len, size_t inn
     if (op == NRK_AXIS_VAR && len <= ddof)
         return fail(NRK_ERR_INVALID_ARGUMENT, "Degrees of freedom %zu >= axis size %zu", ddof, len);
     if (outer * inner == 0) return NRK_OK;
-    if (inner > 1) return strided_reduce<T>(op, x, outer, len, inner, (double)ddof, out, s);
+    if (inner > 1) {
+        // Enough outputs to fill the chip: one thread per output, left to right in the reference's
+        // own order (bit-exact). Fewer: the strided axis is transposed into rows in the stream's
+        // workspace and reduced by the row kernels (compensated sums and variances; extrema and
+        // their indices are the same bits either way).
+        if (outer * inner >= AXIS_DIRECT_OUTPUTS || len == 1)
+            return strided_reduce<T>(op, x, outer, len, inner, (double)ddof, out, s);
+        void *ws = nullptr;
+        NRK_TRY(get_workspace(s, outer * len * inner * sizeof(T), &ws));
+        NRK_TRY(axis_transpose<T>(x, (T *)ws, outer, len, inner, s));
+        x = (const T *)ws;
+        outer *= inner;
+    }
     nrk_reduce_outputs o{};
     switch (op) {
         case NRK_AXIS_SUM: o.sum = out; break;

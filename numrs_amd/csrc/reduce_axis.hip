@@ -74,6 +74,46 @@ nrk_status strided_reduce(int op, const T *x, size_t outer, size_t len, size_t i
     return NRK_OK;
 }
 
+// (outer, len, inner) -> (outer, inner, len): 64 x 64 tiles through LDS (a padded row: no bank
+// conflicts), rows of the tile read and written 64 consecutive elements at a time. One workgroup
+// per tile, the tiles numbered (o, i-tile, j-tile) over a 1-D grid.
+constexpr int AXIS_TILE = 64;
+template <typename T>
+__global__ __launch_bounds__(256) void axis_transpose_kernel(const T *x, T *y, size_t len, size_t inner, size_t ntile_i,
+                                                             size_t ntile_j) {
+    __shared__ T tile[AXIS_TILE][AXIS_TILE + 1];
+    size_t b = blockIdx.x;
+    const size_t jt = b % ntile_j;
+    b /= ntile_j;
+    const size_t it = b % ntile_i, o = b / ntile_i;
+    const size_t i0 = it * AXIS_TILE, j0 = jt * AXIS_TILE;
+    const T *xb = x + o * len * inner;
+    T *yb = y + o * inner * len;
+    const unsigned tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 64 x 4 threads
+    for (unsigned r = ty; r < AXIS_TILE; r += 4) {
+        const size_t j = j0 + r, i = i0 + tx;
+        if (j < len && i < inner) tile[r][tx] = xb[j * inner + i];
+    }
+    __syncthreads();
+    for (unsigned r = ty; r < AXIS_TILE; r += 4) {
+        const size_t i = i0 + r, j = j0 + tx;
+        if (i < inner && j < len) yb[i * len + j] = tile[tx][r];
+    }
+}
+
+template <typename T>
+nrk_status axis_transpose(const T *x, T *y, size_t outer, size_t len, size_t inner, hipStream_t stream) {
+    const size_t ni = ceil_div(inner, (size_t)AXIS_TILE), nj = ceil_div(len, (size_t)AXIS_TILE);
+    const size_t nb = outer * ni * nj;
+    if (nb == 0) return NRK_OK;
+    if (nb > 0x7fffffffu) return fail(NRK_ERR_INVALID_ARGUMENT, "(%zu, %zu, %zu) has too many tiles", outer, len, inner);
+    hipLaunchKernelGGL(axis_transpose_kernel<T>, dim3((unsigned)nb), dim3(256), 0, stream, x, y, len, inner, ni, nj);
+    NRK_HIP(hipGetLastError());
+    return NRK_OK;
+}
+
+template nrk_status axis_transpose<float>(const float *, float *, size_t, size_t, size_t, hipStream_t);
+template nrk_status axis_transpose<double>(const double *, double *, size_t, size_t, size_t, hipStream_t);
 template nrk_status strided_reduce<float>(int, const float *, size_t, size_t, size_t, double, void *, hipStream_t);
 template nrk_status strided_reduce<double>(int, const double *, size_t, size_t, size_t, double, void *, hipStream_t);
 

@@ -189,20 +189,49 @@ def test_axis1_rows_vs_oracle(gpu, oracle, dtype):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-def test_axis0_strided_bit_exact(gpu, oracle, dtype):
-    """Strided (inner > 1) reductions fold in the reference's own left-to-right order: every
-    op is bit-exact against the restatement (math/aggregation.rs:548-561 etc.)."""
-    for outer, length, inner in [(1, 7, 13), (1, 1000, 300), (4, 50, 70), (2, 3, 1025)]:
+def test_axis0_strided_many_outputs_bit_exact(gpu, oracle, dtype):
+    """Strided (inner > 1) reductions with at least 2^18 outputs fold each output in the
+    reference's own left-to-right order: every op is bit-exact against the restatement
+    (math/aggregation.rs:548-561 etc.)."""
+    for outer, length, inner in [(1, 5, 300_000), (3, 4, 90_000)]:
         x = rng.normal(rng.BASE_SEED + 600 + length, outer * length * inner, dtype)
         x[::97] = x[::97] * 3
         for op in ("sum", "mean", "var", "min", "max", "argmin", "argmax"):
             got = K.reduce_axis(op, x, outer, length, inner, ddof=1 if op == "var" else 0)
             want = oracle.ref_axis(K.AXIS[op], x, outer, length, inner, 1 if op == "var" else 0)
             np.testing.assert_array_equal(got, want, err_msg=f"{op} {outer}x{length}x{inner}")
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_axis0_strided_few_outputs(gpu, oracle, dtype):
+    """Strided reductions with fewer outputs: the axis is transposed into rows and reduced by the
+    row kernels. Extrema and their indices are bit-exact against the restatement (they do not
+    depend on the order); sums, means and variances are within the row reductions' bars of the
+    double-double statistics of each output's elements."""
+    for outer, length, inner in [(1, 7, 13), (1, 1000, 300), (4, 50, 70), (2, 3, 1025), (1, 70_000, 3), (5, 999, 65),
+                                 (1, 4097, 129)]:
+        x = rng.normal(rng.BASE_SEED + 600 + length, outer * length * inner, dtype)
+        x[::97] = x[::97] * 3
+        for op in ("min", "max", "argmin", "argmax"):
+            got = K.reduce_axis(op, x, outer, length, inner)
+            want = oracle.ref_axis(K.AXIS[op], x, outer, length, inner, 0)
+            np.testing.assert_array_equal(got, want, err_msg=f"{op} {outer}x{length}x{inner}")
+        cols = x.reshape(outer, length, inner).transpose(0, 2, 1).reshape(outer * inner, length)
+        ex = oracle.dd_row_stats(np.ascontiguousarray(cols), 1)
+        s_abs = np.abs(cols.astype(np.float64)).sum(axis=1)
+        e = EPS[dtype]
+        s = K.reduce_axis("sum", x, outer, length, inner).astype(np.float64)
+        assert np.all(np.abs(s - (ex[:, 0] + ex[:, 1])) <= 8 * e * s_abs + e * np.abs(ex[:, 0])), f"sum {outer}x{length}x{inner}"
+        m = K.reduce_axis("mean", x, outer, length, inner).astype(np.float64)
+        assert np.all(np.abs(m - ex[:, 2]) <= 8 * e * s_abs / length + 2 * e * np.abs(ex[:, 2])), f"mean {outer}x{length}x{inner}"
+        if length > 1:
+            v = K.reduce_axis("var", x, outer, length, inner, ddof=1).astype(np.float64)
+            assert np.all(np.abs(v - ex[:, 3]) <= VAR_C * e * ex[:, 3]), f"var {outer}x{length}x{inner}"
     y = rng.normal(1, 4 * 9, dtype).reshape(4, 9)
     y[2, 3] = np.nan
     mx = K.reduce_axis("max", y, 1, 4, 9)
     assert math.isnan(mx[3]) and not np.isnan(np.delete(mx, 3)).any()
+    assert K.reduce_axis("argmax", y, 1, 4, 9)[3] == oracle.ref_axis(6, y.reshape(-1), 1, 4, 9, 0)[3]
 
 
 def test_axis_errors(gpu):
