@@ -101,13 +101,55 @@ __global__ __launch_bounds__(256) void axis_transpose_kernel(const T *x, T *y, s
     }
 }
 
+// The same with 16-byte vectors in and out, where rows of x (stride inner) and of y (stride len)
+// start on the 16-byte grid: a tile row is 64 / VN vectors, 256 / (64 / VN) rows per pass.
+template <typename T>
+__global__ __launch_bounds__(256) void axis_transpose_vec_kernel(const T *x, T *y, size_t len, size_t inner, size_t ntile_i,
+                                                                 size_t ntile_j) {
+    typedef typename Vec16<T>::type V;
+    constexpr int VN = Vec16<T>::N;
+    constexpr unsigned VPR = AXIS_TILE / VN, RPP = 256 / VPR;
+    __shared__ T tile[AXIS_TILE][AXIS_TILE + 1];
+    size_t b = blockIdx.x;
+    const size_t jt = b % ntile_j;
+    b /= ntile_j;
+    const size_t it = b % ntile_i, o = b / ntile_i;
+    const size_t i0 = it * AXIS_TILE, j0 = jt * AXIS_TILE;
+    const T *xb = x + o * len * inner;
+    T *yb = y + o * inner * len;
+    const unsigned cv = threadIdx.x % VPR, r0 = threadIdx.x / VPR;
+    for (unsigned r = r0; r < AXIS_TILE; r += RPP) {
+        const size_t j = j0 + r, i = i0 + cv * VN;
+        if (j < len && i < inner) {  // inner % VN == 0: the whole vector is inside the row
+            const V v = __builtin_nontemporal_load(reinterpret_cast<const V *>(xb + j * inner + i));
+#pragma unroll
+            for (int e = 0; e < VN; ++e) tile[r][cv * VN + e] = v[e];
+        }
+    }
+    __syncthreads();
+    for (unsigned r = r0; r < AXIS_TILE; r += RPP) {
+        const size_t i = i0 + r, j = j0 + cv * VN;
+        if (i < inner && j < len) {  // len % VN == 0
+            V v;
+#pragma unroll
+            for (int e = 0; e < VN; ++e) v[e] = tile[cv * VN + e][r];
+            *reinterpret_cast<V *>(yb + i * len + j) = v;
+        }
+    }
+}
+
 template <typename T>
 nrk_status axis_transpose(const T *x, T *y, size_t outer, size_t len, size_t inner, hipStream_t stream) {
+    constexpr size_t VN = Vec16<T>::N;
     const size_t ni = ceil_div(inner, (size_t)AXIS_TILE), nj = ceil_div(len, (size_t)AXIS_TILE);
     const size_t nb = outer * ni * nj;
     if (nb == 0) return NRK_OK;
     if (nb > 0x7fffffffu) return fail(NRK_ERR_INVALID_ARGUMENT, "(%zu, %zu, %zu) has too many tiles", outer, len, inner);
-    hipLaunchKernelGGL(axis_transpose_kernel<T>, dim3((unsigned)nb), dim3(256), 0, stream, x, y, len, inner, ni, nj);
+    const bool vec = (uintptr_t)x % 16 == 0 && (uintptr_t)y % 16 == 0 && inner % VN == 0 && len % VN == 0;
+    if (vec)
+        hipLaunchKernelGGL(axis_transpose_vec_kernel<T>, dim3((unsigned)nb), dim3(256), 0, stream, x, y, len, inner, ni, nj);
+    else
+        hipLaunchKernelGGL(axis_transpose_kernel<T>, dim3((unsigned)nb), dim3(256), 0, stream, x, y, len, inner, ni, nj);
     NRK_HIP(hipGetLastError());
     return NRK_OK;
 }
