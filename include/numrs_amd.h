@@ -336,7 +336,11 @@ nrk_status nrk_dev_argmin_f32(const float *a, size_t n, uint64_t *out, nrk_strea
  * :244-261 for max/min with NaN short-circuit; math/statistics/mod.rs:1064-1071 mean,
  * :1403-1494 var = mean(axis) then sum((x-mean)^2)/(len-ddof); :105-183 argmax). len == 0
  * is an error for max/min/argmax/argmin/var (the reference errors on empty input) and gives
- * zeros for sum/mean. For var, len <= ddof is an error (statistics/mod.rs:1423-1428). */
+ * zeros for sum/mean. For var, len <= ddof is an error (statistics/mod.rs:1423-1428).
+ * Here: inner == 1 runs the row kernels (compensated sums and variances, within the bounds the
+ * fused reduction states); inner > 1 with at least 2^18 outputs folds each output left to right
+ * as the reference does (bit-exact); with fewer, the axis is transposed into rows in the
+ * stream's workspace and reduced as rows. Extrema and their indices are bit-exact on every path. */
 enum {
     NRK_AXIS_SUM = 0, NRK_AXIS_MEAN = 1, NRK_AXIS_VAR = 2, NRK_AXIS_MIN = 3,
     NRK_AXIS_MAX = 4, NRK_AXIS_ARGMIN = 5, NRK_AXIS_ARGMAX = 6
