@@ -254,3 +254,22 @@ def test_prod_close_to_exact(gpu):
     exact = math.exp(math.fsum(np.log(x)))
     assert abs(K.prod_f64(x) - exact) / exact < 1e-12
     assert abs(K.prod_f32(x.astype(np.float32)) - exact) / exact < 1e-3
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_axis0_sums_special_values(gpu, dtype):
+    """Sums and means along a strided axis with few outputs (transposed into rows, compensated row
+    sums): an infinity gives an infinite sum (not the NaN of a TwoSum error term), +inf with -inf
+    and NaN give NaN, columns of small integers are exact."""
+    rows, cols = 5000, 12
+    x = np.ones((rows, cols), dtype)
+    x[17, 1] = np.inf
+    x[100, 2] = np.inf
+    x[4000, 2] = -np.inf
+    x[2500, 3] = np.nan
+    x[:, 4] = np.arange(rows, dtype=dtype)
+    s = K.reduce_axis("sum", x.reshape(-1), 1, rows, cols)
+    m = K.reduce_axis("mean", x.reshape(-1), 1, rows, cols)
+    assert s[0] == rows and s[1] == np.inf and np.isnan(s[2]) and np.isnan(s[3])
+    assert s[4] == rows * (rows - 1) // 2 and m[4] == (rows - 1) / 2
+    assert m[0] == 1.0 and m[1] == np.inf
