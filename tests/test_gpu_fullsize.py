@@ -97,3 +97,80 @@ def test_split_gemm_8192_f32(gpu, oracle):
     assert e_nat <= n * u / (1 - n * u)
     assert e6 <= e_nat
     assert e3 <= 2.0 ** -13
+
+
+@pytest.mark.parametrize("n", [(1 << 32) + 4099, 5 << 30])
+def test_whole_array_beyond_2_32_elements(gpu, n):
+    """Whole-array reductions over more than 2^32 binary32 elements (16 and 20 GiB, filled on the
+    device): ones, with 5 and -3 planted past index 2^32 -- in the tail after the last whole chunk
+    for 2^32 + 4099, inside whole chunks for 5 * 2^30. The sum is exact in binary64 (every partial
+    is an integer below 2^53), so it is the binary32 rounding of n; max / argmax and min / argmin
+    give the planted values at their 64-bit indices; the variance is 32 / n."""
+    import ctypes as C
+    from numrs_amd._lib import ReduceOutputs, call
+    from numrs_amd.device import DeviceArray
+
+    print(f"whole array of {n} f32", flush=True)
+    d = DeviceArray((n,), np.float32)
+    call("nrk_memset_zero", C.c_void_p(d.ptr), C.c_size_t(n * 4), None)
+    ops, n_ops = K.chain(("add", 1.0))
+    call("nrk_dev_map_f32", C.cast(ops, C.c_void_p), n_ops, C.c_void_p(d.ptr), C.c_void_p(d.ptr), C.c_size_t(n), None)
+    i_max, i_min = (1 << 32) + 17, (1 << 32) + 4000
+    for i, v in ((i_max, 5.0), (i_min, -3.0)):
+        val = np.array([v], np.float32)
+        call("nrk_memcpy_htod", C.c_void_p(d.ptr + 4 * i), val.ctypes.data_as(C.c_void_p), C.c_size_t(4), None)
+    outs, fo = {}, ReduceOutputs()
+    for f in ("sum", "mean", "var", "min", "max", "argmin", "argmax"):
+        outs[f] = DeviceArray((1,), np.uint64 if f.startswith("arg") else np.float32)
+        setattr(fo, f, outs[f].ptr)
+    empty, _ = K.chain()
+    call("nrk_dev_map_reduce_f32", C.cast(empty, C.c_void_p), 0, C.c_void_p(d.ptr), C.c_size_t(1), C.c_size_t(n), C.c_size_t(0),
+         None, C.byref(fo), None)
+    call("nrk_device_synchronize")
+    got = {k: v.download()[0] for k, v in outs.items()}
+    assert got["sum"] == np.float32(float(n))
+    assert got["mean"] == 1.0
+    assert got["max"] == 5.0 and got["argmax"] == i_max
+    assert got["min"] == -3.0 and got["argmin"] == i_min
+    assert abs(float(got["var"]) - 32.0 / n) <= 1e-6 * 32.0 / n
+    del d
+
+
+def test_rows_beyond_2_32_elements(gpu):
+    """Row and whole-array results over (2^22 + 3) x 1025 binary32 elements (more than 2^32; the
+    short-row kernel, rows off the 16-byte grid): ones, with 5 planted in a late row. Every row sum
+    is 1025 and every row mean 1; the planted row's argmax is its column; the whole-array argmax is
+    the planted element's 64-bit flat index and the whole-array sum is exact before its rounding."""
+    import ctypes as C
+    from numrs_amd._lib import ReduceOutputs, call
+    from numrs_amd.device import DeviceArray
+
+    rows, cols = (1 << 22) + 3, 1025
+    n = rows * cols
+    d = DeviceArray((n,), np.float32)
+    call("nrk_memset_zero", C.c_void_p(d.ptr), C.c_size_t(n * 4), None)
+    ops, n_ops = K.chain(("add", 1.0))
+    call("nrk_dev_map_f32", C.cast(ops, C.c_void_p), n_ops, C.c_void_p(d.ptr), C.c_void_p(d.ptr), C.c_size_t(n), None)
+    r5, c5 = rows - 2, 1000
+    val = np.array([5.0], np.float32)
+    call("nrk_memcpy_htod", C.c_void_p(d.ptr + 4 * (r5 * cols + c5)), val.ctypes.data_as(C.c_void_p), C.c_size_t(4), None)
+    outs, ro, fo = {}, ReduceOutputs(), ReduceOutputs()
+    for f in ("sum", "mean", "max", "argmax"):
+        odt = np.uint64 if f == "argmax" else np.float32
+        outs["row_" + f] = DeviceArray((rows,), odt)
+        outs["full_" + f] = DeviceArray((1,), odt)
+        setattr(ro, f, outs["row_" + f].ptr)
+        setattr(fo, f, outs["full_" + f].ptr)
+    empty, _ = K.chain()
+    call("nrk_dev_map_reduce_f32", C.cast(empty, C.c_void_p), 0, C.c_void_p(d.ptr), C.c_size_t(rows), C.c_size_t(cols),
+         C.c_size_t(0), C.byref(ro), C.byref(fo), None)
+    call("nrk_device_synchronize")
+    got = {k: v.download() for k, v in outs.items()}
+    want_sum = np.full(rows, 1025.0, np.float32)
+    want_sum[r5] = 1029.0
+    np.testing.assert_array_equal(got["row_sum"], want_sum)
+    assert got["row_mean"][0] == 1.0 and got["row_mean"][-1] == 1.0
+    assert got["row_argmax"][r5] == c5 and got["row_max"][r5] == 5.0 and got["row_argmax"][0] == 0
+    assert got["full_argmax"][0] == r5 * cols + c5 and got["full_max"][0] == 5.0
+    assert got["full_sum"][0] == np.float32(float(n + 4))
+    del d
