@@ -136,17 +136,21 @@ def test_whole_array_beyond_2_32_elements(gpu, n):
     del d
 
 
-def test_rows_beyond_2_32_elements(gpu):
-    """Row and whole-array results over (2^22 + 3) x 1025 binary32 elements (more than 2^32; the
-    short-row kernel, rows off the 16-byte grid): ones, with 5 planted in a late row. Every row sum
-    is 1025 and every row mean 1; the planted row's argmax is its column; the whole-array argmax is
-    the planted element's 64-bit flat index and the whole-array sum is exact before its rounding."""
+@pytest.mark.parametrize("shape", [((1 << 22) + 3, 1025), ((1 << 21) + 1, 2049)])
+def test_rows_beyond_2_32_elements(gpu, shape):
+    """Row and whole-array results over more than 2^32 binary32 elements, rows off the 16-byte grid
+    ((2^22 + 3) x 1025: the short-row kernel; (2^21 + 1) x 2049: the general kernel, its
+    whole-array result through the record tree): ones, with 5 planted in a late row. Every row sum
+    is cols (cols + 4 in the planted row) and every row mean 1; the planted row's argmax is its
+    column; the whole-array argmax is the planted element's 64-bit flat index and the whole-array
+    sum is exact before its rounding."""
     import ctypes as C
     from numrs_amd._lib import ReduceOutputs, call
     from numrs_amd.device import DeviceArray
 
-    rows, cols = (1 << 22) + 3, 1025
+    rows, cols = shape
     n = rows * cols
+    print(f"rows {rows} x {cols} f32", flush=True)
     d = DeviceArray((n,), np.float32)
     call("nrk_memset_zero", C.c_void_p(d.ptr), C.c_size_t(n * 4), None)
     ops, n_ops = K.chain(("add", 1.0))
@@ -166,8 +170,8 @@ def test_rows_beyond_2_32_elements(gpu):
          C.c_size_t(0), C.byref(ro), C.byref(fo), None)
     call("nrk_device_synchronize")
     got = {k: v.download() for k, v in outs.items()}
-    want_sum = np.full(rows, 1025.0, np.float32)
-    want_sum[r5] = 1029.0
+    want_sum = np.full(rows, float(cols), np.float32)
+    want_sum[r5] = cols + 4.0
     np.testing.assert_array_equal(got["row_sum"], want_sum)
     assert got["row_mean"][0] == 1.0 and got["row_mean"][-1] == 1.0
     assert got["row_argmax"][r5] == c5 and got["row_max"][r5] == 5.0 and got["row_argmax"][0] == 0
