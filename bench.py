@@ -108,7 +108,7 @@ def log(msg: str) -> None:
 def main() -> int:
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
+        print(json.dumps({"error": "nrk runs on one GPU only; --gpus must be 1 "
                           "(how the path would shard is a design note, not a launched configuration)"}))
         return 2
     if args.pmc_pass:

@@ -448,7 +448,7 @@ nrk_status nrk_dev_std_f32(const float *a, size_t n, size_t ddof, float *out, nr
 /* ------------------------------------------------------------------------------------------ */
 /* No reference counterpart as one call: it fuses `(x * a) + b` (two roundings,
  * array/operations.rs:448-474) with the reductions above, which the reference runs as separate
- * passes over a materialised intermediate (SURVEY call stack E). The input is read once.
+ * passes over a materialised intermediate. The input is read once.
  *
  * x is (rows, cols) row-major; y = chain(x) elementwise. Outputs are device pointers; any may
  * be NULL to skip it. row_* outputs have `rows` elements (reduction along axis 1); full_*

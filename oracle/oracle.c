@@ -10,7 +10,7 @@
  *             source fixes it. Each function cites the reference file:line it restates.
  *             Built with -ffp-contract=off: every a*b+c is two roundings, as in Rust.
  *
- *  2. dd_* / exact_*   High-precision judges: double-double (TwoSum/TwoProd) accumulation and
+ *  2. dd_* / exact_*   High-precision references: double-double (TwoSum/TwoProd) accumulation and
  *             __int128 integer sums. These are the "FP64 oracle" every floating-point result is
  *             measured against (max relative error), and the exact answer for integers.
  *
@@ -632,7 +632,7 @@ void ref_gemm_rowsplit_f64(size_t m, size_t k, size_t n, const double *a, const 
 }
 
 /* ------------------------------------------------------------------------------------------ */
-/* high-precision judges                                                                      */
+/* high-precision references                                                                  */
 /* ------------------------------------------------------------------------------------------ */
 typedef struct { double hi, lo; } dd;
 

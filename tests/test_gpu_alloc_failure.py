@@ -1,4 +1,4 @@
-"""Out-of-memory paths (SURVEY aux item: allocator-hook OOM injection; no reference counterpart).
+"""Out-of-memory paths (an allocator hook injects the failures; no reference counterpart).
 
 nrk_debug_fail_allocations makes one of the library's own device allocations fail as a real
 out-of-memory does. The call that needed it returns NRK_ERR_OUT_OF_MEMORY with what it was

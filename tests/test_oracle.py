@@ -30,7 +30,7 @@ def test_determinism_pin_f32(oracle, known):
 
 
 def test_dd_stats_match_pins_and_closed_form(oracle, known):
-    """The double-double judge hits the pinned values (exact here) and Faulhaber's closed form."""
+    """The double-double reference hits the pinned values (exact here) and Faulhaber's closed form."""
     pins = known["determinism_pins"]
     x = det_data(np.float64)
     st = oracle.dd_stats(x, 0)
