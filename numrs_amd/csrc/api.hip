@@ -148,8 +148,10 @@ nrk_status dev_axis(int32_t op, const T *x, size_t outer, size_t len, size_t inn
         // own order (bit-exact). Fewer: the strided axis is transposed into rows in the stream's
         // workspace and reduced by the row kernels (compensated sums and variances; extrema and
         // their indices are the same bits either way).
-        if (outer * inner >= AXIS_DIRECT_OUTPUTS || len == 1)
+        if (outer * inner >= AXIS_DIRECT_OUTPUTS || len <= 1)
             return strided_reduce<T>(op, x, outer, len, inner, (double)ddof, out, s);
+        if (op == NRK_AXIS_SUM || op == NRK_AXIS_MEAN)  // one read: column tiles over ranges of rows
+            return axis_sum<T>(x, outer, len, inner, op == NRK_AXIS_MEAN, (T *)out, s);
         void *ws = nullptr;
         NRK_TRY(get_workspace(s, outer * len * inner * sizeof(T), &ws));
         NRK_TRY(axis_transpose<T>(x, (T *)ws, outer, len, inner, s));

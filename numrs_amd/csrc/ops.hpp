@@ -66,6 +66,9 @@ nrk_status strided_reduce(int op, const T *x, size_t outer, size_t len, size_t i
 // (outer, len, inner) -> (outer, inner, len), for reductions along a strided axis with few outputs
 template <typename T>
 nrk_status axis_transpose(const T *x, T *y, size_t outer, size_t len, size_t inner, hipStream_t stream);
+// sums / means along a strided axis with few outputs, reading the data once
+template <typename T>
+nrk_status axis_sum(const T *x, size_t outer, size_t len, size_t inner, bool mean, T *out, hipStream_t stream);
 
 nrk_status sum_i64(const int64_t *x, size_t n, int64_t *out, hipStream_t stream);
 template <typename T>

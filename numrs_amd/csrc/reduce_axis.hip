@@ -6,6 +6,7 @@
 // :164-175 argmax, first strict maximum). Adjacent threads own adjacent outputs, so every step of
 // the walk is one coalesced row read across the wavefront.
 #include "ops.hpp"
+#include "reduce_common.hpp"
 
 namespace nrk {
 
@@ -154,6 +155,125 @@ nrk_status axis_transpose(const T *x, T *y, size_t outer, size_t len, size_t inn
     return NRK_OK;
 }
 
+// Sums and means along a strided axis with few outputs, reading the data once. Workgroup
+// (o, column tile, range) walks rows [range start, + rlen) of 64 adjacent outputs in tiles of 64
+// rows staged through LDS (16-byte loads where inner and x allow); thread t sums rows
+// 16 (t / 64) .. + 15 of every tile for output column t % 64 (pairwise in T, then binary64; a
+// double word for binary64 elements), the four row quarters are merged in LDS, and the range's
+// partial goes to part[range][output]. The finish adds each output's partials in range order.
+constexpr int AXS_C = 64, AXS_R = 64;
+template <typename T, bool VEC>
+__global__ __launch_bounds__(256) void axis_sum_tile_kernel(const T *x, size_t len, size_t inner, size_t nct, size_t nrange,
+                                                            size_t rlen, size_t nout, double *phi, double *plo) {
+    typedef typename Vec16<T>::type V;
+    constexpr int VN = Vec16<T>::N;
+    __shared__ T tile[AXS_R][AXS_C + 1];
+    __shared__ double qhi[4][AXS_C], qlo[4][AXS_C];
+    size_t b = blockIdx.x;
+    const size_t r = b % nrange;
+    b /= nrange;
+    const size_t ct = b % nct, o = b / nct;
+    const size_t c0 = ct * AXS_C, j0 = r * rlen, j1 = min(len, j0 + rlen);
+    const T *xb = x + o * len * inner;
+    const unsigned t = threadIdx.x, col = t % AXS_C, quarter = t / AXS_C;
+    double hi = 0.0, lo = 0.0;
+    for (size_t jt = j0; jt < j1; jt += AXS_R) {
+        if constexpr (VEC) {  // thread t: vector t % 16 of rows t / 16 + 16 k
+            constexpr unsigned VPR = AXS_C / VN, RPP = 256 / VPR;
+            const unsigned cv = t % VPR;
+            for (unsigned rr = t / VPR; rr < AXS_R; rr += RPP) {
+                const size_t j = jt + rr, c = c0 + cv * VN;
+                if (j < j1 && c < inner) {  // inner % VN == 0: the whole vector is in the row
+                    const V v = __builtin_nontemporal_load(reinterpret_cast<const V *>(xb + j * inner + c));
+#pragma unroll
+                    for (int e = 0; e < VN; ++e) tile[rr][cv * VN + e] = v[e];
+                } else {
+#pragma unroll
+                    for (int e = 0; e < VN; ++e) tile[rr][cv * VN + e] = (T)0;
+                }
+            }
+        } else {  // thread t: column t % 64 of rows t / 64 + 4 k
+            for (unsigned rr = t / AXS_C; rr < AXS_R; rr += 256 / AXS_C) {
+                const size_t j = jt + rr, c = c0 + col;
+                tile[rr][col] = (j < j1 && c < inner) ? xb[j * inner + c] : (T)0;
+            }
+        }
+        __syncthreads();
+        // 16 rows of this thread's quarter, pairwise (zeros past the range or the row are exact)
+        const T st = pairwise<16, 0, T>([&](int i) { return tile[quarter * 16 + i][col]; });
+        if constexpr (sizeof(T) == 8) {
+            double s2, e2;
+            two_sum(hi, (double)st, s2, e2);
+            dw_fix(s2, rn_add(e2, lo), hi, lo);
+        } else {
+            hi += (double)st;
+        }
+        __syncthreads();
+    }
+    qhi[quarter][col] = hi;
+    qlo[quarter][col] = lo;
+    __syncthreads();
+    if (t < AXS_C && c0 + t < inner) {
+        double h = qhi[0][t], l = qlo[0][t];
+        for (int q = 1; q < 4; ++q) dw_add(h, l, qhi[q][t], qlo[q][t]);
+        const size_t out = o * inner + c0 + t;
+        phi[r * nout + out] = h;
+        plo[r * nout + out] = l;
+    }
+}
+
+// One wavefront per output: lane l adds partials l, l + 64, ... in order, then the butterfly.
+template <typename T>
+__global__ __launch_bounds__(256) void axis_sum_finish_kernel(const double *phi, const double *plo, size_t nout, size_t nrange,
+                                                              double len, int mean, T *out) {
+    const size_t w = (blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6;
+    const unsigned lane = threadIdx.x & 63;
+    if (w >= nout) return;  // whole wavefronts
+    double h = 0.0, l = 0.0;
+    for (size_t r = lane; r < nrange; r += 64) dw_add(h, l, phi[r * nout + w], plo[r * nout + w]);
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        const double bh = __shfl_xor(h, m, 64), bl = __shfl_xor(l, m, 64);
+        dw_add(h, l, bh, bl);
+    }
+    if (lane == 0) {
+        const double sum = h + l;
+        out[w] = (T)(mean ? sum / len : sum);
+    }
+}
+
+template <typename T>
+nrk_status axis_sum(const T *x, size_t outer, size_t len, size_t inner, bool mean, T *out, hipStream_t stream) {
+    constexpr size_t VN = Vec16<T>::N;
+    const size_t nout = outer * inner;
+    if (nout == 0) return NRK_OK;
+    const bool vec = (uintptr_t)x % 16 == 0 && inner % VN == 0;
+    const size_t nct = ceil_div(inner, (size_t)AXS_C);
+    // ranges: about 2048 workgroups in all, at least one tile of rows each; from the shape only
+    const size_t want = std::max<size_t>(1, 2048 / std::max<size_t>(1, outer * nct));
+    const size_t nrange = std::max<size_t>(1, std::min(want, ceil_div(len, (size_t)AXS_R)));
+    const size_t rlen = ceil_div(ceil_div(len, nrange), (size_t)AXS_R) * AXS_R;
+    const size_t nr = ceil_div(len, rlen);  // ranges that hold rows
+    const size_t nb = outer * nct * nr;
+    if (nb > 0x7fffffffu) return fail(NRK_ERR_INVALID_ARGUMENT, "(%zu, %zu, %zu) has too many tiles", outer, len, inner);
+    void *ws = nullptr;
+    NRK_TRY(get_workspace(stream, 2 * nout * nr * sizeof(double), &ws));
+    double *phi = (double *)ws, *plo = phi + nout * nr;
+    if (vec)
+        hipLaunchKernelGGL((axis_sum_tile_kernel<T, true>), dim3((unsigned)nb), dim3(256), 0, stream, x, len, inner, nct, nr, rlen,
+                           nout, phi, plo);
+    else
+        hipLaunchKernelGGL((axis_sum_tile_kernel<T, false>), dim3((unsigned)nb), dim3(256), 0, stream, x, len, inner, nct, nr, rlen,
+                           nout, phi, plo);
+    NRK_HIP(hipGetLastError());
+    hipLaunchKernelGGL(axis_sum_finish_kernel<T>, dim3((unsigned)ceil_div(nout * 64, (size_t)256)), dim3(256), 0, stream, phi, plo,
+                       nout, nr, (double)len, mean ? 1 : 0, out);
+    NRK_HIP(hipGetLastError());
+    return NRK_OK;
+}
+
+template nrk_status axis_sum<float>(const float *, size_t, size_t, size_t, bool, float *, hipStream_t);
+template nrk_status axis_sum<double>(const double *, size_t, size_t, size_t, bool, double *, hipStream_t);
 template nrk_status axis_transpose<float>(const float *, float *, size_t, size_t, size_t, hipStream_t);
 template nrk_status axis_transpose<double>(const double *, double *, size_t, size_t, size_t, hipStream_t);
 template nrk_status strided_reduce<float>(int, const float *, size_t, size_t, size_t, double, void *, hipStream_t);

@@ -204,10 +204,11 @@ def test_axis0_strided_many_outputs_bit_exact(gpu, oracle, dtype):
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_axis0_strided_few_outputs(gpu, oracle, dtype):
-    """Strided reductions with fewer outputs: the axis is transposed into rows and reduced by the
-    row kernels. Extrema and their indices are bit-exact against the restatement (they do not
-    depend on the order); sums, means and variances are within the row reductions' bars of the
-    double-double statistics of each output's elements."""
+    """Strided reductions with fewer outputs: sums and means by column tiles over ranges of the
+    axis, the other ops with the axis transposed into rows for the row kernels. Extrema and their
+    indices are bit-exact against the restatement (they do not depend on the order); sums, means
+    and variances are within the row reductions' bars of the double-double statistics of each
+    output's elements."""
     for outer, length, inner in [(1, 7, 13), (1, 1000, 300), (4, 50, 70), (2, 3, 1025), (1, 70_000, 3), (5, 999, 65),
                                  (1, 4097, 129)]:
         x = rng.normal(rng.BASE_SEED + 600 + length, outer * length * inner, dtype)
@@ -258,9 +259,9 @@ def test_prod_close_to_exact(gpu):
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_axis0_sums_special_values(gpu, dtype):
-    """Sums and means along a strided axis with few outputs (transposed into rows, compensated row
-    sums): an infinity gives an infinite sum (not the NaN of a TwoSum error term), +inf with -inf
-    and NaN give NaN, columns of small integers are exact."""
+    """Sums and means along a strided axis with few outputs (column tiles over ranges of the axis,
+    binary64 partials): an infinity gives an infinite sum (not the NaN of a TwoSum error term),
+    +inf with -inf and NaN give NaN, columns of small integers are exact."""
     rows, cols = 5000, 12
     x = np.ones((rows, cols), dtype)
     x[17, 1] = np.inf
