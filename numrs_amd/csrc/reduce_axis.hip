@@ -160,7 +160,8 @@ nrk_status axis_transpose(const T *x, T *y, size_t outer, size_t len, size_t inn
 // rows staged through LDS (16-byte loads where inner and x allow); thread t sums rows
 // 16 (t / 64) .. + 15 of every tile for output column t % 64 (pairwise in T, then binary64; a
 // double word for binary64 elements), the four row quarters are merged in LDS, and the range's
-// partial goes to part[range][output]. The finish adds each output's partials in range order.
+// partial goes to part[output][range] (contiguous per output, for the finish's lanes). The finish
+// adds each output's partials: lane l takes ranges l, l + 64, ... in order, then the butterfly.
 constexpr int AXS_C = 64, AXS_R = 64;
 template <typename T, bool VEC>
 __global__ __launch_bounds__(256) void axis_sum_tile_kernel(const T *x, size_t len, size_t inner, size_t nct, size_t nrange,
@@ -217,8 +218,8 @@ __global__ __launch_bounds__(256) void axis_sum_tile_kernel(const T *x, size_t l
         double h = qhi[0][t], l = qlo[0][t];
         for (int q = 1; q < 4; ++q) dw_add(h, l, qhi[q][t], qlo[q][t]);
         const size_t out = o * inner + c0 + t;
-        phi[r * nout + out] = h;
-        plo[r * nout + out] = l;
+        phi[out * nrange + r] = h;
+        plo[out * nrange + r] = l;
     }
 }
 
@@ -230,7 +231,7 @@ __global__ __launch_bounds__(256) void axis_sum_finish_kernel(const double *phi,
     const unsigned lane = threadIdx.x & 63;
     if (w >= nout) return;  // whole wavefronts
     double h = 0.0, l = 0.0;
-    for (size_t r = lane; r < nrange; r += 64) dw_add(h, l, phi[r * nout + w], plo[r * nout + w]);
+    for (size_t r = lane; r < nrange; r += 64) dw_add(h, l, phi[w * nrange + r], plo[w * nrange + r]);
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) {
         const double bh = __shfl_xor(h, m, 64), bl = __shfl_xor(l, m, 64);
