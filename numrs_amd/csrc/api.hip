@@ -152,6 +152,7 @@ nrk_status dev_axis(int32_t op, const T *x, size_t outer, size_t len, size_t inn
             return strided_reduce<T>(op, x, outer, len, inner, (double)ddof, out, s);
         if (op == NRK_AXIS_SUM || op == NRK_AXIS_MEAN)  // one read: column tiles over ranges of rows
             return axis_sum<T>(x, outer, len, inner, op == NRK_AXIS_MEAN, (T *)out, s);
+        if (op != NRK_AXIS_VAR) return axis_extrema<T>(op, x, outer, len, inner, out, s);
         void *ws = nullptr;
         NRK_TRY(get_workspace(s, outer * len * inner * sizeof(T), &ws));
         NRK_TRY(axis_transpose<T>(x, (T *)ws, outer, len, inner, s));

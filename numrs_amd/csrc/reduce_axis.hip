@@ -273,6 +273,160 @@ nrk_status axis_sum(const T *x, size_t outer, size_t len, size_t inner, bool mea
     return NRK_OK;
 }
 
+// Extrema along a strided axis with few outputs, the same tiles and ranges: thread t keeps the
+// strict winner (value, row) of its column over its quarter's rows in row order, NaN elements
+// never winning but flagged (and whether row 0 is NaN); quarters and ranges merge with the lower
+// row winning ties, so the result is the sequential fold's: max / min NaN where any element is,
+// argmax / argmin 0 where the first element is NaN or no element wins (all -inf / +inf or NaN).
+template <typename T, bool VEC, bool MX>
+__global__ __launch_bounds__(256) void axis_ext_tile_kernel(const T *x, size_t len, size_t inner, size_t nct, size_t nrange,
+                                                            size_t rlen, T *pv, uint64_t *pi, unsigned *pf) {
+    typedef typename Vec16<T>::type V;
+    constexpr int VN = Vec16<T>::N;
+    __shared__ T tile[AXS_R][AXS_C + 1];
+    __shared__ T qv[4][AXS_C];
+    __shared__ uint64_t qi[4][AXS_C];
+    __shared__ unsigned qf[4][AXS_C];
+    size_t b = blockIdx.x;
+    const size_t r = b % nrange;
+    b /= nrange;
+    const size_t ct = b % nct, o = b / nct;
+    const size_t c0 = ct * AXS_C, j0 = r * rlen, j1 = min(len, j0 + rlen);
+    const T *xb = x + o * len * inner;
+    const unsigned t = threadIdx.x, col = t % AXS_C, quarter = t / AXS_C;
+    T bv = MX ? -dev_inf<T>() : dev_inf<T>();
+    uint64_t bj = NO_INDEX;
+    unsigned flags = 0;  // bit 0: a NaN; bit 1: row 0 is NaN
+    for (size_t jt = j0; jt < j1; jt += AXS_R) {
+        if constexpr (VEC) {
+            constexpr unsigned VPR = AXS_C / VN, RPP = 256 / VPR;
+            const unsigned cv = t % VPR;
+            for (unsigned rr = t / VPR; rr < AXS_R; rr += RPP) {
+                const size_t j = jt + rr, c = c0 + cv * VN;
+                if (j < j1 && c < inner) {
+                    const V v = __builtin_nontemporal_load(reinterpret_cast<const V *>(xb + j * inner + c));
+#pragma unroll
+                    for (int e = 0; e < VN; ++e) tile[rr][cv * VN + e] = v[e];
+                }
+            }
+        } else {
+            for (unsigned rr = t / AXS_C; rr < AXS_R; rr += 256 / AXS_C) {
+                const size_t j = jt + rr, c = c0 + col;
+                if (j < j1 && c < inner) tile[rr][col] = xb[j * inner + c];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const size_t j = jt + quarter * 16 + i;
+            if (j < j1 && c0 + col < inner) {
+                const T v = tile[quarter * 16 + i][col];
+                if (v != v) {
+                    flags |= j == 0 ? 3u : 1u;
+                } else if (MX ? v > bv : v < bv) {
+                    bv = v;
+                    bj = j;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    qv[quarter][col] = bv;
+    qi[quarter][col] = bj;
+    qf[quarter][col] = flags;
+    __syncthreads();
+    if (t < AXS_C && c0 + t < inner) {
+        T v = qv[0][t];
+        uint64_t ix = qi[0][t];
+        unsigned f = qf[0][t];
+        for (int q = 1; q < 4; ++q) {
+            const T w = qv[q][t];
+            const uint64_t wi = qi[q][t];
+            f |= qf[q][t];
+            if (wi != NO_INDEX && (ix == NO_INDEX || (MX ? w > v : w < v) || (w == v && wi < ix))) {
+                v = w;
+                ix = wi;
+            }
+        }
+        const size_t out = o * inner + c0 + t;
+        pv[out * nrange + r] = v;
+        pi[out * nrange + r] = ix;
+        pf[out * nrange + r] = f;
+    }
+}
+
+template <typename T, bool MX>
+__global__ __launch_bounds__(256) void axis_ext_finish_kernel(const T *pv, const uint64_t *pi, const unsigned *pf, size_t nout,
+                                                              size_t nrange, int want_index, void *out) {
+    const size_t w = (blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6;
+    const unsigned lane = threadIdx.x & 63;
+    if (w >= nout) return;  // whole wavefronts
+    T v = MX ? -dev_inf<T>() : dev_inf<T>();
+    uint64_t ix = NO_INDEX;
+    unsigned f = 0;
+    auto take = [&](T bvv, uint64_t bi) {
+        if (bi != NO_INDEX && (ix == NO_INDEX || (MX ? bvv > v : bvv < v) || (bvv == v && bi < ix))) {
+            v = bvv;
+            ix = bi;
+        }
+    };
+    for (size_t r = lane; r < nrange; r += 64) {
+        take(pv[w * nrange + r], pi[w * nrange + r]);
+        f |= pf[w * nrange + r];
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        const T bvv = __shfl_xor(v, m, 64);
+        const unsigned lo = __shfl_xor((unsigned)(ix & 0xffffffffu), m, 64), hi = __shfl_xor((unsigned)(ix >> 32), m, 64);
+        take(bvv, ((uint64_t)hi << 32) | lo);
+        f |= (unsigned)__shfl_xor((int)f, m, 64);
+    }
+    if (lane == 0) {
+        if (want_index)
+            ((uint64_t *)out)[w] = (f & 2u) ? 0 : (ix == NO_INDEX ? 0 : ix);
+        else
+            ((T *)out)[w] = (f & 1u) ? dev_nan<T>() : v;
+    }
+}
+
+template <typename T>
+nrk_status axis_extrema(int op, const T *x, size_t outer, size_t len, size_t inner, void *out, hipStream_t stream) {
+    constexpr size_t VN = Vec16<T>::N;
+    const size_t nout = outer * inner;
+    if (nout == 0) return NRK_OK;
+    const bool vec = (uintptr_t)x % 16 == 0 && inner % VN == 0;
+    const bool mx = op == NRK_AXIS_MAX || op == NRK_AXIS_ARGMAX;
+    const int want_index = (op == NRK_AXIS_ARGMAX || op == NRK_AXIS_ARGMIN) ? 1 : 0;
+    const size_t nct = ceil_div(inner, (size_t)AXS_C);
+    const size_t want = std::max<size_t>(1, 2048 / std::max<size_t>(1, outer * nct));
+    const size_t nrange = std::max<size_t>(1, std::min(want, ceil_div(len, (size_t)AXS_R)));
+    const size_t rlen = ceil_div(ceil_div(len, nrange), (size_t)AXS_R) * AXS_R;
+    const size_t nr = ceil_div(len, rlen);
+    const size_t nb = outer * nct * nr;
+    if (nb > 0x7fffffffu) return fail(NRK_ERR_INVALID_ARGUMENT, "(%zu, %zu, %zu) has too many tiles", outer, len, inner);
+    void *ws = nullptr;
+    const size_t np = nout * nr;
+    NRK_TRY(get_workspace(stream, np * (sizeof(T) + sizeof(uint64_t) + sizeof(unsigned)) + 64, &ws));
+    uint64_t *pi = (uint64_t *)ws;
+    T *pv = (T *)(pi + np);
+    unsigned *pf = (unsigned *)(pv + np);
+    const dim3 g((unsigned)nb), bl(256), gf((unsigned)ceil_div(nout * 64, (size_t)256));
+#define NRK_EXT(V, M)                                                                                                   \
+    do {                                                                                                                \
+        hipLaunchKernelGGL((axis_ext_tile_kernel<T, V, M>), g, bl, 0, stream, x, len, inner, nct, nr, rlen, pv, pi, pf); \
+        hipLaunchKernelGGL((axis_ext_finish_kernel<T, M>), gf, bl, 0, stream, pv, pi, pf, nout, nr, want_index, out);    \
+    } while (0)
+    if (vec && mx) NRK_EXT(true, true);
+    else if (vec) NRK_EXT(true, false);
+    else if (mx) NRK_EXT(false, true);
+    else NRK_EXT(false, false);
+#undef NRK_EXT
+    NRK_HIP(hipGetLastError());
+    return NRK_OK;
+}
+
+template nrk_status axis_extrema<float>(int, const float *, size_t, size_t, size_t, void *, hipStream_t);
+template nrk_status axis_extrema<double>(int, const double *, size_t, size_t, size_t, void *, hipStream_t);
 template nrk_status axis_sum<float>(const float *, size_t, size_t, size_t, bool, float *, hipStream_t);
 template nrk_status axis_sum<double>(const double *, size_t, size_t, size_t, bool, double *, hipStream_t);
 template nrk_status axis_transpose<float>(const float *, float *, size_t, size_t, size_t, hipStream_t);

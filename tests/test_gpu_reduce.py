@@ -204,11 +204,11 @@ def test_axis0_strided_many_outputs_bit_exact(gpu, oracle, dtype):
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_axis0_strided_few_outputs(gpu, oracle, dtype):
-    """Strided reductions with fewer outputs: sums and means by column tiles over ranges of the
-    axis, the other ops with the axis transposed into rows for the row kernels. Extrema and their
-    indices are bit-exact against the restatement (they do not depend on the order); sums, means
-    and variances are within the row reductions' bars of the double-double statistics of each
-    output's elements."""
+    """Strided reductions with fewer outputs: sums, means and extrema by column tiles over ranges
+    of the axis, variances with the axis transposed into rows for the row kernels. Extrema and
+    their indices are bit-exact against the restatement (they do not depend on the order); sums,
+    means and variances are within the row reductions' bars of the double-double statistics of
+    each output's elements."""
     for outer, length, inner in [(1, 7, 13), (1, 1000, 300), (4, 50, 70), (2, 3, 1025), (1, 70_000, 3), (5, 999, 65),
                                  (1, 4097, 129)]:
         x = rng.normal(rng.BASE_SEED + 600 + length, outer * length * inner, dtype)
@@ -255,6 +255,32 @@ def test_prod_close_to_exact(gpu):
     exact = math.exp(math.fsum(np.log(x)))
     assert abs(K.prod_f64(x) - exact) / exact < 1e-12
     assert abs(K.prod_f32(x.astype(np.float32)) - exact) / exact < 1e-3
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_axis0_extrema_special_values(gpu, oracle, dtype):
+    """Extrema along a strided axis with few outputs (column tiles over ranges of the axis), bit-exact
+    against the restatement on the cases where the order of the fold shows: a NaN first element
+    (argmax / argmin 0, max / min NaN), a NaN after the winner, columns of all -inf / +inf (no
+    strict winner: index 0), ties across tiles and ranges (the lowest index wins), +-0."""
+    rows, cols = 9000, 70
+    x = rng.normal(rng.BASE_SEED + 640, rows * cols, dtype).reshape(rows, cols)
+    x[0, 1] = np.nan
+    x[5000, 2] = np.nan
+    x[:, 3] = -np.inf
+    x[:, 4] = np.inf
+    x[:, 5] = 7.0
+    x[:, 6] = 0.0
+    x[3000::1000, 6] = -0.0
+    x[[130, 4100, 8999], 7] = 50.0
+    x[[64, 4096, 8191], 8] = -50.0
+    x[:, 9] = np.nan
+    x[8999, 10] = 60.0
+    x[[0, 8999], 11] = np.nan
+    for op in ("min", "max", "argmin", "argmax"):
+        got = K.reduce_axis(op, x.reshape(-1), 1, rows, cols)
+        want = oracle.ref_axis(K.AXIS[op], x.reshape(-1), 1, rows, cols, 0)
+        np.testing.assert_array_equal(np.asarray(got).view(np.uint8), np.asarray(want).view(np.uint8), err_msg=op)
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
