@@ -339,10 +339,10 @@ nrk_status nrk_dev_argmin_f32(const float *a, size_t n, uint64_t *out, nrk_strea
  * zeros for sum/mean. For var, len <= ddof is an error (statistics/mod.rs:1423-1428).
  * Here: inner == 1 runs the row kernels (compensated sums and variances, within the bounds the
  * fused reduction states); inner > 1 with at least 2^18 outputs folds each output left to right
- * as the reference does (bit-exact); with fewer, sums and means read column tiles over ranges of
- * the axis (binary64 partials, then each output's ranges in order) and the other ops transpose
- * the axis into rows in the stream's workspace and reduce them as rows. Extrema and their indices
- * are bit-exact on every path. */
+ * as the reference does (bit-exact); with fewer, every op reads column tiles over ranges of the
+ * axis once (binary64 partials for sums and means, Chan records for variances, strict winners for
+ * extrema), then merges each output's ranges. Extrema and their indices are bit-exact on every
+ * path. */
 enum {
     NRK_AXIS_SUM = 0, NRK_AXIS_MEAN = 1, NRK_AXIS_VAR = 2, NRK_AXIS_MIN = 3,
     NRK_AXIS_MAX = 4, NRK_AXIS_ARGMIN = 5, NRK_AXIS_ARGMAX = 6

@@ -145,19 +145,14 @@ nrk_status dev_axis(int32_t op, const T *x, size_t outer, size_t len, size_t inn
     if (outer * inner == 0) return NRK_OK;
     if (inner > 1) {
         // Enough outputs to fill the chip: one thread per output, left to right in the reference's
-        // own order (bit-exact). Fewer: the strided axis is transposed into rows in the stream's
-        // workspace and reduced by the row kernels (compensated sums and variances; extrema and
-        // their indices are the same bits either way).
+        // own order (bit-exact). Fewer: column tiles over ranges of the axis read the data once
+        // (compensated sums and Chan variances; extrema and their indices are the same bits).
         if (outer * inner >= AXIS_DIRECT_OUTPUTS || len <= 1)
             return strided_reduce<T>(op, x, outer, len, inner, (double)ddof, out, s);
-        if (op == NRK_AXIS_SUM || op == NRK_AXIS_MEAN)  // one read: column tiles over ranges of rows
+        if (op == NRK_AXIS_SUM || op == NRK_AXIS_MEAN)
             return axis_sum<T>(x, outer, len, inner, op == NRK_AXIS_MEAN, (T *)out, s);
-        if (op != NRK_AXIS_VAR) return axis_extrema<T>(op, x, outer, len, inner, out, s);
-        void *ws = nullptr;
-        NRK_TRY(get_workspace(s, outer * len * inner * sizeof(T), &ws));
-        NRK_TRY(axis_transpose<T>(x, (T *)ws, outer, len, inner, s));
-        x = (const T *)ws;
-        outer *= inner;
+        if (op == NRK_AXIS_VAR) return axis_var<T>(x, outer, len, inner, (double)ddof, (T *)out, s);
+        return axis_extrema<T>(op, x, outer, len, inner, out, s);
     }
     nrk_reduce_outputs o{};
     switch (op) {

@@ -63,15 +63,15 @@ nrk_status gemm_bf16_config(int config, const nrk_gemm_desc &d, hipStream_t stre
 template <typename T>
 nrk_status strided_reduce(int op, const T *x, size_t outer, size_t len, size_t inner, double ddof, void *out,
                           hipStream_t stream);
-// (outer, len, inner) -> (outer, inner, len), for reductions along a strided axis with few outputs
-template <typename T>
-nrk_status axis_transpose(const T *x, T *y, size_t outer, size_t len, size_t inner, hipStream_t stream);
 // sums / means along a strided axis with few outputs, reading the data once
 template <typename T>
 nrk_status axis_sum(const T *x, size_t outer, size_t len, size_t inner, bool mean, T *out, hipStream_t stream);
 // max / min / argmax / argmin along a strided axis with few outputs, reading the data once
 template <typename T>
 nrk_status axis_extrema(int op, const T *x, size_t outer, size_t len, size_t inner, void *out, hipStream_t stream);
+// sample variance (divisor len - ddof) along a strided axis with few outputs, reading the data once
+template <typename T>
+nrk_status axis_var(const T *x, size_t outer, size_t len, size_t inner, double ddof, T *out, hipStream_t stream);
 
 nrk_status sum_i64(const int64_t *x, size_t n, int64_t *out, hipStream_t stream);
 template <typename T>

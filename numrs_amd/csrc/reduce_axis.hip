@@ -75,86 +75,6 @@ nrk_status strided_reduce(int op, const T *x, size_t outer, size_t len, size_t i
     return NRK_OK;
 }
 
-// (outer, len, inner) -> (outer, inner, len): 64 x 64 tiles through LDS (a padded row: no bank
-// conflicts), rows of the tile read and written 64 consecutive elements at a time. One workgroup
-// per tile, the tiles numbered (o, i-tile, j-tile) over a 1-D grid.
-constexpr int AXIS_TILE = 64;
-template <typename T>
-__global__ __launch_bounds__(256) void axis_transpose_kernel(const T *x, T *y, size_t len, size_t inner, size_t ntile_i,
-                                                             size_t ntile_j) {
-    __shared__ T tile[AXIS_TILE][AXIS_TILE + 1];
-    size_t b = blockIdx.x;
-    const size_t jt = b % ntile_j;
-    b /= ntile_j;
-    const size_t it = b % ntile_i, o = b / ntile_i;
-    const size_t i0 = it * AXIS_TILE, j0 = jt * AXIS_TILE;
-    const T *xb = x + o * len * inner;
-    T *yb = y + o * inner * len;
-    const unsigned tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 64 x 4 threads
-    for (unsigned r = ty; r < AXIS_TILE; r += 4) {
-        const size_t j = j0 + r, i = i0 + tx;
-        if (j < len && i < inner) tile[r][tx] = xb[j * inner + i];
-    }
-    __syncthreads();
-    for (unsigned r = ty; r < AXIS_TILE; r += 4) {
-        const size_t i = i0 + r, j = j0 + tx;
-        if (i < inner && j < len) yb[i * len + j] = tile[tx][r];
-    }
-}
-
-// The same with 16-byte vectors in and out, where rows of x (stride inner) and of y (stride len)
-// start on the 16-byte grid: a tile row is 64 / VN vectors, 256 / (64 / VN) rows per pass.
-template <typename T>
-__global__ __launch_bounds__(256) void axis_transpose_vec_kernel(const T *x, T *y, size_t len, size_t inner, size_t ntile_i,
-                                                                 size_t ntile_j) {
-    typedef typename Vec16<T>::type V;
-    constexpr int VN = Vec16<T>::N;
-    constexpr unsigned VPR = AXIS_TILE / VN, RPP = 256 / VPR;
-    __shared__ T tile[AXIS_TILE][AXIS_TILE + 1];
-    size_t b = blockIdx.x;
-    const size_t jt = b % ntile_j;
-    b /= ntile_j;
-    const size_t it = b % ntile_i, o = b / ntile_i;
-    const size_t i0 = it * AXIS_TILE, j0 = jt * AXIS_TILE;
-    const T *xb = x + o * len * inner;
-    T *yb = y + o * inner * len;
-    const unsigned cv = threadIdx.x % VPR, r0 = threadIdx.x / VPR;
-    for (unsigned r = r0; r < AXIS_TILE; r += RPP) {
-        const size_t j = j0 + r, i = i0 + cv * VN;
-        if (j < len && i < inner) {  // inner % VN == 0: the whole vector is inside the row
-            const V v = __builtin_nontemporal_load(reinterpret_cast<const V *>(xb + j * inner + i));
-#pragma unroll
-            for (int e = 0; e < VN; ++e) tile[r][cv * VN + e] = v[e];
-        }
-    }
-    __syncthreads();
-    for (unsigned r = r0; r < AXIS_TILE; r += RPP) {
-        const size_t i = i0 + r, j = j0 + cv * VN;
-        if (i < inner && j < len) {  // len % VN == 0
-            V v;
-#pragma unroll
-            for (int e = 0; e < VN; ++e) v[e] = tile[cv * VN + e][r];
-            *reinterpret_cast<V *>(yb + i * len + j) = v;
-        }
-    }
-}
-
-template <typename T>
-nrk_status axis_transpose(const T *x, T *y, size_t outer, size_t len, size_t inner, hipStream_t stream) {
-    constexpr size_t VN = Vec16<T>::N;
-    const size_t ni = ceil_div(inner, (size_t)AXIS_TILE), nj = ceil_div(len, (size_t)AXIS_TILE);
-    const size_t nb = outer * ni * nj;
-    if (nb == 0) return NRK_OK;
-    if (nb > 0x7fffffffu) return fail(NRK_ERR_INVALID_ARGUMENT, "(%zu, %zu, %zu) has too many tiles", outer, len, inner);
-    const bool vec = (uintptr_t)x % 16 == 0 && (uintptr_t)y % 16 == 0 && inner % VN == 0 && len % VN == 0;
-    if (vec)
-        hipLaunchKernelGGL(axis_transpose_vec_kernel<T>, dim3((unsigned)nb), dim3(256), 0, stream, x, y, len, inner, ni, nj);
-    else
-        hipLaunchKernelGGL(axis_transpose_kernel<T>, dim3((unsigned)nb), dim3(256), 0, stream, x, y, len, inner, ni, nj);
-    NRK_HIP(hipGetLastError());
-    return NRK_OK;
-}
-
 // Sums and means along a strided axis with few outputs, reading the data once. Workgroup
 // (o, column tile, range) walks rows [range start, + rlen) of 64 adjacent outputs in tiles of 64
 // rows staged through LDS (16-byte loads where inner and x allow); thread t sums rows
@@ -163,11 +83,41 @@ nrk_status axis_transpose(const T *x, T *y, size_t outer, size_t len, size_t inn
 // partial goes to part[output][range] (contiguous per output, for the finish's lanes). The finish
 // adds each output's partials: lane l takes ranges l, l + 64, ... in order, then the butterfly.
 constexpr int AXS_C = 64, AXS_R = 64;
+
+// Stage rows jt .. jt + 63 (those below j1) of columns c0 .. c0 + 63 (those below inner) into the
+// LDS tile, zeros elsewhere. VEC (x 16-byte aligned, inner % VN == 0): thread t loads vector
+// t % (64 / VN) of rows t / (64 / VN) + (256 VN / 64) k; otherwise column t % 64 of rows t / 64 + 4 k.
+template <typename T, bool VEC>
+__device__ __forceinline__ void axis_stage(T (&tile)[AXS_R][AXS_C + 1], const T *xb, size_t jt, size_t j1, size_t c0,
+                                           size_t inner, unsigned t) {
+    typedef typename Vec16<T>::type V;
+    constexpr int VN = Vec16<T>::N;
+    if constexpr (VEC) {
+        constexpr unsigned VPR = AXS_C / VN, RPP = 256 / VPR;
+        const unsigned cv = t % VPR;
+        for (unsigned rr = t / VPR; rr < AXS_R; rr += RPP) {
+            const size_t j = jt + rr, c = c0 + cv * VN;
+            if (j < j1 && c < inner) {  // inner % VN == 0: the whole vector is in the row
+                const V v = __builtin_nontemporal_load(reinterpret_cast<const V *>(xb + j * inner + c));
+#pragma unroll
+                for (int e = 0; e < VN; ++e) tile[rr][cv * VN + e] = v[e];
+            } else {
+#pragma unroll
+                for (int e = 0; e < VN; ++e) tile[rr][cv * VN + e] = (T)0;
+            }
+        }
+    } else {
+        const unsigned col = t % AXS_C;
+        for (unsigned rr = t / AXS_C; rr < AXS_R; rr += 256 / AXS_C) {
+            const size_t j = jt + rr, c = c0 + col;
+            tile[rr][col] = (j < j1 && c < inner) ? xb[j * inner + c] : (T)0;
+        }
+    }
+}
+
 template <typename T, bool VEC>
 __global__ __launch_bounds__(256) void axis_sum_tile_kernel(const T *x, size_t len, size_t inner, size_t nct, size_t nrange,
                                                             size_t rlen, size_t nout, double *phi, double *plo) {
-    typedef typename Vec16<T>::type V;
-    constexpr int VN = Vec16<T>::N;
     __shared__ T tile[AXS_R][AXS_C + 1];
     __shared__ double qhi[4][AXS_C], qlo[4][AXS_C];
     size_t b = blockIdx.x;
@@ -179,26 +129,7 @@ __global__ __launch_bounds__(256) void axis_sum_tile_kernel(const T *x, size_t l
     const unsigned t = threadIdx.x, col = t % AXS_C, quarter = t / AXS_C;
     double hi = 0.0, lo = 0.0;
     for (size_t jt = j0; jt < j1; jt += AXS_R) {
-        if constexpr (VEC) {  // thread t: vector t % 16 of rows t / 16 + 16 k
-            constexpr unsigned VPR = AXS_C / VN, RPP = 256 / VPR;
-            const unsigned cv = t % VPR;
-            for (unsigned rr = t / VPR; rr < AXS_R; rr += RPP) {
-                const size_t j = jt + rr, c = c0 + cv * VN;
-                if (j < j1 && c < inner) {  // inner % VN == 0: the whole vector is in the row
-                    const V v = __builtin_nontemporal_load(reinterpret_cast<const V *>(xb + j * inner + c));
-#pragma unroll
-                    for (int e = 0; e < VN; ++e) tile[rr][cv * VN + e] = v[e];
-                } else {
-#pragma unroll
-                    for (int e = 0; e < VN; ++e) tile[rr][cv * VN + e] = (T)0;
-                }
-            }
-        } else {  // thread t: column t % 64 of rows t / 64 + 4 k
-            for (unsigned rr = t / AXS_C; rr < AXS_R; rr += 256 / AXS_C) {
-                const size_t j = jt + rr, c = c0 + col;
-                tile[rr][col] = (j < j1 && c < inner) ? xb[j * inner + c] : (T)0;
-            }
-        }
+        axis_stage<T, VEC>(tile, xb, jt, j1, c0, inner, t);
         __syncthreads();
         // 16 rows of this thread's quarter, pairwise (zeros past the range or the row are exact)
         const T st = pairwise<16, 0, T>([&](int i) { return tile[quarter * 16 + i][col]; });
@@ -281,8 +212,6 @@ nrk_status axis_sum(const T *x, size_t outer, size_t len, size_t inner, bool mea
 template <typename T, bool VEC, bool MX>
 __global__ __launch_bounds__(256) void axis_ext_tile_kernel(const T *x, size_t len, size_t inner, size_t nct, size_t nrange,
                                                             size_t rlen, T *pv, uint64_t *pi, unsigned *pf) {
-    typedef typename Vec16<T>::type V;
-    constexpr int VN = Vec16<T>::N;
     __shared__ T tile[AXS_R][AXS_C + 1];
     __shared__ T qv[4][AXS_C];
     __shared__ uint64_t qi[4][AXS_C];
@@ -298,23 +227,7 @@ __global__ __launch_bounds__(256) void axis_ext_tile_kernel(const T *x, size_t l
     uint64_t bj = NO_INDEX;
     unsigned flags = 0;  // bit 0: a NaN; bit 1: row 0 is NaN
     for (size_t jt = j0; jt < j1; jt += AXS_R) {
-        if constexpr (VEC) {
-            constexpr unsigned VPR = AXS_C / VN, RPP = 256 / VPR;
-            const unsigned cv = t % VPR;
-            for (unsigned rr = t / VPR; rr < AXS_R; rr += RPP) {
-                const size_t j = jt + rr, c = c0 + cv * VN;
-                if (j < j1 && c < inner) {
-                    const V v = __builtin_nontemporal_load(reinterpret_cast<const V *>(xb + j * inner + c));
-#pragma unroll
-                    for (int e = 0; e < VN; ++e) tile[rr][cv * VN + e] = v[e];
-                }
-            }
-        } else {
-            for (unsigned rr = t / AXS_C; rr < AXS_R; rr += 256 / AXS_C) {
-                const size_t j = jt + rr, c = c0 + col;
-                if (j < j1 && c < inner) tile[rr][col] = xb[j * inner + c];
-            }
-        }
+        axis_stage<T, VEC>(tile, xb, jt, j1, c0, inner, t);
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
@@ -425,12 +338,123 @@ nrk_status axis_extrema(int op, const T *x, size_t outer, size_t len, size_t inn
     return NRK_OK;
 }
 
+// Variances along a strided axis with few outputs, the same tiles and ranges: thread t turns its
+// quarter's 16 rows of a tile into a record (count, pairwise sum, M2 about the block's mean, in
+// binary64) and merges it into its running record (Chan, Golub & LeVeque, double-word sums); the
+// quarters, the ranges (lane-strided per output) and the finish's lanes merge the same way.
+template <typename T, bool VEC>
+__global__ __launch_bounds__(256) void axis_var_tile_kernel(const T *x, size_t len, size_t inner, size_t nct, size_t nrange,
+                                                            size_t rlen, double *pn, double *phi, double *plo, double *pm2) {
+    __shared__ T tile[AXS_R][AXS_C + 1];
+    __shared__ double qn[4][AXS_C], qhi[4][AXS_C], qlo[4][AXS_C], qm2[4][AXS_C];
+    size_t b = blockIdx.x;
+    const size_t r = b % nrange;
+    b /= nrange;
+    const size_t ct = b % nct, o = b / nct;
+    const size_t c0 = ct * AXS_C, j0 = r * rlen, j1 = min(len, j0 + rlen);
+    const T *xb = x + o * len * inner;
+    const unsigned t = threadIdx.x, col = t % AXS_C, quarter = t / AXS_C;
+    Part a = part_empty();
+    for (size_t jt = j0; jt < j1; jt += AXS_R) {
+        axis_stage<T, VEC>(tile, xb, jt, j1, c0, inner, t);
+        __syncthreads();
+        const size_t q0 = jt + quarter * 16;
+        const int m = q0 >= j1 ? 0 : (int)min((size_t)16, j1 - q0);
+        if (m > 0) {
+            Part bk = part_empty();
+            const T st = pairwise<16, 0, T>([&](int i) { return tile[quarter * 16 + i][col]; });
+            const double c = (double)st / m;
+            double q = 0.0;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const double d = (double)tile[quarter * 16 + i][col] - c;
+                q = i < m ? fma(d, d, q) : q;
+            }
+            bk.n = m;
+            bk.hi = st;
+            bk.m2 = q;
+            part_merge<F_M2>(a, bk);
+        }
+        __syncthreads();
+    }
+    qn[quarter][col] = a.n;
+    qhi[quarter][col] = a.hi;
+    qlo[quarter][col] = a.lo;
+    qm2[quarter][col] = a.m2;
+    __syncthreads();
+    if (t < AXS_C && c0 + t < inner) {
+        Part v = part_empty();
+        for (int q = 0; q < 4; ++q) {
+            Part w = part_empty();
+            w.n = qn[q][t];
+            w.hi = qhi[q][t];
+            w.lo = qlo[q][t];
+            w.m2 = qm2[q][t];
+            part_merge<F_M2>(v, w);
+        }
+        const size_t at = (o * inner + c0 + t) * nrange + r;
+        pn[at] = v.n;
+        phi[at] = v.hi;
+        plo[at] = v.lo;
+        pm2[at] = v.m2;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void axis_var_finish_kernel(const double *pn, const double *phi, const double *plo,
+                                                              const double *pm2, size_t nout, size_t nrange, double ddof, T *out) {
+    const size_t w = (blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6;
+    const unsigned lane = threadIdx.x & 63;
+    if (w >= nout) return;  // whole wavefronts
+    Part a = part_empty();
+    for (size_t r = lane; r < nrange; r += 64) {
+        Part b = part_empty();
+        b.n = pn[w * nrange + r];
+        b.hi = phi[w * nrange + r];
+        b.lo = plo[w * nrange + r];
+        b.m2 = pm2[w * nrange + r];
+        part_merge<F_M2>(a, b);
+    }
+    wave_merge<F_M2>(a);
+    if (lane == 0) out[w] = (T)(a.m2 / (a.n - ddof));
+}
+
+template <typename T>
+nrk_status axis_var(const T *x, size_t outer, size_t len, size_t inner, double ddof, T *out, hipStream_t stream) {
+    constexpr size_t VN = Vec16<T>::N;
+    const size_t nout = outer * inner;
+    if (nout == 0) return NRK_OK;
+    const bool vec = (uintptr_t)x % 16 == 0 && inner % VN == 0;
+    const size_t nct = ceil_div(inner, (size_t)AXS_C);
+    const size_t want = std::max<size_t>(1, 2048 / std::max<size_t>(1, outer * nct));
+    const size_t nrange = std::max<size_t>(1, std::min(want, ceil_div(len, (size_t)AXS_R)));
+    const size_t rlen = ceil_div(ceil_div(len, nrange), (size_t)AXS_R) * AXS_R;
+    const size_t nr = ceil_div(len, rlen);
+    const size_t nb = outer * nct * nr;
+    if (nb > 0x7fffffffu) return fail(NRK_ERR_INVALID_ARGUMENT, "(%zu, %zu, %zu) has too many tiles", outer, len, inner);
+    void *ws = nullptr;
+    const size_t np = nout * nr;
+    NRK_TRY(get_workspace(stream, 4 * np * sizeof(double), &ws));
+    double *pn = (double *)ws, *phi = pn + np, *plo = phi + np, *pm2 = plo + np;
+    if (vec)
+        hipLaunchKernelGGL((axis_var_tile_kernel<T, true>), dim3((unsigned)nb), dim3(256), 0, stream, x, len, inner, nct, nr, rlen,
+                           pn, phi, plo, pm2);
+    else
+        hipLaunchKernelGGL((axis_var_tile_kernel<T, false>), dim3((unsigned)nb), dim3(256), 0, stream, x, len, inner, nct, nr, rlen,
+                           pn, phi, plo, pm2);
+    NRK_HIP(hipGetLastError());
+    hipLaunchKernelGGL(axis_var_finish_kernel<T>, dim3((unsigned)ceil_div(nout * 64, (size_t)256)), dim3(256), 0, stream, pn, phi,
+                       plo, pm2, nout, nr, ddof, out);
+    NRK_HIP(hipGetLastError());
+    return NRK_OK;
+}
+
+template nrk_status axis_var<float>(const float *, size_t, size_t, size_t, double, float *, hipStream_t);
+template nrk_status axis_var<double>(const double *, size_t, size_t, size_t, double, double *, hipStream_t);
 template nrk_status axis_extrema<float>(int, const float *, size_t, size_t, size_t, void *, hipStream_t);
 template nrk_status axis_extrema<double>(int, const double *, size_t, size_t, size_t, void *, hipStream_t);
 template nrk_status axis_sum<float>(const float *, size_t, size_t, size_t, bool, float *, hipStream_t);
 template nrk_status axis_sum<double>(const double *, size_t, size_t, size_t, bool, double *, hipStream_t);
-template nrk_status axis_transpose<float>(const float *, float *, size_t, size_t, size_t, hipStream_t);
-template nrk_status axis_transpose<double>(const double *, double *, size_t, size_t, size_t, hipStream_t);
 template nrk_status strided_reduce<float>(int, const float *, size_t, size_t, size_t, double, void *, hipStream_t);
 template nrk_status strided_reduce<double>(int, const double *, size_t, size_t, size_t, double, void *, hipStream_t);
 
