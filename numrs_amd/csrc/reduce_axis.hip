@@ -77,48 +77,30 @@ nrk_status strided_reduce(int op, const T *x, size_t outer, size_t len, size_t i
 
 // Sums and means along a strided axis with few outputs, reading the data once. Workgroup
 // (o, column tile, range) walks rows [range start, + rlen) of 64 adjacent outputs in tiles of 64
-// rows staged through LDS (16-byte loads where inner and x allow); thread t sums rows
-// 16 (t / 64) .. + 15 of every tile for output column t % 64 (pairwise in T, then binary64; a
-// double word for binary64 elements), the four row quarters are merged in LDS, and the range's
-// partial goes to part[output][range] (contiguous per output, for the finish's lanes). The finish
-// adds each output's partials: lane l takes ranges l, l + 64, ... in order, then the butterfly.
+// rows; thread t sums rows 16 (t / 64) .. + 15 of every tile for output column t % 64 (pairwise in
+// T, then binary64; a double word for binary64 elements), loading them straight from HBM (the
+// wavefront's lanes read one row's 64 adjacent elements per load, no LDS tile, no barrier). The
+// four row quarters are merged in LDS, and the range's partial goes to part[output][range]
+// (contiguous per output, for the finish's lanes). The finish adds each output's partials: lane l
+// takes ranges l, l + 64, ... in order, then the butterfly.
 constexpr int AXS_C = 64, AXS_R = 64;
 
-// Stage rows jt .. jt + 63 (those below j1) of columns c0 .. c0 + 63 (those below inner) into the
-// LDS tile, zeros elsewhere. VEC (x 16-byte aligned, inner % VN == 0): thread t loads vector
-// t % (64 / VN) of rows t / (64 / VN) + (256 VN / 64) k; otherwise column t % 64 of rows t / 64 + 4 k.
-template <typename T, bool VEC>
-__device__ __forceinline__ void axis_stage(T (&tile)[AXS_R][AXS_C + 1], const T *xb, size_t jt, size_t j1, size_t c0,
-                                           size_t inner, unsigned t) {
-    typedef typename Vec16<T>::type V;
-    constexpr int VN = Vec16<T>::N;
-    if constexpr (VEC) {
-        constexpr unsigned VPR = AXS_C / VN, RPP = 256 / VPR;
-        const unsigned cv = t % VPR;
-        for (unsigned rr = t / VPR; rr < AXS_R; rr += RPP) {
-            const size_t j = jt + rr, c = c0 + cv * VN;
-            if (j < j1 && c < inner) {  // inner % VN == 0: the whole vector is in the row
-                const V v = __builtin_nontemporal_load(reinterpret_cast<const V *>(xb + j * inner + c));
+// Rows q0 .. q0 + 15 of column c, zeros past j1 or past the row, straight from HBM: the lanes of a
+// wavefront take 64 adjacent columns, so each load is one row's 64 contiguous elements, and the 16
+// loads are in flight together.
+template <typename T>
+__device__ __forceinline__ void axis_rows16(const T *xb, size_t q0, size_t j1, size_t c, size_t inner, T (&v)[16]) {
+    const bool in_row = c < inner;
 #pragma unroll
-                for (int e = 0; e < VN; ++e) tile[rr][cv * VN + e] = v[e];
-            } else {
-#pragma unroll
-                for (int e = 0; e < VN; ++e) tile[rr][cv * VN + e] = (T)0;
-            }
-        }
-    } else {
-        const unsigned col = t % AXS_C;
-        for (unsigned rr = t / AXS_C; rr < AXS_R; rr += 256 / AXS_C) {
-            const size_t j = jt + rr, c = c0 + col;
-            tile[rr][col] = (j < j1 && c < inner) ? xb[j * inner + c] : (T)0;
-        }
+    for (int i = 0; i < 16; ++i) {
+        const size_t j = q0 + i;
+        v[i] = (in_row && j < j1) ? __builtin_nontemporal_load(xb + j * inner + c) : (T)0;
     }
 }
 
-template <typename T, bool VEC>
+template <typename T>
 __global__ __launch_bounds__(256) void axis_sum_tile_kernel(const T *x, size_t len, size_t inner, size_t nct, size_t nrange,
                                                             size_t rlen, size_t nout, double *phi, double *plo) {
-    __shared__ T tile[AXS_R][AXS_C + 1];
     __shared__ double qhi[4][AXS_C], qlo[4][AXS_C];
     size_t b = blockIdx.x;
     const size_t r = b % nrange;
@@ -129,10 +111,10 @@ __global__ __launch_bounds__(256) void axis_sum_tile_kernel(const T *x, size_t l
     const unsigned t = threadIdx.x, col = t % AXS_C, quarter = t / AXS_C;
     double hi = 0.0, lo = 0.0;
     for (size_t jt = j0; jt < j1; jt += AXS_R) {
-        axis_stage<T, VEC>(tile, xb, jt, j1, c0, inner, t);
-        __syncthreads();
+        T v[16];
+        axis_rows16(xb, jt + quarter * 16, j1, c0 + col, inner, v);
         // 16 rows of this thread's quarter, pairwise (zeros past the range or the row are exact)
-        const T st = pairwise<16, 0, T>([&](int i) { return tile[quarter * 16 + i][col]; });
+        const T st = pairwise<16, 0, T>([&](int i) { return v[i]; });
         if constexpr (sizeof(T) == 8) {
             double s2, e2;
             two_sum(hi, (double)st, s2, e2);
@@ -140,7 +122,6 @@ __global__ __launch_bounds__(256) void axis_sum_tile_kernel(const T *x, size_t l
         } else {
             hi += (double)st;
         }
-        __syncthreads();
     }
     qhi[quarter][col] = hi;
     qlo[quarter][col] = lo;
@@ -176,10 +157,8 @@ __global__ __launch_bounds__(256) void axis_sum_finish_kernel(const double *phi,
 
 template <typename T>
 nrk_status axis_sum(const T *x, size_t outer, size_t len, size_t inner, bool mean, T *out, hipStream_t stream) {
-    constexpr size_t VN = Vec16<T>::N;
     const size_t nout = outer * inner;
     if (nout == 0) return NRK_OK;
-    const bool vec = (uintptr_t)x % 16 == 0 && inner % VN == 0;
     const size_t nct = ceil_div(inner, (size_t)AXS_C);
     // ranges: about 2048 workgroups in all, at least one tile of rows each; from the shape only
     const size_t want = std::max<size_t>(1, 2048 / std::max<size_t>(1, outer * nct));
@@ -191,12 +170,8 @@ nrk_status axis_sum(const T *x, size_t outer, size_t len, size_t inner, bool mea
     void *ws = nullptr;
     NRK_TRY(get_workspace(stream, 2 * nout * nr * sizeof(double), &ws));
     double *phi = (double *)ws, *plo = phi + nout * nr;
-    if (vec)
-        hipLaunchKernelGGL((axis_sum_tile_kernel<T, true>), dim3((unsigned)nb), dim3(256), 0, stream, x, len, inner, nct, nr, rlen,
-                           nout, phi, plo);
-    else
-        hipLaunchKernelGGL((axis_sum_tile_kernel<T, false>), dim3((unsigned)nb), dim3(256), 0, stream, x, len, inner, nct, nr, rlen,
-                           nout, phi, plo);
+    hipLaunchKernelGGL(axis_sum_tile_kernel<T>, dim3((unsigned)nb), dim3(256), 0, stream, x, len, inner, nct, nr, rlen, nout, phi,
+                       plo);
     NRK_HIP(hipGetLastError());
     hipLaunchKernelGGL(axis_sum_finish_kernel<T>, dim3((unsigned)ceil_div(nout * 64, (size_t)256)), dim3(256), 0, stream, phi, plo,
                        nout, nr, (double)len, mean ? 1 : 0, out);
@@ -209,10 +184,9 @@ nrk_status axis_sum(const T *x, size_t outer, size_t len, size_t inner, bool mea
 // never winning but flagged (and whether row 0 is NaN); quarters and ranges merge with the lower
 // row winning ties, so the result is the sequential fold's: max / min NaN where any element is,
 // argmax / argmin 0 where the first element is NaN or no element wins (all -inf / +inf or NaN).
-template <typename T, bool VEC, bool MX>
+template <typename T, bool MX>
 __global__ __launch_bounds__(256) void axis_ext_tile_kernel(const T *x, size_t len, size_t inner, size_t nct, size_t nrange,
                                                             size_t rlen, T *pv, uint64_t *pi, unsigned *pf) {
-    __shared__ T tile[AXS_R][AXS_C + 1];
     __shared__ T qv[4][AXS_C];
     __shared__ uint64_t qi[4][AXS_C];
     __shared__ unsigned qf[4][AXS_C];
@@ -227,13 +201,13 @@ __global__ __launch_bounds__(256) void axis_ext_tile_kernel(const T *x, size_t l
     uint64_t bj = NO_INDEX;
     unsigned flags = 0;  // bit 0: a NaN; bit 1: row 0 is NaN
     for (size_t jt = j0; jt < j1; jt += AXS_R) {
-        axis_stage<T, VEC>(tile, xb, jt, j1, c0, inner, t);
-        __syncthreads();
+        T vv[16];
+        axis_rows16(xb, jt + quarter * 16, j1, c0 + col, inner, vv);
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
             const size_t j = jt + quarter * 16 + i;
             if (j < j1 && c0 + col < inner) {
-                const T v = tile[quarter * 16 + i][col];
+                const T v = vv[i];
                 if (v != v) {
                     flags |= j == 0 ? 3u : 1u;
                 } else if (MX ? v > bv : v < bv) {
@@ -242,7 +216,6 @@ __global__ __launch_bounds__(256) void axis_ext_tile_kernel(const T *x, size_t l
                 }
             }
         }
-        __syncthreads();
     }
     qv[quarter][col] = bv;
     qi[quarter][col] = bj;
@@ -304,10 +277,8 @@ __global__ __launch_bounds__(256) void axis_ext_finish_kernel(const T *pv, const
 
 template <typename T>
 nrk_status axis_extrema(int op, const T *x, size_t outer, size_t len, size_t inner, void *out, hipStream_t stream) {
-    constexpr size_t VN = Vec16<T>::N;
     const size_t nout = outer * inner;
     if (nout == 0) return NRK_OK;
-    const bool vec = (uintptr_t)x % 16 == 0 && inner % VN == 0;
     const bool mx = op == NRK_AXIS_MAX || op == NRK_AXIS_ARGMAX;
     const int want_index = (op == NRK_AXIS_ARGMAX || op == NRK_AXIS_ARGMIN) ? 1 : 0;
     const size_t nct = ceil_div(inner, (size_t)AXS_C);
@@ -324,16 +295,13 @@ nrk_status axis_extrema(int op, const T *x, size_t outer, size_t len, size_t inn
     T *pv = (T *)(pi + np);
     unsigned *pf = (unsigned *)(pv + np);
     const dim3 g((unsigned)nb), bl(256), gf((unsigned)ceil_div(nout * 64, (size_t)256));
-#define NRK_EXT(V, M)                                                                                                   \
-    do {                                                                                                                \
-        hipLaunchKernelGGL((axis_ext_tile_kernel<T, V, M>), g, bl, 0, stream, x, len, inner, nct, nr, rlen, pv, pi, pf); \
-        hipLaunchKernelGGL((axis_ext_finish_kernel<T, M>), gf, bl, 0, stream, pv, pi, pf, nout, nr, want_index, out);    \
-    } while (0)
-    if (vec && mx) NRK_EXT(true, true);
-    else if (vec) NRK_EXT(true, false);
-    else if (mx) NRK_EXT(false, true);
-    else NRK_EXT(false, false);
-#undef NRK_EXT
+    if (mx) {
+        hipLaunchKernelGGL((axis_ext_tile_kernel<T, true>), g, bl, 0, stream, x, len, inner, nct, nr, rlen, pv, pi, pf);
+        hipLaunchKernelGGL((axis_ext_finish_kernel<T, true>), gf, bl, 0, stream, pv, pi, pf, nout, nr, want_index, out);
+    } else {
+        hipLaunchKernelGGL((axis_ext_tile_kernel<T, false>), g, bl, 0, stream, x, len, inner, nct, nr, rlen, pv, pi, pf);
+        hipLaunchKernelGGL((axis_ext_finish_kernel<T, false>), gf, bl, 0, stream, pv, pi, pf, nout, nr, want_index, out);
+    }
     NRK_HIP(hipGetLastError());
     return NRK_OK;
 }
@@ -342,10 +310,9 @@ nrk_status axis_extrema(int op, const T *x, size_t outer, size_t len, size_t inn
 // quarter's 16 rows of a tile into a record (count, pairwise sum, M2 about the block's mean, in
 // binary64) and merges it into its running record (Chan, Golub & LeVeque, double-word sums); the
 // quarters, the ranges (lane-strided per output) and the finish's lanes merge the same way.
-template <typename T, bool VEC>
+template <typename T>
 __global__ __launch_bounds__(256) void axis_var_tile_kernel(const T *x, size_t len, size_t inner, size_t nct, size_t nrange,
                                                             size_t rlen, double *pn, double *phi, double *plo, double *pm2) {
-    __shared__ T tile[AXS_R][AXS_C + 1];
     __shared__ double qn[4][AXS_C], qhi[4][AXS_C], qlo[4][AXS_C], qm2[4][AXS_C];
     size_t b = blockIdx.x;
     const size_t r = b % nrange;
@@ -356,18 +323,18 @@ __global__ __launch_bounds__(256) void axis_var_tile_kernel(const T *x, size_t l
     const unsigned t = threadIdx.x, col = t % AXS_C, quarter = t / AXS_C;
     Part a = part_empty();
     for (size_t jt = j0; jt < j1; jt += AXS_R) {
-        axis_stage<T, VEC>(tile, xb, jt, j1, c0, inner, t);
-        __syncthreads();
+        T v[16];
+        axis_rows16(xb, jt + quarter * 16, j1, c0 + col, inner, v);
         const size_t q0 = jt + quarter * 16;
         const int m = q0 >= j1 ? 0 : (int)min((size_t)16, j1 - q0);
         if (m > 0) {
             Part bk = part_empty();
-            const T st = pairwise<16, 0, T>([&](int i) { return tile[quarter * 16 + i][col]; });
+            const T st = pairwise<16, 0, T>([&](int i) { return v[i]; });
             const double c = (double)st / m;
             double q = 0.0;
 #pragma unroll
             for (int i = 0; i < 16; ++i) {
-                const double d = (double)tile[quarter * 16 + i][col] - c;
+                const double d = (double)v[i] - c;
                 q = i < m ? fma(d, d, q) : q;
             }
             bk.n = m;
@@ -375,7 +342,6 @@ __global__ __launch_bounds__(256) void axis_var_tile_kernel(const T *x, size_t l
             bk.m2 = q;
             part_merge<F_M2>(a, bk);
         }
-        __syncthreads();
     }
     qn[quarter][col] = a.n;
     qhi[quarter][col] = a.hi;
@@ -421,10 +387,8 @@ __global__ __launch_bounds__(256) void axis_var_finish_kernel(const double *pn, 
 
 template <typename T>
 nrk_status axis_var(const T *x, size_t outer, size_t len, size_t inner, double ddof, T *out, hipStream_t stream) {
-    constexpr size_t VN = Vec16<T>::N;
     const size_t nout = outer * inner;
     if (nout == 0) return NRK_OK;
-    const bool vec = (uintptr_t)x % 16 == 0 && inner % VN == 0;
     const size_t nct = ceil_div(inner, (size_t)AXS_C);
     const size_t want = std::max<size_t>(1, 2048 / std::max<size_t>(1, outer * nct));
     const size_t nrange = std::max<size_t>(1, std::min(want, ceil_div(len, (size_t)AXS_R)));
@@ -436,12 +400,8 @@ nrk_status axis_var(const T *x, size_t outer, size_t len, size_t inner, double d
     const size_t np = nout * nr;
     NRK_TRY(get_workspace(stream, 4 * np * sizeof(double), &ws));
     double *pn = (double *)ws, *phi = pn + np, *plo = phi + np, *pm2 = plo + np;
-    if (vec)
-        hipLaunchKernelGGL((axis_var_tile_kernel<T, true>), dim3((unsigned)nb), dim3(256), 0, stream, x, len, inner, nct, nr, rlen,
-                           pn, phi, plo, pm2);
-    else
-        hipLaunchKernelGGL((axis_var_tile_kernel<T, false>), dim3((unsigned)nb), dim3(256), 0, stream, x, len, inner, nct, nr, rlen,
-                           pn, phi, plo, pm2);
+    hipLaunchKernelGGL(axis_var_tile_kernel<T>, dim3((unsigned)nb), dim3(256), 0, stream, x, len, inner, nct, nr, rlen, pn, phi,
+                       plo, pm2);
     NRK_HIP(hipGetLastError());
     hipLaunchKernelGGL(axis_var_finish_kernel<T>, dim3((unsigned)ceil_div(nout * 64, (size_t)256)), dim3(256), 0, stream, pn, phi,
                        plo, pm2, nout, nr, ddof, out);
