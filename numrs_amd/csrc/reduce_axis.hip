@@ -85,6 +85,25 @@ nrk_status strided_reduce(int op, const T *x, size_t outer, size_t len, size_t i
 // takes ranges l, l + 64, ... in order, then the butterfly.
 constexpr int AXS_C = 64, AXS_R = 64;
 
+// Ranges of the axis for the column-tile kernels: about `groups` workgroups in all, at least one
+// 64-row tile each, from the shape only (so the grouping of a sum is the same on every call).
+// Fewer, longer ranges win while the chip stays full: per-op targets measured on 8192 x 8192,
+// 2^20 x 64, 4096 x 1024 (x 16) and 100000 x 671 (sums 512; extrema 1024; variances 1024 binary32,
+// 512 binary64; 2048 cost 5-20 %, 256 up to 2x).
+struct AxisRanges {
+    size_t nct, rlen, nr, nb;
+};
+static AxisRanges axis_ranges(size_t outer, size_t len, size_t inner, size_t groups) {
+    AxisRanges a;
+    a.nct = ceil_div(inner, (size_t)AXS_C);
+    const size_t want = std::max<size_t>(1, groups / std::max<size_t>(1, outer * a.nct));
+    const size_t nrange = std::max<size_t>(1, std::min(want, ceil_div(len, (size_t)AXS_R)));
+    a.rlen = ceil_div(ceil_div(len, nrange), (size_t)AXS_R) * AXS_R;
+    a.nr = ceil_div(len, a.rlen);  // ranges that hold rows
+    a.nb = outer * a.nct * a.nr;
+    return a;
+}
+
 // Rows q0 .. q0 + 15 of column c, zeros past j1 or past the row, straight from HBM: the lanes of a
 // wavefront take 64 adjacent columns, so each load is one row's 64 contiguous elements, and the 16
 // loads are in flight together.
@@ -159,13 +178,8 @@ template <typename T>
 nrk_status axis_sum(const T *x, size_t outer, size_t len, size_t inner, bool mean, T *out, hipStream_t stream) {
     const size_t nout = outer * inner;
     if (nout == 0) return NRK_OK;
-    const size_t nct = ceil_div(inner, (size_t)AXS_C);
-    // ranges: about 2048 workgroups in all, at least one tile of rows each; from the shape only
-    const size_t want = std::max<size_t>(1, 2048 / std::max<size_t>(1, outer * nct));
-    const size_t nrange = std::max<size_t>(1, std::min(want, ceil_div(len, (size_t)AXS_R)));
-    const size_t rlen = ceil_div(ceil_div(len, nrange), (size_t)AXS_R) * AXS_R;
-    const size_t nr = ceil_div(len, rlen);  // ranges that hold rows
-    const size_t nb = outer * nct * nr;
+    const AxisRanges ar = axis_ranges(outer, len, inner, 512);
+    const size_t nct = ar.nct, rlen = ar.rlen, nr = ar.nr, nb = ar.nb;
     if (nb > 0x7fffffffu) return fail(NRK_ERR_INVALID_ARGUMENT, "(%zu, %zu, %zu) has too many tiles", outer, len, inner);
     void *ws = nullptr;
     NRK_TRY(get_workspace(stream, 2 * nout * nr * sizeof(double), &ws));
@@ -281,12 +295,8 @@ nrk_status axis_extrema(int op, const T *x, size_t outer, size_t len, size_t inn
     if (nout == 0) return NRK_OK;
     const bool mx = op == NRK_AXIS_MAX || op == NRK_AXIS_ARGMAX;
     const int want_index = (op == NRK_AXIS_ARGMAX || op == NRK_AXIS_ARGMIN) ? 1 : 0;
-    const size_t nct = ceil_div(inner, (size_t)AXS_C);
-    const size_t want = std::max<size_t>(1, 2048 / std::max<size_t>(1, outer * nct));
-    const size_t nrange = std::max<size_t>(1, std::min(want, ceil_div(len, (size_t)AXS_R)));
-    const size_t rlen = ceil_div(ceil_div(len, nrange), (size_t)AXS_R) * AXS_R;
-    const size_t nr = ceil_div(len, rlen);
-    const size_t nb = outer * nct * nr;
+    const AxisRanges ar = axis_ranges(outer, len, inner, 1024);
+    const size_t nct = ar.nct, rlen = ar.rlen, nr = ar.nr, nb = ar.nb;
     if (nb > 0x7fffffffu) return fail(NRK_ERR_INVALID_ARGUMENT, "(%zu, %zu, %zu) has too many tiles", outer, len, inner);
     void *ws = nullptr;
     const size_t np = nout * nr;
@@ -389,12 +399,8 @@ template <typename T>
 nrk_status axis_var(const T *x, size_t outer, size_t len, size_t inner, double ddof, T *out, hipStream_t stream) {
     const size_t nout = outer * inner;
     if (nout == 0) return NRK_OK;
-    const size_t nct = ceil_div(inner, (size_t)AXS_C);
-    const size_t want = std::max<size_t>(1, 2048 / std::max<size_t>(1, outer * nct));
-    const size_t nrange = std::max<size_t>(1, std::min(want, ceil_div(len, (size_t)AXS_R)));
-    const size_t rlen = ceil_div(ceil_div(len, nrange), (size_t)AXS_R) * AXS_R;
-    const size_t nr = ceil_div(len, rlen);
-    const size_t nb = outer * nct * nr;
+    const AxisRanges ar = axis_ranges(outer, len, inner, sizeof(T) == 8 ? 512 : 1024);
+    const size_t nct = ar.nct, rlen = ar.rlen, nr = ar.nr, nb = ar.nb;
     if (nb > 0x7fffffffu) return fail(NRK_ERR_INVALID_ARGUMENT, "(%zu, %zu, %zu) has too many tiles", outer, len, inner);
     void *ws = nullptr;
     const size_t np = nout * nr;
