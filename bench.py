@@ -622,7 +622,9 @@ SHAPES = (  # (rows, cols, row results, whole-array results)
 )
 
 
-AXIS0_SHAPES = ((8192, 8192), (1 << 20, 64), (64, 1 << 20))  # (rows, cols): 8192 / 64 outputs transposed, 2^20 direct
+# (rows, cols) reduced along axis 0: 8192 / 64 outputs by column tiles, 2^20 outputs one thread each
+AXIS0_SHAPES = ((8192, 8192), (1 << 20, 64), (64, 1 << 20))
+AXIS0_OPS = (("sum", 0), ("max", 4), ("var", 2))
 
 
 def bench_shapes(device, kernels, call, ReduceOutputs, s, stream, d32, d64):
@@ -630,8 +632,8 @@ def bench_shapes(device, kernels, call, ReduceOutputs, s, stream, d32, d64):
     ops, n_ops = kernels.chain(("mul", 1.5), ("add", -0.25))
     ops_p = C.cast(ops, C.c_void_p)
     out = {"workload": "fused y = 1.5x - 0.25 -> sum/max/argmax/var, row and/or whole-array results, on the first "
-                       "rows*cols elements of the headline x; and sums along axis 0 (nrk_dev_reduce_axis) of the same "
-                       "data; GB/s counts the input bytes once",
+                       "rows*cols elements of the headline x; and sum, max and var along axis 0 (nrk_dev_reduce_axis) of "
+                       "the same data; GB/s counts the input bytes once",
            "hbm_peak_GB/s": HBM_PEAK_GBS}
     for dt, dx, tag in ((np.float32, d32, "f32"), (np.float64, d64, "f64")):
         fn = f"nrk_dev_map_reduce_{tag}"
@@ -663,23 +665,24 @@ def bench_shapes(device, kernels, call, ReduceOutputs, s, stream, d32, d64):
             row[key] = {"us": round(ms * 1e3, 2), "GB/s": round(nbytes / ms / 1e6, 1),
                         "frac_hbm": round(nbytes / ms / 1e6 / HBM_PEAK_GBS, 3)}
             del keep
-        # sums along axis 0 (a strided axis: outer = 1, len = rows, inner = cols) of the same data
+        # reductions along axis 0 (a strided axis: outer = 1, len = rows, inner = cols) of the same data
         dsum = device.DeviceArray((1 << 20,), dt)
         for rows, cols in AXIS0_SHAPES:
-            f = lambda: call(f"nrk_dev_reduce_axis_{tag}", C.c_int32(0), C.c_void_p(dx.ptr), C.c_size_t(1), C.c_size_t(rows),
-                             C.c_size_t(cols), C.c_size_t(0), C.c_void_p(dsum.ptr), C.c_void_p(s))
-            f()
-            stream.synchronize()
-            a, b = device.Event(), device.Event()
-            a.record(s)
-            for _ in range(10):
+            for name, op in AXIS0_OPS:
+                f = lambda: call(f"nrk_dev_reduce_axis_{tag}", C.c_int32(op), C.c_void_p(dx.ptr), C.c_size_t(1),
+                                 C.c_size_t(rows), C.c_size_t(cols), C.c_size_t(0), C.c_void_p(dsum.ptr), C.c_void_p(s))
                 f()
-            b.record(s)
-            stream.synchronize()
-            ms = a.elapsed_ms(b) / 10
-            nbytes = rows * cols * np.dtype(dt).itemsize
-            row[f"{rows}x{cols}:sum_axis0"] = {"us": round(ms * 1e3, 2), "GB/s": round(nbytes / ms / 1e6, 1),
-                                               "frac_hbm": round(nbytes / ms / 1e6 / HBM_PEAK_GBS, 3)}
+                stream.synchronize()
+                a, b = device.Event(), device.Event()
+                a.record(s)
+                for _ in range(10):
+                    f()
+                b.record(s)
+                stream.synchronize()
+                ms = a.elapsed_ms(b) / 10
+                nbytes = rows * cols * np.dtype(dt).itemsize
+                row[f"{rows}x{cols}:{name}_axis0"] = {"us": round(ms * 1e3, 2), "GB/s": round(nbytes / ms / 1e6, 1),
+                                                      "frac_hbm": round(nbytes / ms / 1e6 / HBM_PEAK_GBS, 3)}
         del dsum
         out[tag] = row
         log(f"general shapes {tag}: " + ", ".join(f"{k} {v['us']:.1f} us ({v['frac_hbm']:.2f})" for k, v in row.items()))

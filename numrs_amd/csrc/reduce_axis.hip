@@ -162,7 +162,18 @@ __global__ __launch_bounds__(256) void axis_sum_finish_kernel(const double *phi,
     const unsigned lane = threadIdx.x & 63;
     if (w >= nout) return;  // whole wavefronts
     double h = 0.0, l = 0.0;
-    for (size_t r = lane; r < nrange; r += 64) dw_add(h, l, phi[w * nrange + r], plo[w * nrange + r]);
+    for (size_t r0 = lane; r0 < nrange; r0 += 4 * 64) {  // four loads in flight, added in order
+        double bh[4], bl[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const size_t r = min(r0 + 64 * u, nrange - 1);
+            bh[u] = phi[w * nrange + r];
+            bl[u] = plo[w * nrange + r];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (r0 + 64 * u < nrange) dw_add(h, l, bh[u], bl[u]);
+    }
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) {
         const double bh = __shfl_xor(h, m, 64), bl = __shfl_xor(l, m, 64);
@@ -270,9 +281,24 @@ __global__ __launch_bounds__(256) void axis_ext_finish_kernel(const T *pv, const
             ix = bi;
         }
     };
-    for (size_t r = lane; r < nrange; r += 64) {
-        take(pv[w * nrange + r], pi[w * nrange + r]);
-        f |= pf[w * nrange + r];
+    for (size_t r0 = lane; r0 < nrange; r0 += 4 * 64) {  // four loads in flight, merged in order
+        T bv[4];
+        uint64_t bi[4];
+        unsigned bf[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const size_t r = min(r0 + 64 * u, nrange - 1);
+            bv[u] = pv[w * nrange + r];
+            bi[u] = pi[w * nrange + r];
+            bf[u] = pf[w * nrange + r];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (r0 + 64 * u < nrange) {
+                take(bv[u], bi[u]);
+                f |= bf[u];
+            }
+        }
     }
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) {
@@ -383,13 +409,20 @@ __global__ __launch_bounds__(256) void axis_var_finish_kernel(const double *pn, 
     const unsigned lane = threadIdx.x & 63;
     if (w >= nout) return;  // whole wavefronts
     Part a = part_empty();
-    for (size_t r = lane; r < nrange; r += 64) {
-        Part b = part_empty();
-        b.n = pn[w * nrange + r];
-        b.hi = phi[w * nrange + r];
-        b.lo = plo[w * nrange + r];
-        b.m2 = pm2[w * nrange + r];
-        part_merge<F_M2>(a, b);
+    for (size_t r0 = lane; r0 < nrange; r0 += 4 * 64) {  // four records' loads in flight, merged in order
+        Part b[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const size_t r = min(r0 + 64 * u, nrange - 1);
+            b[u] = part_empty();
+            b[u].n = pn[w * nrange + r];
+            b[u].hi = phi[w * nrange + r];
+            b[u].lo = plo[w * nrange + r];
+            b[u].m2 = pm2[w * nrange + r];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (r0 + 64 * u < nrange) part_merge<F_M2>(a, b[u]);
     }
     wave_merge<F_M2>(a);
     if (lane == 0) out[w] = (T)(a.m2 / (a.n - ddof));
