@@ -10,9 +10,26 @@
 
 namespace nrk {
 
+// f(j, x[j]) for j = j0 .. len - 1 in order, the loads U at a time ahead of the fold (a strided
+// column: one dependent load per element would leave the wavefront waiting on each).
+template <int U, typename T, typename F>
+__device__ __forceinline__ void strided_fold(const T *p, size_t inner, size_t j0, size_t len, F &&f) {
+    size_t j = j0;
+    for (; j + U <= len; j += U) {
+        T v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = p[(j + u) * inner];
+#pragma unroll
+        for (int u = 0; u < U; ++u) f(j + u, v[u]);
+    }
+    for (; j < len; ++j) f(j, p[j * inner]);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void strided_reduce_kernel(int op, const T *x, size_t outer, size_t len, size_t inner,
                                                              T divisor, void *out) {
+    // binary32: 8 loads ahead (64 x 2^20 sums 65 -> 51 us); binary64 one at a time (8 ahead: 90 -> 106 us)
+    constexpr int U = sizeof(T) == 4 ? 8 : 1;
     const size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (id >= outer * inner) return;
     const size_t o = id / inner, i = id % inner;
@@ -22,26 +39,40 @@ __global__ __launch_bounds__(256) void strided_reduce_kernel(int op, const T *x,
         case NRK_AXIS_MEAN:
         case NRK_AXIS_VAR: {
             T s = (T)0;
-            for (size_t j = 0; j < len; ++j) s = rn_add(s, p[j * inner]);
+            strided_fold<U>(p, inner, 0, len, [&](size_t, T v) { s = rn_add(s, v); });
             if (op == NRK_AXIS_SUM) { ((T *)out)[id] = s; break; }
             const T mean = rn_div(s, (T)len);
             if (op == NRK_AXIS_MEAN) { ((T *)out)[id] = mean; break; }
             T q = (T)0;
-            for (size_t j = 0; j < len; ++j) {
-                const T d = rn_sub(p[j * inner], mean);
+            strided_fold<U>(p, inner, 0, len, [&](size_t, T v) {
+                const T d = rn_sub(v, mean);
                 q = rn_add(q, rn_mul(d, d));
-            }
+            });
             ((T *)out)[id] = rn_div(q, divisor);
             break;
         }
         case NRK_AXIS_MAX:
         case NRK_AXIS_MIN: {
+            // the first NaN is the result (the reference stops there); later elements change nothing
             T acc = p[0];
             bool nan = __builtin_isnan(acc);
-            for (size_t j = 1; j < len && !nan; ++j) {
-                const T v = p[j * inner];
-                if (__builtin_isnan(v)) { acc = v; nan = true; break; }
-                if (op == NRK_AXIS_MAX ? (v > acc) : (v < acc)) acc = v;
+            const bool mx = op == NRK_AXIS_MAX;
+            if constexpr (U > 1) {
+                strided_fold<U>(p, inner, 1, len, [&](size_t, T v) {
+                    if (nan) return;
+                    if (__builtin_isnan(v)) {
+                        acc = v;
+                        nan = true;
+                    } else if (mx ? (v > acc) : (v < acc)) {
+                        acc = v;
+                    }
+                });
+            } else {  // one load at a time: stop at the NaN
+                for (size_t j = 1; j < len && !nan; ++j) {
+                    const T v = p[j * inner];
+                    if (__builtin_isnan(v)) { acc = v; nan = true; break; }
+                    if (mx ? (v > acc) : (v < acc)) acc = v;
+                }
             }
             ((T *)out)[id] = acc;
             break;
@@ -50,10 +81,13 @@ __global__ __launch_bounds__(256) void strided_reduce_kernel(int op, const T *x,
         case NRK_AXIS_ARGMIN: {
             T best = p[0];
             uint64_t bi = 0;
-            for (size_t j = 1; j < len; ++j) {
-                const T v = p[j * inner];
-                if (op == NRK_AXIS_ARGMAX ? (v > best) : (v < best)) { best = v; bi = j; }
-            }
+            const bool mx = op == NRK_AXIS_ARGMAX;
+            strided_fold<U>(p, inner, 1, len, [&](size_t j, T v) {
+                if (mx ? (v > best) : (v < best)) {
+                    best = v;
+                    bi = j;
+                }
+            });
             ((uint64_t *)out)[id] = bi;
             break;
         }
