@@ -290,9 +290,9 @@ def test_axis0_extrema_special_values(gpu, oracle, dtype):
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_axis0_sums_special_values(gpu, dtype):
-    """Sums and means along a strided axis with few outputs (column tiles over ranges of the axis,
-    binary64 partials): an infinity gives an infinite sum (not the NaN of a TwoSum error term),
-    +inf with -inf and NaN give NaN, columns of small integers are exact."""
+    """Sums, means and variances along a strided axis with few outputs (column tiles over ranges of
+    the axis): an infinity gives an infinite sum (not the NaN of a TwoSum error term), +inf with
+    -inf and NaN give NaN, columns of small integers are exact; variances as stated below."""
     rows, cols = 5000, 12
     x = np.ones((rows, cols), dtype)
     x[17, 1] = np.inf
@@ -305,3 +305,10 @@ def test_axis0_sums_special_values(gpu, dtype):
     assert s[0] == rows and s[1] == np.inf and np.isnan(s[2]) and np.isnan(s[3])
     assert s[4] == rows * (rows - 1) // 2 and m[4] == (rows - 1) / 2
     assert m[0] == 1.0 and m[1] == np.inf
+    # variances (Chan records over 16-row blocks): a constant column is exactly 0; an infinite or
+    # NaN element gives NaN, as the reference's mean-then-squares does (inf - inf); 0 .. n-1 has
+    # the closed form (n^2 - 1) / 12 (ddof 0)
+    v = K.reduce_axis("var", x.reshape(-1), 1, rows, cols)
+    assert v[0] == 0.0 and np.isnan(v[1]) and np.isnan(v[2]) and np.isnan(v[3])
+    want = (rows * rows - 1) / 12
+    assert abs(float(v[4]) - want) <= VAR_C * EPS[dtype] * want
