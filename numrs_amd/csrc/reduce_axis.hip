@@ -147,7 +147,12 @@ __device__ __forceinline__ void axis_rows16(const T *xb, size_t q0, size_t j1, s
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
         const size_t j = q0 + i;
-        v[i] = (in_row && j < j1) ? __builtin_nontemporal_load(xb + j * inner + c) : (T)0;
+        // binary32 plain loads (8192 x 8192 sums 51.7 -> 49.6 us), binary64 non-temporal ones
+        // (plain: 82.9 -> 92.3 us)
+        if constexpr (sizeof(T) == 4)
+            v[i] = (in_row && j < j1) ? xb[j * inner + c] : (T)0;
+        else
+            v[i] = (in_row && j < j1) ? __builtin_nontemporal_load(xb + j * inner + c) : (T)0;
     }
 }
 
