@@ -97,7 +97,13 @@ __device__ __forceinline__ void lean_load(const T *base, T (&y)[VPT * Vec16<T>::
         const V *bv = reinterpret_cast<const V *>(base) + lane;
 #pragma unroll
         for (int j = 0; j < VPT; ++j) {
-            const V v = __builtin_nontemporal_load(bv + j * 64);
+            // binary32 plain loads, binary64 non-temporal ones (8192 x 8192 fused f32: median
+            // 48.2 -> 46.6 us; binary64 unchanged either way)
+            V v;
+            if constexpr (sizeof(T) == 4)
+                v = bv[j * 64];
+            else
+                v = __builtin_nontemporal_load(bv + j * 64);
 #pragma unroll
             for (int e = 0; e < VN; ++e) y[j * VN + e] = v[e];
         }
