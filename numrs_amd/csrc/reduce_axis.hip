@@ -28,8 +28,9 @@ __device__ __forceinline__ void strided_fold(const T *p, size_t inner, size_t j0
 template <typename T>
 __global__ __launch_bounds__(256) void strided_reduce_kernel(int op, const T *x, size_t outer, size_t len, size_t inner,
                                                              T divisor, void *out) {
-    // binary32: 8 loads ahead (64 x 2^20 sums 65 -> 51 us); binary64 one at a time (8 ahead: 90 -> 106 us)
-    constexpr int U = sizeof(T) == 4 ? 8 : 1;
+    // loads ahead of the fold: binary32 8 (64 x 2^20 sums 65 -> 51 us), binary64 2 (argmin 97 -> 89
+    // us; 4 ahead: sums 88 -> 110 us, 8 ahead: 106 us)
+    constexpr int U = sizeof(T) == 4 ? 8 : 2;
     const size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (id >= outer * inner) return;
     const size_t o = id / inner, i = id % inner;
@@ -57,23 +58,15 @@ __global__ __launch_bounds__(256) void strided_reduce_kernel(int op, const T *x,
             T acc = p[0];
             bool nan = __builtin_isnan(acc);
             const bool mx = op == NRK_AXIS_MAX;
-            if constexpr (U > 1) {
-                strided_fold<U>(p, inner, 1, len, [&](size_t, T v) {
-                    if (nan) return;
-                    if (__builtin_isnan(v)) {
-                        acc = v;
-                        nan = true;
-                    } else if (mx ? (v > acc) : (v < acc)) {
-                        acc = v;
-                    }
-                });
-            } else {  // one load at a time: stop at the NaN
-                for (size_t j = 1; j < len && !nan; ++j) {
-                    const T v = p[j * inner];
-                    if (__builtin_isnan(v)) { acc = v; nan = true; break; }
-                    if (mx ? (v > acc) : (v < acc)) acc = v;
+            strided_fold<U>(p, inner, 1, len, [&](size_t, T v) {
+                if (nan) return;
+                if (__builtin_isnan(v)) {
+                    acc = v;
+                    nan = true;
+                } else if (mx ? (v > acc) : (v < acc)) {
+                    acc = v;
                 }
-            }
+            });
             ((T *)out)[id] = acc;
             break;
         }
